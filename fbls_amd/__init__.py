@@ -1,0 +1,464 @@
+"""MI355X-native FBLS search hot path.
+
+Python face of ``libfbls_hip.so`` (C ABI in ``include/fbls.h``).  The names
+mirror the reference's search interface (isakruas/fbls):
+
+* ``SearchConfig``          src/config.rs:48-131
+* ``durations_days``        src/config.rs:177-182
+* ``period_grid``           src/search/grid.rs:5-27
+* ``family_state``          src/prep/curve.rs:541-569 (prepared curves)
+* ``Gpu``                   src/search/scan.rs:39-99
+* ``DeviceReady.upload``    src/search/scan.rs:101-125
+* ``scan_bank``             src/search/scan.rs:144-357
+* ``scan_periods``          src/search/scan.rs:363-408
+* ``signal_search``         src/search/scan.rs:413-473
+* ``search_prepared``       src/pipeline.rs:41-68 (first pass: scan only)
+
+Every scan runs on the GPU through the HIP kernels; there is no CPU
+fallback, and a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+import time as _time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfbls_hip.so")
+
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i32 = ctypes.c_int32
+_P = ctypes.POINTER
+_vp = ctypes.c_void_p
+
+FBLS_OK = 0
+ERRORS = {1: "FBLS_E_FOLD_BUDGET", 2: "FBLS_E_NONPOSITIVE_PRECISION",
+          3: "FBLS_E_HIP", 4: "FBLS_E_ARG", 5: "FBLS_E_OOM", 6: "FBLS_E_DATA"}
+
+# Every symbol include/fbls.h declares.
+EXPORTS = (
+    "fbls_ctx_create", "fbls_ctx_destroy", "fbls_mem_info",
+    "fbls_ctx_set_launch", "fbls_family_upload", "fbls_family_free",
+    "fbls_family_fields", "fbls_scan_bank", "fbls_scan_periods",
+    "fbls_signal_search", "fbls_period_grid_count", "fbls_period_grid",
+    "fbls_durations_days", "fbls_family_state", "fbls_quantize",
+    "fbls_synth_star", "fbls_last_error",
+)
+
+
+class FblsError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+class Bank(ctypes.Structure):
+    _fields_ = [("durations", _P(_d)), ("rho", _P(_d)), ("nd", _sz),
+                ("ingress", _P(_d)), ("ns", _sz), ("layers", _u32),
+                ("threshold", _d), ("shape_gate", _d),
+                ("phase_oversample", _u32), ("period_batch", _u32),
+                ("max_fold_mib", _d), ("prescan_factor", _u32),
+                ("prescan_fraction", _d)]
+
+
+class ScanStats(ctypes.Structure):
+    _fields_ = [("periods", _u64), ("prescan_survivors", _u64),
+                ("phase_bins", _u64), ("batches", _u64),
+                ("templates_scanned", _u64), ("exact_tile_retries", _u64),
+                ("seconds", _d), ("seconds_fold", _d),
+                ("seconds_cumulative", _d), ("seconds_scan", _d),
+                ("seconds_device", _d)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Row(ctypes.Structure):
+    _fields_ = [("period", _d), ("epoch", _d), ("duration", _d),
+                ("score", _d), ("depth", _d)]
+
+
+class SearchCfgC(ctypes.Structure):
+    _fields_ = [("period_min", _d), ("period_max", _d),
+                ("period_oversample", _d)]
+
+
+class SynthCfg(ctypes.Structure):
+    _fields_ = [("seed", _u64), ("sectors", _u32), ("sector_days", _d),
+                ("gap_days", _d), ("cadence_days", _d), ("t0", _d),
+                ("sigma", _d), ("red_fraction", _d), ("ar_phi", _d),
+                ("period", _d), ("epoch", _d), ("depth", _d),
+                ("duration", _d), ("secondary_depth", _d),
+                ("baseline_gap_days", _d)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libfbls_hip.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.fbls_last_error.restype = ctypes.c_char_p
+        L.fbls_period_grid_count.restype = _sz
+        L.fbls_period_grid_count.argtypes = [_d, _d, _d, _d, _d]
+        L.fbls_period_grid.argtypes = [_d, _d, _d, _d, _d, _P(_d), _sz]
+        L.fbls_durations_days.restype = _sz
+        L.fbls_durations_days.argtypes = [_P(_d), _sz, _P(_d)]
+        L.fbls_quantize.argtypes = [_P(_d), _sz, _P(_i32)]
+        L.fbls_ctx_create.argtypes = [ctypes.c_int, _P(_vp)]
+        L.fbls_ctx_destroy.argtypes = [_vp]
+        L.fbls_ctx_destroy.restype = None
+        L.fbls_family_free.argtypes = [_vp]
+        L.fbls_family_free.restype = None
+        L.fbls_mem_info.argtypes = [_vp, _P(_sz), _P(_sz)]
+        L.fbls_ctx_set_launch.argtypes = [_vp, _u32, _u32]
+        L.fbls_family_upload.argtypes = [_vp, _P(_d), _P(_d), _P(_d), _sz, _d,
+                                         _d, _P(_vp)]
+        L.fbls_family_fields.argtypes = [_vp, _P(_d), _P(_d),
+                                         _P(ctypes.c_int), _P(ctypes.c_int)]
+        L.fbls_scan_bank.argtypes = [_vp, _vp, _P(_d), _sz, _P(Bank), _u32,
+                                     _P(_d), _P(_d), _P(_d), _P(_u32),
+                                     _P(ScanStats)]
+        L.fbls_scan_periods.argtypes = [_vp, _vp, _P(_d), _sz, _P(Bank),
+                                        _P(Row), _P(_sz), _P(ScanStats)]
+        L.fbls_signal_search.argtypes = [
+            _vp, _vp, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC), _P(Row),
+            _sz, _P(_sz), _P(ScanStats), _sz, _P(_sz), _P(_d), _sz, _P(_sz)]
+        L.fbls_family_state.argtypes = [
+            _P(_d), _P(_d), _P(_d), _sz, _P(_d), _sz, ctypes.c_int, _P(_d),
+            _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz)]
+        L.fbls_synth_star.argtypes = [_P(SynthCfg), _P(_d), _P(_d), _P(_d),
+                                      _sz, _P(_sz)]
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != FBLS_OK:
+        raise FblsError(status, lib().fbls_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P(_d)) if a is not None else None
+
+
+# ------------------------------------------------------------------ config
+
+def _default_hours():
+    # config.rs:94: 0.5 * 1.25^i rounded half-to-even at 1e-6 h
+    return [round(0.5 * 1.25 ** i * 1e6) / 1e6 for i in range(13)]
+
+
+@dataclasses.dataclass
+class SearchConfig:
+    """Search constants of the hot path (config.rs:48-131)."""
+    period_min: float = 0.3
+    period_max: float = 10.0
+    durations_hours: list = dataclasses.field(default_factory=_default_hours)
+    threshold: float = 7.1
+    correlated_noise: bool = True
+    ingress_fractions: list = dataclasses.field(
+        default_factory=lambda: [0.0, 0.2, 0.45])
+    shape_layers: int = 4
+    phase_oversample: int = 4
+    shape_gate_fraction: float = 0.6
+    prescan_factor: int = 2
+    prescan_fraction: float = 0.7
+    period_oversample: float = 3.0
+    period_batch: int = 4096
+    max_fold_mib: float = 1024.0
+
+    def validate(self):
+        """config.rs:91-131 (the fields of the search path)."""
+        if not (0.0 < self.period_min < self.period_max):
+            raise ValueError("require 0 < period_min < period_max")
+        if not self.durations_hours or any(
+                not (np.isfinite(d) and d > 0) for d in self.durations_hours):
+            raise ValueError("durations_hours must contain positive finite values")
+        if min(self.durations_hours) / 24.0 >= self.period_min / 4.0:
+            raise ValueError("minimum duration must be below one quarter of period_min")
+        if not self.ingress_fractions or any(
+                not (0.0 <= x < 0.5) for x in self.ingress_fractions):
+            raise ValueError("ingress fractions must be in [0, 0.5)")
+        if (self.shape_layers < 1 or self.phase_oversample < 1
+                or self.period_batch < 1 or self.prescan_factor < 1):
+            raise ValueError("invalid integer search parameter")
+        if not (0.0 < self.prescan_fraction <= 1.0):
+            raise ValueError("prescan_fraction must be in (0, 1]")
+
+    def durations_days(self):
+        return durations_days(self.durations_hours)
+
+    @property
+    def shape_gate(self):
+        # scan.rs:433-437
+        if self.ingress_fractions[0] == 0.0:
+            return self.shape_gate_fraction * self.threshold
+        return 0.0
+
+
+# ------------------------------------------------------------ host helpers
+
+def durations_days(hours):
+    h = _f64(hours)
+    out = np.empty_like(h)
+    m = lib().fbls_durations_days(_dp(h), len(h), _dp(out))
+    return out[:m]
+
+
+def period_grid(minimum, maximum, baseline, minimum_duration, oversample):
+    L = lib()
+    n = L.fbls_period_grid_count(minimum, maximum, baseline, minimum_duration,
+                                 oversample)
+    out = np.empty(n)
+    _check(L.fbls_period_grid(minimum, maximum, baseline, minimum_duration,
+                              oversample, _dp(out), n))
+    return out
+
+
+def quantize(x):
+    x = _f64(x)
+    q = np.empty(len(x), dtype=np.int32)
+    e = lib().fbls_quantize(_dp(x), len(x), q.ctypes.data_as(_P(_i32)))
+    return q, e
+
+
+@dataclasses.dataclass
+class FamilyState:
+    t: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    baseline_norm: float
+    origin: float
+    durations: np.ndarray
+    rho: np.ndarray
+    inflation: float
+    segments: int
+
+
+def family_state(time, flux, sigma, durations, correlated=True):
+    """curve.rs:541-569 for a prepared curve (flux relative to zero)."""
+    time, flux, sigma = _f64(time), _f64(flux), _f64(sigma)
+    durations = _f64(durations)
+    n = len(time)
+    t, v, w = np.empty(n), np.empty(n), np.empty(n)
+    rho = np.empty(len(durations))
+    m, c, org, infl, seg = _sz(), _d(), _d(), _d(), _sz()
+    _check(lib().fbls_family_state(
+        _dp(time), _dp(flux), _dp(sigma), n, _dp(durations), len(durations),
+        int(bool(correlated)), _dp(t), _dp(v), _dp(w), ctypes.byref(m),
+        ctypes.byref(c), ctypes.byref(org), _dp(rho), ctypes.byref(infl),
+        ctypes.byref(seg)))
+    k = m.value
+    return FamilyState(t[:k].copy(), v[:k].copy(), w[:k].copy(), c.value,
+                       org.value, durations, rho, infl.value, seg.value)
+
+
+def synth_star(**kw):
+    """Seeded synthetic light curve: returns (time, flux, flux_err)."""
+    cfg = SynthCfg(seed=1, sectors=1, sector_days=27.4, gap_days=1.0,
+                   cadence_days=1.0 / 720.0, t0=2458000.0, sigma=1e-3,
+                   red_fraction=0.0, ar_phi=0.0, period=0.0, epoch=np.nan,
+                   depth=0.0, duration=0.0, secondary_depth=0.0,
+                   baseline_gap_days=0.0)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    n = _sz()
+    L = lib()
+    _check(L.fbls_synth_star(ctypes.byref(cfg), None, None, None, 0,
+                             ctypes.byref(n)))
+    t, f, e = np.empty(n.value), np.empty(n.value), np.empty(n.value)
+    _check(L.fbls_synth_star(ctypes.byref(cfg), _dp(t), _dp(f), _dp(e),
+                             n.value, ctypes.byref(n)))
+    return t, f, e
+
+
+# ----------------------------------------------------------------- device
+
+class Gpu:
+    """Device context: HIP stream, events and scratch (scan.rs:39-99)."""
+
+    def __init__(self, device=0):
+        self._h = _vp()
+        _check(lib().fbls_ctx_create(device, ctypes.byref(self._h)))
+        self.device = device
+
+    def memory(self):
+        fr, tot = _sz(), _sz()
+        _check(lib().fbls_mem_info(self._h, ctypes.byref(fr), ctypes.byref(tot)))
+        return fr.value, tot.value
+
+    def set_launch(self, lds_bytes=0, block=0):
+        _check(lib().fbls_ctx_set_launch(self._h, lds_bytes, block))
+
+    def close(self):
+        if self._h:
+            lib().fbls_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceReady:
+    """A family's fields resident in HBM (scan.rs:101-125)."""
+
+    def __init__(self, gpu, t, v, w, baseline_norm, origin):
+        t, v, w = _f64(t), _f64(v), _f64(w)
+        self.gpu = gpu
+        self.n = len(t)
+        self.baseline_norm = baseline_norm
+        self.origin = origin
+        self.t_last = float(t[-1])
+        self._h = _vp()
+        _check(lib().fbls_family_upload(gpu._h, _dp(t), _dp(v), _dp(w),
+                                        len(t), baseline_norm, origin,
+                                        ctypes.byref(self._h)))
+
+    @classmethod
+    def upload(cls, gpu, state: FamilyState):
+        return cls(gpu, state.t, state.v, state.w, state.baseline_norm,
+                   state.origin)
+
+    def fields(self):
+        """The fixed-point values the device scans (exact doubles)."""
+        v, w = np.empty(self.n), np.empty(self.n)
+        ev, ew = ctypes.c_int(), ctypes.c_int()
+        _check(lib().fbls_family_fields(self._h, _dp(v), _dp(w),
+                                        ctypes.byref(ev), ctypes.byref(ew)))
+        return v, w, ev.value, ew.value
+
+    def close(self):
+        if self._h:
+            lib().fbls_family_free(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_bank(cfg: SearchConfig, durations, rho, max_fold_mib=None):
+    durations, rho = _f64(durations), _f64(rho)
+    ing = _f64(cfg.ingress_fractions)
+    b = Bank(durations=_dp(durations), rho=_dp(rho), nd=len(durations),
+             ingress=_dp(ing), ns=len(ing), layers=cfg.shape_layers,
+             threshold=cfg.threshold, shape_gate=cfg.shape_gate,
+             phase_oversample=cfg.phase_oversample,
+             period_batch=cfg.period_batch,
+             max_fold_mib=cfg.max_fold_mib if max_fold_mib is None else max_fold_mib,
+             prescan_factor=cfg.prescan_factor,
+             prescan_fraction=cfg.prescan_fraction)
+    b._keep = (durations, rho, ing)  # keep the arrays alive with the struct
+    return b
+
+
+def scan_bank(gpu, dev, periods, bank, nb, winner=True):
+    """Best template per period at nb bins.  Returns records (np x 9), score,
+    depth, winner (np x 4 {phase, width, shape, duration index}) and stats."""
+    periods = _f64(periods)
+    n = len(periods)
+    rec = np.empty((n, 9))
+    score, depth = np.empty(n), np.empty(n)
+    win = np.empty((n, 4), dtype=np.uint32) if winner else None
+    st = ScanStats()
+    _check(lib().fbls_scan_bank(
+        gpu._h, dev._h, _dp(periods), n, ctypes.byref(bank), nb, _dp(rec),
+        _dp(score), _dp(depth),
+        win.ctypes.data_as(_P(_u32)) if win is not None else None,
+        ctypes.byref(st)))
+    return rec, score, depth, win, st.as_dict()
+
+
+def _rows(rows, n):
+    return np.array([(r.period, r.epoch, r.duration, r.score, r.depth)
+                     for r in rows[:n]]).reshape(-1, 5)
+
+
+def scan_periods(gpu, dev, periods, bank):
+    periods = _f64(periods)
+    rows = (Row * max(1, len(periods)))()
+    nrows = _sz()
+    st = ScanStats()
+    _check(lib().fbls_scan_periods(gpu._h, dev._h, _dp(periods), len(periods),
+                                   ctypes.byref(bank), rows,
+                                   ctypes.byref(nrows), ctypes.byref(st)))
+    return _rows(rows, nrows.value), st.as_dict()
+
+
+def signal_search(gpu, dev, durations, rho, cfg: SearchConfig):
+    """One family: returns (rows [k x 5], grid, band stats)."""
+    durations, rho = _f64(durations), _f64(rho)
+    maximum = min(cfg.period_max, dev.t_last)
+    minimum = max(cfg.period_min, 4.0 * durations.min() * (1.0 + 1e-9))
+    cap = 1
+    if maximum > minimum:
+        cap = lib().fbls_period_grid_count(minimum, maximum, dev.t_last,
+                                           float(durations.min()),
+                                           cfg.period_oversample)
+    bank = make_bank(cfg, durations, rho)
+    scfg = SearchCfgC(cfg.period_min, cfg.period_max, cfg.period_oversample)
+    rows = (Row * cap)()
+    stats = (ScanStats * 64)()
+    grid = np.empty(cap)
+    nrows, nbands, gcount = _sz(), _sz(), _sz()
+    _check(lib().fbls_signal_search(
+        gpu._h, dev._h, _dp(durations), _dp(rho), len(durations),
+        ctypes.byref(bank), ctypes.byref(scfg), rows, cap,
+        ctypes.byref(nrows), stats, 64, ctypes.byref(nbands), _dp(grid), cap,
+        ctypes.byref(gcount)))
+    return (_rows(rows, nrows.value), grid[:gcount.value],
+            [s.as_dict() for s in stats[:nbands.value]])
+
+
+def search_prepared(gpu, time, flux, flux_err, cfg: SearchConfig):
+    """pipeline.rs:41-68 then the first pass of search_families
+    (pipeline.rs:102-124): one family with every duration, scanned once.
+    Returns the period table and the schema-3 style statistics."""
+    cfg.validate()
+    t0 = _time.perf_counter()
+    durations = cfg.durations_days()
+    state = family_state(time, flux, flux_err, durations, cfg.correlated_noise)
+    prep = _time.perf_counter() - t0
+    dev = DeviceReady.upload(gpu, state)
+    t1 = _time.perf_counter()
+    rows, grid, bands = signal_search(gpu, dev, durations, state.rho, cfg)
+    scan = _time.perf_counter() - t1
+    return {
+        "period_table": rows,
+        "statistics": {
+            "observations": len(state.t),
+            "periods_tested": len(grid),
+            "qualifying_periods": len(rows),
+            "seconds_preparation": prep,
+            "seconds_scan": scan,
+            "families": [{
+                "family": 0, "observations": len(state.t),
+                "error_inflation": state.inflation,
+                "segments": state.segments,
+                "durations_hours": list(durations * 24.0),
+                "periods": len(grid), "rho": list(state.rho),
+                "bands": bands}],
+        },
+    }

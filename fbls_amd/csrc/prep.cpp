@@ -1,0 +1,378 @@
+// Host-side input preparation of the FBLS hot path (see prep.h).
+// Floating-point operations keep the reference's order and rounding; the
+// library is compiled with -ffp-contract=off so nothing is fused.
+#include "prep.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+namespace fbls {
+
+int quantize(const double* x, size_t n, int32_t* q) {
+  double m = 0.0;
+  for (size_t i = 0; i < n; i++) m = std::max(m, std::fabs(x[i]));
+  int e = 0;
+  if (m > 0.0) {
+    int ex = 0;
+    std::frexp(m, &ex);
+    e = 30 - ex;
+  }
+  for (size_t i = 0; i < n; i++) q[i] = (int32_t)std::nearbyint(std::ldexp(x[i], e));
+  return e;
+}
+
+size_t period_grid_count(double minimum, double maximum, double baseline,
+                         double minimum_duration, double oversample) {
+  // grid.rs:12-13
+  const double log_step = std::log1p(minimum_duration / (oversample * baseline));
+  return (size_t)std::ceil(std::log(maximum / minimum) / log_step) + 1;
+}
+
+void period_grid(double minimum, double maximum, double* out, size_t count) {
+  // grid.rs:14-26
+  const double a = std::log(minimum), b = std::log(maximum);
+  for (size_t i = 0; i < count; i++)
+    out[i] = std::exp(count == 1 ? a : a + (b - a) * (double)i / (double)(count - 1));
+  out[0] = minimum;
+  out[count - 1] = maximum;
+}
+
+size_t durations_days(const double* hours, size_t n, double* out) {
+  std::vector<double> d(hours, hours + n);
+  for (double& x : d) x /= 24.0;
+  std::sort(d.begin(), d.end());
+  d.erase(std::unique(d.begin(), d.end()), d.end());
+  std::copy(d.begin(), d.end(), out);
+  return d.size();
+}
+
+namespace {
+
+// stats.rs:4-11
+double median_lower(const std::vector<double>& values) {
+  std::vector<double> v;
+  v.reserve(values.size());
+  for (double x : values)
+    if (std::isfinite(x)) v.push_back(x);
+  if (v.empty()) return NAN;
+  std::sort(v.begin(), v.end());
+  return v[(v.size() - 1) / 2];
+}
+
+// stats.rs:14-18
+double mad_scale(const std::vector<double>& values) {
+  const double m = median_lower(values);
+  std::vector<double> dev(values.size());
+  for (size_t i = 0; i < values.size(); i++) dev[i] = std::fabs(values[i] - m);
+  return 1.4826 * median_lower(dev);
+}
+
+double max1(double x) { return x >= 1.0 ? x : 1.0; }  // f64::max(x, 1.0), NaN -> 1
+
+struct Noise {
+  std::vector<double> t, deficit, variance;
+  double origin = 0.0, inflation = 1.0;
+  size_t segments = 0;
+};
+
+// curve.rs:89-198 with bin_minutes = 0 and one group.
+int fit_segment_noise(const double* time, const double* flux, const double* err,
+                      size_t n, double gap, Noise* out, std::string* msg) {
+  std::vector<size_t> keep;
+  for (size_t i = 0; i < n; i++) {
+    const double e = err ? err[i] : 1.0;
+    if (std::isfinite(time[i]) && std::isfinite(flux[i]) && std::isfinite(e) && e > 0.0)
+      keep.push_back(i);
+  }
+  std::stable_sort(keep.begin(), keep.end(),
+                   [&](size_t a, size_t b) { return time[a] < time[b]; });
+  if (keep.size() < 6) {
+    *msg = "insufficient usable observations";
+    return FBLS_E_DATA;
+  }
+  const size_t m = keep.size();
+  std::vector<double> t(m), f(m), e(m);
+  for (size_t k = 0; k < m; k++) {
+    t[k] = time[keep[k]];
+    f[k] = flux[keep[k]];
+    e[k] = err ? err[keep[k]] : 1.0;
+  }
+  const double med = median_lower(f);
+  if (!(med > 0.0)) {
+    *msg = "mission flux must have a positive median";
+    return FBLS_E_DATA;
+  }
+  for (size_t k = 0; k < m; k++) {
+    f[k] = f[k] / med - 1.0;
+    e[k] /= med;
+  }
+  const double origin = t[0];
+  for (double& x : t) x -= origin;
+  std::vector<double> deltas, standardized;
+  for (size_t i = 1; i < m; i++) {
+    if (t[i] - t[i - 1] < gap) {
+      const double d = f[i] - f[i - 1];
+      deltas.push_back(d);
+      standardized.push_back(d / std::sqrt(e[i] * e[i] + e[i - 1] * e[i - 1]));
+    }
+  }
+  if (deltas.empty()) {
+    *msg = "no adjacent observations available for noise estimation";
+    return FBLS_E_DATA;
+  }
+  const double inflation = max1(mad_scale(standardized));
+  out->variance.resize(m);
+  if (err) {
+    for (size_t k = 0; k < m; k++) {
+      const double s = e[k] * inflation;
+      out->variance[k] = s * s;
+    }
+  } else {
+    const double md = median_lower(deltas);
+    std::vector<double> dev(deltas.size());
+    for (size_t k = 0; k < deltas.size(); k++) dev[k] = std::fabs(deltas[k] - md);
+    double white = 1.4826 / std::sqrt(2.0) * median_lower(dev);
+    if (!(white >= 1e-12)) white = 1e-12;
+    for (size_t k = 0; k < m; k++) out->variance[k] = white * white;
+  }
+  size_t seg = 1;
+  for (size_t i = 1; i < m; i++)
+    if (t[i] - t[i - 1] > gap) seg++;
+  out->deficit.resize(m);
+  for (size_t k = 0; k < m; k++) out->deficit[k] = -f[k];
+  out->t = std::move(t);
+  out->origin = origin;
+  out->inflation = inflation;
+  out->segments = seg;
+  return 0;
+}
+
+// curve.rs:456-539
+void correlated_noise_factors(const std::vector<double>& time,
+                              const std::vector<double>& residual,
+                              const std::vector<double>& sigma,
+                              const double* durations, size_t nd, double* rho) {
+  const double min_fill = 0.5;
+  const size_t minimum_bins = 20;
+  const size_t n = time.size();
+  std::vector<double> weights(n);
+  for (size_t i = 0; i < n; i++) weights[i] = 1.0 / (sigma[i] * sigma[i]);
+  std::vector<double> steps;
+  for (size_t i = 1; i < n; i++) {
+    const double s = time[i] - time[i - 1];
+    if (s > 0.0) steps.push_back(s);
+  }
+  const double cadence = median_lower(steps);
+  std::vector<double> measured(nd);
+  std::vector<size_t> counts(nd);
+  std::vector<double> z;
+  for (size_t di = 0; di < nd; di++) {
+    const double d = durations[di];
+    z.clear();
+    double total = 0.0, acc = 0.0;
+    size_t cnt = 0;
+    int64_t current = INT64_MIN;
+    auto flush = [&]() {
+      if (cnt > 0 && (double)cnt >= min_fill * d / cadence)
+        z.push_back((acc / total) * std::sqrt(total));
+    };
+    for (size_t i = 0; i < n; i++) {
+      const int64_t cell = (int64_t)std::floor((time[i] - time[0]) / d);
+      if (cell != current) {
+        flush();
+        current = cell;
+        total = 0.0;
+        acc = 0.0;
+        cnt = 0;
+      }
+      total += weights[i];
+      acc += weights[i] * residual[i];
+      cnt++;
+    }
+    flush();
+    counts[di] = z.size();
+    measured[di] = z.size() >= 3 ? max1(mad_scale(z)) : 1.0;
+  }
+  bool any = false;
+  for (size_t c : counts) any = any || c >= minimum_bins;
+  std::vector<double> inherited(nd);
+  if (any) {
+    for (size_t di = 0; di < nd; di++) {
+      const double d = durations[di];
+      size_t anchor = nd;
+      double best = 0.0;
+      for (size_t a = 0; a < nd; a++) {
+        if (counts[a] < minimum_bins) continue;
+        const double dist = std::fabs(std::log(d) - std::log(durations[a]));
+        if (anchor == nd || dist < best) {
+          anchor = a;
+          best = dist;
+        }
+      }
+      const double ratio = d / durations[anchor];
+      inherited[di] = measured[anchor] * std::sqrt(ratio >= 1.0 ? ratio : 1.0);
+    }
+  } else {
+    std::vector<double> st(n);
+    for (size_t i = 0; i < n; i++) st[i] = residual[i] / sigma[i];
+    const double scatter = max1(mad_scale(st));
+    for (size_t di = 0; di < nd; di++) {
+      const double ratio = durations[di] / cadence;
+      inherited[di] = scatter * std::sqrt(ratio >= 1.0 ? ratio : 1.0);
+    }
+  }
+  for (size_t di = 0; di < nd; di++)
+    rho[di] = counts[di] >= minimum_bins
+                  ? measured[di]
+                  : (measured[di] >= inherited[di] ? measured[di] : inherited[di]);
+}
+
+}  // namespace
+
+int family_state(const double* time, const double* flux, const double* sigma,
+                 size_t n, const double* durations, size_t nd, int correlated,
+                 double* t, double* v, double* w, size_t* out_n,
+                 double* baseline_norm, double* origin, double* rho,
+                 double* inflation, size_t* segments, std::string* msg) {
+  // curve.rs:547-548: flux1 = 1 + flux, fit without binning, gap 0.75 d
+  std::vector<double> flux1(n);
+  for (size_t i = 0; i < n; i++) flux1[i] = 1.0 + flux[i];
+  Noise model;
+  int st = fit_segment_noise(time, flux1.data(), sigma, n, 0.75, &model, msg);
+  if (st) return st;
+  const size_t m = model.t.size();
+  // curve.rs:427-450 (prepare_ready)
+  if (m < 6) {
+    *msg = "at least six observations are required";
+    return FBLS_E_DATA;
+  }
+  double c = 0.0;
+  for (size_t i = 0; i < m; i++) {
+    w[i] = 1.0 / model.variance[i];
+    c += w[i];
+  }
+  if (!(std::isfinite(c) && c > 0.0)) {
+    *msg = "invalid baseline precision";
+    return FBLS_E_DATA;
+  }
+  double spy = 0.0;
+  for (size_t i = 0; i < m; i++) spy += w[i] * model.deficit[i];
+  const double mean = spy / c;
+  for (size_t i = 0; i < m; i++) {
+    const double py = w[i] * model.deficit[i];
+    v[i] = py - w[i] * mean;
+    t[i] = model.t[i];
+  }
+  *out_n = m;
+  *baseline_norm = c;
+  *origin = model.origin;
+  *inflation = model.inflation;
+  *segments = model.segments;
+  if (correlated) {
+    std::vector<double> resid(m), sig(m);
+    for (size_t i = 0; i < m; i++) {
+      resid[i] = -model.deficit[i];
+      sig[i] = std::sqrt(model.variance[i]);
+    }
+    correlated_noise_factors(model.t, resid, sig, durations, nd, rho);
+  } else {
+    for (size_t i = 0; i < nd; i++) rho[i] = 1.0;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ synthetic
+
+namespace {
+
+// survey/numeric.rs:345-364
+struct Rng {
+  uint64_t s;
+  explicit Rng(uint64_t seed) : s((seed * 0x9E3779B97F4A7C15ull) | 1ull) {}
+  double uniform() {
+    uint64_t x = s;
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    s = x;
+    return (double)(x >> 11) / (double)(1ull << 53);
+  }
+  double normal(double mean, double sigma) {
+    double u1 = uniform();
+    if (!(u1 >= 1e-300)) u1 = 1e-300;
+    const double u2 = uniform();
+    return mean + sigma * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+  }
+};
+
+}  // namespace
+
+// Light curve of `sectors` contiguous sectors at a fixed cadence, each with a
+// gap at mid-sector, optionally a long gap after half of the sectors.  Flux is
+// 1 + white noise + stationary AR(1) red noise - box transit(s); flux_err is
+// the constant total sigma.  Draw order per point: white, then red innovation.
+int synth_star(const fbls_synth_cfg* cfg, double* time, double* flux,
+               double* flux_err, size_t cap, size_t* n, std::string* msg) {
+  if (!cfg || !n) {
+    *msg = "null argument";
+    return FBLS_E_ARG;
+  }
+  if (!(cfg->cadence_days > 0.0) || !(cfg->sector_days > 0.0) || cfg->sectors == 0) {
+    *msg = "invalid synthetic configuration";
+    return FBLS_E_ARG;
+  }
+  Rng rng(cfg->seed);
+  double epoch = cfg->epoch;
+  if (!std::isfinite(epoch)) epoch = cfg->t0 + cfg->period * rng.uniform();
+  const double var = cfg->sigma * cfg->sigma;
+  const double red_var = var * cfg->red_fraction;
+  const double white = std::sqrt(var - red_var);
+  const double red = std::sqrt(red_var);
+  const double innov = red * std::sqrt(1.0 - cfg->ar_phi * cfg->ar_phi);
+  size_t m = 0;
+  const size_t per = (size_t)std::floor(cfg->sector_days / cfg->cadence_days);
+  for (uint32_t s = 0; s < cfg->sectors; s++) {
+    double start = cfg->t0 + s * cfg->sector_days;
+    if (cfg->baseline_gap_days > 0.0 && s >= (cfg->sectors + 1) / 2)
+      start += cfg->baseline_gap_days;
+    const double mid = start + 0.5 * cfg->sector_days;
+    double ar = red > 0.0 ? rng.normal(0.0, red) : 0.0;
+    for (size_t i = 0; i < per; i++) {
+      const double t = start + (double)i * cfg->cadence_days;
+      if (std::fabs(t - mid) < 0.5 * cfg->gap_days) continue;
+      double x = white > 0.0 ? rng.normal(0.0, white) : 0.0;
+      if (red > 0.0) {
+        ar = cfg->ar_phi * ar + rng.normal(0.0, innov);
+        x += ar;
+      }
+      if (cfg->depth > 0.0 && cfg->period > 0.0) {
+        double ph = std::fmod(t - epoch, cfg->period);
+        if (ph < 0.0) ph += cfg->period;
+        if (ph > 0.5 * cfg->period) ph -= cfg->period;
+        if (std::fabs(ph) <= 0.5 * cfg->duration) x -= cfg->depth;
+        if (cfg->secondary_depth > 0.0) {
+          double ps = std::fmod(t - epoch - 0.5 * cfg->period, cfg->period);
+          if (ps < 0.0) ps += cfg->period;
+          if (ps > 0.5 * cfg->period) ps -= cfg->period;
+          if (std::fabs(ps) <= 0.5 * cfg->duration) x -= cfg->secondary_depth;
+        }
+      }
+      if (m < cap) {
+        if (time) time[m] = t;
+        if (flux) flux[m] = 1.0 + x;
+        if (flux_err) flux_err[m] = cfg->sigma;
+      }
+      m++;
+    }
+  }
+  *n = m;
+  if (m > cap && (time || flux || flux_err)) {
+    *msg = "output capacity too small";
+    return FBLS_E_ARG;
+  }
+  return 0;
+}
+
+}  // namespace fbls

@@ -1,0 +1,35 @@
+// Host-side (CPU) pieces of the hot path's inputs: period grid, durations,
+// the diagonal noise model of a prepared curve, fixed-point quantisation and
+// the synthetic light-curve generator.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/fbls.h"
+
+namespace fbls {
+
+// q = rint(x * 2^e) with e = 30 - exponent(max |x|), so |q| <= 2^30; returns e.
+int quantize(const double* x, size_t n, int32_t* q);
+
+// grid.rs:5-27
+size_t period_grid_count(double minimum, double maximum, double baseline,
+                         double minimum_duration, double oversample);
+void period_grid(double minimum, double maximum, double* out, size_t count);
+
+// config.rs:177-182
+size_t durations_days(const double* hours, size_t n, double* out);
+
+// curve.rs:541-569 for a prepared curve (flux relative to 0, errors given).
+int family_state(const double* time, const double* flux, const double* sigma,
+                 size_t n, const double* durations, size_t nd, int correlated,
+                 double* t, double* v, double* w, size_t* out_n,
+                 double* baseline_norm, double* origin, double* rho,
+                 double* inflation, size_t* segments, std::string* msg);
+
+int synth_star(const fbls_synth_cfg* cfg, double* time, double* flux,
+               double* flux_err, size_t cap, size_t* n, std::string* msg);
+
+}  // namespace fbls

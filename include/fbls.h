@@ -1,0 +1,191 @@
+/*
+ * fbls.h - C ABI of the MI355X-native FBLS search hot path.
+ *
+ * The reference (isakruas/fbls, Rust + cuda-oxide) crosses into device code
+ * through the generated `gpu::LoadedModule` methods and the host drivers in
+ * src/search/scan.rs.  This library replaces that seam at scan_bank
+ * granularity: the caller keeps grid, band, prescan and selection policy if
+ * it wants to, or calls the full family driver fbls_signal_search.
+ *
+ * Conventions
+ *  - Every call is synchronous on return (scan.rs:330-333 copies results to
+ *    the host before returning).
+ *  - The caller owns all host arrays and sizes the outputs; the library owns
+ *    device memory behind opaque handles.  No device pointer crosses the ABI.
+ *  - One fbls_ctx per host thread (scan.rs:47-64 keeps one `Gpu` per thread);
+ *    distinct contexts may be used concurrently, one context must not be.
+ *  - Status codes: FBLS_OK, or one of FBLS_E_*; the message is available from
+ *    fbls_last_error() on the calling thread.  Infinite scores are data
+ *    (score -inf when no template is usable, kernels.rs:823-827).
+ *  - Functions marked [host] run on the CPU only and need no GPU.
+ */
+#ifndef FBLS_H
+#define FBLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FBLS_OK = 0,
+  FBLS_E_FOLD_BUDGET = 1,            /* scan.rs:180-182 */
+  FBLS_E_NONPOSITIVE_PRECISION = 2,  /* scan.rs:335-337 */
+  FBLS_E_HIP = 3,                    /* device/runtime failure */
+  FBLS_E_ARG = 4,                    /* invalid argument / configuration */
+  FBLS_E_OOM = 5,                    /* device allocation failed */
+  FBLS_E_DATA = 6                    /* too few observations, bad flux, ... */
+};
+
+#define FBLS_RECORD_STRIDE 9  /* {P, scale, phase, width, a, norm, d, ingress, rho} */
+#define FBLS_WINNER_STRIDE 4  /* {phase, width, shape, duration index} */
+
+typedef struct fbls_ctx fbls_ctx;
+typedef struct fbls_family fbls_family;
+
+/* Template bank of one band; mirrors `Bank` (scan.rs:127-139). */
+typedef struct {
+  const double* durations; /* nd, ascending (scan.rs:418-424 sorts them) */
+  const double* rho;       /* nd correlated-noise factors */
+  size_t nd;
+  const double* ingress;   /* ns ingress fractions, [0] = 0 is the box */
+  size_t ns;
+  uint32_t layers;         /* staircase layers (config.rs:105) */
+  double threshold;        /* SNR threshold (config.rs:96) */
+  double shape_gate;       /* tile gate for non-box shapes (scan.rs:433-437) */
+  uint32_t phase_oversample;
+  uint32_t period_batch;   /* only enters the `batches` statistic */
+  double max_fold_mib;     /* fold budget; 0 selects min(1024, free/2) */
+  uint32_t prescan_factor;
+  double prescan_fraction;
+} fbls_bank;
+
+/* `ScanStats` (scan.rs:25-37) */
+typedef struct {
+  uint64_t periods, prescan_survivors, phase_bins, batches, templates_scanned,
+      exact_tile_retries;
+  double seconds, seconds_fold, seconds_cumulative, seconds_scan;
+  /* device time of the scan kernels, from HIP events */
+  double seconds_device;
+} fbls_scan_stats;
+
+/* `Row` (scan.rs:16-23) */
+typedef struct {
+  double period, epoch, duration, score, depth;
+} fbls_row;
+
+/* Search options of `signal_search` (scan.rs:413-437, config.rs:48-77). */
+typedef struct {
+  double period_min, period_max, period_oversample;
+} fbls_search_cfg;
+
+/* -- context ------------------------------------------------------------ */
+
+/* `Gpu::new` (scan.rs:66-77): device context, stream and kernels. */
+int fbls_ctx_create(int device, fbls_ctx** out);
+void fbls_ctx_destroy(fbls_ctx* ctx);
+/* `Gpu::memory` (scan.rs:80-90) */
+int fbls_mem_info(fbls_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+/* Launch shape override for tuning: LDS bytes per workgroup (0 = auto) and
+ * workgroup size (256 or 512, 0 = auto).  Results never depend on it. */
+int fbls_ctx_set_launch(fbls_ctx* ctx, uint32_t lds_bytes, uint32_t block);
+
+/* -- family upload ------------------------------------------------------ */
+
+/* `DeviceReady::upload` (scan.rs:110-125).  t: origin-relative times >= 0;
+ * v, w: precision-weighted fields of `prepare_ready` (curve.rs:427-450);
+ * baseline_norm: C = sum of w.  v and w are stored as 32-bit fixed point
+ * with a power-of-two scale per array (|q| <= 2^30), which makes every bin
+ * and interval sum exact. */
+int fbls_family_upload(fbls_ctx* ctx, const double* t, const double* v,
+                       const double* w, size_t n, double baseline_norm,
+                       double origin, fbls_family** out);
+void fbls_family_free(fbls_family* fam);
+/* The fixed-point values the device scans, as doubles (exact), and the
+ * power-of-two exponents: v_dev = round(v * 2^ev) * 2^-ev. */
+int fbls_family_fields(const fbls_family* fam, double* v_out, double* w_out,
+                       int* ev, int* ew);
+
+/* -- the scan ----------------------------------------------------------- */
+
+/* `scan_bank` (scan.rs:144-357): best template per period at nb phase bins.
+ * records: np x 9, score/depth: np, winner: np x 4 or NULL. */
+int fbls_scan_bank(fbls_ctx* ctx, const fbls_family* fam,
+                   const double* periods, size_t np, const fbls_bank* bank,
+                   uint32_t nb, double* records, double* score, double* depth,
+                   uint32_t* winner, fbls_scan_stats* stats);
+
+/* `scan_periods` (scan.rs:363-408): coarse prescan then full scan of the
+ * survivors; rows (capacity np) receive periods whose score >= threshold. */
+int fbls_scan_periods(fbls_ctx* ctx, const fbls_family* fam,
+                      const double* periods, size_t np, const fbls_bank* bank,
+                      fbls_row* rows, size_t* nrows, fbls_scan_stats* stats);
+
+/* `signal_search` (scan.rs:413-473) for one family: period grid, octave
+ * bands, durations below a quarter of each band.  durations/rho may be
+ * unsorted.  rows capacity rows_cap (>= grid count); stats capacity
+ * stats_cap bands.  grid may be NULL; else capacity grid_cap. */
+int fbls_signal_search(fbls_ctx* ctx, const fbls_family* fam,
+                       const double* durations, const double* rho, size_t nd,
+                       const fbls_bank* bank, const fbls_search_cfg* cfg,
+                       fbls_row* rows, size_t rows_cap, size_t* nrows,
+                       fbls_scan_stats* stats, size_t stats_cap,
+                       size_t* nbands, double* grid, size_t grid_cap,
+                       size_t* grid_count);
+
+/* -- host helpers [host] ------------------------------------------------ */
+
+/* [host] `period_grid` (grid.rs:5-27): count first, then fill. */
+size_t fbls_period_grid_count(double minimum, double maximum, double baseline,
+                              double minimum_duration, double oversample);
+int fbls_period_grid(double minimum, double maximum, double baseline,
+                     double minimum_duration, double oversample, double* out,
+                     size_t count);
+
+/* [host] `SearchConfig::durations_days` (config.rs:177-182) */
+size_t fbls_durations_days(const double* hours, size_t n, double* out_days);
+
+/* [host] `family_state` (curve.rs:541-569) for a prepared curve:
+ * fit_segment_noise without binning, prepare_ready and, when correlated is
+ * nonzero, correlated_noise_factors.  Outputs sized n (t, v, w) and nd
+ * (rho); *out_n receives the kept count. */
+int fbls_family_state(const double* time, const double* flux,
+                      const double* sigma, size_t n, const double* durations,
+                      size_t nd, int correlated, double* t, double* v,
+                      double* w, size_t* out_n, double* baseline_norm,
+                      double* origin, double* rho, double* inflation,
+                      size_t* segments);
+
+/* [host] fixed-point representation used on the device; returns e with
+ * q = rint(x * 2^e), |q| <= 2^30. */
+int fbls_quantize(const double* x, size_t n, int32_t* q);
+
+/* [host] seeded synthetic light curve (xorshift64 + Box-Muller as in
+ * survey/numeric.rs:345-364).  See fbls_amd/synth.py for the workloads. */
+typedef struct {
+  uint64_t seed;
+  uint32_t sectors;        /* contiguous 27.4-d sectors */
+  double sector_days;      /* 27.4 */
+  double gap_days;         /* mid-sector gap (1.0) */
+  double cadence_days;     /* 2 min = 1/720 */
+  double t0;               /* absolute start time */
+  double sigma;            /* total white+red noise (relative flux) */
+  double red_fraction;     /* fraction of variance in AR(1), 0 = white */
+  double ar_phi;           /* AR(1) coefficient */
+  double period, epoch, depth, duration; /* injected box; depth 0 = none */
+  double secondary_depth;  /* eclipse at phase 0.5, 0 = none */
+  double baseline_gap_days; /* extra gap inserted after half the sectors */
+} fbls_synth_cfg;
+/* Writes up to cap points; returns the count through *n. */
+int fbls_synth_star(const fbls_synth_cfg* cfg, double* time, double* flux,
+                    double* flux_err, size_t cap, size_t* n);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* fbls_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

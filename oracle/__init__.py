@@ -1,0 +1,278 @@
+"""ctypes view of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker of the HIP path and never
+as the thing measured or shipped.  The oracle restates the reference
+(isakruas/fbls) search path in C; see oracle/fbls_oracle.h for the functions
+and the reference lines each one follows, and tests/test_oracle.py for how
+each is pinned.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_P = ctypes.POINTER
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+class ScanStats(ctypes.Structure):
+    _fields_ = [("periods", _u64), ("prescan_survivors", _u64),
+                ("phase_bins", _u64), ("batches", _u64),
+                ("templates_scanned", _u64)]
+
+
+class Row(ctypes.Structure):
+    _fields_ = [("period", _d), ("epoch", _d), ("duration", _d),
+                ("score", _d), ("depth", _d)]
+
+
+class BankCfg(ctypes.Structure):
+    _fields_ = [("layers", _u32), ("phase_oversample", _u32),
+                ("period_batch", _u32), ("prescan_factor", _u32),
+                ("threshold", _d), ("shape_gate", _d),
+                ("max_fold_mib", _d), ("prescan_fraction", _d)]
+
+
+class SearchCfg(ctypes.Structure):
+    _fields_ = [("period_min", _d), ("period_max", _d),
+                ("period_oversample", _d)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_durations_days.restype = _sz
+        L.orc_period_grid_count.restype = _sz
+        L.orc_period_grid_count.argtypes = [_d, _d, _d, _d, _d]
+        L.orc_period_grid.restype = None
+        L.orc_period_grid.argtypes = [_d, _d, _d, _d, _d, _P(_d), _sz]
+        L.orc_median_lower.restype = _d
+        L.orc_mad_scale.restype = _d
+        L.orc_phase_cell.restype = _u32
+        L.orc_phase_cell.argtypes = [_d, _d, _u32]
+        L.orc_bank_batches.restype = _u64
+        L.orc_bank_batches.argtypes = [_sz, _u32, _u32, _d]
+        L.orc_rng_new.restype = _u64
+        L.orc_rng_new.argtypes = [_u64]
+        L.orc_rng_uniform.restype = _d
+        L.orc_rng_normal.restype = _d
+        L.orc_rng_normal.argtypes = [_P(_u64), _d, _d]
+        L.orc_quantize.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P(_d))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def durations_days(hours):
+    h = _f64(hours)
+    out = np.empty_like(h)
+    m = lib().orc_durations_days(_dp(h), _sz(len(h)), _dp(out))
+    return out[:m]
+
+
+def period_grid(minimum, maximum, baseline, dmin, oversample):
+    L = lib()
+    n = L.orc_period_grid_count(minimum, maximum, baseline, dmin, oversample)
+    out = np.empty(n)
+    L.orc_period_grid(minimum, maximum, baseline, dmin, oversample, _dp(out), n)
+    return out
+
+
+def median_lower(x):
+    x = _f64(x)
+    return lib().orc_median_lower(_dp(x), _sz(len(x)))
+
+
+def mad_scale(x):
+    x = _f64(x)
+    return lib().orc_mad_scale(_dp(x), _sz(len(x)))
+
+
+def fit_segment_noise(time, flux, err, gap=0.75):
+    time, flux = _f64(time), _f64(flux)
+    n = len(time)
+    t, d, v = np.empty(n), np.empty(n), np.empty(n)
+    m, org, infl, seg = _sz(), _d(), _d(), _sz()
+    e = _f64(err) if err is not None else None
+    st = lib().orc_fit_segment_noise(
+        _dp(time), _dp(flux), _dp(e) if e is not None else None, _sz(n),
+        _d(gap), _dp(t), _dp(d), _dp(v), ctypes.byref(m), ctypes.byref(org),
+        ctypes.byref(infl), ctypes.byref(seg))
+    if st:
+        raise ValueError(f"fit_segment_noise status {st}")
+    k = m.value
+    return dict(time=t[:k], deficit=d[:k], variance=v[:k], origin=org.value,
+                inflation=infl.value, segments=seg.value)
+
+
+def prepare_ready(deficit, variance):
+    deficit, variance = _f64(deficit), _f64(variance)
+    n = len(deficit)
+    v, w = np.empty(n), np.empty(n)
+    c, e = _d(), _d()
+    st = lib().orc_prepare_ready(_dp(deficit), _dp(variance), _sz(n), _dp(v),
+                                 _dp(w), ctypes.byref(c), ctypes.byref(e))
+    if st:
+        raise ValueError(f"prepare_ready status {st}")
+    return dict(v=v, w=w, baseline_norm=c.value, residual_energy=e.value)
+
+
+def correlated_noise_factors(time, residual, sigma, durations):
+    time, residual, sigma = _f64(time), _f64(residual), _f64(sigma)
+    durations = _f64(durations)
+    rho = np.empty(len(durations))
+    lib().orc_correlated_noise_factors(_dp(time), _dp(residual), _dp(sigma),
+                                       _sz(len(time)), _dp(durations),
+                                       _sz(len(durations)), _dp(rho))
+    return rho
+
+
+def family_state(time, flux, sigma, durations, correlated=True):
+    """curve.rs:541-569 for a prepared curve (flux relative to zero)."""
+    model = fit_segment_noise(time, 1.0 + _f64(flux), sigma, 0.75)
+    ready = prepare_ready(model["deficit"], model["variance"])
+    if correlated:
+        rho = correlated_noise_factors(model["time"], -model["deficit"],
+                                       np.sqrt(model["variance"]), durations)
+    else:
+        rho = np.ones(len(durations))
+    return dict(t=model["time"], v=ready["v"], w=ready["w"],
+                baseline_norm=ready["baseline_norm"], origin=model["origin"],
+                rho=rho, inflation=model["inflation"],
+                segments=model["segments"])
+
+
+def quantize(x):
+    x = _f64(x)
+    q = np.empty(len(x), dtype=np.int32)
+    e = lib().orc_quantize(_dp(x), _sz(len(x)), q.ctypes.data_as(_P(ctypes.c_int32)))
+    return q, e
+
+
+def phase_cell(t, scale, nb):
+    return lib().orc_phase_cell(t, scale, nb)
+
+
+def _factors(ingress, layers):
+    ing = np.asarray(ingress, dtype=np.float64)
+    return ing
+
+
+def scan_bank(t, v, w, baseline_norm, periods, durations, rho, ingress,
+              layers, gate, nb):
+    """kernels.rs fold/prefix/scan/merge at nb bins, f64 ranking everywhere.
+    Returns (records[np,9], score[np], depth[np], winner[np,4], status)."""
+    t, v, w = _f64(t), _f64(v), _f64(w)
+    periods = _f64(periods)
+    durations, rho, ingress = _f64(durations), _f64(rho), _f64(ingress)
+    np_ = len(periods)
+    rec = np.empty((np_, 9))
+    score = np.empty(np_)
+    depth = np.empty(np_)
+    win = np.empty((np_, 4), dtype=np.uint32)
+    st = lib().orc_scan_bank(
+        _dp(t), _dp(v), _dp(w), _sz(len(t)), _d(baseline_norm), _dp(periods),
+        _sz(np_), _dp(durations), _dp(rho), _sz(len(durations)), _dp(ingress),
+        _sz(len(ingress)), _u32(layers), _d(gate), _u32(nb), _dp(rec),
+        _dp(score), _dp(depth), win.ctypes.data_as(_P(_u32)))
+    return rec, score, depth, win, st
+
+
+def _bank_cfg(cfg):
+    b = BankCfg()
+    b.layers = cfg.shape_layers
+    b.phase_oversample = cfg.phase_oversample
+    b.period_batch = cfg.period_batch
+    b.prescan_factor = cfg.prescan_factor
+    b.threshold = cfg.threshold
+    b.shape_gate = cfg.shape_gate
+    b.max_fold_mib = cfg.max_fold_mib
+    b.prescan_fraction = cfg.prescan_fraction
+    return b
+
+
+def scan_periods(t, v, w, baseline_norm, origin, periods, durations, rho,
+                 ingress, cfg):
+    t, v, w, periods = _f64(t), _f64(v), _f64(w), _f64(periods)
+    durations, rho, ingress = _f64(durations), _f64(rho), _f64(ingress)
+    rows = (Row * max(1, len(periods)))()
+    nrows = _sz()
+    stats = ScanStats()
+    b = _bank_cfg(cfg)
+    st = lib().orc_scan_periods(
+        _dp(t), _dp(v), _dp(w), _sz(len(t)), _d(baseline_norm), _d(origin),
+        _dp(periods), _sz(len(periods)), _dp(durations), _dp(rho),
+        _sz(len(durations)), _dp(ingress), _sz(len(ingress)),
+        ctypes.byref(b), rows, ctypes.byref(nrows), ctypes.byref(stats))
+    out = np.array([(r.period, r.epoch, r.duration, r.score, r.depth)
+                    for r in rows[:nrows.value]]).reshape(-1, 5)
+    return out, stats, st
+
+
+def signal_search(t, v, w, baseline_norm, origin, durations, rho, ingress,
+                  cfg, period_min, period_max, period_oversample=3.0):
+    t, v, w = _f64(t), _f64(v), _f64(w)
+    durations, rho, ingress = _f64(durations), _f64(rho), _f64(ingress)
+    b = _bank_cfg(cfg)
+    s = SearchCfg(period_min, period_max, period_oversample)
+    maximum = min(period_max, t[-1])
+    minimum = max(period_min, 4.0 * durations.min() * (1.0 + 1e-9))
+    cap = 1
+    if maximum > minimum:
+        cap = lib().orc_period_grid_count(minimum, maximum, t[-1],
+                                          durations.min(), period_oversample)
+    rows = (Row * cap)()
+    stats = (ScanStats * 64)()
+    nrows, nbands, gcount = _sz(), _sz(), _sz()
+    st = lib().orc_signal_search(
+        _dp(t), _dp(v), _dp(w), _sz(len(t)), _d(baseline_norm), _d(origin),
+        _dp(durations), _dp(rho), _sz(len(durations)), _dp(ingress),
+        _sz(len(ingress)), ctypes.byref(b), ctypes.byref(s), rows, _sz(cap),
+        ctypes.byref(nrows), stats, ctypes.byref(nbands), ctypes.byref(gcount))
+    out = np.array([(r.period, r.epoch, r.duration, r.score, r.depth)
+                    for r in rows[:nrows.value]]).reshape(-1, 5)
+    band_stats = [dict(periods=x.periods, prescan_survivors=x.prescan_survivors,
+                       phase_bins=x.phase_bins, batches=x.batches,
+                       templates_scanned=x.templates_scanned)
+                  for x in stats[:nbands.value]]
+    return out, band_stats, gcount.value, st
+
+
+def bank_batches(np_, nb, period_batch, max_fold_mib):
+    return lib().orc_bank_batches(np_, nb, period_batch, max_fold_mib)
+
+
+class Rng:
+    """survey/numeric.rs:345-364 restated in C."""
+
+    def __init__(self, seed):
+        self.state = _u64(lib().orc_rng_new(seed))
+
+    def uniform(self):
+        return lib().orc_rng_uniform(ctypes.byref(self.state))
+
+    def normal(self, mean, sigma):
+        return lib().orc_rng_normal(ctypes.byref(self.state), mean, sigma)

@@ -1,0 +1,141 @@
+/*
+ * FP64 CPU oracle of the FBLS per-family period scan.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (fbls_amd/, include/fbls.h) never links or calls it.
+ *
+ * Every function restates the reference implementation (isakruas/fbls) line
+ * by line; citations are path:line in that repository.  Summation order is
+ * sequential and deterministic.  Compile with -ffp-contract=off: every
+ * multiply-add below must round twice, as the reference's Rust code does.
+ *
+ * Pinning (how each function is checked, see tests/test_oracle.py):
+ *   orc_period_grid          grid.rs:34-39 KAT + search.json band fixture
+ *   orc_durations_days       config.rs:94 rounding + search.json config hours
+ *   orc_median_lower/mad     closed forms on small vectors
+ *   orc_fit_segment_noise    curve.rs:636-661 KAT (noise_model_normalizes_and_inflates)
+ *   orc_prepare_ready        closed forms (sum v = 0 up to rounding, C = sum p)
+ *   orc_correlated_noise     closed forms on constructed residuals
+ *   orc_phase_cell           exact integer brute force (long double / i128)
+ *   orc_scan_bank            brute force over all (j, k) templates in a
+ *                            separate pure-Python restatement, the closed
+ *                            form box depth/SNR on noiseless injections,
+ *                            staircase sums vs an explicit dot product
+ *   orc_scan_periods         prescan survivor logic vs two orc_scan_bank calls
+ *   orc_signal_search        search.json fixture (nb, periods, batches,
+ *                            templates_scanned per band)
+ *   orc_rng_*                survey/numeric.rs:345-364 restated; stream KAT
+ */
+#ifndef FBLS_ORACLE_H
+#define FBLS_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* config.rs:94 / config.rs:177-182 */
+size_t orc_durations_days(const double* hours, size_t n, double* out_days);
+
+/* grid.rs:5-27 */
+size_t orc_period_grid_count(double minimum, double maximum, double baseline,
+                             double minimum_duration, double oversample);
+void orc_period_grid(double minimum, double maximum, double baseline,
+                     double minimum_duration, double oversample, double* out,
+                     size_t count);
+
+/* stats.rs:4-18 */
+double orc_median_lower(const double* values, size_t n);
+double orc_mad_scale(const double* values, size_t n);
+
+/* curve.rs:89-198 without binning (bin_minutes = 0) and without groups.
+ * Outputs are sorted, origin-relative; returns 0 or a nonzero error code. */
+int orc_fit_segment_noise(const double* time, const double* flux,
+                          const double* err /* may be NULL */, size_t n,
+                          double gap_days, double* out_time,
+                          double* out_deficit, double* out_variance,
+                          size_t* out_n, double* origin, double* inflation,
+                          size_t* segments);
+
+/* curve.rs:427-450 */
+int orc_prepare_ready(const double* deficit, const double* variance, size_t n,
+                      double* v, double* w, double* baseline_norm,
+                      double* residual_energy);
+
+/* curve.rs:456-539 */
+void orc_correlated_noise_factors(const double* time, const double* residual,
+                                  const double* sigma, size_t n,
+                                  const double* durations, size_t nd,
+                                  double* rho);
+
+/* Fixed-point step of the build's input representation: q = rint(x * 2^e),
+ * e = 30 - ceil-exponent(max |x|), so |q| <= 2^30.  Returns e. */
+int orc_quantize(const double* x, size_t n, int32_t* q);
+
+/* kernels.rs:314-327: floor(t * scale) mod nb */
+uint32_t orc_phase_cell(double t, double scale, uint32_t nb);
+
+/* kernels.rs:464-845,1113-1155 with the f64 ("exact") ranking everywhere.
+ * records: np x 9 {P, scale, phase, width, a, norm, d, ingress, rho};
+ * winner: np x 4 u32 {phase, width, shape, duration index} (may be NULL).
+ * Returns 0, or 2 when a valid template had nonpositive precision
+ * (scan.rs:335-337). */
+int orc_scan_bank(const double* t, const double* v, const double* w, size_t n,
+                  double baseline_norm, const double* periods, size_t np,
+                  const double* durations, const double* rho, size_t nd,
+                  const double* ingress, size_t ns, uint32_t layers,
+                  double gate, uint32_t nb, double* records, double* score,
+                  double* depth, uint32_t* winner);
+
+typedef struct {
+  uint64_t periods, prescan_survivors, phase_bins, batches, templates_scanned;
+} orc_scan_stats;
+
+typedef struct {
+  double period, epoch, duration, score, depth;
+} orc_row;
+
+typedef struct {
+  uint32_t layers, phase_oversample, period_batch, prescan_factor;
+  double threshold, shape_gate, max_fold_mib, prescan_fraction;
+} orc_bank_cfg;
+
+/* scan.rs:363-408.  rows must hold np entries; returns status. */
+int orc_scan_periods(const double* t, const double* v, const double* w,
+                     size_t n, double baseline_norm, double origin,
+                     const double* periods, size_t np, const double* durations,
+                     const double* rho, size_t nd, const double* ingress,
+                     size_t ns, const orc_bank_cfg* cfg, orc_row* rows,
+                     size_t* nrows, orc_scan_stats* stats);
+
+typedef struct {
+  double period_min, period_max, period_oversample;
+} orc_search_cfg;
+
+/* scan.rs:413-473 for one family.  durations/rho unsorted as given.
+ * rows must hold grid-count entries; stats at most 64 bands. */
+int orc_signal_search(const double* t, const double* v, const double* w,
+                      size_t n, double baseline_norm, double origin,
+                      const double* durations, const double* rho, size_t nd,
+                      const double* ingress, size_t ns,
+                      const orc_bank_cfg* bank, const orc_search_cfg* scfg,
+                      orc_row* rows, size_t rows_cap, size_t* nrows,
+                      orc_scan_stats* stats, size_t* nbands,
+                      size_t* grid_count);
+
+/* batch count of scan_bank's memory budget, scan.rs:177-183 */
+uint64_t orc_bank_batches(size_t np, uint32_t nb, uint32_t period_batch,
+                          double max_fold_mib);
+
+/* survey/numeric.rs:345-364 */
+uint64_t orc_rng_new(uint64_t seed);
+double orc_rng_uniform(uint64_t* state);
+double orc_rng_normal(uint64_t* state, double mean, double sigma);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
