@@ -48,7 +48,8 @@ EXPORTS = (
     "fbls_family_fields", "fbls_scan_bank", "fbls_scan_periods",
     "fbls_signal_search", "fbls_period_grid_count", "fbls_period_grid",
     "fbls_durations_days", "fbls_family_state", "fbls_quantize",
-    "fbls_synth_star", "fbls_last_error",
+    "fbls_synth_star", "fbls_last_error", "fbls_plan_bands",
+    "fbls_bank_batches",
 )
 
 
@@ -89,6 +90,11 @@ class SearchCfgC(ctypes.Structure):
                 ("period_oversample", _d)]
 
 
+class BandC(ctypes.Structure):
+    _fields_ = [("lo", _d), ("first", _sz), ("count", _sz), ("nd", _sz),
+                ("nb", _u32), ("nb_coarse", _u32)]
+
+
 class SynthCfg(ctypes.Structure):
     _fields_ = [("seed", _u64), ("sectors", _u32), ("sector_days", _d),
                 ("gap_days", _d), ("cadence_days", _d), ("t0", _d),
@@ -123,7 +129,7 @@ def lib():
         L.fbls_family_free.argtypes = [_vp]
         L.fbls_family_free.restype = None
         L.fbls_mem_info.argtypes = [_vp, _P(_sz), _P(_sz)]
-        L.fbls_ctx_set_launch.argtypes = [_vp, _u32, _u32]
+        L.fbls_ctx_set_launch.argtypes = [_vp, _u32, _u32, _u32]
         L.fbls_family_upload.argtypes = [_vp, _P(_d), _P(_d), _P(_d), _sz, _d,
                                          _d, _P(_vp)]
         L.fbls_family_fields.argtypes = [_vp, _P(_d), _P(_d),
@@ -141,6 +147,10 @@ def lib():
             _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz)]
         L.fbls_synth_star.argtypes = [_P(SynthCfg), _P(_d), _P(_d), _P(_d),
                                       _sz, _P(_sz)]
+        L.fbls_plan_bands.argtypes = [_P(_d), _sz, _P(_d), _sz, _u32, _u32,
+                                      _P(BandC), _sz, _P(_sz)]
+        L.fbls_bank_batches.restype = _u64
+        L.fbls_bank_batches.argtypes = [_sz, _u32, _u32, _d]
         _lib = L
     return _lib
 
@@ -232,6 +242,27 @@ def period_grid(minimum, maximum, baseline, minimum_duration, oversample):
     return out
 
 
+def plan_bands(grid, sorted_durations, phase_oversample=4, prescan_factor=2):
+    """Octave bands of one family's grid (scan.rs:440-471, 377-381)."""
+    grid, sd = _f64(grid), _f64(sorted_durations)
+    n = _sz()
+    L = lib()
+    _check(L.fbls_plan_bands(_dp(grid), len(grid), _dp(sd), len(sd),
+                             phase_oversample, prescan_factor, None, 0,
+                             ctypes.byref(n)))
+    out = (BandC * max(1, n.value))()
+    _check(L.fbls_plan_bands(_dp(grid), len(grid), _dp(sd), len(sd),
+                             phase_oversample, prescan_factor, out, n.value,
+                             ctypes.byref(n)))
+    return [dict(lo=b.lo, first=b.first, count=b.count, nd=b.nd, nb=b.nb,
+                 nb_coarse=b.nb_coarse) for b in out[:n.value]]
+
+
+def bank_batches(np_, nb, period_batch=4096, max_fold_mib=1024.0):
+    """scan.rs:177-183: batches of one scan_bank call (0: over budget)."""
+    return lib().fbls_bank_batches(np_, nb, period_batch, max_fold_mib)
+
+
 def quantize(x):
     x = _f64(x)
     q = np.empty(len(x), dtype=np.int32)
@@ -304,8 +335,8 @@ class Gpu:
         _check(lib().fbls_mem_info(self._h, ctypes.byref(fr), ctypes.byref(tot)))
         return fr.value, tot.value
 
-    def set_launch(self, lds_bytes=0, block=0):
-        _check(lib().fbls_ctx_set_launch(self._h, lds_bytes, block))
+    def set_launch(self, lds_bytes=0, block=0, fold_mode=0):
+        _check(lib().fbls_ctx_set_launch(self._h, lds_bytes, block, fold_mode))
 
     def close(self):
         if self._h:

@@ -65,6 +65,28 @@ struct DevBuf {
   }
 };
 
+// Growable pinned host buffer (staging of the per-launch tables/results).
+struct HostBuf {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return FBLS_OK;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes + bytes / 2, 1 << 16);
+    if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess)
+      return fail(FBLS_E_OOM, "pinned host allocation of %zu bytes failed", want);
+    cap = want;
+    return FBLS_OK;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+};
+
 constexpr size_t kMaxLds = 160 * 1024;  // gfx950 LDS per CU
 constexpr uint32_t kObsAlign = 512;     // observations per wave block
 
@@ -76,7 +98,9 @@ struct fbls_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t lds_override = 0, block_override = 0;
   size_t max_lds = kMaxLds;
-  DevBuf periods, scales, records, score, depth, winner, counters, bank;
+  uint32_t run_fold = 1;
+  DevBuf dev_in, dev_out;
+  HostBuf pin_in, pin_out;
 };
 
 struct fbls_family {
@@ -126,9 +150,10 @@ extern "C" void fbls_ctx_destroy(fbls_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->periods, &c->scales, &c->records, &c->score,
-                    &c->depth, &c->winner, &c->counters, &c->bank})
-    b->release();
+  c->dev_in.release();
+  c->dev_out.release();
+  c->pin_in.release();
+  c->pin_out.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -143,14 +168,16 @@ extern "C" int fbls_mem_info(fbls_ctx* c, size_t* free_bytes, size_t* total) {
 }
 
 extern "C" int fbls_ctx_set_launch(fbls_ctx* c, uint32_t lds_bytes,
-                                   uint32_t block) {
+                                   uint32_t block, uint32_t fold_mode) {
   if (!c) return fail(FBLS_E_ARG, "null context");
+  if (fold_mode > 2) return fail(FBLS_E_ARG, "fold_mode must be 0, 1 or 2");
   if (block != 0 && block != 256 && block != 512)
     return fail(FBLS_E_ARG, "block must be 0, 256 or 512");
   if (lds_bytes > c->max_lds)
     return fail(FBLS_E_ARG, "lds_bytes %u exceeds %zu", lds_bytes, c->max_lds);
   c->lds_override = lds_bytes;
   c->block_override = block;
+  c->run_fold = fold_mode == 2 ? 0u : 1u;
   return FBLS_OK;
 }
 
@@ -263,152 +290,220 @@ uint64_t bank_batch(uint32_t nb, uint32_t period_batch, double max_fold_mib) {
   return (uint64_t)std::floor((max_fold_mib * 1048576.0) / per_period);
 }
 
-struct Launch {
-  uint32_t G, block;
-  size_t lds;
-  uint32_t stride;
+// One unit of scan work: the periods of one reference scan_bank call.
+struct Work {
+  const fbls_family* fam;
+  const double* periods;
+  size_t np;
+  uint32_t nb, nd;
+  const double* dur;  // nd (ascending)
+  const double* rho;  // nd
+  double* records;    // np x 9 (host)
+  double* score;      // np
+  double* depth;      // np
+  uint32_t* winner;   // np x 4 or null
+  uint32_t retries = 0;
+  double cost = 0.0;  // work estimate used to apportion launch time
 };
 
-int plan_launch(fbls_ctx* c, uint32_t nb, size_t np, const fbls_bank* b,
-                Launch* L) {
-  uint32_t nd = (uint32_t)b->nd, ns = (uint32_t)b->ns, layers = b->layers;
-  size_t one = fbls::scan_lds_bytes(nb, 1, nd, ns, layers, &L->stride);
-  if (one > c->max_lds)
-    return fail(FBLS_E_ARG,
-                "%u phase bins need %zu bytes of LDS per period (limit %zu)",
-                nb, one, c->max_lds);
-  size_t budget = c->lds_override ? c->lds_override : 64 * 1024;
-  if (budget < one) budget = one;
-  uint32_t G = (uint32_t)std::max<size_t>(1, budget / one);
-  while (G > 1 && fbls::scan_lds_bytes(nb, G, nd, ns, layers, nullptr) > budget) G--;
-  G = std::min<uint32_t>(G, 16);
-  // keep at least ~2 workgroups per CU in flight when there is enough work
-  uint32_t want_groups = 512;
-  uint32_t cap = (uint32_t)std::max<size_t>(1, np / want_groups);
-  G = std::max<uint32_t>(1, std::min(G, cap));
-  L->G = G;
-  L->lds = fbls::scan_lds_bytes(nb, G, nd, ns, layers, nullptr);
-  L->block = c->block_override ? c->block_override : 512;
-  return FBLS_OK;
-}
+struct Shared {
+  uint32_t ns, layers;
+  const double* ingress;
+  double gate;
+};
 
-// One scan of `np` periods at `nb` bins: the work of one scan_bank call.
-int run_scan(fbls_ctx* c, const fbls_family* f, const double* periods,
-             size_t np, const fbls_bank* b, uint32_t nb, double* records,
-             double* score, double* depth, uint32_t* winner,
-             fbls_scan_stats* stats) {
-  if (np == 0) return FBLS_OK;
-  if (nb < 2 || nb >= 65536)
-    return fail(FBLS_E_ARG, "phase bins must be in [2, 65535], got %u", nb);
-  if (np > 0xffffffffu) return fail(FBLS_E_ARG, "too many periods");
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// Runs every Work in one kernel launch.  Writes each Work's host outputs and
+// retry count; returns the launch's device time through *seconds.
+int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
+             double* seconds) {
+  *seconds = 0.0;
+  size_t total = 0;
+  for (size_t i = 0; i < nw; i++) total += w[i].np;
+  if (total == 0) return FBLS_OK;
+  if (total > 0x7fffffffu) return fail(FBLS_E_ARG, "too many periods in one launch");
   HIP_TRY(hipSetDevice(c->device));
-  Launch L;
-  int st = plan_launch(c, nb, np, b, &L);
-  if (st) return st;
-  const uint32_t nd = (uint32_t)b->nd, ns = (uint32_t)b->ns;
-  std::vector<double> scales(np);
-  double smax = 0.0;
-  for (size_t i = 0; i < np; i++) {
-    if (!(periods[i] > 0.0) || !std::isfinite(periods[i]))
-      return fail(FBLS_E_ARG, "periods must be positive and finite");
-    scales[i] = (double)nb / periods[i];  // scan.rs:204
-    smax = std::max(smax, scales[i]);
+  const uint32_t ns = sh.ns, layers = sh.layers;
+  // LDS size of the launch and periods per workgroup of each segment
+  size_t lds = c->lds_override ? c->lds_override : 64 * 1024;
+  std::vector<size_t> one(nw);
+  for (size_t i = 0; i < nw; i++) {
+    if (w[i].np == 0) continue;
+    if (w[i].nb < 2 || w[i].nb >= 65536)
+      return fail(FBLS_E_ARG, "phase bins must be in [2, 65535], got %u", w[i].nb);
+    one[i] = fbls::scan_lds_bytes(w[i].nb, 1, w[i].nd, ns, layers, nullptr);
+    if (one[i] > c->max_lds)
+      return fail(FBLS_E_ARG, "%u phase bins need %zu bytes of LDS per period (limit %zu)",
+                  w[i].nb, one[i], c->max_lds);
+    lds = std::max(lds, one[i]);
   }
-  // bank arrays: dur | rho | ingress | factors (scan.rs:157-171)
-  std::vector<double> bank(2 * nd + ns + ns * b->layers);
-  for (uint32_t i = 0; i < nd; i++) {
-    bank[i] = b->durations[i];
-    bank[nd + i] = b->rho[i];
-  }
-  for (uint32_t s = 0; s < ns; s++) {
-    bank[2 * nd + s] = b->ingress[s];
-    for (uint32_t l = 0; l < b->layers; l++)
-      bank[2 * nd + ns + s * b->layers + l] =
-          1.0 - 2.0 * b->ingress[s] *
-                    (b->layers > 1 ? (double)l / (double)(b->layers - 1) : 0.0);
-  }
-  if ((st = c->periods.ensure(np * 8)) || (st = c->scales.ensure(np * 8)) ||
-      (st = c->records.ensure(np * 9 * 8)) || (st = c->score.ensure(np * 8)) ||
-      (st = c->depth.ensure(np * 8)) ||
-      (st = c->winner.ensure(winner ? np * 16 : 16)) ||
-      (st = c->counters.ensure(64)) || (st = c->bank.ensure(bank.size() * 8)))
+  const uint32_t g_cap = (uint32_t)std::max<size_t>(1, total / 1024);
+  std::vector<fbls::Segment> segs;
+  std::vector<size_t> seg_work;
+  segs.reserve(nw);
+  uint32_t groups = 0;
+  size_t lds_max = 0;
+  // staging layout (doubles): per segment periods | scales | dur | rho, then
+  // ingress | factors, then the segment table and zeroed counters
+  size_t in_doubles = ns + ns * layers;
+  for (size_t i = 0; i < nw; i++)
+    if (w[i].np) in_doubles += 2 * w[i].np + 2 * w[i].nd;
+  size_t nseg = 0;
+  for (size_t i = 0; i < nw; i++) nseg += w[i].np ? 1 : 0;
+  const size_t seg_off = align16(in_doubles * 8);
+  const size_t cnt_off = align16(seg_off + nseg * sizeof(fbls::Segment));
+  const size_t in_bytes = cnt_off + align16(nseg * 8);
+  const size_t out_bytes = total * (9 + 2) * 8 + total * 16;
+  int st;
+  if ((st = c->dev_in.ensure(in_bytes)) || (st = c->dev_out.ensure(out_bytes)) ||
+      (st = c->pin_in.ensure(in_bytes)) || (st = c->pin_out.ensure(out_bytes)))
     return st;
-  HIP_TRY(hipMemcpyAsync(c->periods.ptr, periods, np * 8, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->scales.ptr, scales.data(), np * 8, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->bank.ptr, bank.data(), bank.size() * 8, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemsetAsync(c->counters.ptr, 0, 64, c->stream));
-  fbls::ScanArgs A;
+  unsigned char* hin = (unsigned char*)c->pin_in.ptr;
+  unsigned char* din = (unsigned char*)c->dev_in.ptr;
+  unsigned char* dout = (unsigned char*)c->dev_out.ptr;
+  double* hd = (double*)hin;
+  size_t cur = 0;
+  const size_t ing_at = cur;
+  for (uint32_t s = 0; s < ns; s++) hd[cur++] = sh.ingress[s];
+  const size_t fac_at = cur;
+  for (uint32_t s = 0; s < ns; s++)
+    for (uint32_t l = 0; l < layers; l++)  // scan.rs:157-171
+      hd[cur++] = 1.0 - 2.0 * sh.ingress[s] *
+                            (layers > 1 ? (double)l / (double)(layers - 1) : 0.0);
+  size_t out_at = 0;  // in periods
+  const size_t rec_base = 0, score_base = total * 9 * 8,
+               depth_base = total * 10 * 8, win_base = total * 11 * 8;
+  for (size_t i = 0; i < nw; i++) {
+    const Work& x = w[i];
+    if (x.np == 0) continue;
+    const fbls_family* f = x.fam;
+    fbls::Segment S;
+    std::memset(&S, 0, sizeof(S));
+    const size_t per_at = cur;
+    double smax = 0.0;
+    for (size_t k = 0; k < x.np; k++) {
+      const double P = x.periods[k];
+      if (!(P > 0.0) || !std::isfinite(P))
+        return fail(FBLS_E_ARG, "periods must be positive and finite");
+      hd[cur + k] = P;
+      const double sc = (double)x.nb / P;  // scan.rs:204
+      hd[cur + x.np + k] = sc;
+      smax = std::max(smax, sc);
+    }
+    cur += 2 * x.np;
+    const size_t dur_at = cur;
+    for (uint32_t k = 0; k < x.nd; k++) {
+      hd[cur + k] = x.dur[k];
+      hd[cur + x.nd + k] = x.rho[k];
+    }
+    cur += 2 * x.nd;
+    if (f->t_max * smax >= 4294967295.0)
+      return fail(FBLS_E_ARG, "t * nb / P exceeds 2^32");
+    S.t = (const double*)f->t.ptr;
+    S.vw = (const int32_t*)f->vw.ptr;
+    S.periods = (const double*)(din + per_at * 8);
+    S.scales = S.periods + x.np;
+    S.dur = (const double*)(din + dur_at * 8);
+    S.rho = S.dur + x.nd;
+    S.records = (double*)(dout + rec_base) + out_at * 9;
+    S.score = (double*)(dout + score_base) + out_at;
+    S.depth = (double*)(dout + depth_base) + out_at;
+    S.winner = x.winner ? (uint32_t*)(dout + win_base) + out_at * 4 : nullptr;
+    S.sv = std::ldexp(1.0, -f->ev);
+    S.sw = std::ldexp(1.0, -f->ew);
+    S.C = f->C;
+    S.invC = 1.0 / f->C;
+    S.n = f->n;
+    S.n_pad = f->n_pad;
+    S.np = (uint32_t)x.np;
+    S.nb = x.nb;
+    S.nd = x.nd;
+    uint32_t G = (uint32_t)std::max<size_t>(1, lds / one[i]);
+    G = std::min<uint32_t>(std::min<uint32_t>(G, 16), g_cap);
+    while (G > 1 && fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr) > lds) G--;
+    S.G = G;
+    fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, &S.pstride);
+    lds_max = std::max(lds_max, fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr));
+    S.wide_cells = (f->t_max * smax >= 16777215.0) ? 1u : 0u;
+    S.group0 = groups;
+    S.index = (uint32_t)segs.size();
+    groups += (uint32_t)((x.np + G - 1) / G);
+    segs.push_back(S);
+    seg_work.push_back(i);
+    out_at += x.np;
+  }
+  std::memcpy(hin + seg_off, segs.data(), segs.size() * sizeof(fbls::Segment));
+  std::memset(hin + cnt_off, 0, align16(nseg * 8));
+  fbls::LaunchArgs A;
   std::memset(&A, 0, sizeof(A));
-  A.t = (const double*)f->t.ptr;
-  A.vw = (const int32_t*)f->vw.ptr;
-  A.n = f->n;
-  A.n_pad = f->n_pad;
-  A.sv = std::ldexp(1.0, -f->ev);
-  A.sw = std::ldexp(1.0, -f->ew);
-  A.C = f->C;
-  A.invC = 1.0 / f->C;
-  A.periods = (const double*)c->periods.ptr;
-  A.scales = (const double*)c->scales.ptr;
-  A.np = (uint32_t)np;
-  A.nb = nb;
-  A.G = L.G;
-  A.nd = nd;
+  A.segs = (const fbls::Segment*)(din + seg_off);
+  A.nseg = (uint32_t)segs.size();
   A.ns = ns;
-  A.layers = b->layers;
-  const double* db = (const double*)c->bank.ptr;
-  A.dur = db;
-  A.rho = db + nd;
-  A.ingress = db + 2 * nd;
-  A.factors = db + 2 * nd + ns;
-  A.gate = b->shape_gate;
-  A.wide_cells = (f->t_max * smax >= 16777215.0) ? 1u : 0u;
-  if (f->t_max * smax >= 4294967295.0)
-    return fail(FBLS_E_ARG, "t * nb / P exceeds 2^32");
-  A.period_stride = L.stride;
-  A.records = (double*)c->records.ptr;
-  A.score = (double*)c->score.ptr;
-  A.depth = (double*)c->depth.ptr;
-  A.winner = winner ? (uint32_t*)c->winner.ptr : nullptr;
-  A.counters = (unsigned int*)c->counters.ptr;
+  A.layers = layers;
+  A.run_fold = c->run_fold;
+  A.ingress = (const double*)(din + ing_at * 8);
+  A.factors = (const double*)(din + fac_at * 8);
+  A.gate = sh.gate;
+  A.counters = (unsigned int*)(din + cnt_off);
+  const uint32_t block = c->block_override ? c->block_override : 512;
+  HIP_TRY(hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  HIP_TRY(fbls::scan_launch(A, L.block, L.lds, c->stream));
+  HIP_TRY(fbls::scan_launch(A, groups, block, lds_max, c->stream));
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
-  unsigned int counters[2];
-  HIP_TRY(hipMemcpyAsync(records, c->records.ptr, np * 9 * 8, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(score, c->score.ptr, np * 8, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(depth, c->depth.ptr, np * 8, hipMemcpyDeviceToHost, c->stream));
-  if (winner)
-    HIP_TRY(hipMemcpyAsync(winner, c->winner.ptr, np * 16, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(counters, c->counters.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+  bool any_winner = false;
+  for (size_t i = 0; i < nw; i++) any_winner = any_winner || (w[i].winner && w[i].np);
+  const size_t copy_bytes = any_winner ? out_bytes : win_base;
+  HIP_TRY(hipMemcpyAsync(c->pin_out.ptr, dout, copy_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(hin + cnt_off, din + cnt_off, nseg * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  if (counters[0])
+  *seconds = ms * 1e-3;
+  const unsigned char* hout = (const unsigned char*)c->pin_out.ptr;
+  const unsigned int* cnt = (const unsigned int*)(hin + cnt_off);
+  bool bad = false;
+  out_at = 0;
+  for (size_t s = 0; s < segs.size(); s++) {
+    Work& x = w[seg_work[s]];
+    std::memcpy(x.records, (const double*)(hout + rec_base) + out_at * 9, x.np * 72);
+    std::memcpy(x.score, (const double*)(hout + score_base) + out_at, x.np * 8);
+    std::memcpy(x.depth, (const double*)(hout + depth_base) + out_at, x.np * 8);
+    if (x.winner)
+      std::memcpy(x.winner, (const uint32_t*)(hout + win_base) + out_at * 4, x.np * 16);
+    bad = bad || cnt[2 * s] != 0;
+    x.retries = cnt[2 * s + 1];
+    out_at += x.np;
+  }
+  if (bad)
     return fail(FBLS_E_NONPOSITIVE_PRECISION,
                 "nonpositive template precision; inspect noise model");
-  if (stats) {
-    stats->exact_tile_retries += counters[1];
-    stats->seconds_device += ms * 1e-3;
-    stats->seconds_scan += ms * 1e-3;
-  }
   return FBLS_OK;
 }
 
-int scan_bank_impl(fbls_ctx* c, const fbls_family* f, const double* periods,
-                   size_t np, const fbls_bank* b, uint32_t nb, double* records,
-                   double* score, double* depth, uint32_t* winner,
-                   fbls_scan_stats* stats) {
-  uint64_t budget = bank_batch(nb, b->period_batch, b->max_fold_mib);
-  if (budget == 0) return fail(FBLS_E_FOLD_BUDGET, "one folded period exceeds max_fold_mib");
-  int st = run_scan(c, f, periods, np, b, nb, records, score, depth, winner, stats);
-  if (st) return st;
-  if (stats) {
-    stats->phase_bins = nb;
-    uint64_t batch = std::min<uint64_t>(std::min<uint64_t>(b->period_batch, budget), np);
-    if (batch) stats->batches += (np + batch - 1) / batch;
-    stats->templates_scanned += (uint64_t)np * b->nd * b->ns * nb;  // scan.rs:352
-  }
-  return FBLS_OK;
+// Work estimate of a segment: fold pairs plus box templates.
+double work_cost(const Work& x) {
+  return (double)x.np * ((double)x.fam->n + 3.0 * x.nb * x.nd);
+}
+
+void account(fbls_scan_stats* st, const Work& x, const fbls_bank* b,
+             uint64_t budget, double seconds) {
+  st->phase_bins = x.nb;
+  uint64_t batch = std::min<uint64_t>(std::min<uint64_t>(b->period_batch, budget), x.np);
+  if (batch) st->batches += (x.np + batch - 1) / batch;
+  st->templates_scanned += (uint64_t)x.np * x.nd * b->ns * x.nb;  // scan.rs:352
+  st->exact_tile_retries += x.retries;
+  st->seconds_device += seconds;
+  st->seconds_scan += seconds;
+}
+
+Shared shared_of(const fbls_bank* b) {
+  Shared sh;
+  sh.ns = (uint32_t)b->ns;
+  sh.layers = b->layers;
+  sh.ingress = b->ingress;
+  sh.gate = b->shape_gate;
+  return sh;
 }
 
 }  // namespace
@@ -422,8 +517,168 @@ extern "C" int fbls_scan_bank(fbls_ctx* c, const fbls_family* f,
     return fail(FBLS_E_ARG, "null argument");
   int st = check_bank(b);
   if (st) return st;
-  return scan_bank_impl(c, f, periods, np, b, nb, records, score, depth, winner, stats);
+  const uint64_t budget = bank_batch(nb, b->period_batch, b->max_fold_mib);
+  if (budget == 0) return fail(FBLS_E_FOLD_BUDGET, "one folded period exceeds max_fold_mib");
+  Work x;
+  x.fam = f;
+  x.periods = periods;
+  x.np = np;
+  x.nb = nb;
+  x.nd = (uint32_t)b->nd;
+  x.dur = b->durations;
+  x.rho = b->rho;
+  x.records = records;
+  x.score = score;
+  x.depth = depth;
+  x.winner = winner;
+  double secs = 0.0;
+  st = run_work(c, &x, 1, shared_of(b), &secs);
+  if (st) return st;
+  if (stats) account(stats, x, b, budget, secs);
+  return FBLS_OK;
 }
+
+namespace {
+
+// Host state of one band (one scan_periods call) across the two launches.
+struct BandRun {
+  const double* periods;
+  size_t np;
+  uint32_t nb, nb_coarse, nd;
+  const double* dur;
+  const double* rho;
+  std::vector<double> rec, score, depth, survivors;
+  fbls_scan_stats stats;
+};
+
+// scan.rs:363-408 for any number of bands of one or more families: every
+// band's first scan (coarse, or full when there is no prescan) in one
+// launch, then every band's full scan of the survivors in a second launch.
+int scan_bands(fbls_ctx* c, const fbls_family* f, std::vector<BandRun>& bands,
+               const fbls_bank* b, fbls_row* rows, size_t* nrows,
+               size_t* row_counts) {
+  const double t0 = now_s();
+  const Shared sh = shared_of(b);
+  std::vector<Work> first, second;
+  std::vector<size_t> first_band, second_band;
+  for (size_t i = 0; i < bands.size(); i++) {
+    BandRun& r = bands[i];
+    std::memset(&r.stats, 0, sizeof(r.stats));
+    r.stats.periods = r.np;
+    if (r.np == 0 || r.nd == 0) continue;
+    const uint32_t nb0 = r.nb_coarse ? r.nb_coarse : r.nb;
+    if (bank_batch(nb0, b->period_batch, b->max_fold_mib) == 0 ||
+        bank_batch(r.nb, b->period_batch, b->max_fold_mib) == 0)
+      return fail(FBLS_E_FOLD_BUDGET, "one folded period exceeds max_fold_mib");
+    r.rec.resize(r.np * 9);
+    r.score.resize(r.np);
+    r.depth.resize(r.np);
+    Work x;
+    x.fam = f;
+    x.periods = r.periods;
+    x.np = r.np;
+    x.nb = nb0;
+    x.nd = r.nd;
+    x.dur = r.dur;
+    x.rho = r.rho;
+    x.records = r.rec.data();
+    x.score = r.score.data();
+    x.depth = r.depth.data();
+    x.winner = nullptr;
+    x.cost = work_cost(x);
+    first.push_back(x);
+    first_band.push_back(i);
+  }
+  double s1 = 0.0;
+  int st = run_work(c, first.data(), first.size(), sh, &s1);
+  if (st) return st;
+  double c1 = 0.0;
+  for (const Work& x : first) c1 += x.cost;
+  const double cut = b->prescan_fraction * b->threshold;
+  std::vector<std::vector<double>> rec2(bands.size()), sc2(bands.size()), dp2(bands.size());
+  for (size_t k = 0; k < first.size(); k++) {
+    BandRun& r = bands[first_band[k]];
+    const Work& x = first[k];
+    account(&r.stats, x, b, bank_batch(x.nb, b->period_batch, b->max_fold_mib),
+            c1 > 0 ? s1 * x.cost / c1 : 0.0);
+    if (!r.nb_coarse) continue;
+    r.survivors.clear();
+    for (size_t i = 0; i < r.np; i++)  // scan.rs:390-395
+      if (std::isfinite(r.score[i]) && r.score[i] >= cut) r.survivors.push_back(r.rec[9 * i]);
+    r.stats.prescan_survivors = r.survivors.size();
+    if (r.survivors.empty()) continue;
+    const size_t m = r.survivors.size();
+    rec2[first_band[k]].resize(m * 9);
+    sc2[first_band[k]].resize(m);
+    dp2[first_band[k]].resize(m);
+    Work y;
+    y.fam = f;
+    y.periods = r.survivors.data();
+    y.np = m;
+    y.nb = r.nb;
+    y.nd = r.nd;
+    y.dur = r.dur;
+    y.rho = r.rho;
+    y.records = rec2[first_band[k]].data();
+    y.score = sc2[first_band[k]].data();
+    y.depth = dp2[first_band[k]].data();
+    y.winner = nullptr;
+    y.cost = work_cost(y);
+    second.push_back(y);
+    second_band.push_back(first_band[k]);
+  }
+  double s2 = 0.0;
+  if (!second.empty()) {
+    st = run_work(c, second.data(), second.size(), sh, &s2);
+    if (st) return st;
+  }
+  double c2 = 0.0;
+  for (const Work& x : second) c2 += x.cost;
+  for (size_t k = 0; k < second.size(); k++) {
+    BandRun& r = bands[second_band[k]];
+    const Work& x = second[k];
+    account(&r.stats, x, b, bank_batch(x.nb, b->period_batch, b->max_fold_mib),
+            c2 > 0 ? s2 * x.cost / c2 : 0.0);
+  }
+  // rows in band order (scan.rs:397-405, 466-467)
+  *nrows = 0;
+  for (size_t i = 0; i < bands.size(); i++) {
+    BandRun& r = bands[i];
+    const bool full_in_first = r.nb_coarse == 0;
+    const size_t m = full_in_first ? (r.nd ? r.np : 0) : r.survivors.size();
+    const double* rec = full_in_first ? r.rec.data() : rec2[i].data();
+    const double* sc = full_in_first ? r.score.data() : sc2[i].data();
+    const double* dp = full_in_first ? r.depth.data() : dp2[i].data();
+    size_t got = 0;
+    for (size_t k = 0; k < m; k++) {
+      if (!(std::isfinite(sc[k]) && sc[k] >= b->threshold)) continue;
+      const double* q = rec + 9 * k;
+      fbls_row& row = rows[(*nrows)++];
+      row.period = q[0];
+      row.epoch = (q[2] + q[3] / 2.0) / q[1] + f->origin;  // scan.rs:345
+      row.duration = q[3] / q[1];
+      row.score = sc[k];
+      row.depth = dp[k];
+      got++;
+    }
+    if (row_counts) row_counts[i] = got;
+  }
+  // wall time of the two launches, apportioned like the device time
+  const double wall = now_s() - t0;
+  double sdev = s1 + s2;
+  for (BandRun& r : bands)
+    r.stats.seconds = sdev > 0 ? wall * r.stats.seconds_device / sdev : 0.0;
+  return FBLS_OK;
+}
+
+uint32_t band_nb(const double* periods, size_t np, const fbls_bank* b) {
+  double dmin = INFINITY;
+  for (size_t i = 0; i < b->nd; i++) dmin = std::min(dmin, b->durations[i]);
+  const double nbf = std::ceil((double)b->phase_oversample * periods[np - 1] / dmin);
+  return nbf < 8.0 ? 8u : (nbf > 4e9 ? 0xffffffffu : (uint32_t)nbf);
+}
+
+}  // namespace
 
 // scan.rs:363-408
 extern "C" int fbls_scan_periods(fbls_ctx* c, const fbls_family* f,
@@ -432,51 +687,30 @@ extern "C" int fbls_scan_periods(fbls_ctx* c, const fbls_family* f,
                                  size_t* nrows, fbls_scan_stats* stats) {
   if (!c || !f || !nrows || (np && (!periods || !rows)))
     return fail(FBLS_E_ARG, "null argument");
-  double t0 = now_s();
-  fbls_scan_stats local;
-  std::memset(&local, 0, sizeof(local));
-  local.periods = np;
   *nrows = 0;
   if (np == 0 || b == nullptr || b->nd == 0) {
-    if (stats) *stats = local;
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->periods = np;
+    }
     return FBLS_OK;
   }
   int st = check_bank(b);
   if (st) return st;
-  double dmin = INFINITY;
-  for (size_t i = 0; i < b->nd; i++) dmin = std::min(dmin, b->durations[i]);
-  double nbf = std::ceil((double)b->phase_oversample * periods[np - 1] / dmin);
-  uint32_t nb = nbf < 8.0 ? 8u : (nbf > 4e9 ? 0xffffffffu : (uint32_t)nbf);
-  std::vector<double> survivors(periods, periods + np);
-  std::vector<double> rec(np * 9), score(np), depth(np);
-  if (b->prescan_factor > 1 && nb / b->prescan_factor >= 8) {
-    st = scan_bank_impl(c, f, periods, np, b, nb / b->prescan_factor, rec.data(),
-                        score.data(), depth.data(), nullptr, &local);
-    if (st) return st;
-    survivors.clear();
-    const double cut = b->prescan_fraction * b->threshold;
-    for (size_t i = 0; i < np; i++)
-      if (std::isfinite(score[i]) && score[i] >= cut) survivors.push_back(rec[9 * i]);
-    local.prescan_survivors = survivors.size();
-  }
-  if (!survivors.empty()) {
-    size_t ns = survivors.size();
-    st = scan_bank_impl(c, f, survivors.data(), ns, b, nb, rec.data(),
-                        score.data(), depth.data(), nullptr, &local);
-    if (st) return st;
-    for (size_t i = 0; i < ns; i++) {
-      if (!(std::isfinite(score[i]) && score[i] >= b->threshold)) continue;
-      const double* r = &rec[9 * i];
-      fbls_row& row = rows[(*nrows)++];
-      row.period = r[0];
-      row.epoch = (r[2] + r[3] / 2.0) / r[1] + f->origin;  // scan.rs:345
-      row.duration = r[3] / r[1];
-      row.score = score[i];
-      row.depth = depth[i];
-    }
-  }
-  local.seconds = now_s() - t0;
-  if (stats) *stats = local;
+  std::vector<BandRun> bands(1);
+  BandRun& r = bands[0];
+  r.periods = periods;
+  r.np = np;
+  r.nb = band_nb(periods, np, b);
+  r.nb_coarse = (b->prescan_factor > 1 && r.nb / b->prescan_factor >= 8)
+                    ? r.nb / b->prescan_factor
+                    : 0u;
+  r.nd = (uint32_t)b->nd;
+  r.dur = b->durations;
+  r.rho = b->rho;
+  st = scan_bands(c, f, bands, b, rows, nrows, nullptr);
+  if (st) return st;
+  if (stats) *stats = r.stats;
   return FBLS_OK;
 }
 
@@ -499,6 +733,32 @@ extern "C" int fbls_period_grid(double minimum, double maximum,
 extern "C" size_t fbls_durations_days(const double* hours, size_t n,
                                       double* out) {
   return fbls::durations_days(hours, n, out);
+}
+
+extern "C" int fbls_plan_bands(const double* grid, size_t count,
+                               const double* sorted_durations, size_t nd,
+                               uint32_t phase_oversample,
+                               uint32_t prescan_factor, fbls_band* bands,
+                               size_t bands_cap, size_t* nbands) {
+  if (!grid || !sorted_durations || !nbands) return fail(FBLS_E_ARG, "null argument");
+  std::vector<fbls_band> plan = fbls::plan_bands(grid, count, sorted_durations, nd,
+                                                 phase_oversample, prescan_factor);
+  *nbands = plan.size();
+  if (bands) {
+    if (plan.size() > bands_cap)
+      return fail(FBLS_E_ARG, "band capacity %zu < %zu", bands_cap, plan.size());
+    std::copy(plan.begin(), plan.end(), bands);
+  }
+  return FBLS_OK;
+}
+
+extern "C" uint64_t fbls_bank_batches(size_t np, uint32_t nb,
+                                      uint32_t period_batch,
+                                      double max_fold_mib) {
+  uint64_t budget = bank_batch(nb, period_batch, max_fold_mib);
+  if (budget == 0 || np == 0) return 0;
+  uint64_t batch = std::min<uint64_t>(std::min<uint64_t>(period_batch, budget), np);
+  return (np + batch - 1) / batch;
 }
 
 // scan.rs:413-473
@@ -545,33 +805,32 @@ extern "C" int fbls_signal_search(fbls_ctx* c, const fbls_family* f,
     sd[i] = ds[i].first;
     sr[i] = ds[i].second;
   }
-  double lo = grid[0];
-  const double last = grid[count - 1];
-  std::vector<double> sel;
-  while (lo <= last) {
-    const double hi = lo * 2.0;
-    sel.clear();
-    for (double p : grid)
-      if (p >= lo && p < hi) sel.push_back(p);
-    size_t chosen = 0;
-    while (chosen < nd && sd[chosen] < lo / 4.0) chosen++;
-    if (!sel.empty() && chosen) {
-      if (stats && *nbands >= stats_cap)
-        return fail(FBLS_E_ARG, "band statistics capacity %zu exceeded", stats_cap);
-      b.durations = sd.data();
-      b.rho = sr.data();
-      b.nd = chosen;
-      size_t got = 0;
-      fbls_scan_stats bs;
-      int st = fbls_scan_periods(c, f, sel.data(), sel.size(), &b, rows + *nrows,
-                                 &got, &bs);
-      if (st) return st;
-      *nrows += got;
-      if (stats) stats[*nbands] = bs;
-      (*nbands)++;
-    }
-    lo = hi;
+  std::vector<fbls_band> plan = fbls::plan_bands(grid.data(), count, sd.data(), nd,
+                                                 bank->phase_oversample,
+                                                 bank->prescan_factor);
+  if (stats && plan.size() > stats_cap)
+    return fail(FBLS_E_ARG, "band statistics capacity %zu < %zu bands", stats_cap,
+                plan.size());
+  int st = check_bank(&b);
+  if (st) return st;
+  std::vector<BandRun> runs(plan.size());
+  for (size_t i = 0; i < plan.size(); i++) {
+    runs[i].periods = grid.data() + plan[i].first;
+    runs[i].np = plan[i].count;
+    runs[i].nb = plan[i].nb;
+    runs[i].nb_coarse = plan[i].nb_coarse;
+    runs[i].nd = (uint32_t)plan[i].nd;
+    runs[i].dur = sd.data();
+    runs[i].rho = sr.data();
   }
+  b.durations = sd.data();
+  b.rho = sr.data();
+  b.nd = nd;
+  st = scan_bands(c, f, runs, &b, rows, nrows, nullptr);
+  if (st) return st;
+  for (size_t i = 0; i < runs.size(); i++)
+    if (stats) stats[i] = runs[i].stats;
+  *nbands = runs.size();
   return FBLS_OK;
 }
 

@@ -48,6 +48,43 @@ size_t durations_days(const double* hours, size_t n, double* out) {
   return d.size();
 }
 
+std::vector<fbls_band> plan_bands(const double* grid, size_t count,
+                                  const double* sd, size_t nd,
+                                  uint32_t phase_oversample,
+                                  uint32_t prescan_factor) {
+  std::vector<fbls_band> out;
+  if (count == 0 || nd == 0) return out;
+  double lo = grid[0];
+  const double last = grid[count - 1];
+  size_t i = 0;
+  while (lo <= last) {
+    const double hi = lo * 2.0;
+    // the grid ascends, so the band's periods [lo, hi) are contiguous
+    while (i < count && grid[i] < lo) i++;
+    size_t j = i;
+    while (j < count && grid[j] < hi) j++;
+    size_t chosen = 0;
+    while (chosen < nd && sd[chosen] < lo / 4.0) chosen++;
+    if (j > i && chosen) {
+      fbls_band b;
+      b.lo = lo;
+      b.first = i;
+      b.count = j - i;
+      b.nd = chosen;
+      // scan.rs:377-379; the durations ascend so the minimum is sd[0]
+      const double nbf = std::ceil((double)phase_oversample * grid[j - 1] / sd[0]);
+      b.nb = nbf < 8.0 ? 8u : (nbf > 4e9 ? 0xffffffffu : (uint32_t)nbf);
+      b.nb_coarse = (prescan_factor > 1 && b.nb / prescan_factor >= 8)
+                        ? b.nb / prescan_factor
+                        : 0u;
+      out.push_back(b);
+    }
+    i = j;
+    lo = hi;
+  }
+  return out;
+}
+
 namespace {
 
 // stats.rs:4-11

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/fbls.h"
 
@@ -21,6 +22,12 @@ void period_grid(double minimum, double maximum, double* out, size_t count);
 
 // config.rs:177-182
 size_t durations_days(const double* hours, size_t n, double* out);
+
+// scan.rs:440-471 (octave bands) and scan.rs:377-381 (nb per band)
+std::vector<fbls_band> plan_bands(const double* grid, size_t count,
+                                  const double* sorted_durations, size_t nd,
+                                  uint32_t phase_oversample,
+                                  uint32_t prescan_factor);
 
 // curve.rs:541-569 for a prepared curve (flux relative to 0, errors given).
 int family_state(const double* time, const double* flux, const double* sigma,

@@ -1,12 +1,13 @@
 // Fused FBLS period scan for gfx950 (MI355X).
 //
-// One workgroup owns G periods of one band.  It folds every observation of
-// the star into per-period LDS histograms (fixed-point values, so the sums
-// are exact and order-independent), turns them into exclusive prefix sums in
-// place, scans every (phase bin, duration, shape) template with a cheap FP64
-// surrogate of the squared merit, resolves near ties with the exact FP64
-// merit, and writes one record per period.  Nothing but the record touches
-// HBM.
+// A launch carries segments (one family at one phase-bin count over a list
+// of periods: the work of one reference scan_bank call).  Each workgroup owns
+// G periods of one segment.  It folds every observation of the family into
+// per-period LDS histograms (fixed-point values, so the sums are exact and
+// order-independent), turns them into exclusive prefix sums in place, scans
+// every (phase bin, duration, shape) template with an FP64 surrogate of the
+// squared merit, resolves near ties with the exact FP64 merit, and writes one
+// record per period.  Nothing but the record touches HBM.
 //
 // Reference semantics restated here (isakruas/fbls):
 //   phase cell           src/search/kernels.rs:314-327
@@ -21,6 +22,10 @@
 //   winner / tie rule    larger merit, then smaller key, then lower lane
 //                        (kernels.rs:154-198, 1134-1143)
 //   winner recompute     src/search/kernels.rs:800-843
+//
+// LDS per period: vw[nb+1] {v, w} double pairs (16 B) then n[nb+1] u32;
+// slot b+1 accumulates bin b, so after the in-place scan slot x holds the
+// exclusive prefix E[x] = sum of bins < x.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -36,15 +41,22 @@ constexpr uint32_t kObsPerWave = 64 * kObsPerLane;
 constexpr double kInf = __builtin_inf();
 constexpr double kNear = 1e-10;       // near-tie window of the surrogate
 
-__device__ __forceinline__ double cum_d(const double* e, uint32_t x,
-                                        uint32_t nb, double tot) {
-  // doubled prefix: cum[x] for x in [0, 2nb]
-  return x <= nb ? e[x] : e[x - nb] + tot;
+struct Cum {
+  const double2* vw;  // exclusive prefix of {v, w}, dequantised
+  const uint32_t* n;  // exclusive prefix of counts
+  double tv, tw;      // totals E[nb]
+  uint32_t tn, nb;
+};
+
+// doubled prefix (kernels.rs:7-10): cum[x] for x in [0, 2nb]
+__device__ __forceinline__ double2 cum_vw(const Cum& c, uint32_t x) {
+  if (x <= c.nb) return c.vw[x];
+  double2 e = c.vw[x - c.nb];
+  return make_double2(e.x + c.tv, e.y + c.tw);
 }
 
-__device__ __forceinline__ uint32_t cum_n(const uint32_t* e, uint32_t x,
-                                          uint32_t nb, uint32_t tot) {
-  return x <= nb ? e[x] : e[x - nb] + tot;
+__device__ __forceinline__ uint32_t cum_n(const Cum& c, uint32_t x) {
+  return x <= c.nb ? c.n[x] : c.n[x - c.nb] + c.tn;
 }
 
 // Reciprocal with two Newton steps: relative error far below kNear.
@@ -82,7 +94,7 @@ __device__ __forceinline__ double surrogate(double a, double hp, double norm,
 }
 
 struct LaneState {
-  double s, second;       // best surrogate and best other candidate
+  double s, second;         // best surrogate and best other candidate
   double a, hp, norm, rho;  // inputs of the best (identical inputs tie exactly)
   uint64_t key;
   uint32_t di;
@@ -125,10 +137,8 @@ __device__ __forceinline__ void state_update(LaneState& st, double s,
 }
 
 // Staircase sums (kernels.rs:84-113) with precomputed layer widths/offsets.
-__device__ __forceinline__ void shape_sums(const double* cv, const double* cw,
-                                           uint32_t nb, double tv, double tw,
-                                           uint32_t jj, double a0, double w0,
-                                           uint32_t layers,
+__device__ __forceinline__ void shape_sums(const Cum& c, uint32_t jj, double a0,
+                                           double w0, uint32_t layers,
                                            const uint32_t* tab, double* pa,
                                            double* php, double* pnorm) {
   double l = (double)layers;
@@ -139,8 +149,10 @@ __device__ __forceinline__ void shape_sums(const double* cv, const double* cw,
     uint32_t e = tab[layer - 1];
     uint32_t width = e & 0xffffu;
     uint32_t start = jj + (e >> 16);
-    double dw = cum_d(cw, start + width, nb, tw) - cum_d(cw, start, nb, tw);
-    double dv = cum_d(cv, start + width, nb, tv) - cum_d(cv, start, nb, tv);
+    double2 hi = cum_vw(c, start + width);
+    double2 lo = cum_vw(c, start);
+    double dw = hi.y - lo.y;
+    double dv = hi.x - lo.x;
     a += dv / l;
     hp += dw / l;
     norm += dw * ((2.0 * (double)layer + 1.0) / (l * l));
@@ -151,12 +163,11 @@ __device__ __forceinline__ void shape_sums(const double* cv, const double* cw,
 }
 
 // Staircase sums with widths computed from the factors (record recompute).
-__device__ void shape_sums_direct(const double* cv, const double* cw,
-                                  uint32_t nb, double tv, double tw,
-                                  uint32_t j, uint32_t k, double a0, double w0,
-                                  double ingress, const double* factors,
-                                  uint32_t si, uint32_t layers, double* pa,
-                                  double* php, double* pnorm) {
+__device__ void shape_sums_direct(const Cum& c, uint32_t j, uint32_t k,
+                                  double a0, double w0, double ingress,
+                                  const double* factors, uint32_t si,
+                                  uint32_t layers, double* pa, double* php,
+                                  double* pnorm) {
   if (ingress == 0.0) {
     *pa = a0;
     *php = w0;
@@ -172,9 +183,10 @@ __device__ void shape_sums_direct(const double* cv, const double* cw,
     double wf = __builtin_floor((double)k * factor + 0.5);
     uint32_t width = wf < 1.0 ? 1u : (uint32_t)wf;
     uint32_t start = j + (k - width) / 2;
-    double dw = cum_d(cw, start + width, nb, tw) - cum_d(cw, start, nb, tw);
-    double dv = cum_d(cv, start + width, nb, tv) - cum_d(cv, start, nb, tv);
-    a += dv / l;
+    double2 hi = cum_vw(c, start + width);
+    double2 lo = cum_vw(c, start);
+    double dw = hi.y - lo.y;
+    a += (hi.x - lo.x) / l;
     hp += dw / l;
     norm += dw * ((2.0 * (double)layer + 1.0) / (l * l));
   }
@@ -199,87 +211,199 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) {
 }
 
 template <typename T>
-__device__ __forceinline__ T wave_excl_scan(T x, uint32_t lane, T* total) {
+__device__ __forceinline__ T wave_excl_scan(T x, uint32_t lane) {
   T incl = x;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     T y = __shfl_up(incl, off, 64);
     if (lane >= (uint32_t)off) incl += y;
   }
-  *total = __shfl(incl, 63, 64);
   return incl - x;
 }
 
-// In-place inclusive scan of slots [1, nb] by one wave, then scale.
-template <typename T>
-__device__ void wave_prefix(T* arr, uint32_t nb, uint32_t lane, T scale,
-                            bool do_scale) {
+// In-place inclusive scan of slots [1, nb] of {v, w} pairs by one wave
+// (exact: integer-valued doubles below 2^53), then the power-of-two scales.
+__device__ void wave_prefix_vw(double2* arr, uint32_t nb, uint32_t lane,
+                               double sv, double sw) {
+  uint32_t len = (nb + 63) / 64;
+  uint32_t lo = 1 + lane * len;
+  uint32_t hi = lo + len;
+  if (hi > nb + 1) hi = nb + 1;
+  double av = 0.0, aw = 0.0;
+  for (uint32_t s = lo; s < hi; s++) {
+    double2 x = arr[s];
+    av += x.x;
+    aw += x.y;
+    arr[s] = make_double2(av, aw);
+  }
+  const bool has = lo < hi;
+  double ev = wave_excl_scan(has ? av : 0.0, lane);
+  double ew = wave_excl_scan(has ? aw : 0.0, lane);
+  for (uint32_t s = lo; s < hi; s++) {
+    double2 x = arr[s];
+    arr[s] = make_double2((x.x + ev) * sv, (x.y + ew) * sw);
+  }
+}
+
+__device__ void wave_prefix_n(uint32_t* arr, uint32_t nb, uint32_t lane) {
   uint32_t len = (nb + 63) / 64;
   len |= 1u;  // odd stride between lanes spreads LDS banks
   uint32_t lo = 1 + lane * len;
   uint32_t hi = lo + len;
   if (hi > nb + 1) hi = nb + 1;
-  T acc = 0;
+  uint32_t acc = 0;
   for (uint32_t s = lo; s < hi; s++) {
     acc += arr[s];
     arr[s] = acc;
   }
-  T total;
-  T excl = wave_excl_scan(lo < hi ? acc : (T)0, lane, &total);
-  for (uint32_t s = lo; s < hi; s++) {
-    T x = arr[s] + excl;
-    arr[s] = do_scale ? x * scale : x;
+  uint32_t ex = wave_excl_scan(lo < hi ? acc : 0u, lane);
+  for (uint32_t s = lo; s < hi; s++) arr[s] += ex;
+}
+
+// floor(fl(t * scale)) mod nb (kernels.rs:314-327).  Narrow: t*scale < 2^24,
+// the quotient from a float reciprocal biased low by ~2^-22 is floor(u/nb)
+// or one less, so one conditional subtraction (as unsigned min) finishes.
+template <bool WIDE>
+__device__ __forceinline__ uint32_t phase_bin(double t, double scale,
+                                              uint32_t nb, float inv_lo) {
+  const uint32_t u = (uint32_t)(t * scale);
+  if (WIDE) return u % nb;
+  const uint32_t q = (uint32_t)((float)u * inv_lo);
+  const uint32_t r = u - __umul24(q, nb);
+  return min(r, r - nb);
+}
+
+__device__ __forceinline__ void lds_add(double* vw, uint32_t* n, uint32_t b,
+                                        double v, double w, uint32_t c) {
+  __hip_atomic_fetch_add(vw + 2 * (b + 1), v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_add(vw + 2 * (b + 1) + 1, w, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_add(n + b + 1, c, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Fold one lane's 8 consecutive observations into one period's histogram.
+// RUN: consecutive observations falling in the same bin are summed in
+// registers and flushed once (exact: integer-valued doubles).
+template <bool WIDE, bool RUN, bool FULL>
+__device__ __forceinline__ void fold_lane(double* vw, uint32_t* n,
+                                          const double* tt, const double* vv,
+                                          const double* ww, double scale,
+                                          uint32_t nb, float inv_lo,
+                                          uint32_t base, uint32_t nobs) {
+  if (RUN && FULL) {
+    uint32_t cb = phase_bin<WIDE>(tt[0], scale, nb, inv_lo);
+    double av = vv[0], aw = ww[0];
+    uint32_t an = 1;
+#pragma unroll
+    for (uint32_t r = 1; r < kObsPerLane; r++) {
+      const uint32_t b = phase_bin<WIDE>(tt[r], scale, nb, inv_lo);
+      const bool flush = b != cb;
+      if (flush) lds_add(vw, n, cb, av, aw, an);
+      const double sv = av + vv[r], sw = aw + ww[r];
+      av = flush ? vv[r] : sv;
+      aw = flush ? ww[r] : sw;
+      an = flush ? 1u : an + 1u;
+      cb = b;
+    }
+    lds_add(vw, n, cb, av, aw, an);
+  } else {
+#pragma unroll
+    for (uint32_t r = 0; r < kObsPerLane; r++) {
+      const uint32_t b = phase_bin<WIDE>(tt[r], scale, nb, inv_lo);
+      if (FULL || base + r < nobs) lds_add(vw, n, b, vv[r], ww[r], 1u);
+    }
   }
 }
 
-struct PeriodView {
-  const double* cv;
-  const double* cw;
-  const uint32_t* cn;
-  double tv, tw;
-  uint32_t tn;
-};
+template <bool WIDE, bool RUN>
+__device__ void fold_all(const Segment& S, unsigned char* hist,
+                         uint32_t p0, uint32_t gcount, uint32_t wave,
+                         uint32_t lane, uint32_t nwaves) {
+  const uint32_t nb = S.nb;
+  const uint32_t nb1 = nb + 1;
+  const float inv_lo = (float)((1.0 / (double)nb) * (1.0 - 0x1p-22));
+  const uint32_t nblk = S.n_pad / kObsPerWave;
+  for (uint32_t blk = wave; blk < nblk; blk += nwaves) {
+    const uint32_t base = blk * kObsPerWave + lane * kObsPerLane;
+    double tt[kObsPerLane], vv[kObsPerLane], ww[kObsPerLane];
+    const double2* tp = (const double2*)(S.t + base);
+    const int4* qp = (const int4*)(S.vw + 2 * (size_t)base);
+#pragma unroll
+    for (uint32_t r = 0; r < kObsPerLane / 2; r++) {
+      double2 d = tp[r];
+      int4 q = qp[r];
+      tt[2 * r] = d.x;
+      tt[2 * r + 1] = d.y;
+      vv[2 * r] = (double)q.x;
+      ww[2 * r] = (double)q.y;
+      vv[2 * r + 1] = (double)q.z;
+      ww[2 * r + 1] = (double)q.w;
+    }
+    const bool full = blk * kObsPerWave + kObsPerWave <= S.n;
+    for (uint32_t g = 0; g < gcount; g++) {
+      const double scale = S.scales[p0 + g];
+      double* vw = (double*)(hist + (size_t)g * S.pstride);
+      uint32_t* n = (uint32_t*)(vw + 2 * nb1);
+      if (full)
+        fold_lane<WIDE, RUN, true>(vw, n, tt, vv, ww, scale, nb, inv_lo, base, S.n);
+      else
+        fold_lane<WIDE, RUN, false>(vw, n, tt, vv, ww, scale, nb, inv_lo, base, S.n);
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t find_segment(const LaunchArgs& A,
+                                                 uint32_t group) {
+  uint32_t lo = 0, hi = A.nseg;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (A.segs[mid].group0 <= group) lo = mid; else hi = mid;
+  }
+  return lo;
+}
 
 template <uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
+__global__ __launch_bounds__(BLOCK) void fused_scan(LaunchArgs A) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   constexpr uint32_t kWaves = BLOCK / 64;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
   const uint32_t wave = tid >> 6;
-  const uint32_t nb = A.nb;
+  const Segment S = A.segs[find_segment(A, blockIdx.x)];
+  const uint32_t nb = S.nb;
   const uint32_t nb1 = nb + 1;
-  const uint32_t p0 = blockIdx.x * A.G;
-  const uint32_t gcount = min(A.G, A.np - p0);
-  const uint32_t nd = A.nd, ns = A.ns, layers = A.layers;
+  const uint32_t p0 = (blockIdx.x - S.group0) * S.G;
+  const uint32_t gcount = min(S.G, S.np - p0);
+  const uint32_t nd = S.nd, ns = A.ns, layers = A.layers;
   const uint32_t tiles = (nb + kTile - 1) / kTile;
   const uint32_t nsl = (ns - 1) * (layers - 1);  // table entries per duration
 
   // ---- LDS carve-up ---------------------------------------------------
-  const uint32_t pstride = A.period_stride;  // bytes per period histogram
   unsigned char* p = lds;
   unsigned char* hist = p;
-  p += (size_t)A.G * pstride;
+  p += (size_t)S.G * S.pstride;
   uint32_t* ktab = (uint32_t*)p;  // [G][nd]: k | kok << 31
-  p += A.G * nd * 4;
+  p += S.G * nd * 4;
   uint32_t* stab = (uint32_t*)p;  // [G][nd][nsl]: width | offset << 16
-  p += A.G * nd * nsl * 4;
+  p += S.G * nd * nsl * 4;
   p = (unsigned char*)(((uintptr_t)p + 7) & ~(uintptr_t)7);
   double* tval = (double*)p;  // [G][tiles]
-  p += A.G * tiles * 8;
+  p += S.G * tiles * 8;
   uint64_t* tkey = (uint64_t*)p;
-  p += A.G * tiles * 8;
+  p += S.G * tiles * 8;
   uint32_t* tdi = (uint32_t*)p;
 
   // ---- phase 0: clear histograms, duration tables ----------------------
   {
     uint4* h4 = (uint4*)hist;
-    const uint32_t words = (uint32_t)((size_t)gcount * pstride / 16);
+    const uint32_t words = (uint32_t)((size_t)gcount * S.pstride / 16);
     for (uint32_t i = tid; i < words; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
     for (uint32_t i = tid; i < gcount * nd; i += BLOCK) {
       uint32_t g = i / nd, di = i % nd;
-      double scale = A.scales[p0 + g];
-      double rk = __builtin_floor(A.dur[di] * scale + 0.5);
+      double scale = S.scales[p0 + g];
+      double rk = __builtin_floor(S.dur[di] * scale + 0.5);
       uint32_t raw_k = rk < 1.0 ? 1u : (rk >= 2147483648.0 ? 0x7fffffffu : (uint32_t)rk);
       uint32_t k = raw_k < nb / 2 ? raw_k : nb / 2;
       uint32_t kok = raw_k < nb / 2 ? 1u : 0u;
@@ -298,85 +422,43 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
   __syncthreads();
 
   // ---- phase 1: fold every observation into the G histograms ----------
-  {
-    const float inv_nb = 1.0f / (float)nb;
-    const uint32_t nblk = A.n_pad / kObsPerWave;
-    for (uint32_t blk = wave; blk < nblk; blk += kWaves) {
-      const uint32_t base = blk * kObsPerWave + lane * kObsPerLane;
-      double tt[kObsPerLane], vv[kObsPerLane], ww[kObsPerLane];
-      const double2* tp = (const double2*)(A.t + base);
-      const int4* qp = (const int4*)(A.vw + 2 * (size_t)base);
-#pragma unroll
-      for (uint32_t r = 0; r < kObsPerLane / 2; r++) {
-        double2 d = tp[r];
-        int4 q = qp[r];
-        tt[2 * r] = d.x;
-        tt[2 * r + 1] = d.y;
-        vv[2 * r] = (double)q.x;
-        ww[2 * r] = (double)q.y;
-        vv[2 * r + 1] = (double)q.z;
-        ww[2 * r + 1] = (double)q.w;
-      }
-      const bool full = blk * kObsPerWave + kObsPerWave <= A.n;
-      for (uint32_t g = 0; g < gcount; g++) {
-        const double scale = A.scales[p0 + g];
-        double* cv = (double*)(hist + (size_t)g * pstride);
-        double* cw = cv + nb1;
-        uint32_t* cn = (uint32_t*)(cw + nb1);
-#pragma unroll
-        for (uint32_t r = 0; r < kObsPerLane; r++) {
-          const uint32_t u = (uint32_t)(tt[r] * scale);
-          uint32_t b;
-          if (A.wide_cells) {
-            b = u % nb;
-          } else {
-            uint32_t q = (uint32_t)((float)u * inv_nb);
-            int32_t rr = (int32_t)(u - q * nb);
-            rr += rr < 0 ? (int32_t)nb : 0;
-            rr -= rr >= (int32_t)nb ? (int32_t)nb : 0;
-            b = (uint32_t)rr;
-          }
-          if (full || base + r < A.n) {
-            __hip_atomic_fetch_add(cv + b + 1, vv[r], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(cw + b + 1, ww[r], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(cn + b + 1, 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-          }
-        }
-      }
-    }
+  if (S.wide_cells) {
+    if (A.run_fold) fold_all<true, true>(S, hist, p0, gcount, wave, lane, kWaves);
+    else fold_all<true, false>(S, hist, p0, gcount, wave, lane, kWaves);
+  } else {
+    if (A.run_fold) fold_all<false, true>(S, hist, p0, gcount, wave, lane, kWaves);
+    else fold_all<false, false>(S, hist, p0, gcount, wave, lane, kWaves);
   }
   __syncthreads();
 
   // ---- phase 2: exclusive prefix sums in place (exact) -----------------
-  for (uint32_t item = wave; item < 3 * gcount; item += kWaves) {
-    uint32_t g = item / 3, f = item % 3;
-    double* cv = (double*)(hist + (size_t)g * pstride);
-    double* cw = cv + nb1;
-    uint32_t* cn = (uint32_t*)(cw + nb1);
-    if (f == 0)
-      wave_prefix<double>(cv, nb, lane, A.sv, true);
-    else if (f == 1)
-      wave_prefix<double>(cw, nb, lane, A.sw, true);
+  for (uint32_t item = wave; item < 2 * gcount; item += kWaves) {
+    uint32_t g = item >> 1;
+    double2* vw = (double2*)(hist + (size_t)g * S.pstride);
+    if (item & 1)
+      wave_prefix_n((uint32_t*)(vw + nb1), nb, lane);
     else
-      wave_prefix<uint32_t>(cn, nb, lane, 1u, false);
+      wave_prefix_vw(vw, nb, lane, S.sv, S.sw);
   }
   __syncthreads();
 
   // ---- phase 3: template scan, one 128-bin tile per wave at a time -----
-  const double c = A.C, inv_c = A.invC;
-  const uint32_t nobs = A.n;
+  const double c = S.C, inv_c = S.invC;
+  const uint32_t nobs = S.n;
+  const double gate = A.gate;
+  const double g2 = gate * gate;
   for (uint32_t item = wave; item < gcount * tiles; item += kWaves) {
     const uint32_t g = item / tiles, tile = item % tiles;
-    PeriodView pv;
-    pv.cv = (const double*)(hist + (size_t)g * pstride);
-    pv.cw = pv.cv + nb1;
-    pv.cn = (const uint32_t*)(pv.cw + nb1);
-    pv.tv = pv.cv[nb];
-    pv.tw = pv.cw[nb];
-    pv.tn = pv.cn[nb];
+    Cum cm;
+    cm.vw = (const double2*)(hist + (size_t)g * S.pstride);
+    cm.n = (const uint32_t*)(cm.vw + nb1);
+    cm.nb = nb;
+    {
+      double2 t = cm.vw[nb];
+      cm.tv = t.x;
+      cm.tw = t.y;
+      cm.tn = cm.n[nb];
+    }
     const uint32_t* kt = ktab + g * nd;
     const uint32_t* st_tab = stab + (size_t)g * nd * nsl;
 
@@ -388,9 +470,10 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
     for (int x = 0; x < 2; x++) {
       jx[x] = tile * kTile + lane + 64 * x;
       jjx[x] = jx[x] < nb ? jx[x] : nb - 1;
-      ev[x] = pv.cv[jjx[x]];
-      ew[x] = pv.cw[jjx[x]];
-      en[x] = pv.cn[jjx[x]];
+      double2 e = cm.vw[jjx[x]];
+      ev[x] = e.x;
+      ew[x] = e.y;
+      en[x] = cm.n[jjx[x]];
       state_init(st[x]);
     }
     bool bad = false;
@@ -399,7 +482,7 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
       const uint32_t kte = kt[di];
       const uint32_t k = kte & 0x7fffffffu;
       const bool kok = (kte >> 31) != 0;
-      const double rho = A.rho[di];
+      const double rho = S.rho[di];
       const double rho2 = rho * rho;
       double bv[2], bw[2];
       bool busable[2];
@@ -407,9 +490,10 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
 #pragma unroll
       for (int x = 0; x < 2; x++) {
         const uint32_t e = jjx[x] + k;
-        const uint32_t cnt = cum_n(pv.cn, e, nb, pv.tn) - en[x];
-        const double w = cum_d(pv.cw, e, nb, pv.tw) - ew[x];
-        const double a = cum_d(pv.cv, e, nb, pv.tv) - ev[x];
+        const uint32_t cnt = cum_n(cm, e) - en[x];
+        const double2 hi = cum_vw(cm, e);
+        const double w = hi.y - ew[x];
+        const double a = hi.x - ev[x];
         const bool usable = jx[x] < nb && kok && cnt >= 3 && cnt + 3 <= nobs;
         const bool valid = usable && a > 0.0;
         bv[x] = a;
@@ -420,11 +504,10 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
           const double s = surrogate(a, w, w, rho, rho2, c, inv_c, &bad);
           state_update(st[x], s, key, di, a, w, w, rho);
           if (s > -kInf) {
-            const double g2 = A.gate * A.gate;
-            if (A.gate <= 0.0 || s >= g2 * (1.0 + kNear)) {
+            if (gate <= 0.0 || s >= g2 * (1.0 + kNear)) {
               pass = true;
             } else if (s >= g2 * (1.0 - kNear)) {
-              pass = pass || exact_merit(a, w, w, rho, c, &bad) >= A.gate;
+              pass = pass || exact_merit(a, w, w, rho, c, &bad) >= gate;
             }
           }
         }
@@ -433,12 +516,12 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
       gmask |= 1u << di;
       for (uint32_t si = 1; si < ns; si++) {
         const uint32_t* tab = st_tab + (di * (ns - 1) + (si - 1)) * (layers - 1);
+        const bool staircase = A.ingress[si] != 0.0;
 #pragma unroll
         for (int x = 0; x < 2; x++) {
           double a = bv[x], hp = bw[x], norm = bw[x];
-          if (A.ingress[si] != 0.0)
-            shape_sums(pv.cv, pv.cw, nb, pv.tv, pv.tw, jjx[x], bv[x], bw[x],
-                       layers, tab, &a, &hp, &norm);
+          if (staircase)
+            shape_sums(cm, jjx[x], bv[x], bw[x], layers, tab, &a, &hp, &norm);
           if (busable[x] && a > 0.0) {
             const uint64_t key = ((uint64_t)si * nb1 + k) * nb1 + jjx[x];
             const double s = surrogate(a, hp, norm, rho, rho2, c, inv_c, &bad);
@@ -497,11 +580,12 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
             const uint32_t kte = kt[di];
             const uint32_t k = kte & 0x7fffffffu;
             const bool kok = (kte >> 31) != 0;
-            const double rho = A.rho[di];
+            const double rho = S.rho[di];
             const uint32_t e = jjx[x] + k;
-            const uint32_t cnt = cum_n(pv.cn, e, nb, pv.tn) - en[x];
-            const double w = cum_d(pv.cw, e, nb, pv.tw) - ew[x];
-            const double a = cum_d(pv.cv, e, nb, pv.tv) - ev[x];
+            const uint32_t cnt = cum_n(cm, e) - en[x];
+            const double2 hi = cum_vw(cm, e);
+            const double w = hi.y - ew[x];
+            const double a = hi.x - ev[x];
             const bool usable = jx[x] < nb && kok && cnt >= 3 && cnt + 3 <= nobs;
             if (usable && a > 0.0) {
               const uint64_t key = (uint64_t)k * nb1 + jjx[x];
@@ -517,8 +601,7 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
               const uint32_t* tab = st_tab + (di * (ns - 1) + (si - 1)) * (layers - 1);
               double sa = a, shp = w, snorm = w;
               if (A.ingress[si] != 0.0)
-                shape_sums(pv.cv, pv.cw, nb, pv.tv, pv.tw, jjx[x], a, w,
-                           layers, tab, &sa, &shp, &snorm);
+                shape_sums(cm, jjx[x], a, w, layers, tab, &sa, &shp, &snorm);
               if (usable && sa > 0.0) {
                 const uint64_t key = ((uint64_t)si * nb1 + k) * nb1 + jjx[x];
                 const double mm = exact_merit(sa, shp, snorm, rho, c, &bad);
@@ -550,10 +633,10 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
         tile_value = bvv;
         tile_key = bkk;
         tile_di = bdd;
-        if (lane == 0) atomicAdd(A.counters + 1, 1u);
+        if (lane == 0) atomicAdd(A.counters + 2 * S.index + 1, 1u);
       }
     }
-    if (__any(bad) && lane == 0) atomicOr(A.counters, 1u);
+    if (__any(bad) && lane == 0) atomicOr(A.counters + 2 * S.index, 1u);
     if (lane == 0) {
       tval[g * tiles + tile] = tile_value;
       tkey[g * tiles + tile] = tile_key;
@@ -580,29 +663,35 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
       key = nb1;  // no usable template: canonical (phase 0, width 1, box)
       bdi = 0;
     }
-    const double* cv = (const double*)(hist + (size_t)g * pstride);
-    const double* cw = cv + nb1;
-    const double tv = cv[nb], tw = cw[nb];
+    Cum cm;
+    cm.vw = (const double2*)(hist + (size_t)g * S.pstride);
+    cm.n = (const uint32_t*)(cm.vw + nb1);
+    cm.nb = nb;
+    cm.tv = cm.vw[nb].x;
+    cm.tw = cm.vw[nb].y;
+    cm.tn = cm.n[nb];
     const uint32_t phase = (uint32_t)(key % nb1);
     const uint32_t width = (uint32_t)((key / nb1) % nb1);
     const uint32_t si = (uint32_t)(key / ((uint64_t)nb1 * nb1));
     const uint32_t sic = si < ns - 1 ? si : ns - 1;
     const double ing = A.ingress[sic];
-    const double r = A.rho[bdi];
-    const double w0 = cum_d(cw, phase + width, nb, tw) - cum_d(cw, phase, nb, tw);
-    const double a00 = cum_d(cv, phase + width, nb, tv) - cum_d(cv, phase, nb, tv);
+    const double r = S.rho[bdi];
+    const double2 hi = cum_vw(cm, phase + width);
+    const double2 lo = cum_vw(cm, phase);
+    const double w0 = hi.y - lo.y;
+    const double a00 = hi.x - lo.x;
     double a0, hp, norm;
-    shape_sums_direct(cv, cw, nb, tv, tw, phase, width, a00, w0, ing,
-                      A.factors, sic, layers, &a0, &hp, &norm);
+    shape_sums_direct(cm, phase, width, a00, w0, ing, A.factors, sic, layers,
+                      &a0, &hp, &norm);
     const double a = a0 / r;
     const double d = hp * hp / c;
     const double q = __builtin_fmax(norm - d, 1e-300);
     const double value = best > -kInf ? a / __builtin_sqrt(q) : -kInf;
     const double depth = a / q * r;
     const uint32_t f = p0 + g;
-    double* rec = A.records + (size_t)f * 9;
-    rec[0] = A.periods[f];
-    rec[1] = A.scales[f];
+    double* rec = S.records + (size_t)f * 9;
+    rec[0] = S.periods[f];
+    rec[1] = S.scales[f];
     rec[2] = (double)phase;
     rec[3] = (double)width;
     rec[4] = a;
@@ -610,10 +699,10 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(ScanArgs A) {
     rec[6] = d;
     rec[7] = ing;
     rec[8] = r;
-    A.score[f] = value;
-    A.depth[f] = depth;
-    if (A.winner) {
-      uint32_t* wn = A.winner + (size_t)f * 4;
+    S.score[f] = value;
+    S.depth[f] = depth;
+    if (S.winner) {
+      uint32_t* wn = S.winner + (size_t)f * 4;
       wn[0] = phase;
       wn[1] = width;
       wn[2] = si;
@@ -648,13 +737,13 @@ hipError_t scan_configure(size_t max_lds) {
                              (int)max_lds);
 }
 
-hipError_t scan_launch(const ScanArgs& args, uint32_t block, size_t lds,
-                       hipStream_t stream) {
-  const uint32_t grid = (args.np + args.G - 1) / args.G;
+hipError_t scan_launch(const LaunchArgs& args, uint32_t groups, uint32_t block,
+                       size_t lds, hipStream_t stream) {
+  if (groups == 0) return hipSuccess;
   if (block == 512)
-    hipLaunchKernelGGL(fused_scan<512>, dim3(grid), dim3(512), lds, stream, args);
+    hipLaunchKernelGGL(fused_scan<512>, dim3(groups), dim3(512), lds, stream, args);
   else
-    hipLaunchKernelGGL(fused_scan<256>, dim3(grid), dim3(256), lds, stream, args);
+    hipLaunchKernelGGL(fused_scan<256>, dim3(groups), dim3(256), lds, stream, args);
   return hipGetLastError();
 }
 

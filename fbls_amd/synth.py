@@ -1,0 +1,49 @@
+"""Synthetic workloads of the benchmark and tests (seeded, deterministic).
+
+Each function returns (time, flux, flux_err) from the C generator
+(fbls_synth_star: xorshift64 + Box-Muller as in survey/numeric.rs:345-364)
+with normalised flux (1 + noise - transits) and constant flux_err.
+"""
+import numpy as np
+
+from . import synth_star
+
+CADENCE = 1.0 / 720.0  # 2 min
+T0 = 2458000.0
+
+
+def single_sector(seed=1):
+    """One 27.4-d sector, 1-d mid-sector gap, white sigma 1000 ppm, box
+    P = 3.7 d, depth 2000 ppm, duration 2.5 h, random epoch."""
+    return synth_star(seed=seed, sectors=1, sector_days=27.4, gap_days=1.0,
+                      cadence_days=CADENCE, t0=T0, sigma=1e-3, period=3.7,
+                      depth=2e-3, duration=2.5 / 24.0)
+
+
+def four_sector_red(seed=2):
+    """Four contiguous sectors (~79k points, 1-d gap per sector), AR(1)
+    phi = 0.9 red noise holding half the variance, total sigma 600 ppm, box
+    P = 11.3 d, depth 500 ppm, duration 4 h, random epoch."""
+    return synth_star(seed=seed, sectors=4, sector_days=27.4, gap_days=1.0,
+                      cadence_days=CADENCE, t0=T0, sigma=600e-6,
+                      red_fraction=0.5, ar_phi=0.9, period=11.3, depth=500e-6,
+                      duration=4.0 / 24.0)
+
+
+def batch_star(i, seed0=3000):
+    """Star i of the 512-star batch: 1-13 sectors, sigma log-uniform
+    200-3000 ppm, half with a box (P log-uniform 0.5-20 d, depth log-uniform
+    200-10000 ppm, duration 1 h + 7 h (P^(1/3) - 0.5^(1/3)) /
+    (20^(1/3) - 0.5^(1/3)))."""
+    rng = np.random.default_rng(seed0 + i)
+    sectors = int(rng.integers(1, 14))
+    sigma = float(np.exp(rng.uniform(np.log(200e-6), np.log(3000e-6))))
+    kw = dict(seed=seed0 + i, sectors=sectors, sector_days=27.4, gap_days=1.0,
+              cadence_days=CADENCE, t0=T0, sigma=sigma)
+    if rng.uniform() < 0.5:
+        p = float(np.exp(rng.uniform(np.log(0.5), np.log(20.0))))
+        depth = float(np.exp(rng.uniform(np.log(200e-6), np.log(1e-2))))
+        dur = (1.0 + 7.0 * (p ** (1 / 3) - 0.5 ** (1 / 3))
+               / (20 ** (1 / 3) - 0.5 ** (1 / 3))) / 24.0
+        kw.update(period=p, depth=depth, duration=dur)
+    return synth_star(**kw), kw

@@ -88,9 +88,12 @@ int fbls_ctx_create(int device, fbls_ctx** out);
 void fbls_ctx_destroy(fbls_ctx* ctx);
 /* `Gpu::memory` (scan.rs:80-90) */
 int fbls_mem_info(fbls_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
-/* Launch shape override for tuning: LDS bytes per workgroup (0 = auto) and
- * workgroup size (256 or 512, 0 = auto).  Results never depend on it. */
-int fbls_ctx_set_launch(fbls_ctx* ctx, uint32_t lds_bytes, uint32_t block);
+/* Launch shape override for tuning: LDS bytes per workgroup (0 = auto),
+ * workgroup size (256 or 512, 0 = auto) and fold mode (0 = auto, 1 =
+ * run-length pre-aggregation in registers, 2 = one LDS atomic per
+ * observation and period).  Results never depend on them. */
+int fbls_ctx_set_launch(fbls_ctx* ctx, uint32_t lds_bytes, uint32_t block,
+                        uint32_t fold_mode);
 
 /* -- family upload ------------------------------------------------------ */
 
@@ -143,6 +146,25 @@ size_t fbls_period_grid_count(double minimum, double maximum, double baseline,
 int fbls_period_grid(double minimum, double maximum, double baseline,
                      double minimum_duration, double oversample, double* out,
                      size_t count);
+
+/* [host] Octave bands of one family (scan.rs:440-471, 377-381): for each
+ * band, its first grid index and period count, the durations used (the
+ * first nd of the sorted durations, those below lo/4), the phase bins nb of
+ * the full pass and nb_coarse of the prescan (0 when there is none). */
+typedef struct {
+  double lo;
+  size_t first, count, nd;
+  uint32_t nb, nb_coarse;
+} fbls_band;
+int fbls_plan_bands(const double* grid, size_t count,
+                    const double* sorted_durations, size_t nd,
+                    uint32_t phase_oversample, uint32_t prescan_factor,
+                    fbls_band* bands, size_t bands_cap, size_t* nbands);
+
+/* [host] batches of one scan_bank call under the fold budget
+ * (scan.rs:177-183); 0 when one period exceeds the budget. */
+uint64_t fbls_bank_batches(size_t np, uint32_t nb, uint32_t period_batch,
+                           double max_fold_mib);
 
 /* [host] `SearchConfig::durations_days` (config.rs:177-182) */
 size_t fbls_durations_days(const double* hours, size_t n, double* out_days);

@@ -71,6 +71,8 @@ def lib():
         L.orc_rng_new.restype = _u64
         L.orc_rng_new.argtypes = [_u64]
         L.orc_rng_uniform.restype = _d
+        L.orc_rng_uniform.argtypes = [_P(_u64)]
+        L.orc_plan_bands.restype = _sz
         L.orc_rng_normal.restype = _d
         L.orc_rng_normal.argtypes = [_P(_u64), _d, _d]
         L.orc_quantize.restype = ctypes.c_int
@@ -176,11 +178,6 @@ def phase_cell(t, scale, nb):
     return lib().orc_phase_cell(t, scale, nb)
 
 
-def _factors(ingress, layers):
-    ing = np.asarray(ingress, dtype=np.float64)
-    return ing
-
-
 def scan_bank(t, v, w, baseline_norm, periods, durations, rho, ingress,
               layers, gate, nb):
     """kernels.rs fold/prefix/scan/merge at nb bins, f64 ranking everywhere.
@@ -259,6 +256,21 @@ def signal_search(t, v, w, baseline_norm, origin, durations, rho, ingress,
                        templates_scanned=x.templates_scanned)
                   for x in stats[:nbands.value]]
     return out, band_stats, gcount.value, st
+
+
+class Band(ctypes.Structure):
+    _fields_ = [("lo", _d), ("first", _sz), ("count", _sz), ("nd", _sz),
+                ("nb", _u32), ("nb_coarse", _u32)]
+
+
+def plan_bands(grid, sorted_durations, phase_oversample=4, prescan_factor=2):
+    grid, sd = _f64(grid), _f64(sorted_durations)
+    out = (Band * 64)()
+    m = lib().orc_plan_bands(_dp(grid), _sz(len(grid)), _dp(sd), _sz(len(sd)),
+                             _u32(phase_oversample), _u32(prescan_factor), out,
+                             _sz(64))
+    return [dict(lo=b.lo, first=b.first, count=b.count, nd=b.nd, nb=b.nb,
+                 nb_coarse=b.nb_coarse) for b in out[:min(m, 64)]]
 
 
 def bank_batches(np_, nb, period_batch, max_fold_mib):

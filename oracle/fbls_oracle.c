@@ -663,6 +663,50 @@ done:
   return st;
 }
 
+/* scan.rs:440-471 with the reference's filter over the whole grid */
+size_t orc_plan_bands(const double* grid, size_t count, const double* sd,
+                      size_t nd, uint32_t phase_oversample,
+                      uint32_t prescan_factor, orc_band* out, size_t cap) {
+  size_t nbands = 0;
+  if (count == 0 || nd == 0) return 0;
+  double lo = grid[0];
+  double last = grid[count - 1];
+  while (lo <= last) {
+    double hi = lo * 2.0;
+    size_t first = (size_t)-1, sel = 0, lastsel = 0;
+    for (size_t i = 0; i < count; i++) {
+      if (grid[i] >= lo && grid[i] < hi) {
+        if (first == (size_t)-1) first = i;
+        sel++;
+        lastsel = i;
+      }
+    }
+    size_t chosen = 0;
+    for (size_t i = 0; i < nd; i++)
+      if (sd[i] < lo / 4.0) chosen++;
+    if (sel && chosen) {
+      double dmin = INFINITY;
+      for (size_t i = 0; i < nd; i++)
+        if (sd[i] < lo / 4.0) dmin = fmin(dmin, sd[i]);
+      double nbf = ceil((double)phase_oversample * grid[lastsel] / dmin);
+      uint32_t nb = nbf < 8.0 ? 8u : (uint32_t)nbf;
+      if (nbands < cap) {
+        orc_band* b = &out[nbands];
+        b->lo = lo;
+        b->first = first;
+        b->count = sel;
+        b->nd = chosen;
+        b->nb = nb;
+        b->nb_coarse =
+            (prescan_factor > 1 && nb / prescan_factor >= 8) ? nb / prescan_factor : 0;
+      }
+      nbands++;
+    }
+    lo = hi;
+  }
+  return nbands;
+}
+
 /* scan.rs:413-473 */
 int orc_signal_search(const double* t, const double* v, const double* w,
                       size_t n, double c, double origin,

@@ -126,6 +126,19 @@ int orc_signal_search(const double* t, const double* v, const double* w,
                       orc_scan_stats* stats, size_t* nbands,
                       size_t* grid_count);
 
+/* Band plan of signal_search (scan.rs:440-471) and scan_periods' nb
+ * (scan.rs:377-381).  Per band: lo, first index, count, nd, nb, nb_coarse
+ * (0 without prescan).  Returns the number of bands (<= cap written). */
+typedef struct {
+  double lo;
+  size_t first, count, nd;
+  uint32_t nb, nb_coarse;
+} orc_band;
+size_t orc_plan_bands(const double* grid, size_t count,
+                      const double* sorted_durations, size_t nd,
+                      uint32_t phase_oversample, uint32_t prescan_factor,
+                      orc_band* out, size_t cap);
+
 /* batch count of scan_bank's memory budget, scan.rs:177-183 */
 uint64_t orc_bank_batches(size_t np, uint32_t nb, uint32_t period_batch,
                           double max_fold_mib);

@@ -77,3 +77,50 @@ def test_scan_bank_vs_unquantised(gpu):
     assert rel.max() < 1e-6
     same = (win == owin).all(axis=1)
     assert same.mean() > 0.99
+
+
+def test_signal_search_bench_star(gpu):
+    """The bench workload end to end (grid, bands, prescan, rows, band
+    statistics) against the oracle's signal_search on the same fields."""
+    from fbls_amd import synth
+
+    t, f, e = synth.four_sector_red(seed=2)
+    cfg = F.SearchConfig(period_min=0.5, period_max=20.0)
+    d = cfg.durations_days()
+    s = F.family_state(t, f - 1.0, e, d, cfg.correlated_noise)
+    dev = F.DeviceReady.upload(gpu, s)
+    vq, wq, _, _ = dev.fields()
+    print(f"[parity] signal_search 4-sector n={len(s.t)}", flush=True)
+    rows, grid, bands = F.signal_search(gpu, dev, d, s.rho, cfg)
+    orows, obands, og, ost = O.signal_search(
+        s.t, vq, wq, s.baseline_norm, s.origin, d, s.rho,
+        cfg.ingress_fractions, cfg, cfg.period_min, cfg.period_max,
+        cfg.period_oversample)
+    assert ost == 0
+    assert len(grid) == og
+    assert len(bands) == len(obands)
+    for b, ob in zip(bands, obands):
+        for k in ("periods", "prescan_survivors", "phase_bins", "batches",
+                  "templates_scanned"):
+            assert b[k] == ob[k], (k, b, ob)
+    np.testing.assert_array_equal(rows, orows)
+    best = rows[np.argmax(rows[:, 3])]
+    assert abs(best[0] - 11.3) < 0.01 and best[3] > 7.1
+
+
+@pytest.mark.parametrize("launch", [(0, 256, 2), (32768, 512, 1), (160 * 1024, 256, 1)])
+def test_launch_shape_does_not_change_results(gpu, launch):
+    """LDS budget, workgroup size and fold mode change the schedule only."""
+    cfg = F.SearchConfig(period_min=0.5, period_max=13.7)
+    s, dev, vq, wq, d = _setup(gpu, cfg, seed=5)
+    periods = np.linspace(0.6, 13.0, 700)
+    bank = F.make_bank(cfg, d, s.rho)
+    print(f"[parity] launch {launch}", flush=True)
+    ref = F.scan_bank(gpu, dev, periods, bank, 1536)
+    gpu.set_launch(*launch)
+    try:
+        got = F.scan_bank(gpu, dev, periods, bank, 1536)
+    finally:
+        gpu.set_launch(0, 0, 0)
+    for a, b in zip(ref[:4], got[:4]):
+        np.testing.assert_array_equal(a, b)

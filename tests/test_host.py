@@ -1,0 +1,137 @@
+"""CPU tests of the product's host side: the C ABI loads and exports every
+declared symbol, and the host helpers (grid, bands, durations, prepared-curve
+noise model, fixed point, synthetic generator) agree with the oracle."""
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fbls_amd as F
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def test_abi_exports_every_declared_symbol():
+    text = open(os.path.join(ROOT, "include", "fbls.h")).read()
+    declared = set(re.findall(r"\b(fbls_[a-z_0-9]+)\s*\(", text))
+    assert declared, "no declarations found"
+    lib = F.lib()
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert declared == set(F.EXPORTS)
+
+
+def test_period_grid_matches_oracle():
+    for args in [(0.5, 20.0, 50.0, 0.02, 3.0), (0.5, 13.7, 27.4, 0.5 / 24, 3.0),
+                 (0.3, 100.0, 730.0, 0.5 / 24, 3.0)]:
+        np.testing.assert_array_equal(F.period_grid(*args), O.period_grid(*args))
+
+
+def test_durations_days():
+    cfg = F.SearchConfig()
+    np.testing.assert_array_equal(cfg.durations_days(),
+                                  O.durations_days(cfg.durations_hours))
+    assert len(cfg.durations_days()) == 13
+
+
+def test_band_plan_matches_oracle_and_report():
+    d = json.load(open(os.path.join(GOLDEN, "search_bands_72668830.json")))
+    cfg_hours = d["config"]["durations_hours"]
+    for f in d["passes"][0]["families"]:
+        hrs = sorted(min(cfg_hours, key=lambda c: abs(c - h))
+                     for h in f["durations_hours"])
+        days = F.durations_days(hrs)
+        pmin = max(0.3, 4.0 * days[0] * (1.0 + 1e-9))
+        step = math.log(100.0 / pmin) / (f["periods"] - 1.5)
+        baseline = days[0] / (3.0 * math.expm1(step))
+        grid = F.period_grid(pmin, 100.0, baseline, days[0], 3.0)
+        assert len(grid) == f["periods"]
+        mine = F.plan_bands(grid, days, 4, 2)
+        ref = O.plan_bands(grid, days, 4, 2)
+        assert mine == ref
+        assert [b["count"] for b in mine] == [b["periods"] for b in f["bands"]]
+
+
+def test_bank_batches_formula():
+    for np_, nb in [(289901, 58), (254845, 14746), (1, 19200), (0, 10)]:
+        assert F.bank_batches(np_, nb) == O.bank_batches(np_, nb, 4096, 1024.0)
+    assert F.bank_batches(10, 1 << 20, 4096, 1.0) == 0  # over budget
+
+
+def test_family_state_matches_oracle():
+    from fbls_amd import synth
+
+    for maker in (synth.single_sector, synth.four_sector_red):
+        t, f, e = maker()
+        d = F.SearchConfig().durations_days()
+        s = F.family_state(t, f - 1.0, e, d)
+        o = O.family_state(t, f - 1.0, e, d)
+        np.testing.assert_array_equal(s.t, o["t"])
+        np.testing.assert_array_equal(s.v, o["v"])
+        np.testing.assert_array_equal(s.w, o["w"])
+        np.testing.assert_array_equal(s.rho, o["rho"])
+        assert s.baseline_norm == o["baseline_norm"]
+        assert s.origin == o["origin"]
+        assert s.inflation == o["inflation"] and s.segments == o["segments"]
+
+
+def test_family_state_kat_and_errors():
+    """curve.rs:636-661 through the product's prepared-curve entry, plus the
+    reference's data errors (curve.rs:114-126, 429-431)."""
+    n = 500
+    t = 1000.0 + np.arange(n) * 0.01
+    f = 2000.0 + 3.0 * np.sin(np.arange(n) * 1.7)
+    s = F.family_state(t, f - 1.0, np.ones(n), [0.05])
+    assert s.origin == 1000.0 and s.t[0] == 0.0
+    assert s.inflation > 2.0 and s.segments == 1
+    with pytest.raises(F.FblsError) as ei:
+        F.family_state(t[:5], f[:5] - 1.0, np.ones(5), [0.05])
+    assert ei.value.code == 6
+    with pytest.raises(F.FblsError):
+        F.family_state(t, -np.ones(n) * 3.0, np.ones(n), [0.05])  # median <= 0
+
+
+def test_quantize_matches_oracle():
+    rng = np.random.default_rng(9)
+    for scale in (1e-3, 1.0, 1e6):
+        x = rng.normal(0, scale, 5000)
+        q1, e1 = F.quantize(x)
+        q2, e2 = O.quantize(x)
+        assert e1 == e2
+        np.testing.assert_array_equal(q1, q2)
+
+
+def test_synth_uses_reference_rng():
+    """The generator's white noise is the reference's Rng stream
+    (survey/numeric.rs:345-364): first point of a noise-only curve."""
+    t, f, e = F.synth_star(seed=77, sectors=1, sigma=1e-3, period=0.0,
+                           depth=0.0, epoch=0.0)
+    r = O.Rng(77)
+    assert f[0] == 1.0 + r.normal(0.0, 1e-3)
+    assert f[1] == 1.0 + r.normal(0.0, 1e-3)
+    assert np.all(e == 1e-3)
+    assert abs(len(t) - (27.4 - 1.0) * 720) < 2
+
+
+def test_synth_injects_box():
+    t, f, e = F.synth_star(seed=1, sectors=1, sigma=0.0, period=3.7,
+                           depth=2e-3, duration=2.5 / 24, epoch=2458001.0)
+    in_tr = np.abs(((t - 2458001.0 + 1.85) % 3.7) - 1.85) <= 2.5 / 48
+    assert np.all(f[in_tr] == 1.0 - 2e-3) and np.all(f[~in_tr] == 1.0)
+
+
+def test_config_validation():
+    F.SearchConfig().validate()
+    with pytest.raises(ValueError):
+        F.SearchConfig(period_min=0.01).validate()  # dmin >= pmin / 4
+    with pytest.raises(ValueError):
+        F.SearchConfig(prescan_fraction=0.0).validate()
+    with pytest.raises(ValueError):
+        F.SearchConfig(ingress_fractions=[0.0, 0.5]).validate()
+    assert F.SearchConfig().shape_gate == 0.6 * 7.1
+    assert F.SearchConfig(ingress_fractions=[0.1]).shape_gate == 0.0
