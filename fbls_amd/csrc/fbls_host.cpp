@@ -109,6 +109,7 @@ struct fbls_family {
   uint32_t n = 0, n_pad = 0;
   double C = 0.0, origin = 0.0, t_max = 0.0, t_last = 0.0;
   int ev = 0, ew = 0;
+  bool sorted = false;   // times ascend
   std::vector<int32_t> vq, wq;
 };
 
@@ -219,6 +220,8 @@ extern "C" int fbls_family_upload(fbls_ctx* c, const double* t,
   f->wq.resize(n);
   f->ev = fbls::quantize(v, n, f->vq.data());
   f->ew = fbls::quantize(w, n, f->wq.data());
+  f->sorted = true;
+  for (size_t i = 1; i < n; i++) f->sorted = f->sorted && t[i] >= t[i - 1];
   std::vector<double> tp(f->n_pad, 0.0);
   std::vector<int32_t> vw(2 * (size_t)f->n_pad, 0);
   std::memcpy(tp.data(), t, n * sizeof(double));
@@ -316,8 +319,25 @@ size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // Runs every Work in one kernel launch.  Writes each Work's host outputs and
 // retry count; returns the launch's device time through *seconds.
-int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
-             double* seconds) {
+// Longest template of a segment, in bins: max over its periods and
+// durations of min(max(1, floor(D nb / P + 0.5)), nb / 2) (kernels.rs:679-680).
+uint32_t longest_template(const Work& x) {
+  double pmin = INFINITY, dmax = 0.0;
+  for (size_t k = 0; k < x.np; k++) pmin = std::min(pmin, x.periods[k]);
+  for (uint32_t k = 0; k < x.nd; k++) dmax = std::max(dmax, x.dur[k]);
+  const double rk = std::floor(dmax * ((double)x.nb / pmin) + 0.5);
+  const uint32_t raw = rk < 1.0 ? 1u : (rk > 1e9 ? 1000000000u : (uint32_t)rk);
+  return std::min(raw, x.nb / 2);
+}
+
+bool needs_split(const fbls_ctx* c, const Work& x, const Shared& sh) {
+  return fbls::scan_lds_bytes(x.nb, 1, x.nd, sh.ns, sh.layers, nullptr) > c->max_lds;
+}
+
+// One kernel launch over the Works: the LDS-resident kernel, or (split) the
+// range-split kernel for periods whose histogram exceeds the LDS.
+int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
+               double* seconds) {
   *seconds = 0.0;
   size_t total = 0;
   for (size_t i = 0; i < nw; i++) total += w[i].np;
@@ -326,16 +346,33 @@ int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
   HIP_TRY(hipSetDevice(c->device));
   const uint32_t ns = sh.ns, layers = sh.layers;
   // LDS size of the launch and periods per workgroup of each segment
-  size_t lds = c->lds_override ? c->lds_override : 64 * 1024;
+  size_t lds = c->lds_override ? c->lds_override : 32 * 1024;
   std::vector<size_t> one(nw);
+  std::vector<uint32_t> range(nw, 0), halo(nw, 0);
   for (size_t i = 0; i < nw; i++) {
     if (w[i].np == 0) continue;
     if (w[i].nb < 2 || w[i].nb >= 65536)
       return fail(FBLS_E_ARG, "phase bins must be in [2, 65535], got %u", w[i].nb);
-    one[i] = fbls::scan_lds_bytes(w[i].nb, 1, w[i].nd, ns, layers, nullptr);
-    if (one[i] > c->max_lds)
-      return fail(FBLS_E_ARG, "%u phase bins need %zu bytes of LDS per period (limit %zu)",
-                  w[i].nb, one[i], c->max_lds);
+    if (split) {
+      if (sh.ingress[0] != 0.0)
+        return fail(FBLS_E_ARG, "phase bins beyond the LDS need ingress[0] = 0");
+      halo[i] = longest_template(w[i]);
+      // widest multiple of 128 bins whose window fits the LDS
+      uint32_t B = 128;
+      while (fbls::split_lds_bytes(w[i].nb, B + 128, halo[i], w[i].nd, ns, layers) <=
+                 c->max_lds &&
+             B < w[i].nb)
+        B += 128;
+      if (fbls::split_lds_bytes(w[i].nb, B, halo[i], w[i].nd, ns, layers) > c->max_lds)
+        return fail(FBLS_E_ARG, "templates of %u bins do not fit the LDS", halo[i]);
+      range[i] = B;
+      one[i] = fbls::split_lds_bytes(w[i].nb, B, halo[i], w[i].nd, ns, layers);
+    } else {
+      one[i] = fbls::scan_lds_bytes(w[i].nb, 1, w[i].nd, ns, layers, nullptr);
+      if (one[i] > c->max_lds)
+        return fail(FBLS_E_ARG, "%u phase bins need %zu bytes of LDS per period (limit %zu)",
+                    w[i].nb, one[i], c->max_lds);
+    }
     lds = std::max(lds, one[i]);
   }
   const uint32_t g_cap = (uint32_t)std::max<size_t>(1, total / 1024);
@@ -419,13 +456,21 @@ int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
     S.np = (uint32_t)x.np;
     S.nb = x.nb;
     S.nd = x.nd;
-    uint32_t G = (uint32_t)std::max<size_t>(1, lds / one[i]);
-    G = std::min<uint32_t>(std::min<uint32_t>(G, 16), g_cap);
-    while (G > 1 && fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr) > lds) G--;
+    uint32_t G = 1;
+    if (split) {
+      S.range_bins = range[i];
+      S.halo_bins = halo[i];
+      lds_max = std::max(lds_max, one[i]);
+    } else {
+      G = (uint32_t)std::max<size_t>(1, lds / one[i]);
+      G = std::min<uint32_t>(std::min<uint32_t>(G, 16), g_cap);
+      while (G > 1 && fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr) > lds) G--;
+      fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, &S.pstride);
+      lds_max = std::max(lds_max, fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr));
+    }
     S.G = G;
-    fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, &S.pstride);
-    lds_max = std::max(lds_max, fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr));
     S.wide_cells = (f->t_max * smax >= 16777215.0) ? 1u : 0u;
+    S.sorted = f->sorted ? 1u : 0u;
     S.group0 = groups;
     S.index = (uint32_t)segs.size();
     groups += (uint32_t)((x.np + G - 1) / G);
@@ -446,10 +491,11 @@ int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
   A.factors = (const double*)(din + fac_at * 8);
   A.gate = sh.gate;
   A.counters = (unsigned int*)(din + cnt_off);
-  const uint32_t block = c->block_override ? c->block_override : 512;
+  // 256 threads: measured fastest on MI355X for every LDS budget 16-48 KiB
+  const uint32_t block = c->block_override ? c->block_override : 256;
   HIP_TRY(hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  HIP_TRY(fbls::scan_launch(A, groups, block, lds_max, c->stream));
+  HIP_TRY(fbls::scan_launch(A, groups, block, lds_max, split, c->stream));
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   bool any_winner = false;
   for (size_t i = 0; i < nw; i++) any_winner = any_winner || (w[i].winner && w[i].np);
@@ -478,6 +524,32 @@ int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
   if (bad)
     return fail(FBLS_E_NONPOSITIVE_PRECISION,
                 "nonpositive template precision; inspect noise model");
+  return FBLS_OK;
+}
+
+// Runs every Work: the LDS-resident kernel for those whose histogram fits,
+// then the range-split kernel for the rest (same stream, two launches).
+int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
+             double* seconds) {
+  std::vector<Work> lo, hi;
+  std::vector<size_t> lo_at, hi_at;
+  for (size_t i = 0; i < nw; i++) {
+    if (needs_split(c, w[i], sh)) {
+      hi.push_back(w[i]);
+      hi_at.push_back(i);
+    } else {
+      lo.push_back(w[i]);
+      lo_at.push_back(i);
+    }
+  }
+  double s1 = 0.0, s2 = 0.0;
+  int st = run_launch(c, lo.data(), lo.size(), sh, false, &s1);
+  if (st) return st;
+  st = run_launch(c, hi.data(), hi.size(), sh, true, &s2);
+  if (st) return st;
+  for (size_t k = 0; k < lo.size(); k++) w[lo_at[k]].retries = lo[k].retries;
+  for (size_t k = 0; k < hi.size(); k++) w[hi_at[k]].retries = hi[k].retries;
+  *seconds = s1 + s2;
   return FBLS_OK;
 }
 

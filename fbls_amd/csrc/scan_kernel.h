@@ -27,6 +27,10 @@ struct Segment {
   uint32_t wide_cells;    // 1: t*scale may reach 2^24, use integer modulo
   uint32_t group0;        // first workgroup of this segment in the launch
   uint32_t index;         // segment index: counters[2 index], [2 index + 1]
+  uint32_t range_bins;    // range-split kernel: bins per range (x128)
+  uint32_t halo_bins;     // range-split kernel: halo >= the longest template
+  uint32_t sorted;        // 1: times ascend (incremental phase bins)
+  uint32_t pad_;
 };
 
 struct LaunchArgs {
@@ -43,9 +47,13 @@ struct LaunchArgs {
 // LDS bytes of one workgroup holding G periods of nb bins.
 size_t scan_lds_bytes(uint32_t nb, uint32_t G, uint32_t nd, uint32_t ns,
                       uint32_t layers, uint32_t* period_stride);
+// LDS bytes of one range-split workgroup (one period).
+size_t split_lds_bytes(uint32_t nb, uint32_t range_bins, uint32_t halo_bins,
+                       uint32_t nd, uint32_t ns, uint32_t layers);
 hipError_t scan_configure(size_t max_lds);
-// groups: total workgroups (sum over segments of ceil(np / G)).
+// groups: total workgroups (sum over segments of ceil(np / G)); split: the
+// range-split kernel (every segment with G = 1 and range_bins set).
 hipError_t scan_launch(const LaunchArgs& args, uint32_t groups, uint32_t block,
-                       size_t lds, hipStream_t stream);
+                       size_t lds, bool split, hipStream_t stream);
 
 }  // namespace fbls

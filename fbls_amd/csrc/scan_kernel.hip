@@ -1,13 +1,17 @@
 // Fused FBLS period scan for gfx950 (MI355X).
 //
 // A launch carries segments (one family at one phase-bin count over a list
-// of periods: the work of one reference scan_bank call).  Each workgroup owns
-// G periods of one segment.  It folds every observation of the family into
-// per-period LDS histograms (fixed-point values, so the sums are exact and
-// order-independent), turns them into exclusive prefix sums in place, scans
-// every (phase bin, duration, shape) template with an FP64 surrogate of the
-// squared merit, resolves near ties with the exact FP64 merit, and writes one
-// record per period.  Nothing but the record touches HBM.
+// of periods: the work of one reference scan_bank call).  In the LDS-resident
+// kernel each workgroup owns G periods of one segment: it folds every
+// observation of the family into per-period LDS histograms (fixed-point
+// values, so the sums are exact and order-independent), turns them into
+// exclusive prefix sums in place, scans every (phase bin, duration, shape)
+// template with an FP64 surrogate of the squared merit, resolves near ties
+// with the exact FP64 merit, and writes one record per period.  When one
+// period's histogram does not fit in LDS (large nb), the range-split kernel
+// folds the family once per bin range (plus a halo as wide as the longest
+// template), carries the exact prefix across ranges and scans each range's
+// tiles in turn.  Nothing but the record touches HBM.
 //
 // Reference semantics restated here (isakruas/fbls):
 //   phase cell           src/search/kernels.rs:314-327
@@ -41,23 +45,43 @@ constexpr uint32_t kObsPerWave = 64 * kObsPerLane;
 constexpr double kInf = __builtin_inf();
 constexpr double kNear = 1e-10;       // near-tie window of the surrogate
 
-struct Cum {
-  const double2* vw;  // exclusive prefix of {v, w}, dequantised
-  const uint32_t* n;  // exclusive prefix of counts
-  double tv, tw;      // totals E[nb]
+// ------------------------------------------------------------ prefix views
+
+// The whole period in LDS; the doubled prefix (kernels.rs:7-10) is virtual:
+// cum[x] = E[x] for x <= nb, E[x - nb] + total above.
+struct WrapCum {
+  const double2* vw;
+  const uint32_t* n;
+  double tv, tw;
   uint32_t tn, nb;
+  __device__ __forceinline__ double2 vw_at(uint32_t x) const {
+    if (x <= nb) return vw[x];
+    double2 e = vw[x - nb];
+    return make_double2(e.x + tv, e.y + tw);
+  }
+  __device__ __forceinline__ uint32_t n_at(uint32_t x) const {
+    return x <= nb ? n[x] : n[x - nb] + tn;
+  }
 };
 
-// doubled prefix (kernels.rs:7-10): cum[x] for x in [0, 2nb]
-__device__ __forceinline__ double2 cum_vw(const Cum& c, uint32_t x) {
-  if (x <= c.nb) return c.vw[x];
-  double2 e = c.vw[x - c.nb];
-  return make_double2(e.x + c.tv, e.y + c.tw);
-}
+// A window [x0, x0 + len] of the doubled prefix: the exact base cum[x0] plus
+// the window's local prefix (all values are integers below 2^53, so the sum
+// is the same double as the full prefix would hold).
+struct WindowCum {
+  const double2* vw;
+  const uint32_t* n;
+  double bv, bw;
+  uint32_t bn, x0;
+  __device__ __forceinline__ double2 vw_at(uint32_t x) const {
+    double2 e = vw[x - x0];
+    return make_double2(bv + e.x, bw + e.y);
+  }
+  __device__ __forceinline__ uint32_t n_at(uint32_t x) const {
+    return bn + n[x - x0];
+  }
+};
 
-__device__ __forceinline__ uint32_t cum_n(const Cum& c, uint32_t x) {
-  return x <= c.nb ? c.n[x] : c.n[x - c.nb] + c.tn;
-}
+// ------------------------------------------------------------ merit helpers
 
 // Reciprocal with two Newton steps: relative error far below kNear.
 __device__ __forceinline__ double rcp_nr(double x) {
@@ -137,8 +161,10 @@ __device__ __forceinline__ void state_update(LaneState& st, double s,
 }
 
 // Staircase sums (kernels.rs:84-113) with precomputed layer widths/offsets.
-__device__ __forceinline__ void shape_sums(const Cum& c, uint32_t jj, double a0,
-                                           double w0, uint32_t layers,
+template <class CumT>
+__device__ __forceinline__ void shape_sums(const CumT& c, uint32_t jj,
+                                           double a0, double w0,
+                                           uint32_t layers,
                                            const uint32_t* tab, double* pa,
                                            double* php, double* pnorm) {
   double l = (double)layers;
@@ -149,8 +175,8 @@ __device__ __forceinline__ void shape_sums(const Cum& c, uint32_t jj, double a0,
     uint32_t e = tab[layer - 1];
     uint32_t width = e & 0xffffu;
     uint32_t start = jj + (e >> 16);
-    double2 hi = cum_vw(c, start + width);
-    double2 lo = cum_vw(c, start);
+    double2 hi = c.vw_at(start + width);
+    double2 lo = c.vw_at(start);
     double dw = hi.y - lo.y;
     double dv = hi.x - lo.x;
     a += dv / l;
@@ -163,7 +189,7 @@ __device__ __forceinline__ void shape_sums(const Cum& c, uint32_t jj, double a0,
 }
 
 // Staircase sums with widths computed from the factors (record recompute).
-__device__ void shape_sums_direct(const Cum& c, uint32_t j, uint32_t k,
+__device__ void shape_sums_direct(const WrapCum& c, uint32_t j, uint32_t k,
                                   double a0, double w0, double ingress,
                                   const double* factors, uint32_t si,
                                   uint32_t layers, double* pa, double* php,
@@ -183,8 +209,8 @@ __device__ void shape_sums_direct(const Cum& c, uint32_t j, uint32_t k,
     double wf = __builtin_floor((double)k * factor + 0.5);
     uint32_t width = wf < 1.0 ? 1u : (uint32_t)wf;
     uint32_t start = j + (k - width) / 2;
-    double2 hi = cum_vw(c, start + width);
-    double2 lo = cum_vw(c, start);
+    double2 hi = c.vw_at(start + width);
+    double2 lo = c.vw_at(start);
     double dw = hi.y - lo.y;
     a += (hi.x - lo.x) / l;
     hp += dw / l;
@@ -194,6 +220,8 @@ __device__ void shape_sums_direct(const Cum& c, uint32_t j, uint32_t k,
   *php = hp;
   *pnorm = norm;
 }
+
+// ------------------------------------------------------- wave primitives
 
 __device__ __forceinline__ double wave_max(double x) {
 #pragma unroll
@@ -221,14 +249,14 @@ __device__ __forceinline__ T wave_excl_scan(T x, uint32_t lane) {
   return incl - x;
 }
 
-// In-place inclusive scan of slots [1, nb] of {v, w} pairs by one wave
+// In-place inclusive scan of slots [1, len] of {v, w} pairs by one wave
 // (exact: integer-valued doubles below 2^53), then the power-of-two scales.
-__device__ void wave_prefix_vw(double2* arr, uint32_t nb, uint32_t lane,
+__device__ void wave_prefix_vw(double2* arr, uint32_t len_slots, uint32_t lane,
                                double sv, double sw) {
-  uint32_t len = (nb + 63) / 64;
+  uint32_t len = (len_slots + 63) / 64;
   uint32_t lo = 1 + lane * len;
   uint32_t hi = lo + len;
-  if (hi > nb + 1) hi = nb + 1;
+  if (hi > len_slots + 1) hi = len_slots + 1;
   double av = 0.0, aw = 0.0;
   for (uint32_t s = lo; s < hi; s++) {
     double2 x = arr[s];
@@ -245,12 +273,12 @@ __device__ void wave_prefix_vw(double2* arr, uint32_t nb, uint32_t lane,
   }
 }
 
-__device__ void wave_prefix_n(uint32_t* arr, uint32_t nb, uint32_t lane) {
-  uint32_t len = (nb + 63) / 64;
+__device__ void wave_prefix_n(uint32_t* arr, uint32_t len_slots, uint32_t lane) {
+  uint32_t len = (len_slots + 63) / 64;
   len |= 1u;  // odd stride between lanes spreads LDS banks
   uint32_t lo = 1 + lane * len;
   uint32_t hi = lo + len;
-  if (hi > nb + 1) hi = nb + 1;
+  if (hi > len_slots + 1) hi = len_slots + 1;
   uint32_t acc = 0;
   for (uint32_t s = lo; s < hi; s++) {
     acc += arr[s];
@@ -260,96 +288,396 @@ __device__ void wave_prefix_n(uint32_t* arr, uint32_t nb, uint32_t lane) {
   for (uint32_t s = lo; s < hi; s++) arr[s] += ex;
 }
 
+// ------------------------------------------------------------------ fold
+
 // floor(fl(t * scale)) mod nb (kernels.rs:314-327).  Narrow: t*scale < 2^24,
 // the quotient from a float reciprocal biased low by ~2^-22 is floor(u/nb)
 // or one less, so one conditional subtraction (as unsigned min) finishes.
+__device__ __forceinline__ uint32_t phase_cell_u(double t, double scale) {
+  return (uint32_t)(t * scale);  // floor for t >= 0
+}
+
 template <bool WIDE>
-__device__ __forceinline__ uint32_t phase_bin(double t, double scale,
-                                              uint32_t nb, float inv_lo) {
-  const uint32_t u = (uint32_t)(t * scale);
+__device__ __forceinline__ uint32_t phase_bin(uint32_t u, uint32_t nb,
+                                              float inv_lo) {
   if (WIDE) return u % nb;
   const uint32_t q = (uint32_t)((float)u * inv_lo);
   const uint32_t r = u - __umul24(q, nb);
   return min(r, r - nb);
 }
 
+// UNI skips the weight field (for callers that rebuild it from the counts).
+template <bool UNI>
 __device__ __forceinline__ void lds_add(double* vw, uint32_t* n, uint32_t b,
                                         double v, double w, uint32_t c) {
   __hip_atomic_fetch_add(vw + 2 * (b + 1), v, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
-  __hip_atomic_fetch_add(vw + 2 * (b + 1) + 1, w, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (!UNI)
+    __hip_atomic_fetch_add(vw + 2 * (b + 1) + 1, w, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
   __hip_atomic_fetch_add(n + b + 1, c, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Fold one lane's 8 consecutive observations into one period's histogram.
 // RUN: consecutive observations falling in the same bin are summed in
-// registers and flushed once (exact: integer-valued doubles).
-template <bool WIDE, bool RUN, bool FULL>
+// registers and flushed once (exact: integer-valued doubles).  INCR (times
+// ascend and the lane's 8 cells span fewer than nb): each bin is the previous
+// one advanced by the difference of floor(t * scale), wrapped once.
+template <bool WIDE, bool RUN, bool FULL, bool UNI, bool INCR>
 __device__ __forceinline__ void fold_lane(double* vw, uint32_t* n,
-                                          const double* tt, const double* vv,
-                                          const double* ww, double scale,
-                                          uint32_t nb, float inv_lo,
-                                          uint32_t base, uint32_t nobs) {
+                                          const uint32_t* u, const double* vv,
+                                          const double* ww, uint32_t nb,
+                                          float inv_lo, uint32_t base,
+                                          uint32_t nobs) {
   if (RUN && FULL) {
-    uint32_t cb = phase_bin<WIDE>(tt[0], scale, nb, inv_lo);
-    double av = vv[0], aw = ww[0];
+    uint32_t cb = phase_bin<WIDE>(u[0], nb, inv_lo);
+    double av = vv[0], aw = UNI ? 0.0 : ww[0];
     uint32_t an = 1;
 #pragma unroll
     for (uint32_t r = 1; r < kObsPerLane; r++) {
-      const uint32_t b = phase_bin<WIDE>(tt[r], scale, nb, inv_lo);
-      const bool flush = b != cb;
-      if (flush) lds_add(vw, n, cb, av, aw, an);
-      const double sv = av + vv[r], sw = aw + ww[r];
+      uint32_t b;
+      bool flush;
+      if (INCR) {
+        const uint32_t d = u[r] - u[r - 1];
+        flush = d != 0;
+        b = cb + d;
+        b = min(b, b - nb);
+      } else {
+        b = phase_bin<WIDE>(u[r], nb, inv_lo);
+        flush = b != cb;
+      }
+      if (flush) lds_add<UNI>(vw, n, cb, av, aw, an);
+      const double sv = av + vv[r];
       av = flush ? vv[r] : sv;
-      aw = flush ? ww[r] : sw;
+      if (!UNI) {
+        const double sw = aw + ww[r];
+        aw = flush ? ww[r] : sw;
+      }
       an = flush ? 1u : an + 1u;
       cb = b;
     }
-    lds_add(vw, n, cb, av, aw, an);
+    lds_add<UNI>(vw, n, cb, av, aw, an);
   } else {
 #pragma unroll
     for (uint32_t r = 0; r < kObsPerLane; r++) {
-      const uint32_t b = phase_bin<WIDE>(tt[r], scale, nb, inv_lo);
-      if (FULL || base + r < nobs) lds_add(vw, n, b, vv[r], ww[r], 1u);
+      const uint32_t b = phase_bin<WIDE>(u[r], nb, inv_lo);
+      if (FULL || base + r < nobs) lds_add<UNI>(vw, n, b, vv[r], ww[r], 1u);
     }
   }
 }
 
-template <bool WIDE, bool RUN>
+// Global-address-space views: the observation loads become global (not flat)
+// loads, so waiting for them never drains the LDS atomics in flight.
+typedef __attribute__((address_space(1))) const double gdouble;
+typedef __attribute__((address_space(1))) const int gint;
+
+__device__ __forceinline__ void load_obs(const Segment& S, uint32_t base,
+                                         double* tt, double* vv, double* ww) {
+  gdouble* tp = (gdouble*)__builtin_assume_aligned(S.t + base, 64);
+  gint* qp = (gint*)__builtin_assume_aligned(S.vw + 2 * (size_t)base, 64);
+#pragma unroll
+  for (uint32_t r = 0; r < kObsPerLane; r++) {
+    tt[r] = tp[r];
+    vv[r] = (double)qp[2 * r];
+    ww[r] = (double)qp[2 * r + 1];
+  }
+}
+
+__device__ __forceinline__ float inv_nb_low(uint32_t nb) {
+  return (float)((1.0 / (double)nb) * (1.0 - 0x1p-22));
+}
+
+template <bool WIDE, bool RUN, bool UNI>
 __device__ void fold_all(const Segment& S, unsigned char* hist,
                          uint32_t p0, uint32_t gcount, uint32_t wave,
                          uint32_t lane, uint32_t nwaves) {
   const uint32_t nb = S.nb;
   const uint32_t nb1 = nb + 1;
-  const float inv_lo = (float)((1.0 / (double)nb) * (1.0 - 0x1p-22));
+  const float inv_lo = inv_nb_low(nb);
   const uint32_t nblk = S.n_pad / kObsPerWave;
   for (uint32_t blk = wave; blk < nblk; blk += nwaves) {
     const uint32_t base = blk * kObsPerWave + lane * kObsPerLane;
     double tt[kObsPerLane], vv[kObsPerLane], ww[kObsPerLane];
-    const double2* tp = (const double2*)(S.t + base);
-    const int4* qp = (const int4*)(S.vw + 2 * (size_t)base);
-#pragma unroll
-    for (uint32_t r = 0; r < kObsPerLane / 2; r++) {
-      double2 d = tp[r];
-      int4 q = qp[r];
-      tt[2 * r] = d.x;
-      tt[2 * r + 1] = d.y;
-      vv[2 * r] = (double)q.x;
-      ww[2 * r] = (double)q.y;
-      vv[2 * r + 1] = (double)q.z;
-      ww[2 * r + 1] = (double)q.w;
-    }
+    load_obs(S, base, tt, vv, ww);
     const bool full = blk * kObsPerWave + kObsPerWave <= S.n;
     for (uint32_t g = 0; g < gcount; g++) {
       const double scale = S.scales[p0 + g];
       double* vw = (double*)(hist + (size_t)g * S.pstride);
       uint32_t* n = (uint32_t*)(vw + 2 * nb1);
-      if (full)
-        fold_lane<WIDE, RUN, true>(vw, n, tt, vv, ww, scale, nb, inv_lo, base, S.n);
-      else
-        fold_lane<WIDE, RUN, false>(vw, n, tt, vv, ww, scale, nb, inv_lo, base, S.n);
+      uint32_t u[kObsPerLane];
+#pragma unroll
+      for (uint32_t r = 0; r < kObsPerLane; r++) u[r] = phase_cell_u(tt[r], scale);
+      if (!full) {
+        fold_lane<WIDE, RUN, false, UNI, false>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
+      } else if (RUN && S.sorted && __all(u[kObsPerLane - 1] - u[0] < nb)) {
+        fold_lane<WIDE, RUN, true, UNI, true>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
+      } else {
+        fold_lane<WIDE, RUN, true, UNI, false>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
+      }
+    }
+  }
+}
+
+// Fold into the window [x0, x0 + wlen) of the bins (mod nb) of one period.
+template <bool WIDE>
+__device__ void fold_window(const Segment& S, double* vw, uint32_t* n,
+                            double scale, uint32_t x0, uint32_t wlen,
+                            uint32_t wave, uint32_t lane, uint32_t nwaves) {
+  const uint32_t nb = S.nb;
+  const float inv_lo = inv_nb_low(nb);
+  const uint32_t nblk = S.n_pad / kObsPerWave;
+  for (uint32_t blk = wave; blk < nblk; blk += nwaves) {
+    const uint32_t base = blk * kObsPerWave + lane * kObsPerLane;
+    double tt[kObsPerLane], vv[kObsPerLane], ww[kObsPerLane];
+    load_obs(S, base, tt, vv, ww);
+#pragma unroll
+    for (uint32_t r = 0; r < kObsPerLane; r++) {
+      const uint32_t b = phase_bin<WIDE>(phase_cell_u(tt[r], scale), nb, inv_lo);
+      const uint32_t rel = b >= x0 ? b - x0 : b + nb - x0;
+      if (rel < wlen && base + r < S.n) lds_add<false>(vw, n, rel, vv[r], ww[r], 1u);
+    }
+  }
+}
+
+// ------------------------------------------------------------ tile scan
+
+struct TileCtx {
+  const uint32_t* kt;     // [nd] k | kok << 31
+  const uint32_t* stab;   // [nd][ns-1][layers-1] width | offset << 16
+  const double* rho;      // [nd]
+  const double* ingress;  // [ns]
+  double c, inv_c, gate, g2;
+  uint32_t nb, nd, ns, layers, nobs, lane;
+};
+
+struct TileOut {
+  double value, a, hp, norm;
+  uint64_t key;
+  uint32_t di;
+};
+
+// The winner of one 128-bin tile (kernels.rs:608-845): every duration's box
+// for both of the lane's bins, the tile gate, the staircase shapes, the
+// surrogate lane states, the exact value of the winner and an exact rescan
+// when the top is near-tied.  Wave-uniform result.
+template <class CumT>
+__device__ TileOut scan_tile(const CumT& cm, const TileCtx& k, uint32_t tile,
+                             bool* bad, unsigned int* rescans) {
+  const uint32_t nb = k.nb, nb1 = nb + 1, nd = k.nd, ns = k.ns;
+  const uint32_t layers = k.layers, lane = k.lane;
+  const uint32_t nsl = (ns - 1) * (layers - 1);
+  uint32_t jx[2], jjx[2];
+  double ev[2], ew[2];
+  uint32_t en[2];
+  LaneState st[2];
+#pragma unroll
+  for (int x = 0; x < 2; x++) {
+    jx[x] = tile * kTile + lane + 64 * x;
+    jjx[x] = jx[x] < nb ? jx[x] : nb - 1;
+    double2 e = cm.vw_at(jjx[x]);
+    ev[x] = e.x;
+    ew[x] = e.y;
+    en[x] = cm.n_at(jjx[x]);
+    state_init(st[x]);
+  }
+  uint32_t gmask = 0;
+  for (uint32_t di = 0; di < nd; di++) {
+    const uint32_t kte = k.kt[di];
+    const uint32_t kk = kte & 0x7fffffffu;
+    const bool kok = (kte >> 31) != 0;
+    const double rho = k.rho[di];
+    const double rho2 = rho * rho;
+    double bv[2], bw[2];
+    bool busable[2];
+    bool pass = false;
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const uint32_t e = jjx[x] + kk;
+      const uint32_t cnt = cm.n_at(e) - en[x];
+      const double2 hi = cm.vw_at(e);
+      const double w = hi.y - ew[x];
+      const double a = hi.x - ev[x];
+      const bool usable = jx[x] < nb && kok && cnt >= 3 && cnt + 3 <= k.nobs;
+      const bool valid = usable && a > 0.0;
+      bv[x] = a;
+      bw[x] = w;
+      busable[x] = usable;
+      if (valid) {
+        const uint64_t key = (uint64_t)kk * nb1 + jjx[x];
+        const double s = surrogate(a, w, w, rho, rho2, k.c, k.inv_c, bad);
+        state_update(st[x], s, key, di, a, w, w, rho);
+        if (s > -kInf) {
+          if (k.gate <= 0.0 || s >= k.g2 * (1.0 + kNear)) {
+            pass = true;
+          } else if (s >= k.g2 * (1.0 - kNear)) {
+            pass = pass || exact_merit(a, w, w, rho, k.c, bad) >= k.gate;
+          }
+        }
+      }
+    }
+    if (!__any(pass)) continue;
+    gmask |= 1u << di;
+    for (uint32_t si = 1; si < ns; si++) {
+      const uint32_t* tab = k.stab + (di * (ns - 1) + (si - 1)) * (layers - 1);
+      const bool staircase = k.ingress[si] != 0.0;
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        double a = bv[x], hp = bw[x], norm = bw[x];
+        if (staircase) shape_sums(cm, jjx[x], bv[x], bw[x], layers, tab, &a, &hp, &norm);
+        if (busable[x] && a > 0.0) {
+          const uint64_t key = ((uint64_t)si * nb1 + kk) * nb1 + jjx[x];
+          const double s = surrogate(a, hp, norm, rho, rho2, k.c, k.inv_c, bad);
+          state_update(st[x], s, key, di, a, hp, norm, rho);
+        }
+      }
+    }
+  }
+  (void)nsl;
+
+  TileOut out;
+  out.value = -kInf;
+  out.key = ~0ull;
+  out.di = 0;
+  out.a = out.hp = out.norm = 0.0;
+  const double m = wave_max(__builtin_fmax(st[0].s, st[1].s));
+  if (!(m > -kInf)) return out;
+  const double thr = m * (1.0 - kNear);
+  bool near[2];
+  uint64_t kmin = ~0ull;
+#pragma unroll
+  for (int x = 0; x < 2; x++) {
+    near[x] = st[x].s >= thr;
+    if (st[x].s == m && st[x].key < kmin) kmin = st[x].key;
+  }
+  const uint64_t top_key = wave_min_u64(kmin);
+  const uint64_t owner_mask = __ballot(st[0].key == top_key || st[1].key == top_key);
+  const int owner = __builtin_ctzll(owner_mask);
+  const int ox = __shfl((st[1].key == top_key) ? 1 : 0, owner, 64);
+  const double ta = __shfl(ox ? st[1].a : st[0].a, owner, 64);
+  const double thp = __shfl(ox ? st[1].hp : st[0].hp, owner, 64);
+  const double tno = __shfl(ox ? st[1].norm : st[0].norm, owner, 64);
+  const double trh = __shfl(ox ? st[1].rho : st[0].rho, owner, 64);
+  const uint32_t tdi0 = __shfl(ox ? st[1].di : st[0].di, owner, 64);
+  bool need = false;
+#pragma unroll
+  for (int x = 0; x < 2; x++) {
+    if (near[x]) {
+      const bool same = st[x].a == ta && st[x].hp == thp && st[x].norm == tno &&
+                        st[x].rho == trh;
+      need = need || !same || st[x].second >= st[x].s * (1.0 - kNear);
+    }
+  }
+  if (!__any(need)) {
+    out.key = top_key;
+    out.di = tdi0;
+    out.a = ta;
+    out.hp = thp;
+    out.norm = tno;
+    out.value = exact_merit(ta, thp, tno, trh, k.c, bad);
+    return out;
+  }
+  // exact rescan of the near states (kernels.rs:662-798 retry pass)
+  double eb[2] = {-kInf, -kInf};
+  uint64_t ek[2] = {~0ull, ~0ull};
+  uint32_t ed[2] = {0, 0};
+  for (int x = 0; x < 2; x++) {
+    if (!near[x]) continue;
+    for (uint32_t di = 0; di < nd; di++) {
+      const uint32_t kte = k.kt[di];
+      const uint32_t kk = kte & 0x7fffffffu;
+      const bool kok = (kte >> 31) != 0;
+      const double rho = k.rho[di];
+      const uint32_t e = jjx[x] + kk;
+      const uint32_t cnt = cm.n_at(e) - en[x];
+      const double2 hi = cm.vw_at(e);
+      const double w = hi.y - ew[x];
+      const double a = hi.x - ev[x];
+      const bool usable = jx[x] < nb && kok && cnt >= 3 && cnt + 3 <= k.nobs;
+      if (usable && a > 0.0) {
+        const uint64_t key = (uint64_t)kk * nb1 + jjx[x];
+        const double mm = exact_merit(a, w, w, rho, k.c, bad);
+        if (mm > eb[x] || (mm == eb[x] && key < ek[x])) {
+          eb[x] = mm;
+          ek[x] = key;
+          ed[x] = di;
+        }
+      }
+      if (!((gmask >> di) & 1u)) continue;
+      for (uint32_t si = 1; si < ns; si++) {
+        const uint32_t* tab = k.stab + (di * (ns - 1) + (si - 1)) * (layers - 1);
+        double sa = a, shp = w, snorm = w;
+        if (k.ingress[si] != 0.0)
+          shape_sums(cm, jjx[x], a, w, layers, tab, &sa, &shp, &snorm);
+        if (usable && sa > 0.0) {
+          const uint64_t key = ((uint64_t)si * nb1 + kk) * nb1 + jjx[x];
+          const double mm = exact_merit(sa, shp, snorm, rho, k.c, bad);
+          if (mm > eb[x] || (mm == eb[x] && key < ek[x])) {
+            eb[x] = mm;
+            ek[x] = key;
+            ed[x] = di;
+          }
+        }
+      }
+    }
+  }
+  const int bx = (eb[1] > eb[0] || (eb[1] == eb[0] && ek[1] < ek[0])) ? 1 : 0;
+  double bv = eb[bx];
+  uint64_t bk = ek[bx];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    double ov = __shfl_xor(bv, off, 64);
+    uint64_t okk = __shfl_xor(bk, off, 64);
+    if (ov > bv || (ov == bv && okk < bk)) {
+      bv = ov;
+      bk = okk;
+    }
+  }
+  const uint64_t wmask = __ballot((ek[0] == bk && eb[0] == bv) || (ek[1] == bk && eb[1] == bv));
+  const int wl = __builtin_ctzll(wmask);
+  const int wx = __shfl((ek[1] == bk && eb[1] == bv) ? 1 : 0, wl, 64);
+  out.value = bv;
+  out.key = bk;
+  out.di = __shfl(wx ? ed[1] : ed[0], wl, 64);
+  // the winner's sums, recomputed once (wave-uniform) from its key
+  const uint32_t wj = (uint32_t)(bk % nb1);
+  const uint32_t wk = (uint32_t)((bk / nb1) % nb1);
+  const uint32_t wsi = (uint32_t)(bk / ((uint64_t)nb1 * nb1));
+  const double2 hi = cm.vw_at(wj + wk);
+  const double2 lo = cm.vw_at(wj);
+  double a = hi.x - lo.x, w = hi.y - lo.y, hp = w, norm = w;
+  if (wsi > 0 && k.ingress[wsi] != 0.0)
+    shape_sums(cm, wj, hi.x - lo.x, w, layers,
+               k.stab + (out.di * (ns - 1) + (wsi - 1)) * (layers - 1), &a, &hp,
+               &norm);
+  out.a = a;
+  out.hp = hp;
+  out.norm = norm;
+  if (lane == 0) atomicAdd(rescans, 1u);
+  return out;
+}
+
+// Per-period duration tables: k and its usability flag, and the staircase
+// layer widths/offsets (scan.rs:157-171, kernels.rs:104-106,679-681).
+__device__ void duration_tables(const Segment& S, const LaunchArgs& A,
+                                uint32_t p, uint32_t* ktab, uint32_t* stab,
+                                uint32_t tid, uint32_t block) {
+  const uint32_t nb = S.nb, nd = S.nd, ns = A.ns, layers = A.layers;
+  for (uint32_t di = tid; di < nd; di += block) {
+    double scale = S.scales[p];
+    double rk = __builtin_floor(S.dur[di] * scale + 0.5);
+    uint32_t raw_k = rk < 1.0 ? 1u : (rk >= 2147483648.0 ? 0x7fffffffu : (uint32_t)rk);
+    uint32_t k = raw_k < nb / 2 ? raw_k : nb / 2;
+    uint32_t kok = raw_k < nb / 2 ? 1u : 0u;
+    ktab[di] = k | (kok << 31);
+    for (uint32_t si = 1; si < ns; si++) {
+      for (uint32_t l = 1; l < layers; l++) {
+        double factor = A.factors[si * layers + l];
+        double wf = __builtin_floor((double)k * factor + 0.5);
+        uint32_t width = wf < 1.0 ? 1u : (uint32_t)wf;
+        uint32_t off = (k - width) / 2;
+        stab[(di * (ns - 1) + (si - 1)) * (layers - 1) + (l - 1)] = width | (off << 16);
+      }
     }
   }
 }
@@ -364,8 +692,48 @@ __device__ __forceinline__ uint32_t find_segment(const LaunchArgs& A,
   return lo;
 }
 
+// Record of the period's winner (kernels.rs:800-843, 1144-1154).
+__device__ void write_record(const Segment& S, uint32_t f, uint64_t key,
+                             uint32_t di, bool valid, uint32_t ns,
+                             const double* ingress, double a0, double hp,
+                             double norm) {
+  const uint32_t nb1 = S.nb + 1;
+  const uint32_t phase = (uint32_t)(key % nb1);
+  const uint32_t width = (uint32_t)((key / nb1) % nb1);
+  const uint32_t si = (uint32_t)(key / ((uint64_t)nb1 * nb1));
+  const uint32_t sic = si < ns - 1 ? si : ns - 1;
+  const double r = S.rho[di];
+  const double a = a0 / r;
+  const double d = hp * hp / S.C;
+  const double q = __builtin_fmax(norm - d, 1e-300);
+  const double value = valid ? a / __builtin_sqrt(q) : -kInf;
+  const double depth = a / q * r;
+  double* rec = S.records + (size_t)f * 9;
+  rec[0] = S.periods[f];
+  rec[1] = S.scales[f];
+  rec[2] = (double)phase;
+  rec[3] = (double)width;
+  rec[4] = a;
+  rec[5] = norm;
+  rec[6] = d;
+  rec[7] = ingress[sic];
+  rec[8] = r;
+  S.score[f] = value;
+  S.depth[f] = depth;
+  if (S.winner) {
+    uint32_t* wn = S.winner + (size_t)f * 4;
+    wn[0] = phase;
+    wn[1] = width;
+    wn[2] = si;
+    wn[3] = di;
+  }
+}
+
+// ------------------------------------------------- LDS-resident kernel
+
 template <uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK) void fused_scan(LaunchArgs A) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void fused_scan(LaunchArgs A) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   constexpr uint32_t kWaves = BLOCK / 64;
   const uint32_t tid = threadIdx.x;
@@ -378,15 +746,14 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(LaunchArgs A) {
   const uint32_t gcount = min(S.G, S.np - p0);
   const uint32_t nd = S.nd, ns = A.ns, layers = A.layers;
   const uint32_t tiles = (nb + kTile - 1) / kTile;
-  const uint32_t nsl = (ns - 1) * (layers - 1);  // table entries per duration
+  const uint32_t nsl = (ns - 1) * (layers - 1);
 
-  // ---- LDS carve-up ---------------------------------------------------
   unsigned char* p = lds;
   unsigned char* hist = p;
   p += (size_t)S.G * S.pstride;
-  uint32_t* ktab = (uint32_t*)p;  // [G][nd]: k | kok << 31
+  uint32_t* ktab = (uint32_t*)p;  // [G][nd]
   p += S.G * nd * 4;
-  uint32_t* stab = (uint32_t*)p;  // [G][nd][nsl]: width | offset << 16
+  uint32_t* stab = (uint32_t*)p;  // [G][nd][nsl]
   p += S.G * nd * nsl * 4;
   p = (unsigned char*)(((uintptr_t)p + 7) & ~(uintptr_t)7);
   double* tval = (double*)p;  // [G][tiles]
@@ -395,43 +762,27 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(LaunchArgs A) {
   p += S.G * tiles * 8;
   uint32_t* tdi = (uint32_t*)p;
 
-  // ---- phase 0: clear histograms, duration tables ----------------------
+  // ---- phase 0: clear histograms, duration tables
   {
     uint4* h4 = (uint4*)hist;
     const uint32_t words = (uint32_t)((size_t)gcount * S.pstride / 16);
     for (uint32_t i = tid; i < words; i += BLOCK) h4[i] = make_uint4(0, 0, 0, 0);
-    for (uint32_t i = tid; i < gcount * nd; i += BLOCK) {
-      uint32_t g = i / nd, di = i % nd;
-      double scale = S.scales[p0 + g];
-      double rk = __builtin_floor(S.dur[di] * scale + 0.5);
-      uint32_t raw_k = rk < 1.0 ? 1u : (rk >= 2147483648.0 ? 0x7fffffffu : (uint32_t)rk);
-      uint32_t k = raw_k < nb / 2 ? raw_k : nb / 2;
-      uint32_t kok = raw_k < nb / 2 ? 1u : 0u;
-      ktab[i] = k | (kok << 31);
-      for (uint32_t si = 1; si < ns; si++) {
-        for (uint32_t l = 1; l < layers; l++) {
-          double factor = A.factors[si * layers + l];
-          double wf = __builtin_floor((double)k * factor + 0.5);
-          uint32_t width = wf < 1.0 ? 1u : (uint32_t)wf;
-          uint32_t off = (k - width) / 2;
-          stab[(i * (ns - 1) + (si - 1)) * (layers - 1) + (l - 1)] = width | (off << 16);
-        }
-      }
-    }
+    for (uint32_t g = wave; g < gcount; g += kWaves)
+      duration_tables(S, A, p0 + g, ktab + g * nd, stab + g * nd * nsl, lane, 64);
   }
   __syncthreads();
 
-  // ---- phase 1: fold every observation into the G histograms ----------
+  // ---- phase 1: fold every observation into the G histograms
   if (S.wide_cells) {
-    if (A.run_fold) fold_all<true, true>(S, hist, p0, gcount, wave, lane, kWaves);
-    else fold_all<true, false>(S, hist, p0, gcount, wave, lane, kWaves);
+    if (!A.run_fold) fold_all<true, false, false>(S, hist, p0, gcount, wave, lane, kWaves);
+    else fold_all<true, true, false>(S, hist, p0, gcount, wave, lane, kWaves);
   } else {
-    if (A.run_fold) fold_all<false, true>(S, hist, p0, gcount, wave, lane, kWaves);
-    else fold_all<false, false>(S, hist, p0, gcount, wave, lane, kWaves);
+    if (!A.run_fold) fold_all<false, false, false>(S, hist, p0, gcount, wave, lane, kWaves);
+    else fold_all<false, true, false>(S, hist, p0, gcount, wave, lane, kWaves);
   }
   __syncthreads();
 
-  // ---- phase 2: exclusive prefix sums in place (exact) -----------------
+  // ---- phase 2: exclusive prefix sums in place (exact)
   for (uint32_t item = wave; item < 2 * gcount; item += kWaves) {
     uint32_t g = item >> 1;
     double2* vw = (double2*)(hist + (size_t)g * S.pstride);
@@ -442,228 +793,64 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(LaunchArgs A) {
   }
   __syncthreads();
 
-  // ---- phase 3: template scan, one 128-bin tile per wave at a time -----
-  const double c = S.C, inv_c = S.invC;
-  const uint32_t nobs = S.n;
-  const double gate = A.gate;
-  const double g2 = gate * gate;
+  // ---- phase 3: template scan, one 128-bin tile per wave at a time
+  TileCtx k;
+  k.rho = S.rho;
+  k.ingress = A.ingress;
+  k.c = S.C;
+  k.inv_c = S.invC;
+  k.gate = A.gate;
+  k.g2 = A.gate * A.gate;
+  k.nb = nb;
+  k.nd = nd;
+  k.ns = ns;
+  k.layers = layers;
+  k.nobs = S.n;
+  k.lane = lane;
+  unsigned int* rescans = A.counters + 2 * S.index + 1;
   for (uint32_t item = wave; item < gcount * tiles; item += kWaves) {
     const uint32_t g = item / tiles, tile = item % tiles;
-    Cum cm;
+    WrapCum cm;
     cm.vw = (const double2*)(hist + (size_t)g * S.pstride);
     cm.n = (const uint32_t*)(cm.vw + nb1);
     cm.nb = nb;
-    {
-      double2 t = cm.vw[nb];
-      cm.tv = t.x;
-      cm.tw = t.y;
-      cm.tn = cm.n[nb];
-    }
-    const uint32_t* kt = ktab + g * nd;
-    const uint32_t* st_tab = stab + (size_t)g * nd * nsl;
-
-    uint32_t jx[2], jjx[2];
-    double ev[2], ew[2];
-    uint32_t en[2];
-    LaneState st[2];
-#pragma unroll
-    for (int x = 0; x < 2; x++) {
-      jx[x] = tile * kTile + lane + 64 * x;
-      jjx[x] = jx[x] < nb ? jx[x] : nb - 1;
-      double2 e = cm.vw[jjx[x]];
-      ev[x] = e.x;
-      ew[x] = e.y;
-      en[x] = cm.n[jjx[x]];
-      state_init(st[x]);
-    }
+    const double2 t = cm.vw[nb];
+    cm.tv = t.x;
+    cm.tw = t.y;
+    cm.tn = cm.n[nb];
+    k.kt = ktab + g * nd;
+    k.stab = stab + (size_t)g * nd * nsl;
     bool bad = false;
-    uint32_t gmask = 0;
-    for (uint32_t di = 0; di < nd; di++) {
-      const uint32_t kte = kt[di];
-      const uint32_t k = kte & 0x7fffffffu;
-      const bool kok = (kte >> 31) != 0;
-      const double rho = S.rho[di];
-      const double rho2 = rho * rho;
-      double bv[2], bw[2];
-      bool busable[2];
-      bool pass = false;
-#pragma unroll
-      for (int x = 0; x < 2; x++) {
-        const uint32_t e = jjx[x] + k;
-        const uint32_t cnt = cum_n(cm, e) - en[x];
-        const double2 hi = cum_vw(cm, e);
-        const double w = hi.y - ew[x];
-        const double a = hi.x - ev[x];
-        const bool usable = jx[x] < nb && kok && cnt >= 3 && cnt + 3 <= nobs;
-        const bool valid = usable && a > 0.0;
-        bv[x] = a;
-        bw[x] = w;
-        busable[x] = usable;
-        if (valid) {
-          const uint64_t key = (uint64_t)k * nb1 + jjx[x];
-          const double s = surrogate(a, w, w, rho, rho2, c, inv_c, &bad);
-          state_update(st[x], s, key, di, a, w, w, rho);
-          if (s > -kInf) {
-            if (gate <= 0.0 || s >= g2 * (1.0 + kNear)) {
-              pass = true;
-            } else if (s >= g2 * (1.0 - kNear)) {
-              pass = pass || exact_merit(a, w, w, rho, c, &bad) >= gate;
-            }
-          }
-        }
-      }
-      if (!__any(pass)) continue;
-      gmask |= 1u << di;
-      for (uint32_t si = 1; si < ns; si++) {
-        const uint32_t* tab = st_tab + (di * (ns - 1) + (si - 1)) * (layers - 1);
-        const bool staircase = A.ingress[si] != 0.0;
-#pragma unroll
-        for (int x = 0; x < 2; x++) {
-          double a = bv[x], hp = bw[x], norm = bw[x];
-          if (staircase)
-            shape_sums(cm, jjx[x], bv[x], bw[x], layers, tab, &a, &hp, &norm);
-          if (busable[x] && a > 0.0) {
-            const uint64_t key = ((uint64_t)si * nb1 + k) * nb1 + jjx[x];
-            const double s = surrogate(a, hp, norm, rho, rho2, c, inv_c, &bad);
-            state_update(st[x], s, key, di, a, hp, norm, rho);
-          }
-        }
-      }
-    }
-
-    // ---- tile winner: surrogate top, exact value, near-tie rescan -------
-    const double m = wave_max(__builtin_fmax(st[0].s, st[1].s));
-    double tile_value = -kInf;
-    uint64_t tile_key = ~0ull;
-    uint32_t tile_di = 0;
-    if (m > -kInf) {
-      const double thr = m * (1.0 - kNear);
-      bool near[2];
-      uint64_t kk = ~0ull;
-#pragma unroll
-      for (int x = 0; x < 2; x++) {
-        near[x] = st[x].s >= thr;
-        if (st[x].s == m && st[x].key < kk) kk = st[x].key;
-      }
-      const uint64_t top_key = wave_min_u64(kk);
-      const uint64_t owner_mask =
-          __ballot(st[0].key == top_key || st[1].key == top_key);
-      const int owner = __builtin_ctzll(owner_mask);
-      const int ox = __shfl((st[1].key == top_key) ? 1 : 0, owner, 64);
-      const double ta = __shfl(ox ? st[1].a : st[0].a, owner, 64);
-      const double thp = __shfl(ox ? st[1].hp : st[0].hp, owner, 64);
-      const double tno = __shfl(ox ? st[1].norm : st[0].norm, owner, 64);
-      const double trh = __shfl(ox ? st[1].rho : st[0].rho, owner, 64);
-      const uint32_t tdi0 = __shfl(ox ? st[1].di : st[0].di, owner, 64);
-      bool need = false;
-#pragma unroll
-      for (int x = 0; x < 2; x++) {
-        if (near[x]) {
-          const bool same = st[x].a == ta && st[x].hp == thp &&
-                            st[x].norm == tno && st[x].rho == trh;
-          need = need || !same || st[x].second >= st[x].s * (1.0 - kNear);
-        }
-      }
-      if (!__any(need)) {
-        tile_key = top_key;
-        tile_di = tdi0;
-        tile_value = exact_merit(ta, thp, tno, trh, c, &bad);
-      } else {
-        // exact rescan of the near states (kernels.rs:662-798 retry pass)
-        double ebest[2] = {-kInf, -kInf};
-        uint64_t ekey[2] = {~0ull, ~0ull};
-        uint32_t edi[2] = {0, 0};
-#pragma unroll
-        for (int x = 0; x < 2; x++) {
-          if (!near[x]) continue;
-          for (uint32_t di = 0; di < nd; di++) {
-            const uint32_t kte = kt[di];
-            const uint32_t k = kte & 0x7fffffffu;
-            const bool kok = (kte >> 31) != 0;
-            const double rho = S.rho[di];
-            const uint32_t e = jjx[x] + k;
-            const uint32_t cnt = cum_n(cm, e) - en[x];
-            const double2 hi = cum_vw(cm, e);
-            const double w = hi.y - ew[x];
-            const double a = hi.x - ev[x];
-            const bool usable = jx[x] < nb && kok && cnt >= 3 && cnt + 3 <= nobs;
-            if (usable && a > 0.0) {
-              const uint64_t key = (uint64_t)k * nb1 + jjx[x];
-              const double mm = exact_merit(a, w, w, rho, c, &bad);
-              if (mm > ebest[x] || (mm == ebest[x] && key < ekey[x])) {
-                ebest[x] = mm;
-                ekey[x] = key;
-                edi[x] = di;
-              }
-            }
-            if (!((gmask >> di) & 1u)) continue;
-            for (uint32_t si = 1; si < ns; si++) {
-              const uint32_t* tab = st_tab + (di * (ns - 1) + (si - 1)) * (layers - 1);
-              double sa = a, shp = w, snorm = w;
-              if (A.ingress[si] != 0.0)
-                shape_sums(cm, jjx[x], a, w, layers, tab, &sa, &shp, &snorm);
-              if (usable && sa > 0.0) {
-                const uint64_t key = ((uint64_t)si * nb1 + k) * nb1 + jjx[x];
-                const double mm = exact_merit(sa, shp, snorm, rho, c, &bad);
-                if (mm > ebest[x] || (mm == ebest[x] && key < ekey[x])) {
-                  ebest[x] = mm;
-                  ekey[x] = key;
-                  edi[x] = di;
-                }
-              }
-            }
-          }
-        }
-        // wave argmax by (value desc, key asc)
-        int bx = (ebest[1] > ebest[0] || (ebest[1] == ebest[0] && ekey[1] < ekey[0])) ? 1 : 0;
-        double bvv = ebest[bx];
-        uint64_t bkk = ekey[bx];
-        uint32_t bdd = edi[bx];
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          double ov = __shfl_xor(bvv, off, 64);
-          uint64_t ok = __shfl_xor(bkk, off, 64);
-          uint32_t od = __shfl_xor(bdd, off, 64);
-          if (ov > bvv || (ov == bvv && ok < bkk)) {
-            bvv = ov;
-            bkk = ok;
-            bdd = od;
-          }
-        }
-        tile_value = bvv;
-        tile_key = bkk;
-        tile_di = bdd;
-        if (lane == 0) atomicAdd(A.counters + 2 * S.index + 1, 1u);
-      }
-    }
+    const TileOut o = scan_tile(cm, k, tile, &bad, rescans);
     if (__any(bad) && lane == 0) atomicOr(A.counters + 2 * S.index, 1u);
     if (lane == 0) {
-      tval[g * tiles + tile] = tile_value;
-      tkey[g * tiles + tile] = tile_key;
-      tdi[g * tiles + tile] = tile_di;
+      tval[g * tiles + tile] = o.value;
+      tkey[g * tiles + tile] = o.key;
+      tdi[g * tiles + tile] = o.di;
     }
   }
   __syncthreads();
 
-  // ---- phase 4: merge tiles (kernels.rs:1113-1155), exact record -------
+  // ---- phase 4: merge tiles (kernels.rs:1113-1155), exact record
   for (uint32_t g = tid; g < gcount; g += BLOCK) {
     double best = -kInf;
     uint64_t key = ~0ull;
     uint32_t bdi = 0;
     for (uint32_t t = 0; t < tiles; t++) {
       double v = tval[g * tiles + t];
-      uint64_t k = tkey[g * tiles + t];
-      if (v > -kInf && (v > best || (v == best && k < key))) {
+      uint64_t kk = tkey[g * tiles + t];
+      if (v > -kInf && (v > best || (v == best && kk < key))) {
         best = v;
-        key = k;
+        key = kk;
         bdi = tdi[g * tiles + t];
       }
     }
-    if (!(best > -kInf)) {
+    const bool valid = best > -kInf;
+    if (!valid) {
       key = nb1;  // no usable template: canonical (phase 0, width 1, box)
       bdi = 0;
     }
-    Cum cm;
+    WrapCum cm;
     cm.vw = (const double2*)(hist + (size_t)g * S.pstride);
     cm.n = (const uint32_t*)(cm.vw + nb1);
     cm.nb = nb;
@@ -674,40 +861,154 @@ __global__ __launch_bounds__(BLOCK) void fused_scan(LaunchArgs A) {
     const uint32_t width = (uint32_t)((key / nb1) % nb1);
     const uint32_t si = (uint32_t)(key / ((uint64_t)nb1 * nb1));
     const uint32_t sic = si < ns - 1 ? si : ns - 1;
-    const double ing = A.ingress[sic];
-    const double r = S.rho[bdi];
-    const double2 hi = cum_vw(cm, phase + width);
-    const double2 lo = cum_vw(cm, phase);
-    const double w0 = hi.y - lo.y;
-    const double a00 = hi.x - lo.x;
+    const double2 hi = cm.vw_at(phase + width);
+    const double2 lo = cm.vw_at(phase);
     double a0, hp, norm;
-    shape_sums_direct(cm, phase, width, a00, w0, ing, A.factors, sic, layers,
-                      &a0, &hp, &norm);
-    const double a = a0 / r;
-    const double d = hp * hp / c;
-    const double q = __builtin_fmax(norm - d, 1e-300);
-    const double value = best > -kInf ? a / __builtin_sqrt(q) : -kInf;
-    const double depth = a / q * r;
-    const uint32_t f = p0 + g;
-    double* rec = S.records + (size_t)f * 9;
-    rec[0] = S.periods[f];
-    rec[1] = S.scales[f];
-    rec[2] = (double)phase;
-    rec[3] = (double)width;
-    rec[4] = a;
-    rec[5] = norm;
-    rec[6] = d;
-    rec[7] = ing;
-    rec[8] = r;
-    S.score[f] = value;
-    S.depth[f] = depth;
-    if (S.winner) {
-      uint32_t* wn = S.winner + (size_t)f * 4;
-      wn[0] = phase;
-      wn[1] = width;
-      wn[2] = si;
-      wn[3] = bdi;
+    shape_sums_direct(cm, phase, width, hi.x - lo.x, hi.y - lo.y,
+                      A.ingress[sic], A.factors, sic, layers, &a0, &hp, &norm);
+    write_record(S, p0 + g, key, bdi, valid, ns, A.ingress, a0, hp, norm);
+  }
+}
+
+// ----------------------------------------------------- range-split kernel
+
+// One period per workgroup; the bins are processed in ranges of
+// S.range_bins with a halo of S.halo_bins (>= the longest template).
+template <uint32_t BLOCK>
+__global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  constexpr uint32_t kWaves = BLOCK / 64;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63;
+  const uint32_t wave = tid >> 6;
+  const Segment S = A.segs[find_segment(A, blockIdx.x)];
+  const uint32_t nb = S.nb;
+  const uint32_t nb1 = nb + 1;
+  const uint32_t f = blockIdx.x - S.group0;  // G = 1
+  const uint32_t nd = S.nd, ns = A.ns, layers = A.layers;
+  const uint32_t tiles = (nb + kTile - 1) / kTile;
+  const uint32_t nsl = (ns - 1) * (layers - 1);
+  const uint32_t B = S.range_bins, H = S.halo_bins;
+  const uint32_t slots = B + H + 1;
+
+  unsigned char* p = lds;
+  double2* wvw = (double2*)p;  // [slots]
+  p += (size_t)slots * 16;
+  uint32_t* wn = (uint32_t*)p;  // [slots]
+  p += ((size_t)slots * 4 + 15) & ~(size_t)15;
+  uint32_t* ktab = (uint32_t*)p;
+  p += nd * 4;
+  uint32_t* stab = (uint32_t*)p;
+  p += nd * nsl * 4;
+  p = (unsigned char*)(((uintptr_t)p + 7) & ~(uintptr_t)7);
+  double* tval = (double*)p;  // [tiles] x {value, a, hp, norm}
+  p += tiles * 32;
+  uint64_t* tkey = (uint64_t*)p;
+  p += tiles * 8;
+  uint32_t* tdi = (uint32_t*)p;
+  p += tiles * 4;
+  p = (unsigned char*)(((uintptr_t)p + 7) & ~(uintptr_t)7);
+  double* carry = (double*)p;  // {bv, bw, bin0 v, bin0 w}, bn as double
+
+  if (wave == 0) duration_tables(S, A, f, ktab, stab, lane, 64);
+  if (tid == 0) {
+    carry[0] = 0.0;
+    carry[1] = 0.0;
+    carry[4] = 0.0;
+  }
+  TileCtx k;
+  k.kt = ktab;
+  k.stab = stab;
+  k.rho = S.rho;
+  k.ingress = A.ingress;
+  k.c = S.C;
+  k.inv_c = S.invC;
+  k.gate = A.gate;
+  k.g2 = A.gate * A.gate;
+  k.nb = nb;
+  k.nd = nd;
+  k.ns = ns;
+  k.layers = layers;
+  k.nobs = S.n;
+  k.lane = lane;
+  unsigned int* rescans = A.counters + 2 * S.index + 1;
+  const double scale = S.scales[f];
+  for (uint32_t x0 = 0; x0 < nb; x0 += B) {
+    const uint32_t W = min(B, nb - x0);
+    const uint32_t wlen = W + H;
+    {
+      uint4* z = (uint4*)wvw;
+      for (uint32_t i = tid; i < slots; i += BLOCK) z[i] = make_uint4(0, 0, 0, 0);
+      for (uint32_t i = tid; i < slots; i += BLOCK) wn[i] = 0;
     }
+    __syncthreads();
+    if (S.wide_cells)
+      fold_window<true>(S, (double*)wvw, wn, scale, x0, wlen, wave, lane, kWaves);
+    else
+      fold_window<false>(S, (double*)wvw, wn, scale, x0, wlen, wave, lane, kWaves);
+    __syncthreads();
+    if (wave == 0) wave_prefix_vw(wvw, wlen, lane, S.sv, S.sw);
+    if (wave == 1 % kWaves) wave_prefix_n(wn, wlen, lane);
+    __syncthreads();
+    WindowCum cm;
+    cm.vw = wvw;
+    cm.n = wn;
+    cm.bv = carry[0];
+    cm.bw = carry[1];
+    cm.bn = (uint32_t)carry[4];
+    cm.x0 = x0;
+    if (x0 == 0 && tid == 0) {
+      carry[2] = wvw[1].x;
+      carry[3] = wvw[1].y;
+    }
+    const uint32_t t0 = x0 / kTile, t1 = (x0 + W + kTile - 1) / kTile;
+    for (uint32_t tile = t0 + wave; tile < t1; tile += kWaves) {
+      bool bad = false;
+      const TileOut o = scan_tile(cm, k, tile, &bad, rescans);
+      if (__any(bad) && lane == 0) atomicOr(A.counters + 2 * S.index, 1u);
+      if (lane == 0) {
+        tval[4 * tile] = o.value;
+        tval[4 * tile + 1] = o.a;
+        tval[4 * tile + 2] = o.hp;
+        tval[4 * tile + 3] = o.norm;
+        tkey[tile] = o.key;
+        tdi[tile] = o.di;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      carry[0] = cm.bv + wvw[W].x;
+      carry[1] = cm.bw + wvw[W].y;
+      carry[4] = (double)(cm.bn + wn[W]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double best = -kInf, a0 = 0.0, hp = 0.0, norm = 0.0;
+    uint64_t key = ~0ull;
+    uint32_t bdi = 0;
+    for (uint32_t t = 0; t < tiles; t++) {
+      double v = tval[4 * t];
+      uint64_t kk = tkey[t];
+      if (v > -kInf && (v > best || (v == best && kk < key))) {
+        best = v;
+        key = kk;
+        bdi = tdi[t];
+        a0 = tval[4 * t + 1];
+        hp = tval[4 * t + 2];
+        norm = tval[4 * t + 3];
+      }
+    }
+    const bool valid = best > -kInf;
+    if (!valid) {
+      // canonical (phase 0, width 1, box): the sums of bin 0
+      key = nb1;
+      bdi = 0;
+      a0 = carry[2];
+      hp = carry[3];
+      norm = carry[3];
+    }
+    write_record(S, f, key, bdi, valid, ns, A.ingress, a0, hp, norm);
   }
 }
 
@@ -727,23 +1028,45 @@ size_t scan_lds_bytes(uint32_t nb, uint32_t G, uint32_t nd, uint32_t ns,
   return bytes;
 }
 
+size_t split_lds_bytes(uint32_t nb, uint32_t range_bins, uint32_t halo_bins,
+                       uint32_t nd, uint32_t ns, uint32_t layers) {
+  const size_t slots = (size_t)range_bins + halo_bins + 1;
+  const uint32_t tiles = (nb + kTile - 1) / kTile;
+  size_t bytes = slots * 16 + ((slots * 4 + 15) & ~(size_t)15);
+  bytes += (size_t)nd * 4 + (size_t)nd * (ns - 1) * (layers - 1) * 4;
+  bytes = (bytes + 7) & ~(size_t)7;
+  bytes += (size_t)tiles * (32 + 8 + 4);
+  bytes = (bytes + 7) & ~(size_t)7;
+  bytes += 5 * 8;
+  return bytes;
+}
+
 hipError_t scan_configure(size_t max_lds) {
-  hipError_t e = hipFuncSetAttribute((const void*)fused_scan<256>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)max_lds);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)fused_scan<512>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)max_lds);
+  const void* fns[] = {(const void*)fused_scan<256>, (const void*)fused_scan<512>,
+                       (const void*)fused_scan_split<256>,
+                       (const void*)fused_scan_split<512>};
+  for (const void* fn : fns) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)max_lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t scan_launch(const LaunchArgs& args, uint32_t groups, uint32_t block,
-                       size_t lds, hipStream_t stream) {
+                       size_t lds, bool split, hipStream_t stream) {
   if (groups == 0) return hipSuccess;
-  if (block == 512)
-    hipLaunchKernelGGL(fused_scan<512>, dim3(groups), dim3(512), lds, stream, args);
-  else
-    hipLaunchKernelGGL(fused_scan<256>, dim3(groups), dim3(256), lds, stream, args);
+  if (split) {
+    if (block == 512)
+      hipLaunchKernelGGL(fused_scan_split<512>, dim3(groups), dim3(512), lds, stream, args);
+    else
+      hipLaunchKernelGGL(fused_scan_split<256>, dim3(groups), dim3(256), lds, stream, args);
+  } else {
+    if (block == 512)
+      hipLaunchKernelGGL(fused_scan<512>, dim3(groups), dim3(512), lds, stream, args);
+    else
+      hipLaunchKernelGGL(fused_scan<256>, dim3(groups), dim3(256), lds, stream, args);
+  }
   return hipGetLastError();
 }
 

@@ -124,3 +124,69 @@ def test_launch_shape_does_not_change_results(gpu, launch):
         gpu.set_launch(0, 0, 0)
     for a, b in zip(ref[:4], got[:4]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("nb", [9000, 12000, 19200])
+def test_range_split_kernel_bit_exact(gpu, nb):
+    """Phase-bin counts whose histogram exceeds the LDS run on the range-split
+    kernel; results stay bit-identical to the oracle."""
+    cfg = F.SearchConfig(period_min=0.5, period_max=100.0)
+    s, dev, vq, wq, d = _setup(gpu, cfg, seed=7, sectors=2, period=23.0,
+                               depth=3e-3, duration=6.0 / 24)
+    rng = np.random.default_rng(nb)
+    lo = 4 * d[-1] * nb / (nb // 2)  # keep k = D nb / P well below nb / 2
+    periods = np.sort(rng.uniform(max(lo, 20.0), 54.0, 120))
+    periods = np.concatenate([periods, [23.0, 46.0]])
+    _compare(gpu, dev, s, vq, wq, d, periods, nb, cfg, f"split nb{nb}")
+
+
+def test_signal_search_long_baseline(gpu):
+    """A two-year baseline with a one-year gap at periods up to 100 d: the
+    top bands use the range-split kernel (nb up to 19200)."""
+    cfg = F.SearchConfig(period_min=80.0, period_max=100.0)
+    t, f, e = F.synth_star(seed=11, sectors=2, sigma=1e-3, period=87.0,
+                           depth=2e-3, duration=5.0 / 24,
+                           baseline_gap_days=365.0)
+    d = cfg.durations_days()
+    s = F.family_state(t, f - 1.0, e, d, cfg.correlated_noise)
+    dev = F.DeviceReady.upload(gpu, s)
+    vq, wq, _, _ = dev.fields()
+    print(f"[parity] long baseline n={len(s.t)} T={s.t[-1]:.0f}", flush=True)
+    rows, grid, bands = F.signal_search(gpu, dev, d, s.rho, cfg)
+    orows, obands, og, ost = O.signal_search(
+        s.t, vq, wq, s.baseline_norm, s.origin, d, s.rho,
+        cfg.ingress_fractions, cfg, cfg.period_min, cfg.period_max,
+        cfg.period_oversample)
+    assert ost == 0 and len(grid) == og
+    assert max(b["phase_bins"] for b in bands) > 8000
+    for b, ob in zip(bands, obands):
+        for k in ("periods", "prescan_survivors", "phase_bins",
+                  "templates_scanned"):
+            assert b[k] == ob[k], (k, b, ob)
+    np.testing.assert_array_equal(rows, orows)
+
+
+def test_nonuniform_weights_and_unsorted_times(gpu):
+    """Per-point errors (the general three-field fold) and a shuffled upload
+    (the direct phase-bin path) give the oracle's results bit for bit."""
+    cfg = F.SearchConfig(period_min=0.5, period_max=13.7)
+    t, f, e = F.synth_star(seed=21, sectors=1, sigma=1e-3, period=2.9,
+                           depth=3e-3, duration=2.0 / 24)
+    rng = np.random.default_rng(21)
+    e = e * rng.uniform(0.7, 1.4, len(e))
+    d = cfg.durations_days()
+    s = F.family_state(t, f - 1.0, e, d, cfg.correlated_noise)
+    dev = F.DeviceReady.upload(gpu, s)
+    vq, wq, _, _ = dev.fields()
+    assert len(np.unique(wq)) > 1
+    periods = np.linspace(0.6, 13.0, 500)
+    for nb in (192, 1536):
+        _compare(gpu, dev, s, vq, wq, d, periods, nb, cfg, f"nonuniform nb{nb}")
+    perm = rng.permutation(len(s.t))
+    shuffled = F.DeviceReady(gpu, s.t[perm], s.v[perm], s.w[perm],
+                             s.baseline_norm, s.origin)
+    bank = F.make_bank(cfg, d, s.rho)
+    a = F.scan_bank(gpu, dev, periods, bank, 768)
+    b = F.scan_bank(gpu, shuffled, periods, bank, 768)
+    for x, y in zip(a[:4], b[:4]):
+        np.testing.assert_array_equal(x, y)
