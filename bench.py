@@ -65,6 +65,27 @@ def calibrate_copy(mib=256, reps=4):
     print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps}))
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_four_sector_star.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the scan kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (already scaled by the
+    copy-kernel calibration).  The counters need their own profiler runs, so
+    the figure is read from the summary those runs produced."""
+    import re
+
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    if not m:
+        return None
+    return {"bytes_per_launch": float(m.group(1)),
+            "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
+
+
 def run_gpu(cfg, d, state, steps, warmup, launch=(0, 0, 0)):
     import fbls_amd as F
 
@@ -74,12 +95,14 @@ def run_gpu(cfg, d, state, steps, warmup, launch=(0, 0, 0)):
     for _ in range(warmup):
         F.signal_search(gpu, dev, d, state.rho, cfg)
     per_step = []
+    gpu.launch_totals(reset=True)
     t0 = time.perf_counter()
     for _ in range(steps):
         rows, grid, bands = F.signal_search(gpu, dev, d, state.rho, cfg)
         per_step.append(bands)
     elapsed = time.perf_counter() - t0
-    return gpu, dev, rows, grid, per_step, elapsed
+    totals = gpu.launch_totals(reset=True)
+    return gpu, dev, rows, grid, per_step, elapsed, totals
 
 
 def cpu_baseline(cfg, d, state):
@@ -124,16 +147,19 @@ def main():
 
     cfg, d, state = workload()
     n = len(state.t)
-    gpu, dev, rows, grid, per_step, elapsed = run_gpu(
+    gpu, dev, rows, grid, per_step, elapsed, tot = run_gpu(
         cfg, d, state, args.steps, args.warmup,
         (args.lds, args.block, args.fold_mode))
     bands = per_step[-1]
     templates = sum(b["templates_scanned"] for b in bands)
     launch_periods = sum(b["periods"] + b["prescan_survivors"] for b in bands)
-    dev_s = np.mean([sum(b["seconds_device"] for b in s) for s in per_step])
-    launches = sum(2 if b["prescan_survivors"] else 1 for b in bands)
-    bytes_alg = launch_periods * (16.0 * n + 88.0)
-    achieved = bytes_alg / dev_s / 1e9
+    # per launch of the scan kernel, from the library's HIP events on its
+    # own stream over the timed steps
+    launches = tot["launches"] / args.steps
+    launch_s = tot["seconds"] / tot["launches"]
+    bytes_launch = (16.0 * tot["fold_pairs"] + 88.0 * tot["periods"]) / tot["launches"]
+    achieved = bytes_launch / launch_s / 1e9
+    traffic = measured_traffic()
     ms = elapsed / args.steps * 1e3
     value = templates / (elapsed / args.steps)
     best = rows[np.argmax(rows[:, 3])] if len(rows) else None
@@ -169,11 +195,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "fused_scan",
+            "traffic": None if traffic is None else traffic["bytes_per_launch"],
+            "kernel": "fused_scan<256>",
             "launches_per_step": launches,
-            "kernel_ms_per_step": dev_s * 1e3,
+            "launch_ms": launch_s * 1e3,
+            "algorithmic_bytes_per_launch": bytes_launch,
+            "kernel_ms_per_step": tot["seconds"] / args.steps * 1e3,
             "bytes_per_period": 16.0 * n + 88.0,
+            "traffic_source": None if traffic is None else traffic["source"],
         },
     }
     if not args.no_cpu:

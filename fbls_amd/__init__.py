@@ -49,7 +49,7 @@ EXPORTS = (
     "fbls_signal_search", "fbls_period_grid_count", "fbls_period_grid",
     "fbls_durations_days", "fbls_family_state", "fbls_quantize",
     "fbls_synth_star", "fbls_last_error", "fbls_plan_bands",
-    "fbls_bank_batches",
+    "fbls_bank_batches", "fbls_ctx_launch_totals",
 )
 
 
@@ -78,6 +78,11 @@ class ScanStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LaunchTotals(ctypes.Structure):
+    _fields_ = [("launches", _u64), ("periods", _u64), ("fold_pairs", _u64),
+                ("seconds", _d)]
 
 
 class Row(ctypes.Structure):
@@ -130,6 +135,7 @@ def lib():
         L.fbls_family_free.restype = None
         L.fbls_mem_info.argtypes = [_vp, _P(_sz), _P(_sz)]
         L.fbls_ctx_set_launch.argtypes = [_vp, _u32, _u32, _u32]
+        L.fbls_ctx_launch_totals.argtypes = [_vp, _P(LaunchTotals), ctypes.c_int]
         L.fbls_family_upload.argtypes = [_vp, _P(_d), _P(_d), _P(_d), _sz, _d,
                                          _d, _P(_vp)]
         L.fbls_family_fields.argtypes = [_vp, _P(_d), _P(_d),
@@ -337,6 +343,14 @@ class Gpu:
 
     def set_launch(self, lds_bytes=0, block=0, fold_mode=0):
         _check(lib().fbls_ctx_set_launch(self._h, lds_bytes, block, fold_mode))
+
+    def launch_totals(self, reset=False):
+        """Kernel launches on this context since the last reset: count,
+        periods, fold pairs and HIP-event seconds."""
+        out = LaunchTotals()
+        _check(lib().fbls_ctx_launch_totals(self._h, ctypes.byref(out),
+                                            1 if reset else 0))
+        return {k: getattr(out, k) for k, _ in out._fields_}
 
     def close(self):
         if self._h:

@@ -99,6 +99,7 @@ struct fbls_ctx {
   uint32_t lds_override = 0, block_override = 0;
   size_t max_lds = kMaxLds;
   uint32_t run_fold = 1;
+  fbls_launch_totals totals = {0, 0, 0, 0.0};
   DevBuf dev_in, dev_out;
   HostBuf pin_in, pin_out;
 };
@@ -179,6 +180,14 @@ extern "C" int fbls_ctx_set_launch(fbls_ctx* c, uint32_t lds_bytes,
   c->lds_override = lds_bytes;
   c->block_override = block;
   c->run_fold = fold_mode == 2 ? 0u : 1u;
+  return FBLS_OK;
+}
+
+extern "C" int fbls_ctx_launch_totals(fbls_ctx* c, fbls_launch_totals* out,
+                                      int reset) {
+  if (!c || !out) return fail(FBLS_E_ARG, "null argument");
+  *out = c->totals;
+  if (reset) c->totals = fbls_launch_totals{0, 0, 0, 0.0};
   return FBLS_OK;
 }
 
@@ -506,6 +515,12 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   *seconds = ms * 1e-3;
+  c->totals.launches += 1;
+  c->totals.seconds += *seconds;
+  for (size_t i = 0; i < nw; i++) {
+    c->totals.periods += w[i].np;
+    c->totals.fold_pairs += (uint64_t)w[i].np * w[i].fam->n;
+  }
   const unsigned char* hout = (const unsigned char*)c->pin_out.ptr;
   const unsigned int* cnt = (const unsigned int*)(hin + cnt_off);
   bool bad = false;

@@ -95,6 +95,17 @@ int fbls_mem_info(fbls_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 int fbls_ctx_set_launch(fbls_ctx* ctx, uint32_t lds_bytes, uint32_t block,
                         uint32_t fold_mode);
 
+/* Running totals of the scan kernel launches made on this context, timed
+ * with HIP events on the context's stream (the profile counters of
+ * `seconds_device`, scan.rs:354-356, summed over launches).  fold_pairs =
+ * sum over launched periods of the family's observation count.  reset != 0
+ * zeroes the totals after reading. */
+typedef struct {
+  uint64_t launches, periods, fold_pairs;
+  double seconds;
+} fbls_launch_totals;
+int fbls_ctx_launch_totals(fbls_ctx* ctx, fbls_launch_totals* out, int reset);
+
 /* -- family upload ------------------------------------------------------ */
 
 /* `DeviceReady::upload` (scan.rs:110-125).  t: origin-relative times >= 0;
