@@ -49,7 +49,8 @@ EXPORTS = (
     "fbls_signal_search", "fbls_period_grid_count", "fbls_period_grid",
     "fbls_durations_days", "fbls_family_state", "fbls_quantize",
     "fbls_synth_star", "fbls_last_error", "fbls_plan_bands",
-    "fbls_bank_batches", "fbls_ctx_launch_totals",
+    "fbls_bank_batches", "fbls_ctx_launch_totals", "fbls_compare",
+    "fbls_evidence_parameters",
 )
 
 
@@ -83,6 +84,36 @@ class ScanStats(ctypes.Structure):
 class LaunchTotals(ctypes.Structure):
     _fields_ = [("launches", _u64), ("periods", _u64), ("fold_pairs", _u64),
                 ("seconds", _d)]
+
+
+class EvidenceCfg(ctypes.Structure):
+    _fields_ = [("ingress", _P(_d)), ("ns", _sz), ("layers", _u32),
+                ("phase_oversample", _u32), ("threshold", _d),
+                ("evidence_rtol", _d), ("planet_prior", _d),
+                ("binary_prior", _d), ("planet_depth_scale", _d),
+                ("binary_depth_scale", _d),
+                ("diluted_binary_depth_scale", _d),
+                ("stellar_radius_solar", _d), ("planet_radius_scale_rj", _d),
+                ("target_flux_fraction", _d)]
+
+
+class EvidenceRow(ctypes.Structure):
+    _fields_ = [(k, _d) for k in (
+        "period", "epoch", "duration", "snr", "depth", "planet_log_odds",
+        "log_bf_planet_noise", "log_bf_binary_noise",
+        "binary_secondary_ratio", "binary_secondary_phase",
+        "binary_odd_even_ratio", "binary_period_days", "ingress_fraction")] + [
+        ("selection_flags", _u32), ("phase", _u32), ("duration_index", _u32),
+        ("shape", _u32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EvidenceDebug(ctypes.Structure):
+    _fields_ = [("screen", _P(_d)), ("screen_cap", _sz), ("parts", _P(_d)),
+                ("parts_cap", _sz), ("floors", _d * 4), ("tiles", _sz),
+                ("nb", _sz)]
 
 
 class Row(ctypes.Structure):
@@ -136,6 +167,10 @@ def lib():
         L.fbls_mem_info.argtypes = [_vp, _P(_sz), _P(_sz)]
         L.fbls_ctx_set_launch.argtypes = [_vp, _u32, _u32, _u32]
         L.fbls_ctx_launch_totals.argtypes = [_vp, _P(LaunchTotals), ctypes.c_int]
+        L.fbls_compare.argtypes = [_vp, _vp, _d, _P(_d), _P(_d), _sz, _d,
+                                   _P(EvidenceCfg), _P(EvidenceRow), _P(_sz),
+                                   _P(EvidenceDebug)]
+        L.fbls_evidence_parameters.argtypes = [_P(EvidenceCfg), _P(_d)]
         L.fbls_family_upload.argtypes = [_vp, _P(_d), _P(_d), _P(_d), _sz, _d,
                                          _d, _P(_vp)]
         L.fbls_family_fields.argtypes = [_vp, _P(_d), _P(_d),
@@ -150,7 +185,8 @@ def lib():
             _sz, _P(_sz), _P(ScanStats), _sz, _P(_sz), _P(_d), _sz, _P(_sz)]
         L.fbls_family_state.argtypes = [
             _P(_d), _P(_d), _P(_d), _sz, _P(_d), _sz, ctypes.c_int, _P(_d),
-            _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz)]
+            _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz),
+            _P(_d)]
         L.fbls_synth_star.argtypes = [_P(SynthCfg), _P(_d), _P(_d), _P(_d),
                                       _sz, _P(_sz)]
         L.fbls_plan_bands.argtypes = [_P(_d), _sz, _P(_d), _sz, _u32, _u32,
@@ -199,6 +235,16 @@ class SearchConfig:
     period_oversample: float = 3.0
     period_batch: int = 4096
     max_fold_mib: float = 1024.0
+    # model comparison (config.rs:33-42,76-85)
+    evidence_rtol: float = 1e-12
+    planet_prior: float = 0.01
+    binary_prior: float = 0.01
+    planet_depth_scale: float = 0.01
+    binary_depth_scale: float = 0.1
+    diluted_binary_depth_scale: float = 0.01
+    stellar_radius_solar: float = None
+    planet_radius_scale_rj: float = 1.0
+    target_flux_fraction: float = 1.0
 
     def validate(self):
         """config.rs:91-131 (the fields of the search path)."""
@@ -287,6 +333,7 @@ class FamilyState:
     rho: np.ndarray
     inflation: float
     segments: int
+    residual_energy: float = 0.0
 
 
 def family_state(time, flux, sigma, durations, correlated=True):
@@ -296,15 +343,16 @@ def family_state(time, flux, sigma, durations, correlated=True):
     n = len(time)
     t, v, w = np.empty(n), np.empty(n), np.empty(n)
     rho = np.empty(len(durations))
-    m, c, org, infl, seg = _sz(), _d(), _d(), _d(), _sz()
+    m, c, org, infl, seg, en = _sz(), _d(), _d(), _d(), _sz(), _d()
     _check(lib().fbls_family_state(
         _dp(time), _dp(flux), _dp(sigma), n, _dp(durations), len(durations),
         int(bool(correlated)), _dp(t), _dp(v), _dp(w), ctypes.byref(m),
         ctypes.byref(c), ctypes.byref(org), _dp(rho), ctypes.byref(infl),
-        ctypes.byref(seg)))
+        ctypes.byref(seg), ctypes.byref(en)))
     k = m.value
     return FamilyState(t[:k].copy(), v[:k].copy(), w[:k].copy(), c.value,
-                       org.value, durations, rho, infl.value, seg.value)
+                       org.value, durations, rho, infl.value, seg.value,
+                       en.value)
 
 
 def synth_star(**kw):
@@ -402,6 +450,64 @@ class DeviceReady:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------- evidence
+
+def evidence_cfg(cfg: SearchConfig):
+    """fbls_evidence_cfg of a SearchConfig (the ingress array is kept alive
+    on the returned structure)."""
+    ing = _f64(cfg.ingress_fractions)
+    c = EvidenceCfg(
+        ingress=_dp(ing), ns=len(ing), layers=cfg.shape_layers,
+        phase_oversample=cfg.phase_oversample, threshold=cfg.threshold,
+        evidence_rtol=cfg.evidence_rtol, planet_prior=cfg.planet_prior,
+        binary_prior=cfg.binary_prior,
+        planet_depth_scale=cfg.planet_depth_scale,
+        binary_depth_scale=cfg.binary_depth_scale,
+        diluted_binary_depth_scale=cfg.diluted_binary_depth_scale,
+        stellar_radius_solar=(cfg.stellar_radius_solar
+                              if cfg.stellar_radius_solar else 0.0),
+        planet_radius_scale_rj=cfg.planet_radius_scale_rj,
+        target_flux_fraction=cfg.target_flux_fraction)
+    c._keep = ing
+    return c
+
+
+def evidence_parameters(cfg: SearchConfig):
+    """parameters (evidence.rs:665-689): the 30 model constants."""
+    c = evidence_cfg(cfg)
+    out = np.empty(30)
+    _check(lib().fbls_evidence_parameters(ctypes.byref(c), _dp(out)))
+    return out
+
+
+def compare_search(gpu, dev, period, durations, rho, residual_energy,
+                   cfg: SearchConfig, debug=False):
+    """compare_search (evidence.rs:693-1003) on the GPU: the joint-evidence
+    and SNR winners at one trial period as row dicts (merged, flags 3, when
+    identical).  With debug, also {screen, parts, floors, tiles, nb}."""
+    d, r = _f64(durations), _f64(rho)
+    c = evidence_cfg(cfg)
+    rows = (EvidenceRow * 2)()
+    nrows = _sz()
+    dbg = None
+    if debug:
+        cap = 4 * (int(cfg.phase_oversample * period / max(d.min(), 1e-9)) + 8)
+        screen, parts = np.zeros(cap * 4), np.zeros(cap * 30)
+        dbg = EvidenceDebug(screen=_dp(screen), screen_cap=screen.size,
+                            parts=_dp(parts), parts_cap=parts.size)
+    _check(lib().fbls_compare(gpu._h, dev._h, float(period), _dp(d), _dp(r),
+                              len(d), float(residual_energy), ctypes.byref(c),
+                              rows, ctypes.byref(nrows),
+                              ctypes.byref(dbg) if dbg is not None else None))
+    out = [rows[i].as_dict() for i in range(nrows.value)]
+    if not debug:
+        return out
+    k = dbg.tiles
+    return out, {"screen": screen[:4 * k].reshape(k, 4),
+                 "parts": parts[:30 * k].reshape(k, 2, 15),
+                 "floors": np.array(dbg.floors[:]), "tiles": k, "nb": dbg.nb}
 
 
 def make_bank(cfg: SearchConfig, durations, rho, max_fold_mib=None):

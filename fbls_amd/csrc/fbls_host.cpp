@@ -16,11 +16,15 @@
 #include "prep.h"
 #include "scan_kernel.h"
 
+#include "host_internal.h"
+
+using namespace fbls::host;
+
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+int fbls::host::fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -30,89 +34,11 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                     \
-  do {                                                                    \
-    hipError_t _e = (expr);                                               \
-    if (_e != hipSuccess)                                                 \
-      return fail(FBLS_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
-  } while (0)
-
-double now_s() {
+double fbls::host::now_s() {
   return std::chrono::duration<double>(
              std::chrono::steady_clock::now().time_since_epoch())
       .count();
 }
-
-// Growable device buffer.
-struct DevBuf {
-  void* ptr = nullptr;
-  size_t cap = 0;
-  int ensure(size_t bytes) {
-    if (bytes <= cap) return FBLS_OK;
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-    size_t want = std::max<size_t>(bytes, 4096);
-    if (hipMalloc(&ptr, want) != hipSuccess)
-      return fail(FBLS_E_OOM, "device allocation of %zu bytes failed", want);
-    cap = want;
-    return FBLS_OK;
-  }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-  }
-};
-
-// Growable pinned host buffer (staging of the per-launch tables/results).
-struct HostBuf {
-  void* ptr = nullptr;
-  size_t cap = 0;
-  int ensure(size_t bytes) {
-    if (bytes <= cap) return FBLS_OK;
-    if (ptr) (void)hipHostFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-    size_t want = std::max<size_t>(bytes + bytes / 2, 1 << 16);
-    if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess)
-      return fail(FBLS_E_OOM, "pinned host allocation of %zu bytes failed", want);
-    cap = want;
-    return FBLS_OK;
-  }
-  void release() {
-    if (ptr) (void)hipHostFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-  }
-};
-
-constexpr size_t kMaxLds = 160 * 1024;  // gfx950 LDS per CU
-constexpr uint32_t kObsAlign = 512;     // observations per wave block
-
-}  // namespace
-
-struct fbls_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  uint32_t lds_override = 0, block_override = 0;
-  size_t max_lds = kMaxLds;
-  uint32_t run_fold = 1;
-  fbls_launch_totals totals = {0, 0, 0, 0.0};
-  DevBuf dev_in, dev_out;
-  HostBuf pin_in, pin_out;
-};
-
-struct fbls_family {
-  fbls_ctx* ctx = nullptr;
-  DevBuf t, vw;
-  uint32_t n = 0, n_pad = 0;
-  double C = 0.0, origin = 0.0, t_max = 0.0, t_last = 0.0;
-  int ev = 0, ew = 0;
-  bool sorted = false;   // times ascend
-  std::vector<int32_t> vq, wq;
-};
 
 extern "C" const char* fbls_last_error(void) { return g_last_error.c_str(); }
 
@@ -154,6 +80,7 @@ extern "C" void fbls_ctx_destroy(fbls_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->dev_in.release();
   c->dev_out.release();
+  c->dev_ev.release();
   c->pin_in.release();
   c->pin_out.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -930,11 +857,13 @@ extern "C" int fbls_family_state(const double* time, const double* flux,
                                  double* w, size_t* out_n,
                                  double* baseline_norm, double* origin,
                                  double* rho, double* inflation,
-                                 size_t* segments) {
+                                 size_t* segments, double* residual_energy) {
   std::string msg;
+  double energy = 0.0;
   int st = fbls::family_state(time, flux, sigma, n, durations, nd, correlated,
                               t, v, w, out_n, baseline_norm, origin, rho,
-                              inflation, segments, &msg);
+                              inflation, segments, &energy, &msg);
+  if (residual_energy) *residual_energy = energy;
   if (st) return fail(st, "%s", msg.c_str());
   return FBLS_OK;
 }

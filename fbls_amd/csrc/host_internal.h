@@ -1,0 +1,97 @@
+// Internal host-side types shared by the C ABI sources (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/fbls.h"
+
+namespace fbls {
+namespace host {
+
+// Records the thread's last error message and returns code.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+double now_s();
+
+// Growable device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return FBLS_OK;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 4096);
+    if (hipMalloc(&ptr, want) != hipSuccess)
+      return fail(FBLS_E_OOM, "device allocation of %zu bytes failed", want);
+    cap = want;
+    return FBLS_OK;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+};
+
+// Growable pinned host buffer (staging of the per-launch tables/results).
+struct HostBuf {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return FBLS_OK;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes + bytes / 2, 1 << 16);
+    if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess)
+      return fail(FBLS_E_OOM, "pinned host allocation of %zu bytes failed", want);
+    cap = want;
+    return FBLS_OK;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+};
+
+constexpr size_t kMaxLds = 160 * 1024;  // gfx950 LDS per CU
+constexpr uint32_t kObsAlign = 512;     // observations per wave block
+
+}  // namespace host
+}  // namespace fbls
+
+#define HIP_TRY(expr)                                                     \
+  do {                                                                    \
+    hipError_t _e = (expr);                                               \
+    if (_e != hipSuccess)                                                 \
+      return fbls::host::fail(FBLS_E_HIP, "%s failed: %s", #expr,         \
+                              hipGetErrorString(_e));                     \
+  } while (0)
+
+struct fbls_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t lds_override = 0, block_override = 0;
+  size_t max_lds = fbls::host::kMaxLds;
+  uint32_t run_fold = 1;
+  fbls_launch_totals totals = {0, 0, 0, 0.0};
+  fbls::host::DevBuf dev_in, dev_out, dev_ev;
+  fbls::host::HostBuf pin_in, pin_out;
+};
+
+struct fbls_family {
+  fbls_ctx* ctx = nullptr;
+  fbls::host::DevBuf t, vw;
+  uint32_t n = 0, n_pad = 0;
+  double C = 0.0, origin = 0.0, t_max = 0.0, t_last = 0.0;
+  int ev = 0, ew = 0;
+  bool sorted = false;   // times ascend
+  std::vector<int32_t> vq, wq;
+};

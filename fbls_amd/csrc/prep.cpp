@@ -272,7 +272,8 @@ int family_state(const double* time, const double* flux, const double* sigma,
                  size_t n, const double* durations, size_t nd, int correlated,
                  double* t, double* v, double* w, size_t* out_n,
                  double* baseline_norm, double* origin, double* rho,
-                 double* inflation, size_t* segments, std::string* msg) {
+                 double* inflation, size_t* segments, double* residual_energy,
+                 std::string* msg) {
   // curve.rs:547-548: flux1 = 1 + flux, fit without binning, gap 0.75 d
   std::vector<double> flux1(n);
   for (size_t i = 0; i < n; i++) flux1[i] = 1.0 + flux[i];
@@ -302,6 +303,10 @@ int family_state(const double* time, const double* flux, const double* sigma,
     v[i] = py - w[i] * mean;
     t[i] = model.t[i];
   }
+  // E = sum y v (curve.rs:440)
+  double energy = 0.0;
+  for (size_t i = 0; i < m; i++) energy += model.deficit[i] * v[i];
+  *residual_energy = energy;
   *out_n = m;
   *baseline_norm = c;
   *origin = model.origin;

@@ -34,7 +34,8 @@ int family_state(const double* time, const double* flux, const double* sigma,
                  size_t n, const double* durations, size_t nd, int correlated,
                  double* t, double* v, double* w, size_t* out_n,
                  double* baseline_norm, double* origin, double* rho,
-                 double* inflation, size_t* segments, std::string* msg);
+                 double* inflation, size_t* segments, double* residual_energy,
+                 std::string* msg);
 
 int synth_star(const fbls_synth_cfg* cfg, double* time, double* flux,
                double* flux_err, size_t cap, size_t* n, std::string* msg);

@@ -149,6 +149,63 @@ int fbls_signal_search(fbls_ctx* ctx, const fbls_family* fam,
                        size_t* nbands, double* grid, size_t grid_cap,
                        size_t* grid_count);
 
+/* -- evidence (model comparison) ---------------------------------------- */
+
+/* SearchConfig fields read by `compare_search` (config.rs:33-42,53-85).
+ * stellar_radius_solar <= 0 means None. */
+typedef struct {
+  const double* ingress;      /* ns ingress fractions (shapes) */
+  size_t ns;
+  uint32_t layers;            /* staircase layers */
+  uint32_t phase_oversample;
+  double threshold;           /* SNR threshold */
+  double evidence_rtol;       /* pruning tolerance, 0 = exhaustive */
+  double planet_prior, binary_prior;
+  double planet_depth_scale, binary_depth_scale, diluted_binary_depth_scale;
+  double stellar_radius_solar, planet_radius_scale_rj, target_flux_fraction;
+} fbls_evidence_cfg;
+
+/* `EvidenceRow` (evidence.rs:637-653) plus the decoded winner: phase bin,
+ * index into the period's sorted durations below P/4, shape index. */
+typedef struct {
+  double period, epoch, duration, snr, depth, planet_log_odds,
+      log_bf_planet_noise, log_bf_binary_noise, binary_secondary_ratio,
+      binary_secondary_phase, binary_odd_even_ratio, binary_period_days,
+      ingress_fraction;
+  uint32_t selection_flags; /* 1 joint winner, 2 SNR winner, 3 both */
+  uint32_t phase, duration_index, shape;
+} fbls_evidence_row;
+
+/* Intermediate device arrays of one comparison, for inspection (every
+ * pointer may be NULL; capacities in doubles).  screen: tiles x 4 after the
+ * amplitude screen; parts: tiles x 2 x 15 after the joint pass (untouched
+ * when the screen is below threshold); floors: {floor_p, floor_b, pilot
+ * key, passed}.  tiles = ceil(nb / 64). */
+typedef struct {
+  double* screen;
+  size_t screen_cap;
+  double* parts;
+  size_t parts_cap;
+  double floors[4];
+  size_t tiles, nb;
+} fbls_evidence_debug;
+
+/* `compare_search` (evidence.rs:693-1003): joint planet / binary / noise
+ * evidence of one family at one trial period.  durations/rho: the family's
+ * bank (any order; those below period/4 are used, sorted).
+ * residual_energy: E of `prepare_ready` (curve.rs:440).  rows receives up
+ * to 2 rows (*nrows): the joint-evidence winner and the SNR winner, merged
+ * into one with flags 3 when identical; none when no template reaches the
+ * threshold.  debug may be NULL. */
+int fbls_compare(fbls_ctx* ctx, const fbls_family* fam, double period,
+                 const double* durations, const double* rho, size_t nd,
+                 double residual_energy, const fbls_evidence_cfg* cfg,
+                 fbls_evidence_row rows[2], size_t* nrows,
+                 fbls_evidence_debug* debug);
+
+/* [host] `parameters` (evidence.rs:665-689): the 30 model constants. */
+int fbls_evidence_parameters(const fbls_evidence_cfg* cfg, double out[30]);
+
 /* -- host helpers [host] ------------------------------------------------ */
 
 /* [host] `period_grid` (grid.rs:5-27): count first, then fill. */
@@ -183,13 +240,14 @@ size_t fbls_durations_days(const double* hours, size_t n, double* out_days);
 /* [host] `family_state` (curve.rs:541-569) for a prepared curve:
  * fit_segment_noise without binning, prepare_ready and, when correlated is
  * nonzero, correlated_noise_factors.  Outputs sized n (t, v, w) and nd
- * (rho); *out_n receives the kept count. */
+ * (rho); *out_n receives the kept count; residual_energy (may be NULL) is
+ * E = sum y v of prepare_ready (curve.rs:440). */
 int fbls_family_state(const double* time, const double* flux,
                       const double* sigma, size_t n, const double* durations,
                       size_t nd, int correlated, double* t, double* v,
                       double* w, size_t* out_n, double* baseline_norm,
                       double* origin, double* rho, double* inflation,
-                      size_t* segments);
+                      size_t* segments, double* residual_energy);
 
 /* [host] fixed-point representation used on the device; returns e with
  * q = rint(x * 2^e), |q| <= 2^30. */

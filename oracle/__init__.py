@@ -51,6 +51,30 @@ class SearchCfg(ctypes.Structure):
                 ("period_oversample", _d)]
 
 
+class EvidenceCfg(ctypes.Structure):
+    _fields_ = [("ingress", _P(_d)), ("ns", _sz), ("layers", _u32),
+                ("phase_oversample", _u32), ("threshold", _d),
+                ("evidence_rtol", _d), ("planet_prior", _d),
+                ("binary_prior", _d), ("planet_depth_scale", _d),
+                ("binary_depth_scale", _d),
+                ("diluted_binary_depth_scale", _d),
+                ("stellar_radius_solar", _d), ("planet_radius_scale_rj", _d),
+                ("target_flux_fraction", _d)]
+
+
+class EvidenceRow(ctypes.Structure):
+    _fields_ = [(k, _d) for k in (
+        "period", "epoch", "duration", "snr", "depth", "planet_log_odds",
+        "log_bf_planet_noise", "log_bf_binary_noise",
+        "binary_secondary_ratio", "binary_secondary_phase",
+        "binary_odd_even_ratio", "binary_period_days", "ingress_fraction")] + [
+        ("selection_flags", _u32), ("phase", _u32), ("duration_index", _u32),
+        ("shape", _u32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -76,6 +100,23 @@ def lib():
         L.orc_rng_normal.restype = _d
         L.orc_rng_normal.argtypes = [_P(_u64), _d, _d]
         L.orc_quantize.restype = ctypes.c_int
+        for f in ("orc_exp", "orc_log", "orc_erfc"):
+            getattr(L, f).restype = _d
+            getattr(L, f).argtypes = [_d]
+        L.orc_log_add.restype = _d
+        L.orc_log_add.argtypes = [_d, _d]
+        L.orc_positive_amplitude_evidence.restype = _d
+        L.orc_positive_amplitude_evidence.argtypes = [_d, _d, _d]
+        L.orc_subspace_upper.restype = _d
+        L.orc_subspace_upper.argtypes = [_d] * 6
+        L.orc_evidence_use_libm.argtypes = [ctypes.c_int]
+        L.orc_evidence_use_libm.restype = None
+        L.orc_evidence_parameters.argtypes = [_P(EvidenceCfg), _P(_d)]
+        L.orc_evidence_parameters.restype = None
+        L.orc_compare_search_ex.argtypes = [
+            _P(_d), _P(_d), _P(_d), _sz, _d, _d, _d, _P(_d), _P(_d), _sz, _d,
+            _P(EvidenceCfg), _P(EvidenceRow), _P(_sz), _P(_d), _sz, _P(_d),
+            _sz, _P(_d), _P(_sz), _P(_sz)]
         _lib = L
     return _lib
 
@@ -288,3 +329,91 @@ class Rng:
 
     def normal(self, mean, sigma):
         return lib().orc_rng_normal(ctypes.byref(self.state), mean, sigma)
+
+
+# ------------------------------------------------------------ evidence
+
+def exp(x):
+    return lib().orc_exp(float(x))
+
+
+def log(x):
+    return lib().orc_log(float(x))
+
+
+def erfc(x):
+    return lib().orc_erfc(float(x))
+
+
+def log_add(x, y):
+    return lib().orc_log_add(float(x), float(y))
+
+
+def positive_amplitude_evidence(a, q, tau):
+    return lib().orc_positive_amplitude_evidence(float(a), float(q), float(tau))
+
+
+def subspace_upper(a, b, q, r, cross, rho2=1.0):
+    return lib().orc_subspace_upper(float(a), float(b), float(q), float(r),
+                                    float(cross), float(rho2))
+
+
+def use_libm(on):
+    """Switch the evidence restatement between its own exp/log and glibc's."""
+    lib().orc_evidence_use_libm(1 if on else 0)
+
+
+def evidence_cfg(cfg):
+    """EvidenceCfg from a SearchConfig-like object; the ingress array is
+    kept alive on the returned structure."""
+    ing = np.ascontiguousarray(cfg.ingress_fractions, dtype=np.float64)
+    c = EvidenceCfg(
+        ingress=_dp(ing), ns=len(ing), layers=cfg.shape_layers,
+        phase_oversample=cfg.phase_oversample, threshold=cfg.threshold,
+        evidence_rtol=cfg.evidence_rtol, planet_prior=cfg.planet_prior,
+        binary_prior=cfg.binary_prior,
+        planet_depth_scale=cfg.planet_depth_scale,
+        binary_depth_scale=cfg.binary_depth_scale,
+        diluted_binary_depth_scale=cfg.diluted_binary_depth_scale,
+        stellar_radius_solar=(cfg.stellar_radius_solar
+                              if cfg.stellar_radius_solar else 0.0),
+        planet_radius_scale_rj=cfg.planet_radius_scale_rj,
+        target_flux_fraction=cfg.target_flux_fraction)
+    c._keep = ing
+    return c
+
+
+def evidence_parameters(cfg):
+    c = evidence_cfg(cfg)
+    out = np.empty(30)
+    lib().orc_evidence_parameters(ctypes.byref(c), _dp(out))
+    return out
+
+
+def compare_search(t, v, w, baseline_norm, origin, residual_energy,
+                   durations, rho, period, cfg, debug=False):
+    """compare_search (evidence.rs:693-1003) for one family at one period.
+    Returns a list of row dicts; with debug, also
+    {screen, parts, floors, tiles, nb}."""
+    t, v, w = _f64(t), _f64(v), _f64(w)
+    d, r = _f64(durations), _f64(rho)
+    c = evidence_cfg(cfg)
+    rows = (EvidenceRow * 2)()
+    nrows = _sz()
+    cap = 4 * (int(cfg.phase_oversample * period / max(d.min(), 1e-9)) + 8)
+    screen = np.zeros(cap * 4)
+    parts = np.zeros(cap * 30)
+    floors = np.zeros(3)
+    tiles, nb = _sz(), _sz()
+    lib().orc_compare_search_ex(
+        _dp(t), _dp(v), _dp(w), len(t), baseline_norm, origin,
+        residual_energy, _dp(d), _dp(r), len(d), period, ctypes.byref(c),
+        rows, ctypes.byref(nrows), _dp(screen), screen.size, _dp(parts),
+        parts.size, _dp(floors), ctypes.byref(tiles), ctypes.byref(nb))
+    out = [rows[i].as_dict() for i in range(nrows.value)]
+    if not debug:
+        return out
+    k = tiles.value
+    return out, {"screen": screen[:4 * k].reshape(k, 4),
+                 "parts": parts[:30 * k].reshape(k, 2, 15),
+                 "floors": floors, "tiles": k, "nb": nb.value}

@@ -143,6 +143,60 @@ size_t orc_plan_bands(const double* grid, size_t count,
 uint64_t orc_bank_batches(size_t np, uint32_t nb, uint32_t period_batch,
                           double max_fold_mib);
 
+/* ------------------------------------------------------------ evidence
+ * src/search/evidence.rs and src/search/math.rs (oracle/evidence.c). */
+
+/* SearchConfig fields read by compare_search (config.rs:33-42,53-85).
+ * stellar_radius_solar <= 0 means None. */
+typedef struct {
+  const double* ingress;
+  size_t ns;
+  uint32_t layers, phase_oversample;
+  double threshold, evidence_rtol;
+  double planet_prior, binary_prior, planet_depth_scale, binary_depth_scale,
+      diluted_binary_depth_scale, stellar_radius_solar, planet_radius_scale_rj,
+      target_flux_fraction;
+} orc_evidence_cfg;
+
+/* EvidenceRow (evidence.rs:637-653) plus the decoded winner indices. */
+typedef struct {
+  double period, epoch, duration, snr, depth, planet_log_odds,
+      log_bf_planet_noise, log_bf_binary_noise, binary_secondary_ratio,
+      binary_secondary_phase, binary_odd_even_ratio, binary_period_days,
+      ingress_fraction;
+  uint32_t selection_flags, phase, duration_index, shape;
+} orc_evidence_row;
+
+double orc_exp(double x);
+double orc_log(double x);
+void orc_evidence_use_libm(int on);
+double orc_erfc(double x);                                   /* math.rs:8-100 */
+double orc_log_add(double x, double y);                      /* math.rs:103-110 */
+double orc_positive_amplitude_evidence(double a, double q, double tau); /* math.rs:115-140 */
+double orc_subspace_upper(double a, double b, double q, double r,
+                          double cross, double rho2);        /* math.rs:145-177 */
+void orc_evidence_parameters(const orc_evidence_cfg* cfg, double* p30); /* evidence.rs:665-689 */
+/* evidence.rs:693-1003.  rows holds 2 entries; *nrows in 0..2. */
+int orc_compare_search(const double* t, const double* v, const double* w,
+                       size_t n, double baseline_norm, double origin,
+                       double residual_energy, const double* durations,
+                       const double* rho, size_t nd, double period,
+                       const orc_evidence_cfg* cfg, orc_evidence_row* rows,
+                       size_t* nrows);
+/* The same with the intermediate device arrays of the reference copied out
+ * (each may be NULL; sizes in doubles): screen = tiles x 4 after
+ * comparison_screen, parts = tiles x 2 x 15 after the full joint_tiles
+ * (zeros when the screen is below threshold), floors = {floor_p, floor_b,
+ * pilot key}; *tiles_out = ceil(nb / 64), *nb_out = nb. */
+int orc_compare_search_ex(const double* t, const double* v, const double* w,
+                          size_t n, double baseline_norm, double origin,
+                          double residual_energy, const double* durations,
+                          const double* rho, size_t nd, double period,
+                          const orc_evidence_cfg* cfg, orc_evidence_row* rows,
+                          size_t* nrows, double* screen, size_t screen_cap,
+                          double* parts, size_t parts_cap, double* floors,
+                          size_t* tiles_out, size_t* nb_out);
+
 /* survey/numeric.rs:345-364 */
 uint64_t orc_rng_new(uint64_t seed);
 double orc_rng_uniform(uint64_t* state);
