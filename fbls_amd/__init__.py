@@ -50,7 +50,8 @@ EXPORTS = (
     "fbls_durations_days", "fbls_family_state", "fbls_quantize",
     "fbls_synth_star", "fbls_last_error", "fbls_plan_bands",
     "fbls_bank_batches", "fbls_ctx_launch_totals", "fbls_compare",
-    "fbls_evidence_parameters",
+    "fbls_evidence_parameters", "fbls_family_upload_packed",
+    "fbls_signal_search_batch",
 )
 
 
@@ -183,6 +184,11 @@ def lib():
         L.fbls_signal_search.argtypes = [
             _vp, _vp, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC), _P(Row),
             _sz, _P(_sz), _P(ScanStats), _sz, _P(_sz), _P(_d), _sz, _P(_sz)]
+        L.fbls_family_upload_packed.argtypes = [
+            _vp, _sz, _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_d), _P(_vp)]
+        L.fbls_signal_search_batch.argtypes = [
+            _vp, _P(_vp), _sz, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
+            _P(Row), _sz, _P(_sz), _P(ScanStats)]
         L.fbls_family_state.argtypes = [
             _P(_d), _P(_d), _P(_d), _sz, _P(_d), _sz, ctypes.c_int, _P(_d),
             _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz),
@@ -432,6 +438,33 @@ class DeviceReady:
         return cls(gpu, state.t, state.v, state.w, state.baseline_norm,
                    state.origin)
 
+    @classmethod
+    def upload_packed(cls, gpu, states):
+        """Many families in one device allocation (ragged CSR layout)."""
+        k = len(states)
+        offs = np.zeros(k + 1, dtype=np.uintp)
+        offs[1:] = np.cumsum([len(s.t) for s in states])
+        t = np.concatenate([_f64(s.t) for s in states])
+        v = np.concatenate([_f64(s.v) for s in states])
+        w = np.concatenate([_f64(s.w) for s in states])
+        c = _f64([s.baseline_norm for s in states])
+        o = _f64([s.origin for s in states])
+        hs = (_vp * k)()
+        _check(lib().fbls_family_upload_packed(
+            gpu._h, k, offs.ctypes.data_as(_P(_sz)), _dp(t), _dp(v), _dp(w),
+            _dp(c), _dp(o), hs))
+        out = []
+        for i, st in enumerate(states):
+            d = cls.__new__(cls)
+            d.gpu = gpu
+            d.n = len(st.t)
+            d.baseline_norm = st.baseline_norm
+            d.origin = st.origin
+            d.t_last = float(st.t[-1])
+            d._h = _vp(hs[i])
+            out.append(d)
+        return out
+
     def fields(self):
         """The fixed-point values the device scans (exact doubles)."""
         v, w = np.empty(self.n), np.empty(self.n)
@@ -581,6 +614,37 @@ def signal_search(gpu, dev, durations, rho, cfg: SearchConfig):
         ctypes.byref(gcount)))
     return (_rows(rows, nrows.value), grid[:gcount.value],
             [s.as_dict() for s in stats[:nbands.value]])
+
+
+def signal_search_batch(gpu, devs, durations, rhos, cfg: SearchConfig,
+                        rows_cap=None):
+    """Many stars: returns ([rows k_s x 5 per star], [stats per star])."""
+    durations = _f64(durations)
+    rho = np.ascontiguousarray(np.asarray(rhos, dtype=np.float64).reshape(
+        len(devs), len(durations)))
+    bank = make_bank(cfg, durations, rho[0])
+    scfg = SearchCfgC(cfg.period_min, cfg.period_max, cfg.period_oversample)
+    hs = (_vp * len(devs))(*[d._h.value for d in devs])
+    offs = np.zeros(len(devs) + 1, dtype=np.uintp)
+    stats = (ScanStats * len(devs))()
+    if rows_cap is None:  # rows per star <= its grid count
+        dmin = float(durations.min())
+        rows_cap = 0
+        for d in devs:
+            hi = min(cfg.period_max, d.t_last)
+            lo = max(cfg.period_min, 4.0 * dmin * (1.0 + 1e-9))
+            if hi > lo:
+                rows_cap += lib().fbls_period_grid_count(lo, hi, d.t_last, dmin,
+                                                         cfg.period_oversample)
+    rows = (Row * max(rows_cap, 1))()
+    _check(lib().fbls_signal_search_batch(
+        gpu._h, hs, len(devs), _dp(durations), _dp(rho), len(durations),
+        ctypes.byref(bank), ctypes.byref(scfg), rows, rows_cap,
+        offs.ctypes.data_as(_P(_sz)), stats))
+    need = int(offs[-1])
+    allrows = _rows(rows, need)
+    return ([allrows[int(offs[i]):int(offs[i + 1])] for i in range(len(devs))],
+            [s.as_dict() for s in stats])
 
 
 def search_prepared(gpu, time, flux, flux_err, cfg: SearchConfig):

@@ -233,8 +233,8 @@ extern "C" int fbls_compare(fbls_ctx* c, const fbls_family* f, double period,
   unsigned char* dbase = (unsigned char*)c->dev_ev.ptr;
   const double* dtab = (const double*)(dbase + tab_off);
   fbls::EvArgs A{};
-  A.t = (const double*)f->t.ptr;
-  A.vw = (const int32_t*)f->vw.ptr;
+  A.t = f->t_dev;
+  A.vw = f->vw_dev;
   A.n = n;
   A.nb = (uint32_t)nb;
   A.nb2 = (uint32_t)nb2;

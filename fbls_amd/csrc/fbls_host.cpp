@@ -124,12 +124,14 @@ extern "C" int fbls_quantize(const double* x, size_t n, int32_t* q) {
   return fbls::quantize(x, n, q);
 }
 
-extern "C" int fbls_family_upload(fbls_ctx* c, const double* t,
-                                  const double* v, const double* w, size_t n,
-                                  double baseline_norm, double origin,
-                                  fbls_family** out) {
-  if (!c || !t || !v || !w || !out) return fail(FBLS_E_ARG, "null argument");
-  *out = nullptr;
+namespace {
+
+// Validates one family and fills its host-side fields (fixed point, sizes);
+// the device pointers are set by the caller.
+int prepare_family(fbls_ctx* c, const double* t, const double* v,
+                   const double* w, size_t n, double baseline_norm,
+                   double origin, fbls_family* f) {
+  if (!t || !v || !w) return fail(FBLS_E_ARG, "null argument");
   if (n < 6) return fail(FBLS_E_DATA, "at least six observations are required");
   if (n > (1u << 22))
     return fail(FBLS_E_ARG, "at most 2^22 observations per family (exact sums)");
@@ -143,8 +145,6 @@ extern "C" int fbls_family_upload(fbls_ctx* c, const double* t,
       return fail(FBLS_E_DATA, "fields must be finite");
     tmax = std::max(tmax, t[i]);
   }
-  HIP_TRY(hipSetDevice(c->device));
-  fbls_family* f = new fbls_family();
   f->ctx = c;
   f->n = (uint32_t)n;
   f->n_pad = (uint32_t)((n + kObsAlign - 1) / kObsAlign * kObsAlign);
@@ -158,36 +158,113 @@ extern "C" int fbls_family_upload(fbls_ctx* c, const double* t,
   f->ew = fbls::quantize(w, n, f->wq.data());
   f->sorted = true;
   for (size_t i = 1; i < n; i++) f->sorted = f->sorted && t[i] >= t[i - 1];
-  std::vector<double> tp(f->n_pad, 0.0);
-  std::vector<int32_t> vw(2 * (size_t)f->n_pad, 0);
-  std::memcpy(tp.data(), t, n * sizeof(double));
-  for (size_t i = 0; i < n; i++) {
+  return FBLS_OK;
+}
+
+// Device image of one family: t[n_pad] then vw[2 n_pad] (16 B per point).
+size_t family_bytes(const fbls_family* f) { return (size_t)f->n_pad * 16; }
+
+void write_family_image(const fbls_family* f, const double* t, unsigned char* dst) {
+  double* tp = (double*)dst;
+  int32_t* vw = (int32_t*)(dst + (size_t)f->n_pad * 8);
+  std::memcpy(tp, t, (size_t)f->n * 8);
+  std::fill(tp + f->n, tp + f->n_pad, 0.0);
+  for (uint32_t i = 0; i < f->n; i++) {
     vw[2 * i] = f->vq[i];
     vw[2 * i + 1] = f->wq[i];
   }
-  int st = f->t.ensure(tp.size() * sizeof(double));
-  if (!st) st = f->vw.ensure(vw.size() * sizeof(int32_t));
-  if (st) {
-    fbls_family_free(f);
-    return st;
+  std::fill(vw + 2 * (size_t)f->n, vw + 2 * (size_t)f->n_pad, 0);
+}
+
+// Uploads the families (already prepared) into one arena.
+int upload_arena(fbls_ctx* c, std::vector<fbls_family*>& fams,
+                 const std::vector<const double*>& times) {
+  size_t total = 0;
+  for (fbls_family* f : fams) total += family_bytes(f);
+  auto arena = std::make_shared<DevArena>();
+  arena->device = c->device;
+  HIP_TRY(hipSetDevice(c->device));
+  if (int st = arena->buf.ensure(total)) return st;
+  std::vector<unsigned char> image(total);
+  size_t at = 0;
+  for (size_t k = 0; k < fams.size(); k++) {
+    write_family_image(fams[k], times[k], image.data() + at);
+    at += family_bytes(fams[k]);
   }
-  if (hipMemcpy(f->t.ptr, tp.data(), tp.size() * sizeof(double),
-                hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(f->vw.ptr, vw.data(), vw.size() * sizeof(int32_t),
-                hipMemcpyHostToDevice) != hipSuccess) {
-    fbls_family_free(f);
+  if (hipMemcpy(arena->buf.ptr, image.data(), total, hipMemcpyHostToDevice) != hipSuccess)
     return fail(FBLS_E_HIP, "family upload failed");
+  at = 0;
+  for (fbls_family* f : fams) {
+    unsigned char* base = (unsigned char*)arena->buf.ptr + at;
+    f->arena = arena;
+    f->t_dev = (const double*)base;
+    f->vw_dev = (const int32_t*)(base + (size_t)f->n_pad * 8);
+    at += family_bytes(f);
+  }
+  return FBLS_OK;
+}
+
+}  // namespace
+
+extern "C" int fbls_family_upload(fbls_ctx* c, const double* t,
+                                  const double* v, const double* w, size_t n,
+                                  double baseline_norm, double origin,
+                                  fbls_family** out) {
+  if (!c || !out) return fail(FBLS_E_ARG, "null argument");
+  *out = nullptr;
+  fbls_family* f = new fbls_family();
+  int st = prepare_family(c, t, v, w, n, baseline_norm, origin, f);
+  std::vector<fbls_family*> one{f};
+  if (!st) st = upload_arena(c, one, {t});
+  if (st) {
+    delete f;
+    return st;
   }
   *out = f;
   return FBLS_OK;
 }
 
+extern "C" int fbls_family_upload_packed(fbls_ctx* c, size_t nfam,
+                                         const size_t* offsets, const double* t,
+                                         const double* v, const double* w,
+                                         const double* baseline_norm,
+                                         const double* origin,
+                                         fbls_family** out) {
+  if (!c || !offsets || !t || !v || !w || !baseline_norm || !origin || !out)
+    return fail(FBLS_E_ARG, "null argument");
+  std::vector<fbls_family*> fams;
+  std::vector<const double*> times;
+  int st = FBLS_OK;
+  for (size_t k = 0; k < nfam && !st; k++) {
+    out[k] = nullptr;
+    if (offsets[k + 1] < offsets[k]) {
+      st = fail(FBLS_E_ARG, "offsets must be nondecreasing");
+      break;
+    }
+    const size_t a = offsets[k], n = offsets[k + 1] - offsets[k];
+    fbls_family* f = new fbls_family();
+    fams.push_back(f);
+    times.push_back(t + a);
+    st = prepare_family(c, t + a, v + a, w + a, n, baseline_norm[k], origin[k], f);
+    if (st) {
+      char msg[600];
+      std::snprintf(msg, sizeof(msg), "family %zu: %s", k, fbls_last_error());
+      st = fail(st, "%s", msg);
+    }
+  }
+  if (!st && !fams.empty()) st = upload_arena(c, fams, times);
+  if (st) {
+    for (fbls_family* f : fams) delete f;
+    for (size_t k = 0; k < nfam; k++) out[k] = nullptr;
+    return st;
+  }
+  for (size_t k = 0; k < nfam; k++) out[k] = fams[k];
+  return FBLS_OK;
+}
+
 extern "C" void fbls_family_free(fbls_family* f) {
   if (!f) return;
-  if (f->ctx) (void)hipSetDevice(f->ctx->device);
-  f->t.release();
-  f->vw.release();
-  delete f;
+  delete f;  // the arena goes with its last family
 }
 
 extern "C" int fbls_family_fields(const fbls_family* f, double* v_out,
@@ -373,8 +450,8 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
     cur += 2 * x.nd;
     if (f->t_max * smax >= 4294967295.0)
       return fail(FBLS_E_ARG, "t * nb / P exceeds 2^32");
-    S.t = (const double*)f->t.ptr;
-    S.vw = (const int32_t*)f->vw.ptr;
+    S.t = f->t_dev;
+    S.vw = f->vw_dev;
     S.periods = (const double*)(din + per_at * 8);
     S.scales = S.periods + x.np;
     S.dur = (const double*)(din + dur_at * 8);
@@ -556,6 +633,7 @@ namespace {
 
 // Host state of one band (one scan_periods call) across the two launches.
 struct BandRun {
+  const fbls_family* fam;
   const double* periods;
   size_t np;
   uint32_t nb, nb_coarse, nd;
@@ -568,9 +646,8 @@ struct BandRun {
 // scan.rs:363-408 for any number of bands of one or more families: every
 // band's first scan (coarse, or full when there is no prescan) in one
 // launch, then every band's full scan of the survivors in a second launch.
-int scan_bands(fbls_ctx* c, const fbls_family* f, std::vector<BandRun>& bands,
-               const fbls_bank* b, fbls_row* rows, size_t* nrows,
-               size_t* row_counts) {
+int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
+               fbls_row* rows, size_t* nrows, size_t* row_counts) {
   const double t0 = now_s();
   const Shared sh = shared_of(b);
   std::vector<Work> first, second;
@@ -588,7 +665,7 @@ int scan_bands(fbls_ctx* c, const fbls_family* f, std::vector<BandRun>& bands,
     r.score.resize(r.np);
     r.depth.resize(r.np);
     Work x;
-    x.fam = f;
+    x.fam = r.fam;
     x.periods = r.periods;
     x.np = r.np;
     x.nb = nb0;
@@ -626,7 +703,7 @@ int scan_bands(fbls_ctx* c, const fbls_family* f, std::vector<BandRun>& bands,
     sc2[first_band[k]].resize(m);
     dp2[first_band[k]].resize(m);
     Work y;
-    y.fam = f;
+    y.fam = r.fam;
     y.periods = r.survivors.data();
     y.np = m;
     y.nb = r.nb;
@@ -669,7 +746,7 @@ int scan_bands(fbls_ctx* c, const fbls_family* f, std::vector<BandRun>& bands,
       const double* q = rec + 9 * k;
       fbls_row& row = rows[(*nrows)++];
       row.period = q[0];
-      row.epoch = (q[2] + q[3] / 2.0) / q[1] + f->origin;  // scan.rs:345
+      row.epoch = (q[2] + q[3] / 2.0) / q[1] + r.fam->origin;  // scan.rs:345
       row.duration = q[3] / q[1];
       row.score = sc[k];
       row.depth = dp[k];
@@ -713,6 +790,7 @@ extern "C" int fbls_scan_periods(fbls_ctx* c, const fbls_family* f,
   if (st) return st;
   std::vector<BandRun> bands(1);
   BandRun& r = bands[0];
+  r.fam = f;
   r.periods = periods;
   r.np = np;
   r.nb = band_nb(periods, np, b);
@@ -722,7 +800,7 @@ extern "C" int fbls_scan_periods(fbls_ctx* c, const fbls_family* f,
   r.nd = (uint32_t)b->nd;
   r.dur = b->durations;
   r.rho = b->rho;
-  st = scan_bands(c, f, bands, b, rows, nrows, nullptr);
+  st = scan_bands(c, bands, b, rows, nrows, nullptr);
   if (st) return st;
   if (stats) *stats = r.stats;
   return FBLS_OK;
@@ -775,6 +853,77 @@ extern "C" uint64_t fbls_bank_batches(size_t np, uint32_t nb,
   return (np + batch - 1) / batch;
 }
 
+namespace {
+
+// One family's search plan (scan.rs:413-471): sorted bank, grid, bands.
+struct SearchPlan {
+  const fbls_family* fam = nullptr;
+  std::vector<double> sd, sr, grid;
+  std::vector<fbls_band> bands;
+};
+
+int plan_search(const fbls_family* f, const double* durations, const double* rho,
+                size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
+                SearchPlan* p) {
+  p->fam = f;
+  p->grid.clear();
+  p->bands.clear();
+  if (nd == 0) return FBLS_OK;
+  std::vector<std::pair<double, double>> ds(nd);
+  for (size_t i = 0; i < nd; i++) {
+    if (!(durations[i] > 0.0 && std::isfinite(durations[i])) ||
+        !(rho[i] > 0.0 && std::isfinite(rho[i])))
+      return fail(FBLS_E_ARG, "durations and rho must be positive and finite");
+    ds[i] = {durations[i], rho[i]};
+  }
+  std::stable_sort(ds.begin(), ds.end(),
+                   [](const std::pair<double, double>& a,
+                      const std::pair<double, double>& b) { return a.first < b.first; });
+  p->sd.resize(nd);
+  p->sr.resize(nd);
+  for (size_t i = 0; i < nd; i++) {
+    p->sd[i] = ds[i].first;
+    p->sr[i] = ds[i].second;
+  }
+  const double baseline = f->t_last;  // last relative time (scan.rs:425)
+  const double maximum = std::min(cfg->period_max, baseline);
+  const double minimum = std::max(cfg->period_min, 4.0 * p->sd[0] * (1.0 + 1e-9));
+  if (maximum <= minimum) return FBLS_OK;
+  const size_t count = fbls::period_grid_count(minimum, maximum, baseline, p->sd[0],
+                                               cfg->period_oversample);
+  p->grid.resize(count);
+  fbls::period_grid(minimum, maximum, p->grid.data(), count);
+  p->bands = fbls::plan_bands(p->grid.data(), count, p->sd.data(), nd,
+                              bank->phase_oversample, bank->prescan_factor);
+  return FBLS_OK;
+}
+
+void append_runs(const SearchPlan& p, std::vector<BandRun>& runs) {
+  for (const fbls_band& band : p.bands) {
+    BandRun r;
+    r.fam = p.fam;
+    r.periods = p.grid.data() + band.first;
+    r.np = band.count;
+    r.nb = band.nb;
+    r.nb_coarse = band.nb_coarse;
+    r.nd = (uint32_t)band.nd;
+    r.dur = p.sd.data();
+    r.rho = p.sr.data();
+    runs.push_back(std::move(r));
+  }
+}
+
+// Gpu::fold_budget_mib (scan.rs:93-98)
+fbls_bank budgeted(const fbls_bank* bank) {
+  fbls_bank b = *bank;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess)
+    b.max_fold_mib = std::min(b.max_fold_mib, (double)fr / 1048576.0 / 2.0);
+  return b;
+}
+
+}  // namespace
+
 // scan.rs:413-473
 extern "C" int fbls_signal_search(fbls_ctx* c, const fbls_family* f,
                                   const double* durations, const double* rho,
@@ -790,61 +939,118 @@ extern "C" int fbls_signal_search(fbls_ctx* c, const fbls_family* f,
   *nbands = 0;
   *grid_count = 0;
   if (nd == 0) return FBLS_OK;
-  std::vector<std::pair<double, double>> ds(nd);
-  for (size_t i = 0; i < nd; i++) ds[i] = {durations[i], rho[i]};
-  std::stable_sort(ds.begin(), ds.end(),
-                   [](const std::pair<double, double>& a,
-                      const std::pair<double, double>& b) { return a.first < b.first; });
-  const double baseline = f->t_last;  // last relative time (scan.rs:425)
-  const double maximum = std::min(cfg->period_max, baseline);
-  const double minimum = std::max(cfg->period_min, 4.0 * ds[0].first * (1.0 + 1e-9));
-  if (maximum <= minimum) return FBLS_OK;
-  size_t count = fbls::period_grid_count(minimum, maximum, baseline, ds[0].first,
-                                         cfg->period_oversample);
-  std::vector<double> grid(count);
-  fbls::period_grid(minimum, maximum, grid.data(), count);
+  if (!durations || !rho) return fail(FBLS_E_ARG, "null argument");
+  SearchPlan p;
+  int st = plan_search(f, durations, rho, nd, bank, cfg, &p);
+  if (st) return st;
+  const size_t count = p.grid.size();
+  if (count == 0) return FBLS_OK;
   *grid_count = count;
   if (grid_out) {
     if (grid_cap < count) return fail(FBLS_E_ARG, "grid capacity %zu < %zu", grid_cap, count);
-    std::memcpy(grid_out, grid.data(), count * sizeof(double));
+    std::memcpy(grid_out, p.grid.data(), count * sizeof(double));
   }
   if (rows_cap < count) return fail(FBLS_E_ARG, "rows capacity %zu < grid %zu", rows_cap, count);
-  // Gpu::fold_budget_mib (scan.rs:93-98)
-  fbls_bank b = *bank;
-  size_t fr = 0, tot = 0;
-  if (hipMemGetInfo(&fr, &tot) == hipSuccess)
-    b.max_fold_mib = std::min(b.max_fold_mib, (double)fr / 1048576.0 / 2.0);
-  std::vector<double> sd(nd), sr(nd);
-  for (size_t i = 0; i < nd; i++) {
-    sd[i] = ds[i].first;
-    sr[i] = ds[i].second;
-  }
-  std::vector<fbls_band> plan = fbls::plan_bands(grid.data(), count, sd.data(), nd,
-                                                 bank->phase_oversample,
-                                                 bank->prescan_factor);
-  if (stats && plan.size() > stats_cap)
+  if (stats && p.bands.size() > stats_cap)
     return fail(FBLS_E_ARG, "band statistics capacity %zu < %zu bands", stats_cap,
-                plan.size());
-  int st = check_bank(&b);
-  if (st) return st;
-  std::vector<BandRun> runs(plan.size());
-  for (size_t i = 0; i < plan.size(); i++) {
-    runs[i].periods = grid.data() + plan[i].first;
-    runs[i].np = plan[i].count;
-    runs[i].nb = plan[i].nb;
-    runs[i].nb_coarse = plan[i].nb_coarse;
-    runs[i].nd = (uint32_t)plan[i].nd;
-    runs[i].dur = sd.data();
-    runs[i].rho = sr.data();
-  }
-  b.durations = sd.data();
-  b.rho = sr.data();
+                p.bands.size());
+  fbls_bank b = budgeted(bank);
+  b.durations = p.sd.data();
+  b.rho = p.sr.data();
   b.nd = nd;
-  st = scan_bands(c, f, runs, &b, rows, nrows, nullptr);
+  st = check_bank(&b);
+  if (st) return st;
+  std::vector<BandRun> runs;
+  append_runs(p, runs);
+  st = scan_bands(c, runs, &b, rows, nrows, nullptr);
   if (st) return st;
   for (size_t i = 0; i < runs.size(); i++)
     if (stats) stats[i] = runs[i].stats;
   *nbands = runs.size();
+  return FBLS_OK;
+}
+
+// Many stars (one family each) through the same two-launch band loop: the
+// coarse passes of every band of every star in a chunk share one launch,
+// their fine passes a second.  Chunks hold at most kBatchPeriods periods.
+extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* fams,
+                                        size_t nfam, const double* durations,
+                                        const double* rho, size_t nd,
+                                        const fbls_bank* bank,
+                                        const fbls_search_cfg* cfg, fbls_row* rows,
+                                        size_t rows_cap, size_t* row_offsets,
+                                        fbls_scan_stats* star_stats) {
+  if (!c || (nfam && !fams) || !bank || !cfg || !row_offsets || (nd && (!durations || !rho)))
+    return fail(FBLS_E_ARG, "null argument");
+  constexpr size_t kBatchPeriods = size_t(1) << 21;
+  for (size_t k = 0; k <= nfam; k++) row_offsets[k] = 0;
+  if (star_stats) std::memset(star_stats, 0, nfam * sizeof(fbls_scan_stats));
+  if (nd == 0 || nfam == 0) return FBLS_OK;
+  fbls_bank b = budgeted(bank);
+  size_t written = 0, needed = 0;
+  size_t s0 = 0;
+  while (s0 < nfam) {
+    // one chunk of stars
+    std::vector<SearchPlan> plans;
+    std::vector<BandRun> runs;
+    std::vector<size_t> run_star;
+    size_t periods = 0, s1 = s0;
+    while (s1 < nfam && (s1 == s0 || periods < kBatchPeriods)) {
+      if (!fams[s1]) return fail(FBLS_E_ARG, "null family %zu", s1);
+      plans.emplace_back();
+      int st = plan_search(fams[s1], durations, rho + s1 * nd, nd, bank, cfg, &plans.back());
+      if (st) return st;
+      periods += plans.back().grid.size();
+      s1++;
+    }
+    for (size_t k = 0; k < plans.size(); k++) {
+      const size_t before = runs.size();
+      append_runs(plans[k], runs);
+      for (size_t i = before; i < runs.size(); i++) run_star.push_back(s0 + k);
+    }
+    // every plan shares the sorted bank durations (rho differs per star)
+    if (!plans.empty()) {
+      b.durations = plans[0].sd.data();
+      b.rho = plans[0].sr.data();
+      b.nd = nd;
+      int st = check_bank(&b);
+      if (st) return st;
+    }
+    std::vector<fbls_row> chunk_rows(periods ? periods : 1);
+    std::vector<size_t> counts(runs.size(), 0);
+    size_t nchunk = 0;
+    if (!runs.empty()) {
+      int st = scan_bands(c, runs, &b, chunk_rows.data(), &nchunk, counts.data());
+      if (st) return st;
+    }
+    size_t at = 0;
+    for (size_t i = 0; i < runs.size(); i++) {
+      const size_t s = run_star[i];
+      row_offsets[s + 1] += counts[i];
+      for (size_t k = 0; k < counts[i]; k++, at++) {
+        if (written < rows_cap && rows) rows[written] = chunk_rows[at];
+        written++;
+      }
+      if (star_stats) {
+        fbls_scan_stats& a = star_stats[s];
+        const fbls_scan_stats& r = runs[i].stats;
+        a.periods += r.periods;
+        a.prescan_survivors += r.prescan_survivors;
+        a.phase_bins = std::max(a.phase_bins, r.phase_bins);
+        a.batches += r.batches;
+        a.templates_scanned += r.templates_scanned;
+        a.exact_tile_retries += r.exact_tile_retries;
+        a.seconds += r.seconds;
+        a.seconds_scan += r.seconds_scan;
+        a.seconds_device += r.seconds_device;
+      }
+    }
+    needed += nchunk;
+    s0 = s1;
+  }
+  for (size_t k = 0; k < nfam; k++) row_offsets[k + 1] += row_offsets[k];
+  if (needed > rows_cap)
+    return fail(FBLS_E_ARG, "rows capacity %zu < %zu rows (offsets filled)", rows_cap, needed);
   return FBLS_OK;
 }
 

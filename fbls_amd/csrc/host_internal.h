@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "../../include/fbls.h"
@@ -60,6 +61,17 @@ struct HostBuf {
   }
 };
 
+// Device memory shared by the families of one upload (one family, or a
+// packed batch of stars); released with the last family that uses it.
+struct DevArena {
+  int device = 0;
+  DevBuf buf;
+  ~DevArena() {
+    (void)hipSetDevice(device);
+    buf.release();
+  }
+};
+
 constexpr size_t kMaxLds = 160 * 1024;  // gfx950 LDS per CU
 constexpr uint32_t kObsAlign = 512;     // observations per wave block
 
@@ -88,7 +100,9 @@ struct fbls_ctx {
 
 struct fbls_family {
   fbls_ctx* ctx = nullptr;
-  fbls::host::DevBuf t, vw;
+  std::shared_ptr<fbls::host::DevArena> arena;
+  const double* t_dev = nullptr;    // n_pad origin-relative times (pad 0)
+  const int32_t* vw_dev = nullptr;  // n_pad x {v, w} fixed point (pad 0)
   uint32_t n = 0, n_pad = 0;
   double C = 0.0, origin = 0.0, t_max = 0.0, t_last = 0.0;
   int ev = 0, ew = 0;

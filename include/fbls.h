@@ -117,6 +117,16 @@ int fbls_family_upload(fbls_ctx* ctx, const double* t, const double* v,
                        const double* w, size_t n, double baseline_norm,
                        double origin, fbls_family** out);
 void fbls_family_free(fbls_family* fam);
+/* Packed upload of many families (stars) into one device allocation (a
+ * ragged CSR layout): family k holds observations offsets[k] ..
+ * offsets[k+1] - 1 of t, v, w, with baseline_norm[k] and origin[k].  Writes
+ * nfam handles to out; each is freed with fbls_family_free (the allocation
+ * goes with the last).  The per-star `DeviceReady::upload` of `fbls batch`
+ * (batch.rs:76, scan.rs:110-125) for a whole batch at once. */
+int fbls_family_upload_packed(fbls_ctx* ctx, size_t nfam, const size_t* offsets,
+                              const double* t, const double* v, const double* w,
+                              const double* baseline_norm, const double* origin,
+                              fbls_family** out);
 /* The fixed-point values the device scans, as doubles (exact), and the
  * power-of-two exponents: v_dev = round(v * 2^ev) * 2^-ev. */
 int fbls_family_fields(const fbls_family* fam, double* v_out, double* w_out,
@@ -148,6 +158,23 @@ int fbls_signal_search(fbls_ctx* ctx, const fbls_family* fam,
                        fbls_scan_stats* stats, size_t stats_cap,
                        size_t* nbands, double* grid, size_t grid_cap,
                        size_t* grid_count);
+
+/* `signal_search` of many stars (one family each) as `fbls batch` runs
+ * them (batch.rs:61-106), with the work of all stars merged: the coarse
+ * passes of every band of up to 2^21 grid periods' worth of stars share one
+ * launch, their fine passes a second.  durations: the nd bank durations
+ * (any order, shared); rho: nfam x nd factors, in the order of durations.
+ * rows: the rows of star k are rows[row_offsets[k] .. row_offsets[k+1]);
+ * row_offsets holds nfam + 1 entries and is filled even when rows_cap is
+ * too small (then FBLS_E_ARG; rows_cap >= the sum of the stars' grid counts
+ * always suffices).  star_stats (nfam, may be NULL): the band
+ * statistics summed per star (phase_bins = the largest). */
+int fbls_signal_search_batch(fbls_ctx* ctx, const fbls_family* const* fams,
+                             size_t nfam, const double* durations,
+                             const double* rho, size_t nd,
+                             const fbls_bank* bank, const fbls_search_cfg* cfg,
+                             fbls_row* rows, size_t rows_cap,
+                             size_t* row_offsets, fbls_scan_stats* star_stats);
 
 /* -- evidence (model comparison) ---------------------------------------- */
 
