@@ -12,7 +12,7 @@ mirror the reference's search interface (isakruas/fbls):
 * ``scan_bank``             src/search/scan.rs:144-357
 * ``scan_periods``          src/search/scan.rs:363-408
 * ``signal_search``         src/search/scan.rs:413-473
-* ``search_prepared``       src/pipeline.rs:41-68 (first pass: scan only)
+* ``search_prepared``       src/pipeline.rs:41-221 (passes, selection, evidence)
 
 Every scan runs on the GPU through the HIP kernels; there is no CPU
 fallback, and a missing library or device raises.
@@ -51,7 +51,7 @@ EXPORTS = (
     "fbls_synth_star", "fbls_last_error", "fbls_plan_bands",
     "fbls_bank_batches", "fbls_ctx_launch_totals", "fbls_compare",
     "fbls_evidence_parameters", "fbls_family_upload_packed",
-    "fbls_signal_search_batch",
+    "fbls_signal_search_batch", "fbls_search_prepared",
 )
 
 
@@ -111,10 +111,45 @@ class EvidenceRow(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PipelineCfgC(ctypes.Structure):
+    pass
+
+
 class EvidenceDebug(ctypes.Structure):
     _fields_ = [("screen", _P(_d)), ("screen_cap", _sz), ("parts", _P(_d)),
                 ("parts_cap", _sz), ("floors", _d * 4), ("tiles", _sz),
                 ("nb", _sz)]
+
+
+PEAK_FIELDS_D1 = ("period_days", "epoch_bjd", "duration_hours", "snr",
+                  "snr_folded", "snr_without_strongest_event", "depth",
+                  "depth_ppm")
+PEAK_FIELDS_U = ("events", "family", "pass_", "row")
+PEAK_FIELDS_D2 = ("planet_log_odds", "log_bf_planet_noise",
+                  "log_bf_binary_noise", "binary_secondary_ratio",
+                  "binary_secondary_phase", "binary_odd_even_ratio",
+                  "binary_period_days", "ingress_fraction",
+                  "evidence_epoch_bjd", "evidence_duration_hours")
+
+
+class Peak(ctypes.Structure):
+    _fields_ = ([(k, _d) for k in PEAK_FIELDS_D1] + [(k, _u64) for k in PEAK_FIELDS_U]
+                + [(k, _d) for k in PEAK_FIELDS_D2]
+                + [("model_preferred", _u32), ("binary_diagnostics_complete", _u32)])
+
+    def as_dict(self):
+        d = {k.rstrip("_"): getattr(self, k) for k, _ in self._fields_}
+        d["model_preferred"] = bool(d["model_preferred"])
+        d["binary_diagnostics_complete"] = bool(d["binary_diagnostics_complete"])
+        return d
+
+
+class PassStats(ctypes.Structure):
+    _fields_ = [("pass_", _u64), ("rows", _u64), ("periods", _u64),
+                ("accepted", _u64), ("seconds_scan", _d), ("seconds_select", _d)]
+
+    def as_dict(self):
+        return {k.rstrip("_"): getattr(self, k) for k, _ in self._fields_}
 
 
 class Row(ctypes.Structure):
@@ -125,6 +160,14 @@ class Row(ctypes.Structure):
 class SearchCfgC(ctypes.Structure):
     _fields_ = [("period_min", _d), ("period_max", _d),
                 ("period_oversample", _d)]
+
+
+PipelineCfgC._fields_ = [
+    ("durations", _P(_d)), ("nd", _sz), ("correlated", ctypes.c_int),
+    ("bank", Bank), ("search", SearchCfgC), ("max_candidates", _sz),
+    ("passes", _sz), ("min_events", _sz), ("evidence_candidates", _sz),
+    ("model_comparison", ctypes.c_int), ("planet_log_odds_min", _d),
+    ("evidence", EvidenceCfg)]
 
 
 class BandC(ctypes.Structure):
@@ -172,6 +215,9 @@ def lib():
                                    _P(EvidenceCfg), _P(EvidenceRow), _P(_sz),
                                    _P(EvidenceDebug)]
         L.fbls_evidence_parameters.argtypes = [_P(EvidenceCfg), _P(_d)]
+        L.fbls_search_prepared.argtypes = [
+            _vp, _P(_d), _P(_d), _P(_d), _sz, _P(PipelineCfgC), _P(Peak), _sz,
+            _P(_sz), _P(PassStats), _sz, _P(_sz)]
         L.fbls_family_upload.argtypes = [_vp, _P(_d), _P(_d), _P(_d), _sz, _d,
                                          _d, _P(_vp)]
         L.fbls_family_fields.argtypes = [_vp, _P(_d), _P(_d),
@@ -241,6 +287,13 @@ class SearchConfig:
     period_oversample: float = 3.0
     period_batch: int = 4096
     max_fold_mib: float = 1024.0
+    # passes and selection (config.rs:25-34,68-77)
+    max_candidates: int = 20
+    passes: int = 3
+    min_events: int = 2
+    model_comparison: bool = True
+    evidence_candidates: int = 32
+    planet_log_odds_min: float = 0.0
     # model comparison (config.rs:33-42,76-85)
     evidence_rtol: float = 1e-12
     planet_prior: float = 0.01
@@ -647,33 +700,49 @@ def signal_search_batch(gpu, devs, durations, rhos, cfg: SearchConfig,
             [s.as_dict() for s in stats])
 
 
+def pipeline_cfg(cfg: SearchConfig):
+    """fbls_pipeline_cfg of a SearchConfig (arrays kept alive on it)."""
+    d = cfg.durations_days()
+    bank = make_bank(cfg, d, np.ones(len(d)))
+    ev = evidence_cfg(cfg)
+    c = PipelineCfgC(
+        durations=_dp(d), nd=len(d), correlated=int(bool(cfg.correlated_noise)),
+        bank=bank, search=SearchCfgC(cfg.period_min, cfg.period_max,
+                                     cfg.period_oversample),
+        max_candidates=cfg.max_candidates, passes=cfg.passes,
+        min_events=cfg.min_events, evidence_candidates=cfg.evidence_candidates,
+        model_comparison=int(bool(cfg.model_comparison)),
+        planet_log_odds_min=cfg.planet_log_odds_min, evidence=ev)
+    c._keep = (d, bank, ev)
+    return c
+
+
 def search_prepared(gpu, time, flux, flux_err, cfg: SearchConfig):
-    """pipeline.rs:41-68 then the first pass of search_families
-    (pipeline.rs:102-124): one family with every duration, scanned once.
-    Returns the period table and the schema-3 style statistics."""
-    cfg.validate()
-    t0 = _time.perf_counter()
-    durations = cfg.durations_days()
-    state = family_state(time, flux, flux_err, durations, cfg.correlated_noise)
-    prep = _time.perf_counter() - t0
-    dev = DeviceReady.upload(gpu, state)
-    t1 = _time.perf_counter()
-    rows, grid, bands = signal_search(gpu, dev, durations, state.rho, cfg)
-    scan = _time.perf_counter() - t1
-    return {
-        "period_table": rows,
-        "statistics": {
-            "observations": len(state.t),
-            "periods_tested": len(grid),
-            "qualifying_periods": len(rows),
-            "seconds_preparation": prep,
-            "seconds_scan": scan,
-            "families": [{
-                "family": 0, "observations": len(state.t),
-                "error_inflation": state.inflation,
-                "segments": state.segments,
-                "durations_hours": list(durations * 24.0),
-                "periods": len(grid), "rho": list(state.rho),
-                "bands": bands}],
-        },
-    }
+    """search_prepared + search_families (pipeline.rs:41-221, without single
+    events) on the GPU: returns (peaks in acceptance order as dicts, pass
+    statistics)."""
+    time, flux, flux_err = _f64(time), _f64(flux), _f64(flux_err)
+    c = pipeline_cfg(cfg)
+    cap = max(cfg.max_candidates, 1)
+    peaks = (Peak * cap)()
+    passes = (PassStats * max(cfg.passes, 1))()
+    npk, npa = _sz(), _sz()
+    _check(lib().fbls_search_prepared(
+        gpu._h, _dp(time), _dp(flux), _dp(flux_err), len(time), ctypes.byref(c),
+        peaks, cap, ctypes.byref(npk), passes, len(passes), ctypes.byref(npa)))
+    return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
+            [passes[i].as_dict() for i in range(min(npa.value, len(passes)))])
+
+
+def by_signal(peaks):
+    """signal_candidates order (pipeline.rs:239-240)."""
+    return sorted(peaks, key=lambda p: -p["snr"])
+
+
+def by_model(peaks):
+    """candidates order (pipeline.rs:241-259)."""
+    def key(p):
+        ev = p["log_bf_planet_noise"] if p["model_preferred"] else float("nan")
+        v = ev if ev == ev else p["planet_log_odds"]
+        return (0 if p["model_preferred"] else 1, -(v if v == v else -np.inf))
+    return sorted(peaks, key=key)

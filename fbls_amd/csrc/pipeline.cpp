@@ -1,0 +1,545 @@
+// Multi-pass prepared search (isakruas/fbls src/pipeline.rs:41-221 without
+// single events): per pass the signal search of the family on the GPU, the
+// greedy peak selection with event-overlap rescoring (src/select/peaks.rs),
+// masking of the lead peak and a rebuilt family state; then the model
+// comparison of the strongest peaks.  Rescoring runs on the GPU
+// (select_kernel.hip) over the family's fixed-point fields, so every sum is
+// exact; the host-side statistics use the same fixed-point values.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "../../include/fbls.h"
+#include "host_internal.h"
+#include "prep.h"
+#include "select_kernel.h"
+
+using namespace fbls::host;
+
+namespace {
+
+double rem_euclid(double a, double b) {
+  const double r = std::fmod(a, b);
+  return r < 0.0 ? r + std::fabs(b) : r;
+}
+
+// curve.rs:572-585
+void transit_mask(const std::vector<double>& time, double period, double epoch,
+                  double duration, double margin, std::vector<uint8_t>& out) {
+  out.resize(time.size());
+  for (size_t i = 0; i < time.size(); i++) {
+    const double phase = rem_euclid(time[i] - epoch + period / 2.0, period) - period / 2.0;
+    out[i] = std::fabs(phase) <= (1.0 + margin) * duration / 2.0;
+  }
+}
+
+// peaks.rs:66-83
+size_t event_count(const std::vector<double>& time, double period, double epoch,
+                   double duration, size_t minimum_points) {
+  std::vector<uint8_t> inside;
+  transit_mask(time, period, epoch, duration, 0.0, inside);
+  std::map<int64_t, size_t> counts;
+  for (size_t i = 0; i < time.size(); i++)
+    if (inside[i]) counts[(int64_t)std::round((time[i] - epoch) / period)]++;
+  size_t events = 0;
+  for (const auto& kv : counts) events += kv.second >= minimum_points;
+  return events;
+}
+
+struct Curve {
+  std::vector<double> t, f, s;  // absolute time, flux, sigma
+};
+
+// A family: its curve, its state and the device copy of the state.
+struct Fam {
+  Curve curve;
+  std::vector<double> t, v, w, rho;  // relative time, fixed-point fields
+  double C = 0.0, origin = 0.0, energy = 0.0;
+  fbls_family* dev = nullptr;
+  Fam() = default;
+  Fam(const Fam&) = delete;
+  Fam& operator=(const Fam&) = delete;
+  ~Fam() { fbls_family_free(dev); }
+};
+
+// curve.rs:541-569 then the upload; fails below six observations
+int build_fam(fbls_ctx* c, Curve curve, const double* dur, size_t nd, int correlated,
+              Fam* out) {
+  const size_t n = curve.t.size();
+  std::vector<double> t(n + 1), v(n + 1), w(n + 1), rho(nd);
+  size_t m = 0, segs = 0;
+  double C = 0.0, origin = 0.0, infl = 0.0, energy = 0.0;
+  std::string msg;
+  int st = fbls::family_state(curve.t.data(), curve.f.data(), curve.s.data(), n, dur, nd,
+                              correlated, t.data(), v.data(), w.data(), &m, &C, &origin,
+                              rho.data(), &infl, &segs, &energy, &msg);
+  if (st) return fail(st, "%s", msg.c_str());
+  fbls_family* dev = nullptr;
+  st = fbls_family_upload(c, t.data(), v.data(), w.data(), m, C, origin, &dev);
+  if (st) return st;
+  out->curve = std::move(curve);
+  out->t.assign(t.begin(), t.begin() + m);
+  out->v.resize(m);
+  out->w.resize(m);
+  for (size_t i = 0; i < m; i++) {
+    out->v[i] = std::ldexp((double)dev->vq[i], -dev->ev);
+    out->w[i] = std::ldexp((double)dev->wq[i], -dev->ew);
+  }
+  out->rho = rho;
+  out->C = C;
+  out->origin = origin;
+  out->energy = energy;
+  fbls_family_free(out->dev);
+  out->dev = dev;
+  return FBLS_OK;
+}
+
+Curve masked(const Curve& c, double period, double epoch, double duration, double margin) {
+  std::vector<uint8_t> m;
+  transit_mask(c.t, period, epoch, duration, margin, m);
+  Curve out;
+  for (size_t i = 0; i < c.t.size(); i++)
+    if (!m[i]) {
+      out.t.push_back(c.t[i]);
+      out.f.push_back(c.f[i]);
+      out.s.push_back(c.s[i]);
+    }
+  return out;
+}
+
+// peaks.rs:85-94
+double nearest_rho(const double* dur, const std::vector<double>& rho, size_t nd, double d) {
+  size_t best = 0;
+  for (size_t i = 1; i < nd; i++)
+    if (std::fabs(dur[i] - d) < std::fabs(dur[best] - d)) best = i;
+  return rho[best];
+}
+
+struct Entry {
+  double v;
+  size_t i;
+};
+// BinaryHeap<Entry> order (peaks.rs:367-379): larger score, then smaller row
+struct Below {
+  bool operator()(const Entry& x, const Entry& y) const {
+    return y.v > x.v || (y.v == x.v && y.i < x.i);
+  }
+};
+using Heap = std::priority_queue<Entry, std::vector<Entry>, Below>;
+
+// Selection state of one pass (one family).
+struct Selector {
+  fbls_ctx* c;
+  const fbls_row* table;
+  size_t nrows;
+  const Fam* fam;
+  const double* dur;
+  size_t nd;
+  const std::vector<double>* time_all;
+  std::vector<uint8_t>* excluded;
+  std::vector<size_t> pos;
+  double threshold;
+  const double* d_time_all = nullptr;  // merged times on the device (borrowed)
+  DevBuf d_ex_all, d_ex_fam, d_rows, d_out;
+  ~Selector() {
+    d_ex_all.release();
+    d_ex_fam.release();
+    d_rows.release();
+    d_out.release();
+  }
+
+  // Rescorer::evaluate (peaks.rs:148-240) on the GPU
+  int evaluate(const std::vector<size_t>& idx, std::vector<double>& score,
+               std::vector<double>& required) {
+    const size_t m = idx.size();
+    const size_t n_all = time_all->size(), n = fam->t.size();
+    std::vector<double> rows(3 * m);
+    for (size_t s = 0; s < m; s++) {
+      const fbls_row& r = table[idx[s]];
+      rows[3 * s] = r.period;
+      rows[3 * s + 1] = r.epoch;
+      rows[3 * s + 2] = r.duration;
+    }
+    std::vector<uint8_t> ex_fam(n + 1, 0);
+    for (size_t i = 0; i < n; i++) ex_fam[i] = (*excluded)[pos[i]];
+    int st;
+    if ((st = d_ex_all.ensure(n_all + 1)) || (st = d_ex_fam.ensure(n + 1)) ||
+        (st = d_rows.ensure(rows.size() * 8)) || (st = d_out.ensure(m * 24)))
+      return st;
+    HIP_TRY(hipMemcpyAsync(d_ex_all.ptr, excluded->data(), n_all, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_ex_fam.ptr, ex_fam.data(), n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_rows.ptr, rows.data(), rows.size() * 8, hipMemcpyHostToDevice,
+                           c->stream));
+    fbls::CountArgs ca;
+    ca.time_all = d_time_all;
+    ca.excluded = (const uint8_t*)d_ex_all.ptr;
+    ca.n = (uint32_t)n_all;
+    ca.rows = (const double*)d_rows.ptr;
+    ca.counts = (uint32_t*)((unsigned char*)d_out.ptr + m * 16);
+    HIP_TRY(fbls::count_launch(ca, (uint32_t)m, c->stream));
+    fbls::RescoreArgs ra;
+    ra.t = fam->dev->t_dev;
+    ra.vw = fam->dev->vw_dev;
+    ra.excluded = (const uint8_t*)d_ex_fam.ptr;
+    ra.n = (uint32_t)n;
+    ra.origin = fam->origin;
+    ra.rows = (const double*)d_rows.ptr;
+    ra.sums = (long long*)d_out.ptr;
+    HIP_TRY(fbls::rescore_launch(ra, (uint32_t)m, c->stream));
+    std::vector<unsigned char> out(m * 24);
+    HIP_TRY(hipMemcpyAsync(out.data(), d_out.ptr, m * 24, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const long long* sums = (const long long*)out.data();
+    const uint32_t* counts = (const uint32_t*)(out.data() + m * 16);
+    score.resize(m);
+    required.resize(m);
+    for (size_t s = 0; s < m; s++) {
+      const double total = (double)std::max<uint32_t>(counts[2 * s], 1);
+      const double unshared = (double)counts[2 * s + 1] / total;
+      required[s] = std::fmax(threshold * std::sqrt(unshared), 0.5 * threshold);
+      const double a = std::ldexp((double)sums[2 * s], -fam->dev->ev);
+      const double hp = std::ldexp((double)sums[2 * s + 1], -fam->dev->ew);
+      const double norm = hp;
+      const double q = norm - hp * hp / fam->C;
+      score[s] = (q > 0.0 && a > 0.0)
+                     ? (a / nearest_rho(dur, fam->rho, nd, table[idx[s]].duration)) /
+                           std::sqrt(std::fmax(q, 1e-300))
+                     : -INFINITY;
+    }
+    return FBLS_OK;
+  }
+
+  // peaks.rs:243-299
+  double without_strongest_event(const fbls_row& row) const {
+    struct Acc {
+      double a = 0.0, hp = 0.0, w = 0.0;
+    };
+    std::map<int64_t, Acc> sums;
+    std::vector<std::pair<size_t, int64_t>> members;
+    for (size_t i = 0; i < fam->t.size(); i++) {
+      if ((*excluded)[pos[i]]) continue;
+      const double ta = fam->t[i] + fam->origin;
+      const double phase =
+          rem_euclid(ta - row.epoch + row.period / 2.0, row.period) - row.period / 2.0;
+      if (std::fabs(phase) <= row.duration / 2.0) {
+        const int64_t cycle = (int64_t)std::round((ta - row.epoch) / row.period);
+        Acc& e = sums[cycle];
+        e.a += fam->v[i];
+        e.hp += fam->w[i];
+        e.w += fam->w[i];
+        members.emplace_back(i, cycle);
+      }
+    }
+    if (sums.empty()) return -INFINITY;
+    const double rho = nearest_rho(dur, fam->rho, nd, row.duration);
+    int64_t strongest = 0;
+    double best = -INFINITY;
+    for (const auto& kv : sums) {
+      const Acc& e = kv.second;
+      const double s = e.a / rho / std::sqrt(std::fmax(e.w - e.hp * e.hp / fam->C, 1e-300));
+      if (s > best) {
+        best = s;
+        strongest = kv.first;
+      }
+    }
+    double a = 0.0, hp = 0.0, norm = 0.0;
+    for (const auto& m : members)
+      if (m.second != strongest) {
+        a += fam->v[m.first];
+        hp += fam->w[m.first];
+        norm += fam->w[m.first];
+      }
+    const double q = norm - hp * hp / fam->C;
+    return (q > 0.0 && a > 0.0) ? a / rho / std::sqrt(std::fmax(q, 1e-300)) : -INFINITY;
+  }
+};
+
+// peaks.rs:383-419
+int refresh(Selector& S, std::vector<double>& remaining, std::vector<uint8_t>& fresh,
+            Heap& heap) {
+  const size_t block = 8192;
+  std::vector<size_t> stale;
+  std::vector<Entry> kept;
+  while (!heap.empty()) {
+    const Entry e = heap.top();
+    heap.pop();
+    if (e.v != remaining[e.i]) continue;
+    if (fresh[e.i]) {
+      kept.push_back(e);
+    } else {
+      stale.push_back(e.i);
+      if (stale.size() == block) break;
+    }
+  }
+  for (const Entry& e : kept) heap.push(e);
+  if (stale.empty()) return FBLS_OK;
+  std::vector<double> score, required;
+  if (int st = S.evaluate(stale, score, required)) return st;
+  for (size_t m = 0; m < stale.size(); m++) {
+    const size_t i = stale[m];
+    remaining[i] = score[m] >= required[m] ? score[m] : -INFINITY;
+    fresh[i] = 1;
+    if (std::isfinite(remaining[i])) heap.push(Entry{remaining[i], i});
+  }
+  return FBLS_OK;
+}
+
+void empty_evidence(fbls_peak* p) {
+  const double nan = std::nan("");
+  p->planet_log_odds = p->log_bf_planet_noise = p->log_bf_binary_noise = nan;
+  p->binary_secondary_ratio = p->binary_secondary_phase = nan;
+  p->binary_odd_even_ratio = p->binary_period_days = p->ingress_fraction = nan;
+  p->evidence_epoch_bjd = p->evidence_duration_hours = nan;
+  p->model_preferred = 0;
+  p->binary_diagnostics_complete = 0;
+}
+
+// peaks.rs:308-487 for one family
+int select_peaks(Selector& S, size_t limit, size_t min_events,
+                 std::vector<std::vector<uint8_t>>& masks, size_t pass,
+                 std::vector<fbls_peak>& accepted) {
+  const double margin = 0.5;
+  const size_t max_iterations = 20000;
+  if (S.nrows == 0) return FBLS_OK;
+  const std::vector<double>& time_all = *S.time_all;
+  std::vector<uint8_t>& excluded = *S.excluded;
+  const bool any_ex = std::any_of(excluded.begin(), excluded.end(), [](uint8_t x) { return x; });
+  std::vector<double> remaining(S.nrows);
+  std::vector<uint8_t> fresh(S.nrows, any_ex ? 0 : 1);
+  Heap heap;
+  for (size_t i = 0; i < S.nrows; i++) {
+    const double sc = S.table[i].score;
+    remaining[i] = (std::isfinite(sc) && sc >= S.threshold) ? sc : -INFINITY;
+    if (std::isfinite(remaining[i])) heap.push(Entry{remaining[i], i});
+  }
+  std::vector<uint8_t> mask, wide;
+  size_t done = 0, iterations = 0;
+  while (done < limit && iterations < max_iterations) {
+    iterations++;
+    while (!heap.empty() && heap.top().v != remaining[heap.top().i]) heap.pop();
+    if (heap.empty()) break;
+    const size_t i = heap.top().i;
+    if (!fresh[i]) {
+      if (int st = refresh(S, remaining, fresh, heap)) return st;
+      continue;
+    }
+    heap.pop();
+    const double isolated = remaining[i];
+    remaining[i] = -INFINITY;
+    const fbls_row& row = S.table[i];
+    transit_mask(time_all, row.period, row.epoch, row.duration, 0.0, mask);
+    size_t total = 0;
+    for (uint8_t x : mask) total += x;
+    bool clash = false;
+    for (const auto& other : masks) {
+      size_t both = 0;
+      for (size_t j = 0; j < mask.size(); j++) both += mask[j] && other[j];
+      if ((total ? (double)both / (double)total : 0.0) >= 0.5) {
+        clash = true;
+        break;
+      }
+    }
+    if (clash) continue;
+    const size_t events = event_count(time_all, row.period, row.epoch, row.duration, 3);
+    if (events < min_events) continue;
+    const double leftover = S.without_strongest_event(row);
+    if (min_events > 1 && leftover < 0.5 * S.threshold) continue;
+    transit_mask(time_all, row.period, row.epoch, row.duration, margin, wide);
+    for (size_t j = 0; j < excluded.size(); j++) excluded[j] |= wide[j];
+    masks.push_back(wide);
+    fbls_peak p;
+    std::memset(&p, 0, sizeof(p));
+    p.period_days = row.period;
+    p.epoch_bjd = row.epoch;
+    p.duration_hours = 24.0 * row.duration;
+    p.snr = isolated;
+    p.snr_folded = row.score;
+    p.snr_without_strongest_event = leftover;
+    p.depth = row.depth;
+    p.depth_ppm = 1e6 * row.depth;
+    p.events = events;
+    p.family = 0;
+    p.pass = pass;
+    p.row = i;
+    empty_evidence(&p);
+    accepted.push_back(p);
+    done++;
+    std::fill(fresh.begin(), fresh.end(), 0);
+    if (int st = refresh(S, remaining, fresh, heap)) return st;
+  }
+  return FBLS_OK;
+}
+
+int check_pipeline(const fbls_pipeline_cfg* cfg) {
+  if (!cfg || !cfg->durations || cfg->nd == 0) return fail(FBLS_E_ARG, "no durations");
+  if (cfg->passes < 1) return fail(FBLS_E_ARG, "passes must be >= 1");
+  if (!cfg->bank.ingress || cfg->bank.ns == 0) return fail(FBLS_E_ARG, "no shapes");
+  return FBLS_OK;
+}
+
+}  // namespace
+
+extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const double* flux,
+                                    const double* err, size_t n,
+                                    const fbls_pipeline_cfg* cfg, fbls_peak* peaks,
+                                    size_t peaks_cap, size_t* npeaks,
+                                    fbls_pass_stats* passes, size_t passes_cap,
+                                    size_t* npasses) {
+  if (!c || (n && (!time || !flux || !err)) || !npeaks || !npasses)
+    return fail(FBLS_E_ARG, "null argument");
+  *npeaks = 0;
+  *npasses = 0;
+  if (int st = check_pipeline(cfg)) return st;
+  const double* dur = cfg->durations;
+  const size_t nd = cfg->nd;
+  // pipeline.rs:47-61
+  std::vector<size_t> order;
+  for (size_t i = 0; i < n; i++)
+    if (std::isfinite(time[i]) && std::isfinite(flux[i]) && std::isfinite(err[i]) && err[i] > 0.0)
+      order.push_back(i);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return time[a] < time[b]; });
+  Curve curve;
+  for (size_t i : order) {
+    curve.t.push_back(time[i]);
+    curve.f.push_back(flux[i]);
+    curve.s.push_back(err[i]);
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  auto cur = std::make_unique<Fam>();
+  if (int st = build_fam(c, curve, dur, nd, cfg->correlated, cur.get())) return st;
+  // merged_time (pipeline.rs:16-24)
+  std::vector<double> time_all = cur->curve.t;
+  std::sort(time_all.begin(), time_all.end());
+  time_all.erase(std::unique(time_all.begin(), time_all.end()), time_all.end());
+  if (time_all.size() < 6) return fail(FBLS_E_DATA, "at least six usable observations are required");
+  const Curve original = cur->curve;
+  std::vector<uint8_t> excluded(time_all.size(), 0);
+  std::vector<std::vector<uint8_t>> masks;
+  std::vector<fbls_peak> found;
+  struct Guard {
+    DevBuf b;
+    ~Guard() { b.release(); }
+  } g;
+  DevBuf& d_time_all = g.b;
+  if (int st = d_time_all.ensure(time_all.size() * 8)) return st;
+  HIP_TRY(hipMemcpy(d_time_all.ptr, time_all.data(), time_all.size() * 8, hipMemcpyHostToDevice));
+  fbls_bank bank = cfg->bank;
+  for (size_t pass = 0; pass < cfg->passes; pass++) {
+    const double t0 = now_s();
+    // signal_search of the family (scan.rs:413-473)
+    const double d0 = *std::min_element(dur, dur + nd);
+    const double hi = std::min(cfg->search.period_max, cur->dev->t_last);
+    const double lo = std::max(cfg->search.period_min, 4.0 * d0 * (1.0 + 1e-9));
+    size_t cap = 1;
+    if (hi > lo)
+      cap = std::max<size_t>(1, fbls_period_grid_count(lo, hi, cur->dev->t_last, d0,
+                                                       cfg->search.period_oversample));
+    std::vector<fbls_row> table(cap);
+    std::vector<fbls_scan_stats> bstats(64);
+    size_t nrows = 0, nbands = 0, grid_count = 0;
+    bank.durations = dur;
+    bank.rho = cur->rho.data();
+    bank.nd = nd;
+    int st = fbls_signal_search(c, cur->dev, dur, cur->rho.data(), nd, &bank, &cfg->search,
+                                table.data(), cap, &nrows, bstats.data(), bstats.size(),
+                                &nbands, nullptr, 0, &grid_count);
+    if (st) return st;
+    const double t1 = now_s();
+    Selector S;
+    S.c = c;
+    S.table = table.data();
+    S.nrows = nrows;
+    S.fam = cur.get();
+    S.dur = dur;
+    S.nd = nd;
+    S.time_all = &time_all;
+    S.excluded = &excluded;
+    S.threshold = cfg->bank.threshold;
+    S.pos.resize(cur->t.size());
+    for (size_t i = 0; i < cur->t.size(); i++) {
+      const size_t p = std::lower_bound(time_all.begin(), time_all.end(), cur->t[i] + cur->origin) -
+                       time_all.begin();
+      S.pos[i] = std::min(p, time_all.size() - 1);
+    }
+    S.d_time_all = (const double*)d_time_all.ptr;
+    const size_t before = found.size();
+    st = select_peaks(S, cfg->max_candidates - before, cfg->min_events, masks, pass + 1, found);
+    if (st) return st;
+    if (*npasses < passes_cap && passes) {
+      fbls_pass_stats& ps = passes[*npasses];
+      ps.pass = pass + 1;
+      ps.rows = nrows;
+      ps.periods = grid_count;
+      ps.accepted = found.size() - before;
+      ps.seconds_scan = t1 - t0;
+      ps.seconds_select = now_s() - t1;
+    }
+    (*npasses)++;
+    if (found.size() == before) break;
+    const fbls_peak lead = found[before];
+    if (pass + 1 < cfg->passes) {
+      Curve next = masked(cur->curve, lead.period_days, lead.epoch_bjd,
+                          lead.duration_hours / 24.0, 0.5);
+      auto fam = std::make_unique<Fam>();
+      if ((st = build_fam(c, std::move(next), dur, nd, cfg->correlated, fam.get()))) return st;
+      cur = std::move(fam);
+      if (cur->curve.t.size() < 6) break;
+    }
+  }
+  // model comparison of the strongest peaks (pipeline.rs:169-221)
+  if (cfg->model_comparison && !found.empty()) {
+    std::vector<size_t> ord(found.size());
+    for (size_t i = 0; i < ord.size(); i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](size_t a, size_t b) { return found[a].snr > found[b].snr; });
+    for (size_t oi = 0; oi < ord.size() && oi < cfg->evidence_candidates; oi++) {
+      fbls_peak& p = found[ord[oi]];
+      Curve cv = original;
+      for (const fbls_peak& o : found) {
+        if (o.row == p.row && o.pass == p.pass && o.family == p.family) continue;
+        cv = masked(cv, o.period_days, o.epoch_bjd, o.duration_hours / 24.0, 0.5);
+      }
+      if (cv.t.size() < 6) continue;
+      Fam s;
+      if (int st = build_fam(c, std::move(cv), dur, nd, cfg->correlated, &s)) return st;
+      fbls_evidence_row rows[2];
+      size_t nr = 0;
+      if (int st = fbls_compare(c, s.dev, p.period_days, dur, s.rho.data(), nd, s.energy,
+                                &cfg->evidence, rows, &nr, nullptr))
+        return st;
+      const fbls_evidence_row* r = nullptr;
+      for (size_t q = 0; q < nr && !r; q++)
+        if (rows[q].selection_flags & 1) r = &rows[q];
+      if (!r && nr) r = &rows[0];
+      if (!r) continue;
+      auto fin = [](double x) { return std::isfinite(x) ? x : std::nan(""); };
+      p.planet_log_odds = fin(r->planet_log_odds);
+      p.log_bf_planet_noise = fin(r->log_bf_planet_noise);
+      p.log_bf_binary_noise = fin(r->log_bf_binary_noise);
+      p.binary_secondary_ratio = fin(r->binary_secondary_ratio);
+      p.binary_secondary_phase = fin(r->binary_secondary_phase);
+      p.binary_odd_even_ratio = fin(r->binary_odd_even_ratio);
+      p.binary_period_days = fin(r->binary_period_days);
+      p.ingress_fraction = fin(r->ingress_fraction);
+      p.evidence_epoch_bjd = r->epoch;
+      p.evidence_duration_hours = 24.0 * r->duration;
+      p.model_preferred = r->planet_log_odds >= cfg->planet_log_odds_min;
+      p.binary_diagnostics_complete =
+          std::isfinite(r->binary_secondary_ratio) && std::isfinite(r->binary_secondary_phase) &&
+          std::isfinite(r->binary_odd_even_ratio) && std::isfinite(r->binary_period_days);
+    }
+  }
+  *npeaks = found.size();
+  if (peaks) std::memcpy(peaks, found.data(), std::min(found.size(), peaks_cap) * sizeof(fbls_peak));
+  if (found.size() > peaks_cap)
+    return fail(FBLS_E_ARG, "peaks capacity %zu < %zu", peaks_cap, found.size());
+  return FBLS_OK;
+}

@@ -233,6 +233,54 @@ int fbls_compare(fbls_ctx* ctx, const fbls_family* fam, double period,
 /* [host] `parameters` (evidence.rs:665-689): the 30 model constants. */
 int fbls_evidence_parameters(const fbls_evidence_cfg* cfg, double out[30]);
 
+/* -- multi-pass search ---------------------------------------------------- */
+
+/* `Peak` (src/select/peaks.rs:19-45).  Options that are None are NaN. */
+typedef struct {
+  double period_days, epoch_bjd, duration_hours, snr, snr_folded,
+      snr_without_strongest_event, depth, depth_ppm;
+  uint64_t events, family, pass, row;
+  double planet_log_odds, log_bf_planet_noise, log_bf_binary_noise,
+      binary_secondary_ratio, binary_secondary_phase, binary_odd_even_ratio,
+      binary_period_days, ingress_fraction, evidence_epoch_bjd,
+      evidence_duration_hours;
+  uint32_t model_preferred, binary_diagnostics_complete;
+} fbls_peak;
+
+/* One pass of the search (the `passes[]` statistics of pipeline.rs:143). */
+typedef struct {
+  uint64_t pass, rows, periods, accepted;
+  double seconds_scan, seconds_select;
+} fbls_pass_stats;
+
+/* SearchConfig fields of the prepared search (config.rs:5-88). */
+typedef struct {
+  const double* durations;   /* days, sorted unique (`durations_days`) */
+  size_t nd;
+  int correlated;            /* correlated_noise */
+  fbls_bank bank;            /* shapes, layers, threshold (also the selection
+                                threshold), gate, prescan, budget; its
+                                durations/rho are ignored */
+  fbls_search_cfg search;
+  size_t max_candidates, passes, min_events, evidence_candidates;
+  int model_comparison;
+  double planet_log_odds_min;
+  fbls_evidence_cfg evidence;
+} fbls_pipeline_cfg;
+
+/* `search_prepared` + `search_families` (pipeline.rs:41-221) without the
+ * single-event scan: keeps finite points with positive errors sorted by
+ * time, builds the family state, then per pass runs the signal search,
+ * the greedy peak selection with GPU rescoring (peaks.rs:308-487) and the
+ * masking of the lead peak; finally the model comparison of the
+ * evidence_candidates strongest peaks.  Peaks in acceptance order
+ * (capacity peaks_cap, *npeaks is the total); pass statistics capacity
+ * passes_cap. */
+int fbls_search_prepared(fbls_ctx* ctx, const double* time, const double* flux,
+                         const double* err, size_t n, const fbls_pipeline_cfg* cfg,
+                         fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
+                         fbls_pass_stats* passes, size_t passes_cap, size_t* npasses);
+
 /* -- host helpers [host] ------------------------------------------------ */
 
 /* [host] `period_grid` (grid.rs:5-27): count first, then fill. */

@@ -75,6 +75,40 @@ class EvidenceRow(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class OrcPeak(ctypes.Structure):
+    _fields_ = ([(k, _d) for k in (
+        "period_days", "epoch_bjd", "duration_hours", "snr", "snr_folded",
+        "snr_without_strongest_event", "depth", "depth_ppm")]
+        + [(k, _u64) for k in ("events", "family", "pass_", "row")]
+        + [(k, _d) for k in (
+            "planet_log_odds", "log_bf_planet_noise", "log_bf_binary_noise",
+            "binary_secondary_ratio", "binary_secondary_phase",
+            "binary_odd_even_ratio", "binary_period_days", "ingress_fraction",
+            "evidence_epoch_bjd", "evidence_duration_hours")]
+        + [("model_preferred", _u32), ("binary_diagnostics_complete", _u32)])
+
+    def as_dict(self):
+        d = {k.rstrip("_"): getattr(self, k) for k, _ in self._fields_}
+        d["model_preferred"] = bool(d["model_preferred"])
+        d["binary_diagnostics_complete"] = bool(d["binary_diagnostics_complete"])
+        return d
+
+
+class OrcPassStats(ctypes.Structure):
+    _fields_ = [("pass_", _u64), ("rows", _u64), ("periods", _u64), ("accepted", _u64)]
+
+    def as_dict(self):
+        return {k.rstrip("_"): getattr(self, k) for k, _ in self._fields_}
+
+
+class OrcPipelineCfg(ctypes.Structure):
+    _fields_ = [("durations", _P(_d)), ("nd", _sz), ("ingress", _P(_d)), ("ns", _sz),
+                ("correlated", ctypes.c_int), ("bank", BankCfg), ("search", SearchCfg),
+                ("max_candidates", _sz), ("passes", _sz), ("min_events", _sz),
+                ("evidence_candidates", _sz), ("model_comparison", ctypes.c_int),
+                ("planet_log_odds_min", _d), ("evidence", EvidenceCfg)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -113,6 +147,18 @@ def lib():
         L.orc_evidence_use_libm.restype = None
         L.orc_evidence_parameters.argtypes = [_P(EvidenceCfg), _P(_d)]
         L.orc_evidence_parameters.restype = None
+        L.orc_search_prepared.argtypes = [
+            _P(_d), _P(_d), _P(_d), _sz, _P(OrcPipelineCfg), _P(OrcPeak), _sz,
+            _P(_sz), _P(OrcPassStats), _P(_sz)]
+        L.orc_transit_mask.argtypes = [_P(_d), _sz, _d, _d, _d, _d, _P(ctypes.c_uint8)]
+        L.orc_transit_mask.restype = None
+        L.orc_event_count.argtypes = [_P(_d), _sz, _d, _d, _d, _sz]
+        L.orc_event_count.restype = _sz
+        L.orc_rescore_row.argtypes = [_P(_d), _P(_d), _P(_d), _P(ctypes.c_uint8), _sz,
+                                      _d, _d, _d, _d, _P(_d)]
+        L.orc_rescore_row.restype = None
+        L.orc_count_row.argtypes = [_P(_d), _P(ctypes.c_uint8), _sz, _d, _d, _d, _P(_u32)]
+        L.orc_count_row.restype = None
         L.orc_compare_search_ex.argtypes = [
             _P(_d), _P(_d), _P(_d), _sz, _d, _d, _d, _P(_d), _P(_d), _sz, _d,
             _P(EvidenceCfg), _P(EvidenceRow), _P(_sz), _P(_d), _sz, _P(_d),
@@ -417,3 +463,65 @@ def compare_search(t, v, w, baseline_norm, origin, residual_energy,
     return out, {"screen": screen[:4 * k].reshape(k, 4),
                  "parts": parts[:30 * k].reshape(k, 2, 15),
                  "floors": floors, "tiles": k, "nb": nb.value}
+
+
+# ------------------------------------------------------------ pipeline
+
+def transit_mask(time, period, epoch, duration, margin=0.0):
+    time = _f64(time)
+    out = np.zeros(len(time), dtype=np.uint8)
+    lib().orc_transit_mask(_dp(time), len(time), period, epoch, duration, margin,
+                           out.ctypes.data_as(_P(ctypes.c_uint8)))
+    return out.astype(bool)
+
+
+def event_count(time, period, epoch, duration, minimum_points=3):
+    time = _f64(time)
+    return lib().orc_event_count(_dp(time), len(time), period, epoch, duration,
+                                 minimum_points)
+
+
+def rescore_row(time, v, w, excluded, origin, period, epoch, duration):
+    time, v, w = _f64(time), _f64(v), _f64(w)
+    ex = np.ascontiguousarray(excluded, dtype=np.uint8)
+    out = np.zeros(3)
+    lib().orc_rescore_row(_dp(time), _dp(v), _dp(w), ex.ctypes.data_as(_P(ctypes.c_uint8)),
+                          len(time), origin, period, epoch, duration, _dp(out))
+    return out
+
+
+def count_row(time_all, excluded, period, epoch, duration):
+    t = _f64(time_all)
+    ex = np.ascontiguousarray(excluded, dtype=np.uint8)
+    out = np.zeros(2, dtype=np.uint32)
+    lib().orc_count_row(_dp(t), ex.ctypes.data_as(_P(ctypes.c_uint8)), len(t), period,
+                        epoch, duration, out.ctypes.data_as(_P(_u32)))
+    return out
+
+
+def search_prepared(time, flux, err, cfg):
+    """search_prepared + search_families (pipeline.rs:41-221, no single
+    events): (peaks in acceptance order, pass statistics)."""
+    time, flux, err = _f64(time), _f64(flux), _f64(err)
+    d = _f64(cfg.durations_days())
+    ing = _f64(cfg.ingress_fractions)
+    ev = evidence_cfg(cfg)
+    c = OrcPipelineCfg(
+        durations=_dp(d), nd=len(d), ingress=_dp(ing), ns=len(ing),
+        correlated=int(bool(cfg.correlated_noise)), bank=_bank_cfg(cfg),
+        search=SearchCfg(cfg.period_min, cfg.period_max, cfg.period_oversample),
+        max_candidates=cfg.max_candidates, passes=cfg.passes,
+        min_events=cfg.min_events, evidence_candidates=cfg.evidence_candidates,
+        model_comparison=int(bool(cfg.model_comparison)),
+        planet_log_odds_min=cfg.planet_log_odds_min, evidence=ev)
+    cap = max(cfg.max_candidates, 1)
+    peaks = (OrcPeak * cap)()
+    passes = (OrcPassStats * max(cfg.passes, 1))()
+    npk, npa = _sz(), _sz()
+    st = lib().orc_search_prepared(_dp(time), _dp(flux), _dp(err), len(time),
+                                   ctypes.byref(c), peaks, cap, ctypes.byref(npk),
+                                   passes, ctypes.byref(npa))
+    if st:
+        raise ValueError(f"search_prepared status {st}")
+    return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
+            [passes[i].as_dict() for i in range(npa.value)])

@@ -197,6 +197,62 @@ int orc_compare_search_ex(const double* t, const double* v, const double* w,
                           double* parts, size_t parts_cap, double* floors,
                           size_t* tiles_out, size_t* nb_out);
 
+/* ------------------------------------------------------------ pipeline
+ * Multi-pass prepared search with peak selection (oracle/pipeline.c). */
+
+/* curve.rs:572-585 */
+void orc_transit_mask(const double* time, size_t n, double period,
+                      double epoch, double duration, double margin,
+                      uint8_t* out);
+/* peaks.rs:66-83 */
+size_t orc_event_count(const double* time, size_t n, double period,
+                       double epoch, double duration, size_t minimum_points);
+/* kernels.rs:934-989 for one row: {a, hp, norm} */
+void orc_rescore_row(const double* time, const double* v, const double* w,
+                     const uint8_t* excluded, size_t n, double origin,
+                     double period, double epoch, double duration, double* out3);
+/* kernels.rs:992-1042 for one row: {in transit, in transit and not excluded} */
+void orc_count_row(const double* time_all, const uint8_t* excluded, size_t n,
+                   double period, double epoch, double duration,
+                   uint32_t* out2);
+
+/* Peak (peaks.rs:19-45); None is NaN, flags are 0/1. */
+typedef struct {
+  double period_days, epoch_bjd, duration_hours, snr, snr_folded,
+      snr_without_strongest_event, depth, depth_ppm;
+  uint64_t events, family, pass, row;
+  double planet_log_odds, log_bf_planet_noise, log_bf_binary_noise,
+      binary_secondary_ratio, binary_secondary_phase, binary_odd_even_ratio,
+      binary_period_days, ingress_fraction, evidence_epoch_bjd,
+      evidence_duration_hours;
+  uint32_t model_preferred, binary_diagnostics_complete;
+} orc_peak;
+
+typedef struct {
+  uint64_t pass, rows, periods, accepted;
+} orc_pass_stats;
+
+typedef struct {
+  const double* durations; /* days, sorted unique (config.rs durations_days) */
+  size_t nd;
+  const double* ingress;
+  size_t ns;
+  int correlated;
+  orc_bank_cfg bank;       /* bank.threshold is the selection threshold */
+  orc_search_cfg search;
+  size_t max_candidates, passes, min_events, evidence_candidates;
+  int model_comparison;
+  double planet_log_odds_min;
+  orc_evidence_cfg evidence;
+} orc_pipeline_cfg;
+
+/* pipeline.rs:41-221 (search_prepared without single events): peaks in
+ * acceptance order (capacity cap), pass statistics (capacity passes). */
+int orc_search_prepared(const double* time, const double* flux,
+                        const double* err, size_t n, const orc_pipeline_cfg* cfg,
+                        orc_peak* peaks, size_t cap, size_t* npeaks,
+                        orc_pass_stats* pstats, size_t* npasses);
+
 /* survey/numeric.rs:345-364 */
 uint64_t orc_rng_new(uint64_t seed);
 double orc_rng_uniform(uint64_t* state);

@@ -1,0 +1,733 @@
+/*
+ * FP64 CPU oracle of the multi-pass prepared search: signal search per
+ * pass, greedy peak selection with event-overlap rescoring, masking of the
+ * lead peak, and the model comparison of the strongest peaks.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the library's pipeline.
+ * Restates (isakruas/fbls):
+ *   transit_mask / mask_curve           src/prep/curve.rs:572-603
+ *   event_count, nearest_rho, positions src/select/peaks.rs:66-103
+ *   rescore_rows / count_rows           src/search/kernels.rs:934-1042
+ *   Rescorer::evaluate                  src/select/peaks.rs:148-240
+ *   without_strongest_event             src/select/peaks.rs:243-299
+ *   select_peaks                        src/select/peaks.rs:308-487
+ *   search_prepared / search_families   src/pipeline.rs:41-221
+ * The fields v and w are quantised exactly as the library stores them
+ * (orc_quantize), so every sum here equals the device's.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fbls_oracle.h"
+
+/* ------------------------------------------------------------ helpers */
+
+static double rem_euclid(double a, double b) {
+  double r = fmod(a, b);
+  return r < 0.0 ? r + fabs(b) : r;
+}
+
+/* curve.rs:572-585 */
+void orc_transit_mask(const double* time, size_t n, double period,
+                      double epoch, double duration, double margin,
+                      uint8_t* out) {
+  for (size_t i = 0; i < n; i++) {
+    double phase = rem_euclid(time[i] - epoch + period / 2.0, period) - period / 2.0;
+    out[i] = fabs(phase) <= (1.0 + margin) * duration / 2.0;
+  }
+}
+
+static int cmp_i64(const void* a, const void* b) {
+  int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+  return x < y ? -1 : x > y;
+}
+
+/* peaks.rs:66-83 */
+size_t orc_event_count(const double* time, size_t n, double period,
+                       double epoch, double duration, size_t minimum_points) {
+  uint8_t* inside = (uint8_t*)malloc(n ? n : 1);
+  orc_transit_mask(time, n, period, epoch, duration, 0.0, inside);
+  int64_t* cyc = (int64_t*)malloc((n ? n : 1) * sizeof(int64_t));
+  size_t m = 0;
+  for (size_t i = 0; i < n; i++)
+    if (inside[i]) cyc[m++] = (int64_t)round((time[i] - epoch) / period);
+  qsort(cyc, m, sizeof(int64_t), cmp_i64);
+  size_t events = 0;
+  for (size_t i = 0; i < m;) {
+    size_t j = i;
+    while (j < m && cyc[j] == cyc[i]) j++;
+    if (j - i >= minimum_points) events++;
+    i = j;
+  }
+  free(inside);
+  free(cyc);
+  return events;
+}
+
+/* kernels.rs:934-989 (one row, sequential) */
+void orc_rescore_row(const double* time, const double* v, const double* w,
+                     const uint8_t* excluded, size_t n, double origin,
+                     double period, double epoch, double duration, double* out3) {
+  double inverse = 1.0 / period, half = duration / 2.0;
+  double a = 0.0, hp = 0.0, norm = 0.0;
+  for (size_t i = 0; i < n; i++) {
+    if (excluded[i]) continue;
+    double x = (time[i] + origin - epoch) * inverse + 0.5;
+    double phase = (x - floor(x) - 0.5) * period;
+    if (fabs(phase) <= half) {
+      a += v[i];
+      hp += w[i];
+      norm += w[i];
+    }
+  }
+  out3[0] = a;
+  out3[1] = hp;
+  out3[2] = norm;
+}
+
+/* kernels.rs:992-1042 (one row, sequential) */
+void orc_count_row(const double* time_all, const uint8_t* excluded, size_t n,
+                   double period, double epoch, double duration,
+                   uint32_t* out2) {
+  double inverse = 1.0 / period, half = duration / 2.0;
+  uint32_t total = 0, unshared = 0;
+  for (size_t i = 0; i < n; i++) {
+    double x = (time_all[i] - epoch) * inverse + 0.5;
+    double phase = (x - floor(x) - 0.5) * period;
+    if (fabs(phase) <= half) {
+      total++;
+      unshared += excluded[i] == 0;
+    }
+  }
+  out2[0] = total;
+  out2[1] = unshared;
+}
+
+/* one prepared family: the curve it was built from and its state */
+typedef struct {
+  double *ct, *cf, *cs; /* curve: absolute time, flux, sigma */
+  size_t cn;
+  double *t, *v, *w; /* state: relative time, quantised fields */
+  size_t n;
+  double C, origin, energy;
+  double* rho;
+  size_t nd;
+} fam_t;
+
+static void fam_free(fam_t* f) {
+  free(f->ct);
+  free(f->cf);
+  free(f->cs);
+  free(f->t);
+  free(f->v);
+  free(f->w);
+  free(f->rho);
+  memset(f, 0, sizeof(*f));
+}
+
+/* curve.rs:541-569 plus the library's fixed-point step */
+static int fam_build(const double* ct, const double* cf, const double* cs,
+                     size_t cn, const double* durations, size_t nd,
+                     int correlated, fam_t* f) {
+  memset(f, 0, sizeof(*f));
+  f->cn = cn;
+  f->ct = (double*)malloc((cn + 1) * 8);
+  f->cf = (double*)malloc((cn + 1) * 8);
+  f->cs = (double*)malloc((cn + 1) * 8);
+  memcpy(f->ct, ct, cn * 8);
+  memcpy(f->cf, cf, cn * 8);
+  memcpy(f->cs, cs, cn * 8);
+  double* flux1 = (double*)malloc((cn + 1) * 8);
+  for (size_t i = 0; i < cn; i++) flux1[i] = 1.0 + cf[i];
+  double* dt = (double*)malloc((cn + 1) * 8);
+  double* dd = (double*)malloc((cn + 1) * 8);
+  double* dv = (double*)malloc((cn + 1) * 8);
+  size_t m = 0, segs = 0;
+  double infl = 0.0;
+  int st = orc_fit_segment_noise(ct, flux1, cs, cn, 0.75, dt, dd, dv, &m,
+                                 &f->origin, &infl, &segs);
+  free(flux1);
+  if (st) {
+    free(dt);
+    free(dd);
+    free(dv);
+    return st;
+  }
+  f->n = m;
+  f->t = dt;
+  f->v = (double*)malloc((m + 1) * 8);
+  f->w = (double*)malloc((m + 1) * 8);
+  st = orc_prepare_ready(dd, dv, m, f->v, f->w, &f->C, &f->energy);
+  if (st) {
+    free(dd);
+    free(dv);
+    return st;
+  }
+  f->nd = nd;
+  f->rho = (double*)malloc((nd + 1) * 8);
+  if (correlated) {
+    double* resid = (double*)malloc((m + 1) * 8);
+    double* sig = (double*)malloc((m + 1) * 8);
+    for (size_t i = 0; i < m; i++) {
+      resid[i] = -dd[i];
+      sig[i] = sqrt(dv[i]);
+    }
+    orc_correlated_noise_factors(dt, resid, sig, m, durations, nd, f->rho);
+    free(resid);
+    free(sig);
+  } else {
+    for (size_t i = 0; i < nd; i++) f->rho[i] = 1.0;
+  }
+  free(dd);
+  free(dv);
+  /* fixed point as the library uploads it */
+  int32_t* q = (int32_t*)malloc((m + 1) * 4);
+  int e = orc_quantize(f->v, m, q);
+  for (size_t i = 0; i < m; i++) f->v[i] = ldexp((double)q[i], -e);
+  e = orc_quantize(f->w, m, q);
+  for (size_t i = 0; i < m; i++) f->w[i] = ldexp((double)q[i], -e);
+  free(q);
+  return 0;
+}
+
+/* curve.rs:587-603: the family rebuilt from the curve without the masked
+ * observations */
+static int fam_masked(const fam_t* src, double period, double epoch,
+                      double duration, double margin, const double* durations,
+                      size_t nd, int correlated, fam_t* out, size_t* kept) {
+  uint8_t* m = (uint8_t*)malloc(src->cn + 1);
+  orc_transit_mask(src->ct, src->cn, period, epoch, duration, margin, m);
+  double* t = (double*)malloc((src->cn + 1) * 8);
+  double* fl = (double*)malloc((src->cn + 1) * 8);
+  double* s = (double*)malloc((src->cn + 1) * 8);
+  size_t k = 0;
+  for (size_t i = 0; i < src->cn; i++)
+    if (!m[i]) {
+      t[k] = src->ct[i];
+      fl[k] = src->cf[i];
+      s[k] = src->cs[i];
+      k++;
+    }
+  free(m);
+  *kept = k;
+  /* family_state fails below six observations (curve.rs:429-431) */
+  int st = fam_build(t, fl, s, k, durations, nd, correlated, out);
+  free(t);
+  free(fl);
+  free(s);
+  return st;
+}
+
+/* peaks.rs:85-94 */
+static double nearest_rho(const double* durations, const double* rho, size_t nd,
+                          double duration) {
+  size_t best = 0;
+  for (size_t i = 1; i < nd; i++)
+    if (fabs(durations[i] - duration) < fabs(durations[best] - duration)) best = i;
+  return rho[best];
+}
+
+static size_t lower_bound(const double* v, size_t n, double x) {
+  size_t lo = 0, hi = n;
+  while (lo < hi) {
+    size_t mid = lo + (hi - lo) / 2;
+    if (v[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+/* ---------------------------------------------------------- selection */
+
+typedef struct {
+  double v;
+  size_t i;
+} entry_t;
+
+typedef struct {
+  entry_t* a;
+  size_t n, cap;
+} heap_t;
+
+/* BinaryHeap<Entry> order (peaks.rs:367-379): larger score first, then the
+ * smaller row index */
+static int above(entry_t x, entry_t y) { return x.v > y.v || (x.v == y.v && x.i < y.i); }
+
+static void heap_push(heap_t* h, entry_t e) {
+  if (h->n == h->cap) {
+    h->cap = h->cap ? 2 * h->cap : 64;
+    h->a = (entry_t*)realloc(h->a, h->cap * sizeof(entry_t));
+  }
+  size_t k = h->n++;
+  h->a[k] = e;
+  while (k > 0) {
+    size_t p = (k - 1) / 2;
+    if (!above(h->a[k], h->a[p])) break;
+    entry_t t = h->a[k];
+    h->a[k] = h->a[p];
+    h->a[p] = t;
+    k = p;
+  }
+}
+
+static entry_t heap_pop(heap_t* h) {
+  entry_t top = h->a[0];
+  h->a[0] = h->a[--h->n];
+  size_t k = 0;
+  for (;;) {
+    size_t l = 2 * k + 1, r = l + 1, b = k;
+    if (l < h->n && above(h->a[l], h->a[b])) b = l;
+    if (r < h->n && above(h->a[r], h->a[b])) b = r;
+    if (b == k) break;
+    entry_t t = h->a[k];
+    h->a[k] = h->a[b];
+    h->a[b] = t;
+    k = b;
+  }
+  return top;
+}
+
+typedef struct {
+  const orc_row* table;
+  size_t nrows;
+  const fam_t* fam; /* one family (prepared mode) */
+  const size_t* pos;
+  const double* time_all;
+  size_t n_all;
+  const double* durations;
+  size_t nd;
+  double threshold;
+  uint8_t* excluded;
+} sel_ctx;
+
+/* peaks.rs:148-240 */
+static void evaluate(const sel_ctx* S, const size_t* idx, size_t m, double* score,
+                     double* required) {
+  const fam_t* f = S->fam;
+  uint8_t* ex_fam = (uint8_t*)malloc(f->n + 1);
+  for (size_t i = 0; i < f->n; i++) ex_fam[i] = S->excluded[S->pos[i]];
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long s = 0; s < (long long)m; s++) {
+    const orc_row* r = &S->table[idx[s]];
+    uint32_t c2[2];
+    orc_count_row(S->time_all, S->excluded, S->n_all, r->period, r->epoch, r->duration, c2);
+    double total = (double)(c2[0] > 1 ? c2[0] : 1);
+    double unshared = (double)c2[1] / total;
+    required[s] = fmax(S->threshold * sqrt(unshared), 0.5 * S->threshold);
+    double sums[3];
+    orc_rescore_row(f->t, f->v, f->w, ex_fam, f->n, f->origin, r->period, r->epoch,
+                    r->duration, sums);
+    double q = sums[2] - sums[1] * sums[1] / f->C;
+    score[s] = (q > 0.0 && sums[0] > 0.0)
+                   ? (sums[0] / nearest_rho(S->durations, f->rho, S->nd, r->duration)) /
+                         sqrt(fmax(q, 1e-300))
+                   : -INFINITY;
+  }
+  free(ex_fam);
+}
+
+typedef struct {
+  int64_t cycle;
+  double a, hp, w;
+} cyc_t;
+
+/* peaks.rs:243-299 */
+static double without_strongest_event(const sel_ctx* S, const orc_row* row) {
+  const fam_t* f = S->fam;
+  cyc_t* c = (cyc_t*)malloc((f->n + 1) * sizeof(cyc_t));
+  size_t* mem_i = (size_t*)malloc((f->n + 1) * sizeof(size_t));
+  int64_t* mem_c = (int64_t*)malloc((f->n + 1) * sizeof(int64_t));
+  size_t nc = 0, nm = 0;
+  for (size_t i = 0; i < f->n; i++) {
+    if (S->excluded[S->pos[i]]) continue;
+    double ta = f->t[i] + f->origin;
+    double phase = rem_euclid(ta - row->epoch + row->period / 2.0, row->period) - row->period / 2.0;
+    if (fabs(phase) <= row->duration / 2.0) {
+      int64_t cycle = (int64_t)round((ta - row->epoch) / row->period);
+      size_t k = 0;
+      while (k < nc && c[k].cycle != cycle) k++;
+      if (k == nc) {
+        c[nc].cycle = cycle;
+        c[nc].a = c[nc].hp = c[nc].w = 0.0;
+        nc++;
+      }
+      c[k].a += f->v[i];
+      c[k].hp += f->w[i];
+      c[k].w += f->w[i];
+      mem_i[nm] = i;
+      mem_c[nm] = cycle;
+      nm++;
+    }
+  }
+  double value = -INFINITY;
+  if (nc) {
+    double rho = nearest_rho(S->durations, f->rho, S->nd, row->duration);
+    /* BTreeMap order: ascending cycle; the first strict maximum wins */
+    int64_t strongest = 0;
+    double best = -INFINITY;
+    int64_t prev = INT64_MIN;
+    for (size_t done = 0; done < nc; done++) {
+      size_t k = nc;
+      for (size_t j = 0; j < nc; j++)
+        if (c[j].cycle > prev && (k == nc || c[j].cycle < c[k].cycle)) k = j;
+      prev = c[k].cycle;
+      double s = c[k].a / rho / sqrt(fmax(c[k].w - c[k].hp * c[k].hp / f->C, 1e-300));
+      if (s > best) {
+        best = s;
+        strongest = c[k].cycle;
+      }
+    }
+    double a = 0.0, hp = 0.0, norm = 0.0;
+    for (size_t j = 0; j < nm; j++)
+      if (mem_c[j] != strongest) {
+        a += f->v[mem_i[j]];
+        hp += f->w[mem_i[j]];
+        norm += f->w[mem_i[j]];
+      }
+    double q = norm - hp * hp / f->C;
+    value = (q > 0.0 && a > 0.0) ? a / rho / sqrt(fmax(q, 1e-300)) : -INFINITY;
+  }
+  free(c);
+  free(mem_i);
+  free(mem_c);
+  return value;
+}
+
+typedef struct {
+  sel_ctx* S;
+  double* remaining;
+  uint8_t* fresh;
+  heap_t* heap;
+} refresh_t;
+
+/* peaks.rs:383-419 */
+static void refresh(refresh_t* R) {
+  const size_t block = 8192;
+  size_t* stale = (size_t*)malloc(block * sizeof(size_t));
+  size_t ns = 0;
+  heap_t kept = {0, 0, 0};
+  while (R->heap->n) {
+    entry_t e = heap_pop(R->heap);
+    if (e.v != R->remaining[e.i]) continue;
+    if (R->fresh[e.i]) {
+      heap_push(&kept, e);
+    } else {
+      stale[ns++] = e.i;
+      if (ns == block) break;
+    }
+  }
+  for (size_t k = 0; k < kept.n; k++) heap_push(R->heap, kept.a[k]);
+  free(kept.a);
+  if (ns) {
+    double* sc = (double*)malloc(ns * 8);
+    double* rq = (double*)malloc(ns * 8);
+    evaluate(R->S, stale, ns, sc, rq);
+    for (size_t m = 0; m < ns; m++) {
+      size_t i = stale[m];
+      R->remaining[i] = sc[m] >= rq[m] ? sc[m] : -INFINITY;
+      R->fresh[i] = 1;
+      if (isfinite(R->remaining[i])) {
+        entry_t e = {R->remaining[i], i};
+        heap_push(R->heap, e);
+      }
+    }
+    free(sc);
+    free(rq);
+  }
+  free(stale);
+}
+
+static size_t popcount_u8(const uint8_t* a, size_t n) {
+  size_t c = 0;
+  for (size_t i = 0; i < n; i++) c += a[i] != 0;
+  return c;
+}
+
+/* peaks.rs:308-487 for one family.  masks: n_masks x n_all (grown);
+ * accepted peaks appended to out (capacity cap). */
+static void select_peaks(sel_ctx* S, size_t limit, size_t min_events,
+                         uint8_t** masks, size_t* n_masks, size_t pass,
+                         orc_peak* out, size_t* nout, size_t cap) {
+  const double margin = 0.5;
+  const size_t max_iterations = 20000;
+  if (S->nrows == 0) return;
+  double* remaining = (double*)malloc(S->nrows * 8);
+  uint8_t* fresh = (uint8_t*)malloc(S->nrows);
+  int any_ex = popcount_u8(S->excluded, S->n_all) > 0;
+  heap_t heap = {0, 0, 0};
+  for (size_t i = 0; i < S->nrows; i++) {
+    double sc = S->table[i].score;
+    remaining[i] = (isfinite(sc) && sc >= S->threshold) ? sc : -INFINITY;
+    fresh[i] = !any_ex;
+    if (isfinite(remaining[i])) {
+      entry_t e = {remaining[i], i};
+      heap_push(&heap, e);
+    }
+  }
+  refresh_t R = {S, remaining, fresh, &heap};
+  uint8_t* mask = (uint8_t*)malloc(S->n_all + 1);
+  size_t accepted = 0, iterations = 0;
+  while (accepted < limit && iterations < max_iterations) {
+    iterations++;
+    while (heap.n && heap.a[0].v != remaining[heap.a[0].i]) heap_pop(&heap);
+    if (!heap.n) break;
+    size_t i = heap.a[0].i;
+    if (!fresh[i]) {
+      refresh(&R);
+      continue;
+    }
+    heap_pop(&heap);
+    double isolated = remaining[i];
+    remaining[i] = -INFINITY;
+    const orc_row* row = &S->table[i];
+    orc_transit_mask(S->time_all, S->n_all, row->period, row->epoch, row->duration, 0.0, mask);
+    size_t total = popcount_u8(mask, S->n_all);
+    int clash = 0;
+    for (size_t k = 0; k < *n_masks && !clash; k++) {
+      size_t both = 0;
+      for (size_t j = 0; j < S->n_all; j++) both += mask[j] && masks[k][j];
+      double ov = total ? (double)both / (double)total : 0.0;
+      clash = ov >= 0.5;
+    }
+    if (clash) continue;
+    size_t events = orc_event_count(S->time_all, S->n_all, row->period, row->epoch,
+                                    row->duration, 3);
+    if (events < min_events) continue;
+    double leftover = without_strongest_event(S, row);
+    if (min_events > 1 && leftover < 0.5 * S->threshold) continue;
+    uint8_t* wide = (uint8_t*)malloc(S->n_all + 1);
+    orc_transit_mask(S->time_all, S->n_all, row->period, row->epoch, row->duration, margin, wide);
+    for (size_t j = 0; j < S->n_all; j++) S->excluded[j] |= wide[j];
+    masks[(*n_masks)++] = wide;
+    if (*nout < cap) {
+      orc_peak* p = &out[*nout];
+      memset(p, 0, sizeof(*p));
+      p->period_days = row->period;
+      p->epoch_bjd = row->epoch;
+      p->duration_hours = 24.0 * row->duration;
+      p->snr = isolated;
+      p->snr_folded = row->score;
+      p->snr_without_strongest_event = leftover;
+      p->depth = row->depth;
+      p->depth_ppm = 1e6 * row->depth;
+      p->events = events;
+      p->family = 0;
+      p->pass = pass;
+      p->row = i;
+      p->planet_log_odds = p->log_bf_planet_noise = p->log_bf_binary_noise = NAN;
+      p->binary_secondary_ratio = p->binary_secondary_phase = NAN;
+      p->binary_odd_even_ratio = p->binary_period_days = p->ingress_fraction = NAN;
+      p->evidence_epoch_bjd = p->evidence_duration_hours = NAN;
+      (*nout)++;
+    }
+    accepted++;
+    memset(fresh, 0, S->nrows);
+    refresh(&R);
+  }
+  free(mask);
+  free(remaining);
+  free(fresh);
+  free(heap.a);
+}
+
+/* ------------------------------------------------------------ pipeline */
+
+static int cmp_dbl(const void* a, const void* b) {
+  double x = *(const double*)a, y = *(const double*)b;
+  return x < y ? -1 : x > y;
+}
+
+int orc_search_prepared(const double* time, const double* flux,
+                        const double* err, size_t n, const orc_pipeline_cfg* cfg,
+                        orc_peak* peaks, size_t cap, size_t* npeaks,
+                        orc_pass_stats* pstats, size_t* npasses) {
+  *npeaks = 0;
+  *npasses = 0;
+  /* pipeline.rs:47-61: finite, positive error, sorted by time (stable) */
+  size_t* order = (size_t*)malloc((n + 1) * sizeof(size_t));
+  size_t k = 0;
+  for (size_t i = 0; i < n; i++)
+    if (isfinite(time[i]) && isfinite(flux[i]) && isfinite(err[i]) && err[i] > 0.0) order[k++] = i;
+  for (size_t i = 1; i < k; i++) { /* stable insertion sort by time */
+    size_t x = order[i], j = i;
+    while (j > 0 && time[order[j - 1]] > time[x]) {
+      order[j] = order[j - 1];
+      j--;
+    }
+    order[j] = x;
+  }
+  double* ct = (double*)malloc((k + 1) * 8);
+  double* cf = (double*)malloc((k + 1) * 8);
+  double* cs = (double*)malloc((k + 1) * 8);
+  for (size_t i = 0; i < k; i++) {
+    ct[i] = time[order[i]];
+    cf[i] = flux[order[i]];
+    cs[i] = err[order[i]];
+  }
+  free(order);
+  const double* dur = cfg->durations;
+  const size_t nd = cfg->nd;
+  fam_t cur, original;
+  int st = fam_build(ct, cf, cs, k, dur, nd, cfg->correlated, &cur);
+  free(ct);
+  free(cf);
+  free(cs);
+  if (st) return st;
+  st = fam_build(cur.ct, cur.cf, cur.cs, cur.cn, dur, nd, cfg->correlated, &original);
+  if (st) {
+    fam_free(&cur);
+    return st;
+  }
+  /* merged_time (pipeline.rs:16-24): one family, sorted, deduplicated */
+  double* time_all = (double*)malloc((cur.cn + 1) * 8);
+  memcpy(time_all, cur.ct, cur.cn * 8);
+  qsort(time_all, cur.cn, 8, cmp_dbl);
+  size_t n_all = 0;
+  for (size_t i = 0; i < cur.cn; i++)
+    if (n_all == 0 || time_all[i] != time_all[n_all - 1]) time_all[n_all++] = time_all[i];
+  if (n_all < 6) {
+    free(time_all);
+    fam_free(&cur);
+    fam_free(&original);
+    return 6;
+  }
+  uint8_t* excluded = (uint8_t*)calloc(n_all, 1);
+  uint8_t** masks = (uint8_t**)calloc(cfg->max_candidates + 1, sizeof(uint8_t*));
+  size_t n_masks = 0;
+  for (size_t pass = 0; pass < cfg->passes; pass++) {
+    size_t grid_count = 0, nrows = 0, nbands = 0;
+    size_t cap_rows = 1 << 22;
+    orc_row* table = (orc_row*)malloc(cap_rows * sizeof(orc_row));
+    orc_scan_stats bstats[64];
+    st = orc_signal_search(cur.t, cur.v, cur.w, cur.n, cur.C, cur.origin, dur, cur.rho, nd,
+                           cfg->ingress, cfg->ns, &cfg->bank, &cfg->search, table, cap_rows,
+                           &nrows, bstats, &nbands, &grid_count);
+    if (st) {
+      free(table);
+      break;
+    }
+    orc_pass_stats* ps = &pstats[*npasses];
+    memset(ps, 0, sizeof(*ps));
+    ps->pass = pass + 1;
+    ps->rows = nrows;
+    ps->periods = grid_count;
+    (*npasses)++;
+    size_t* pos = (size_t*)malloc((cur.n + 1) * sizeof(size_t));
+    for (size_t i = 0; i < cur.n; i++) {
+      size_t p = lower_bound(time_all, n_all, cur.t[i] + cur.origin);
+      pos[i] = p < n_all - 1 ? p : n_all - 1;
+    }
+    sel_ctx S = {table, nrows, &cur, pos, time_all, n_all, dur, nd, cfg->bank.threshold, excluded};
+    size_t before = *npeaks;
+    select_peaks(&S, cfg->max_candidates - before, cfg->min_events, masks, &n_masks, pass + 1,
+                 peaks, npeaks, cap);
+    free(pos);
+    free(table);
+    ps->accepted = *npeaks - before;
+    if (*npeaks == before) break;
+    orc_peak lead = peaks[before];
+    if (pass + 1 < cfg->passes) {
+      fam_t next;
+      size_t kept = 0;
+      st = fam_masked(&cur, lead.period_days, lead.epoch_bjd, lead.duration_hours / 24.0, 0.5,
+                      dur, nd, cfg->correlated, &next, &kept);
+      if (st) break;
+      fam_free(&cur);
+      cur = next;
+      if (kept < 6) break;
+    }
+  }
+  /* evidence (pipeline.rs:169-221) */
+  if (!st && cfg->model_comparison && *npeaks) {
+    size_t np = *npeaks;
+    size_t* ord = (size_t*)malloc(np * sizeof(size_t));
+    for (size_t i = 0; i < np; i++) ord[i] = i;
+    for (size_t i = 1; i < np; i++) { /* stable sort by snr, descending */
+      size_t x = ord[i], j = i;
+      while (j > 0 && peaks[ord[j - 1]].snr < peaks[x].snr) {
+        ord[j] = ord[j - 1];
+        j--;
+      }
+      ord[j] = x;
+    }
+    for (size_t oi = 0; oi < np && oi < cfg->evidence_candidates; oi++) {
+      orc_peak* p = &peaks[ord[oi]];
+      fam_t c;
+      memset(&c, 0, sizeof(c));
+      c.cn = original.cn;
+      c.ct = (double*)malloc((c.cn + 1) * 8);
+      c.cf = (double*)malloc((c.cn + 1) * 8);
+      c.cs = (double*)malloc((c.cn + 1) * 8);
+      memcpy(c.ct, original.ct, c.cn * 8);
+      memcpy(c.cf, original.cf, c.cn * 8);
+      memcpy(c.cs, original.cs, c.cn * 8);
+      for (size_t q = 0; q < np; q++) {
+        const orc_peak* o = &peaks[q];
+        if (o->row == p->row && o->pass == p->pass && o->family == p->family) continue;
+        uint8_t* m = (uint8_t*)malloc(c.cn + 1);
+        orc_transit_mask(c.ct, c.cn, o->period_days, o->epoch_bjd, o->duration_hours / 24.0,
+                         0.5, m);
+        size_t kk = 0;
+        for (size_t i = 0; i < c.cn; i++)
+          if (!m[i]) {
+            c.ct[kk] = c.ct[i];
+            c.cf[kk] = c.cf[i];
+            c.cs[kk] = c.cs[i];
+            kk++;
+          }
+        c.cn = kk;
+        free(m);
+      }
+      if (c.cn >= 6) {
+        fam_t s;
+        st = fam_build(c.ct, c.cf, c.cs, c.cn, dur, nd, cfg->correlated, &s);
+        if (st == 0) {
+          orc_evidence_row rows[2];
+          size_t nr = 0;
+          orc_compare_search(s.t, s.v, s.w, s.n, s.C, s.origin, s.energy, dur, s.rho, nd,
+                             p->period_days, &cfg->evidence, rows, &nr);
+          const orc_evidence_row* r = NULL;
+          for (size_t q = 0; q < nr; q++)
+            if (rows[q].selection_flags & 1) {
+              r = &rows[q];
+              break;
+            }
+          if (!r && nr) r = &rows[0];
+          if (r) {
+#define FIN(x) (isfinite(x) ? (x) : NAN)
+            p->planet_log_odds = FIN(r->planet_log_odds);
+            p->log_bf_planet_noise = FIN(r->log_bf_planet_noise);
+            p->log_bf_binary_noise = FIN(r->log_bf_binary_noise);
+            p->binary_secondary_ratio = FIN(r->binary_secondary_ratio);
+            p->binary_secondary_phase = FIN(r->binary_secondary_phase);
+            p->binary_odd_even_ratio = FIN(r->binary_odd_even_ratio);
+            p->binary_period_days = FIN(r->binary_period_days);
+            p->ingress_fraction = FIN(r->ingress_fraction);
+#undef FIN
+            p->evidence_epoch_bjd = r->epoch;
+            p->evidence_duration_hours = 24.0 * r->duration;
+            p->model_preferred = r->planet_log_odds >= cfg->planet_log_odds_min;
+            p->binary_diagnostics_complete =
+                isfinite(r->binary_secondary_ratio) && isfinite(r->binary_secondary_phase) &&
+                isfinite(r->binary_odd_even_ratio) && isfinite(r->binary_period_days);
+          }
+          fam_free(&s);
+        }
+      }
+      fam_free(&c);
+      if (st) break;
+    }
+    free(ord);
+  }
+  for (size_t i = 0; i < n_masks; i++) free(masks[i]);
+  free(masks);
+  free(excluded);
+  free(time_all);
+  fam_free(&cur);
+  fam_free(&original);
+  return st;
+}
