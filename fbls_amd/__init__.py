@@ -51,7 +51,10 @@ EXPORTS = (
     "fbls_synth_star", "fbls_last_error", "fbls_plan_bands",
     "fbls_bank_batches", "fbls_ctx_launch_totals", "fbls_compare",
     "fbls_evidence_parameters", "fbls_family_upload_packed",
-    "fbls_signal_search_batch", "fbls_search_prepared",
+    "fbls_signal_search_batch", "fbls_search_prepared", "fbls_cnn_create",
+    "fbls_cnn_destroy", "fbls_cnn_param_count", "fbls_cnn_get_params",
+    "fbls_cnn_set_params", "fbls_cnn_forward", "fbls_cnn_loss_grad",
+    "fbls_cnn_train_step",
 )
 
 
@@ -152,6 +155,11 @@ class PassStats(ctypes.Structure):
         return {k.rstrip("_"): getattr(self, k) for k, _ in self._fields_}
 
 
+class CnnCfg(ctypes.Structure):
+    _fields_ = [("in_channels", _u32), ("width", _u32), ("layers", _u32),
+                ("kernel", _u32), ("pos_weight", ctypes.c_float)]
+
+
 class Row(ctypes.Structure):
     _fields_ = [("period", _d), ("epoch", _d), ("duration", _d),
                 ("score", _d), ("depth", _d)]
@@ -215,6 +223,18 @@ def lib():
                                    _P(EvidenceCfg), _P(EvidenceRow), _P(_sz),
                                    _P(EvidenceDebug)]
         L.fbls_evidence_parameters.argtypes = [_P(EvidenceCfg), _P(_d)]
+        _fp = _P(ctypes.c_float)
+        L.fbls_cnn_create.argtypes = [_vp, _P(CnnCfg), _u64, _P(_vp)]
+        L.fbls_cnn_destroy.argtypes = [_vp]
+        L.fbls_cnn_destroy.restype = None
+        L.fbls_cnn_param_count.argtypes = [_vp]
+        L.fbls_cnn_param_count.restype = _sz
+        L.fbls_cnn_get_params.argtypes = [_vp, _fp]
+        L.fbls_cnn_set_params.argtypes = [_vp, _fp]
+        L.fbls_cnn_forward.argtypes = [_vp, _fp, _u32, _u32, _fp]
+        L.fbls_cnn_loss_grad.argtypes = [_vp, _fp, _fp, _u32, _u32, _fp, _fp]
+        L.fbls_cnn_train_step.argtypes = [_vp, _fp, _fp, _u32, _u32, ctypes.c_float,
+                                          ctypes.c_float, _fp]
         L.fbls_search_prepared.argtypes = [
             _vp, _P(_d), _P(_d), _P(_d), _sz, _P(PipelineCfgC), _P(Peak), _sz,
             _P(_sz), _P(PassStats), _sz, _P(_sz)]
@@ -746,3 +766,71 @@ def by_model(peaks):
         v = ev if ev == ev else p["planet_log_odds"]
         return (0 if p["model_preferred"] else 1, -(v if v == v else -np.inf))
     return sorted(peaks, key=key)
+
+
+# ----------------------------------------------------- band-proposal network
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_P(ctypes.c_float))
+
+
+class Cnn:
+    """The band-proposal network on the GPU (fbls_cnn_*)."""
+
+    def __init__(self, gpu, in_channels=8, width=64, layers=6, kernel=7,
+                 pos_weight=20.0, seed=1):
+        self.gpu = gpu
+        self.cfg = CnnCfg(in_channels, width, layers, kernel, pos_weight)
+        self._h = _vp()
+        _check(lib().fbls_cnn_create(gpu._h, ctypes.byref(self.cfg), seed,
+                                     ctypes.byref(self._h)))
+        self.count = lib().fbls_cnn_param_count(self._h)
+
+    def params(self):
+        out = np.empty(self.count, dtype=np.float32)
+        _check(lib().fbls_cnn_get_params(self._h, _fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.count
+        _check(lib().fbls_cnn_set_params(self._h, _fptr(p)))
+
+    def forward(self, x):
+        x = _f32(x)
+        B, L, _ = x.shape
+        out = np.empty((B, L), dtype=np.float32)
+        _check(lib().fbls_cnn_forward(self._h, _fptr(x), B, L, _fptr(out)))
+        return out
+
+    def loss_grad(self, x, y):
+        x, y = _f32(x), _f32(y)
+        B, L, _ = x.shape
+        loss = ctypes.c_float()
+        g = np.empty(self.count, dtype=np.float32)
+        _check(lib().fbls_cnn_loss_grad(self._h, _fptr(x), _fptr(y), B, L,
+                                        ctypes.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def train_step(self, x, y, lr=1e-3, weight_decay=1e-4):
+        x, y = _f32(x), _f32(y)
+        B, L, _ = x.shape
+        loss = ctypes.c_float()
+        _check(lib().fbls_cnn_train_step(self._h, _fptr(x), _fptr(y), B, L, lr,
+                                         weight_decay, ctypes.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self._h:
+            lib().fbls_cnn_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

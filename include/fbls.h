@@ -281,6 +281,38 @@ int fbls_search_prepared(fbls_ctx* ctx, const double* time, const double* flux,
                          fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
                          fbls_pass_stats* passes, size_t passes_cap, size_t* npasses);
 
+/* -- learned band proposal --------------------------------------------- */
+
+/* A 1-D convolutional network over a coarse periodogram (L period bins x
+ * in_channels duration groups): `layers` Conv1d layers of `width` channels
+ * and odd `kernel`, GELU, then a per-bin sigmoid head; trained with a
+ * binary cross-entropy (positive weight pos_weight) on labels from the full
+ * GPU scan.  No counterpart in the reference: it proposes which period bins
+ * the full-resolution scan visits.  BF16 MFMA GEMMs with FP32 accumulation,
+ * FP32 master weights, AdamW (beta 0.9 / 0.999, eps 1e-8, decoupled weight
+ * decay on weights only).  Tensors are channels-last: x[b][l][c]. */
+typedef struct fbls_cnn fbls_cnn;
+typedef struct {
+  uint32_t in_channels, width, layers, kernel;
+  float pos_weight;
+} fbls_cnn_cfg;
+int fbls_cnn_create(fbls_ctx* ctx, const fbls_cnn_cfg* cfg, uint64_t seed, fbls_cnn** out);
+void fbls_cnn_destroy(fbls_cnn* net);
+/* Flat FP32 parameters: per layer W[cout][k][cin] then b[cout]; then the
+ * head w[width], b. */
+size_t fbls_cnn_param_count(const fbls_cnn* net);
+int fbls_cnn_get_params(fbls_cnn* net, float* out);
+/* Sets the parameters and resets the optimiser state. */
+int fbls_cnn_set_params(fbls_cnn* net, const float* in);
+/* logits[b][l] of x[b][l][c] */
+int fbls_cnn_forward(fbls_cnn* net, const float* x, uint32_t B, uint32_t L, float* logits);
+/* Mean loss over the B x L bins and its gradient (flat, may be NULL). */
+int fbls_cnn_loss_grad(fbls_cnn* net, const float* x, const float* y, uint32_t B, uint32_t L,
+                       float* loss, float* grads);
+/* One AdamW step on the batch; *loss is the loss before the step. */
+int fbls_cnn_train_step(fbls_cnn* net, const float* x, const float* y, uint32_t B, uint32_t L,
+                        float lr, float weight_decay, float* loss);
+
 /* -- host helpers [host] ------------------------------------------------ */
 
 /* [host] `period_grid` (grid.rs:5-27): count first, then fill. */
