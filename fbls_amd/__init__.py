@@ -54,7 +54,7 @@ EXPORTS = (
     "fbls_signal_search_batch", "fbls_search_prepared", "fbls_cnn_create",
     "fbls_cnn_destroy", "fbls_cnn_param_count", "fbls_cnn_get_params",
     "fbls_cnn_set_params", "fbls_cnn_forward", "fbls_cnn_loss_grad",
-    "fbls_cnn_train_step",
+    "fbls_cnn_train_step", "fbls_coarse_features",
 )
 
 
@@ -254,7 +254,9 @@ def lib():
             _vp, _sz, _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_d), _P(_vp)]
         L.fbls_signal_search_batch.argtypes = [
             _vp, _P(_vp), _sz, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
-            _P(Row), _sz, _P(_sz), _P(ScanStats)]
+            _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8)]
+        L.fbls_coarse_features.argtypes = [_vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz,
+                                           _u32, _u32, _P(_d)]
         L.fbls_family_state.argtypes = [
             _P(_d), _P(_d), _P(_d), _sz, _P(_d), _sz, ctypes.c_int, _P(_d),
             _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz),
@@ -690,8 +692,10 @@ def signal_search(gpu, dev, durations, rho, cfg: SearchConfig):
 
 
 def signal_search_batch(gpu, devs, durations, rhos, cfg: SearchConfig,
-                        rows_cap=None):
-    """Many stars: returns ([rows k_s x 5 per star], [stats per star])."""
+                        rows_cap=None, keep=None):
+    """Many stars: returns ([rows k_s x 5 per star], [stats per star]).
+    keep: optional list of per-star boolean arrays over each star's grid;
+    only those periods are scanned (each exactly as in the full search)."""
     durations = _f64(durations)
     rho = np.ascontiguousarray(np.asarray(rhos, dtype=np.float64).reshape(
         len(devs), len(durations)))
@@ -710,10 +714,15 @@ def signal_search_batch(gpu, devs, durations, rhos, cfg: SearchConfig,
                 rows_cap += lib().fbls_period_grid_count(lo, hi, d.t_last, dmin,
                                                          cfg.period_oversample)
     rows = (Row * max(rows_cap, 1))()
+    kp = None
+    if keep is not None:
+        kflat = np.ascontiguousarray(np.concatenate(
+            [np.asarray(k, dtype=np.uint8) for k in keep]), dtype=np.uint8)
+        kp = kflat.ctypes.data_as(_P(ctypes.c_uint8))
     _check(lib().fbls_signal_search_batch(
         gpu._h, hs, len(devs), _dp(durations), _dp(rho), len(durations),
         ctypes.byref(bank), ctypes.byref(scfg), rows, rows_cap,
-        offs.ctypes.data_as(_P(_sz)), stats))
+        offs.ctypes.data_as(_P(_sz)), stats, kp))
     need = int(offs[-1])
     allrows = _rows(rows, need)
     return ([allrows[int(offs[i]):int(offs[i + 1])] for i in range(len(devs))],
@@ -834,3 +843,14 @@ class Cnn:
             self.close()
         except Exception:
             pass
+
+
+def coarse_features(gpu, devs, periods, durations, nb_min=16, nb_max=1024):
+    """Coarse periodogram (fbls_coarse_features): [star, period, duration]."""
+    periods, durations = _f64(periods), _f64(durations)
+    hs = (_vp * len(devs))(*[d._h.value for d in devs])
+    out = np.empty((len(devs), len(periods), len(durations)))
+    _check(lib().fbls_coarse_features(gpu._h, hs, len(devs), _dp(periods), len(periods),
+                                      _dp(durations), len(durations), nb_min, nb_max,
+                                      _dp(out)))
+    return out

@@ -860,7 +860,19 @@ struct SearchPlan {
   const fbls_family* fam = nullptr;
   std::vector<double> sd, sr, grid;
   std::vector<fbls_band> bands;
+  std::vector<std::vector<double>> kept;  // per band, when a keep mask applies
 };
+
+// Restricts each band to the grid periods whose keep flag is set; each kept
+// period is scanned exactly as in the full search (same nb, same prescan).
+void apply_keep(SearchPlan& p, const uint8_t* keep) {
+  p.kept.assign(p.bands.size(), {});
+  for (size_t b = 0; b < p.bands.size(); b++)
+    for (size_t i = 0; i < p.bands[b].count; i++) {
+      const size_t g = p.bands[b].first + i;
+      if (keep[g]) p.kept[b].push_back(p.grid[g]);
+    }
+}
 
 int plan_search(const fbls_family* f, const double* durations, const double* rho,
                 size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
@@ -899,11 +911,17 @@ int plan_search(const fbls_family* f, const double* durations, const double* rho
 }
 
 void append_runs(const SearchPlan& p, std::vector<BandRun>& runs) {
-  for (const fbls_band& band : p.bands) {
+  for (size_t b = 0; b < p.bands.size(); b++) {
+    const fbls_band& band = p.bands[b];
     BandRun r;
     r.fam = p.fam;
-    r.periods = p.grid.data() + band.first;
-    r.np = band.count;
+    if (p.kept.empty()) {
+      r.periods = p.grid.data() + band.first;
+      r.np = band.count;
+    } else {
+      r.periods = p.kept[b].data();
+      r.np = p.kept[b].size();
+    }
     r.nb = band.nb;
     r.nb_coarse = band.nb_coarse;
     r.nd = (uint32_t)band.nd;
@@ -979,7 +997,8 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
                                         const fbls_bank* bank,
                                         const fbls_search_cfg* cfg, fbls_row* rows,
                                         size_t rows_cap, size_t* row_offsets,
-                                        fbls_scan_stats* star_stats) {
+                                        fbls_scan_stats* star_stats,
+                                        const uint8_t* keep) {
   if (!c || (nfam && !fams) || !bank || !cfg || !row_offsets || (nd && (!durations || !rho)))
     return fail(FBLS_E_ARG, "null argument");
   constexpr size_t kBatchPeriods = size_t(1) << 21;
@@ -988,7 +1007,7 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
   if (nd == 0 || nfam == 0) return FBLS_OK;
   fbls_bank b = budgeted(bank);
   size_t written = 0, needed = 0;
-  size_t s0 = 0;
+  size_t s0 = 0, keep_at = 0;
   while (s0 < nfam) {
     // one chunk of stars
     std::vector<SearchPlan> plans;
@@ -1000,6 +1019,10 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
       plans.emplace_back();
       int st = plan_search(fams[s1], durations, rho + s1 * nd, nd, bank, cfg, &plans.back());
       if (st) return st;
+      if (keep) {
+        apply_keep(plans.back(), keep + keep_at);
+        keep_at += plans.back().grid.size();
+      }
       periods += plans.back().grid.size();
       s1++;
     }

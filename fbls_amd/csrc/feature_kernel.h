@@ -1,0 +1,31 @@
+// Launch interface of the coarse-periodogram kernel (feature_kernel.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fbls {
+
+struct FeatureStar {
+  const double* t;    // n origin-relative times
+  const int32_t* vw;  // n x {v, w} fixed point
+  uint32_t n;
+  uint32_t pad_;
+  double sv, sw;      // fixed point -> value
+  double C;           // baseline norm
+};
+
+struct FeatureArgs {
+  const FeatureStar* stars;
+  const double* periods;    // np
+  const uint32_t* nb;       // np phase bins per period
+  uint32_t np, ng;
+  const double* durations;  // ng
+  double* out;              // stars x np x ng
+};
+
+size_t feature_lds_bytes(uint32_t nb_max);
+hipError_t feature_launch(const FeatureArgs& a, uint32_t nstars, uint32_t nb_max,
+                          hipStream_t s);
+
+}  // namespace fbls

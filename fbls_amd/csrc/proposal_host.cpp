@@ -1,0 +1,76 @@
+// Host side of the coarse periodogram that feeds the band-proposal network
+// (feature_kernel.hip): one launch for every (star, period) of a batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/fbls.h"
+#include "feature_kernel.h"
+#include "host_internal.h"
+
+using namespace fbls::host;
+
+extern "C" int fbls_coarse_features(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
+                                    const double* periods, size_t np, const double* durations,
+                                    size_t ng, uint32_t nb_min, uint32_t nb_max, double* out) {
+  if (!c || (nfam && !fams) || !periods || !durations || !out) return fail(FBLS_E_ARG, "null argument");
+  if (np == 0 || ng == 0 || nfam == 0) return FBLS_OK;
+  if (nb_min < 2 || nb_max < nb_min || nb_max > 4096)
+    return fail(FBLS_E_ARG, "phase bins must satisfy 2 <= nb_min <= nb_max <= 4096");
+  if ((uint64_t)nfam * np > 0x7fffffffull) return fail(FBLS_E_ARG, "too many (star, period) pairs");
+  double d0 = INFINITY;
+  for (size_t g = 0; g < ng; g++) {
+    if (!(durations[g] > 0.0 && std::isfinite(durations[g])))
+      return fail(FBLS_E_ARG, "durations must be positive and finite");
+    d0 = std::min(d0, durations[g]);
+  }
+  std::vector<uint32_t> nb(np);
+  for (size_t i = 0; i < np; i++) {
+    if (!(periods[i] > 0.0 && std::isfinite(periods[i])))
+      return fail(FBLS_E_ARG, "periods must be positive and finite");
+    const double want = std::ceil(2.0 * periods[i] / d0);
+    nb[i] = (uint32_t)std::min<double>(nb_max, std::max<double>(nb_min, want));
+  }
+  const double pmin = *std::min_element(periods, periods + np);
+  std::vector<fbls::FeatureStar> stars(nfam);
+  for (size_t k = 0; k < nfam; k++) {
+    const fbls_family* f = fams[k];
+    if (!f) return fail(FBLS_E_ARG, "null family %zu", k);
+    if (f->t_max * ((double)nb_max / pmin) > 9.0e15) return fail(FBLS_E_ARG, "phase cells exceed 2^53");
+    stars[k].t = f->t_dev;
+    stars[k].vw = f->vw_dev;
+    stars[k].n = f->n;
+    stars[k].sv = std::ldexp(1.0, -f->ev);
+    stars[k].sw = std::ldexp(1.0, -f->ew);
+    stars[k].C = f->C;
+  }
+  const size_t b_stars = nfam * sizeof(fbls::FeatureStar);
+  const size_t o_per = (b_stars + 255) & ~(size_t)255;
+  const size_t o_nb = o_per + ((np * 8 + 255) & ~(size_t)255);
+  const size_t o_dur = o_nb + ((np * 4 + 255) & ~(size_t)255);
+  const size_t o_out = o_dur + ((ng * 8 + 255) & ~(size_t)255);
+  const size_t out_bytes = nfam * np * ng * 8;
+  HIP_TRY(hipSetDevice(c->device));
+  if (int st = c->dev_in.ensure(o_out + out_bytes)) return st;
+  unsigned char* d = (unsigned char*)c->dev_in.ptr;
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(d, stars.data(), b_stars, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d + o_per, periods, np * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d + o_nb, nb.data(), np * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d + o_dur, durations, ng * 8, hipMemcpyHostToDevice, s));
+  fbls::FeatureArgs A;
+  A.stars = (const fbls::FeatureStar*)d;
+  A.periods = (const double*)(d + o_per);
+  A.nb = (const uint32_t*)(d + o_nb);
+  A.np = (uint32_t)np;
+  A.ng = (uint32_t)ng;
+  A.durations = (const double*)(d + o_dur);
+  A.out = (double*)(d + o_out);
+  HIP_TRY(fbls::feature_launch(A, (uint32_t)nfam, nb_max, s));
+  HIP_TRY(hipMemcpyAsync(out, d + o_out, out_bytes, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return FBLS_OK;
+}

@@ -168,13 +168,18 @@ int fbls_signal_search(fbls_ctx* ctx, const fbls_family* fam,
  * row_offsets holds nfam + 1 entries and is filled even when rows_cap is
  * too small (then FBLS_E_ARG; rows_cap >= the sum of the stars' grid counts
  * always suffices).  star_stats (nfam, may be NULL): the band
- * statistics summed per star (phase_bins = the largest). */
+ * statistics summed per star (phase_bins = the largest).  keep (may be
+ * NULL): one flag per grid period of every star, concatenated in star order
+ * (each star's grid as fbls_period_grid); only flagged periods are scanned,
+ * each exactly as in the full search (same bands, nb and prescan), so the
+ * rows are the full search's rows at those periods. */
 int fbls_signal_search_batch(fbls_ctx* ctx, const fbls_family* const* fams,
                              size_t nfam, const double* durations,
                              const double* rho, size_t nd,
                              const fbls_bank* bank, const fbls_search_cfg* cfg,
                              fbls_row* rows, size_t rows_cap,
-                             size_t* row_offsets, fbls_scan_stats* star_stats);
+                             size_t* row_offsets, fbls_scan_stats* star_stats,
+                             const uint8_t* keep);
 
 /* -- evidence (model comparison) ---------------------------------------- */
 
@@ -312,6 +317,15 @@ int fbls_cnn_loss_grad(fbls_cnn* net, const float* x, const float* y, uint32_t B
 /* One AdamW step on the batch; *loss is the loss before the step. */
 int fbls_cnn_train_step(fbls_cnn* net, const float* x, const float* y, uint32_t B, uint32_t L,
                         float lr, float weight_decay, float* loss);
+
+/* Coarse periodogram (the network's input): for each star k and period i,
+ * out[k][i][g] = the largest box SNR a / sqrt(w - w^2 / C) over phase of a
+ * transit of duration durations[g] at nb_i = clamp(ceil(2 P_i / d_0),
+ * nb_min, nb_max) phase bins (d_0 the shortest duration; windows with at
+ * least 3 points and at most n - 3; -inf when none).  One launch. */
+int fbls_coarse_features(fbls_ctx* ctx, const fbls_family* const* fams, size_t nfam,
+                         const double* periods, size_t np, const double* durations,
+                         size_t ng, uint32_t nb_min, uint32_t nb_max, double* out);
 
 /* -- host helpers [host] ------------------------------------------------ */
 

@@ -150,6 +150,9 @@ def lib():
         L.orc_search_prepared.argtypes = [
             _P(_d), _P(_d), _P(_d), _sz, _P(OrcPipelineCfg), _P(OrcPeak), _sz,
             _P(_sz), _P(OrcPassStats), _P(_sz)]
+        L.orc_coarse_features.argtypes = [_P(_d), _P(_d), _P(_d), _sz, _d, _P(_d), _sz,
+                                          _P(_d), _sz, _u32, _u32, _P(_d)]
+        L.orc_coarse_features.restype = None
         L.orc_transit_mask.argtypes = [_P(_d), _sz, _d, _d, _d, _d, _P(ctypes.c_uint8)]
         L.orc_transit_mask.restype = None
         L.orc_event_count.argtypes = [_P(_d), _sz, _d, _d, _d, _sz]
@@ -525,3 +528,14 @@ def search_prepared(time, flux, err, cfg):
         raise ValueError(f"search_prepared status {st}")
     return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
             [passes[i].as_dict() for i in range(npa.value)])
+
+
+def coarse_features(t, v, w, baseline_norm, periods, durations, nb_min=16, nb_max=1024):
+    """Coarse periodogram of one star: [period, duration]."""
+    t, v, w = _f64(t), _f64(v), _f64(w)
+    periods, durations = _f64(periods), _f64(durations)
+    out = np.empty((len(periods), len(durations)))
+    lib().orc_coarse_features(_dp(t), _dp(v), _dp(w), len(t), baseline_norm, _dp(periods),
+                              len(periods), _dp(durations), len(durations), nb_min, nb_max,
+                              _dp(out))
+    return out

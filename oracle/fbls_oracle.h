@@ -253,6 +253,13 @@ int orc_search_prepared(const double* time, const double* flux,
                         orc_peak* peaks, size_t cap, size_t* npeaks,
                         orc_pass_stats* pstats, size_t* npasses);
 
+/* Coarse periodogram of the band-proposal network (oracle/proposal.c):
+ * out[i][g] for np periods and ng durations; see fbls_coarse_features. */
+void orc_coarse_features(const double* t, const double* v, const double* w, size_t n,
+                         double C, const double* periods, size_t np,
+                         const double* durations, size_t ng, uint32_t nb_min,
+                         uint32_t nb_max, double* out);
+
 /* survey/numeric.rs:345-364 */
 uint64_t orc_rng_new(uint64_t seed);
 double orc_rng_uniform(uint64_t* state);
