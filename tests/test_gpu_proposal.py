@@ -62,7 +62,7 @@ def test_keep_mask_rows_are_full_rows(gpu):
 def test_trained_proposal_recall_and_gain(gpu):
     net, pc, extra = PR.load(gpu)
     cfg = PR.search_config(pc)
-    res = PR.make_states(range(192), 4_950_000, cfg, workers=8)
+    res = PR.make_states(range(192), 4_950_000, cfg, workers=1)
     ev = PR.EvalSet(gpu, [r[0] for r in res], [r[1] for r in res], pc)
     topk, tau = int(extra["topk"]), float(extra["tau"])
     fb = tuple(float(v) for v in np.atleast_1d(extra["fallback"]))
