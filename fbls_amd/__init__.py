@@ -54,7 +54,8 @@ EXPORTS = (
     "fbls_signal_search_batch", "fbls_search_prepared", "fbls_cnn_create",
     "fbls_cnn_destroy", "fbls_cnn_param_count", "fbls_cnn_get_params",
     "fbls_cnn_set_params", "fbls_cnn_forward", "fbls_cnn_loss_grad",
-    "fbls_cnn_train_step", "fbls_coarse_features",
+    "fbls_cnn_train_step", "fbls_coarse_features", "fbls_ctx_set_graphs",
+    "fbls_autotune",
 )
 
 
@@ -160,6 +161,10 @@ class CnnCfg(ctypes.Structure):
                 ("kernel", _u32), ("pos_weight", ctypes.c_float)]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("lds_bytes", _u32), ("block", _u32), ("seconds", _d)]
+
+
 class Row(ctypes.Structure):
     _fields_ = [("period", _d), ("epoch", _d), ("duration", _d),
                 ("score", _d), ("depth", _d)]
@@ -219,6 +224,9 @@ def lib():
         L.fbls_mem_info.argtypes = [_vp, _P(_sz), _P(_sz)]
         L.fbls_ctx_set_launch.argtypes = [_vp, _u32, _u32, _u32]
         L.fbls_ctx_launch_totals.argtypes = [_vp, _P(LaunchTotals), ctypes.c_int]
+        L.fbls_ctx_set_graphs.argtypes = [_vp, ctypes.c_int, _P(_u64)]
+        L.fbls_autotune.argtypes = [_vp, _vp, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
+                                    _u32, _P(TuneEntry), _sz, _P(_sz), _P(TuneEntry)]
         L.fbls_compare.argtypes = [_vp, _vp, _d, _P(_d), _P(_d), _sz, _d,
                                    _P(EvidenceCfg), _P(EvidenceRow), _P(_sz),
                                    _P(EvidenceDebug)]
@@ -472,6 +480,14 @@ class Gpu:
 
     def set_launch(self, lds_bytes=0, block=0, fold_mode=0):
         _check(lib().fbls_ctx_set_launch(self._h, lds_bytes, block, fold_mode))
+
+    def set_graphs(self, enable=None):
+        """Evidence calls through a replayed hipGraph (True) or eager
+        launches (False); returns the number of graph submissions so far."""
+        n = _u64()
+        _check(lib().fbls_ctx_set_graphs(self._h, -1 if enable is None else int(bool(enable)),
+                                         ctypes.byref(n)))
+        return n.value
 
     def launch_totals(self, reset=False):
         """Kernel launches on this context since the last reset: count,
@@ -744,6 +760,22 @@ def pipeline_cfg(cfg: SearchConfig):
         planet_log_odds_min=cfg.planet_log_odds_min, evidence=ev)
     c._keep = (d, bank, ev)
     return c
+
+
+def autotune(gpu, dev, durations, rho, cfg: SearchConfig, repeats=3):
+    """fbls_autotune: the fastest launch shape for this star's search, kept
+    on the context.  Returns (best, table) as dicts."""
+    durations, rho = _f64(durations), _f64(rho)
+    bank = make_bank(cfg, durations, rho)
+    scfg = SearchCfgC(cfg.period_min, cfg.period_max, cfg.period_oversample)
+    table = (TuneEntry * 32)()
+    n = _sz()
+    best = TuneEntry()
+    _check(lib().fbls_autotune(gpu._h, dev._h, _dp(durations), _dp(rho), len(durations),
+                               ctypes.byref(bank), ctypes.byref(scfg), repeats, table, 32,
+                               ctypes.byref(n), ctypes.byref(best)))
+    as_d = lambda e: {"lds_bytes": e.lds_bytes, "block": e.block, "seconds": e.seconds}
+    return as_d(best), [as_d(table[i]) for i in range(min(n.value, 32))]
 
 
 def search_prepared(gpu, time, flux, flux_err, cfg: SearchConfig):

@@ -259,15 +259,58 @@ extern "C" int fbls_compare(fbls_ctx* c, const fbls_family* f, double period,
   fbls::evidence_bind_scratch(&A, dbase);
 
   HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(hipMemcpyAsync((void*)dtab, tab, tab_bytes, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(fbls::evidence_launch(A, c->stream));
   double* hout = (double*)c->pin_out.ptr;
-  HIP_TRY(hipMemcpyAsync(hout, A.floors, 4 * 8, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(hout + 4, A.parts, tiles * 2 * fbls::kEvPart * 8,
-                         hipMemcpyDeviceToHost, c->stream));
-  if (debug && debug->screen)
-    HIP_TRY(hipMemcpyAsync(hout + 4 + tiles * 2 * fbls::kEvPart, A.screen, tiles * 4 * 8,
-                           hipMemcpyDeviceToHost, c->stream));
+  // The call's whole device sequence: tables in, six kernels, results out.
+  auto enqueue = [&]() -> hipError_t {
+    hipError_t e;
+    if ((e = hipMemcpyAsync((void*)dtab, tab, tab_bytes, hipMemcpyHostToDevice, c->stream)))
+      return e;
+    if ((e = fbls::evidence_launch(A, c->stream))) return e;
+    if ((e = hipMemcpyAsync(hout, A.floors, 4 * 8, hipMemcpyDeviceToHost, c->stream))) return e;
+    if ((e = hipMemcpyAsync(hout + 4, A.parts, tiles * 2 * fbls::kEvPart * 8,
+                            hipMemcpyDeviceToHost, c->stream)))
+      return e;
+    if (debug && debug->screen)
+      return hipMemcpyAsync(hout + 4 + tiles * 2 * fbls::kEvPart, A.screen, tiles * 4 * 8,
+                            hipMemcpyDeviceToHost, c->stream);
+    return hipSuccess;
+  };
+  if (c->use_graphs) {
+    // captured per call and updated into the cached executable graph (one
+    // submission instead of nine); a new topology instantiates afresh
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t ee = enqueue();
+    const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    if (ee != hipSuccess || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(FBLS_E_HIP, "evidence graph capture failed: %s",
+                  hipGetErrorString(ee != hipSuccess ? ee : ce));
+    }
+    bool ready = false;
+    if (c->ev_exec) {
+      hipGraphExecUpdateResult r;
+      hipGraphNode_t bad = nullptr;
+      ready = hipGraphExecUpdate(c->ev_exec, g, &bad, &r) == hipSuccess &&
+              r == hipGraphExecUpdateSuccess;
+      if (!ready) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(c->ev_exec);
+        c->ev_exec = nullptr;
+      }
+    }
+    hipError_t ie = hipSuccess;
+    if (!ready) ie = hipGraphInstantiate(&c->ev_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      c->ev_exec = nullptr;
+      return fail(FBLS_E_HIP, "evidence graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+    HIP_TRY(hipGraphLaunch(c->ev_exec, c->stream));
+    c->graph_launches++;
+  } else {
+    HIP_TRY(enqueue());
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   const double* floors = hout;
   const double* parts = hout + 4;

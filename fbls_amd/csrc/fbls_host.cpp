@@ -78,6 +78,7 @@ extern "C" void fbls_ctx_destroy(fbls_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ev_exec) (void)hipGraphExecDestroy(c->ev_exec);
   c->dev_in.release();
   c->dev_out.release();
   c->dev_ev.release();
@@ -107,6 +108,13 @@ extern "C" int fbls_ctx_set_launch(fbls_ctx* c, uint32_t lds_bytes,
   c->lds_override = lds_bytes;
   c->block_override = block;
   c->run_fold = fold_mode == 2 ? 0u : 1u;
+  return FBLS_OK;
+}
+
+extern "C" int fbls_ctx_set_graphs(fbls_ctx* c, int enable, uint64_t* graph_launches) {
+  if (!c) return fail(FBLS_E_ARG, "null context");
+  if (enable >= 0) c->use_graphs = enable ? 1 : 0;
+  if (graph_launches) *graph_launches = c->graph_launches;
   return FBLS_OK;
 }
 
@@ -1103,5 +1111,71 @@ extern "C" int fbls_synth_star(const fbls_synth_cfg* cfg, double* time,
   std::string msg;
   int st = fbls::synth_star(cfg, time, flux, flux_err, cap, n, &msg);
   if (st) return fail(st, "%s", msg.c_str());
+  return FBLS_OK;
+}
+
+// ---------------------------------------------------------------- autotune
+
+// Times fbls_signal_search of one family under every candidate launch shape
+// (LDS budget per workgroup, which sets the periods per workgroup, and the
+// workgroup size), checks that every shape returns bit-identical rows, and
+// keeps the fastest on the context.
+extern "C" int fbls_autotune(fbls_ctx* c, const fbls_family* f, const double* durations,
+                             const double* rho, size_t nd, const fbls_bank* bank,
+                             const fbls_search_cfg* cfg, uint32_t repeats,
+                             fbls_tune_entry* table, size_t table_cap, size_t* ntable,
+                             fbls_tune_entry* best) {
+  if (!c || !f || !bank || !cfg || !ntable || !best) return fail(FBLS_E_ARG, "null argument");
+  *ntable = 0;
+  const uint32_t lds_opts[] = {16384, 24576, 32768, 49152, 65536, 98304};
+  const uint32_t block_opts[] = {256, 512};
+  const uint32_t saved_lds = c->lds_override, saved_block = c->block_override;
+  const double d0 = *std::min_element(durations, durations + nd);
+  const double hi = std::min(cfg->period_max, f->t_last);
+  const double lo = std::max(cfg->period_min, 4.0 * d0 * (1.0 + 1e-9));
+  const size_t cap = hi > lo ? std::max<size_t>(1, fbls::period_grid_count(
+                                                       lo, hi, f->t_last, d0,
+                                                       cfg->period_oversample))
+                             : 1;
+  std::vector<fbls_row> rows(cap), ref;
+  std::vector<fbls_scan_stats> st(64);
+  size_t nrows = 0, nbands = 0, gc = 0;
+  best->seconds = INFINITY;
+  int status = FBLS_OK;
+  for (uint32_t block : block_opts) {
+    for (uint32_t lds : lds_opts) {
+      if (lds > c->max_lds) continue;
+      c->lds_override = lds;
+      c->block_override = block;
+      double secs = INFINITY;
+      for (uint32_t r = 0; r <= std::max<uint32_t>(repeats, 1); r++) {
+        fbls_launch_totals before = c->totals;
+        status = fbls_signal_search(c, f, durations, rho, nd, bank, cfg, rows.data(), cap,
+                                    &nrows, st.data(), st.size(), &nbands, nullptr, 0, &gc);
+        if (status) break;
+        if (r > 0) secs = std::min(secs, c->totals.seconds - before.seconds);  // r = 0 warms up
+      }
+      if (status) break;
+      if (ref.empty()) {
+        ref.assign(rows.begin(), rows.begin() + nrows);
+      } else if (nrows != ref.size() ||
+                 std::memcmp(ref.data(), rows.data(), nrows * sizeof(fbls_row)) != 0) {
+        status = fail(FBLS_E_HIP, "launch shape lds %u block %u changed the rows", lds, block);
+        break;
+      }
+      fbls_tune_entry e{lds, block, secs};
+      if (table && *ntable < table_cap) table[*ntable] = e;
+      (*ntable)++;
+      if (secs < best->seconds) *best = e;
+    }
+    if (status) break;
+  }
+  if (status) {
+    c->lds_override = saved_lds;
+    c->block_override = saved_block;
+    return status;
+  }
+  c->lds_override = best->lds_bytes;
+  c->block_override = best->block;
   return FBLS_OK;
 }

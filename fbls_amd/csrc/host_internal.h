@@ -94,6 +94,9 @@ struct fbls_ctx {
   size_t max_lds = fbls::host::kMaxLds;
   uint32_t run_fold = 1;
   fbls_launch_totals totals = {0, 0, 0, 0.0};
+  int use_graphs = 1;                  // evidence calls replay a hipGraph
+  hipGraphExec_t ev_exec = nullptr;    // the evidence call's executable graph
+  uint64_t graph_launches = 0;
   fbls::host::DevBuf dev_in, dev_out, dev_ev;
   fbls::host::HostBuf pin_in, pin_out;
 };

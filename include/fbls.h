@@ -106,6 +106,28 @@ typedef struct {
 } fbls_launch_totals;
 int fbls_ctx_launch_totals(fbls_ctx* ctx, fbls_launch_totals* out, int reset);
 
+/* Launch-shape autotuner: times fbls_signal_search of `fam` under each
+ * candidate (LDS budget per workgroup 16-96 KiB, which sets the periods per
+ * workgroup, x workgroup size 256/512; best of `repeats` after a warm-up),
+ * fails with FBLS_E_HIP if any shape changes the rows, and keeps the
+ * fastest on the context (as fbls_ctx_set_launch).  table (capacity
+ * table_cap, may be NULL) receives every measured shape. */
+typedef struct {
+  uint32_t lds_bytes, block;
+  double seconds;  /* kernel seconds of one search */
+} fbls_tune_entry;
+int fbls_autotune(fbls_ctx* ctx, const fbls_family* fam, const double* durations,
+                  const double* rho, size_t nd, const fbls_bank* bank,
+                  const fbls_search_cfg* cfg, uint32_t repeats, fbls_tune_entry* table,
+                  size_t table_cap, size_t* ntable, fbls_tune_entry* best);
+
+/* hipGraph replay of the evidence call (fbls_compare: the table upload, six
+ * kernels and the copies back are captured per call and updated into one
+ * cached executable graph).  enable: 1 on (default), 0 off (eager launches),
+ * negative leaves it unchanged; *graph_launches (may be NULL) counts graph
+ * submissions on this context.  Results never depend on it. */
+int fbls_ctx_set_graphs(fbls_ctx* ctx, int enable, uint64_t* graph_launches);
+
 /* -- family upload ------------------------------------------------------ */
 
 /* `DeviceReady::upload` (scan.rs:110-125).  t: origin-relative times >= 0;
