@@ -55,7 +55,8 @@ EXPORTS = (
     "fbls_cnn_destroy", "fbls_cnn_param_count", "fbls_cnn_get_params",
     "fbls_cnn_set_params", "fbls_cnn_forward", "fbls_cnn_loss_grad",
     "fbls_cnn_train_step", "fbls_coarse_features", "fbls_ctx_set_graphs",
-    "fbls_autotune",
+    "fbls_autotune", "fbls_biweight_flatten", "fbls_prepare_families",
+    "fbls_search_transits",
 )
 
 
@@ -165,6 +166,11 @@ class TuneEntry(ctypes.Structure):
     _fields_ = [("lds_bytes", _u32), ("block", _u32), ("seconds", _d)]
 
 
+class PrepCfg(ctypes.Structure):
+    _fields_ = [(k, _d) for k in ("bin_minutes", "window_min", "window_factor",
+                                  "window_max", "clip_sigma", "edge_trim")]
+
+
 class Row(ctypes.Structure):
     _fields_ = [("period", _d), ("epoch", _d), ("duration", _d),
                 ("score", _d), ("depth", _d)]
@@ -225,6 +231,10 @@ def lib():
         L.fbls_ctx_set_launch.argtypes = [_vp, _u32, _u32, _u32]
         L.fbls_ctx_launch_totals.argtypes = [_vp, _P(LaunchTotals), ctypes.c_int]
         L.fbls_ctx_set_graphs.argtypes = [_vp, ctypes.c_int, _P(_u64)]
+        L.fbls_biweight_flatten.argtypes = [_vp, _P(_d), _P(_d), _sz, _d, _d, _d, _u32, _P(_d)]
+        L.fbls_prepare_families.argtypes = [
+            _vp, _P(_d), _P(_d), _P(_d), _P(ctypes.c_int64), _sz, _P(_d), _sz, _P(PrepCfg),
+            _P(_sz), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz), _P(ctypes.c_uint8)]
         L.fbls_autotune.argtypes = [_vp, _vp, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
                                     _u32, _P(TuneEntry), _sz, _P(_sz), _P(TuneEntry)]
         L.fbls_compare.argtypes = [_vp, _vp, _d, _P(_d), _P(_d), _sz, _d,
@@ -246,6 +256,9 @@ def lib():
         L.fbls_search_prepared.argtypes = [
             _vp, _P(_d), _P(_d), _P(_d), _sz, _P(PipelineCfgC), _P(Peak), _sz,
             _P(_sz), _P(PassStats), _sz, _P(_sz)]
+        L.fbls_search_transits.argtypes = [
+            _vp, _P(_d), _P(_d), _P(_d), _P(ctypes.c_int64), _sz, _P(PrepCfg),
+            _P(PipelineCfgC), _P(Peak), _sz, _P(_sz), _P(PassStats), _sz, _P(_sz)]
         L.fbls_family_upload.argtypes = [_vp, _P(_d), _P(_d), _P(_d), _sz, _d,
                                          _d, _P(_vp)]
         L.fbls_family_fields.argtypes = [_vp, _P(_d), _P(_d),
@@ -317,6 +330,13 @@ class SearchConfig:
     period_oversample: float = 3.0
     period_batch: int = 4096
     max_fold_mib: float = 1024.0
+    # raw-mode preparation (config.rs:11-16,54-59)
+    bin_minutes: float = 10.0
+    window_days: float = 0.4
+    window_factor: float = 3.0
+    window_max_days: float = 3.5
+    edge_trim_windows: float = 0.5
+    clip_sigma: float = 4.0
     # passes and selection (config.rs:25-34,68-77)
     max_candidates: int = 20
     passes: int = 3
@@ -886,3 +906,71 @@ def coarse_features(gpu, devs, periods, durations, nb_min=16, nb_max=1024):
                                       _dp(durations), len(durations), nb_min, nb_max,
                                       _dp(out)))
     return out
+
+
+# ------------------------------------------------------------ raw mode
+
+def biweight_flatten(gpu, time, rel, window, gap=0.5, c=5.0, iterations=10):
+    """fbls_biweight_flatten: (1 + rel) / biweight location - 1 on the GPU."""
+    time, rel = _f64(time), _f64(rel)
+    out = np.empty(len(time))
+    _check(lib().fbls_biweight_flatten(gpu._h, _dp(time), _dp(rel), len(time), window, gap,
+                                       c, iterations, _dp(out)))
+    return out
+
+
+def prep_cfg(cfg: SearchConfig):
+    return PrepCfg(cfg.bin_minutes, cfg.window_days, cfg.window_factor, cfg.window_max_days,
+                   cfg.clip_sigma, cfg.edge_trim_windows)
+
+
+def prepare_families(gpu, time, flux, err, group, cfg: SearchConfig):
+    """fbls_prepare_families: list of family dicts (time, flux, sigma,
+    window, clipped, indices of the durations it searches)."""
+    time, flux = _f64(time), _f64(flux)
+    n = len(time)
+    e = _f64(err) if err is not None else None
+    g = np.ascontiguousarray(group, dtype=np.int64) if group is not None else None
+    d = cfg.durations_days()
+    nd = len(d)
+    pc = prep_cfg(cfg)
+    nf = _sz()
+    fn = np.zeros(nd, dtype=np.uintp)
+    ft, ff, fs = np.zeros(nd * n), np.zeros(nd * n), np.zeros(nd * n)
+    fw = np.zeros(nd)
+    fc = np.zeros(nd, dtype=np.uintp)
+    fm = np.zeros(nd * nd, dtype=np.uint8)
+    _check(lib().fbls_prepare_families(
+        gpu._h, _dp(time), _dp(flux), _dp(e) if e is not None else None,
+        g.ctypes.data_as(_P(ctypes.c_int64)) if g is not None else None, n, _dp(d), nd,
+        ctypes.byref(pc), ctypes.byref(nf), fn.ctypes.data_as(_P(_sz)), _dp(ft), _dp(ff),
+        _dp(fs), _dp(fw), fc.ctypes.data_as(_P(_sz)), fm.ctypes.data_as(_P(ctypes.c_uint8))))
+    out = []
+    for k in range(nf.value):
+        m = int(fn[k])
+        out.append(dict(time=ft[k * n:k * n + m], flux=ff[k * n:k * n + m],
+                        sigma=fs[k * n:k * n + m], window=float(fw[k]), clipped=int(fc[k]),
+                        indices=np.nonzero(fm[k * nd:(k + 1) * nd])[0]))
+    return out
+
+
+def search_transits(gpu, time, flux, err, group, cfg: SearchConfig):
+    """search_transits (pipeline.rs:27-37, 70-221, without single events) on
+    raw flux: the families of prepare_families searched together.  Returns
+    (peaks in acceptance order as dicts, pass statistics)."""
+    time, flux = _f64(time), _f64(flux)
+    e = _f64(err) if err is not None else None
+    g = np.ascontiguousarray(group, dtype=np.int64) if group is not None else None
+    c = pipeline_cfg(cfg)
+    pc = prep_cfg(cfg)
+    cap = max(cfg.max_candidates, 1)
+    peaks = (Peak * cap)()
+    passes = (PassStats * max(cfg.passes, 1))()
+    npk, npa = _sz(), _sz()
+    _check(lib().fbls_search_transits(
+        gpu._h, _dp(time), _dp(flux), _dp(e) if e is not None else None,
+        g.ctypes.data_as(_P(ctypes.c_int64)) if g is not None else None, len(time),
+        ctypes.byref(pc), ctypes.byref(c), peaks, cap, ctypes.byref(npk), passes, len(passes),
+        ctypes.byref(npa)))
+    return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
+            [passes[i].as_dict() for i in range(min(npa.value, len(passes)))])

@@ -16,6 +16,10 @@ namespace host {
 // Records the thread's last error message and returns code.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 double now_s();
+// The biweight detrend of sorted times on the context's device
+// (raw_host.cpp): out[i] = (1 + rel[i]) / location_i - 1.
+int biweight_gpu(fbls_ctx* c, const std::vector<double>& t, const std::vector<double>& rel,
+                 double window, double gap, double cst, int iterations, std::vector<double>* out);
 
 // Growable device buffer.
 struct DevBuf {

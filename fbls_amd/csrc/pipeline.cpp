@@ -1,8 +1,10 @@
-// Multi-pass prepared search (isakruas/fbls src/pipeline.rs:41-221 without
-// single events): per pass the signal search of the family on the GPU, the
-// greedy peak selection with event-overlap rescoring (src/select/peaks.rs),
-// masking of the lead peak and a rebuilt family state; then the model
-// comparison of the strongest peaks.  Rescoring runs on the GPU
+// Multi-pass search (isakruas/fbls src/pipeline.rs:27-221 without single
+// events): per pass the signal search of every family on the GPU, the
+// greedy peak selection with event-overlap rescoring over all families
+// (src/select/peaks.rs), masking of the lead peak in every family and
+// rebuilt family states; then the model comparison of the strongest peaks.
+// Prepared input is one family with every duration; raw input is one family
+// per detrending window (prepare_families).  Rescoring runs on the GPU
 // (select_kernel.hip) over the family's fixed-point fields, so every sum is
 // exact; the host-side statistics use the same fixed-point values.
 #include <hip/hip_runtime.h>
@@ -60,6 +62,8 @@ struct Curve {
 // A family: its curve, its state and the device copy of the state.
 struct Fam {
   Curve curve;
+  std::vector<double> dur;           // the family's durations (days)
+  std::vector<size_t> indices;       // their positions in the bank
   std::vector<double> t, v, w, rho;  // relative time, fixed-point fields
   double C = 0.0, origin = 0.0, energy = 0.0;
   fbls_family* dev = nullptr;
@@ -70,8 +74,10 @@ struct Fam {
 };
 
 // curve.rs:541-569 then the upload; fails below six observations
-int build_fam(fbls_ctx* c, Curve curve, const double* dur, size_t nd, int correlated,
-              Fam* out) {
+int build_fam(fbls_ctx* c, Curve curve, const std::vector<double>& durs,
+              const std::vector<size_t>& indices, int correlated, Fam* out) {
+  const double* dur = durs.data();
+  const size_t nd = durs.size();
   const size_t n = curve.t.size();
   std::vector<double> t(n + 1), v(n + 1), w(n + 1), rho(nd);
   size_t m = 0, segs = 0;
@@ -85,6 +91,8 @@ int build_fam(fbls_ctx* c, Curve curve, const double* dur, size_t nd, int correl
   st = fbls_family_upload(c, t.data(), v.data(), w.data(), m, C, origin, &dev);
   if (st) return st;
   out->curve = std::move(curve);
+  out->dur = durs;
+  out->indices = indices;
   out->t.assign(t.begin(), t.begin() + m);
   out->v.resize(m);
   out->w.resize(m);
@@ -134,17 +142,16 @@ struct Below {
 };
 using Heap = std::priority_queue<Entry, std::vector<Entry>, Below>;
 
-// Selection state of one pass (one family).
+// Selection state of one pass over every family.
 struct Selector {
   fbls_ctx* c;
   const fbls_row* table;
   size_t nrows;
-  const Fam* fam;
-  const double* dur;
-  size_t nd;
+  std::vector<const Fam*> fams;
+  const std::vector<size_t>* family;  // family of each row
   const std::vector<double>* time_all;
   std::vector<uint8_t>* excluded;
-  std::vector<size_t> pos;
+  std::vector<std::vector<size_t>> pos;  // per family: index in time_all
   double threshold;
   const double* d_time_all = nullptr;  // merged times on the device (borrowed)
   DevBuf d_ex_all, d_ex_fam, d_rows, d_out;
@@ -155,11 +162,12 @@ struct Selector {
     d_out.release();
   }
 
-  // Rescorer::evaluate (peaks.rs:148-240) on the GPU
+  // Rescorer::evaluate (peaks.rs:148-240) on the GPU: the counts over the
+  // merged times for every row, then the box sums per family
   int evaluate(const std::vector<size_t>& idx, std::vector<double>& score,
                std::vector<double>& required) {
     const size_t m = idx.size();
-    const size_t n_all = time_all->size(), n = fam->t.size();
+    const size_t n_all = time_all->size();
     std::vector<double> rows(3 * m);
     for (size_t s = 0; s < m; s++) {
       const fbls_row& r = table[idx[s]];
@@ -167,14 +175,11 @@ struct Selector {
       rows[3 * s + 1] = r.epoch;
       rows[3 * s + 2] = r.duration;
     }
-    std::vector<uint8_t> ex_fam(n + 1, 0);
-    for (size_t i = 0; i < n; i++) ex_fam[i] = (*excluded)[pos[i]];
     int st;
-    if ((st = d_ex_all.ensure(n_all + 1)) || (st = d_ex_fam.ensure(n + 1)) ||
-        (st = d_rows.ensure(rows.size() * 8)) || (st = d_out.ensure(m * 24)))
+    if ((st = d_ex_all.ensure(n_all + 1)) || (st = d_rows.ensure(rows.size() * 8)) ||
+        (st = d_out.ensure(m * 24)))
       return st;
     HIP_TRY(hipMemcpyAsync(d_ex_all.ptr, excluded->data(), n_all, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(d_ex_fam.ptr, ex_fam.data(), n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(d_rows.ptr, rows.data(), rows.size() * 8, hipMemcpyHostToDevice,
                            c->stream));
     fbls::CountArgs ca;
@@ -184,47 +189,71 @@ struct Selector {
     ca.rows = (const double*)d_rows.ptr;
     ca.counts = (uint32_t*)((unsigned char*)d_out.ptr + m * 16);
     HIP_TRY(fbls::count_launch(ca, (uint32_t)m, c->stream));
-    fbls::RescoreArgs ra;
-    ra.t = fam->dev->t_dev;
-    ra.vw = fam->dev->vw_dev;
-    ra.excluded = (const uint8_t*)d_ex_fam.ptr;
-    ra.n = (uint32_t)n;
-    ra.origin = fam->origin;
-    ra.rows = (const double*)d_rows.ptr;
-    ra.sums = (long long*)d_out.ptr;
-    HIP_TRY(fbls::rescore_launch(ra, (uint32_t)m, c->stream));
     std::vector<unsigned char> out(m * 24);
-    HIP_TRY(hipMemcpyAsync(out.data(), d_out.ptr, m * 24, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(out.data() + m * 16, (unsigned char*)d_out.ptr + m * 16, m * 8,
+                           hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const long long* sums = (const long long*)out.data();
     const uint32_t* counts = (const uint32_t*)(out.data() + m * 16);
-    score.resize(m);
+    score.assign(m, -INFINITY);
     required.resize(m);
     for (size_t s = 0; s < m; s++) {
       const double total = (double)std::max<uint32_t>(counts[2 * s], 1);
       const double unshared = (double)counts[2 * s + 1] / total;
       required[s] = std::fmax(threshold * std::sqrt(unshared), 0.5 * threshold);
-      const double a = std::ldexp((double)sums[2 * s], -fam->dev->ev);
-      const double hp = std::ldexp((double)sums[2 * s + 1], -fam->dev->ew);
-      const double norm = hp;
-      const double q = norm - hp * hp / fam->C;
-      score[s] = (q > 0.0 && a > 0.0)
-                     ? (a / nearest_rho(dur, fam->rho, nd, table[idx[s]].duration)) /
-                           std::sqrt(std::fmax(q, 1e-300))
-                     : -INFINITY;
+    }
+    for (size_t k = 0; k < fams.size(); k++) {
+      std::vector<size_t> slots;
+      for (size_t s = 0; s < m; s++)
+        if ((*family)[idx[s]] == k) slots.push_back(s);
+      if (slots.empty()) continue;
+      const Fam* fam = fams[k];
+      const size_t n = fam->t.size(), q = slots.size();
+      std::vector<uint8_t> ex_fam(n + 1, 0);
+      for (size_t i = 0; i < n; i++) ex_fam[i] = (*excluded)[pos[k][i]];
+      std::vector<double> frows(3 * q);
+      for (size_t j = 0; j < q; j++)
+        for (int r = 0; r < 3; r++) frows[3 * j + r] = rows[3 * slots[j] + r];
+      if ((st = d_ex_fam.ensure(n + 1))) return st;
+      HIP_TRY(hipMemcpyAsync(d_ex_fam.ptr, ex_fam.data(), n, hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(d_rows.ptr, frows.data(), q * 24, hipMemcpyHostToDevice, c->stream));
+      fbls::RescoreArgs ra;
+      ra.t = fam->dev->t_dev;
+      ra.vw = fam->dev->vw_dev;
+      ra.excluded = (const uint8_t*)d_ex_fam.ptr;
+      ra.n = (uint32_t)n;
+      ra.origin = fam->origin;
+      ra.rows = (const double*)d_rows.ptr;
+      ra.sums = (long long*)d_out.ptr;
+      HIP_TRY(fbls::rescore_launch(ra, (uint32_t)q, c->stream));
+      HIP_TRY(hipMemcpyAsync(out.data(), d_out.ptr, q * 16, hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      const long long* sums = (const long long*)out.data();
+      for (size_t j = 0; j < q; j++) {
+        const size_t s = slots[j];
+        const double a = std::ldexp((double)sums[2 * j], -fam->dev->ev);
+        const double hp = std::ldexp((double)sums[2 * j + 1], -fam->dev->ew);
+        const double norm = hp;
+        const double qq = norm - hp * hp / fam->C;
+        score[s] = (qq > 0.0 && a > 0.0)
+                       ? (a / nearest_rho(fam->dur.data(), fam->rho, fam->dur.size(),
+                                          table[idx[s]].duration)) /
+                             std::sqrt(std::fmax(qq, 1e-300))
+                       : -INFINITY;
+      }
     }
     return FBLS_OK;
   }
 
   // peaks.rs:243-299
-  double without_strongest_event(const fbls_row& row) const {
+  double without_strongest_event(const fbls_row& row, size_t k) const {
     struct Acc {
       double a = 0.0, hp = 0.0, w = 0.0;
     };
+    const Fam* fam = fams[k];
     std::map<int64_t, Acc> sums;
     std::vector<std::pair<size_t, int64_t>> members;
     for (size_t i = 0; i < fam->t.size(); i++) {
-      if ((*excluded)[pos[i]]) continue;
+      if ((*excluded)[pos[k][i]]) continue;
       const double ta = fam->t[i] + fam->origin;
       const double phase =
           rem_euclid(ta - row.epoch + row.period / 2.0, row.period) - row.period / 2.0;
@@ -238,7 +267,7 @@ struct Selector {
       }
     }
     if (sums.empty()) return -INFINITY;
-    const double rho = nearest_rho(dur, fam->rho, nd, row.duration);
+    const double rho = nearest_rho(fam->dur.data(), fam->rho, fam->dur.size(), row.duration);
     int64_t strongest = 0;
     double best = -INFINITY;
     for (const auto& kv : sums) {
@@ -301,7 +330,7 @@ void empty_evidence(fbls_peak* p) {
   p->binary_diagnostics_complete = 0;
 }
 
-// peaks.rs:308-487 for one family
+// peaks.rs:308-487
 int select_peaks(Selector& S, size_t limit, size_t min_events,
                  std::vector<std::vector<uint8_t>>& masks, size_t pass,
                  std::vector<fbls_peak>& accepted) {
@@ -349,7 +378,8 @@ int select_peaks(Selector& S, size_t limit, size_t min_events,
     if (clash) continue;
     const size_t events = event_count(time_all, row.period, row.epoch, row.duration, 3);
     if (events < min_events) continue;
-    const double leftover = S.without_strongest_event(row);
+    const size_t k = (*S.family)[i];
+    const double leftover = S.without_strongest_event(row, k);
     if (min_events > 1 && leftover < 0.5 * S.threshold) continue;
     transit_mask(time_all, row.period, row.epoch, row.duration, margin, wide);
     for (size_t j = 0; j < excluded.size(); j++) excluded[j] |= wide[j];
@@ -365,7 +395,7 @@ int select_peaks(Selector& S, size_t limit, size_t min_events,
     p.depth = row.depth;
     p.depth_ppm = 1e6 * row.depth;
     p.events = events;
-    p.family = 0;
+    p.family = k;
     p.pass = pass;
     p.row = i;
     empty_evidence(&p);
@@ -384,43 +414,33 @@ int check_pipeline(const fbls_pipeline_cfg* cfg) {
   return FBLS_OK;
 }
 
-}  // namespace
-
-extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const double* flux,
-                                    const double* err, size_t n,
-                                    const fbls_pipeline_cfg* cfg, fbls_peak* peaks,
-                                    size_t peaks_cap, size_t* npeaks,
-                                    fbls_pass_stats* passes, size_t passes_cap,
-                                    size_t* npasses) {
-  if (!c || (n && (!time || !flux || !err)) || !npeaks || !npasses)
-    return fail(FBLS_E_ARG, "null argument");
-  *npeaks = 0;
-  *npasses = 0;
-  if (int st = check_pipeline(cfg)) return st;
-  const double* dur = cfg->durations;
-  const size_t nd = cfg->nd;
-  // pipeline.rs:47-61
-  std::vector<size_t> order;
-  for (size_t i = 0; i < n; i++)
-    if (std::isfinite(time[i]) && std::isfinite(flux[i]) && std::isfinite(err[i]) && err[i] > 0.0)
-      order.push_back(i);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t b) { return time[a] < time[b]; });
-  Curve curve;
-  for (size_t i : order) {
-    curve.t.push_back(time[i]);
-    curve.f.push_back(flux[i]);
-    curve.s.push_back(err[i]);
+// pipeline.rs:70-221 over the families
+int search_families(fbls_ctx* c, std::vector<std::pair<Curve, std::vector<size_t>>> input,
+                    const fbls_pipeline_cfg* cfg, fbls_peak* peaks, size_t peaks_cap,
+                    size_t* npeaks, fbls_pass_stats* passes, size_t passes_cap,
+                    size_t* npasses) {
+  const size_t nfam = input.size();
+  std::vector<std::unique_ptr<Fam>> cur(nfam);
+  std::vector<Curve> original(nfam);
+  std::vector<double> time_all;
+  for (size_t k = 0; k < nfam; k++) {
+    std::vector<double> durs;
+    for (size_t i : input[k].second) {
+      if (i >= cfg->nd) return fail(FBLS_E_ARG, "duration index %zu out of range", i);
+      durs.push_back(cfg->durations[i]);
+    }
+    if (durs.empty()) return fail(FBLS_E_ARG, "family %zu has no durations", k);
+    cur[k] = std::make_unique<Fam>();
+    if (int st = build_fam(c, std::move(input[k].first), durs, input[k].second, cfg->correlated,
+                           cur[k].get()))
+      return st;
+    original[k] = cur[k]->curve;
+    time_all.insert(time_all.end(), cur[k]->curve.t.begin(), cur[k]->curve.t.end());
   }
-  HIP_TRY(hipSetDevice(c->device));
-  auto cur = std::make_unique<Fam>();
-  if (int st = build_fam(c, curve, dur, nd, cfg->correlated, cur.get())) return st;
   // merged_time (pipeline.rs:16-24)
-  std::vector<double> time_all = cur->curve.t;
   std::sort(time_all.begin(), time_all.end());
   time_all.erase(std::unique(time_all.begin(), time_all.end()), time_all.end());
   if (time_all.size() < 6) return fail(FBLS_E_DATA, "at least six usable observations are required");
-  const Curve original = cur->curve;
   std::vector<uint8_t> excluded(time_all.size(), 0);
   std::vector<std::vector<uint8_t>> masks;
   std::vector<fbls_peak> found;
@@ -431,53 +451,69 @@ extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const doubl
   DevBuf& d_time_all = g.b;
   if (int st = d_time_all.ensure(time_all.size() * 8)) return st;
   HIP_TRY(hipMemcpy(d_time_all.ptr, time_all.data(), time_all.size() * 8, hipMemcpyHostToDevice));
-  fbls_bank bank = cfg->bank;
   for (size_t pass = 0; pass < cfg->passes; pass++) {
     const double t0 = now_s();
-    // signal_search of the family (scan.rs:413-473)
-    const double d0 = *std::min_element(dur, dur + nd);
-    const double hi = std::min(cfg->search.period_max, cur->dev->t_last);
-    const double lo = std::max(cfg->search.period_min, 4.0 * d0 * (1.0 + 1e-9));
-    size_t cap = 1;
-    if (hi > lo)
-      cap = std::max<size_t>(1, fbls_period_grid_count(lo, hi, cur->dev->t_last, d0,
-                                                       cfg->search.period_oversample));
-    std::vector<fbls_row> table(cap);
-    std::vector<fbls_scan_stats> bstats(64);
-    size_t nrows = 0, nbands = 0, grid_count = 0;
-    bank.durations = dur;
-    bank.rho = cur->rho.data();
-    bank.nd = nd;
-    int st = fbls_signal_search(c, cur->dev, dur, cur->rho.data(), nd, &bank, &cfg->search,
-                                table.data(), cap, &nrows, bstats.data(), bstats.size(),
-                                &nbands, nullptr, 0, &grid_count);
-    if (st) return st;
+    // signal_search of every family (scan.rs:413-473), tables concatenated
+    std::vector<fbls_row> table;
+    std::vector<size_t> family;
+    size_t grid_total = 0;
+    for (size_t k = 0; k < nfam; k++) {
+      const Fam& f = *cur[k];
+      const size_t nd = f.dur.size();
+      const double d0 = *std::min_element(f.dur.begin(), f.dur.end());
+      const double hi = std::min(cfg->search.period_max, f.dev->t_last);
+      const double lo = std::max(cfg->search.period_min, 4.0 * d0 * (1.0 + 1e-9));
+      size_t cap = 1;
+      if (hi > lo)
+        cap = std::max<size_t>(1, fbls_period_grid_count(lo, hi, f.dev->t_last, d0,
+                                                         cfg->search.period_oversample));
+      const size_t base = table.size();
+      table.resize(base + cap);
+      std::vector<fbls_scan_stats> bstats(64);
+      size_t nrows = 0, nbands = 0, grid_count = 0;
+      fbls_bank bank = cfg->bank;
+      bank.durations = f.dur.data();
+      bank.rho = f.rho.data();
+      bank.nd = nd;
+      int st = fbls_signal_search(c, f.dev, f.dur.data(), f.rho.data(), nd, &bank, &cfg->search,
+                                  table.data() + base, cap, &nrows, bstats.data(), bstats.size(),
+                                  &nbands, nullptr, 0, &grid_count);
+      if (st) return st;
+      table.resize(base + nrows);
+      family.resize(base + nrows, k);
+      grid_total += grid_count;
+    }
     const double t1 = now_s();
     Selector S;
     S.c = c;
     S.table = table.data();
-    S.nrows = nrows;
-    S.fam = cur.get();
-    S.dur = dur;
-    S.nd = nd;
+    S.nrows = table.size();
+    S.family = &family;
     S.time_all = &time_all;
     S.excluded = &excluded;
     S.threshold = cfg->bank.threshold;
-    S.pos.resize(cur->t.size());
-    for (size_t i = 0; i < cur->t.size(); i++) {
-      const size_t p = std::lower_bound(time_all.begin(), time_all.end(), cur->t[i] + cur->origin) -
-                       time_all.begin();
-      S.pos[i] = std::min(p, time_all.size() - 1);
+    S.fams.resize(nfam);
+    S.pos.resize(nfam);
+    for (size_t k = 0; k < nfam; k++) {
+      const Fam& f = *cur[k];
+      S.fams[k] = &f;
+      S.pos[k].resize(f.t.size());
+      for (size_t i = 0; i < f.t.size(); i++) {
+        const size_t p =
+            std::lower_bound(time_all.begin(), time_all.end(), f.t[i] + f.origin) -
+            time_all.begin();
+        S.pos[k][i] = std::min(p, time_all.size() - 1);
+      }
     }
     S.d_time_all = (const double*)d_time_all.ptr;
     const size_t before = found.size();
-    st = select_peaks(S, cfg->max_candidates - before, cfg->min_events, masks, pass + 1, found);
+    int st = select_peaks(S, cfg->max_candidates - before, cfg->min_events, masks, pass + 1, found);
     if (st) return st;
     if (*npasses < passes_cap && passes) {
       fbls_pass_stats& ps = passes[*npasses];
       ps.pass = pass + 1;
-      ps.rows = nrows;
-      ps.periods = grid_count;
+      ps.rows = table.size();
+      ps.periods = grid_total;
       ps.accepted = found.size() - before;
       ps.seconds_scan = t1 - t0;
       ps.seconds_select = now_s() - t1;
@@ -486,12 +522,18 @@ extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const doubl
     if (found.size() == before) break;
     const fbls_peak lead = found[before];
     if (pass + 1 < cfg->passes) {
-      Curve next = masked(cur->curve, lead.period_days, lead.epoch_bjd,
-                          lead.duration_hours / 24.0, 0.5);
-      auto fam = std::make_unique<Fam>();
-      if ((st = build_fam(c, std::move(next), dur, nd, cfg->correlated, fam.get()))) return st;
-      cur = std::move(fam);
-      if (cur->curve.t.size() < 6) break;
+      bool small = false;
+      for (size_t k = 0; k < nfam; k++) {
+        Curve next = masked(cur[k]->curve, lead.period_days, lead.epoch_bjd,
+                            lead.duration_hours / 24.0, 0.5);
+        auto fam = std::make_unique<Fam>();
+        if ((st = build_fam(c, std::move(next), cur[k]->dur, cur[k]->indices, cfg->correlated,
+                            fam.get())))
+          return st;
+        cur[k] = std::move(fam);
+        small |= cur[k]->curve.t.size() < 6;
+      }
+      if (small) break;
     }
   }
   // model comparison of the strongest peaks (pipeline.rs:169-221)
@@ -502,18 +544,20 @@ extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const doubl
                      [&](size_t a, size_t b) { return found[a].snr > found[b].snr; });
     for (size_t oi = 0; oi < ord.size() && oi < cfg->evidence_candidates; oi++) {
       fbls_peak& p = found[ord[oi]];
-      Curve cv = original;
+      const Fam& base = *cur[p.family];
+      Curve cv = original[p.family];
       for (const fbls_peak& o : found) {
         if (o.row == p.row && o.pass == p.pass && o.family == p.family) continue;
         cv = masked(cv, o.period_days, o.epoch_bjd, o.duration_hours / 24.0, 0.5);
       }
       if (cv.t.size() < 6) continue;
       Fam s;
-      if (int st = build_fam(c, std::move(cv), dur, nd, cfg->correlated, &s)) return st;
+      if (int st = build_fam(c, std::move(cv), base.dur, base.indices, cfg->correlated, &s))
+        return st;
       fbls_evidence_row rows[2];
       size_t nr = 0;
-      if (int st = fbls_compare(c, s.dev, p.period_days, dur, s.rho.data(), nd, s.energy,
-                                &cfg->evidence, rows, &nr, nullptr))
+      if (int st = fbls_compare(c, s.dev, p.period_days, s.dur.data(), s.rho.data(),
+                                s.dur.size(), s.energy, &cfg->evidence, rows, &nr, nullptr))
         return st;
       const fbls_evidence_row* r = nullptr;
       for (size_t q = 0; q < nr && !r; q++)
@@ -542,4 +586,72 @@ extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const doubl
   if (found.size() > peaks_cap)
     return fail(FBLS_E_ARG, "peaks capacity %zu < %zu", peaks_cap, found.size());
   return FBLS_OK;
+}
+
+}  // namespace
+
+extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const double* flux,
+                                    const double* err, size_t n,
+                                    const fbls_pipeline_cfg* cfg, fbls_peak* peaks,
+                                    size_t peaks_cap, size_t* npeaks,
+                                    fbls_pass_stats* passes, size_t passes_cap,
+                                    size_t* npasses) {
+  if (!c || (n && (!time || !flux || !err)) || !npeaks || !npasses)
+    return fail(FBLS_E_ARG, "null argument");
+  *npeaks = 0;
+  *npasses = 0;
+  if (int st = check_pipeline(cfg)) return st;
+  // pipeline.rs:47-61
+  std::vector<size_t> order;
+  for (size_t i = 0; i < n; i++)
+    if (std::isfinite(time[i]) && std::isfinite(flux[i]) && std::isfinite(err[i]) && err[i] > 0.0)
+      order.push_back(i);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return time[a] < time[b]; });
+  Curve curve;
+  for (size_t i : order) {
+    curve.t.push_back(time[i]);
+    curve.f.push_back(flux[i]);
+    curve.s.push_back(err[i]);
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  std::vector<size_t> all(cfg->nd);
+  for (size_t i = 0; i < cfg->nd; i++) all[i] = i;
+  std::vector<std::pair<Curve, std::vector<size_t>>> input;
+  input.emplace_back(std::move(curve), std::move(all));
+  return search_families(c, std::move(input), cfg, peaks, peaks_cap, npeaks, passes, passes_cap,
+                         npasses);
+}
+
+extern "C" int fbls_search_transits(fbls_ctx* c, const double* time, const double* flux,
+                                    const double* err, const int64_t* group, size_t n,
+                                    const fbls_prep_cfg* prep, const fbls_pipeline_cfg* cfg,
+                                    fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
+                                    fbls_pass_stats* passes, size_t passes_cap,
+                                    size_t* npasses) {
+  if (!c || !prep || (n && (!time || !flux)) || !npeaks || !npasses)
+    return fail(FBLS_E_ARG, "null argument");
+  *npeaks = 0;
+  *npasses = 0;
+  if (int st = check_pipeline(cfg)) return st;
+  HIP_TRY(hipSetDevice(c->device));
+  std::vector<fbls::Family> fams;
+  std::string msg;
+  auto flatten = [c](const std::vector<double>& t, const std::vector<double>& rel, double w,
+                     double gap, double cst, int it, std::vector<double>* out) {
+    return biweight_gpu(c, t, rel, w, gap, cst, it, out);
+  };
+  int st = fbls::prepare_families(time, flux, err, group, n, cfg->durations, cfg->nd, prep,
+                                  flatten, &fams, &msg);
+  if (st) return msg.empty() ? st : fail(st, "%s", msg.c_str());
+  std::vector<std::pair<Curve, std::vector<size_t>>> input;
+  for (fbls::Family& f : fams) {
+    Curve cv;
+    cv.t = std::move(f.time);
+    cv.f = std::move(f.flux);
+    cv.s = std::move(f.sigma);
+    input.emplace_back(std::move(cv), std::move(f.indices));
+  }
+  return search_families(c, std::move(input), cfg, peaks, peaks_cap, npeaks, passes, passes_cap,
+                         npasses);
 }

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <map>
 #include <vector>
 
 namespace fbls {
@@ -114,9 +115,11 @@ struct Noise {
   size_t segments = 0;
 };
 
-// curve.rs:89-198 with bin_minutes = 0 and one group.
+// curve.rs:89-198: per-group median normalisation, optional inverse-variance
+// binning (never across groups), robust inflation from adjacent differences.
 int fit_segment_noise(const double* time, const double* flux, const double* err,
-                      size_t n, double gap, Noise* out, std::string* msg) {
+                      const int64_t* group, size_t n, double bin_minutes, double gap,
+                      Noise* out, std::string* msg) {
   std::vector<size_t> keep;
   for (size_t i = 0; i < n; i++) {
     const double e = err ? err[i] : 1.0;
@@ -129,27 +132,81 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
     *msg = "insufficient usable observations";
     return FBLS_E_DATA;
   }
-  const size_t m = keep.size();
+  size_t m = keep.size();
   std::vector<double> t(m), f(m), e(m);
+  std::vector<int64_t> sec(m);
   for (size_t k = 0; k < m; k++) {
     t[k] = time[keep[k]];
     f[k] = flux[keep[k]];
     e[k] = err ? err[keep[k]] : 1.0;
+    sec[k] = group ? group[keep[k]] : 0;
   }
-  const double med = median_lower(f);
-  if (!(med > 0.0)) {
-    *msg = "mission flux must have a positive median";
-    return FBLS_E_DATA;
-  }
-  for (size_t k = 0; k < m; k++) {
-    f[k] = f[k] / med - 1.0;
-    e[k] /= med;
+  std::vector<int64_t> labels = sec;
+  std::sort(labels.begin(), labels.end());
+  labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
+  for (int64_t label : labels) {
+    std::vector<double> vals;
+    for (size_t k = 0; k < m; k++)
+      if (sec[k] == label) vals.push_back(f[k]);
+    const double med = median_lower(vals);
+    if (!(med > 0.0)) {
+      *msg = "mission flux must have a positive median";
+      return FBLS_E_DATA;
+    }
+    for (size_t k = 0; k < m; k++)
+      if (sec[k] == label) {
+        f[k] = f[k] / med - 1.0;
+        e[k] /= med;
+      }
   }
   const double origin = t[0];
   for (double& x : t) x -= origin;
+  if (bin_minutes > 0.0) {
+    const double width = bin_minutes / 1440.0;
+    std::map<std::pair<int64_t, int64_t>, size_t> slot;
+    std::vector<std::pair<int64_t, int64_t>> keys(m);
+    for (size_t k = 0; k < m; k++) {
+      keys[k] = {(int64_t)std::floor(t[k] / width), sec[k]};
+      slot.emplace(keys[k], 0);
+    }
+    size_t u = 0;
+    std::vector<int64_t> usec;
+    for (auto& kv : slot) {
+      kv.second = u++;
+      usec.push_back(kv.first.second);
+    }
+    std::vector<double> den(u, 0.0), st(u, 0.0), sf(u, 0.0);
+    for (size_t k = 0; k < m; k++) {
+      const size_t j = slot[keys[k]];
+      const double w = 1.0 / (e[k] * e[k]);
+      den[j] += w;
+      st[j] += t[k] * w;
+      sf[j] += f[k] * w;
+    }
+    std::vector<double> bt(u), bf(u), be(u);
+    for (size_t j = 0; j < u; j++) {
+      bt[j] = st[j] / den[j];
+      bf[j] = sf[j] / den[j];
+      be[j] = 1.0 / std::sqrt(den[j]);
+    }
+    std::vector<size_t> order(u);
+    for (size_t j = 0; j < u; j++) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return bt[a] < bt[b]; });
+    t.resize(u);
+    f.resize(u);
+    e.resize(u);
+    sec.resize(u);
+    for (size_t j = 0; j < u; j++) {
+      t[j] = bt[order[j]];
+      f[j] = bf[order[j]];
+      e[j] = be[order[j]];
+      sec[j] = usec[order[j]];
+    }
+    m = u;
+  }
   std::vector<double> deltas, standardized;
   for (size_t i = 1; i < m; i++) {
-    if (t[i] - t[i - 1] < gap) {
+    if (t[i] - t[i - 1] < gap && sec[i] == sec[i - 1]) {
       const double d = f[i] - f[i - 1];
       deltas.push_back(d);
       standardized.push_back(d / std::sqrt(e[i] * e[i] + e[i - 1] * e[i - 1]));
@@ -176,7 +233,7 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
   }
   size_t seg = 1;
   for (size_t i = 1; i < m; i++)
-    if (t[i] - t[i - 1] > gap) seg++;
+    if (t[i] - t[i - 1] > gap || sec[i] != sec[i - 1]) seg++;
   out->deficit.resize(m);
   for (size_t k = 0; k < m; k++) out->deficit[k] = -f[k];
   out->t = std::move(t);
@@ -184,6 +241,11 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
   out->inflation = inflation;
   out->segments = seg;
   return 0;
+}
+
+int fit_segment_noise(const double* time, const double* flux, const double* err,
+                      size_t n, double gap, Noise* out, std::string* msg) {
+  return fit_segment_noise(time, flux, err, nullptr, n, 0.0, gap, out, msg);
 }
 
 // curve.rs:456-539
@@ -267,6 +329,82 @@ void correlated_noise_factors(const std::vector<double>& time,
 }
 
 }  // namespace
+
+// curve.rs:331-350: distance of each point to the nearest edge of its
+// gap-separated segment
+static std::vector<double> segment_edges(const std::vector<double>& t, double gap) {
+  const size_t n = t.size();
+  std::vector<double> out(n, 0.0);
+  size_t start = 0;
+  for (size_t i = 1; i <= n; i++) {
+    if (i == n || t[i] - t[i - 1] > gap) {
+      for (size_t j = start; j < i; j++)
+        out[j] = std::min(t[j] - t[start], t[i - 1] - t[j]);
+      start = i;
+    }
+  }
+  return out;
+}
+
+std::vector<std::pair<double, std::vector<size_t>>> detrending_windows(
+    const double* durations, size_t nd, double minimum, double factor, double maximum) {
+  // curve.rs:355-377: clamp(factor D, min, max) rounded to 1e-6 d
+  std::vector<double> w(nd);
+  for (size_t i = 0; i < nd; i++) {
+    const double x = std::min(std::max(factor * durations[i], minimum), maximum);
+    w[i] = std::round(x * 1e6) / 1e6;
+  }
+  std::vector<double> uniq = w;
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  std::vector<std::pair<double, std::vector<size_t>>> out;
+  for (double x : uniq) {
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < nd; i++)
+      if (w[i] == x) idx.push_back(i);
+    out.emplace_back(x, idx);
+  }
+  return out;
+}
+
+// curve.rs:387-425
+int prepare_families(const double* time, const double* flux, const double* err,
+                     const int64_t* group, size_t n, const double* durations, size_t nd,
+                     const fbls_prep_cfg* cfg, const Flatten& flatten,
+                     std::vector<Family>* out, std::string* msg) {
+  Noise diag;
+  int st = fit_segment_noise(time, flux, err, group, n, cfg->bin_minutes, 0.75, &diag, msg);
+  if (st) return st;
+  const size_t m = diag.t.size();
+  std::vector<double> sigma(m), relative(m);
+  for (size_t i = 0; i < m; i++) {
+    sigma[i] = std::sqrt(diag.variance[i]);
+    relative[i] = -diag.deficit[i];
+  }
+  const std::vector<double> edge = segment_edges(diag.t, 0.5);
+  out->clear();
+  for (auto& wi : detrending_windows(durations, nd, cfg->window_min, cfg->window_factor,
+                                     cfg->window_max)) {
+    const double window = wi.first;
+    std::vector<double> flat;
+    if ((st = flatten(diag.t, relative, window, 0.5, 5.0, 10, &flat))) return st;
+    const double scale = mad_scale(flat);
+    Family fam;
+    fam.window = window;
+    fam.indices = wi.second;
+    for (size_t i = 0; i < m; i++) {
+      if (std::isfinite(flat[i]) && flat[i] <= cfg->clip_sigma * scale &&
+          edge[i] >= cfg->edge_trim * window) {
+        fam.time.push_back(diag.t[i] + diag.origin);
+        fam.flux.push_back(flat[i]);
+        fam.sigma.push_back(sigma[i]);
+      }
+    }
+    fam.clipped = m - fam.time.size();
+    out->push_back(std::move(fam));
+  }
+  return 0;
+}
 
 int family_state(const double* time, const double* flux, const double* sigma,
                  size_t n, const double* durations, size_t nd, int correlated,

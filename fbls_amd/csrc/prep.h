@@ -5,7 +5,9 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <functional>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/fbls.h"
@@ -36,6 +38,27 @@ int family_state(const double* time, const double* flux, const double* sigma,
                  double* baseline_norm, double* origin, double* rho,
                  double* inflation, size_t* segments, double* residual_energy,
                  std::string* msg);
+
+// One detrended family of the raw path (curve.rs:23-41 PreparedCurve).
+struct Family {
+  std::vector<double> time, flux, sigma;  // absolute time, relative flux, sigma
+  double window = 0.0;
+  size_t clipped = 0;
+  std::vector<size_t> indices;            // bank durations of this family
+};
+// The biweight detrend (curve.rs:204-248): out[i] = (1 + rel[i]) /
+// location_i - 1 over sorted times t, windows per gap-separated segment.
+using Flatten = std::function<int(const std::vector<double>& t, const std::vector<double>& rel,
+                                  double window, double gap, double c, int iterations,
+                                  std::vector<double>* out)>;
+// curve.rs:355-377
+std::vector<std::pair<double, std::vector<size_t>>> detrending_windows(
+    const double* durations, size_t nd, double minimum, double factor, double maximum);
+// curve.rs:387-425
+int prepare_families(const double* time, const double* flux, const double* err,
+                     const int64_t* group, size_t n, const double* durations, size_t nd,
+                     const fbls_prep_cfg* cfg, const Flatten& flatten,
+                     std::vector<Family>* out, std::string* msg);
 
 int synth_star(const fbls_synth_cfg* cfg, double* time, double* flux,
                double* flux_err, size_t cap, size_t* n, std::string* msg);

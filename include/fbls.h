@@ -349,6 +349,51 @@ int fbls_coarse_features(fbls_ctx* ctx, const fbls_family* const* fams, size_t n
                          const double* periods, size_t np, const double* durations,
                          size_t ng, uint32_t nb_min, uint32_t nb_max, double* out);
 
+/* -- raw-mode preparation ----------------------------------------------- */
+
+/* Preparation options of the raw path (config.rs:11-16, defaults 10, 0.4,
+ * 3, 3.5, 4, 0.5). */
+typedef struct {
+  double bin_minutes, window_min, window_factor, window_max, clip_sigma, edge_trim;
+} fbls_prep_cfg;
+
+/* `biweight_flatten_gpu` (curve.rs:256-329, kernel kernels.rs:1049-1110):
+ * Tukey biweight location in a sliding window of `window` days per
+ * gap-separated segment of the sorted times, on the GPU; out[i] =
+ * (1 + rel[i]) / location_i - 1.  Exact lower medians; windows of any size
+ * (the reference falls back to its host loop beyond 4096 points; here the
+ * kernel reads such windows from global memory). */
+int fbls_biweight_flatten(fbls_ctx* ctx, const double* time, const double* rel, size_t n,
+                          double window, double gap, double c, uint32_t iterations,
+                          double* out);
+
+/* `prepare_families` (curve.rs:387-425): per-group normalisation and
+ * inverse-variance binning (fit_segment_noise, curve.rs:89-198), one
+ * biweight-detrended family per detrending window, positive outliers
+ * (flux > clip_sigma x MAD scale) and points within edge_trim windows of a
+ * segment edge dropped.  err and group may be NULL.  Family k (< *nfam <=
+ * nd): fam_n[k] points at fam_time/fam_flux/fam_sigma[k * n ..] (absolute
+ * time), its window, the clipped count, and fam_members[k * nd + i] = 1 for
+ * the durations it searches. */
+int fbls_prepare_families(fbls_ctx* ctx, const double* time, const double* flux,
+                          const double* err, const int64_t* group, size_t n,
+                          const double* durations, size_t nd, const fbls_prep_cfg* cfg,
+                          size_t* nfam, size_t* fam_n, double* fam_time, double* fam_flux,
+                          double* fam_sigma, double* fam_window, size_t* fam_clipped,
+                          uint8_t* fam_members);
+
+/* `search_transits` (pipeline.rs:27-37, 70-221, without single events):
+ * fbls_prepare_families on the raw flux, then the multi-pass search of
+ * fbls_search_prepared over all families together (one signal search per
+ * family and pass, tables concatenated, peaks selected across families,
+ * every family masked with the pass's lead peak).  fbls_peak.family is the
+ * family index of fbls_prepare_families.  err and group may be NULL. */
+int fbls_search_transits(fbls_ctx* ctx, const double* time, const double* flux,
+                         const double* err, const int64_t* group, size_t n,
+                         const fbls_prep_cfg* prep, const fbls_pipeline_cfg* cfg,
+                         fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
+                         fbls_pass_stats* passes, size_t passes_cap, size_t* npasses);
+
 /* -- host helpers [host] ------------------------------------------------ */
 
 /* [host] `period_grid` (grid.rs:5-27): count first, then fill. */

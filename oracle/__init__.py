@@ -109,6 +109,11 @@ class OrcPipelineCfg(ctypes.Structure):
                 ("planet_log_odds_min", _d), ("evidence", EvidenceCfg)]
 
 
+class PrepCfg(ctypes.Structure):
+    _fields_ = [(k, _d) for k in ("bin_minutes", "window_min", "window_factor",
+                                  "window_max", "clip_sigma", "edge_trim")]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -153,6 +158,17 @@ def lib():
         L.orc_coarse_features.argtypes = [_P(_d), _P(_d), _P(_d), _sz, _d, _P(_d), _sz,
                                           _P(_d), _sz, _u32, _u32, _P(_d)]
         L.orc_coarse_features.restype = None
+        L.orc_fit_segment_noise_ex.argtypes = [
+            _P(_d), _P(_d), _P(_d), _P(ctypes.c_int64), _sz, _d, _d, _P(_d), _P(_d), _P(_d),
+            _P(_sz), _P(_d), _P(_d), _P(_sz)]
+        L.orc_biweight_flatten.argtypes = [_P(_d), _P(_d), _sz, _d, _d, _d, _sz, _P(_d)]
+        L.orc_biweight_flatten.restype = None
+        L.orc_prepare_families.argtypes = [
+            _P(_d), _P(_d), _P(_d), _P(ctypes.c_int64), _sz, _P(_d), _sz, _P(PrepCfg),
+            _P(_sz), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz), _P(ctypes.c_uint8)]
+        L.orc_search_transits.argtypes = [
+            _P(_d), _P(_d), _P(_d), _P(ctypes.c_int64), _sz, _P(PrepCfg), _P(OrcPipelineCfg),
+            _P(OrcPeak), _sz, _P(_sz), _P(OrcPassStats), _P(_sz)]
         L.orc_transit_mask.argtypes = [_P(_d), _sz, _d, _d, _d, _d, _P(ctypes.c_uint8)]
         L.orc_transit_mask.restype = None
         L.orc_event_count.argtypes = [_P(_d), _sz, _d, _d, _d, _sz]
@@ -502,10 +518,7 @@ def count_row(time_all, excluded, period, epoch, duration):
     return out
 
 
-def search_prepared(time, flux, err, cfg):
-    """search_prepared + search_families (pipeline.rs:41-221, no single
-    events): (peaks in acceptance order, pass statistics)."""
-    time, flux, err = _f64(time), _f64(flux), _f64(err)
+def _pipeline_cfg(cfg, keep):
     d = _f64(cfg.durations_days())
     ing = _f64(cfg.ingress_fractions)
     ev = evidence_cfg(cfg)
@@ -517,6 +530,16 @@ def search_prepared(time, flux, err, cfg):
         min_events=cfg.min_events, evidence_candidates=cfg.evidence_candidates,
         model_comparison=int(bool(cfg.model_comparison)),
         planet_log_odds_min=cfg.planet_log_odds_min, evidence=ev)
+    keep.extend([d, ing])
+    return c
+
+
+def search_prepared(time, flux, err, cfg):
+    """search_prepared + search_families (pipeline.rs:41-221, no single
+    events): (peaks in acceptance order, pass statistics)."""
+    time, flux, err = _f64(time), _f64(flux), _f64(err)
+    keep = []
+    c = _pipeline_cfg(cfg, keep)
     cap = max(cfg.max_candidates, 1)
     peaks = (OrcPeak * cap)()
     passes = (OrcPassStats * max(cfg.passes, 1))()
@@ -539,3 +562,92 @@ def coarse_features(t, v, w, baseline_norm, periods, durations, nb_min=16, nb_ma
                               len(periods), _dp(durations), len(durations), nb_min, nb_max,
                               _dp(out))
     return out
+
+
+# ------------------------------------------------------------ raw mode
+
+def _i64p(a):
+    return None if a is None else a.ctypes.data_as(_P(ctypes.c_int64))
+
+
+def fit_segment_noise_ex(time, flux, err=None, group=None, bin_minutes=0.0, gap=0.75):
+    time, flux = _f64(time), _f64(flux)
+    n = len(time)
+    e = _f64(err) if err is not None else None
+    g = np.ascontiguousarray(group, dtype=np.int64) if group is not None else None
+    t, d, v = np.empty(n), np.empty(n), np.empty(n)
+    m, org, infl, seg = _sz(), _d(), _d(), _sz()
+    st = lib().orc_fit_segment_noise_ex(
+        _dp(time), _dp(flux), _dp(e) if e is not None else None, _i64p(g), n, bin_minutes,
+        gap, _dp(t), _dp(d), _dp(v), ctypes.byref(m), ctypes.byref(org), ctypes.byref(infl),
+        ctypes.byref(seg))
+    if st:
+        raise ValueError(f"fit_segment_noise status {st}")
+    k = m.value
+    return dict(time=t[:k], deficit=d[:k], variance=v[:k], origin=org.value,
+                inflation=infl.value, segments=seg.value)
+
+
+def biweight_flatten(time, rel, window, gap=0.5, c=5.0, iterations=10):
+    time, rel = _f64(time), _f64(rel)
+    out = np.empty(len(time))
+    lib().orc_biweight_flatten(_dp(time), _dp(rel), len(time), window, gap, c, iterations,
+                               _dp(out))
+    return out
+
+
+def prep_cfg(cfg):
+    return PrepCfg(cfg.bin_minutes, cfg.window_days, cfg.window_factor, cfg.window_max_days,
+                   cfg.clip_sigma, cfg.edge_trim_windows)
+
+
+def prepare_families(time, flux, err, group, durations, cfg):
+    """prepare_families (curve.rs:387-425): list of family dicts."""
+    time, flux = _f64(time), _f64(flux)
+    n = len(time)
+    e = _f64(err) if err is not None else None
+    g = np.ascontiguousarray(group, dtype=np.int64) if group is not None else None
+    d = _f64(durations)
+    nd = len(d)
+    pc = prep_cfg(cfg)
+    nf = _sz()
+    fn = np.zeros(nd, dtype=np.uintp)
+    ft, ff, fs = np.zeros(nd * n), np.zeros(nd * n), np.zeros(nd * n)
+    fw = np.zeros(nd)
+    fc = np.zeros(nd, dtype=np.uintp)
+    fm = np.zeros(nd * nd, dtype=np.uint8)
+    st = lib().orc_prepare_families(
+        _dp(time), _dp(flux), _dp(e) if e is not None else None, _i64p(g), n, _dp(d), nd,
+        ctypes.byref(pc), ctypes.byref(nf), fn.ctypes.data_as(_P(_sz)), _dp(ft), _dp(ff),
+        _dp(fs), _dp(fw), fc.ctypes.data_as(_P(_sz)), fm.ctypes.data_as(_P(ctypes.c_uint8)))
+    if st:
+        raise ValueError(f"prepare_families status {st}")
+    out = []
+    for k in range(nf.value):
+        m = int(fn[k])
+        out.append(dict(time=ft[k * n:k * n + m], flux=ff[k * n:k * n + m],
+                        sigma=fs[k * n:k * n + m], window=float(fw[k]), clipped=int(fc[k]),
+                        indices=np.nonzero(fm[k * nd:(k + 1) * nd])[0]))
+    return out
+
+
+def search_transits(time, flux, err, group, cfg):
+    """search_transits (pipeline.rs:27-37, 70-221, no single events) on raw
+    flux: (peaks in acceptance order, pass statistics)."""
+    time, flux = _f64(time), _f64(flux)
+    e = _f64(err) if err is not None else None
+    g = np.ascontiguousarray(group, dtype=np.int64) if group is not None else None
+    keep = []
+    c = _pipeline_cfg(cfg, keep)
+    pc = prep_cfg(cfg)
+    cap = max(cfg.max_candidates, 1)
+    peaks = (OrcPeak * cap)()
+    passes = (OrcPassStats * max(cfg.passes, 1))()
+    npk, npa = _sz(), _sz()
+    st = lib().orc_search_transits(_dp(time), _dp(flux), _dp(e) if e is not None else None,
+                                   _i64p(g), len(time), ctypes.byref(pc), ctypes.byref(c),
+                                   peaks, cap, ctypes.byref(npk), passes, ctypes.byref(npa))
+    if st:
+        raise ValueError(f"search_transits status {st}")
+    return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
+            [passes[i].as_dict() for i in range(npa.value)])

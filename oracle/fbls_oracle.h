@@ -260,6 +260,32 @@ void orc_coarse_features(const double* t, const double* v, const double* w, size
                          const double* durations, size_t ng, uint32_t nb_min,
                          uint32_t nb_max, double* out);
 
+/* ------------------------------------------------------------ raw mode
+ * oracle/raw.c: curve.rs:89-198 with groups and binning, and the biweight
+ * detrend of curve.rs:204-248. */
+int orc_fit_segment_noise_ex(const double* time, const double* flux, const double* err,
+                             const int64_t* group, size_t n, double bin_minutes,
+                             double gap, double* out_t, double* out_def, double* out_var,
+                             size_t* out_n, double* origin, double* inflation,
+                             size_t* segments);
+void orc_biweight_flatten(const double* time, const double* rel, size_t n, double window,
+                          double gap, double c, size_t iterations, double* out);
+typedef struct {
+  double bin_minutes, window_min, window_factor, window_max, clip_sigma, edge_trim;
+} orc_prep_cfg;
+int orc_prepare_families(const double* time, const double* flux, const double* err,
+                         const int64_t* group, size_t n, const double* durations, size_t nd,
+                         const orc_prep_cfg* cfg, size_t* nfam, size_t* fam_n,
+                         double* fam_time, double* fam_flux, double* fam_sigma,
+                         double* fam_window, size_t* fam_clipped, uint8_t* fam_members);
+
+/* pipeline.rs:27-37 then 70-221 (search_transits without single events):
+ * the families of orc_prepare_families searched together. */
+int orc_search_transits(const double* time, const double* flux, const double* err,
+                        const int64_t* group, size_t n, const orc_prep_cfg* prep,
+                        const orc_pipeline_cfg* cfg, orc_peak* peaks, size_t cap,
+                        size_t* npeaks, orc_pass_stats* pstats, size_t* npasses);
+
 /* survey/numeric.rs:345-364 */
 uint64_t orc_rng_new(uint64_t seed);
 double orc_rng_uniform(uint64_t* state);

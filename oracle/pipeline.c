@@ -113,6 +113,8 @@ typedef struct {
   size_t n;
   double C, origin, energy;
   double* rho;
+  double* dur; /* the family's durations (days) */
+  size_t* idx; /* their positions in the bank */
   size_t nd;
 } fam_t;
 
@@ -124,14 +126,21 @@ static void fam_free(fam_t* f) {
   free(f->v);
   free(f->w);
   free(f->rho);
+  free(f->dur);
+  free(f->idx);
   memset(f, 0, sizeof(*f));
 }
 
 /* curve.rs:541-569 plus the library's fixed-point step */
 static int fam_build(const double* ct, const double* cf, const double* cs,
-                     size_t cn, const double* durations, size_t nd,
-                     int correlated, fam_t* f) {
+                     size_t cn, const double* durations, const size_t* indices,
+                     size_t nd, int correlated, fam_t* f) {
   memset(f, 0, sizeof(*f));
+  f->nd = nd;
+  f->dur = (double*)malloc((nd + 1) * 8);
+  f->idx = (size_t*)malloc((nd + 1) * sizeof(size_t));
+  memcpy(f->dur, durations, nd * 8);
+  memcpy(f->idx, indices, nd * sizeof(size_t));
   f->cn = cn;
   f->ct = (double*)malloc((cn + 1) * 8);
   f->cf = (double*)malloc((cn + 1) * 8);
@@ -153,6 +162,7 @@ static int fam_build(const double* ct, const double* cf, const double* cs,
     free(dt);
     free(dd);
     free(dv);
+    fam_free(f);
     return st;
   }
   f->n = m;
@@ -163,9 +173,9 @@ static int fam_build(const double* ct, const double* cf, const double* cs,
   if (st) {
     free(dd);
     free(dv);
+    fam_free(f);
     return st;
   }
-  f->nd = nd;
   f->rho = (double*)malloc((nd + 1) * 8);
   if (correlated) {
     double* resid = (double*)malloc((m + 1) * 8);
@@ -195,8 +205,8 @@ static int fam_build(const double* ct, const double* cf, const double* cs,
 /* curve.rs:587-603: the family rebuilt from the curve without the masked
  * observations */
 static int fam_masked(const fam_t* src, double period, double epoch,
-                      double duration, double margin, const double* durations,
-                      size_t nd, int correlated, fam_t* out, size_t* kept) {
+                      double duration, double margin, int correlated, fam_t* out,
+                      size_t* kept) {
   uint8_t* m = (uint8_t*)malloc(src->cn + 1);
   orc_transit_mask(src->ct, src->cn, period, epoch, duration, margin, m);
   double* t = (double*)malloc((src->cn + 1) * 8);
@@ -213,7 +223,7 @@ static int fam_masked(const fam_t* src, double period, double epoch,
   free(m);
   *kept = k;
   /* family_state fails below six observations (curve.rs:429-431) */
-  int st = fam_build(t, fl, s, k, durations, nd, correlated, out);
+  int st = fam_build(t, fl, s, k, src->dur, src->idx, src->nd, correlated, out);
   free(t);
   free(fl);
   free(s);
@@ -294,12 +304,12 @@ static entry_t heap_pop(heap_t* h) {
 typedef struct {
   const orc_row* table;
   size_t nrows;
-  const fam_t* fam; /* one family (prepared mode) */
-  const size_t* pos;
+  const size_t* family; /* family of each row */
+  fam_t* fams;
+  size_t nfam;
+  size_t** pos; /* per family: index of each observation in time_all */
   const double* time_all;
   size_t n_all;
-  const double* durations;
-  size_t nd;
   double threshold;
   uint8_t* excluded;
 } sel_ctx;
@@ -307,26 +317,32 @@ typedef struct {
 /* peaks.rs:148-240 */
 static void evaluate(const sel_ctx* S, const size_t* idx, size_t m, double* score,
                      double* required) {
-  const fam_t* f = S->fam;
-  uint8_t* ex_fam = (uint8_t*)malloc(f->n + 1);
-  for (size_t i = 0; i < f->n; i++) ex_fam[i] = S->excluded[S->pos[i]];
+  uint8_t** ex_fam = (uint8_t**)malloc(S->nfam * sizeof(uint8_t*));
+  for (size_t k = 0; k < S->nfam; k++) {
+    const fam_t* f = &S->fams[k];
+    ex_fam[k] = (uint8_t*)malloc(f->n + 1);
+    for (size_t i = 0; i < f->n; i++) ex_fam[k][i] = S->excluded[S->pos[k][i]];
+  }
 #pragma omp parallel for schedule(dynamic, 16)
   for (long long s = 0; s < (long long)m; s++) {
     const orc_row* r = &S->table[idx[s]];
+    const size_t fk = S->family[idx[s]];
+    const fam_t* f = &S->fams[fk];
     uint32_t c2[2];
     orc_count_row(S->time_all, S->excluded, S->n_all, r->period, r->epoch, r->duration, c2);
     double total = (double)(c2[0] > 1 ? c2[0] : 1);
     double unshared = (double)c2[1] / total;
     required[s] = fmax(S->threshold * sqrt(unshared), 0.5 * S->threshold);
     double sums[3];
-    orc_rescore_row(f->t, f->v, f->w, ex_fam, f->n, f->origin, r->period, r->epoch,
+    orc_rescore_row(f->t, f->v, f->w, ex_fam[fk], f->n, f->origin, r->period, r->epoch,
                     r->duration, sums);
     double q = sums[2] - sums[1] * sums[1] / f->C;
     score[s] = (q > 0.0 && sums[0] > 0.0)
-                   ? (sums[0] / nearest_rho(S->durations, f->rho, S->nd, r->duration)) /
+                   ? (sums[0] / nearest_rho(f->dur, f->rho, f->nd, r->duration)) /
                          sqrt(fmax(q, 1e-300))
                    : -INFINITY;
   }
+  for (size_t k = 0; k < S->nfam; k++) free(ex_fam[k]);
   free(ex_fam);
 }
 
@@ -336,14 +352,15 @@ typedef struct {
 } cyc_t;
 
 /* peaks.rs:243-299 */
-static double without_strongest_event(const sel_ctx* S, const orc_row* row) {
-  const fam_t* f = S->fam;
+static double without_strongest_event(const sel_ctx* S, const orc_row* row, size_t fk) {
+  const fam_t* f = &S->fams[fk];
+  const size_t* pos = S->pos[fk];
   cyc_t* c = (cyc_t*)malloc((f->n + 1) * sizeof(cyc_t));
   size_t* mem_i = (size_t*)malloc((f->n + 1) * sizeof(size_t));
   int64_t* mem_c = (int64_t*)malloc((f->n + 1) * sizeof(int64_t));
   size_t nc = 0, nm = 0;
   for (size_t i = 0; i < f->n; i++) {
-    if (S->excluded[S->pos[i]]) continue;
+    if (S->excluded[pos[i]]) continue;
     double ta = f->t[i] + f->origin;
     double phase = rem_euclid(ta - row->epoch + row->period / 2.0, row->period) - row->period / 2.0;
     if (fabs(phase) <= row->duration / 2.0) {
@@ -365,7 +382,7 @@ static double without_strongest_event(const sel_ctx* S, const orc_row* row) {
   }
   double value = -INFINITY;
   if (nc) {
-    double rho = nearest_rho(S->durations, f->rho, S->nd, row->duration);
+    double rho = nearest_rho(f->dur, f->rho, f->nd, row->duration);
     /* BTreeMap order: ascending cycle; the first strict maximum wins */
     int64_t strongest = 0;
     double best = -INFINITY;
@@ -447,7 +464,7 @@ static size_t popcount_u8(const uint8_t* a, size_t n) {
   return c;
 }
 
-/* peaks.rs:308-487 for one family.  masks: n_masks x n_all (grown);
+/* peaks.rs:308-487.  masks: n_masks x n_all (grown);
  * accepted peaks appended to out (capacity cap). */
 static void select_peaks(sel_ctx* S, size_t limit, size_t min_events,
                          uint8_t** masks, size_t* n_masks, size_t pass,
@@ -497,7 +514,7 @@ static void select_peaks(sel_ctx* S, size_t limit, size_t min_events,
     size_t events = orc_event_count(S->time_all, S->n_all, row->period, row->epoch,
                                     row->duration, 3);
     if (events < min_events) continue;
-    double leftover = without_strongest_event(S, row);
+    double leftover = without_strongest_event(S, row, S->family[i]);
     if (min_events > 1 && leftover < 0.5 * S->threshold) continue;
     uint8_t* wide = (uint8_t*)malloc(S->n_all + 1);
     orc_transit_mask(S->time_all, S->n_all, row->period, row->epoch, row->duration, margin, wide);
@@ -515,7 +532,7 @@ static void select_peaks(sel_ctx* S, size_t limit, size_t min_events,
       p->depth = row->depth;
       p->depth_ppm = 1e6 * row->depth;
       p->events = events;
-      p->family = 0;
+      p->family = S->family[i];
       p->pass = pass;
       p->row = i;
       p->planet_log_odds = p->log_bf_planet_noise = p->log_bf_binary_noise = NAN;
@@ -539,6 +556,213 @@ static void select_peaks(sel_ctx* S, size_t limit, size_t min_events,
 static int cmp_dbl(const void* a, const void* b) {
   double x = *(const double*)a, y = *(const double*)b;
   return x < y ? -1 : x > y;
+}
+
+/* pipeline.rs:70-221 over the families (curve k: fn[k] points at ft/ff/fs
+ * + foff[k], searching the bank durations fidx[fmoff[k] .. fmoff[k + 1]]) */
+static int search_families(size_t nfam, const double* ft, const double* ff, const double* fs,
+                           const size_t* foff, const size_t* fn, const size_t* fidx,
+                           const size_t* fmoff, const orc_pipeline_cfg* cfg, orc_peak* peaks,
+                           size_t cap, size_t* npeaks, orc_pass_stats* pstats,
+                           size_t* npasses) {
+  const double* dur = cfg->durations;
+  int st = 0;
+  fam_t* cur = (fam_t*)calloc(nfam + 1, sizeof(fam_t));
+  fam_t* original = (fam_t*)calloc(nfam + 1, sizeof(fam_t));
+  size_t built = 0;
+  for (; built < nfam && !st; built++) {
+    const size_t m = fmoff[built + 1] - fmoff[built];
+    double* d = (double*)malloc((m + 1) * 8);
+    for (size_t i = 0; i < m; i++) d[i] = dur[fidx[fmoff[built] + i]];
+    st = fam_build(ft + foff[built], ff + foff[built], fs + foff[built], fn[built], d,
+                   fidx + fmoff[built], m, cfg->correlated, &cur[built]);
+    if (!st)
+      st = fam_build(ft + foff[built], ff + foff[built], fs + foff[built], fn[built], d,
+                     fidx + fmoff[built], m, cfg->correlated, &original[built]);
+    free(d);
+  }
+  if (st) {
+    for (size_t k = 0; k < nfam; k++) {
+      fam_free(&cur[k]);
+      fam_free(&original[k]);
+    }
+    free(cur);
+    free(original);
+    return st;
+  }
+  /* merged_time (pipeline.rs:16-24): sorted, deduplicated */
+  size_t total = 0;
+  for (size_t k = 0; k < nfam; k++) total += cur[k].cn;
+  double* time_all = (double*)malloc((total + 1) * 8);
+  size_t n_all = 0;
+  for (size_t k = 0; k < nfam; k++) {
+    memcpy(time_all + n_all, cur[k].ct, cur[k].cn * 8);
+    n_all += cur[k].cn;
+  }
+  qsort(time_all, n_all, 8, cmp_dbl);
+  size_t u = 0;
+  for (size_t i = 0; i < n_all; i++)
+    if (u == 0 || time_all[i] != time_all[u - 1]) time_all[u++] = time_all[i];
+  n_all = u;
+  uint8_t* excluded = (uint8_t*)calloc(n_all + 1, 1);
+  uint8_t** masks = (uint8_t**)calloc(cfg->max_candidates + 1, sizeof(uint8_t*));
+  size_t n_masks = 0;
+  if (n_all < 6) st = 6;
+  for (size_t pass = 0; !st && pass < cfg->passes; pass++) {
+    size_t nrows = 0, grid_total = 0;
+    size_t cap_rows = 1 << 22;
+    orc_row* table = (orc_row*)malloc(cap_rows * sizeof(orc_row));
+    size_t* family = (size_t*)malloc(cap_rows * sizeof(size_t));
+    for (size_t k = 0; k < nfam && !st; k++) {
+      size_t grid_count = 0, nr = 0, nbands = 0;
+      orc_scan_stats bstats[64];
+      st = orc_signal_search(cur[k].t, cur[k].v, cur[k].w, cur[k].n, cur[k].C, cur[k].origin,
+                             cur[k].dur, cur[k].rho, cur[k].nd, cfg->ingress, cfg->ns, &cfg->bank,
+                             &cfg->search, table + nrows, cap_rows - nrows, &nr, bstats, &nbands,
+                             &grid_count);
+      for (size_t i = 0; i < nr; i++) family[nrows + i] = k;
+      nrows += nr;
+      grid_total += grid_count;
+    }
+    if (st) {
+      free(table);
+      free(family);
+      break;
+    }
+    orc_pass_stats* ps = &pstats[*npasses];
+    memset(ps, 0, sizeof(*ps));
+    ps->pass = pass + 1;
+    ps->rows = nrows;
+    ps->periods = grid_total;
+    (*npasses)++;
+    size_t** pos = (size_t**)malloc(nfam * sizeof(size_t*));
+    for (size_t k = 0; k < nfam; k++) {
+      pos[k] = (size_t*)malloc((cur[k].n + 1) * sizeof(size_t));
+      for (size_t i = 0; i < cur[k].n; i++) {
+        size_t p = lower_bound(time_all, n_all, cur[k].t[i] + cur[k].origin);
+        pos[k][i] = p < n_all - 1 ? p : n_all - 1;
+      }
+    }
+    sel_ctx S = {table, nrows, family, cur, nfam, pos, time_all, n_all, cfg->bank.threshold,
+                 excluded};
+    size_t before = *npeaks;
+    select_peaks(&S, cfg->max_candidates - before, cfg->min_events, masks, &n_masks, pass + 1,
+                 peaks, npeaks, cap);
+    for (size_t k = 0; k < nfam; k++) free(pos[k]);
+    free(pos);
+    free(table);
+    free(family);
+    ps->accepted = *npeaks - before;
+    if (*npeaks == before) break;
+    orc_peak lead = peaks[before];
+    if (pass + 1 < cfg->passes) {
+      int small = 0;
+      for (size_t k = 0; k < nfam && !st; k++) {
+        fam_t next;
+        size_t kept = 0;
+        st = fam_masked(&cur[k], lead.period_days, lead.epoch_bjd, lead.duration_hours / 24.0,
+                        0.5, cfg->correlated, &next, &kept);
+        if (st) break;
+        fam_free(&cur[k]);
+        cur[k] = next;
+        small |= kept < 6;
+      }
+      if (st || small) break;
+    }
+  }
+  /* evidence (pipeline.rs:169-221) */
+  if (!st && cfg->model_comparison && *npeaks) {
+    size_t np = *npeaks;
+    size_t* ord = (size_t*)malloc(np * sizeof(size_t));
+    for (size_t i = 0; i < np; i++) ord[i] = i;
+    for (size_t i = 1; i < np; i++) { /* stable sort by snr, descending */
+      size_t x = ord[i], j = i;
+      while (j > 0 && peaks[ord[j - 1]].snr < peaks[x].snr) {
+        ord[j] = ord[j - 1];
+        j--;
+      }
+      ord[j] = x;
+    }
+    for (size_t oi = 0; oi < np && oi < cfg->evidence_candidates; oi++) {
+      orc_peak* p = &peaks[ord[oi]];
+      const fam_t* base = &original[p->family];
+      size_t cn = base->cn;
+      double* ct = (double*)malloc((cn + 1) * 8);
+      double* cf = (double*)malloc((cn + 1) * 8);
+      double* cs = (double*)malloc((cn + 1) * 8);
+      memcpy(ct, base->ct, cn * 8);
+      memcpy(cf, base->cf, cn * 8);
+      memcpy(cs, base->cs, cn * 8);
+      uint8_t* m = (uint8_t*)malloc(cn + 1);
+      for (size_t q = 0; q < np; q++) {
+        const orc_peak* o = &peaks[q];
+        if (o->row == p->row && o->pass == p->pass && o->family == p->family) continue;
+        orc_transit_mask(ct, cn, o->period_days, o->epoch_bjd, o->duration_hours / 24.0, 0.5, m);
+        size_t kk = 0;
+        for (size_t i = 0; i < cn; i++)
+          if (!m[i]) {
+            ct[kk] = ct[i];
+            cf[kk] = cf[i];
+            cs[kk] = cs[i];
+            kk++;
+          }
+        cn = kk;
+      }
+      free(m);
+      if (cn >= 6) {
+        fam_t s;
+        st = fam_build(ct, cf, cs, cn, base->dur, base->idx, base->nd, cfg->correlated, &s);
+        if (st == 0) {
+          orc_evidence_row rows[2];
+          size_t nr = 0;
+          orc_compare_search(s.t, s.v, s.w, s.n, s.C, s.origin, s.energy, s.dur, s.rho, s.nd,
+                             p->period_days, &cfg->evidence, rows, &nr);
+          const orc_evidence_row* r = NULL;
+          for (size_t q = 0; q < nr; q++)
+            if (rows[q].selection_flags & 1) {
+              r = &rows[q];
+              break;
+            }
+          if (!r && nr) r = &rows[0];
+          if (r) {
+#define FIN(x) (isfinite(x) ? (x) : NAN)
+            p->planet_log_odds = FIN(r->planet_log_odds);
+            p->log_bf_planet_noise = FIN(r->log_bf_planet_noise);
+            p->log_bf_binary_noise = FIN(r->log_bf_binary_noise);
+            p->binary_secondary_ratio = FIN(r->binary_secondary_ratio);
+            p->binary_secondary_phase = FIN(r->binary_secondary_phase);
+            p->binary_odd_even_ratio = FIN(r->binary_odd_even_ratio);
+            p->binary_period_days = FIN(r->binary_period_days);
+            p->ingress_fraction = FIN(r->ingress_fraction);
+#undef FIN
+            p->evidence_epoch_bjd = r->epoch;
+            p->evidence_duration_hours = 24.0 * r->duration;
+            p->model_preferred = r->planet_log_odds >= cfg->planet_log_odds_min;
+            p->binary_diagnostics_complete =
+                isfinite(r->binary_secondary_ratio) && isfinite(r->binary_secondary_phase) &&
+                isfinite(r->binary_odd_even_ratio) && isfinite(r->binary_period_days);
+          }
+          fam_free(&s);
+        }
+      }
+      free(ct);
+      free(cf);
+      free(cs);
+      if (st) break;
+    }
+    free(ord);
+  }
+  for (size_t i = 0; i < n_masks; i++) free(masks[i]);
+  free(masks);
+  free(excluded);
+  free(time_all);
+  for (size_t k = 0; k < nfam; k++) {
+    fam_free(&cur[k]);
+    fam_free(&original[k]);
+  }
+  free(cur);
+  free(original);
+  return st;
 }
 
 int orc_search_prepared(const double* time, const double* flux,
@@ -569,165 +793,60 @@ int orc_search_prepared(const double* time, const double* flux,
     cs[i] = err[order[i]];
   }
   free(order);
-  const double* dur = cfg->durations;
-  const size_t nd = cfg->nd;
-  fam_t cur, original;
-  int st = fam_build(ct, cf, cs, k, dur, nd, cfg->correlated, &cur);
+  size_t* idx = (size_t*)malloc((cfg->nd + 1) * sizeof(size_t));
+  for (size_t i = 0; i < cfg->nd; i++) idx[i] = i;
+  const size_t foff[1] = {0}, fn[1] = {k}, fmoff[2] = {0, cfg->nd};
+  int st = search_families(1, ct, cf, cs, foff, fn, idx, fmoff, cfg, peaks, cap, npeaks, pstats,
+                           npasses);
+  free(idx);
   free(ct);
   free(cf);
   free(cs);
-  if (st) return st;
-  st = fam_build(cur.ct, cur.cf, cur.cs, cur.cn, dur, nd, cfg->correlated, &original);
-  if (st) {
-    fam_free(&cur);
-    return st;
+  return st;
+}
+
+/* pipeline.rs:27-37: prepare_families (curve.rs:387-425) then the search */
+int orc_search_transits(const double* time, const double* flux, const double* err,
+                        const int64_t* group, size_t n, const orc_prep_cfg* prep,
+                        const orc_pipeline_cfg* cfg, orc_peak* peaks, size_t cap,
+                        size_t* npeaks, orc_pass_stats* pstats, size_t* npasses) {
+  *npeaks = 0;
+  *npasses = 0;
+  const size_t nd = cfg->nd;
+  size_t nfam = 0;
+  size_t* fn = (size_t*)malloc((nd + 1) * sizeof(size_t));
+  double* ft = (double*)malloc((nd * n + 1) * 8);
+  double* ff = (double*)malloc((nd * n + 1) * 8);
+  double* fs = (double*)malloc((nd * n + 1) * 8);
+  double* fw = (double*)malloc((nd + 1) * 8);
+  size_t* fc = (size_t*)malloc((nd + 1) * sizeof(size_t));
+  uint8_t* mem = (uint8_t*)malloc(nd * nd + 1);
+  int st = orc_prepare_families(time, flux, err, group, n, cfg->durations, nd, prep, &nfam, fn,
+                                ft, ff, fs, fw, fc, mem);
+  if (!st) {
+    size_t* foff = (size_t*)malloc((nfam + 1) * sizeof(size_t));
+    size_t* fidx = (size_t*)malloc((nfam * nd + 1) * sizeof(size_t));
+    size_t* fmoff = (size_t*)malloc((nfam + 2) * sizeof(size_t));
+    size_t m = 0;
+    for (size_t k = 0; k < nfam; k++) {
+      foff[k] = k * n;
+      fmoff[k] = m;
+      for (size_t i = 0; i < nd; i++)
+        if (mem[k * nd + i]) fidx[m++] = i;
+    }
+    fmoff[nfam] = m;
+    st = search_families(nfam, ft, ff, fs, foff, fn, fidx, fmoff, cfg, peaks, cap, npeaks, pstats,
+                         npasses);
+    free(foff);
+    free(fidx);
+    free(fmoff);
   }
-  /* merged_time (pipeline.rs:16-24): one family, sorted, deduplicated */
-  double* time_all = (double*)malloc((cur.cn + 1) * 8);
-  memcpy(time_all, cur.ct, cur.cn * 8);
-  qsort(time_all, cur.cn, 8, cmp_dbl);
-  size_t n_all = 0;
-  for (size_t i = 0; i < cur.cn; i++)
-    if (n_all == 0 || time_all[i] != time_all[n_all - 1]) time_all[n_all++] = time_all[i];
-  if (n_all < 6) {
-    free(time_all);
-    fam_free(&cur);
-    fam_free(&original);
-    return 6;
-  }
-  uint8_t* excluded = (uint8_t*)calloc(n_all, 1);
-  uint8_t** masks = (uint8_t**)calloc(cfg->max_candidates + 1, sizeof(uint8_t*));
-  size_t n_masks = 0;
-  for (size_t pass = 0; pass < cfg->passes; pass++) {
-    size_t grid_count = 0, nrows = 0, nbands = 0;
-    size_t cap_rows = 1 << 22;
-    orc_row* table = (orc_row*)malloc(cap_rows * sizeof(orc_row));
-    orc_scan_stats bstats[64];
-    st = orc_signal_search(cur.t, cur.v, cur.w, cur.n, cur.C, cur.origin, dur, cur.rho, nd,
-                           cfg->ingress, cfg->ns, &cfg->bank, &cfg->search, table, cap_rows,
-                           &nrows, bstats, &nbands, &grid_count);
-    if (st) {
-      free(table);
-      break;
-    }
-    orc_pass_stats* ps = &pstats[*npasses];
-    memset(ps, 0, sizeof(*ps));
-    ps->pass = pass + 1;
-    ps->rows = nrows;
-    ps->periods = grid_count;
-    (*npasses)++;
-    size_t* pos = (size_t*)malloc((cur.n + 1) * sizeof(size_t));
-    for (size_t i = 0; i < cur.n; i++) {
-      size_t p = lower_bound(time_all, n_all, cur.t[i] + cur.origin);
-      pos[i] = p < n_all - 1 ? p : n_all - 1;
-    }
-    sel_ctx S = {table, nrows, &cur, pos, time_all, n_all, dur, nd, cfg->bank.threshold, excluded};
-    size_t before = *npeaks;
-    select_peaks(&S, cfg->max_candidates - before, cfg->min_events, masks, &n_masks, pass + 1,
-                 peaks, npeaks, cap);
-    free(pos);
-    free(table);
-    ps->accepted = *npeaks - before;
-    if (*npeaks == before) break;
-    orc_peak lead = peaks[before];
-    if (pass + 1 < cfg->passes) {
-      fam_t next;
-      size_t kept = 0;
-      st = fam_masked(&cur, lead.period_days, lead.epoch_bjd, lead.duration_hours / 24.0, 0.5,
-                      dur, nd, cfg->correlated, &next, &kept);
-      if (st) break;
-      fam_free(&cur);
-      cur = next;
-      if (kept < 6) break;
-    }
-  }
-  /* evidence (pipeline.rs:169-221) */
-  if (!st && cfg->model_comparison && *npeaks) {
-    size_t np = *npeaks;
-    size_t* ord = (size_t*)malloc(np * sizeof(size_t));
-    for (size_t i = 0; i < np; i++) ord[i] = i;
-    for (size_t i = 1; i < np; i++) { /* stable sort by snr, descending */
-      size_t x = ord[i], j = i;
-      while (j > 0 && peaks[ord[j - 1]].snr < peaks[x].snr) {
-        ord[j] = ord[j - 1];
-        j--;
-      }
-      ord[j] = x;
-    }
-    for (size_t oi = 0; oi < np && oi < cfg->evidence_candidates; oi++) {
-      orc_peak* p = &peaks[ord[oi]];
-      fam_t c;
-      memset(&c, 0, sizeof(c));
-      c.cn = original.cn;
-      c.ct = (double*)malloc((c.cn + 1) * 8);
-      c.cf = (double*)malloc((c.cn + 1) * 8);
-      c.cs = (double*)malloc((c.cn + 1) * 8);
-      memcpy(c.ct, original.ct, c.cn * 8);
-      memcpy(c.cf, original.cf, c.cn * 8);
-      memcpy(c.cs, original.cs, c.cn * 8);
-      for (size_t q = 0; q < np; q++) {
-        const orc_peak* o = &peaks[q];
-        if (o->row == p->row && o->pass == p->pass && o->family == p->family) continue;
-        uint8_t* m = (uint8_t*)malloc(c.cn + 1);
-        orc_transit_mask(c.ct, c.cn, o->period_days, o->epoch_bjd, o->duration_hours / 24.0,
-                         0.5, m);
-        size_t kk = 0;
-        for (size_t i = 0; i < c.cn; i++)
-          if (!m[i]) {
-            c.ct[kk] = c.ct[i];
-            c.cf[kk] = c.cf[i];
-            c.cs[kk] = c.cs[i];
-            kk++;
-          }
-        c.cn = kk;
-        free(m);
-      }
-      if (c.cn >= 6) {
-        fam_t s;
-        st = fam_build(c.ct, c.cf, c.cs, c.cn, dur, nd, cfg->correlated, &s);
-        if (st == 0) {
-          orc_evidence_row rows[2];
-          size_t nr = 0;
-          orc_compare_search(s.t, s.v, s.w, s.n, s.C, s.origin, s.energy, dur, s.rho, nd,
-                             p->period_days, &cfg->evidence, rows, &nr);
-          const orc_evidence_row* r = NULL;
-          for (size_t q = 0; q < nr; q++)
-            if (rows[q].selection_flags & 1) {
-              r = &rows[q];
-              break;
-            }
-          if (!r && nr) r = &rows[0];
-          if (r) {
-#define FIN(x) (isfinite(x) ? (x) : NAN)
-            p->planet_log_odds = FIN(r->planet_log_odds);
-            p->log_bf_planet_noise = FIN(r->log_bf_planet_noise);
-            p->log_bf_binary_noise = FIN(r->log_bf_binary_noise);
-            p->binary_secondary_ratio = FIN(r->binary_secondary_ratio);
-            p->binary_secondary_phase = FIN(r->binary_secondary_phase);
-            p->binary_odd_even_ratio = FIN(r->binary_odd_even_ratio);
-            p->binary_period_days = FIN(r->binary_period_days);
-            p->ingress_fraction = FIN(r->ingress_fraction);
-#undef FIN
-            p->evidence_epoch_bjd = r->epoch;
-            p->evidence_duration_hours = 24.0 * r->duration;
-            p->model_preferred = r->planet_log_odds >= cfg->planet_log_odds_min;
-            p->binary_diagnostics_complete =
-                isfinite(r->binary_secondary_ratio) && isfinite(r->binary_secondary_phase) &&
-                isfinite(r->binary_odd_even_ratio) && isfinite(r->binary_period_days);
-          }
-          fam_free(&s);
-        }
-      }
-      fam_free(&c);
-      if (st) break;
-    }
-    free(ord);
-  }
-  for (size_t i = 0; i < n_masks; i++) free(masks[i]);
-  free(masks);
-  free(excluded);
-  free(time_all);
-  fam_free(&cur);
-  fam_free(&original);
+  free(fn);
+  free(ft);
+  free(ff);
+  free(fs);
+  free(fw);
+  free(fc);
+  free(mem);
   return st;
 }
