@@ -56,7 +56,7 @@ EXPORTS = (
     "fbls_cnn_set_params", "fbls_cnn_forward", "fbls_cnn_loss_grad",
     "fbls_cnn_train_step", "fbls_coarse_features", "fbls_ctx_set_graphs",
     "fbls_autotune", "fbls_biweight_flatten", "fbls_prepare_families",
-    "fbls_search_transits",
+    "fbls_search_transits", "fbls_single_event_search",
 )
 
 
@@ -186,7 +186,47 @@ PipelineCfgC._fields_ = [
     ("bank", Bank), ("search", SearchCfgC), ("max_candidates", _sz),
     ("passes", _sz), ("min_events", _sz), ("evidence_candidates", _sz),
     ("model_comparison", ctypes.c_int), ("planet_log_odds_min", _d),
-    ("evidence", EvidenceCfg)]
+    ("evidence", EvidenceCfg), ("single_events", ctypes.c_int)]
+
+
+class EventStats(ctypes.Structure):
+    _fields_ = [("templates_scanned", _u64), ("threshold_survivors", _u64),
+                ("overflow", _u32), ("phase_bins", _u32)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["overflow"] = bool(d["overflow"])
+        return d
+
+
+class SingleEvent(ctypes.Structure):
+    _fields_ = [(k, _d) for k in ("epoch_bjd", "duration_hours", "snr", "depth",
+                                  "depth_ppm")] + [("points", _u64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FamilyPass(ctypes.Structure):
+    _fields_ = ([(k, _u64) for k in ("pass_", "family", "observations", "clipped",
+                                     "segments", "periods", "nd")]
+                + [("window_days", _d), ("error_inflation", _d),
+                   ("durations_hours", _d * 64), ("rho", _d * 64),
+                   ("band_first", _u64), ("nbands", _u64)])
+
+
+class SearchReport(ctypes.Structure):
+    _fields_ = [("events", _P(SingleEvent)), ("events_cap", _sz), ("nevents", _sz),
+                ("event_stats", _P(EventStats)), ("event_stats_cap", _sz),
+                ("nevent_stats", _sz),
+                ("period_table", _P(Row)), ("period_table_cap", _sz),
+                ("period_table_rows", _sz), ("period_table_family", _P(_u64)),
+                ("families", _P(FamilyPass)), ("families_cap", _sz), ("nfamilies", _sz),
+                ("bands", _P(ScanStats)), ("bands_cap", _sz), ("nbands", _sz),
+                ("observations", _u64), ("periods_tested", _u64), ("nfam", _u64),
+                ("baseline_days", _d), ("seconds_preparation", _d), ("seconds_search", _d),
+                ("seconds_scan", _d), ("seconds_select", _d), ("seconds_evidence", _d),
+                ("seconds_single_events", _d)]
 
 
 class BandC(ctypes.Structure):
@@ -255,10 +295,14 @@ def lib():
                                           ctypes.c_float, _fp]
         L.fbls_search_prepared.argtypes = [
             _vp, _P(_d), _P(_d), _P(_d), _sz, _P(PipelineCfgC), _P(Peak), _sz,
-            _P(_sz), _P(PassStats), _sz, _P(_sz)]
+            _P(_sz), _P(PassStats), _sz, _P(_sz), _P(SearchReport)]
         L.fbls_search_transits.argtypes = [
             _vp, _P(_d), _P(_d), _P(_d), _P(ctypes.c_int64), _sz, _P(PrepCfg),
-            _P(PipelineCfgC), _P(Peak), _sz, _P(_sz), _P(PassStats), _sz, _P(_sz)]
+            _P(PipelineCfgC), _P(Peak), _sz, _P(_sz), _P(PassStats), _sz, _P(_sz),
+            _P(SearchReport)]
+        L.fbls_single_event_search.argtypes = [
+            _vp, _vp, _P(_d), _P(_d), _sz, _P(_d), _sz, _u32, _d, _u32, _sz, _P(Row),
+            _P(_sz), _P(EventStats)]
         L.fbls_family_upload.argtypes = [_vp, _P(_d), _P(_d), _P(_d), _sz, _d,
                                          _d, _P(_vp)]
         L.fbls_family_fields.argtypes = [_vp, _P(_d), _P(_d),
@@ -344,6 +388,7 @@ class SearchConfig:
     model_comparison: bool = True
     evidence_candidates: int = 32
     planet_log_odds_min: float = 0.0
+    single_events: bool = True
     # model comparison (config.rs:33-42,76-85)
     evidence_rtol: float = 1e-12
     planet_prior: float = 0.01
@@ -777,9 +822,80 @@ def pipeline_cfg(cfg: SearchConfig):
         max_candidates=cfg.max_candidates, passes=cfg.passes,
         min_events=cfg.min_events, evidence_candidates=cfg.evidence_candidates,
         model_comparison=int(bool(cfg.model_comparison)),
-        planet_log_odds_min=cfg.planet_log_odds_min, evidence=ev)
+        planet_log_odds_min=cfg.planet_log_odds_min, evidence=ev,
+        single_events=int(bool(cfg.single_events)))
     c._keep = (d, bank, ev)
     return c
+
+
+def _new_report(cfg: SearchConfig, table_cap=0):
+    """A fbls_search_report with room for every family (at most one per
+    duration), every band, the selected events and table_cap rows of the
+    pass-1 period table."""
+    nd = len(cfg.durations_days())
+    fam_cap = nd * max(cfg.passes, 1)
+    r = SearchReport()
+    keep = dict(events=(SingleEvent * max(cfg.max_candidates, 1))(),
+                stats=(EventStats * nd)(), fams=(FamilyPass * fam_cap)(),
+                bands=(ScanStats * (fam_cap * 64))(),
+                table=(Row * max(table_cap, 1))(),
+                table_family=np.zeros(max(table_cap, 1), dtype=np.uint64))
+    r.events, r.events_cap = keep["events"], cfg.max_candidates
+    r.event_stats, r.event_stats_cap = keep["stats"], nd
+    r.families, r.families_cap = keep["fams"], fam_cap
+    r.bands, r.bands_cap = keep["bands"], fam_cap * 64
+    r.period_table, r.period_table_cap = keep["table"], table_cap
+    r.period_table_family = keep["table_family"].ctypes.data_as(_P(_u64))
+    r._keep = keep
+    return r
+
+
+def _report_dict(r: SearchReport):
+    k = r._keep
+    fams = []
+    for i in range(min(r.nfamilies, r.families_cap)):
+        f = r.families[i]
+        nd = min(int(f.nd), 64)
+        fams.append(dict(
+            pass_=int(f.pass_), family=int(f.family), window_days=f.window_days,
+            observations=int(f.observations), clipped=int(f.clipped),
+            error_inflation=f.error_inflation, segments=int(f.segments),
+            durations_hours=list(f.durations_hours[:nd]), periods=int(f.periods),
+            rho=list(f.rho[:nd]),
+            bands=[r.bands[b].as_dict() for b in range(int(f.band_first),
+                                                        int(f.band_first + f.nbands))
+                   if b < r.bands_cap]))
+    nt = min(r.period_table_rows, r.period_table_cap)
+    table = [dict(period=k["table"][i].period, epoch=k["table"][i].epoch,
+                  duration=k["table"][i].duration, score=k["table"][i].score,
+                  depth=k["table"][i].depth) for i in range(nt)]
+    return dict(
+        events=[r.events[i].as_dict() for i in range(min(r.nevents, r.events_cap))],
+        event_stats=[r.event_stats[i].as_dict()
+                     for i in range(min(r.nevent_stats, r.event_stats_cap))],
+        families=fams, period_table=table,
+        period_table_family=k["table_family"][:nt].astype(int).tolist(),
+        period_table_rows=int(r.period_table_rows), observations=int(r.observations),
+        periods_tested=int(r.periods_tested), nfam=int(r.nfam),
+        baseline_days=r.baseline_days, seconds_preparation=r.seconds_preparation,
+        seconds_search=r.seconds_search, seconds_scan=r.seconds_scan,
+        seconds_select=r.seconds_select, seconds_evidence=r.seconds_evidence,
+        seconds_single_events=r.seconds_single_events)
+
+
+def single_event_search(gpu, dev, durations, rho, cfg: SearchConfig, capacity=131072):
+    """fbls_single_event_search of one uploaded family: (rows as dicts, stats)."""
+    durations, rho = _f64(durations), _f64(rho)
+    ing = _f64(cfg.ingress_fractions)
+    rows = (Row * max(capacity, 1))()
+    nr = _sz()
+    st = EventStats()
+    _check(lib().fbls_single_event_search(
+        gpu._h, dev._h, _dp(durations), _dp(rho), len(durations), _dp(ing), len(ing),
+        cfg.shape_layers, cfg.threshold, cfg.phase_oversample, capacity, rows,
+        ctypes.byref(nr), ctypes.byref(st)))
+    keys = ("period", "epoch", "duration", "score", "depth")
+    return [{k: getattr(rows[i], k) for k in keys} for i in range(nr.value)], st.as_dict()
 
 
 def autotune(gpu, dev, durations, rho, cfg: SearchConfig, repeats=3):
@@ -798,21 +914,26 @@ def autotune(gpu, dev, durations, rho, cfg: SearchConfig, repeats=3):
     return as_d(best), [as_d(table[i]) for i in range(min(n.value, 32))]
 
 
-def search_prepared(gpu, time, flux, flux_err, cfg: SearchConfig):
-    """search_prepared + search_families (pipeline.rs:41-221, without single
-    events) on the GPU: returns (peaks in acceptance order as dicts, pass
-    statistics)."""
+def search_prepared(gpu, time, flux, flux_err, cfg: SearchConfig, report=False,
+                    table_cap=0):
+    """search_prepared + search_families (pipeline.rs:41-238) on the GPU:
+    returns (peaks in acceptance order as dicts, pass statistics), plus the
+    report dict (single events, family and band statistics, the first
+    table_cap rows of the pass-1 period table, timings) when report."""
     time, flux, flux_err = _f64(time), _f64(flux), _f64(flux_err)
     c = pipeline_cfg(cfg)
     cap = max(cfg.max_candidates, 1)
     peaks = (Peak * cap)()
     passes = (PassStats * max(cfg.passes, 1))()
     npk, npa = _sz(), _sz()
+    rep = _new_report(cfg, table_cap)
     _check(lib().fbls_search_prepared(
         gpu._h, _dp(time), _dp(flux), _dp(flux_err), len(time), ctypes.byref(c),
-        peaks, cap, ctypes.byref(npk), passes, len(passes), ctypes.byref(npa)))
-    return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
-            [passes[i].as_dict() for i in range(min(npa.value, len(passes)))])
+        peaks, cap, ctypes.byref(npk), passes, len(passes), ctypes.byref(npa),
+        ctypes.byref(rep)))
+    out = ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
+           [passes[i].as_dict() for i in range(min(npa.value, len(passes)))])
+    return out + (_report_dict(rep),) if report else out
 
 
 def by_signal(peaks):
@@ -954,10 +1075,11 @@ def prepare_families(gpu, time, flux, err, group, cfg: SearchConfig):
     return out
 
 
-def search_transits(gpu, time, flux, err, group, cfg: SearchConfig):
-    """search_transits (pipeline.rs:27-37, 70-221, without single events) on
-    raw flux: the families of prepare_families searched together.  Returns
-    (peaks in acceptance order as dicts, pass statistics)."""
+def search_transits(gpu, time, flux, err, group, cfg: SearchConfig, report=False,
+                    table_cap=0):
+    """search_transits (pipeline.rs:27-37, 70-238) on raw flux: the families
+    of prepare_families searched together.  Returns (peaks in acceptance
+    order as dicts, pass statistics), plus the report dict when report."""
     time, flux = _f64(time), _f64(flux)
     e = _f64(err) if err is not None else None
     g = np.ascontiguousarray(group, dtype=np.int64) if group is not None else None
@@ -967,10 +1089,12 @@ def search_transits(gpu, time, flux, err, group, cfg: SearchConfig):
     peaks = (Peak * cap)()
     passes = (PassStats * max(cfg.passes, 1))()
     npk, npa = _sz(), _sz()
+    rep = _new_report(cfg, table_cap)
     _check(lib().fbls_search_transits(
         gpu._h, _dp(time), _dp(flux), _dp(e) if e is not None else None,
         g.ctypes.data_as(_P(ctypes.c_int64)) if g is not None else None, len(time),
         ctypes.byref(pc), ctypes.byref(c), peaks, cap, ctypes.byref(npk), passes, len(passes),
-        ctypes.byref(npa)))
-    return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
-            [passes[i].as_dict() for i in range(min(npa.value, len(passes)))])
+        ctypes.byref(npa), ctypes.byref(rep)))
+    out = ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
+           [passes[i].as_dict() for i in range(min(npa.value, len(passes)))])
+    return out + (_report_dict(rep),) if report else out

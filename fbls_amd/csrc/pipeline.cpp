@@ -65,7 +65,8 @@ struct Fam {
   std::vector<double> dur;           // the family's durations (days)
   std::vector<size_t> indices;       // their positions in the bank
   std::vector<double> t, v, w, rho;  // relative time, fixed-point fields
-  double C = 0.0, origin = 0.0, energy = 0.0;
+  double C = 0.0, origin = 0.0, energy = 0.0, inflation = 0.0;
+  size_t segments = 0;
   fbls_family* dev = nullptr;
   Fam() = default;
   Fam(const Fam&) = delete;
@@ -104,6 +105,8 @@ int build_fam(fbls_ctx* c, Curve curve, const std::vector<double>& durs,
   out->C = C;
   out->origin = origin;
   out->energy = energy;
+  out->inflation = infl;
+  out->segments = segs;
   fbls_family_free(out->dev);
   out->dev = dev;
   return FBLS_OK;
@@ -407,6 +410,59 @@ int select_peaks(Selector& S, size_t limit, size_t min_events,
   return FBLS_OK;
 }
 
+// peaks.rs:490-550
+std::vector<fbls_single_event> select_events(const std::vector<fbls_row>& table,
+                                             const std::vector<double>& time, double threshold,
+                                             size_t limit,
+                                             const std::vector<std::vector<uint8_t>>& taken) {
+  std::vector<fbls_single_event> output;
+  std::vector<std::vector<uint8_t>> masks = taken;
+  if (table.empty()) return output;
+  std::vector<double> remaining(table.size());
+  for (size_t i = 0; i < table.size(); i++) {
+    const double sc = table[i].score;
+    remaining[i] = (std::isfinite(sc) && sc >= threshold) ? sc : -INFINITY;
+  }
+  std::vector<uint8_t> inside(time.size());
+  size_t iterations = 0;
+  while (output.size() < limit && iterations < 5000) {
+    iterations++;
+    size_t i = 0;  // Iterator::max_by: the last of equal maxima
+    for (size_t j = 1; j < table.size(); j++)
+      if (remaining[j] >= remaining[i]) i = j;
+    if (!std::isfinite(remaining[i])) break;
+    const fbls_row row = table[i];
+    size_t points = 0;
+    for (size_t j = 0; j < time.size(); j++) {
+      inside[j] = std::fabs(time[j] - row.epoch) <= row.duration / 2.0;
+      points += inside[j];
+    }
+    for (size_t j = 0; j < table.size(); j++)
+      if (std::fabs(table[j].epoch - row.epoch) <= (table[j].duration + row.duration) / 2.0)
+        remaining[j] = -INFINITY;
+    bool clash = points < 3;
+    for (size_t m = 0; m < masks.size() && !clash; m++) {
+      size_t both = 0;
+      for (size_t j = 0; j < time.size(); j++) both += inside[j] && masks[m][j];
+      clash = (points ? (double)both / (double)points : 0.0) >= 0.5;
+    }
+    if (clash) continue;
+    std::vector<uint8_t> wide(time.size());
+    for (size_t j = 0; j < time.size(); j++)
+      wide[j] = std::fabs(time[j] - row.epoch) <= 0.75 * row.duration;
+    masks.push_back(std::move(wide));
+    fbls_single_event e;
+    e.epoch_bjd = row.epoch;
+    e.duration_hours = 24.0 * row.duration;
+    e.snr = row.score;
+    e.depth = row.depth;
+    e.depth_ppm = 1e6 * row.depth;
+    e.points = points;
+    output.push_back(e);
+  }
+  return output;
+}
+
 int check_pipeline(const fbls_pipeline_cfg* cfg) {
   if (!cfg || !cfg->durations || cfg->nd == 0) return fail(FBLS_E_ARG, "no durations");
   if (cfg->passes < 1) return fail(FBLS_E_ARG, "passes must be >= 1");
@@ -414,24 +470,39 @@ int check_pipeline(const fbls_pipeline_cfg* cfg) {
   return FBLS_OK;
 }
 
-// pipeline.rs:70-221 over the families
-int search_families(fbls_ctx* c, std::vector<std::pair<Curve, std::vector<size_t>>> input,
-                    const fbls_pipeline_cfg* cfg, fbls_peak* peaks, size_t peaks_cap,
-                    size_t* npeaks, fbls_pass_stats* passes, size_t passes_cap,
-                    size_t* npasses) {
+// One family handed to the search: its curve, bank durations, detrending
+// window (NaN for prepared input) and clipped count.
+struct FamInput {
+  Curve curve;
+  std::vector<size_t> indices;
+  double window = std::nan("");
+  size_t clipped = 0;
+};
+
+template <class T>
+void put(T* arr, size_t cap, size_t* count, const T& x) {
+  if (arr && *count < cap) arr[*count] = x;
+  (*count)++;
+}
+
+// pipeline.rs:70-289 over the families
+int search_families(fbls_ctx* c, std::vector<FamInput> input, const fbls_pipeline_cfg* cfg,
+                    double start, fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
+                    fbls_pass_stats* passes, size_t passes_cap, size_t* npasses,
+                    fbls_search_report* report) {
   const size_t nfam = input.size();
   std::vector<std::unique_ptr<Fam>> cur(nfam);
   std::vector<Curve> original(nfam);
   std::vector<double> time_all;
   for (size_t k = 0; k < nfam; k++) {
     std::vector<double> durs;
-    for (size_t i : input[k].second) {
+    for (size_t i : input[k].indices) {
       if (i >= cfg->nd) return fail(FBLS_E_ARG, "duration index %zu out of range", i);
       durs.push_back(cfg->durations[i]);
     }
     if (durs.empty()) return fail(FBLS_E_ARG, "family %zu has no durations", k);
     cur[k] = std::make_unique<Fam>();
-    if (int st = build_fam(c, std::move(input[k].first), durs, input[k].second, cfg->correlated,
+    if (int st = build_fam(c, std::move(input[k].curve), durs, input[k].indices, cfg->correlated,
                            cur[k].get()))
       return st;
     original[k] = cur[k]->curve;
@@ -441,6 +512,17 @@ int search_families(fbls_ctx* c, std::vector<std::pair<Curve, std::vector<size_t
   std::sort(time_all.begin(), time_all.end());
   time_all.erase(std::unique(time_all.begin(), time_all.end()), time_all.end());
   if (time_all.size() < 6) return fail(FBLS_E_DATA, "at least six usable observations are required");
+  fbls_search_report dummy;
+  std::memset(&dummy, 0, sizeof(dummy));
+  fbls_search_report& R = report ? *report : dummy;
+  R.nevents = R.nevent_stats = R.period_table_rows = R.nfamilies = R.nbands = 0;
+  R.observations = time_all.size();
+  R.baseline_days = time_all.back() - time_all.front();
+  R.nfam = nfam;
+  R.periods_tested = 0;
+  R.seconds_preparation = now_s() - start;
+  R.seconds_scan = R.seconds_select = R.seconds_evidence = R.seconds_single_events = 0.0;
+  const double search_start = now_s();
   std::vector<uint8_t> excluded(time_all.size(), 0);
   std::vector<std::vector<uint8_t>> masks;
   std::vector<fbls_peak> found;
@@ -482,8 +564,34 @@ int search_families(fbls_ctx* c, std::vector<std::pair<Curve, std::vector<size_t
       table.resize(base + nrows);
       family.resize(base + nrows, k);
       grid_total += grid_count;
+      if (pass == 0) R.periods_tested += grid_count;
+      fbls_family_pass fp;
+      std::memset(&fp, 0, sizeof(fp));
+      fp.pass = pass + 1;
+      fp.family = k;
+      fp.window_days = input[k].window;
+      fp.observations = f.curve.t.size();
+      fp.clipped = input[k].clipped;
+      fp.error_inflation = f.inflation;
+      fp.segments = f.segments;
+      fp.periods = grid_count;
+      fp.nd = nd;
+      for (size_t i = 0; i < nd && i < 64; i++) {
+        fp.durations_hours[i] = f.dur[i] * 24.0;
+        fp.rho[i] = f.rho[i];
+      }
+      fp.band_first = R.nbands;
+      fp.nbands = std::min(nbands, bstats.size());
+      for (size_t b = 0; b < fp.nbands; b++) put(R.bands, R.bands_cap, &R.nbands, bstats[b]);
+      put(R.families, R.families_cap, &R.nfamilies, fp);
     }
     const double t1 = now_s();
+    R.seconds_scan += t1 - t0;
+    if (pass == 0)
+      for (size_t i = 0; i < table.size(); i++) {
+        put(R.period_table, R.period_table_cap, &R.period_table_rows, table[i]);
+        if (R.period_table_family && i < R.period_table_cap) R.period_table_family[i] = family[i];
+      }
     Selector S;
     S.c = c;
     S.table = table.data();
@@ -509,6 +617,7 @@ int search_families(fbls_ctx* c, std::vector<std::pair<Curve, std::vector<size_t
     const size_t before = found.size();
     int st = select_peaks(S, cfg->max_candidates - before, cfg->min_events, masks, pass + 1, found);
     if (st) return st;
+    R.seconds_select += now_s() - t1;
     if (*npasses < passes_cap && passes) {
       fbls_pass_stats& ps = passes[*npasses];
       ps.pass = pass + 1;
@@ -537,6 +646,7 @@ int search_families(fbls_ctx* c, std::vector<std::pair<Curve, std::vector<size_t
     }
   }
   // model comparison of the strongest peaks (pipeline.rs:169-221)
+  const double t_evidence = now_s();
   if (cfg->model_comparison && !found.empty()) {
     std::vector<size_t> ord(found.size());
     for (size_t i = 0; i < ord.size(); i++) ord[i] = i;
@@ -580,7 +690,44 @@ int search_families(fbls_ctx* c, std::vector<std::pair<Curve, std::vector<size_t
           std::isfinite(r->binary_secondary_ratio) && std::isfinite(r->binary_secondary_phase) &&
           std::isfinite(r->binary_odd_even_ratio) && std::isfinite(r->binary_period_days);
     }
+    R.seconds_evidence = now_s() - t_evidence;
   }
+  // single events (pipeline.rs:222-238) over the families as the passes left them
+  const double t_events = now_s();
+  if (cfg->single_events) {
+    const size_t capacity = 131072;
+    std::vector<fbls_row> all;
+    std::vector<fbls_row> rows(capacity);
+    for (size_t k = 0; k < nfam; k++) {
+      const Fam& f = *cur[k];
+      size_t nr = 0;
+      fbls_event_stats es;
+      if (int st = fbls_single_event_search(
+              c, f.dev, f.dur.data(), f.rho.data(), f.dur.size(), cfg->bank.ingress,
+              cfg->bank.ns, cfg->bank.layers, cfg->bank.threshold, cfg->bank.phase_oversample,
+              capacity, rows.data(), &nr, &es))
+        return st;
+      put(R.event_stats, R.event_stats_cap, &R.nevent_stats, es);
+      std::vector<fbls_row> table(rows.begin(), rows.begin() + nr);
+      for (const fbls_single_event& e :
+           select_events(table, time_all, cfg->bank.threshold, cfg->max_candidates, masks)) {
+        fbls_row r;
+        r.period = INFINITY;
+        r.epoch = e.epoch_bjd;
+        r.duration = e.duration_hours / 24.0;
+        r.score = e.snr;
+        r.depth = e.depth;
+        all.push_back(r);
+      }
+    }
+    std::stable_sort(all.begin(), all.end(),
+                     [](const fbls_row& a, const fbls_row& b) { return a.score > b.score; });
+    for (const fbls_single_event& e :
+         select_events(all, time_all, cfg->bank.threshold, cfg->max_candidates, masks))
+      put(R.events, R.events_cap, &R.nevents, e);
+  }
+  R.seconds_single_events = now_s() - t_events;
+  R.seconds_search = now_s() - search_start;
   *npeaks = found.size();
   if (peaks) std::memcpy(peaks, found.data(), std::min(found.size(), peaks_cap) * sizeof(fbls_peak));
   if (found.size() > peaks_cap)
@@ -595,7 +742,8 @@ extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const doubl
                                     const fbls_pipeline_cfg* cfg, fbls_peak* peaks,
                                     size_t peaks_cap, size_t* npeaks,
                                     fbls_pass_stats* passes, size_t passes_cap,
-                                    size_t* npasses) {
+                                    size_t* npasses, fbls_search_report* report) {
+  const double start = now_s();
   if (!c || (n && (!time || !flux || !err)) || !npeaks || !npasses)
     return fail(FBLS_E_ARG, "null argument");
   *npeaks = 0;
@@ -617,10 +765,11 @@ extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const doubl
   HIP_TRY(hipSetDevice(c->device));
   std::vector<size_t> all(cfg->nd);
   for (size_t i = 0; i < cfg->nd; i++) all[i] = i;
-  std::vector<std::pair<Curve, std::vector<size_t>>> input;
-  input.emplace_back(std::move(curve), std::move(all));
-  return search_families(c, std::move(input), cfg, peaks, peaks_cap, npeaks, passes, passes_cap,
-                         npasses);
+  std::vector<FamInput> input(1);
+  input[0].curve = std::move(curve);
+  input[0].indices = std::move(all);
+  return search_families(c, std::move(input), cfg, start, peaks, peaks_cap, npeaks, passes,
+                         passes_cap, npasses, report);
 }
 
 extern "C" int fbls_search_transits(fbls_ctx* c, const double* time, const double* flux,
@@ -628,7 +777,8 @@ extern "C" int fbls_search_transits(fbls_ctx* c, const double* time, const doubl
                                     const fbls_prep_cfg* prep, const fbls_pipeline_cfg* cfg,
                                     fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
                                     fbls_pass_stats* passes, size_t passes_cap,
-                                    size_t* npasses) {
+                                    size_t* npasses, fbls_search_report* report) {
+  const double start = now_s();
   if (!c || !prep || (n && (!time || !flux)) || !npeaks || !npasses)
     return fail(FBLS_E_ARG, "null argument");
   *npeaks = 0;
@@ -644,14 +794,17 @@ extern "C" int fbls_search_transits(fbls_ctx* c, const double* time, const doubl
   int st = fbls::prepare_families(time, flux, err, group, n, cfg->durations, cfg->nd, prep,
                                   flatten, &fams, &msg);
   if (st) return msg.empty() ? st : fail(st, "%s", msg.c_str());
-  std::vector<std::pair<Curve, std::vector<size_t>>> input;
+  std::vector<FamInput> input;
   for (fbls::Family& f : fams) {
-    Curve cv;
-    cv.t = std::move(f.time);
-    cv.f = std::move(f.flux);
-    cv.s = std::move(f.sigma);
-    input.emplace_back(std::move(cv), std::move(f.indices));
+    FamInput in;
+    in.curve.t = std::move(f.time);
+    in.curve.f = std::move(f.flux);
+    in.curve.s = std::move(f.sigma);
+    in.indices = std::move(f.indices);
+    in.window = f.window;
+    in.clipped = f.clipped;
+    input.push_back(std::move(in));
   }
-  return search_families(c, std::move(input), cfg, peaks, peaks_cap, npeaks, passes, passes_cap,
-                         npasses);
+  return search_families(c, std::move(input), cfg, start, peaks, peaks_cap, npeaks, passes,
+                         passes_cap, npasses, report);
 }

@@ -293,20 +293,74 @@ typedef struct {
   int model_comparison;
   double planet_log_odds_min;
   fbls_evidence_cfg evidence;
+  int single_events;         /* run the single-event scan (pipeline.rs:225-238) */
 } fbls_pipeline_cfg;
 
-/* `search_prepared` + `search_families` (pipeline.rs:41-221) without the
- * single-event scan: keeps finite points with positive errors sorted by
- * time, builds the family state, then per pass runs the signal search,
- * the greedy peak selection with GPU rescoring (peaks.rs:308-487) and the
- * masking of the lead peak; finally the model comparison of the
- * evidence_candidates strongest peaks.  Peaks in acceptance order
+/* `EventStats` (events.rs:11-16) of one family, plus its bin count. */
+typedef struct {
+  uint64_t templates_scanned, threshold_survivors;
+  uint32_t overflow, phase_bins;
+} fbls_event_stats;
+
+/* `SingleEvent` (src/select/peaks.rs:47-55). */
+typedef struct {
+  double epoch_bjd, duration_hours, snr, depth, depth_ppm;
+  uint64_t points;
+} fbls_single_event;
+
+/* One family in one pass (`statistics.passes[].families[]`, pipeline.rs:
+ * 112-118).  window_days is NaN for prepared input; durations_hours and
+ * rho list the first min(nd, 64) durations of the family. */
+typedef struct {
+  uint64_t pass, family, observations, clipped, segments, periods, nd;
+  double window_days, error_inflation;
+  double durations_hours[64], rho[64];
+  uint64_t band_first, nbands;  /* the family's bands in fbls_search_report.bands */
+} fbls_family_pass;
+
+/* Everything the schema-3 report holds beyond peaks and pass totals
+ * (pipeline.rs:262-288).  The caller provides the arrays (NULL and 0 skip
+ * one); the call sets every count to the number available, writes at most
+ * the capacity, and fills the scalars. */
+typedef struct {
+  fbls_single_event* events;      size_t events_cap;      size_t nevents;
+  fbls_event_stats* event_stats;  size_t event_stats_cap; size_t nevent_stats;
+  fbls_row* period_table;         size_t period_table_cap; size_t period_table_rows;
+  uint64_t* period_table_family;  /* period_table_cap entries, or NULL */
+  fbls_family_pass* families;     size_t families_cap;    size_t nfamilies;
+  fbls_scan_stats* bands;         size_t bands_cap;       size_t nbands;
+  uint64_t observations, periods_tested, nfam;
+  double baseline_days, seconds_preparation, seconds_search, seconds_scan,
+      seconds_select, seconds_evidence, seconds_single_events;
+} fbls_search_report;
+
+/* `search_prepared` + `search_families` (pipeline.rs:41-238): keeps finite
+ * points with positive errors sorted by time, builds the family state, then
+ * per pass runs the signal search, the greedy peak selection with GPU
+ * rescoring (peaks.rs:308-487) and the masking of the lead peak; then the
+ * model comparison of the evidence_candidates strongest peaks and, when
+ * cfg->single_events, the single-event scan of every family with the
+ * greedy event selection (peaks.rs:490-550).  Peaks in acceptance order
  * (capacity peaks_cap, *npeaks is the total); pass statistics capacity
- * passes_cap. */
+ * passes_cap; report (may be NULL) receives the single events and the
+ * statistics of the schema-3 report. */
 int fbls_search_prepared(fbls_ctx* ctx, const double* time, const double* flux,
                          const double* err, size_t n, const fbls_pipeline_cfg* cfg,
                          fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
-                         fbls_pass_stats* passes, size_t passes_cap, size_t* npasses);
+                         fbls_pass_stats* passes, size_t passes_cap, size_t* npasses,
+                         fbls_search_report* report);
+
+/* `single_event_search` (events.rs:18-153, kernels.rs:849-930): every
+ * template at the trial period baseline + 2 max(D) + 1e-6 (nb = max(8,
+ * ceil(phase_oversample x trial / min(D))) bins) whose score reaches the
+ * threshold, as rows with period +inf, ordered by (phase bin, duration in
+ * ascending order, shape).  The family's times must ascend.  More than
+ * `capacity` survivors set stats->overflow and return no rows. */
+int fbls_single_event_search(fbls_ctx* ctx, const fbls_family* fam, const double* durations,
+                             const double* rho, size_t nd, const double* ingress, size_t ns,
+                             uint32_t layers, double threshold, uint32_t phase_oversample,
+                             size_t capacity, fbls_row* rows, size_t* nrows,
+                             fbls_event_stats* stats);
 
 /* -- learned band proposal --------------------------------------------- */
 
@@ -392,7 +446,8 @@ int fbls_search_transits(fbls_ctx* ctx, const double* time, const double* flux,
                          const double* err, const int64_t* group, size_t n,
                          const fbls_prep_cfg* prep, const fbls_pipeline_cfg* cfg,
                          fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
-                         fbls_pass_stats* passes, size_t passes_cap, size_t* npasses);
+                         fbls_pass_stats* passes, size_t passes_cap, size_t* npasses,
+                         fbls_search_report* report);
 
 /* -- host helpers [host] ------------------------------------------------ */
 

@@ -106,7 +106,31 @@ class OrcPipelineCfg(ctypes.Structure):
                 ("correlated", ctypes.c_int), ("bank", BankCfg), ("search", SearchCfg),
                 ("max_candidates", _sz), ("passes", _sz), ("min_events", _sz),
                 ("evidence_candidates", _sz), ("model_comparison", ctypes.c_int),
-                ("planet_log_odds_min", _d), ("evidence", EvidenceCfg)]
+                ("planet_log_odds_min", _d), ("evidence", EvidenceCfg),
+                ("single_events", ctypes.c_int)]
+
+
+class EventStats(ctypes.Structure):
+    _fields_ = [("templates_scanned", _u64), ("threshold_survivors", _u64),
+                ("overflow", _u32), ("phase_bins", _u32)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["overflow"] = bool(d["overflow"])
+        return d
+
+
+class SingleEvent(ctypes.Structure):
+    _fields_ = [(k, _d) for k in ("epoch_bjd", "duration_hours", "snr", "depth",
+                                  "depth_ppm")] + [("points", _u64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EventsOut(ctypes.Structure):
+    _fields_ = [("events", _P(SingleEvent)), ("events_cap", _sz), ("nevents", _sz),
+                ("stats", _P(EventStats)), ("stats_cap", _sz), ("nstats", _sz)]
 
 
 class PrepCfg(ctypes.Structure):
@@ -154,7 +178,14 @@ def lib():
         L.orc_evidence_parameters.restype = None
         L.orc_search_prepared.argtypes = [
             _P(_d), _P(_d), _P(_d), _sz, _P(OrcPipelineCfg), _P(OrcPeak), _sz,
-            _P(_sz), _P(OrcPassStats), _P(_sz)]
+            _P(_sz), _P(OrcPassStats), _P(_sz), _P(EventsOut)]
+        L.orc_single_event_search.argtypes = [
+            _P(_d), _P(_d), _P(_d), _sz, _d, _d, _P(_d), _P(_d), _sz, _P(_d), _sz, _sz, _d,
+            _u32, _sz, _P(Row), _P(_sz), _P(EventStats)]
+        L.orc_single_event_search.restype = None
+        L.orc_select_events.argtypes = [_P(Row), _sz, _P(_d), _sz, _d, _sz,
+                                        _P(_P(ctypes.c_uint8)), _sz, _P(SingleEvent)]
+        L.orc_select_events.restype = _sz
         L.orc_coarse_features.argtypes = [_P(_d), _P(_d), _P(_d), _sz, _d, _P(_d), _sz,
                                           _P(_d), _sz, _u32, _u32, _P(_d)]
         L.orc_coarse_features.restype = None
@@ -168,7 +199,7 @@ def lib():
             _P(_sz), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz), _P(ctypes.c_uint8)]
         L.orc_search_transits.argtypes = [
             _P(_d), _P(_d), _P(_d), _P(ctypes.c_int64), _sz, _P(PrepCfg), _P(OrcPipelineCfg),
-            _P(OrcPeak), _sz, _P(_sz), _P(OrcPassStats), _P(_sz)]
+            _P(OrcPeak), _sz, _P(_sz), _P(OrcPassStats), _P(_sz), _P(EventsOut)]
         L.orc_transit_mask.argtypes = [_P(_d), _sz, _d, _d, _d, _d, _P(ctypes.c_uint8)]
         L.orc_transit_mask.restype = None
         L.orc_event_count.argtypes = [_P(_d), _sz, _d, _d, _d, _sz]
@@ -529,12 +560,26 @@ def _pipeline_cfg(cfg, keep):
         max_candidates=cfg.max_candidates, passes=cfg.passes,
         min_events=cfg.min_events, evidence_candidates=cfg.evidence_candidates,
         model_comparison=int(bool(cfg.model_comparison)),
-        planet_log_odds_min=cfg.planet_log_odds_min, evidence=ev)
+        planet_log_odds_min=cfg.planet_log_odds_min, evidence=ev,
+        single_events=int(bool(getattr(cfg, "single_events", False))))
     keep.extend([d, ing])
     return c
 
 
-def search_prepared(time, flux, err, cfg):
+def _events_out(cfg, nfam_cap, keep):
+    cap = max(cfg.max_candidates, 1)
+    ev = (SingleEvent * cap)()
+    st = (EventStats * max(nfam_cap, 1))()
+    keep.extend([ev, st])
+    return EventsOut(ev, cap, 0, st, len(st), 0)
+
+
+def _events_result(out):
+    return dict(events=[out.events[i].as_dict() for i in range(min(out.nevents, out.events_cap))],
+                event_stats=[out.stats[i].as_dict() for i in range(min(out.nstats, out.stats_cap))])
+
+
+def search_prepared(time, flux, err, cfg, with_events=False):
     """search_prepared + search_families (pipeline.rs:41-221, no single
     events): (peaks in acceptance order, pass statistics)."""
     time, flux, err = _f64(time), _f64(flux), _f64(err)
@@ -544,13 +589,15 @@ def search_prepared(time, flux, err, cfg):
     peaks = (OrcPeak * cap)()
     passes = (OrcPassStats * max(cfg.passes, 1))()
     npk, npa = _sz(), _sz()
+    eo = _events_out(cfg, 1, keep)
     st = lib().orc_search_prepared(_dp(time), _dp(flux), _dp(err), len(time),
                                    ctypes.byref(c), peaks, cap, ctypes.byref(npk),
-                                   passes, ctypes.byref(npa))
+                                   passes, ctypes.byref(npa), ctypes.byref(eo))
     if st:
         raise ValueError(f"search_prepared status {st}")
-    return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
-            [passes[i].as_dict() for i in range(npa.value)])
+    out = ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
+           [passes[i].as_dict() for i in range(npa.value)])
+    return out + (_events_result(eo),) if with_events else out
 
 
 def coarse_features(t, v, w, baseline_norm, periods, durations, nb_min=16, nb_max=1024):
@@ -631,7 +678,7 @@ def prepare_families(time, flux, err, group, durations, cfg):
     return out
 
 
-def search_transits(time, flux, err, group, cfg):
+def search_transits(time, flux, err, group, cfg, with_events=False):
     """search_transits (pipeline.rs:27-37, 70-221, no single events) on raw
     flux: (peaks in acceptance order, pass statistics)."""
     time, flux = _f64(time), _f64(flux)
@@ -644,10 +691,43 @@ def search_transits(time, flux, err, group, cfg):
     peaks = (OrcPeak * cap)()
     passes = (OrcPassStats * max(cfg.passes, 1))()
     npk, npa = _sz(), _sz()
+    eo = _events_out(cfg, len(cfg.durations_days()), keep)
     st = lib().orc_search_transits(_dp(time), _dp(flux), _dp(e) if e is not None else None,
                                    _i64p(g), len(time), ctypes.byref(pc), ctypes.byref(c),
-                                   peaks, cap, ctypes.byref(npk), passes, ctypes.byref(npa))
+                                   peaks, cap, ctypes.byref(npk), passes, ctypes.byref(npa),
+                                   ctypes.byref(eo))
     if st:
         raise ValueError(f"search_transits status {st}")
-    return ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
-            [passes[i].as_dict() for i in range(npa.value)])
+    out = ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
+           [passes[i].as_dict() for i in range(npa.value)])
+    return out + (_events_result(eo),) if with_events else out
+
+
+def single_event_search(t, v, w, C, origin, durations, rho, cfg, capacity=131072):
+    """events.rs:18-153 on one family's fields: (rows as dicts, stats)."""
+    t, v, w = _f64(t), _f64(v), _f64(w)
+    d, r = _f64(durations), _f64(rho)
+    ing = _f64(cfg.ingress_fractions)
+    rows = (Row * max(capacity, 1))()
+    nr = _sz()
+    st = EventStats()
+    lib().orc_single_event_search(_dp(t), _dp(v), _dp(w), len(t), C, origin, _dp(d), _dp(r),
+                                  len(d), _dp(ing), len(ing), cfg.shape_layers, cfg.threshold,
+                                  cfg.phase_oversample, capacity, rows, ctypes.byref(nr),
+                                  ctypes.byref(st))
+    keys = ("period", "epoch", "duration", "score", "depth")
+    return [{k: getattr(rows[i], k) for k in keys} for i in range(nr.value)], st.as_dict()
+
+
+def select_events(rows, time, threshold, limit, masks=()):
+    """peaks.rs:490-550: rows are dicts with epoch, duration, score, depth."""
+    tab = (Row * max(len(rows), 1))()
+    for i, r in enumerate(rows):
+        tab[i] = Row(float("inf"), r["epoch"], r["duration"], r["score"], r["depth"])
+    time = _f64(time)
+    ms = [np.ascontiguousarray(m, dtype=np.uint8) for m in masks]
+    mp = (_P(ctypes.c_uint8) * max(len(ms), 1))(*[m.ctypes.data_as(_P(ctypes.c_uint8)) for m in ms])
+    out = (SingleEvent * max(limit, 1))()
+    n = lib().orc_select_events(tab, len(rows), _dp(time), len(time), threshold, limit, mp,
+                                len(ms), out)
+    return [out[i].as_dict() for i in range(n)]

@@ -244,14 +244,47 @@ typedef struct {
   int model_comparison;
   double planet_log_odds_min;
   orc_evidence_cfg evidence;
+  int single_events;
 } orc_pipeline_cfg;
+
+/* events.rs:11-16 (EventStats) plus the bin count; peaks.rs:47-55 */
+typedef struct {
+  uint64_t templates_scanned, threshold_survivors;
+  uint32_t overflow, phase_bins;
+} orc_event_stats;
+typedef struct {
+  double epoch_bjd, duration_hours, snr, depth, depth_ppm;
+  uint64_t points;
+} orc_single_event;
+/* single events of a search: selected events (capacity events_cap) and the
+ * scan statistics of each family (capacity stats_cap); counts are totals */
+typedef struct {
+  orc_single_event* events;
+  size_t events_cap, nevents;
+  orc_event_stats* stats;
+  size_t stats_cap, nstats;
+} orc_events_out;
+
+/* events.rs:18-153: every template at the single trial period whose score
+ * reaches the threshold, in (phase bin, ascending duration, shape) order.
+ * rows: capacity x {period, epoch, duration, score, depth}. */
+void orc_single_event_search(const double* t, const double* v, const double* w, size_t n,
+                             double C, double origin, const double* durations,
+                             const double* rho, size_t nd, const double* ingress, size_t ns,
+                             size_t layers, double threshold, uint32_t phase_oversample,
+                             size_t capacity, orc_row* rows, size_t* nrows,
+                             orc_event_stats* stats);
+/* peaks.rs:490-550: masks n_masks x n_time; out capacity limit */
+size_t orc_select_events(const orc_row* table, size_t nrows, const double* time,
+                         size_t n_time, double threshold, size_t limit,
+                         uint8_t* const* masks, size_t n_masks, orc_single_event* out);
 
 /* pipeline.rs:41-221 (search_prepared without single events): peaks in
  * acceptance order (capacity cap), pass statistics (capacity passes). */
 int orc_search_prepared(const double* time, const double* flux,
                         const double* err, size_t n, const orc_pipeline_cfg* cfg,
                         orc_peak* peaks, size_t cap, size_t* npeaks,
-                        orc_pass_stats* pstats, size_t* npasses);
+                        orc_pass_stats* pstats, size_t* npasses, orc_events_out* events);
 
 /* Coarse periodogram of the band-proposal network (oracle/proposal.c):
  * out[i][g] for np periods and ng durations; see fbls_coarse_features. */
@@ -284,7 +317,8 @@ int orc_prepare_families(const double* time, const double* flux, const double* e
 int orc_search_transits(const double* time, const double* flux, const double* err,
                         const int64_t* group, size_t n, const orc_prep_cfg* prep,
                         const orc_pipeline_cfg* cfg, orc_peak* peaks, size_t cap,
-                        size_t* npeaks, orc_pass_stats* pstats, size_t* npasses);
+                        size_t* npeaks, orc_pass_stats* pstats, size_t* npasses,
+                        orc_events_out* events);
 
 /* survey/numeric.rs:345-364 */
 uint64_t orc_rng_new(uint64_t seed);

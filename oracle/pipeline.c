@@ -564,7 +564,7 @@ static int search_families(size_t nfam, const double* ft, const double* ff, cons
                            const size_t* foff, const size_t* fn, const size_t* fidx,
                            const size_t* fmoff, const orc_pipeline_cfg* cfg, orc_peak* peaks,
                            size_t cap, size_t* npeaks, orc_pass_stats* pstats,
-                           size_t* npasses) {
+                           size_t* npasses, orc_events_out* events) {
   const double* dur = cfg->durations;
   int st = 0;
   fam_t* cur = (fam_t*)calloc(nfam + 1, sizeof(fam_t));
@@ -752,6 +752,52 @@ static int search_families(size_t nfam, const double* ft, const double* ff, cons
     }
     free(ord);
   }
+  /* single events (pipeline.rs:222-238) of the families as the passes left them */
+  if (!st && cfg->single_events && events) {
+    const size_t capacity = 131072;
+    const size_t limit = cfg->max_candidates;
+    orc_row* rows = (orc_row*)malloc(capacity * sizeof(orc_row));
+    orc_row* all = (orc_row*)malloc((nfam * limit + 1) * sizeof(orc_row));
+    orc_single_event* sel = (orc_single_event*)malloc((limit + 1) * sizeof(orc_single_event));
+    size_t nall = 0;
+    events->nstats = 0;
+    for (size_t k = 0; k < nfam; k++) {
+      const fam_t* f = &cur[k];
+      size_t nr = 0;
+      orc_event_stats es;
+      orc_single_event_search(f->t, f->v, f->w, f->n, f->C, f->origin, f->dur, f->rho, f->nd,
+                              cfg->ingress, cfg->ns, cfg->bank.layers, cfg->bank.threshold,
+                              cfg->bank.phase_oversample, capacity, rows, &nr, &es);
+      if (events->stats && events->nstats < events->stats_cap) events->stats[events->nstats] = es;
+      events->nstats++;
+      size_t m = orc_select_events(rows, nr, time_all, n_all, cfg->bank.threshold, limit, masks,
+                                   n_masks, sel);
+      for (size_t q = 0; q < m; q++) {
+        orc_row* r = &all[nall++];
+        r->period = INFINITY;
+        r->epoch = sel[q].epoch_bjd;
+        r->duration = sel[q].duration_hours / 24.0;
+        r->score = sel[q].snr;
+        r->depth = sel[q].depth;
+      }
+    }
+    for (size_t i = 1; i < nall; i++) { /* stable sort by score, descending */
+      orc_row x = all[i];
+      size_t j = i;
+      while (j > 0 && all[j - 1].score < x.score) {
+        all[j] = all[j - 1];
+        j--;
+      }
+      all[j] = x;
+    }
+    size_t m = orc_select_events(all, nall, time_all, n_all, cfg->bank.threshold, limit, masks,
+                                 n_masks, sel);
+    events->nevents = m;
+    for (size_t q = 0; q < m && q < events->events_cap; q++) events->events[q] = sel[q];
+    free(rows);
+    free(all);
+    free(sel);
+  }
   for (size_t i = 0; i < n_masks; i++) free(masks[i]);
   free(masks);
   free(excluded);
@@ -768,7 +814,7 @@ static int search_families(size_t nfam, const double* ft, const double* ff, cons
 int orc_search_prepared(const double* time, const double* flux,
                         const double* err, size_t n, const orc_pipeline_cfg* cfg,
                         orc_peak* peaks, size_t cap, size_t* npeaks,
-                        orc_pass_stats* pstats, size_t* npasses) {
+                        orc_pass_stats* pstats, size_t* npasses, orc_events_out* events) {
   *npeaks = 0;
   *npasses = 0;
   /* pipeline.rs:47-61: finite, positive error, sorted by time (stable) */
@@ -797,7 +843,7 @@ int orc_search_prepared(const double* time, const double* flux,
   for (size_t i = 0; i < cfg->nd; i++) idx[i] = i;
   const size_t foff[1] = {0}, fn[1] = {k}, fmoff[2] = {0, cfg->nd};
   int st = search_families(1, ct, cf, cs, foff, fn, idx, fmoff, cfg, peaks, cap, npeaks, pstats,
-                           npasses);
+                           npasses, events);
   free(idx);
   free(ct);
   free(cf);
@@ -809,7 +855,8 @@ int orc_search_prepared(const double* time, const double* flux,
 int orc_search_transits(const double* time, const double* flux, const double* err,
                         const int64_t* group, size_t n, const orc_prep_cfg* prep,
                         const orc_pipeline_cfg* cfg, orc_peak* peaks, size_t cap,
-                        size_t* npeaks, orc_pass_stats* pstats, size_t* npasses) {
+                        size_t* npeaks, orc_pass_stats* pstats, size_t* npasses,
+                        orc_events_out* events) {
   *npeaks = 0;
   *npasses = 0;
   const size_t nd = cfg->nd;
@@ -836,7 +883,7 @@ int orc_search_transits(const double* time, const double* flux, const double* er
     }
     fmoff[nfam] = m;
     st = search_families(nfam, ft, ff, fs, foff, fn, fidx, fmoff, cfg, peaks, cap, npeaks, pstats,
-                         npasses);
+                         npasses, events);
     free(foff);
     free(fidx);
     free(fmoff);
