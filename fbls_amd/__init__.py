@@ -56,7 +56,8 @@ EXPORTS = (
     "fbls_cnn_set_params", "fbls_cnn_forward", "fbls_cnn_loss_grad",
     "fbls_cnn_train_step", "fbls_coarse_features", "fbls_ctx_set_graphs",
     "fbls_autotune", "fbls_biweight_flatten", "fbls_prepare_families",
-    "fbls_search_transits", "fbls_single_event_search",
+    "fbls_search_transits", "fbls_single_event_search", "fbls_read_inputs",
+    "fbls_format_f64", "fbls_json_reformat", "fbls_cli",
 )
 
 
@@ -300,6 +301,12 @@ def lib():
             _vp, _P(_d), _P(_d), _P(_d), _P(ctypes.c_int64), _sz, _P(PrepCfg),
             _P(PipelineCfgC), _P(Peak), _sz, _P(_sz), _P(PassStats), _sz, _P(_sz),
             _P(SearchReport)]
+        L.fbls_read_inputs.argtypes = [_P(ctypes.c_char_p), _sz, _P(_d), _P(_d), _P(_d),
+                                       _P(ctypes.c_int64), _sz, _P(_sz), _P(ctypes.c_int)]
+        L.fbls_format_f64.argtypes = [_d, ctypes.c_char_p, _sz]
+        L.fbls_format_f64.restype = _sz
+        L.fbls_json_reformat.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _sz, _P(_sz)]
+        L.fbls_cli.argtypes = [ctypes.c_int, _P(ctypes.c_char_p)]
         L.fbls_single_event_search.argtypes = [
             _vp, _vp, _P(_d), _P(_d), _sz, _P(_d), _sz, _u32, _d, _u32, _sz, _P(Row),
             _P(_sz), _P(EventStats)]
@@ -1098,3 +1105,50 @@ def search_transits(gpu, time, flux, err, group, cfg: SearchConfig, report=False
     out = ([peaks[i].as_dict() for i in range(min(npk.value, cap))],
            [passes[i].as_dict() for i in range(min(npa.value, len(passes)))])
     return out + (_report_dict(rep),) if report else out
+
+
+# ------------------------------------------------------------ formats
+
+def read_inputs(paths):
+    """fbls_read_inputs (main.rs:50-132): dict of time, flux, err (None when
+    the files carry no flux_err) and group."""
+    enc = [os.fsencode(p) for p in paths]
+    arr = (ctypes.c_char_p * max(len(enc), 1))(*enc)
+    n, he = _sz(), ctypes.c_int()
+    _check(lib().fbls_read_inputs(arr, len(enc), None, None, None, None, 0, ctypes.byref(n),
+                                  ctypes.byref(he)))
+    m = n.value
+    t, f, e = np.empty(m), np.empty(m), np.empty(m)
+    g = np.empty(m, dtype=np.int64)
+    _check(lib().fbls_read_inputs(arr, len(enc), _dp(t), _dp(f), _dp(e),
+                                  g.ctypes.data_as(_P(ctypes.c_int64)), m, ctypes.byref(n),
+                                  ctypes.byref(he)))
+    return dict(time=t, flux=f, err=e if he.value else None, group=g)
+
+
+def format_f64(x):
+    """The report's text of a double (fbls_format_f64)."""
+    buf = ctypes.create_string_buffer(64)
+    lib().fbls_format_f64(float(x), buf, 64)
+    return buf.value.decode()
+
+
+def json_reformat(text):
+    """fbls_json_reformat: JSON text re-printed in the report layout."""
+    raw = text.encode() if isinstance(text, str) else text
+    n = _sz()
+    _check(lib().fbls_json_reformat(raw, None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _check(lib().fbls_json_reformat(raw, buf, n.value + 1, ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def cli(args):
+    """fbls_cli in this process: args[0] is "search" or "batch"; returns the
+    exit status."""
+    enc = [os.fsencode(a) for a in args]
+    arr = (ctypes.c_char_p * max(len(enc), 1))(*enc)
+    return lib().fbls_cli(len(enc), arr)
+
+
+CLI_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fbls")

@@ -125,7 +125,10 @@ int fbls_autotune(fbls_ctx* ctx, const fbls_family* fam, const double* durations
  * kernels and the copies back are captured per call and updated into one
  * cached executable graph).  enable: 1 on (default), 0 off (eager launches),
  * negative leaves it unchanged; *graph_launches (may be NULL) counts graph
- * submissions on this context.  Results never depend on it. */
+ * submissions on this context.  Results never depend on it.  The HIP
+ * runtime fails allocations made by other threads while a stream of the
+ * process is being captured (and then the capture too), so a process that
+ * uses several contexts from concurrent threads turns graphs off on each. */
 int fbls_ctx_set_graphs(fbls_ctx* ctx, int enable, uint64_t* graph_launches);
 
 /* -- family upload ------------------------------------------------------ */
@@ -515,6 +518,32 @@ typedef struct {
 /* Writes up to cap points; returns the count through *n. */
 int fbls_synth_star(const fbls_synth_cfg* cfg, double* time, double* flux,
                     double* flux_err, size_t cap, size_t* n);
+
+/* -- formats and the command-line host [host] ----------------------------- */
+
+/* [host] `read_inputs` (main.rs:50-132) over NPZ files (npz.rs:17-113):
+ * time_bjd and flux required, flux_err in all files or none, sector a
+ * scalar or per observation; group numbers the (file, sector) pairs in
+ * order of first appearance.  Writes min(cap, total) values of each array
+ * (NULL skips one) and the total through *n; FITS input is not supported. */
+int fbls_read_inputs(const char* const* paths, size_t npaths, double* time, double* flux,
+                     double* err, int64_t* group, size_t cap, size_t* n, int* has_err);
+
+/* [host] The shortest text that reads back to x, as the schema-3 report
+ * prints doubles ("0.0", "1e-12", "null" when not finite).  Returns its
+ * length; writes at most cap - 1 characters and a terminator. */
+size_t fbls_format_f64(double x, char* buf, size_t cap);
+
+/* [host] JSON text re-printed in the report's layout (keys sorted, two-space
+ * indent); *len receives the full length. */
+int fbls_json_reformat(const char* text, char* out, size_t cap, size_t* len);
+
+/* The `fbls` command line (main.rs:137-246, batch.rs:36-109): argv[0] is
+ * "search" (--input FILES [--output F] [--save-table F] [--prepared]
+ * [--device N] [--pmin D] [--pmax D] [--prescan F] [--threshold S]) or
+ * "batch" (--list FILE --output-dir DIR [--workers K] [--devices 0,1]
+ * [--pmax D]).  Returns 0, or 1 after printing "fbls: message". */
+int fbls_cli(int argc, const char* const* argv);
 
 /* Thread-local message of the last failing call on this thread. */
 const char* fbls_last_error(void);
