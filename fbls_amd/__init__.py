@@ -57,7 +57,7 @@ EXPORTS = (
     "fbls_cnn_train_step", "fbls_coarse_features", "fbls_ctx_set_graphs",
     "fbls_autotune", "fbls_biweight_flatten", "fbls_prepare_families",
     "fbls_search_transits", "fbls_single_event_search", "fbls_read_inputs",
-    "fbls_format_f64", "fbls_json_reformat", "fbls_cli",
+    "fbls_format_f64", "fbls_json_reformat", "fbls_cli", "fbls_coarse_inputs",
 )
 
 
@@ -329,6 +329,8 @@ def lib():
             _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8)]
         L.fbls_coarse_features.argtypes = [_vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz,
                                            _u32, _u32, _P(_d)]
+        L.fbls_coarse_inputs.argtypes = [_vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz,
+                                           _u32, _u32, _P(ctypes.c_float)]
         L.fbls_family_state.argtypes = [
             _P(_d), _P(_d), _P(_d), _sz, _P(_d), _sz, ctypes.c_int, _P(_d),
             _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz),
@@ -738,20 +740,22 @@ def scan_bank(gpu, dev, periods, bank, nb, winner=True):
     return rec, score, depth, win, st.as_dict()
 
 
-def _rows(rows, n):
-    return np.array([(r.period, r.epoch, r.duration, r.score, r.depth)
-                     for r in rows[:n]]).reshape(-1, 5)
+def _row_buffer(n):
+    """An uninitialised (n, 5) float64 array laid out as n fbls_row structs,
+    and its pointer."""
+    buf = np.empty((max(n, 1), 5))
+    return buf, buf.ctypes.data_as(_P(Row))
 
 
 def scan_periods(gpu, dev, periods, bank):
     periods = _f64(periods)
-    rows = (Row * max(1, len(periods)))()
+    buf, rows = _row_buffer(len(periods))
     nrows = _sz()
     st = ScanStats()
     _check(lib().fbls_scan_periods(gpu._h, dev._h, _dp(periods), len(periods),
                                    ctypes.byref(bank), rows,
                                    ctypes.byref(nrows), ctypes.byref(st)))
-    return _rows(rows, nrows.value), st.as_dict()
+    return buf[:nrows.value].copy(), st.as_dict()
 
 
 def signal_search(gpu, dev, durations, rho, cfg: SearchConfig):
@@ -766,7 +770,7 @@ def signal_search(gpu, dev, durations, rho, cfg: SearchConfig):
                                            cfg.period_oversample)
     bank = make_bank(cfg, durations, rho)
     scfg = SearchCfgC(cfg.period_min, cfg.period_max, cfg.period_oversample)
-    rows = (Row * cap)()
+    buf, rows = _row_buffer(cap)
     stats = (ScanStats * 64)()
     grid = np.empty(cap)
     nrows, nbands, gcount = _sz(), _sz(), _sz()
@@ -775,7 +779,7 @@ def signal_search(gpu, dev, durations, rho, cfg: SearchConfig):
         ctypes.byref(bank), ctypes.byref(scfg), rows, cap,
         ctypes.byref(nrows), stats, 64, ctypes.byref(nbands), _dp(grid), cap,
         ctypes.byref(gcount)))
-    return (_rows(rows, nrows.value), grid[:gcount.value],
+    return (buf[:nrows.value].copy(), grid[:gcount.value],
             [s.as_dict() for s in stats[:nbands.value]])
 
 
@@ -801,7 +805,7 @@ def signal_search_batch(gpu, devs, durations, rhos, cfg: SearchConfig,
             if hi > lo:
                 rows_cap += lib().fbls_period_grid_count(lo, hi, d.t_last, dmin,
                                                          cfg.period_oversample)
-    rows = (Row * max(rows_cap, 1))()
+    buf, rows = _row_buffer(rows_cap)
     kp = None
     if keep is not None:
         kflat = np.ascontiguousarray(np.concatenate(
@@ -811,9 +815,7 @@ def signal_search_batch(gpu, devs, durations, rhos, cfg: SearchConfig,
         gpu._h, hs, len(devs), _dp(durations), _dp(rho), len(durations),
         ctypes.byref(bank), ctypes.byref(scfg), rows, rows_cap,
         offs.ctypes.data_as(_P(_sz)), stats, kp))
-    need = int(offs[-1])
-    allrows = _rows(rows, need)
-    return ([allrows[int(offs[i]):int(offs[i + 1])] for i in range(len(devs))],
+    return ([buf[int(offs[i]):int(offs[i + 1])] for i in range(len(devs))],
             [s.as_dict() for s in stats])
 
 
@@ -1023,6 +1025,18 @@ class Cnn:
             self.close()
         except Exception:
             pass
+
+
+def coarse_inputs(gpu, devs, periods, durations, nb_min=16, nb_max=1024):
+    """The network input (fbls_coarse_inputs): clip(SNR, -5, 40) / 8 as
+    float32, [star, period, duration]."""
+    periods, durations = _f64(periods), _f64(durations)
+    hs = (_vp * len(devs))(*[d._h.value for d in devs])
+    out = np.empty((len(devs), len(periods), len(durations)), dtype=np.float32)
+    _check(lib().fbls_coarse_inputs(gpu._h, hs, len(devs), _dp(periods), len(periods),
+                                    _dp(durations), len(durations), nb_min, nb_max,
+                                    out.ctypes.data_as(_P(ctypes.c_float))))
+    return out
 
 
 def coarse_features(gpu, devs, periods, durations, nb_min=16, nb_max=1024):

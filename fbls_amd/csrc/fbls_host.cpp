@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fbls.h"
@@ -640,6 +641,23 @@ extern "C" int fbls_scan_bank(fbls_ctx* c, const fbls_family* f,
 namespace {
 
 // Host state of one band (one scan_periods call) across the two launches.
+// A host array that is not zero-filled on allocation (every element the
+// scan reads back is written first).
+struct Raw {
+  std::unique_ptr<double[]> p;
+  size_t n = 0;
+  void resize(size_t m) {
+    if (m > n) {
+      p.reset(new double[m]);
+      n = m;
+    }
+  }
+  double* data() { return p.get(); }
+  const double* data() const { return p.get(); }
+  double& operator[](size_t i) { return p[i]; }
+  const double& operator[](size_t i) const { return p[i]; }
+};
+
 struct BandRun {
   const fbls_family* fam;
   const double* periods;
@@ -647,7 +665,8 @@ struct BandRun {
   uint32_t nb, nb_coarse, nd;
   const double* dur;
   const double* rho;
-  std::vector<double> rec, score, depth, survivors;
+  Raw rec, score, depth;
+  std::vector<double> survivors;
   fbls_scan_stats stats;
 };
 
@@ -694,7 +713,7 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
   double c1 = 0.0;
   for (const Work& x : first) c1 += x.cost;
   const double cut = b->prescan_fraction * b->threshold;
-  std::vector<std::vector<double>> rec2(bands.size()), sc2(bands.size()), dp2(bands.size());
+  std::vector<Raw> rec2(bands.size()), sc2(bands.size()), dp2(bands.size());
   for (size_t k = 0; k < first.size(); k++) {
     BandRun& r = bands[first_band[k]];
     const Work& x = first[k];
@@ -864,11 +883,15 @@ extern "C" uint64_t fbls_bank_batches(size_t np, uint32_t nb,
 namespace {
 
 // One family's search plan (scan.rs:413-471): sorted bank, grid, bands.
+// With a keep mask the grid is not materialised: the bands are found by
+// binary search on the grid formula and only kept periods are evaluated.
 struct SearchPlan {
   const fbls_family* fam = nullptr;
   std::vector<double> sd, sr, grid;
   std::vector<fbls_band> bands;
   std::vector<std::vector<double>> kept;  // per band, when a keep mask applies
+  fbls::GridFormula formula;
+  size_t count = 0;                       // grid periods
 };
 
 // Restricts each band to the grid periods whose keep flag is set; each kept
@@ -878,13 +901,13 @@ void apply_keep(SearchPlan& p, const uint8_t* keep) {
   for (size_t b = 0; b < p.bands.size(); b++)
     for (size_t i = 0; i < p.bands[b].count; i++) {
       const size_t g = p.bands[b].first + i;
-      if (keep[g]) p.kept[b].push_back(p.grid[g]);
+      if (keep[g]) p.kept[b].push_back(p.grid.empty() ? p.formula(g) : p.grid[g]);
     }
 }
 
 int plan_search(const fbls_family* f, const double* durations, const double* rho,
                 size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
-                SearchPlan* p) {
+                SearchPlan* p, bool lazy = false) {
   p->fam = f;
   p->grid.clear();
   p->bands.clear();
@@ -911,10 +934,58 @@ int plan_search(const fbls_family* f, const double* durations, const double* rho
   if (maximum <= minimum) return FBLS_OK;
   const size_t count = fbls::period_grid_count(minimum, maximum, baseline, p->sd[0],
                                                cfg->period_oversample);
+  p->count = count;
+  p->formula = fbls::GridFormula(minimum, maximum, count);
+  if (lazy) {
+    p->bands = fbls::plan_bands(p->formula, p->sd.data(), nd, bank->phase_oversample,
+                                bank->prescan_factor);
+    return FBLS_OK;
+  }
   p->grid.resize(count);
   fbls::period_grid(minimum, maximum, p->grid.data(), count);
   p->bands = fbls::plan_bands(p->grid.data(), count, p->sd.data(), nd,
                               bank->phase_oversample, bank->prescan_factor);
+  return FBLS_OK;
+}
+
+// Plans stars [s0, s1) on up to `threads` host threads; keep (may be null)
+// is the batch's flat mask and keep_at[s - s0] each star's offset in it.
+int plan_stars(const fbls_family* const* fams, size_t s0, size_t s1, const double* durations,
+               const double* rho, size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
+               const uint8_t* keep, const std::vector<size_t>& keep_at,
+               std::vector<SearchPlan>* plans) {
+  const size_t m = s1 - s0;
+  plans->assign(m, SearchPlan());
+  std::vector<int> status(m, FBLS_OK);
+  std::vector<std::string> message(m);
+  auto work = [&](size_t a, size_t e) {
+    for (size_t k = a; k < e; k++) {
+      const size_t s = s0 + k;
+      if (!fams[s]) {
+        status[k] = FBLS_E_ARG;
+        message[k] = "null family " + std::to_string(s);
+        return;
+      }
+      status[k] = plan_search(fams[s], durations, rho + s * nd, nd, bank, cfg, &(*plans)[k],
+                              keep != nullptr);
+      if (status[k]) {
+        message[k] = fbls_last_error();
+        return;
+      }
+      if (keep) apply_keep((*plans)[k], keep + keep_at[k]);
+    }
+  };
+  const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), (m + 31) / 32);
+  if (nt <= 1) {
+    work(0, m);
+  } else {
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nt; t++) pool.emplace_back(work, m * t / nt, m * (t + 1) / nt);
+    for (std::thread& th : pool) th.join();
+  }
+  for (size_t k = 0; k < m; k++)
+    if (status[k]) return fail(status[k], "%s", message[k].c_str());
   return FBLS_OK;
 }
 
@@ -1016,24 +1087,39 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
   fbls_bank b = budgeted(bank);
   size_t written = 0, needed = 0;
   size_t s0 = 0, keep_at = 0;
+  // grid sizes first: they set the chunks and each star's keep offset
+  std::vector<size_t> grid_size(nfam, 0);
+  {
+    double dmin = INFINITY;
+    for (size_t i = 0; i < nd; i++) dmin = std::min(dmin, durations[i]);
+    for (size_t s = 0; s < nfam; s++) {
+      if (!fams[s]) return fail(FBLS_E_ARG, "null family %zu", s);
+      const double baseline = fams[s]->t_last;
+      const double maximum = std::min(cfg->period_max, baseline);
+      const double minimum = std::max(cfg->period_min, 4.0 * dmin * (1.0 + 1e-9));
+      if (maximum > minimum)
+        grid_size[s] = fbls::period_grid_count(minimum, maximum, baseline, dmin,
+                                               cfg->period_oversample);
+    }
+  }
   while (s0 < nfam) {
     // one chunk of stars
     std::vector<SearchPlan> plans;
     std::vector<BandRun> runs;
     std::vector<size_t> run_star;
     size_t periods = 0, s1 = s0;
+    std::vector<size_t> keep_off;
     while (s1 < nfam && (s1 == s0 || periods < kBatchPeriods)) {
-      if (!fams[s1]) return fail(FBLS_E_ARG, "null family %zu", s1);
-      plans.emplace_back();
-      int st = plan_search(fams[s1], durations, rho + s1 * nd, nd, bank, cfg, &plans.back());
-      if (st) return st;
-      if (keep) {
-        apply_keep(plans.back(), keep + keep_at);
-        keep_at += plans.back().grid.size();
-      }
-      periods += plans.back().grid.size();
+      keep_off.push_back(keep_at);
+      keep_at += grid_size[s1];
+      periods += grid_size[s1];
       s1++;
     }
+    if (int st = plan_stars(fams, s0, s1, durations, rho, nd, bank, cfg, keep, keep_off, &plans))
+      return st;
+    for (size_t k = 0; k < plans.size(); k++)
+      if (plans[k].count != grid_size[s0 + k])
+        return fail(FBLS_E_ARG, "grid size of star %zu changed while planning", s0 + k);
     for (size_t k = 0; k < plans.size(); k++) {
       const size_t before = runs.size();
       append_runs(plans[k], runs);
@@ -1047,11 +1133,11 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
       int st = check_bank(&b);
       if (st) return st;
     }
-    std::vector<fbls_row> chunk_rows(periods ? periods : 1);
+    std::unique_ptr<fbls_row[]> chunk_rows(new fbls_row[periods ? periods : 1]);
     std::vector<size_t> counts(runs.size(), 0);
     size_t nchunk = 0;
     if (!runs.empty()) {
-      int st = scan_bands(c, runs, &b, chunk_rows.data(), &nchunk, counts.data());
+      int st = scan_bands(c, runs, &b, chunk_rows.get(), &nchunk, counts.data());
       if (st) return st;
     }
     size_t at = 0;

@@ -21,11 +21,13 @@ struct FeatureArgs {
   const uint32_t* nb;       // np phase bins per period
   uint32_t np, ng;
   const double* durations;  // ng
-  double* out;              // stars x np x ng
+  double* out;              // stars x np x ng best SNRs, or null
+  float* inputs;            // stars x np x ng network inputs, or null
+  uint32_t nb_max;          // largest nb (sets the LDS slots per period)
+  uint32_t G;               // periods per workgroup
 };
 
-size_t feature_lds_bytes(uint32_t nb_max);
-hipError_t feature_launch(const FeatureArgs& a, uint32_t nstars, uint32_t nb_max,
-                          hipStream_t s);
+size_t feature_lds_bytes(uint32_t nb_max, uint32_t G);
+hipError_t feature_launch(const FeatureArgs& a, uint32_t nstars, hipStream_t s);
 
 }  // namespace fbls

@@ -2,13 +2,19 @@
 // of a batch and every period of a shared log-spaced list, the largest box
 // SNR over phase for each of a few representative durations.
 //
-// One workgroup (256 threads) per (star, period): the star's fixed-point
+// One workgroup (256 threads) per star and group of G periods (each lane's
+// observations are loaded once for the whole group): the star's fixed-point
 // fields are folded into an LDS histogram of nb bins (nb = clamp(ceil(2 P /
-// d_0), nb_min, nb_max), d_0 the shortest duration) with exact 64-bit
-// integer sums, turned into an exclusive prefix, and every phase j of every
-// duration d_g (width k = max(1, floor(d_g nb / P + 0.5)) bins, circular) is
-// scored as a / sqrt(w - w^2 / C) when it holds at least three points and a
-// positive precision.  The value is -inf when no phase qualifies.
+// d_0), nb_min, nb_max), d_0 the shortest duration) with exact integer sums
+// held in doubles (runs of consecutive observations in one bin are summed in
+// registers first; the bins of a lane's 8 ascending times follow from the
+// first by the differences of floor(t * scale)), turned into an exclusive
+// prefix by the waves in parallel.  For every duration d_g (width k =
+// max(1, floor(d_g nb / P + 0.5)) bins, circular) the phases j that hold at
+// least three points (and at most n - 3) with a positive precision q = w -
+// w^2 / C qualify; the value is a / sqrt(q) at the qualifying phase with the
+// largest key a |a| / q (a monotone function of a / sqrt(q); the smallest
+// phase on equal keys), -inf when none qualifies.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -21,86 +27,255 @@ namespace {
 constexpr uint32_t kBlock = 256;
 constexpr double kInf = __builtin_inf();
 
+// floor(t * scale) mod nb, plus one (slot b + 1 accumulates bin b)
+__device__ __forceinline__ uint32_t slot_of(double t, double scale, uint32_t nb) {
+  const double cell = floor(t * scale);
+  if (cell < 4294967296.0) return (uint32_t)cell % nb + 1;
+  return (uint32_t)((uint64_t)cell % nb) + 1;
+}
+
+// The sums are integers below 2^53 held in doubles, so every addition is
+// exact and the LDS double atomics are order-independent.
+__device__ __forceinline__ void flush(double* hv, double* hw, uint32_t* hn, uint32_t b,
+                                      double v, double w, uint32_t c) {
+  __hip_atomic_fetch_add(hv + b, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_add(hw + b, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_add(hn + b, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Inclusive scan of slots 0..nb of one array by one wave: lane l owns a
+// contiguous block of slots; block sums, a wave scan of them, then the
+// running sums written back (slot 0 holds zero, so slot x ends as the sum
+// of bins < x).
+template <class T>
+__device__ void wave_inclusive(T* a, uint32_t len, uint32_t lane) {
+  const uint32_t per = (len + 63) / 64;
+  const uint32_t b0 = min(lane * per, len), b1 = min(b0 + per, len);
+  T sum = 0;
+  for (uint32_t x = b0; x < b1; x++) sum += a[x];
+  T inc = sum;
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const T y = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += y;
+  }
+  T run = inc - sum;
+  for (uint32_t x = b0; x < b1; x++) {
+    run += a[x];
+    a[x] = run;
+  }
+}
+
+// template width in bins for duration D: max(1, floor(D * scale + 0.5)), <= nb
+__device__ __forceinline__ uint32_t width_of(double D, double scale, uint32_t nb) {
+  const double kf = floor(D * scale + 0.5);
+  uint32_t k = kf < 1.0 ? 1u : (uint32_t)kf;
+  return k > nb ? nb : k;
+}
+
+// Sums of the circular window [j, j + k) from the exclusive prefix; true
+// when it holds 3 .. n - 3 points and a positive precision q = w - w^2 / C.
+__device__ __forceinline__ bool window(const double* gv, const double* gw, const uint32_t* gn,
+                                       uint32_t nb, uint32_t tn, uint32_t j, uint32_t k,
+                                       const FeatureStar& S, double* pa, double* pq) {
+  const uint32_t e = j + k;
+  double a, w;
+  uint32_t c;
+  if (e <= nb) {
+    a = (gv[e] - gv[j]) * S.sv;
+    w = (gw[e] - gw[j]) * S.sw;
+    c = gn[e] - gn[j];
+  } else {
+    a = (gv[e - nb] + (gv[nb] - gv[j])) * S.sv;
+    w = (gw[e - nb] + (gw[nb] - gw[j])) * S.sw;
+    c = gn[e - nb] + (gn[nb] - gn[j]);
+  }
+  const double q = w - w * w / S.C;
+  *pa = a;
+  *pq = q;
+  return c >= 3 && c + 3 <= tn && q > 0.0;
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+  for (int off = 32; off >= 1; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+  return x;
+}
+
 __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  const uint32_t star = blockIdx.x / A.np, pi = blockIdx.x % A.np;
+  const uint32_t groups = (A.np + A.G - 1) / A.G;
+  const uint32_t star = blockIdx.x / groups, p0 = (blockIdx.x % groups) * A.G;
+  const uint32_t gcount = min(A.G, A.np - p0);
   const FeatureStar S = A.stars[star];
-  const double P = A.periods[pi];
-  const uint32_t nb = A.nb[pi];
-  const double scale = (double)nb / P;
-  long long* hv = (long long*)lds;   // [nb + 1]
-  long long* hw = hv + (nb + 1);     // [nb + 1]
-  uint32_t* hn = (uint32_t*)(hw + (nb + 1));  // [nb + 1]
-  __shared__ double red[kBlock];
-  for (uint32_t b = threadIdx.x; b <= nb; b += kBlock) {
+  const uint32_t stride = A.nb_max + 1;  // slots per period
+  double* hv = (double*)lds;             // [G][stride]
+  double* hw = hv + (size_t)A.G * stride;
+  uint32_t* hn = (uint32_t*)(hw + (size_t)A.G * stride);
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (uint32_t b = tid; b < gcount * stride; b += kBlock) {
     hv[b] = 0;
     hw[b] = 0;
     hn[b] = 0;
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < S.n; i += kBlock) {
-    const uint64_t cell = (uint64_t)floor(S.t[i] * scale);
-    const uint32_t b = (uint32_t)(cell % nb) + 1;  // slot b + 1 holds bin b
-    atomicAdd((unsigned long long*)&hv[b], (unsigned long long)(long long)S.vw[2 * i]);
-    atomicAdd((unsigned long long*)&hw[b], (unsigned long long)(long long)S.vw[2 * i + 1]);
-    atomicAdd(&hn[b], 1u);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {  // exclusive prefix: slot x = sum of bins < x
-    for (uint32_t b = 1; b <= nb; b++) {
-      hv[b] += hv[b - 1];
-      hw[b] += hw[b - 1];
-      hn[b] += hn[b - 1];
+  // fold: each lane loads 8 consecutive observations once per step and, for
+  // every period of the group, sums runs of equal bins in registers (times
+  // are sorted, so neighbours share bins) before one LDS update per run
+  for (uint32_t base = 0; base < S.n; base += kBlock * 8) {
+    const uint32_t i0 = base + tid * 8;
+    if (i0 >= S.n) continue;
+    const uint32_t cnt = min(8u, S.n - i0);
+    double tt[8], vv[8], ww[8];
+#pragma unroll
+    for (uint32_t r = 0; r < 8; r++) {
+      const uint32_t i = r < cnt ? i0 + r : i0;
+      tt[r] = S.t[i];
+      vv[r] = (double)S.vw[2 * (size_t)i];
+      ww[r] = (double)S.vw[2 * (size_t)i + 1];
+    }
+    for (uint32_t g = 0; g < gcount; g++) {
+      const uint32_t nb = A.nb[p0 + g];
+      const double scale = (double)nb / A.periods[p0 + g];
+      double* gv = hv + (size_t)g * stride;
+      double* gw = hw + (size_t)g * stride;
+      uint32_t* gn = hn + (size_t)g * stride;
+      // the first bin by a modulo, the next ones from the differences of
+      // floor(t * scale) when the cells ascend and span less than nb
+      double cell[8];
+#pragma unroll
+      for (uint32_t r = 0; r < 8; r++) cell[r] = floor(tt[r] * scale);
+      bool incr = cell[0] < 4294967296.0 && cell[cnt - 1] - cell[0] < (double)nb;
+#pragma unroll
+      for (uint32_t r = 1; r < 8; r++) incr = incr && (r >= cnt || cell[r] >= cell[r - 1]);
+      const uint32_t b0 = slot_of(tt[0], scale, nb);
+      uint32_t cur = b0;
+      double av = vv[0], aw = ww[0];
+      uint32_t ac = 1;
+#pragma unroll
+      for (uint32_t r = 1; r < 8; r++) {
+        if (r < cnt) {
+          uint32_t b;
+          if (incr) {
+            b = b0 + (uint32_t)(cell[r] - cell[0]);
+            b = b > nb ? b - nb : b;
+          } else {
+            b = slot_of(tt[r], scale, nb);
+          }
+          if (b == cur) {
+            av += vv[r];
+            aw += ww[r];
+            ac++;
+          } else {
+            flush(gv, gw, gn, cur, av, aw, ac);
+            cur = b;
+            av = vv[r];
+            aw = ww[r];
+            ac = 1;
+          }
+        }
+      }
+      flush(gv, gw, gn, cur, av, aw, ac);
     }
   }
   __syncthreads();
-  const uint32_t tn = hn[nb];
-  for (uint32_t g = 0; g < A.ng; g++) {
-    const double kf = floor(A.durations[g] * scale + 0.5);
-    uint32_t k = kf < 1.0 ? 1u : (uint32_t)kf;
-    if (k > nb) k = nb;
-    double best = -kInf;
-    for (uint32_t j = threadIdx.x; j < nb; j += kBlock) {
-      const uint32_t e = j + k;
-      double a, w;
-      uint32_t cnt;
-      if (e <= nb) {
-        a = (double)(hv[e] - hv[j]) * S.sv;
-        w = (double)(hw[e] - hw[j]) * S.sw;
-        cnt = hn[e] - hn[j];
-      } else {
-        a = (double)(hv[e - nb] + (hv[nb] - hv[j])) * S.sv;
-        w = (double)(hw[e - nb] + (hw[nb] - hw[j])) * S.sw;
-        cnt = hn[e - nb] + (hn[nb] - hn[j]);
+  // exclusive prefix of every (period, field): slot x = sum of bins < x
+  for (uint32_t task = wave; task < 3 * gcount; task += kBlock / 64) {
+    const uint32_t g = task / 3, f = task % 3;
+    const uint32_t len = A.nb[p0 + g] + 1;
+    if (f == 0) wave_inclusive(hv + (size_t)g * stride, len, lane);
+    if (f == 1) wave_inclusive(hw + (size_t)g * stride, len, lane);
+    if (f == 2) wave_inclusive(hn + (size_t)g * stride, len, lane);
+  }
+  __syncthreads();
+  // best SNR over phase per (period, duration): the phase with the largest
+  // key a|a| / q (monotone in a / sqrt(q); the smallest phase on ties) is
+  // found by wave reductions, then its SNR a / sqrt(q) is computed once;
+  // 64 pairs at a time
+  __shared__ double red_key[kBlock / 64][64];
+  __shared__ uint32_t red_j[kBlock / 64][64];
+  const uint32_t pairs = gcount * A.ng;
+  for (uint32_t q0 = 0; q0 < pairs; q0 += 64) {
+    const uint32_t q1 = min(q0 + 64, pairs);
+    for (uint32_t q = q0; q < q1; q++) {
+      const uint32_t g = q / A.ng, d = q % A.ng;
+      const uint32_t nb = A.nb[p0 + g];
+      const double scale = (double)nb / A.periods[p0 + g];
+      const double* gv = hv + (size_t)g * stride;
+      const double* gw = hw + (size_t)g * stride;
+      const uint32_t* gn = hn + (size_t)g * stride;
+      const uint32_t tn = gn[nb];
+      const uint32_t k = width_of(A.durations[d], scale, nb);
+      double best = -kInf;
+      uint32_t bj = 0xffffffffu;
+      for (uint32_t j = tid; j < nb; j += kBlock) {
+        double a, qq;
+        if (window(gv, gw, gn, nb, tn, j, k, S, &a, &qq)) {
+          const double key = a * fabs(a) / qq;
+          if (key > best) {
+            best = key;
+            bj = j;
+          }
+        }
       }
-      const double q = w - w * w / S.C;
-      if (cnt >= 3 && cnt + 3 <= tn && q > 0.0) {
-        const double snr = a / sqrt(q);
-        best = snr > best ? snr : best;
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double ok = __shfl_xor(best, off, 64);
+        const uint32_t oj = __shfl_xor(bj, off, 64);
+        if (ok > best || (ok == best && oj < bj)) {
+          best = ok;
+          bj = oj;
+        }
+      }
+      if (lane == 0) {
+        red_key[wave][q - q0] = best;
+        red_j[wave][q - q0] = bj;
       }
     }
-    red[threadIdx.x] = best;
     __syncthreads();
-    for (uint32_t s = kBlock / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-      __syncthreads();
+    if (tid < q1 - q0) {
+      double best = red_key[0][tid];
+      uint32_t bj = red_j[0][tid];
+      for (uint32_t w = 1; w < kBlock / 64; w++) {
+        const double ok = red_key[w][tid];
+        const uint32_t oj = red_j[w][tid];
+        if (ok > best || (ok == best && oj < bj)) {
+          best = ok;
+          bj = oj;
+        }
+      }
+      const uint32_t q = q0 + tid, g = q / A.ng, d = q % A.ng;
+      double m = -kInf;
+      if (bj != 0xffffffffu) {
+        const uint32_t nb = A.nb[p0 + g];
+        const double scale = (double)nb / A.periods[p0 + g];
+        double a, qq;
+        window(hv + (size_t)g * stride, hw + (size_t)g * stride, hn + (size_t)g * stride, nb,
+               hn[(size_t)g * stride + nb], bj, width_of(A.durations[d], scale, nb), S, &a,
+               &qq);
+        m = a / sqrt(qq);
+      }
+      const size_t at = ((size_t)star * A.np + p0 + g) * A.ng + d;
+      if (A.out) A.out[at] = m;
+      if (A.inputs) {  // the network input: clip(SNR, -5, 40) / 8, 0 when none
+        const double x = m > -kInf ? m : 0.0;
+        A.inputs[at] = (float)(fmin(fmax(x, -5.0), 40.0) / 8.0);
+      }
     }
-    if (threadIdx.x == 0) A.out[((size_t)star * A.np + pi) * A.ng + g] = red[0];
     __syncthreads();
   }
 }
 
 }  // namespace
 
-size_t feature_lds_bytes(uint32_t nb_max) { return (size_t)(nb_max + 1) * 20 + 16; }
+size_t feature_lds_bytes(uint32_t nb_max, uint32_t G) {
+  return (size_t)G * (nb_max + 1) * 20 + 16;
+}
 
-hipError_t feature_launch(const FeatureArgs& a, uint32_t nstars, uint32_t nb_max,
-                          hipStream_t s) {
-  const size_t lds = feature_lds_bytes(nb_max);
+hipError_t feature_launch(const FeatureArgs& a, uint32_t nstars, hipStream_t s) {
+  const size_t lds = feature_lds_bytes(a.nb_max, a.G);
   hipError_t e = hipFuncSetAttribute((const void*)coarse_features,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(coarse_features, dim3(nstars * a.np), dim3(kBlock), lds, s, a);
+  const uint32_t groups = (a.np + a.G - 1) / a.G;
+  hipLaunchKernelGGL(coarse_features, dim3(nstars * groups), dim3(kBlock), lds, s, a);
   return hipGetLastError();
 }
 

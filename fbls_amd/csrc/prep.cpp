@@ -40,6 +40,16 @@ void period_grid(double minimum, double maximum, double* out, size_t count) {
   out[count - 1] = maximum;
 }
 
+GridFormula::GridFormula(double mn, double mx, size_t n)
+    : minimum(mn), maximum(mx), a(std::log(mn)), b(std::log(mx)), count(n) {}
+
+double GridFormula::operator()(size_t i) const {
+  // period_grid: the endpoints are exact, the rest exp of the interpolation
+  if (i + 1 == count) return maximum;
+  if (i == 0) return minimum;
+  return std::exp(a + (b - a) * (double)i / (double)(count - 1));
+}
+
 size_t durations_days(const double* hours, size_t n, double* out) {
   std::vector<double> d(hours, hours + n);
   for (double& x : d) x /= 24.0;
@@ -74,6 +84,52 @@ std::vector<fbls_band> plan_bands(const double* grid, size_t count,
       b.nd = chosen;
       // scan.rs:377-379; the durations ascend so the minimum is sd[0]
       const double nbf = std::ceil((double)phase_oversample * grid[j - 1] / sd[0]);
+      b.nb = nbf < 8.0 ? 8u : (nbf > 4e9 ? 0xffffffffu : (uint32_t)nbf);
+      b.nb_coarse = (prescan_factor > 1 && b.nb / prescan_factor >= 8)
+                        ? b.nb / prescan_factor
+                        : 0u;
+      out.push_back(b);
+    }
+    i = j;
+    lo = hi;
+  }
+  return out;
+}
+
+std::vector<fbls_band> plan_bands(const GridFormula& grid, const double* sd, size_t nd,
+                                  uint32_t phase_oversample, uint32_t prescan_factor) {
+  // the same bands as plan_bands over the materialised grid: the grid
+  // ascends, so the first index at or above a bound is a binary search
+  std::vector<fbls_band> out;
+  const size_t count = grid.count;
+  if (count == 0 || nd == 0) return out;
+  auto first_at_least = [&](size_t from, double bound) {
+    size_t lo = from, hi = count;
+    while (lo < hi) {
+      const size_t mid = lo + (hi - lo) / 2;
+      if (grid(mid) < bound)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return lo;
+  };
+  double lo = grid(0);
+  const double last = grid(count - 1);
+  size_t i = 0;
+  while (lo <= last) {
+    const double hi = lo * 2.0;
+    i = first_at_least(i, lo);
+    const size_t j = first_at_least(i, hi);
+    size_t chosen = 0;
+    while (chosen < nd && sd[chosen] < lo / 4.0) chosen++;
+    if (j > i && chosen) {
+      fbls_band b;
+      b.lo = lo;
+      b.first = i;
+      b.count = j - i;
+      b.nd = chosen;
+      const double nbf = std::ceil((double)phase_oversample * grid(j - 1) / sd[0]);
       b.nb = nbf < 8.0 ? 8u : (nbf > 4e9 ? 0xffffffffu : (uint32_t)nbf);
       b.nb_coarse = (prescan_factor > 1 && b.nb / prescan_factor >= 8)
                         ? b.nb / prescan_factor

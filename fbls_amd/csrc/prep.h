@@ -21,6 +21,14 @@ int quantize(const double* x, size_t n, int32_t* q);
 size_t period_grid_count(double minimum, double maximum, double baseline,
                          double minimum_duration, double oversample);
 void period_grid(double minimum, double maximum, double* out, size_t count);
+// Element i of period_grid(minimum, maximum, ., count), evaluated alone.
+struct GridFormula {
+  double minimum = 0.0, maximum = 0.0, a = 0.0, b = 0.0;
+  size_t count = 0;
+  GridFormula() = default;
+  GridFormula(double mn, double mx, size_t n);
+  double operator()(size_t i) const;
+};
 
 // config.rs:177-182
 size_t durations_days(const double* hours, size_t n, double* out);
@@ -30,6 +38,9 @@ std::vector<fbls_band> plan_bands(const double* grid, size_t count,
                                   const double* sorted_durations, size_t nd,
                                   uint32_t phase_oversample,
                                   uint32_t prescan_factor);
+// plan_bands over a grid given by its formula: band edges by binary search.
+std::vector<fbls_band> plan_bands(const GridFormula& grid, const double* sd, size_t nd,
+                                  uint32_t phase_oversample, uint32_t prescan_factor);
 
 // curve.rs:541-569 for a prepared curve (flux relative to 0, errors given).
 int family_state(const double* time, const double* flux, const double* sigma,

@@ -13,10 +13,13 @@
 
 using namespace fbls::host;
 
-extern "C" int fbls_coarse_features(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
-                                    const double* periods, size_t np, const double* durations,
-                                    size_t ng, uint32_t nb_min, uint32_t nb_max, double* out) {
-  if (!c || (nfam && !fams) || !periods || !durations || !out) return fail(FBLS_E_ARG, "null argument");
+namespace {
+
+int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const double* periods,
+           size_t np, const double* durations, size_t ng, uint32_t nb_min, uint32_t nb_max,
+           double* out, float* inputs) {
+  if (!c || (nfam && !fams) || !periods || !durations || (!out && !inputs))
+    return fail(FBLS_E_ARG, "null argument");
   if (np == 0 || ng == 0 || nfam == 0) return FBLS_OK;
   if (nb_min < 2 || nb_max < nb_min || nb_max > 4096)
     return fail(FBLS_E_ARG, "phase bins must satisfy 2 <= nb_min <= nb_max <= 4096");
@@ -52,7 +55,7 @@ extern "C" int fbls_coarse_features(fbls_ctx* c, const fbls_family* const* fams,
   const size_t o_nb = o_per + ((np * 8 + 255) & ~(size_t)255);
   const size_t o_dur = o_nb + ((np * 4 + 255) & ~(size_t)255);
   const size_t o_out = o_dur + ((ng * 8 + 255) & ~(size_t)255);
-  const size_t out_bytes = nfam * np * ng * 8;
+  const size_t out_bytes = nfam * np * ng * (out ? 8 : 4);
   HIP_TRY(hipSetDevice(c->device));
   if (int st = c->dev_in.ensure(o_out + out_bytes)) return st;
   unsigned char* d = (unsigned char*)c->dev_in.ptr;
@@ -68,9 +71,33 @@ extern "C" int fbls_coarse_features(fbls_ctx* c, const fbls_family* const* fams,
   A.np = (uint32_t)np;
   A.ng = (uint32_t)ng;
   A.durations = (const double*)(d + o_dur);
-  A.out = (double*)(d + o_out);
-  HIP_TRY(fbls::feature_launch(A, (uint32_t)nfam, nb_max, s));
-  HIP_TRY(hipMemcpyAsync(out, d + o_out, out_bytes, hipMemcpyDeviceToHost, s));
+  A.out = out ? (double*)(d + o_out) : nullptr;
+  A.inputs = out ? nullptr : (float*)(d + o_out);
+  // periods per workgroup: as many as 24 KiB of LDS holds, at most 4 (more
+  // LDS per workgroup costs occupancy, which the fold needs more than reuse)
+  const uint32_t nb_top = *std::max_element(nb.begin(), nb.end());
+  A.nb_max = nb_top;
+  A.G = 4;
+  while (A.G > 1 && fbls::feature_lds_bytes(nb_top, A.G) > 24 * 1024) A.G--;
+  HIP_TRY(fbls::feature_launch(A, (uint32_t)nfam, s));
+  HIP_TRY(hipMemcpyAsync(out ? (void*)out : (void*)inputs, d + o_out, out_bytes,
+                         hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return FBLS_OK;
+}
+
+}  // namespace
+
+extern "C" int fbls_coarse_features(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
+                                    const double* periods, size_t np, const double* durations,
+                                    size_t ng, uint32_t nb_min, uint32_t nb_max, double* out) {
+  if (!out) return fail(FBLS_E_ARG, "null argument");
+  return coarse(c, fams, nfam, periods, np, durations, ng, nb_min, nb_max, out, nullptr);
+}
+
+extern "C" int fbls_coarse_inputs(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
+                                  const double* periods, size_t np, const double* durations,
+                                  size_t ng, uint32_t nb_min, uint32_t nb_max, float* out) {
+  if (!out) return fail(FBLS_E_ARG, "null argument");
+  return coarse(c, fams, nfam, periods, np, durations, ng, nb_min, nb_max, nullptr, out);
 }

@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import (SearchConfig, Cnn, DeviceReady, Gpu, coarse_features, family_state,
+from . import (SearchConfig, Cnn, DeviceReady, Gpu, coarse_inputs, family_state,
                signal_search_batch, synth_star)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -170,9 +170,9 @@ def chunk_data(gpu, states, pc: ProposalConfig, full=True, timed=False):
     gd = group_durations(cfg, pc.groups)
     centers = bin_centers(pc)
     if timed:
-        coarse_features(gpu, devs, centers, gd, pc.nb_min, pc.nb_max)
+        coarse_inputs(gpu, devs, centers, gd, pc.nb_min, pc.nb_max)
     t0 = time.perf_counter()
-    raw = coarse_features(gpu, devs, centers, gd, pc.nb_min, pc.nb_max)
+    x = coarse_inputs(gpu, devs, centers, gd, pc.nb_min, pc.nb_max)
     t_feat = time.perf_counter() - t0
     rows = stats = None
     t_full = 0.0
@@ -183,7 +183,7 @@ def chunk_data(gpu, states, pc: ProposalConfig, full=True, timed=False):
         t0 = time.perf_counter()
         rows, stats = signal_search_batch(gpu, devs, cfg.durations_days(), rhos, cfg)
         t_full = time.perf_counter() - t0
-    return devs, transform(raw), rows, stats, t_feat, t_full
+    return devs, x, rows, stats, t_feat, t_full
 
 
 # ---------------------------------------------------------------- proposal

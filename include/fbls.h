@@ -406,6 +406,12 @@ int fbls_coarse_features(fbls_ctx* ctx, const fbls_family* const* fams, size_t n
                          const double* periods, size_t np, const double* durations,
                          size_t ng, uint32_t nb_min, uint32_t nb_max, double* out);
 
+/* The network input of the same periodogram, computed on the device:
+ * clip(SNR, -5, 40) / 8 as float32, 0 where no phase qualifies. */
+int fbls_coarse_inputs(fbls_ctx* ctx, const fbls_family* const* fams, size_t nfam,
+                       const double* periods, size_t np, const double* durations,
+                       size_t ng, uint32_t nb_min, uint32_t nb_max, float* out);
+
 /* -- raw-mode preparation ----------------------------------------------- */
 
 /* Preparation options of the raw path (config.rs:11-16, defaults 10, 0.4,

@@ -47,7 +47,9 @@ void orc_coarse_features(const double* t, const double* v, const double* w, size
       double kf = floor(durations[g] * scale + 0.5);
       uint32_t k = kf < 1.0 ? 1u : (uint32_t)kf;
       if (k > nb) k = nb;
-      double best = -INFINITY;
+      /* the qualifying phase with the largest key a|a|/q (first on ties),
+       * then its SNR */
+      double best_key = -INFINITY, best = -INFINITY;
       for (uint32_t j = 0; j < nb; j++) {
         uint32_t e = j + k, cnt;
         double a, ww;
@@ -62,8 +64,11 @@ void orc_coarse_features(const double* t, const double* v, const double* w, size
         }
         double q = ww - ww * ww / C;
         if (cnt >= 3 && cnt + 3 <= cn[nb] && q > 0.0) {
-          double snr = a / sqrt(q);
-          if (snr > best) best = snr;
+          double key = a * fabs(a) / q;
+          if (key > best_key) {
+            best_key = key;
+            best = a / sqrt(q);
+          }
         }
       }
       out[i * ng + g] = best;

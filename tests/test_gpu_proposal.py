@@ -36,6 +36,9 @@ def test_coarse_features_bit_exact(gpu):
         want = O.coarse_features(s.t, vq, wq, s.baseline_norm, periods, gd, pc.nb_min,
                                  pc.nb_max)
         np.testing.assert_array_equal(got[k], want)
+    # the network input computed on the device is the host transform of them
+    inputs = F.coarse_inputs(gpu, devs, periods, gd, pc.nb_min, pc.nb_max)
+    np.testing.assert_array_equal(inputs, PR.transform(got))
 
 
 def test_keep_mask_rows_are_full_rows(gpu):
