@@ -124,7 +124,7 @@ def learned_pruned(gpu, res):
     ev = PR.EvalSet(gpu, [r[0] for r in res], [r[1] for r in res], pc)
     topk, tau = int(extra["topk"]), float(extra["tau"])
     fb = tuple(float(v) for v in np.atleast_1d(extra["fallback"]))
-    sw = ev.sweep(net, tau=tau, points=[(topk, fb)])
+    sw = ev.sweep(net, points=[(tau, topk, fb)])
     ev.close()
     r = sw["learned"][0]
     raw = sw["raw_snr_topk"][0]
@@ -141,7 +141,8 @@ def learned_pruned(gpu, res):
         "trial_evals_pruned": r["templates_pruned"],
         "kept_period_fraction": r["kept_period_fraction"],
         "fallback_stars": r["fallback_stars"],
-        "raw_snr_ranking_recall_same_top_k": raw["recall"],
+        "raw_snr_ranking_recall_same_kept_bins": raw["recall"],
+        "raw_snr_ranking_kept_period_fraction": raw["kept_period_fraction"],
     }
 
 

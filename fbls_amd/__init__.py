@@ -58,6 +58,7 @@ EXPORTS = (
     "fbls_autotune", "fbls_biweight_flatten", "fbls_prepare_families",
     "fbls_search_transits", "fbls_single_event_search", "fbls_read_inputs",
     "fbls_format_f64", "fbls_json_reformat", "fbls_cli", "fbls_coarse_inputs",
+    "fbls_cnn_set_fused", "fbls_signal_search_bins",
 )
 
 
@@ -291,6 +292,7 @@ def lib():
         L.fbls_cnn_get_params.argtypes = [_vp, _fp]
         L.fbls_cnn_set_params.argtypes = [_vp, _fp]
         L.fbls_cnn_forward.argtypes = [_vp, _fp, _u32, _u32, _fp]
+        L.fbls_cnn_set_fused.argtypes = [_vp, ctypes.c_int]
         L.fbls_cnn_loss_grad.argtypes = [_vp, _fp, _fp, _u32, _u32, _fp, _fp]
         L.fbls_cnn_train_step.argtypes = [_vp, _fp, _fp, _u32, _u32, ctypes.c_float,
                                           ctypes.c_float, _fp]
@@ -329,6 +331,9 @@ def lib():
             _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8)]
         L.fbls_coarse_features.argtypes = [_vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz,
                                            _u32, _u32, _P(_d)]
+        L.fbls_signal_search_bins.argtypes = [
+            _vp, _P(_vp), _sz, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
+            _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8), _sz, _P(_d)]
         L.fbls_coarse_inputs.argtypes = [_vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz,
                                            _u32, _u32, _P(ctypes.c_float)]
         L.fbls_family_state.argtypes = [
@@ -819,6 +824,41 @@ def signal_search_batch(gpu, devs, durations, rhos, cfg: SearchConfig,
             [s.as_dict() for s in stats])
 
 
+def signal_search_bins(gpu, devs, durations, rhos, cfg: SearchConfig, bin_keep, edges,
+                       rows_cap=None):
+    """Many stars, each restricted to the grid periods of its kept bins
+    (fbls_signal_search_bins; bin k is edges[k] <= P < edges[k + 1]).
+    Returns ([rows k_s x 5 per star], [stats per star])."""
+    durations = _f64(durations)
+    rho = np.ascontiguousarray(np.asarray(rhos, dtype=np.float64).reshape(
+        len(devs), len(durations)))
+    bk = np.ascontiguousarray(bin_keep, dtype=np.uint8).reshape(len(devs), -1)
+    edges = _f64(edges)
+    assert len(edges) == bk.shape[1] + 1
+    bank = make_bank(cfg, durations, rho[0])
+    scfg = SearchCfgC(cfg.period_min, cfg.period_max, cfg.period_oversample)
+    hs = (_vp * len(devs))(*[d._h.value for d in devs])
+    offs = np.zeros(len(devs) + 1, dtype=np.uintp)
+    stats = (ScanStats * len(devs))()
+    if rows_cap is None:
+        dmin = float(durations.min())
+        rows_cap = 0
+        for d in devs:
+            hi = min(cfg.period_max, d.t_last)
+            lo = max(cfg.period_min, 4.0 * dmin * (1.0 + 1e-9))
+            if hi > lo:
+                rows_cap += lib().fbls_period_grid_count(lo, hi, d.t_last, dmin,
+                                                         cfg.period_oversample)
+    buf, rows = _row_buffer(rows_cap)
+    _check(lib().fbls_signal_search_bins(
+        gpu._h, hs, len(devs), _dp(durations), _dp(rho), len(durations),
+        ctypes.byref(bank), ctypes.byref(scfg), rows, rows_cap,
+        offs.ctypes.data_as(_P(_sz)), stats, bk.ctypes.data_as(_P(ctypes.c_uint8)),
+        bk.shape[1], _dp(edges)))
+    return ([buf[int(offs[i]):int(offs[i + 1])] for i in range(len(devs))],
+            [s.as_dict() for s in stats])
+
+
 def pipeline_cfg(cfg: SearchConfig):
     """fbls_pipeline_cfg of a SearchConfig (arrays kept alive on it)."""
     d = cfg.durations_days()
@@ -990,6 +1030,11 @@ class Cnn:
         p = _f32(p)
         assert p.size == self.count
         _check(lib().fbls_cnn_set_params(self._h, _fptr(p)))
+
+    def set_fused(self, enable):
+        """Fused conv + GELU inference kernels (default) or the training
+        kernels; the logits are bit-identical."""
+        _check(lib().fbls_cnn_set_fused(self._h, 1 if enable else 0))
 
     def forward(self, x):
         x = _f32(x)

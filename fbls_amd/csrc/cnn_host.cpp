@@ -35,9 +35,11 @@ struct fbls_cnn {
   long long rows = 0;
   std::vector<DevBuf> Xcol, Y, H;
   DevBuf X0, z, y, terms, dz, dHa, dHb, dY, dYb, dXcol, part, scalar;
+  DevBuf Ha, Hb;  // inference activations (ping-pong)
+  int fused = 1;  // inference through conv_gelu when the shapes allow it
   ~fbls_cnn() {
     for (DevBuf* b : {&P, &PB, &G, &Mo, &Vo, &X0, &z, &y, &terms, &dz, &dHa, &dHb, &dY, &dYb,
-                      &dXcol, &part, &scalar})
+                      &dXcol, &part, &scalar, &Ha, &Hb})
       b->release();
     for (auto* v : {&Xcol, &Y, &H})
       for (DevBuf& b : *v) b.release();
@@ -82,6 +84,52 @@ int ensure_acts(fbls_cnn* n, long long rows) {
 }
 
 constexpr int kSplitChunk = 2048;  // rows of K per split of the weight-gradient GEMM
+
+// Inference: every layer as one fused implicit-GEMM conv + bias + GELU
+// launch (no im2col matrix, no fp32 pre-activations), then the head.
+int infer(fbls_cnn* n, const float* x, uint32_t B, uint32_t L) {
+  const long long rows = (long long)B * L;
+  const int Cin = (int)n->cfg.in_channels, W = (int)n->cfg.width, kw = (int)n->cfg.kernel;
+  int st;
+  if ((st = n->X0.ensure(rows * Cin * 6)) || (st = n->z.ensure(rows * 4)) ||
+      (st = n->Ha.ensure(rows * W * 2)) || (st = n->Hb.ensure(rows * W * 2)))
+    return st;
+  hipStream_t s = n->c->stream;
+  float* x32 = (float*)n->X0.ptr;
+  cn::bf16* x16 = (cn::bf16*)((unsigned char*)n->X0.ptr + rows * Cin * 4);
+  HIP_TRY(hipMemcpyAsync(x32, x, rows * Cin * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(cn::to_bf16(x32, x16, rows * Cin, s));
+  const float* P = (const float*)n->P.ptr;
+  const cn::bf16* PB = (const cn::bf16*)n->PB.ptr;
+  const cn::bf16* in = x16;
+  int C = Cin;
+  for (uint32_t l = 0; l < n->cfg.layers; l++) {
+    cn::ConvArgs a;
+    a.X = in;
+    a.W = PB + n->w_off[l];
+    a.bias = P + n->b_off[l];
+    a.H = (cn::bf16*)((l & 1) ? n->Hb.ptr : n->Ha.ptr);
+    a.B = (int)B;
+    a.L = (int)L;
+    a.cin = C;
+    a.kw = kw;
+    HIP_TRY(cn::conv_gelu(a, s));
+    in = a.H;
+    C = W;
+  }
+  HIP_TRY(cn::head_forward(in, P + n->head_w, P + n->head_b, (float*)n->z.ptr, rows, W, s));
+  return FBLS_OK;
+}
+
+// The fused path needs 64 channels, 8 or 64 inputs per layer and aligned
+// weight rows.
+bool fusable(const fbls_cnn* n) {
+  if (n->cfg.width != 64) return false;
+  if (n->cfg.in_channels != 8 && n->cfg.in_channels != 64) return false;
+  for (uint32_t l = 0; l < n->cfg.layers; l++)
+    if (n->w_off[l] % 8) return false;
+  return true;
+}
 
 // forward of B x L bins; x host fp32 (B, L, in_channels)
 int forward(fbls_cnn* n, const float* x, uint32_t B, uint32_t L) {
@@ -279,7 +327,7 @@ extern "C" int fbls_cnn_forward(fbls_cnn* n, const float* x, uint32_t B, uint32_
   if (int st = check_batch(n, x, B, L)) return st;
   if (!logits) return fail(FBLS_E_ARG, "null argument");
   HIP_TRY(hipSetDevice(n->c->device));
-  if (int st = forward(n, x, B, L)) return st;
+  if (int st = (n->fused && fusable(n)) ? infer(n, x, B, L) : forward(n, x, B, L)) return st;
   HIP_TRY(hipMemcpyAsync(logits, n->z.ptr, (size_t)B * L * 4, hipMemcpyDeviceToHost, n->c->stream));
   HIP_TRY(hipStreamSynchronize(n->c->stream));
   return FBLS_OK;
@@ -323,5 +371,11 @@ extern "C" int fbls_cnn_train_step(fbls_cnn* n, const float* x, const float* y, 
   HIP_TRY(cn::adamw(P + n->head_b, PB + n->head_b, G + n->head_b, M + n->head_b, V + n->head_b,
                     1, lr, 0.9f, 0.999f, 1e-8f, 0.0f, n->step, s));
   HIP_TRY(hipStreamSynchronize(s));
+  return FBLS_OK;
+}
+
+extern "C" int fbls_cnn_set_fused(fbls_cnn* n, int enable) {
+  if (!n) return fail(FBLS_E_ARG, "null argument");
+  n->fused = enable ? 1 : 0;
   return FBLS_OK;
 }

@@ -26,6 +26,20 @@ hipError_t gemm(const GemmArgs& g, int splits, hipStream_t s);
 hipError_t reduce_splits(const float* part, float* out, int splits, long long mn,
                          hipStream_t s);
 
+// Inference Conv1d (channels-last, zero padding, 64 output channels) with
+// bias and GELU fused: H[b][l][n] = bf16(GELU(bias[n] + sum over (k, c) of
+// W[n][k cin + c] X[b][l + k - kw/2][c])), bit-identical to im2col + gemm +
+// bias_gelu.  cin is 8 or 64.
+struct ConvArgs {
+  const bf16* X;     // B x L x cin
+  const bf16* W;     // 64 x (kw cin)
+  const float* bias; // 64
+  bf16* H;           // B x L x 64
+  int B, L, cin, kw;
+};
+size_t conv_gelu_lds(int cin, int kw);
+hipError_t conv_gelu(const ConvArgs& a, hipStream_t s);
+
 // Xcol[(b L + l)][k C + c] = X[b][l + k - kw/2][c] (0 outside), bf16
 hipError_t im2col(const bf16* X, bf16* Xcol, int B, int L, int C, int kw, hipStream_t s);
 // dX[b][l][c] = sum over k of dXcol[(b L + l - k + kw/2)][k C + c] (fp32)

@@ -75,6 +75,89 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       }
 }
 
+__device__ __forceinline__ float gelu(float x) {
+  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+// Conv1d + bias + GELU for inference, as one implicit GEMM: a workgroup
+// owns 128 consecutive bins of one sample and all 64 output channels.  The
+// input rows the 128 bins need (with the kw - 1 halo, zero outside the
+// sample) and the layer's bf16 weights W[n][k] (k = tap * CIN + c) are
+// staged in LDS; the MFMA A operand of output row r and k group kq..kq+7 is
+// X[r + tap][c..c+7] (eight channels of one tap, CIN % 8 == 0), so no
+// im2col matrix is formed.  The k steps, operand values and MFMA order are
+// those of gemm_kernel over the im2col matrix, so H is bit-identical to the
+// im2col + gemm + bias_gelu path.
+constexpr int kConvRows = 128, kConvN = 64;
+
+template <int CIN>
+__global__ __launch_bounds__(256) void conv_gelu_kernel(ConvArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  constexpr int XS = CIN + 8;  // padded LDS row of the input, in bf16
+  const int kw = a.kw, half = kw / 2, K = kw * CIN, KP = (K + 31) / 32 * 32;
+  const int WS = KP + 8;       // padded LDS row of the weights, in bf16
+  bf16* Xs = (bf16*)smem;                                   // [kConvRows + kw - 1][XS]
+  bf16* Ws = Xs + (size_t)(kConvRows + kw - 1) * XS;        // [kConvN][WS]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tiles = (a.L + kConvRows - 1) / kConvRows;
+  const int b = blockIdx.x / tiles, l0 = (blockIdx.x % tiles) * kConvRows;
+  // weights: 16-byte pieces, zero beyond K
+  for (int e = tid; e < kConvN * (KP / 8); e += 256) {
+    const int n = e / (KP / 8), k8 = (e % (KP / 8)) * 8;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (k8 < K) v = *(const uint4*)(a.W + (size_t)n * K + k8);
+    *(uint4*)(Ws + (size_t)n * WS + k8) = v;
+  }
+  // input rows l0 - half .. l0 + kConvRows + half - 1 of sample b
+  const int prow = kConvRows + kw - 1;
+  for (int e = tid; e < prow * (CIN / 8); e += 256) {
+    const int p = e / (CIN / 8), c8 = (e % (CIN / 8)) * 8;
+    const int l = l0 - half + p;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (l >= 0 && l < a.L) v = *(const uint4*)(a.X + ((size_t)b * a.L + l) * CIN + c8);
+    *(uint4*)(Xs + (size_t)p * XS + c8) = v;
+  }
+  __syncthreads();
+  const int wm = wave * 32;  // this wave's 32 rows, all 64 columns
+  f32x4 acc[2][4];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int kq = 8 * (lane >> 4), r16 = lane & 15;
+  for (int k0 = 0; k0 < KP; k0 += 32) {
+    const int kk = k0 + kq;
+    bf16x8 av[2], bv[4];
+    if (kk < K) {
+      const int tap = kk / CIN, c = kk % CIN;
+      for (int i = 0; i < 2; i++)
+        av[i] = *(const bf16x8*)(Xs + (size_t)(wm + 16 * i + r16 + tap) * XS + c);
+    } else {
+      for (int i = 0; i < 2; i++) {
+        const uint4 z = make_uint4(0, 0, 0, 0);
+        av[i] = *(const bf16x8*)&z;
+      }
+    }
+    for (int j = 0; j < 4; j++) bv[j] = *(const bf16x8*)(Ws + (size_t)(16 * j + r16) * WS + kk);
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 4; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+  }
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 4; j++)
+      for (int r = 0; r < 4; r++) {
+        const int l = l0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int n = 16 * j + r16;
+        if (l < a.L) {
+          const float y = acc[i][j][r] + a.bias[n];
+          a.H[((size_t)b * a.L + l) * kConvN + n] = f2bf(gelu(y));
+        }
+      }
+}
+
 __global__ void reduce_kernel(const float* part, float* out, int splits, long long mn) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= mn) return;
@@ -109,15 +192,6 @@ __global__ void col2im_kernel(const float* dXcol, float* dX, int B, int L, int C
     if (r >= 0 && r < L) s += dXcol[((long long)b * L + r) * kc + (long long)k * C + c];
   }
   dX[i] = s;
-}
-
-__device__ __forceinline__ float gelu(float x) {
-  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
-}
-__device__ __forceinline__ float gelu_grad(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
-  return cdf + x * pdf;
 }
 
 __global__ void bias_gelu_kernel(float* Y, const float* bias, bf16* H, long long M, int N) {
@@ -244,6 +318,26 @@ hipError_t gemm(const GemmArgs& g, int splits, hipStream_t s) {
 
 hipError_t reduce_splits(const float* part, float* out, int splits, long long mn, hipStream_t s) {
   hipLaunchKernelGGL(reduce_kernel, dim3(blocks(mn)), dim3(256), 0, s, part, out, splits, mn);
+  return hipGetLastError();
+}
+
+size_t conv_gelu_lds(int cin, int kw) {
+  const int K = kw * cin, KP = (K + 31) / 32 * 32;
+  return ((size_t)(kConvRows + kw - 1) * (cin + 8) + (size_t)kConvN * (KP + 8)) * 2;
+}
+
+hipError_t conv_gelu(const ConvArgs& a, hipStream_t s) {
+  const int tiles = (a.L + kConvRows - 1) / kConvRows;
+  const size_t lds = conv_gelu_lds(a.cin, a.kw);
+  const void* fn = a.cin == 8 ? (const void*)conv_gelu_kernel<8>
+                              : (const void*)conv_gelu_kernel<64>;
+  if (a.cin != 8 && a.cin != 64) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  if (a.cin == 8)
+    hipLaunchKernelGGL(conv_gelu_kernel<8>, dim3(a.B * tiles), dim3(256), lds, s, a);
+  else
+    hipLaunchKernelGGL(conv_gelu_kernel<64>, dim3(a.B * tiles), dim3(256), lds, s, a);
   return hipGetLastError();
 }
 

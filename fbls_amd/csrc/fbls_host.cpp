@@ -905,6 +905,31 @@ void apply_keep(SearchPlan& p, const uint8_t* keep) {
     }
 }
 
+// Restricts each band to the grid periods inside the kept bins (edges[k] <=
+// P < edges[k + 1]); index ranges by binary search on the grid formula.
+void apply_bins(SearchPlan& p, const uint8_t* bin_keep, size_t nbins, const double* edges) {
+  p.kept.assign(p.bands.size(), {});
+  auto first_at_least = [&](size_t lo, size_t hi, double bound) {
+    while (lo < hi) {
+      const size_t mid = lo + (hi - lo) / 2;
+      if (p.formula(mid) < bound)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return lo;
+  };
+  for (size_t b = 0; b < p.bands.size(); b++) {
+    const size_t first = p.bands[b].first, end = first + p.bands[b].count;
+    for (size_t k = 0; k < nbins; k++) {
+      if (!bin_keep[k]) continue;
+      const size_t i0 = first_at_least(first, end, edges[k]);
+      const size_t i1 = first_at_least(i0, end, edges[k + 1]);
+      for (size_t i = i0; i < i1; i++) p.kept[b].push_back(p.formula(i));
+    }
+  }
+}
+
 int plan_search(const fbls_family* f, const double* durations, const double* rho,
                 size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
                 SearchPlan* p, bool lazy = false) {
@@ -948,11 +973,22 @@ int plan_search(const fbls_family* f, const double* durations, const double* rho
   return FBLS_OK;
 }
 
-// Plans stars [s0, s1) on up to `threads` host threads; keep (may be null)
-// is the batch's flat mask and keep_at[s - s0] each star's offset in it.
+// A restriction of the batch: a per-period mask (keep, offsets keep_at per
+// star) or per-star bins (bin_keep, nbins, edges); both null for the full
+// search.
+struct Restriction {
+  const uint8_t* keep = nullptr;
+  const uint8_t* bin_keep = nullptr;
+  size_t nbins = 0;
+  const double* edges = nullptr;
+  bool any() const { return keep || bin_keep; }
+};
+
+// Plans stars [s0, s1) on up to 16 host threads under the restriction
+// (keep_at[s - s0] is star s's offset in a per-period mask).
 int plan_stars(const fbls_family* const* fams, size_t s0, size_t s1, const double* durations,
                const double* rho, size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
-               const uint8_t* keep, const std::vector<size_t>& keep_at,
+               const Restriction& rs, const std::vector<size_t>& keep_at,
                std::vector<SearchPlan>* plans) {
   const size_t m = s1 - s0;
   plans->assign(m, SearchPlan());
@@ -967,12 +1003,13 @@ int plan_stars(const fbls_family* const* fams, size_t s0, size_t s1, const doubl
         return;
       }
       status[k] = plan_search(fams[s], durations, rho + s * nd, nd, bank, cfg, &(*plans)[k],
-                              keep != nullptr);
+                              rs.any());
       if (status[k]) {
         message[k] = fbls_last_error();
         return;
       }
-      if (keep) apply_keep((*plans)[k], keep + keep_at[k]);
+      if (rs.keep) apply_keep((*plans)[k], rs.keep + keep_at[k]);
+      if (rs.bin_keep) apply_bins((*plans)[k], rs.bin_keep + s * rs.nbins, rs.nbins, rs.edges);
     }
   };
   const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
@@ -1070,14 +1107,12 @@ extern "C" int fbls_signal_search(fbls_ctx* c, const fbls_family* f,
 // Many stars (one family each) through the same two-launch band loop: the
 // coarse passes of every band of every star in a chunk share one launch,
 // their fine passes a second.  Chunks hold at most kBatchPeriods periods.
-extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* fams,
-                                        size_t nfam, const double* durations,
-                                        const double* rho, size_t nd,
-                                        const fbls_bank* bank,
-                                        const fbls_search_cfg* cfg, fbls_row* rows,
-                                        size_t rows_cap, size_t* row_offsets,
-                                        fbls_scan_stats* star_stats,
-                                        const uint8_t* keep) {
+namespace {
+
+int batch_search(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
+                 const double* durations, const double* rho, size_t nd, const fbls_bank* bank,
+                 const fbls_search_cfg* cfg, fbls_row* rows, size_t rows_cap,
+                 size_t* row_offsets, fbls_scan_stats* star_stats, const Restriction& rs) {
   if (!c || (nfam && !fams) || !bank || !cfg || !row_offsets || (nd && (!durations || !rho)))
     return fail(FBLS_E_ARG, "null argument");
   constexpr size_t kBatchPeriods = size_t(1) << 21;
@@ -1102,6 +1137,21 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
                                                cfg->period_oversample);
     }
   }
+  // a restricted search plans every star first (its grids are never
+  // materialised) and chunks by the periods actually scanned, so a pruned
+  // batch needs few launches; the full search plans chunk by chunk
+  std::vector<SearchPlan> all;
+  std::vector<size_t> scanned = grid_size;
+  if (rs.any()) {
+    std::vector<size_t> off(nfam, 0);
+    for (size_t s = 1; s < nfam; s++) off[s] = off[s - 1] + grid_size[s - 1];
+    if (int st = plan_stars(fams, 0, nfam, durations, rho, nd, bank, cfg, rs, off, &all))
+      return st;
+    for (size_t s = 0; s < nfam; s++) {
+      scanned[s] = 0;
+      for (const auto& k : all[s].kept) scanned[s] += k.size();
+    }
+  }
   while (s0 < nfam) {
     // one chunk of stars
     std::vector<SearchPlan> plans;
@@ -1112,11 +1162,15 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
     while (s1 < nfam && (s1 == s0 || periods < kBatchPeriods)) {
       keep_off.push_back(keep_at);
       keep_at += grid_size[s1];
-      periods += grid_size[s1];
+      periods += scanned[s1];
       s1++;
     }
-    if (int st = plan_stars(fams, s0, s1, durations, rho, nd, bank, cfg, keep, keep_off, &plans))
+    if (rs.any()) {
+      for (size_t k = s0; k < s1; k++) plans.push_back(std::move(all[k]));
+    } else if (int st = plan_stars(fams, s0, s1, durations, rho, nd, bank, cfg, rs, keep_off,
+                                   &plans)) {
       return st;
+    }
     for (size_t k = 0; k < plans.size(); k++)
       if (plans[k].count != grid_size[s0 + k])
         return fail(FBLS_E_ARG, "grid size of star %zu changed while planning", s0 + k);
@@ -1169,6 +1223,39 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
   if (needed > rows_cap)
     return fail(FBLS_E_ARG, "rows capacity %zu < %zu rows (offsets filled)", rows_cap, needed);
   return FBLS_OK;
+}
+
+}  // namespace
+
+extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* fams,
+                                        size_t nfam, const double* durations,
+                                        const double* rho, size_t nd,
+                                        const fbls_bank* bank,
+                                        const fbls_search_cfg* cfg, fbls_row* rows,
+                                        size_t rows_cap, size_t* row_offsets,
+                                        fbls_scan_stats* star_stats,
+                                        const uint8_t* keep) {
+  Restriction rs;
+  rs.keep = keep;
+  return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
+                      star_stats, rs);
+}
+
+extern "C" int fbls_signal_search_bins(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
+                                       const double* durations, const double* rho, size_t nd,
+                                       const fbls_bank* bank, const fbls_search_cfg* cfg,
+                                       fbls_row* rows, size_t rows_cap, size_t* row_offsets,
+                                       fbls_scan_stats* star_stats, const uint8_t* bin_keep,
+                                       size_t nbins, const double* edges) {
+  if (!bin_keep || !edges || nbins == 0) return fail(FBLS_E_ARG, "null argument");
+  for (size_t k = 0; k < nbins; k++)
+    if (!(edges[k + 1] > edges[k])) return fail(FBLS_E_ARG, "bin edges must ascend");
+  Restriction rs;
+  rs.bin_keep = bin_keep;
+  rs.nbins = nbins;
+  rs.edges = edges;
+  return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
+                      star_stats, rs);
 }
 
 // ------------------------------------------------------------ host helpers

@@ -28,7 +28,7 @@ import time
 import numpy as np
 
 from . import (SearchConfig, Cnn, DeviceReady, Gpu, coarse_inputs, family_state,
-               signal_search_batch, synth_star)
+               signal_search_batch, signal_search_bins, synth_star)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "data", "proposal.npz")
@@ -69,6 +69,29 @@ def bin_of(periods, pc: ProposalConfig):
     lo, hi = math.log(pc.period_min), math.log(pc.period_max)
     b = np.floor((np.log(np.asarray(periods)) - lo) / (hi - lo) * pc.bins)
     return np.clip(b, 0, pc.bins - 1).astype(np.int64)
+
+
+def bin_edges(pc: ProposalConfig):
+    """Period edges of the bins (bin k: edges[k] <= P < edges[k + 1]); the
+    outer edges are open, as bin_of clips periods outside the range."""
+    lo, hi = math.log(pc.period_min), math.log(pc.period_max)
+    e = np.exp(lo + (hi - lo) * np.arange(pc.bins + 1) / pc.bins)
+    e[0], e[-1] = 0.0, np.inf
+    return e
+
+
+def keep_bins(probs, pc: ProposalConfig, tau, topk):
+    """Per star: bins with p >= tau plus the topk most probable, dilated by
+    keep_dilate bins on each side (vectorised over stars)."""
+    sel = probs >= tau
+    if topk:
+        idx = np.argpartition(-probs, topk - 1, axis=1)[:, :topk]
+        np.put_along_axis(sel, idx, True, axis=1)
+    out = sel.copy()
+    for o in range(1, pc.keep_dilate + 1):
+        out[:, o:] |= sel[:, :-o]
+        out[:, :-o] |= sel[:, o:]
+    return out
 
 
 def transform(raw):
@@ -212,11 +235,21 @@ def sigmoid(z):
     return 1.0 / (1.0 + np.exp(-z))
 
 
-def predict(net, x, batch=32):
-    out = []
+def predict(net, x, batch=512):
+    """Bin probabilities (float32) of the network for inputs x."""
+    out = np.empty(x.shape[:2], dtype=np.float32)
+    with np.errstate(over="ignore"):  # exp overflow is p = 0
+        _predict_into(net, x, batch, out)
+    return out
+
+
+def _predict_into(net, x, batch, out):
     for i in range(0, len(x), batch):
-        out.append(sigmoid(net.forward(x[i:i + batch]).astype(np.float64)))
-    return np.concatenate(out)
+        z = net.forward(x[i:i + batch])
+        np.negative(z, out=z)
+        np.exp(z, out=z)
+        z += 1.0
+        np.reciprocal(z, out=out[i:i + batch])
 
 
 def train(net, x, y, epochs=8, batch=32, lr=2e-3, weight_decay=1e-4, seed=0, log=None):
@@ -239,10 +272,11 @@ def recovered(best, injected):
                      for (p, _), inj in zip(best, injected)])
 
 
-# operating points: (top-k bins, fallback band): a star whose largest bin
-# probability lies in [lo, hi) is uncertain and gets the full scan
-POINTS = [(k, fb) for k in (16, 32, 64, 128)
-          for fb in ((0.0, 0.0), (0.05, 0.5), (0.02, 0.7), (0.01, 0.9))]
+# operating points: (tau, top-k bins, fallback band): a star keeps its bins
+# with p >= tau plus its top-k bins; a star whose largest bin probability
+# lies in [lo, hi) is uncertain and gets the full scan
+POINTS = [(tau, k, fb) for tau in (0.05, 0.1, 0.15, 0.2) for k in (0, 4, 8)
+          for fb in ((0.0, 0.0), (0.02, 0.1))]
 
 
 class EvalSet:
@@ -257,21 +291,25 @@ class EvalSet:
         self.best_full = best_rows(rows, self.cfg.threshold)
         self.templates_full = int(sum(s["templates_scanned"] for s in stats))
         self.grids = grids_of(self.devs, self.cfg)
+        self.edges = bin_edges(pc)
         self.rf = recovered(self.best_full, injected)
 
     def pruned(self, probs, t_net, tau, topk, fallback):
+        """The pruned search from the probabilities: bin selection (with the
+        fallback stars searched in full) and the bin-restricted batch search,
+        both timed."""
+        t0 = time.perf_counter()
         conf = probs.max(axis=1)
         lo, hi = fallback
         fall = (conf >= lo) & (conf < hi)
-        keeps = keep_masks(probs, self.grids, self.pc, tau, topk)
-        for s in np.nonzero(fall)[0]:
-            keeps[s] = np.ones(len(self.grids[s]), dtype=bool)
-        t0 = time.perf_counter()
-        prow, pstats = signal_search_batch(self.gpu, self.devs, self.cfg.durations_days(),
-                                           [s.rho for s in self.states], self.cfg, keep=keeps)
+        bins = keep_bins(probs, self.pc, tau, topk)
+        bins[fall] = True
+        prow, pstats = signal_search_bins(self.gpu, self.devs, self.cfg.durations_days(),
+                                          [s.rho for s in self.states], self.cfg, bins,
+                                          self.edges)
         t_pruned = time.perf_counter() - t0
         rp = recovered(best_rows(prow, self.cfg.threshold), self.injected)
-        kept = sum(int(k.sum()) for k in keeps)
+        kept = sum(int(st["periods"]) for st in pstats)
         total = sum(len(g) for g in self.grids)
         n = len(self.states)
         return {
@@ -291,9 +329,11 @@ class EvalSet:
             "stars_per_hour_pruned": n * 3600.0 / (self.t_feat + t_net + t_pruned),
         }
 
-    def sweep(self, net, tau=0.5, points=POINTS):
-        """Every operating point, for the learned proposal and for the raw
-        coarse SNR ranked the same way (the method without the network)."""
+    def sweep(self, net, points=POINTS):
+        """Every operating point for the learned proposal, and beside each
+        the method without the network: every star's top bins by raw coarse
+        SNR, as many per star as the learned rule keeps on average."""
+        predict(net, self.x[:8])  # first-call set-up outside the timing
         t0 = time.perf_counter()
         probs = predict(net, self.x)
         t_net = time.perf_counter() - t0
@@ -301,11 +341,14 @@ class EvalSet:
         raw = raw / max(float(raw.max()), 1e-9)
         out = {"learned": [], "raw_snr_topk": []}
         raw_at = {}
-        for topk, fb in points:
-            out["learned"].append(self.pruned(probs, t_net, tau, topk, fb))
-            if topk not in raw_at:  # no fallback for the raw ranking
-                raw_at[topk] = self.pruned(raw, 0.0, 2.0, topk, (0.0, 0.0))
-            out["raw_snr_topk"].append(raw_at[topk])
+        for tau, topk, fb in points:
+            r = self.pruned(probs, t_net, tau, topk, fb)
+            out["learned"].append(r)
+            sel = (probs >= tau).sum(axis=1)
+            k_raw = max(1, int(round(float(np.maximum(sel, topk).mean()))))
+            if k_raw not in raw_at:  # no threshold and no fallback
+                raw_at[k_raw] = self.pruned(raw, 0.0, 2.0, k_raw, (0.0, 0.0))
+            out["raw_snr_topk"].append(raw_at[k_raw])
         return out
 
     def close(self):
@@ -351,6 +394,8 @@ def main():
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--report", default=None)
     ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--reselect", action="store_true",
+                    help="keep the saved weights; redo the operating point and report")
     args = ap.parse_args()
     pc = ProposalConfig()
     cfg = search_config(pc)
@@ -373,7 +418,15 @@ def main():
     xs, ys = [], []
     seed_train, seed_val, seed_test = 4_000_000, 4_100_000, 4_200_000
     t0 = time.perf_counter()
-    for c0 in range(0, args.train, args.chunk):
+    if args.reselect:
+        net, _, _ = load(gpu)
+        prior_path = os.path.join(HERE, "data", "proposal_report.json")
+        prior = json.load(open(prior_path)) if os.path.exists(prior_path) else {}
+        t_train = float(prior.get("train_seconds", 0.0))
+        losses = [float(prior.get("final_loss", 0.0))]
+        args.train = int(prior.get("train_stars", args.train))
+        args.epochs = int(prior.get("epochs", args.epochs))
+    for c0 in range(0, 0 if args.reselect else args.train, args.chunk):
         idx = range(c0, min(args.train, c0 + args.chunk))
         states, _ = states_of(idx, seed_train)
         devs, x, rows, _, _, _ = chunk_data(gpu, states, pc)
@@ -382,15 +435,16 @@ def main():
         for dv in devs:
             dv.close()
         log(f"train stars {idx.stop} ({time.perf_counter() - t0:.1f} s)")
-    x = np.concatenate(xs)
-    y = np.concatenate(ys)
-    log(f"positives {int((y.sum(axis=1) > 0).sum())} of {len(y)} stars")
-    net = Cnn(gpu, in_channels=pc.groups, width=pc.width, layers=pc.layers, kernel=pc.kernel,
-              pos_weight=pc.pos_weight, seed=7)
-    t1 = time.perf_counter()
-    losses = train(net, x, y, epochs=args.epochs, log=log)
-    t_train = time.perf_counter() - t1
-    log(f"trained in {t_train:.1f} s")
+    if not args.reselect:
+        x = np.concatenate(xs)
+        y = np.concatenate(ys)
+        log(f"positives {int((y.sum(axis=1) > 0).sum())} of {len(y)} stars")
+        net = Cnn(gpu, in_channels=pc.groups, width=pc.width, layers=pc.layers,
+                  kernel=pc.kernel, pos_weight=pc.pos_weight, seed=7)
+        t1 = time.perf_counter()
+        losses = train(net, x, y, epochs=args.epochs, log=log)
+        t_train = time.perf_counter() - t1
+        log(f"trained in {t_train:.1f} s")
     # operating point: the fastest reaching 99.5 % recall on validation stars
     vs, vi = states_of(range(args.validate), seed_val)
     val = EvalSet(gpu, vs, vi, pc)
@@ -399,14 +453,14 @@ def main():
     point = fastest(vsweep["learned"], 0.99) or max(vsweep["learned"],
                                                    key=lambda r: r["recall"])
     topk, fallback, tau = point["topk"], point["fallback"], point["tau"]
-    log(f"operating point top-k {topk} fallback {fallback}: validation {point}")
+    log(f"operating point tau {tau} top-k {topk} fallback {fallback}: validation {point}")
     hs, hi = states_of(range(args.heldout), seed_test)
     pool.shutdown()
     learned, raw = [], []
     for c0 in range(0, len(hs), args.chunk):
         ev = EvalSet(gpu, hs[c0:c0 + args.chunk], hi[c0:c0 + args.chunk], pc)
-        sw = ev.sweep(net, tau=tau, points=[(topk, fallback)] + [p for p in POINTS
-                                                               if p != (topk, fallback)])
+        op = (tau, topk, tuple(fallback))
+        sw = ev.sweep(net, points=[op] + [p for p in POINTS if p != op])
         learned.append(sw["learned"])
         raw.append(sw["raw_snr_topk"])
         ev.close()
@@ -418,6 +472,8 @@ def main():
         for j in range(len(chunks[0])):
             rs = [c[j] for c in chunks]
             m = {k: rs[0][k] for k in ("topk", "fallback", "tau")}
+            m["kept_period_fraction_weighted"] = float(
+                sum(r["kept_period_fraction"] * r["stars"] for r in rs) / sum(r["stars"] for r in rs))
             for k in ("stars", "full_recovered", "pruned_recovered", "fallback_stars",
                       "templates_full", "templates_pruned"):
                 m[k] = int(sum(r[k] for r in rs))

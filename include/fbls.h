@@ -206,6 +206,20 @@ int fbls_signal_search_batch(fbls_ctx* ctx, const fbls_family* const* fams,
                              size_t* row_offsets, fbls_scan_stats* star_stats,
                              const uint8_t* keep);
 
+/* fbls_signal_search_batch restricted by period bins instead of a
+ * per-period mask: bin k covers the periods edges[k] <= P < edges[k + 1]
+ * (edges ascending, nbins + 1 values) and star s scans only the grid
+ * periods of its bins with bin_keep[s * nbins + k] set, each exactly as in
+ * the full search.  Only those periods are evaluated (the bands come from a
+ * binary search on the grid), so the host cost follows the kept periods.
+ * No counterpart in the reference: the learned proposal's pruned search. */
+int fbls_signal_search_bins(fbls_ctx* ctx, const fbls_family* const* fams, size_t nfam,
+                            const double* durations, const double* rho, size_t nd,
+                            const fbls_bank* bank, const fbls_search_cfg* cfg,
+                            fbls_row* rows, size_t rows_cap, size_t* row_offsets,
+                            fbls_scan_stats* star_stats, const uint8_t* bin_keep,
+                            size_t nbins, const double* edges);
+
 /* -- evidence (model comparison) ---------------------------------------- */
 
 /* SearchConfig fields read by `compare_search` (config.rs:33-42,53-85).
@@ -390,6 +404,12 @@ int fbls_cnn_get_params(fbls_cnn* net, float* out);
 int fbls_cnn_set_params(fbls_cnn* net, const float* in);
 /* logits[b][l] of x[b][l][c] */
 int fbls_cnn_forward(fbls_cnn* net, const float* x, uint32_t B, uint32_t L, float* logits);
+
+/* Inference path of fbls_cnn_forward: 1 (default) runs each layer as one
+ * fused implicit-GEMM conv + bias + GELU kernel when the shapes allow it
+ * (64 channels, 8 or 64 inputs), 0 the im2col + GEMM + epilogue kernels of
+ * training.  Both give bit-identical logits. */
+int fbls_cnn_set_fused(fbls_cnn* net, int enable);
 /* Mean loss over the B x L bins and its gradient (flat, may be NULL). */
 int fbls_cnn_loss_grad(fbls_cnn* net, const float* x, const float* y, uint32_t B, uint32_t L,
                        float* loss, float* grads);

@@ -79,3 +79,19 @@ def test_training_reduces_loss(gpu):
         last = net.train_step(x, y, lr=2e-3)
     print("[cnn] loss", first, "->", last, flush=True)
     assert last < 0.3 * first
+
+
+@pytest.mark.parametrize("L", [2048, 200])
+def test_fused_inference_is_bit_identical(gpu, L):
+    # the proposal network's shape: 8 inputs, 64 channels, 6 layers, kernel 7;
+    # L = 200 leaves a partial 128-bin tile at each sample's end
+    print(f"[cnn] fused inference L={L}", flush=True)
+    net = F.Cnn(gpu, in_channels=8, width=64, layers=6, kernel=7, seed=9)
+    x, _ = _data(3, L, 8, 13)
+    fused = net.forward(x)
+    net.set_fused(False)
+    plain = net.forward(x)
+    net.set_fused(True)
+    np.testing.assert_array_equal(fused, plain)
+    orc = C.Net(8, 64, 6, 7, 20.0)
+    assert _rel(fused, orc.forward(net.params().astype(np.float64), x, bf=True)) < 1e-2

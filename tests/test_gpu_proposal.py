@@ -59,6 +59,20 @@ def test_keep_mask_rows_are_full_rows(gpu):
         want = np.array([r for r in rows[k] if r[0] in kept]).reshape(-1, 5)
         np.testing.assert_array_equal(prow[k], want)
         assert pstats[k]["periods"] == int(keeps[k].sum())
+    # bin restriction: the rows of the grid periods inside the kept bins
+    edges = PR.bin_edges(pc)
+    bins = rng.uniform(size=(len(states), pc.bins)) < 0.05
+    bins[1] = True  # one star keeps every bin: its full search
+    brow, bstats = F.signal_search_bins(gpu, devs, d, [s.rho for s in states], cfg, bins,
+                                        edges)
+    for k in range(len(states)):
+        b = np.searchsorted(edges, grids[k], side="right") - 1
+        inside = bins[k][b]
+        kept = set(grids[k][inside].tolist())
+        want = np.array([r for r in rows[k] if r[0] in kept]).reshape(-1, 5)
+        np.testing.assert_array_equal(brow[k], want)
+        assert bstats[k]["periods"] == int(inside.sum())
+    np.testing.assert_array_equal(brow[1], rows[1])
 
 
 @pytest.mark.skipif(not os.path.exists(PR.DEFAULT_WEIGHTS), reason="no trained proposal")
@@ -69,7 +83,7 @@ def test_trained_proposal_recall_and_gain(gpu):
     ev = PR.EvalSet(gpu, [r[0] for r in res], [r[1] for r in res], pc)
     topk, tau = int(extra["topk"]), float(extra["tau"])
     fb = tuple(float(v) for v in np.atleast_1d(extra["fallback"]))
-    sw = ev.sweep(net, tau=tau, points=[(topk, fb), (topk, (0.0, 0.0))])
+    sw = ev.sweep(net, points=[(tau, topk, fb), (tau, topk, (0.0, 0.0))])
     ev.close()
     at = sw["learned"][0]
     print("[proposal]", at, flush=True)
