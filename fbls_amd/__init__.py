@@ -58,7 +58,7 @@ EXPORTS = (
     "fbls_autotune", "fbls_biweight_flatten", "fbls_prepare_families",
     "fbls_search_transits", "fbls_single_event_search", "fbls_read_inputs",
     "fbls_format_f64", "fbls_json_reformat", "fbls_cli", "fbls_coarse_inputs",
-    "fbls_cnn_set_fused", "fbls_signal_search_bins",
+    "fbls_cnn_set_fused", "fbls_signal_search_bins", "fbls_propose",
 )
 
 
@@ -331,6 +331,8 @@ def lib():
             _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8)]
         L.fbls_coarse_features.argtypes = [_vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz,
                                            _u32, _u32, _P(_d)]
+        L.fbls_propose.argtypes = [_vp, _vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz, _u32,
+                                   _u32, _P(ctypes.c_float)]
         L.fbls_signal_search_bins.argtypes = [
             _vp, _P(_vp), _sz, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
             _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8), _sz, _P(_d)]
@@ -1081,6 +1083,18 @@ def coarse_inputs(gpu, devs, periods, durations, nb_min=16, nb_max=1024):
     _check(lib().fbls_coarse_inputs(gpu._h, hs, len(devs), _dp(periods), len(periods),
                                     _dp(durations), len(durations), nb_min, nb_max,
                                     out.ctypes.data_as(_P(ctypes.c_float))))
+    return out
+
+
+def propose(gpu, net, devs, periods, durations, nb_min=16, nb_max=1024):
+    """fbls_propose: the network's logits [star, period bin] from the coarse
+    periodogram computed and consumed on the device."""
+    periods, durations = _f64(periods), _f64(durations)
+    hs = (_vp * len(devs))(*[d._h.value for d in devs])
+    out = np.empty((len(devs), len(periods)), dtype=np.float32)
+    _check(lib().fbls_propose(gpu._h, net._h, hs, len(devs), _dp(periods), len(periods),
+                              _dp(durations), len(durations), nb_min, nb_max,
+                              out.ctypes.data_as(_P(ctypes.c_float))))
     return out
 
 

@@ -86,7 +86,8 @@ int ensure_acts(fbls_cnn* n, long long rows) {
 constexpr int kSplitChunk = 2048;  // rows of K per split of the weight-gradient GEMM
 
 // Inference: every layer as one fused implicit-GEMM conv + bias + GELU
-// launch (no im2col matrix, no fp32 pre-activations), then the head.
+// launch (no im2col matrix, no fp32 pre-activations), then the head.  x is
+// host input, or null when the input is already staged in X0.
 int infer(fbls_cnn* n, const float* x, uint32_t B, uint32_t L) {
   const long long rows = (long long)B * L;
   const int Cin = (int)n->cfg.in_channels, W = (int)n->cfg.width, kw = (int)n->cfg.kernel;
@@ -97,7 +98,7 @@ int infer(fbls_cnn* n, const float* x, uint32_t B, uint32_t L) {
   hipStream_t s = n->c->stream;
   float* x32 = (float*)n->X0.ptr;
   cn::bf16* x16 = (cn::bf16*)((unsigned char*)n->X0.ptr + rows * Cin * 4);
-  HIP_TRY(hipMemcpyAsync(x32, x, rows * Cin * 4, hipMemcpyHostToDevice, s));
+  if (x) HIP_TRY(hipMemcpyAsync(x32, x, rows * Cin * 4, hipMemcpyHostToDevice, s));
   HIP_TRY(cn::to_bf16(x32, x16, rows * Cin, s));
   const float* P = (const float*)n->P.ptr;
   const cn::bf16* PB = (const cn::bf16*)n->PB.ptr;
@@ -379,3 +380,32 @@ extern "C" int fbls_cnn_set_fused(fbls_cnn* n, int enable) {
   n->fused = enable ? 1 : 0;
   return FBLS_OK;
 }
+
+namespace fbls {
+namespace host {
+
+int cnn_stage_input(fbls_cnn* n, uint32_t B, uint32_t L, float** device_input) {
+  const long long rows = (long long)B * L;
+  if (B == 0 || L == 0 || (uint64_t)rows > (1ull << 24))
+    return fail(FBLS_E_ARG, "batch of %u x %u bins out of range", B, L);
+  HIP_TRY(hipSetDevice(n->c->device));
+  if (int st = n->X0.ensure(rows * n->cfg.in_channels * 6)) return st;
+  *device_input = (float*)n->X0.ptr;
+  return FBLS_OK;
+}
+
+int cnn_run_staged(fbls_cnn* n, uint32_t B, uint32_t L, float* logits) {
+  if (!(n->fused && fusable(n)))
+    return fail(FBLS_E_ARG, "staged inference needs the fused network shape");
+  if (int st = infer(n, nullptr, B, L)) return st;
+  HIP_TRY(hipMemcpyAsync(logits, n->z.ptr, (size_t)B * L * 4, hipMemcpyDeviceToHost,
+                         n->c->stream));
+  HIP_TRY(hipStreamSynchronize(n->c->stream));
+  return FBLS_OK;
+}
+
+fbls_ctx* cnn_context(const fbls_cnn* n) { return n->c; }
+uint32_t cnn_in_channels(const fbls_cnn* n) { return n->cfg.in_channels; }
+
+}  // namespace host
+}  // namespace fbls

@@ -100,6 +100,7 @@ __device__ __forceinline__ double wave_max(double x) {
   return x;
 }
 
+template <uint32_t NG>
 __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   const uint32_t groups = (A.np + A.G - 1) / A.G;
@@ -188,78 +189,103 @@ __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
   __syncthreads();
   // best SNR over phase per (period, duration): the phase with the largest
   // key a|a| / q (monotone in a / sqrt(q); the smallest phase on ties) is
-  // found by wave reductions, then its SNR a / sqrt(q) is computed once;
-  // 64 pairs at a time
-  __shared__ double red_key[kBlock / 64][64];
-  __shared__ uint32_t red_j[kBlock / 64][64];
-  const uint32_t pairs = gcount * A.ng;
-  for (uint32_t q0 = 0; q0 < pairs; q0 += 64) {
-    const uint32_t q1 = min(q0 + 64, pairs);
-    for (uint32_t q = q0; q < q1; q++) {
-      const uint32_t g = q / A.ng, d = q % A.ng;
-      const uint32_t nb = A.nb[p0 + g];
-      const double scale = (double)nb / A.periods[p0 + g];
-      const double* gv = hv + (size_t)g * stride;
-      const double* gw = hw + (size_t)g * stride;
-      const uint32_t* gn = hn + (size_t)g * stride;
-      const uint32_t tn = gn[nb];
-      const uint32_t k = width_of(A.durations[d], scale, nb);
-      double best = -kInf;
-      uint32_t bj = 0xffffffffu;
-      for (uint32_t j = tid; j < nb; j += kBlock) {
-        double a, qq;
-        if (window(gv, gw, gn, nb, tn, j, k, S, &a, &qq)) {
-          const double key = a * fabs(a) / qq;
-          if (key > best) {
-            best = key;
-            bj = j;
+  // found per thread (each thread reads its phase's prefix once for every
+  // duration), then by wave and workgroup reductions; its SNR a / sqrt(q) is
+  // computed once at the end
+  __shared__ double red_key[kBlock / 64][4 * NG];
+  __shared__ uint32_t red_j[kBlock / 64][4 * NG];
+  for (uint32_t g = 0; g < gcount; g++) {
+    const uint32_t nb = A.nb[p0 + g];
+    const double scale = (double)nb / A.periods[p0 + g];
+    const double* gv = hv + (size_t)g * stride;
+    const double* gw = hw + (size_t)g * stride;
+    const uint32_t* gn = hn + (size_t)g * stride;
+    const uint32_t tn = gn[nb];
+    const double tv = gv[nb], tw = gw[nb];
+    uint32_t kd[NG];
+    double best[NG];
+    uint32_t bj[NG];
+#pragma unroll
+    for (uint32_t d = 0; d < NG; d++) {
+      kd[d] = d < A.ng ? width_of(A.durations[d], scale, nb) : 1u;
+      best[d] = -kInf;
+      bj[d] = 0xffffffffu;
+    }
+    for (uint32_t j = tid; j < nb; j += kBlock) {
+      const double vj = gv[j], wj = gw[j];
+      const uint32_t nj = gn[j];
+#pragma unroll
+      for (uint32_t d = 0; d < NG; d++) {
+        if (d < A.ng) {
+          const uint32_t e = j + kd[d];
+          double a, w;
+          uint32_t c;
+          if (e <= nb) {
+            a = (gv[e] - vj) * S.sv;
+            w = (gw[e] - wj) * S.sw;
+            c = gn[e] - nj;
+          } else {  // exact integers: the order of the sums does not matter
+            a = (gv[e - nb] + (tv - vj)) * S.sv;
+            w = (gw[e - nb] + (tw - wj)) * S.sw;
+            c = gn[e - nb] + (tn - nj);
+          }
+          const double qq = w - w * w / S.C;
+          if (c >= 3 && c + 3 <= tn && qq > 0.0) {
+            const double key = a * fabs(a) / qq;
+            if (key > best[d]) {
+              best[d] = key;
+              bj[d] = j;
+            }
           }
         }
       }
+    }
+#pragma unroll
+    for (uint32_t d = 0; d < NG; d++) {
+      double bk = best[d];
+      uint32_t b = bj[d];
       for (int off = 32; off >= 1; off >>= 1) {
-        const double ok = __shfl_xor(best, off, 64);
-        const uint32_t oj = __shfl_xor(bj, off, 64);
-        if (ok > best || (ok == best && oj < bj)) {
-          best = ok;
-          bj = oj;
+        const double ok = __shfl_xor(bk, off, 64);
+        const uint32_t oj = __shfl_xor(b, off, 64);
+        if (ok > bk || (ok == bk && oj < b)) {
+          bk = ok;
+          b = oj;
         }
       }
-      if (lane == 0) {
-        red_key[wave][q - q0] = best;
-        red_j[wave][q - q0] = bj;
+      if (lane == 0 && d < A.ng) {
+        red_key[wave][g * NG + d] = bk;
+        red_j[wave][g * NG + d] = b;
       }
     }
-    __syncthreads();
-    if (tid < q1 - q0) {
-      double best = red_key[0][tid];
-      uint32_t bj = red_j[0][tid];
-      for (uint32_t w = 1; w < kBlock / 64; w++) {
-        const double ok = red_key[w][tid];
-        const uint32_t oj = red_j[w][tid];
-        if (ok > best || (ok == best && oj < bj)) {
-          best = ok;
-          bj = oj;
-        }
-      }
-      const uint32_t q = q0 + tid, g = q / A.ng, d = q % A.ng;
-      double m = -kInf;
-      if (bj != 0xffffffffu) {
-        const uint32_t nb = A.nb[p0 + g];
-        const double scale = (double)nb / A.periods[p0 + g];
-        double a, qq;
-        window(hv + (size_t)g * stride, hw + (size_t)g * stride, hn + (size_t)g * stride, nb,
-               hn[(size_t)g * stride + nb], bj, width_of(A.durations[d], scale, nb), S, &a,
-               &qq);
-        m = a / sqrt(qq);
-      }
-      const size_t at = ((size_t)star * A.np + p0 + g) * A.ng + d;
-      if (A.out) A.out[at] = m;
-      if (A.inputs) {  // the network input: clip(SNR, -5, 40) / 8, 0 when none
-        const double x = m > -kInf ? m : 0.0;
-        A.inputs[at] = (float)(fmin(fmax(x, -5.0), 40.0) / 8.0);
+  }
+  __syncthreads();
+  if (tid < gcount * NG && tid % NG < A.ng) {
+    const uint32_t g = tid / NG, d = tid % NG;
+    double bk = red_key[0][tid];
+    uint32_t b = red_j[0][tid];
+    for (uint32_t w = 1; w < kBlock / 64; w++) {
+      const double ok = red_key[w][tid];
+      const uint32_t oj = red_j[w][tid];
+      if (ok > bk || (ok == bk && oj < b)) {
+        bk = ok;
+        b = oj;
       }
     }
-    __syncthreads();
+    double m = -kInf;
+    if (b != 0xffffffffu) {
+      const uint32_t nb = A.nb[p0 + g];
+      const double scale = (double)nb / A.periods[p0 + g];
+      double a, qq;
+      window(hv + (size_t)g * stride, hw + (size_t)g * stride, hn + (size_t)g * stride, nb,
+             hn[(size_t)g * stride + nb], b, width_of(A.durations[d], scale, nb), S, &a, &qq);
+      m = a / sqrt(qq);
+    }
+    const size_t at = ((size_t)star * A.np + p0 + g) * A.ng + d;
+    if (A.out) A.out[at] = m;
+    if (A.inputs) {  // the network input: clip(SNR, -5, 40) / 8, 0 when none
+      const double x = m > -kInf ? m : 0.0;
+      A.inputs[at] = (float)(fmin(fmax(x, -5.0), 40.0) / 8.0);
+    }
   }
 }
 
@@ -271,11 +297,15 @@ size_t feature_lds_bytes(uint32_t nb_max, uint32_t G) {
 
 hipError_t feature_launch(const FeatureArgs& a, uint32_t nstars, hipStream_t s) {
   const size_t lds = feature_lds_bytes(a.nb_max, a.G);
-  hipError_t e = hipFuncSetAttribute((const void*)coarse_features,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (a.ng > 16 || a.G > 4) return hipErrorInvalidValue;
+  const void* fn = a.ng <= 8 ? (const void*)coarse_features<8> : (const void*)coarse_features<16>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const uint32_t groups = (a.np + a.G - 1) / a.G;
-  hipLaunchKernelGGL(coarse_features, dim3(nstars * groups), dim3(kBlock), lds, s, a);
+  if (a.ng <= 8)
+    hipLaunchKernelGGL(coarse_features<8>, dim3(nstars * groups), dim3(kBlock), lds, s, a);
+  else
+    hipLaunchKernelGGL(coarse_features<16>, dim3(nstars * groups), dim3(kBlock), lds, s, a);
   return hipGetLastError();
 }
 

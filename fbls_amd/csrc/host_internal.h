@@ -16,6 +16,13 @@ namespace host {
 // Records the thread's last error message and returns code.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 double now_s();
+// The network's device input buffer for a B x L batch (channels-last
+// float32, in_channels per bin), and the fused inference from it into host
+// logits (cnn_host.cpp).
+int cnn_stage_input(fbls_cnn* net, uint32_t B, uint32_t L, float** device_input);
+int cnn_run_staged(fbls_cnn* net, uint32_t B, uint32_t L, float* logits);
+fbls_ctx* cnn_context(const fbls_cnn* net);
+uint32_t cnn_in_channels(const fbls_cnn* net);
 // The biweight detrend of sorted times on the context's device
 // (raw_host.cpp): out[i] = (1 + rel[i]) / location_i - 1.
 int biweight_gpu(fbls_ctx* c, const std::vector<double>& t, const std::vector<double>& rel,

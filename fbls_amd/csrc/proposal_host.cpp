@@ -15,12 +15,16 @@ using namespace fbls::host;
 
 namespace {
 
+// The periodogram into host `out` (doubles) or host `inputs` (network
+// inputs), or, with device_inputs, the network inputs left in that device
+// buffer (no copy back).
 int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const double* periods,
            size_t np, const double* durations, size_t ng, uint32_t nb_min, uint32_t nb_max,
-           double* out, float* inputs) {
-  if (!c || (nfam && !fams) || !periods || !durations || (!out && !inputs))
+           double* out, float* inputs, float* device_inputs = nullptr) {
+  if (!c || (nfam && !fams) || !periods || !durations || (!out && !inputs && !device_inputs))
     return fail(FBLS_E_ARG, "null argument");
   if (np == 0 || ng == 0 || nfam == 0) return FBLS_OK;
+  if (ng > 16) return fail(FBLS_E_ARG, "at most 16 durations");
   if (nb_min < 2 || nb_max < nb_min || nb_max > 4096)
     return fail(FBLS_E_ARG, "phase bins must satisfy 2 <= nb_min <= nb_max <= 4096");
   if ((uint64_t)nfam * np > 0x7fffffffull) return fail(FBLS_E_ARG, "too many (star, period) pairs");
@@ -55,7 +59,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   const size_t o_nb = o_per + ((np * 8 + 255) & ~(size_t)255);
   const size_t o_dur = o_nb + ((np * 4 + 255) & ~(size_t)255);
   const size_t o_out = o_dur + ((ng * 8 + 255) & ~(size_t)255);
-  const size_t out_bytes = nfam * np * ng * (out ? 8 : 4);
+  const size_t out_bytes = device_inputs ? 0 : nfam * np * ng * (out ? 8 : 4);
   HIP_TRY(hipSetDevice(c->device));
   if (int st = c->dev_in.ensure(o_out + out_bytes)) return st;
   unsigned char* d = (unsigned char*)c->dev_in.ptr;
@@ -72,7 +76,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   A.ng = (uint32_t)ng;
   A.durations = (const double*)(d + o_dur);
   A.out = out ? (double*)(d + o_out) : nullptr;
-  A.inputs = out ? nullptr : (float*)(d + o_out);
+  A.inputs = out ? nullptr : (device_inputs ? device_inputs : (float*)(d + o_out));
   // periods per workgroup: as many as 24 KiB of LDS holds, at most 4 (more
   // LDS per workgroup costs occupancy, which the fold needs more than reuse)
   const uint32_t nb_top = *std::max_element(nb.begin(), nb.end());
@@ -80,6 +84,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   A.G = 4;
   while (A.G > 1 && fbls::feature_lds_bytes(nb_top, A.G) > 24 * 1024) A.G--;
   HIP_TRY(fbls::feature_launch(A, (uint32_t)nfam, s));
+  if (device_inputs) return FBLS_OK;  // stream-ordered before the network
   HIP_TRY(hipMemcpyAsync(out ? (void*)out : (void*)inputs, d + o_out, out_bytes,
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -100,4 +105,30 @@ extern "C" int fbls_coarse_inputs(fbls_ctx* c, const fbls_family* const* fams, s
                                   size_t ng, uint32_t nb_min, uint32_t nb_max, float* out) {
   if (!out) return fail(FBLS_E_ARG, "null argument");
   return coarse(c, fams, nfam, periods, np, durations, ng, nb_min, nb_max, nullptr, out);
+}
+
+extern "C" int fbls_propose(fbls_ctx* c, fbls_cnn* net, const fbls_family* const* fams,
+                            size_t nfam, const double* periods, size_t np,
+                            const double* durations, size_t ng, uint32_t nb_min,
+                            uint32_t nb_max, float* logits) {
+  if (!c || !net || !logits) return fail(FBLS_E_ARG, "null argument");
+  if (nfam == 0 || np == 0) return FBLS_OK;
+  if (fbls::host::cnn_context(net) != c)
+    return fail(FBLS_E_ARG, "the network belongs to another context");
+  if (fbls::host::cnn_in_channels(net) != ng)
+    return fail(FBLS_E_ARG, "the network reads %u durations, not %zu",
+                fbls::host::cnn_in_channels(net), ng);
+  // in batches of stars whose activations stay within a bounded footprint
+  const size_t per = std::max<size_t>(1, ((size_t)1 << 22) / np);
+  for (size_t s0 = 0; s0 < nfam; s0 += per) {
+    const size_t m = std::min(per, nfam - s0);
+    float* staged = nullptr;
+    if (int st = fbls::host::cnn_stage_input(net, (uint32_t)m, (uint32_t)np, &staged)) return st;
+    if (int st = coarse(c, fams + s0, m, periods, np, durations, ng, nb_min, nb_max, nullptr,
+                        nullptr, staged))
+      return st;
+    if (int st = fbls::host::cnn_run_staged(net, (uint32_t)m, (uint32_t)np, logits + s0 * np))
+      return st;
+  }
+  return FBLS_OK;
 }

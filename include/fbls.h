@@ -418,10 +418,12 @@ int fbls_cnn_train_step(fbls_cnn* net, const float* x, const float* y, uint32_t 
                         float lr, float weight_decay, float* loss);
 
 /* Coarse periodogram (the network's input): for each star k and period i,
- * out[k][i][g] = the largest box SNR a / sqrt(w - w^2 / C) over phase of a
- * transit of duration durations[g] at nb_i = clamp(ceil(2 P_i / d_0),
- * nb_min, nb_max) phase bins (d_0 the shortest duration; windows with at
- * least 3 points and at most n - 3; -inf when none).  One launch. */
+ * out[k][i][g] = the box SNR a / sqrt(w - w^2 / C) at the phase with the
+ * largest a |a| / (w - w^2 / C) (the first such phase; this is the largest
+ * SNR over phase) for a transit of duration durations[g] at nb_i =
+ * clamp(ceil(2 P_i / d_0), nb_min, nb_max) phase bins (d_0 the shortest
+ * duration; windows with at least 3 points and at most n - 3; -inf when
+ * none).  At most 16 durations.  One launch. */
 int fbls_coarse_features(fbls_ctx* ctx, const fbls_family* const* fams, size_t nfam,
                          const double* periods, size_t np, const double* durations,
                          size_t ng, uint32_t nb_min, uint32_t nb_max, double* out);
@@ -431,6 +433,15 @@ int fbls_coarse_features(fbls_ctx* ctx, const fbls_family* const* fams, size_t n
 int fbls_coarse_inputs(fbls_ctx* ctx, const fbls_family* const* fams, size_t nfam,
                        const double* periods, size_t np, const double* durations,
                        size_t ng, uint32_t nb_min, uint32_t nb_max, float* out);
+
+/* The proposal in one call: the network inputs of fbls_coarse_inputs are
+ * computed into the network's device buffer and the fused inference runs
+ * on them (no copy of the inputs to the host and back); logits[k][i] for
+ * star k and period bin i.  net must belong to ctx and read ng durations;
+ * stars go through in batches of about 2^22 bins. */
+int fbls_propose(fbls_ctx* ctx, fbls_cnn* net, const fbls_family* const* fams, size_t nfam,
+                 const double* periods, size_t np, const double* durations, size_t ng,
+                 uint32_t nb_min, uint32_t nb_max, float* logits);
 
 /* -- raw-mode preparation ----------------------------------------------- */
 

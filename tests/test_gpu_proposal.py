@@ -39,6 +39,11 @@ def test_coarse_features_bit_exact(gpu):
     # the network input computed on the device is the host transform of them
     inputs = F.coarse_inputs(gpu, devs, periods, gd, pc.nb_min, pc.nb_max)
     np.testing.assert_array_equal(inputs, PR.transform(got))
+    # the device-resident proposal equals the network run on those inputs
+    net = F.Cnn(gpu, in_channels=pc.groups, width=pc.width, layers=pc.layers,
+                kernel=pc.kernel, seed=3)
+    logits = F.propose(gpu, net, devs, periods, gd, pc.nb_min, pc.nb_max)
+    np.testing.assert_array_equal(logits, net.forward(inputs))
 
 
 def test_keep_mask_rows_are_full_rows(gpu):
