@@ -114,6 +114,66 @@ def proposal_stars(n_stars):
     return PR.make_states(range(n_stars), 4_300_000, PR.search_config(pc), workers=16)
 
 
+def _batch_state(args):
+    i, durations, correlated = args
+    from fbls_amd import family_state, synth
+
+    (t, f, e), kw = synth.batch_star(i)
+    return family_state(t, f - 1.0, e, durations, correlated), kw.get("period", 0.0)
+
+
+def batch_stars(n_stars):
+    """The 512-star batch of uneven length (1-13 sectors), prepared on the
+    host before the GPU is touched (worker processes)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    import fbls_amd as F
+
+    cfg = F.SearchConfig(period_min=0.5, period_max=20.0)
+    jobs = [(i, cfg.durations_days(), cfg.correlated_noise) for i in range(n_stars)]
+    with cf.ProcessPoolExecutor(16, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_batch_state, jobs, chunksize=4))
+
+
+def batched(gpu, res):
+    """The full scan of every star of the batch in merged launches
+    (fbls_signal_search_batch over one packed upload)."""
+    import fbls_amd as F
+    from fbls_amd import proposal as PR
+
+    cfg = F.SearchConfig(period_min=0.5, period_max=20.0)
+    states = [r[0] for r in res]
+    devs = F.DeviceReady.upload_packed(gpu, states)
+    d = cfg.durations_days()
+    rhos = [s.rho for s in states]
+    F.signal_search_batch(gpu, devs[:8], d, rhos[:8], cfg)  # staging growth outside timing
+    gpu.launch_totals(reset=True)
+    t0 = time.perf_counter()
+    rows, stats = F.signal_search_batch(gpu, devs, d, rhos, cfg)
+    dt = time.perf_counter() - t0
+    tot = gpu.launch_totals(reset=True)
+    templates = sum(st["templates_scanned"] for st in stats)
+    best = PR.best_rows(rows, cfg.threshold)
+    inj = [r[1] for r in res]
+    found = sum(1 for (p, _), c in zip(best, inj) if c > 0 and p == p and PR.alias(p, c) < 1e-3)
+    for dv in devs:
+        dv.close()
+    sizes = [len(s.t) for s in states]
+    return {
+        "workload": f"{len(res)} stars, 1-13 sectors ({min(sizes)}-{max(sizes)} points), "
+                    "sigma 200-3000 ppm, half with a box P 0.5-20 d; periods 0.5-20 d",
+        "stars_per_hour": len(res) * 3600.0 / dt,
+        "trial_evals_per_s": templates / dt,
+        "seconds": dt,
+        "kernel_seconds": tot["seconds"],
+        "launches": tot["launches"],
+        "periods": int(sum(st["periods"] for st in stats)),
+        "injections_recovered": found,
+        "injections": sum(1 for c in inj if c > 0),
+    }
+
+
 def learned_pruned(gpu, res):
     """The learned-pruned search against the full scan at the committed
     network's operating point."""
@@ -176,6 +236,8 @@ def main():
                     help="1 run-length, 2 one atomic per pair")
     ap.add_argument("--proposal-stars", type=int, default=512,
                     help="stars of the learned-pruned comparison (0 = skip)")
+    ap.add_argument("--batch-stars", type=int, default=512,
+                    help="stars of the batched full scan of uneven stars (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale counters")
     args = ap.parse_args()
@@ -189,6 +251,7 @@ def main():
         sys.exit(2)
 
     pstars = proposal_stars(args.proposal_stars) if args.proposal_stars > 0 else None
+    bstars = batch_stars(args.batch_stars) if args.batch_stars > 0 else None
     cfg, d, state = workload()
     n = len(state.t)
     gpu, dev, rows, grid, per_step, elapsed, tot = run_gpu(
@@ -251,6 +314,8 @@ def main():
     }
     if pstars:
         out["learned_pruned"] = learned_pruned(gpu, pstars)
+    if bstars:
+        out["batched"] = batched(gpu, bstars)
     if not args.no_cpu:
         cb, _ = cpu_baseline(cfg, d, state)
         out["cpu_baseline"] = cb
