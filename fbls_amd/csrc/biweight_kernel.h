@@ -18,6 +18,11 @@ struct BiweightArgs {
   double* out;             // n: flux / location - 1
 };
 
+// Blocks whose union of windows may exceed the staging size (read from
+// global memory, counting selection of the medians).
 hipError_t biweight_launch(const BiweightArgs& a, uint32_t nblocks, hipStream_t s);
+// Blocks whose union of windows holds at most kBiweightWindow values (sorted
+// once in LDS).
+hipError_t biweight_sorted_launch(const BiweightArgs& a, uint32_t nblocks, hipStream_t s);
 
 }  // namespace fbls

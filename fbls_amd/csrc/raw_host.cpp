@@ -42,15 +42,22 @@ int biweight_gpu(fbls_ctx* c, const std::vector<double>& t, const std::vector<do
       start = i;
     }
   }
-  std::vector<uint32_t> blocks;
+  // workgroups whose windows' union fits the staging size go to the sorted
+  // kernel (listed first), the others to the counting kernel
+  std::vector<uint32_t> staged, wide;
   for (size_t first = 0; first < n;) {
     size_t count = 1;
     while (count < fbls::kBiweightBlock && first + count < n &&
            hi[first + count] - lo[first] <= fbls::kBiweightWindow)
       count++;
-    blocks.insert(blocks.end(), {(uint32_t)first, (uint32_t)count, lo[first], hi[first + count - 1]});
+    const uint32_t ulo = lo[first], uhi = hi[first + count - 1];
+    std::vector<uint32_t>& list = uhi - ulo <= fbls::kBiweightWindow ? staged : wide;
+    list.insert(list.end(), {(uint32_t)first, (uint32_t)count, ulo, uhi});
     first += count;
   }
+  const size_t n_staged = staged.size() / 4;
+  std::vector<uint32_t> blocks = staged;
+  blocks.insert(blocks.end(), wide.begin(), wide.end());
   const size_t nb = blocks.size() / 4;
   const size_t b_flux = n * 8, b_idx = n * 4, b_blk = blocks.size() * 4;
   const size_t o_lo = (b_flux + 255) & ~(size_t)255;
@@ -73,7 +80,9 @@ int biweight_gpu(fbls_ctx* c, const std::vector<double>& t, const std::vector<do
   A.c = cst;
   A.iterations = (uint32_t)iterations;
   A.out = (double*)(d + o_out);
-  HIP_TRY(fbls::biweight_launch(A, (uint32_t)nb, s));
+  HIP_TRY(fbls::biweight_sorted_launch(A, (uint32_t)n_staged, s));
+  A.blocks = (const uint32_t*)(d + o_blk) + 4 * n_staged;
+  HIP_TRY(fbls::biweight_launch(A, (uint32_t)(nb - n_staged), s));
   HIP_TRY(hipMemcpyAsync(out->data(), d + o_out, n * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return FBLS_OK;

@@ -47,3 +47,27 @@ def batch_star(i, seed0=3000):
                / (20 ** (1 / 3) - 0.5 ** (1 / 3))) / 24.0
         kw.update(period=p, depth=depth, duration=dur)
     return synth_star(**kw), kw
+
+
+def write_npz(path, workload="four_sector_red", seed=None, raw=True):
+    """Writes a workload as an NPZ light curve (time_bjd, flux, flux_err,
+    sector).  raw: the flux in counts (x 12000) with a slow 6-day trend of
+    0.4 %, as the raw search expects; otherwise the normalised flux."""
+    fn = {"single_sector": single_sector, "four_sector_red": four_sector_red}[workload]
+    t, f, e = fn() if seed is None else fn(seed=seed)
+    sector = (np.floor((t - t[0]) / 28.4)).astype(np.int64)
+    if raw:
+        slow = 1.0 + 0.004 * np.sin(2.0 * np.pi * (t - t[0]) / 6.0)
+        f, e = 12000.0 * f * slow, 12000.0 * e
+    np.savez(path, time_bjd=t, flux=f, flux_err=e, sector=sector)
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="write a synthetic workload as NPZ")
+    ap.add_argument("workload", choices=["single_sector", "four_sector_red"])
+    ap.add_argument("out")
+    ap.add_argument("--normalised", action="store_true")
+    a = ap.parse_args()
+    write_npz(a.out, a.workload, raw=not a.normalised)
