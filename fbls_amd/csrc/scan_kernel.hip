@@ -396,13 +396,21 @@ __device__ void fold_all(const Segment& S, unsigned char* hist,
   const uint32_t nb1 = nb + 1;
   const float inv_lo = inv_nb_low(nb);
   const uint32_t nblk = S.n_pad / kObsPerWave;
+  // Lane l holds the scale of period p0 + l (gcount <= 16), loaded once; a
+  // period's scale is then a v_readlane.  A memory read inside the period
+  // loop would wait on the counter the LDS atomics share and drain them.
+  const double my_scale = lane < gcount ? ((gdouble*)S.scales)[p0 + lane] : 0.0;
+  const uint64_t my_bits = (uint64_t)__double_as_longlong(my_scale);
+  const int my_lo = (int)(uint32_t)my_bits, my_hi = (int)(uint32_t)(my_bits >> 32);
   for (uint32_t blk = wave; blk < nblk; blk += nwaves) {
     const uint32_t base = blk * kObsPerWave + lane * kObsPerLane;
     double tt[kObsPerLane], vv[kObsPerLane], ww[kObsPerLane];
     load_obs(S, base, tt, vv, ww);
     const bool full = blk * kObsPerWave + kObsPerWave <= S.n;
     for (uint32_t g = 0; g < gcount; g++) {
-      const double scale = S.scales[p0 + g];
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(my_lo, (int)g);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(my_hi, (int)g);
+      const double scale = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
       double* vw = (double*)(hist + (size_t)g * S.pstride);
       uint32_t* n = (uint32_t*)(vw + 2 * nb1);
       uint32_t u[kObsPerLane];
