@@ -14,8 +14,10 @@ benchmark (min over m in {1/3, 1/2, 1, 2, 3} of |P / P_inj - m| / m < 1e-3,
 tools/benchmark.py:26-27, src/validate.rs:41-49).
 
 Usage (on a GPU):
-    python -m fbls_amd.proposal --train 12000 --heldout 4000 \\
+    python -m fbls_amd.proposal --train 200000 --skip-eval --out w.npz
+    python -m fbls_amd.proposal --reselect --validate 2000 --heldout 20000 \\
         --out fbls_amd/data/proposal.npz --report report.json
+(the second reads the weights and training figures from fbls_amd/data)
 """
 import argparse
 import concurrent.futures as cf
@@ -394,14 +396,17 @@ def _noop(_):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=12000)
+    ap.add_argument("--train", type=int, default=200_000)
     ap.add_argument("--validate", type=int, default=2000)
-    ap.add_argument("--heldout", type=int, default=4000)
+    ap.add_argument("--heldout", type=int, default=20_000)
     ap.add_argument("--chunk", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--report", default=None)
     ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--skip-eval", action="store_true",
+                    help="train and save only; the operating point and the held-out "
+                         "report come from a later --reselect run")
     ap.add_argument("--reselect", action="store_true",
                     help="keep the saved weights; redo the operating point and report")
     args = ap.parse_args()
@@ -418,91 +423,91 @@ def main():
     gpu = Gpu(0)
     d = cfg.durations_days()
 
-    def states_of(indices, seed0):
-        res = list(pool.map(_make_state, [(i, seed0, d, cfg.correlated_noise)
-                                          for i in indices], chunksize=32))
-        return [r[0] for r in res], [r[1] for r in res]
+    def chunks(total, seed0):
+        """(states, injected periods) chunk by chunk; the host prepares the
+        next chunk in the worker processes while the GPU works on this one."""
+        spans = [range(c0, min(total, c0 + args.chunk)) for c0 in range(0, total, args.chunk)]
 
-    xs, ys = [], []
+        def submit(idx):
+            return pool.map(_make_state, [(i, seed0, d, cfg.correlated_noise) for i in idx],
+                            chunksize=32)
+
+        pending = submit(spans[0]) if spans else None
+        for k, idx in enumerate(spans):
+            res = list(pending)
+            pending = submit(spans[k + 1]) if k + 1 < len(spans) else None
+            yield idx, [r[0] for r in res], [r[1] for r in res]
+
     seed_train, seed_val, seed_test = 4_000_000, 4_100_000, 4_200_000
     t0 = time.perf_counter()
+    prior_path = os.path.join(HERE, "data", "proposal_report.json")
     if args.reselect:
         net, _, _ = load(gpu)
-        prior_path = os.path.join(HERE, "data", "proposal_report.json")
         prior = json.load(open(prior_path)) if os.path.exists(prior_path) else {}
         t_train = float(prior.get("train_seconds", 0.0))
         losses = [float(prior.get("final_loss", 0.0))]
+        positives = int(prior.get("train_positives", 0))
         args.train = int(prior.get("train_stars", args.train))
         args.epochs = int(prior.get("epochs", args.epochs))
-    for c0 in range(0, 0 if args.reselect else args.train, args.chunk):
-        idx = range(c0, min(args.train, c0 + args.chunk))
-        states, _ = states_of(idx, seed_train)
-        devs, x, rows, _, _, _ = chunk_data(gpu, states, pc)
-        xs.append(x)
-        ys.append(labels_from(best_rows(rows, cfg.threshold), pc))
-        for dv in devs:
-            dv.close()
-        log(f"train stars {idx.stop} ({time.perf_counter() - t0:.1f} s)")
-    if not args.reselect:
-        x = np.concatenate(xs)
-        y = np.concatenate(ys)
-        log(f"positives {int((y.sum(axis=1) > 0).sum())} of {len(y)} stars")
+    else:
+        # the whole training set's inputs (float32, 64 KiB per star) and labels
+        x = np.empty((args.train, pc.bins, pc.groups), dtype=np.float32)
+        y = np.empty((args.train, pc.bins), dtype=np.float32)
+        for idx, states, _ in chunks(args.train, seed_train):
+            devs, xc, rows, _, _, _ = chunk_data(gpu, states, pc)
+            x[idx.start:idx.stop] = xc
+            y[idx.start:idx.stop] = labels_from(best_rows(rows, cfg.threshold), pc)
+            for dv in devs:
+                dv.close()
+            log(f"train stars {idx.stop} ({time.perf_counter() - t0:.1f} s)")
+        positives = int((y.sum(axis=1) > 0).sum())
+        log(f"positives {positives} of {len(y)} stars")
         net = Cnn(gpu, in_channels=pc.groups, width=pc.width, layers=pc.layers,
                   kernel=pc.kernel, pos_weight=pc.pos_weight, seed=7)
         t1 = time.perf_counter()
         losses = train(net, x, y, epochs=args.epochs, log=log)
         t_train = time.perf_counter() - t1
+        del x, y
         log(f"trained in {t_train:.1f} s")
-    # operating point: the fastest reaching 99.5 % recall on validation stars
-    vs, vi = states_of(range(args.validate), seed_val)
-    val = EvalSet(gpu, vs, vi, pc)
-    vsweep = val.sweep(net)
-    val.close()
-    point = fastest(vsweep["learned"], 0.99) or max(vsweep["learned"],
-                                                   key=lambda r: r["recall"])
+    if args.skip_eval:
+        pool.shutdown()
+        save(net, pc, args.out, tau=0.1, topk=0, fallback=np.zeros(2))
+        report = {"train_stars": args.train, "epochs": args.epochs,
+                  "train_positives": positives, "train_seconds": t_train,
+                  "final_loss": float(np.mean(losses[-50:])),
+                  "config": dataclasses.asdict(pc)}
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump(report, f, indent=1)
+        return
+    # operating point: the fastest reaching 99 % recall on validation stars
+    vsweeps = []
+    for _, vs, vi in chunks(args.validate, seed_val):
+        val = EvalSet(gpu, vs, vi, pc)
+        vsweeps.append(val.sweep(net)["learned"])
+        val.close()
+    vmerged = _merge(vsweeps)
+    point = fastest(vmerged, 0.99) or max(vmerged, key=lambda r: r["recall"])
     topk, fallback, tau = point["topk"], point["fallback"], point["tau"]
     log(f"operating point tau {tau} top-k {topk} fallback {fallback}: validation {point}")
-    hs, hi = states_of(range(args.heldout), seed_test)
-    pool.shutdown()
     learned, raw = [], []
-    for c0 in range(0, len(hs), args.chunk):
-        ev = EvalSet(gpu, hs[c0:c0 + args.chunk], hi[c0:c0 + args.chunk], pc)
-        op = (tau, topk, tuple(fallback))
+    op = (tau, topk, tuple(fallback))
+    for idx, hs, hi in chunks(args.heldout, seed_test):
+        ev = EvalSet(gpu, hs, hi, pc)
         sw = ev.sweep(net, points=[op] + [p for p in POINTS if p != op])
         learned.append(sw["learned"])
         raw.append(sw["raw_snr_topk"])
         ev.close()
-        log(f"held-out {min(len(hs), c0 + args.chunk)} stars")
-
-    def merge(chunks):
-        """Sum the chunks of each operating point."""
-        out = []
-        for j in range(len(chunks[0])):
-            rs = [c[j] for c in chunks]
-            m = {k: rs[0][k] for k in ("topk", "fallback", "tau")}
-            m["kept_period_fraction_weighted"] = float(
-                sum(r["kept_period_fraction"] * r["stars"] for r in rs) / sum(r["stars"] for r in rs))
-            for k in ("stars", "full_recovered", "pruned_recovered", "fallback_stars",
-                      "templates_full", "templates_pruned"):
-                m[k] = int(sum(r[k] for r in rs))
-            for k in ("seconds_full_scan", "seconds_proposal", "seconds_pruned_scan"):
-                m[k] = float(sum(r[k] for r in rs))
-            m["recall"] = m["pruned_recovered"] / max(m["full_recovered"], 1)
-            m["kept_period_fraction"] = float(np.mean([r["kept_period_fraction"] for r in rs]))
-            m["stars_per_hour_full"] = m["stars"] * 3600.0 / m["seconds_full_scan"]
-            m["stars_per_hour_pruned"] = m["stars"] * 3600.0 / (
-                m["seconds_proposal"] + m["seconds_pruned_scan"])
-            m["speedup"] = m["stars_per_hour_pruned"] / m["stars_per_hour_full"]
-            m["evals_ratio"] = m["templates_full"] / max(m["templates_pruned"], 1)
-            out.append(m)
-        return out
-
-    ml, mr = merge(learned), merge(raw)
+        log(f"held-out {idx.stop} stars ({time.perf_counter() - t0:.1f} s)")
+    pool.shutdown()
+    ml, mr = _merge(learned), _merge(raw)
     report = {
-        "train_stars": args.train, "validation_stars": args.validate,
+        "train_stars": args.train, "train_positives": positives,
+        "validation_stars": args.validate,
         "heldout_stars": args.heldout, "epochs": args.epochs,
         "train_seconds": t_train, "final_loss": float(np.mean(losses[-50:])),
         "operating_point": {"tau": tau, "topk": topk, "fallback": fallback},
+        "validation_at_operating_point": point,
         "heldout_at_operating_point": ml[0],
         "heldout_learned_sweep": ml, "heldout_raw_snr_sweep": mr,
         "config": dataclasses.asdict(pc),
@@ -512,6 +517,30 @@ def main():
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1)
+
+
+def _merge(chunks):
+    """Per operating point, the sums over chunks of stars (recall, speed and
+    kept fraction of the whole set)."""
+    out = []
+    for j in range(len(chunks[0])):
+        rs = [c[j] for c in chunks]
+        m = {k: rs[0][k] for k in ("topk", "fallback", "tau")}
+        for k in ("stars", "full_recovered", "pruned_recovered", "fallback_stars",
+                  "templates_full", "templates_pruned"):
+            m[k] = int(sum(r[k] for r in rs))
+        for k in ("seconds_full_scan", "seconds_proposal", "seconds_pruned_scan"):
+            m[k] = float(sum(r[k] for r in rs))
+        m["recall"] = m["pruned_recovered"] / max(m["full_recovered"], 1)
+        m["kept_period_fraction"] = float(
+            sum(r["kept_period_fraction"] * r["stars"] for r in rs) / m["stars"])
+        m["stars_per_hour_full"] = m["stars"] * 3600.0 / m["seconds_full_scan"]
+        m["stars_per_hour_pruned"] = m["stars"] * 3600.0 / (
+            m["seconds_proposal"] + m["seconds_pruned_scan"])
+        m["speedup"] = m["stars_per_hour_pruned"] / m["stars_per_hour_full"]
+        m["evals_ratio"] = m["templates_full"] / max(m["templates_pruned"], 1)
+        out.append(m)
+    return out
 
 
 if __name__ == "__main__":
