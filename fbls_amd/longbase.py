@@ -1,0 +1,287 @@
+"""Long-baseline batch: learned-pruned search with a full-scan fallback.
+
+Stars of 1-26 sectors in two seasons around a one-year gap, periods
+0.5-100 d (synth.long_baseline_star), searched three ways on one GPU:
+
+* the full scan: fbls_signal_search_batch over every grid period of every
+  star (the reference's search, prescan factor 2);
+* the learned-pruned scan: the band-proposal network reads a coarse
+  periodogram of each star and the bin-restricted batch search
+  (fbls_signal_search_bins) visits only the grid periods of the bins it
+  proposes; stars whose largest bin probability falls in a fallback band
+  [lo, hi) are not trusted to the network and get the full scan.
+
+The coarse periodogram keeps the coherence of the single-sector one the
+network was trained on: its bins are log-spaced with a relative width that
+shrinks as 1 / baseline (the grid's own period step, grid.rs:12-13, scales
+the same way), so that the phase drift across a bin over the whole
+baseline is the same as over one sector, and it is computed on the star
+binned to 10 minutes (sums of v and w per bin, the bin's mean time), which
+leaves the box statistic of every window of >= 30 minutes unchanged up to
+the phase smear of one bin.  Stars with the same sector count share their
+baseline, hence their bins: they go through the proposal and the pruned
+search as one group.
+
+Recall is the share of the full scan's recovered injections (transits and
+eclipsing-binary pairs, alias rule of tools/benchmark.py:26-27) that the
+pruned search also recovers; stars/hour count every second of the pruned
+path (binning, upload of the binned stars, periodogram, network, bin
+selection, planning and scans).
+"""
+import concurrent.futures as cf
+import math
+import time
+
+import numpy as np
+
+from . import (DeviceReady, FamilyState, SearchConfig, family_state, propose,
+               signal_search_batch, signal_search_bins, synth)
+from . import proposal as PR
+
+PERIOD_MIN, PERIOD_MAX = 0.5, 100.0
+BIN_MINUTES = 10.0
+
+
+def search_config():
+    return SearchConfig(period_min=PERIOD_MIN, period_max=PERIOD_MAX)
+
+
+def _make_state(args):
+    i, seed0, durations, correlated = args
+    (t, f, e), kw = synth.long_baseline_star(i, seed0)
+    s = family_state(t, f - 1.0, e, durations, correlated)
+    return s, kw.get("period", 0.0), kw["kind"], kw["sectors"]
+
+
+def make_states(indices, seed0, workers=16):
+    """[(state, injected period or 0, kind, sectors)] of the given stars."""
+    cfg = search_config()
+    jobs = [(i, seed0, cfg.durations_days(), cfg.correlated_noise) for i in indices]
+    if workers <= 1:
+        return [_make_state(j) for j in jobs]
+    import multiprocessing as mp
+
+    with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_make_state, jobs, chunksize=2))
+
+
+def binned_state(s: FamilyState, minutes=BIN_MINUTES):
+    """The star summed into consecutive time bins of the given width: v and
+    w add (the box statistic reads only their window sums), the bin's time
+    is the mean of its points' times."""
+    key = np.floor(s.t / (minutes / 1440.0)).astype(np.int64)
+    starts = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    count = np.diff(np.r_[starts, len(key)])
+    t = np.add.reduceat(s.t, starts) / count
+    v = np.add.reduceat(s.v, starts)
+    w = np.add.reduceat(s.w, starts)
+    return FamilyState(t, v, w, s.baseline_norm, s.origin, s.durations, s.rho,
+                       s.inflation, s.segments)
+
+
+def feature_bins(t_last, pc: PR.ProposalConfig, period_min=PERIOD_MIN,
+                 period_max=PERIOD_MAX):
+    """(centers, edges) of a star's coarse periodogram: log-spaced bins over
+    [period_min, min(period_max, baseline)] whose relative width is the
+    single-sector network's (pc: bins over [pc.period_min, pc.period_max]
+    for a 27.4-d sector) scaled by 27.4 d / baseline.  edges[0] = 0 and
+    edges[-1] = inf, as every grid period belongs to some bin."""
+    hi = min(period_max, t_last)
+    ref = math.log(pc.period_max / pc.period_min) / pc.bins
+    step = ref * 27.4 / max(t_last, 27.4)
+    lo_l, hi_l = math.log(period_min), math.log(hi)
+    nbins = max(1, int(math.ceil((hi_l - lo_l) / step)))
+    step = (hi_l - lo_l) / nbins
+    centers = np.exp(lo_l + (np.arange(nbins) + 0.5) * step)
+    edges = np.exp(lo_l + step * np.arange(nbins + 1))
+    edges[0], edges[-1] = 0.0, np.inf
+    return centers, edges
+
+
+def groups_of(states):
+    """Star indices grouped by baseline (equal sector counts share it)."""
+    out = {}
+    for k, s in enumerate(states):
+        out.setdefault(float(s.t[-1]), []).append(k)
+    return out
+
+
+def keep_from(probs, dilate, tau):
+    """Bins with p >= tau, dilated by `dilate` bins on each side."""
+    sel = probs >= tau
+    out = sel.copy()
+    for o in range(1, dilate + 1):
+        out[o:] |= sel[:-o]
+        out[:-o] |= sel[o:]
+    return out
+
+
+class LongBatch:
+    """A set of long-baseline stars resident on the GPU with their full-scan
+    results; evaluates the learned-pruned search at operating points."""
+
+    def __init__(self, gpu, res, net, pc, scan_chunk=32):
+        self.gpu, self.net, self.pc = gpu, net, pc
+        self.cfg = search_config()
+        self.states = [r[0] for r in res]
+        self.injected = [r[1] for r in res]
+        self.kinds = [r[2] for r in res]
+        self.sectors = [r[3] for r in res]
+        self.rhos = [s.rho for s in self.states]
+        self.d = self.cfg.durations_days()
+        self.devs = DeviceReady.upload_packed(gpu, self.states)
+        self.groups = groups_of(self.states)
+        # full scan (warm the staging buffers on the first chunk first)
+        signal_search_batch(gpu, self.devs[:2], self.d, self.rhos[:2], self.cfg)
+        t0 = time.perf_counter()
+        best, templates, periods = [], 0, 0
+        for c0 in range(0, len(self.devs), scan_chunk):
+            rows, stats = signal_search_batch(gpu, self.devs[c0:c0 + scan_chunk], self.d,
+                                              self.rhos[c0:c0 + scan_chunk], self.cfg)
+            best += PR.best_rows(rows, self.cfg.threshold)
+            templates += sum(st["templates_scanned"] for st in stats)
+            periods += sum(st["periods"] for st in stats)
+        self.t_full = time.perf_counter() - t0
+        self.templates_full, self.periods_full = templates, periods
+        self.best_full = best
+        self.rf = PR.recovered(best, self.injected)
+
+    def proposal(self):
+        """Per star: bin probabilities and edges; the seconds of binning,
+        upload and the device-resident periodogram + network."""
+        gd = PR.group_durations(self.cfg, self.pc.groups)
+        t0 = time.perf_counter()
+        binned = [binned_state(s) for s in self.states]
+        bdevs = DeviceReady.upload_packed(self.gpu, binned)
+        probs, edges = [None] * len(self.states), [None] * len(self.states)
+        for t_last, idx in self.groups.items():
+            centers, e = feature_bins(t_last, self.pc)
+            per = max(1, (1 << 24) // len(centers))  # network batch limit
+            for c0 in range(0, len(idx), per):
+                part = idx[c0:c0 + per]
+                z = propose(self.gpu, self.net, [bdevs[k] for k in part], centers, gd,
+                            self.pc.nb_min, self.pc.nb_max)
+                with np.errstate(over="ignore"):  # exp overflow is p = 0
+                    p = 1.0 / (1.0 + np.exp(-z))
+                for j, k in enumerate(part):
+                    probs[k], edges[k] = p[j], e
+        dt = time.perf_counter() - t0
+        for b in bdevs:
+            b.close()
+        return probs, edges, dt
+
+    def pruned(self, probs, edges, t_prop, tau, fallback):
+        """The bin-restricted search of every group at (tau, fallback band),
+        timed from bin selection to the rows."""
+        lo, hi = fallback
+        t0 = time.perf_counter()
+        best = [None] * len(self.states)
+        templates = periods = nfall = 0
+        for _, idx in self.groups.items():
+            keep = np.stack([keep_from(probs[k], self.pc.keep_dilate, tau) for k in idx])
+            for j, k in enumerate(idx):
+                conf = float(probs[k].max())
+                if lo <= conf < hi:
+                    keep[j] = True
+                    nfall += 1
+            rows, stats = signal_search_bins(self.gpu, [self.devs[k] for k in idx], self.d,
+                                             [self.rhos[k] for k in idx], self.cfg, keep,
+                                             edges[idx[0]])
+            for k, b in zip(idx, PR.best_rows(rows, self.cfg.threshold)):
+                best[k] = b
+            templates += sum(st["templates_scanned"] for st in stats)
+            periods += sum(st["periods"] for st in stats)
+        t_pruned = time.perf_counter() - t0
+        rp = PR.recovered(best, self.injected)
+        n = len(self.states)
+        eb = np.array([k == "eb" for k in self.kinds])
+        return {
+            "stars": n, "tau": tau, "fallback": list(fallback),
+            "full_recovered": int(self.rf.sum()),
+            "pruned_recovered": int((self.rf & rp).sum()),
+            "recall": float((self.rf & rp).sum() / max(self.rf.sum(), 1)),
+            "eb_full_recovered": int((self.rf & eb).sum()),
+            "eb_pruned_recovered": int((self.rf & rp & eb).sum()),
+            "fallback_stars": nfall,
+            "kept_period_fraction": periods / max(self.periods_full, 1),
+            "templates_full": self.templates_full, "templates_pruned": templates,
+            "seconds_full_scan": self.t_full, "seconds_proposal": t_prop,
+            "seconds_pruned_scan": t_pruned,
+            "stars_per_hour_full": n * 3600.0 / self.t_full,
+            "stars_per_hour_pruned": n * 3600.0 / (t_prop + t_pruned),
+            "speedup": self.t_full / (t_prop + t_pruned),
+        }
+
+    def sweep(self, points):
+        probs, edges, t_prop = self.proposal()
+        return [self.pruned(probs, edges, t_prop, tau, fb) for tau, fb in points]
+
+    def summary(self):
+        inj = np.array([p > 0 for p in self.injected])
+        return {
+            "stars": len(self.states),
+            "sectors_min": int(min(self.sectors)), "sectors_max": int(max(self.sectors)),
+            "points_max": int(max(len(s.t) for s in self.states)),
+            "transits": int(sum(k == "transit" for k in self.kinds)),
+            "eclipsing_binaries": int(sum(k == "eb" for k in self.kinds)),
+            "full_recovered_of_injected": [int(self.rf.sum()), int(inj.sum())],
+            "full_scan_seconds": self.t_full,
+            "full_scan_stars_per_hour": len(self.states) * 3600.0 / self.t_full,
+            "full_scan_trial_evals_per_s": self.templates_full / self.t_full,
+            "full_scan_periods": self.periods_full,
+        }
+
+    def close(self):
+        for d in self.devs:
+            d.close()
+
+
+POINTS = [(tau, fb) for tau in (0.02, 0.05, 0.1) for fb in ((0.0, 0.0), (0.005, 0.1))]
+
+
+def main():
+    import argparse
+    import json
+
+    from . import Gpu
+
+    ap = argparse.ArgumentParser(description="long-baseline learned-pruned batch")
+    ap.add_argument("--validate", type=int, default=48)
+    ap.add_argument("--heldout", type=int, default=96)
+    ap.add_argument("--report", default=None)
+    ap.add_argument("--workers", type=int, default=16)
+    args = ap.parse_args()
+
+    def log(msg):
+        print(f"[longbase] {msg}", flush=True)
+
+    vres = make_states(range(args.validate), 5_100_000, args.workers)
+    hres = make_states(range(args.heldout), 5_000_000, args.workers)
+    log(f"prepared {len(vres)} + {len(hres)} stars")
+    gpu = Gpu(0)
+    net, pc, _ = PR.load(gpu)
+    val = LongBatch(gpu, vres, net, pc)
+    log(f"validation full scan {val.summary()}")
+    vs = val.sweep(POINTS)
+    val.close()
+    for r in vs:
+        log(f"validation {r}")
+    point = PR.fastest(vs, 0.99) or max(vs, key=lambda r: r["recall"])
+    op = (point["tau"], tuple(point["fallback"]))
+    held = LongBatch(gpu, hres, net, pc)
+    summary = held.summary()
+    log(f"held-out full scan {summary}")
+    hs = held.sweep([op] + [p for p in POINTS if p != op])
+    held.close()
+    report = {"operating_point": {"tau": op[0], "fallback": list(op[1])},
+              "validation_stars": len(vres), "validation_at_operating_point": point,
+              "heldout": summary, "heldout_at_operating_point": hs[0],
+              "heldout_sweep": hs}
+    log(json.dumps(hs[0]))
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(report, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

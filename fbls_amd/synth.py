@@ -49,6 +49,43 @@ def batch_star(i, seed0=3000):
     return synth_star(**kw), kw
 
 
+def _duration_days(p, p_max):
+    """1 h + 7 h (P^(1/3) - 0.5^(1/3)) / (p_max^(1/3) - 0.5^(1/3)), in days."""
+    return (1.0 + 7.0 * (p ** (1 / 3) - 0.5 ** (1 / 3))
+            / (p_max ** (1 / 3) - 0.5 ** (1 / 3))) / 24.0
+
+
+def long_baseline_star(i, seed0=5_000_000):
+    """Star i of the long-baseline batch: 1-26 sectors in two seasons of
+    ceil(k/2) and floor(k/2) contiguous 27.4-d sectors separated by a
+    365-d gap (up to ~510k points over ~1080 d); sigma log-uniform
+    200-3000 ppm of which an AR(1) (phi = 0.9) share uniform in [0, 0.5] of
+    the variance; 30 % carry a box (P log-uniform 0.5-100 d, depth
+    log-uniform 200-10000 ppm, duration 1-8 h growing as P^(1/3)), 5 % an
+    eclipsing-binary-like pair (primary depth log-uniform 2000-30000 ppm,
+    secondary at phase 0.5 with a depth ratio uniform in [0.1, 0.9], the
+    same period and duration laws).  Returns ((time, flux, flux_err), kw)
+    with kw["kind"] in {"none", "transit", "eb"}."""
+    rng = np.random.default_rng(seed0 + i)
+    sectors = int(rng.integers(1, 27))
+    sigma = float(np.exp(rng.uniform(np.log(200e-6), np.log(3000e-6))))
+    kw = dict(seed=seed0 + i, sectors=sectors, sector_days=27.4, gap_days=1.0,
+              cadence_days=CADENCE, t0=T0, sigma=sigma,
+              red_fraction=float(rng.uniform(0.0, 0.5)), ar_phi=0.9,
+              baseline_gap_days=365.0 if sectors > 1 else 0.0)
+    u = rng.uniform()
+    p = float(np.exp(rng.uniform(np.log(0.5), np.log(100.0))))
+    if u < 0.30:
+        depth = float(np.exp(rng.uniform(np.log(200e-6), np.log(1e-2))))
+        kw.update(period=p, depth=depth, duration=_duration_days(p, 100.0))
+    elif u < 0.35:
+        depth = float(np.exp(rng.uniform(np.log(2e-3), np.log(3e-2))))
+        kw.update(period=p, depth=depth, duration=_duration_days(p, 100.0),
+                  secondary_depth=depth * float(rng.uniform(0.1, 0.9)))
+    kind = "none" if u >= 0.35 else ("transit" if u < 0.30 else "eb")
+    return synth_star(**kw), dict(kw, kind=kind)
+
+
 def write_npz(path, workload="four_sector_red", seed=None, raw=True):
     """Writes a workload as an NPZ light curve (time_bjd, flux, flux_err,
     sector).  raw: the flux in counts (x 12000) with a slow 6-day trend of

@@ -1,0 +1,120 @@
+"""GPU: the long-baseline batch (1-26 sectors around a one-year gap,
+periods 0.5-100 d).
+
+* A 26-sector star (494k points, 1,077-d baseline, 821k grid periods):
+  periods sampled from every octave band, scanned at the band's full and
+  coarse phase-bin counts (192 ... 19,200: the LDS kernel and the
+  range-split kernel), are bit-identical to the FP64 oracle on the same
+  fixed-point fields.
+* The coarse periodogram of the 10-minute binned star at its
+  baseline-scaled bins is bit-identical to the oracle's.
+* The grouped bin-restricted search of a batch returns the full scan's
+  rows when every bin is kept, and the full scan recovers a deep injected
+  transit of a long-baseline star.
+"""
+import numpy as np
+import pytest
+
+import fbls_amd as F
+import oracle as O
+from fbls_amd import longbase as L
+from fbls_amd import proposal as PR
+from fbls_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _long_star(period=47.3, depth=1.5e-3):
+    kw = dict(seed=5_900_001, sectors=26, sector_days=27.4, gap_days=1.0,
+              cadence_days=synth.CADENCE, t0=synth.T0, sigma=1e-3, red_fraction=0.3,
+              ar_phi=0.9, baseline_gap_days=365.0, period=period, depth=depth,
+              duration=synth._duration_days(period, 100.0))
+    t, f, e = F.synth_star(**kw)
+    cfg = L.search_config()
+    d = cfg.durations_days()
+    return cfg, d, F.family_state(t, f - 1.0, e, d, cfg.correlated_noise)
+
+
+@pytest.fixture(scope="module")
+def long_star(gpu):
+    cfg, d, s = _long_star()
+    dev = F.DeviceReady.upload(gpu, s)
+    vq, wq, _, _ = dev.fields()
+    yield cfg, d, s, dev, vq, wq
+    dev.close()
+
+
+def test_long_star_every_band_bit_exact(gpu, long_star):
+    cfg, d, s, dev, vq, wq = long_star
+    grid = F.period_grid(cfg.period_min, min(cfg.period_max, s.t[-1]), s.t[-1], d.min(),
+                         cfg.period_oversample)
+    bands = F.plan_bands(grid, np.sort(d), cfg.phase_oversample, cfg.prescan_factor)
+    assert len(s.t) > 490_000 and len(grid) > 800_000
+    assert max(b["nb"] for b in bands) == 19_200
+    rng = np.random.default_rng(5)
+    for b in bands:
+        pick = np.sort(rng.choice(b["count"], 10, replace=False)) + b["first"]
+        periods = grid[pick]
+        dd = np.sort(d)[:b["nd"]]
+        bank = F.make_bank(cfg, dd, s.rho[np.argsort(d)][:b["nd"]])
+        for nb in (b["nb"], b["nb_coarse"]):
+            print(f"[longbase] band lo={b['lo']} nb={nb} n={len(s.t)}", flush=True)
+            got = F.scan_bank(gpu, dev, periods, bank, nb)
+            want = O.scan_bank(s.t, vq, wq, s.baseline_norm, periods, dd,
+                               s.rho[np.argsort(d)][:b["nd"]], cfg.ingress_fractions,
+                               cfg.shape_layers, cfg.shape_gate, nb)
+            assert want[4] == 0
+            for x, y in zip(got[:4], want[:4]):
+                np.testing.assert_array_equal(x, y)
+
+
+def test_binned_features_bit_exact(gpu, long_star):
+    cfg, d, s, dev, vq, wq = long_star
+    pc = PR.ProposalConfig()
+    b = L.binned_state(s)
+    bdev = F.DeviceReady.upload_packed(gpu, [b])
+    centers, _ = L.feature_bins(float(s.t[-1]), pc)
+    periods = centers[::997]
+    gd = PR.group_durations(cfg, pc.groups)
+    print(f"[longbase] features n={len(b.t)} bins={len(centers)}", flush=True)
+    got = F.coarse_features(gpu, bdev, periods, gd, pc.nb_min, pc.nb_max)
+    bv, bw, _, _ = bdev[0].fields()
+    want = O.coarse_features(b.t, bv, bw, b.baseline_norm, periods, gd, pc.nb_min,
+                             pc.nb_max)
+    np.testing.assert_array_equal(got[0], want)
+    # the injected period's bin stands out of the periodogram
+    near = np.abs(np.log(centers / 47.3)) < 2e-4
+    at = F.coarse_features(gpu, bdev, centers[near], gd, pc.nb_min, pc.nb_max)
+    assert np.nanmax(at) > 8.0
+    bdev[0].close()
+
+
+def test_grouped_bins_keep_all_is_full_search(gpu):
+    res = L.make_states(range(4), 5_200_000, workers=1)
+    states = [r[0] for r in res]
+    cfg = L.search_config()
+    d = cfg.durations_days()
+    devs = F.DeviceReady.upload_packed(gpu, states)
+    rhos = [s.rho for s in states]
+    print(f"[longbase] grouped batch sectors={[r[3] for r in res]}", flush=True)
+    rows, stats = F.signal_search_batch(gpu, devs, d, rhos, cfg)
+    pc = PR.ProposalConfig()
+    for _, idx in L.groups_of(states).items():
+        _, edges = L.feature_bins(float(states[idx[0]].t[-1]), pc)
+        keep = np.ones((len(idx), len(edges) - 1), dtype=bool)
+        brow, bstats = F.signal_search_bins(gpu, [devs[k] for k in idx], d,
+                                            [rhos[k] for k in idx], cfg, keep, edges)
+        for j, k in enumerate(idx):
+            np.testing.assert_array_equal(brow[j], rows[k])
+            assert bstats[j]["periods"] == stats[k]["periods"]
+    for dv in devs:
+        dv.close()
+
+
+def test_long_star_full_scan_recovers_transit(gpu, long_star):
+    cfg, d, s, dev, vq, wq = long_star
+    rows, grid, bands = F.signal_search(gpu, dev, d, s.rho, cfg)
+    best = rows[np.argmax(rows[:, 3])]
+    print(f"[longbase] best P={best[0]:.5f} SNR={best[3]:.2f} periods={len(grid)}",
+          flush=True)
+    assert PR.alias(best[0], 47.3) < 1e-3
