@@ -206,6 +206,48 @@ def learned_pruned(gpu, res):
     }
 
 
+def long_stars(n_stars):
+    """Fresh long-baseline stars (1-26 sectors around a one-year gap,
+    periods 0.5-100 d; seeds never used in training or selection)."""
+    from fbls_amd import longbase as LB
+
+    return LB.make_states(range(n_stars), 5_400_000, workers=16)
+
+
+def long_baseline(gpu, res):
+    """The learned-pruned search of long-baseline stars (the network
+    fine-tuned on them, at its committed operating point) against their
+    full scan."""
+    from fbls_amd import longbase as LB
+    from fbls_amd import proposal as PR
+
+    net, pc, extra = PR.load(gpu, LB.WEIGHTS)
+    op = (float(extra["tau"]), int(extra["topk"]),
+          tuple(float(v) for v in np.atleast_1d(extra["fallback"])))
+    lb = LB.LongBatch(gpu, res, net, pc)
+    summary = lb.summary()
+    r = lb.sweep([op])[0]
+    lb.close()
+    return {
+        "workload": f"{len(res)} stars, {summary['sectors_min']}-{summary['sectors_max']} "
+                    "sectors in two seasons around a 365-d gap (up to "
+                    f"{summary['points_max']} points), 30% transits + 5% eclipsing "
+                    "binaries; periods 0.5-100 d",
+        "operating_point": {"tau": op[0], "top_k_bins": op[1], "fallback_band": list(op[2])},
+        "recall_vs_full_scan": r["recall"],
+        "full_scan_recovered": r["full_recovered"],
+        "injections": summary["transits"] + summary["eclipsing_binaries"],
+        "stars_per_hour_full": r["stars_per_hour_full"],
+        "stars_per_hour_pruned": r["stars_per_hour_pruned"],
+        "speedup": r["speedup"],
+        "full_scan_trial_evals_per_s": summary["full_scan_trial_evals_per_s"],
+        "trial_evals_full": r["templates_full"],
+        "trial_evals_pruned": r["templates_pruned"],
+        "kept_period_fraction": r["kept_period_fraction"],
+        "fallback_stars": r["fallback_stars"],
+    }
+
+
 def cpu_baseline(cfg, d, state):
     import oracle as O
 
@@ -238,6 +280,8 @@ def main():
                     help="stars of the learned-pruned comparison (0 = skip)")
     ap.add_argument("--batch-stars", type=int, default=512,
                     help="stars of the batched full scan of uneven stars (0 = skip)")
+    ap.add_argument("--long-stars", type=int, default=32,
+                    help="long-baseline stars of the learned-pruned comparison (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale counters")
     args = ap.parse_args()
@@ -252,6 +296,7 @@ def main():
 
     pstars = proposal_stars(args.proposal_stars) if args.proposal_stars > 0 else None
     bstars = batch_stars(args.batch_stars) if args.batch_stars > 0 else None
+    lstars = long_stars(args.long_stars) if args.long_stars > 0 else None
     cfg, d, state = workload()
     n = len(state.t)
     gpu, dev, rows, grid, per_step, elapsed, tot = run_gpu(
@@ -317,6 +362,8 @@ def main():
         out["learned_pruned"] = learned_pruned(gpu, pstars)
     if bstars:
         out["batched"] = batched(gpu, bstars)
+    if lstars:
+        out["long_baseline"] = long_baseline(gpu, lstars)
     if not args.no_cpu:
         cb, _ = cpu_baseline(cfg, d, state)
         out["cpu_baseline"] = cb

@@ -30,12 +30,13 @@ selection, planning and scans).
 """
 import concurrent.futures as cf
 import math
+import os
 import time
 
 import numpy as np
 
-from . import (DeviceReady, FamilyState, SearchConfig, family_state, propose,
-               signal_search_batch, signal_search_bins, synth)
+from . import (DeviceReady, FamilyState, SearchConfig, coarse_inputs, family_state,
+               propose, signal_search_batch, signal_search_bins, synth)
 from . import proposal as PR
 
 PERIOD_MIN, PERIOD_MAX = 0.5, 100.0
@@ -47,16 +48,23 @@ def search_config():
 
 
 def _make_state(args):
-    i, seed0, durations, correlated = args
-    (t, f, e), kw = synth.long_baseline_star(i, seed0)
+    i, seed0, durations, correlated, mix = args
+    (t, f, e), kw = synth.long_baseline_star(i, seed0, *mix)
     s = family_state(t, f - 1.0, e, durations, correlated)
     return s, kw.get("period", 0.0), kw["kind"], kw["sectors"]
 
 
-def make_states(indices, seed0, workers=16):
-    """[(state, injected period or 0, kind, sectors)] of the given stars."""
+def _jobs(indices, seed0, mix):
     cfg = search_config()
-    jobs = [(i, seed0, cfg.durations_days(), cfg.correlated_noise) for i in indices]
+    return [(i, seed0, cfg.durations_days(), cfg.correlated_noise, mix) for i in indices]
+
+
+def make_states(indices, seed0, workers=16, mix=(0.30, 0.05), pool=None):
+    """[(state, injected period or 0, kind, sectors)] of the given stars
+    (mix: the shares of transits and eclipsing binaries)."""
+    jobs = _jobs(indices, seed0, mix)
+    if pool is not None:
+        return list(pool.map(_make_state, jobs, chunksize=2))
     if workers <= 1:
         return [_make_state(j) for j in jobs]
     import multiprocessing as mp
@@ -106,9 +114,12 @@ def groups_of(states):
     return out
 
 
-def keep_from(probs, dilate, tau):
-    """Bins with p >= tau, dilated by `dilate` bins on each side."""
+def keep_from(probs, dilate, tau, topk=0):
+    """Bins with p >= tau plus the topk most probable, dilated by `dilate`
+    bins on each side."""
     sel = probs >= tau
+    if topk:
+        sel[np.argpartition(-probs, topk - 1)[:topk]] = True
     out = sel.copy()
     for o in range(1, dilate + 1):
         out[o:] |= sel[:-o]
@@ -120,7 +131,7 @@ class LongBatch:
     """A set of long-baseline stars resident on the GPU with their full-scan
     results; evaluates the learned-pruned search at operating points."""
 
-    def __init__(self, gpu, res, net, pc, scan_chunk=32):
+    def __init__(self, gpu, res, net, pc, scan_chunk=32, log=None):
         self.gpu, self.net, self.pc = gpu, net, pc
         self.cfg = search_config()
         self.states = [r[0] for r in res]
@@ -141,6 +152,9 @@ class LongBatch:
             best += PR.best_rows(rows, self.cfg.threshold)
             templates += sum(st["templates_scanned"] for st in stats)
             periods += sum(st["periods"] for st in stats)
+            if log:
+                log(f"full scan {min(len(self.devs), c0 + scan_chunk)} of {len(self.devs)} "
+                    f"stars ({time.perf_counter() - t0:.1f} s)")
         self.t_full = time.perf_counter() - t0
         self.templates_full, self.periods_full = templates, periods
         self.best_full = best
@@ -170,15 +184,16 @@ class LongBatch:
             b.close()
         return probs, edges, dt
 
-    def pruned(self, probs, edges, t_prop, tau, fallback):
-        """The bin-restricted search of every group at (tau, fallback band),
-        timed from bin selection to the rows."""
+    def pruned(self, probs, edges, t_prop, tau, topk, fallback):
+        """The bin-restricted search of every group at (tau, top-k, fallback
+        band), timed from bin selection to the rows."""
         lo, hi = fallback
         t0 = time.perf_counter()
         best = [None] * len(self.states)
         templates = periods = nfall = 0
         for _, idx in self.groups.items():
-            keep = np.stack([keep_from(probs[k], self.pc.keep_dilate, tau) for k in idx])
+            keep = np.stack([keep_from(probs[k], self.pc.keep_dilate, tau, topk)
+                             for k in idx])
             for j, k in enumerate(idx):
                 conf = float(probs[k].max())
                 if lo <= conf < hi:
@@ -196,7 +211,7 @@ class LongBatch:
         n = len(self.states)
         eb = np.array([k == "eb" for k in self.kinds])
         return {
-            "stars": n, "tau": tau, "fallback": list(fallback),
+            "stars": n, "tau": tau, "topk": topk, "fallback": list(fallback),
             "full_recovered": int(self.rf.sum()),
             "pruned_recovered": int((self.rf & rp).sum()),
             "recall": float((self.rf & rp).sum() / max(self.rf.sum(), 1)),
@@ -214,7 +229,7 @@ class LongBatch:
 
     def sweep(self, points):
         probs, edges, t_prop = self.proposal()
-        return [self.pruned(probs, edges, t_prop, tau, fb) for tau, fb in points]
+        return [self.pruned(probs, edges, t_prop, tau, k, fb) for tau, k, fb in points]
 
     def summary(self):
         inj = np.array([p > 0 for p in self.injected])
@@ -236,18 +251,96 @@ class LongBatch:
             d.close()
 
 
-POINTS = [(tau, fb) for tau in (0.02, 0.05, 0.1) for fb in ((0.0, 0.0), (0.005, 0.1))]
+POINTS = [(tau, k, fb) for tau in (0.02, 0.05, 0.1, 0.2) for k in (0, 8)
+          for fb in ((0.0, 0.0), (0.005, 0.1))]
+WEIGHTS = os.path.join(PR.HERE, "data", "proposal_long.npz")
+REPORT = os.path.join(PR.HERE, "data", "longbase_report.json")
+ALIASES = (1 / 3, 0.5, 1.0, 2.0, 3.0)
+
+
+def training_set(gpu, pc, indices, seed0, pool, log, crop=2048, negatives=2,
+                 mix=(0.6, 0.1), chunk=96, seed=0):
+    """Network inputs and labels for fine-tuning on long-baseline stars:
+    crops of `crop` bins of each star's coarse periodogram (one around its
+    label, `negatives` at random offsets).  Labels come from the GPU scan
+    itself, restricted to the bins around the injected period and its
+    aliases (P/3 ... 3P): the best row >= threshold there that is an alias
+    of the injection marks its bin +- label_dilate (the single-sector
+    labels mark the full scan's best row the same way)."""
+    cfg = search_config()
+    d = cfg.durations_days()
+    gd = PR.group_durations(cfg, pc.groups)
+    rng = np.random.default_rng(seed)
+    xs, ys = [], []
+    spans = [indices[c:c + chunk] for c in range(0, len(indices), chunk)]
+
+    def submit(span):
+        return pool.map(_make_state, _jobs(span, seed0, mix), chunksize=2)
+
+    pending = submit(spans[0])
+    for k, span in enumerate(spans):
+        res = list(pending)
+        pending = submit(spans[k + 1]) if k + 1 < len(spans) else None
+        states = [r[0] for r in res]
+        devs = DeviceReady.upload_packed(gpu, states)
+        bdevs = DeviceReady.upload_packed(gpu, [binned_state(st) for st in states])
+        for t_last, idx in groups_of(states).items():
+            centers, edges = feature_bins(t_last, pc)
+            x = np.empty((len(idx), len(centers), pc.groups), dtype=np.float32)
+            per = max(1, (1 << 24) // len(centers))
+            for c0 in range(0, len(idx), per):
+                part = idx[c0:c0 + per]
+                x[c0:c0 + len(part)] = coarse_inputs(gpu, [bdevs[j] for j in part], centers,
+                                                     gd, pc.nb_min, pc.nb_max)
+            nbin = len(centers)
+            keep = np.zeros((len(idx), nbin), dtype=bool)
+            for j, i in enumerate(idx):
+                p = res[i][1]
+                if p > 0:
+                    for m in ALIASES:
+                        b = int(np.searchsorted(edges, p * m, side="right")) - 1
+                        if 0 <= b < nbin:
+                            keep[j, max(0, b - 3):b + 4] = True
+            rows, _ = signal_search_bins(gpu, [devs[i] for i in idx], d,
+                                         [states[i].rho for i in idx], cfg, keep, edges)
+            best = PR.best_rows(rows, cfg.threshold)
+            for j, i in enumerate(idx):
+                y = np.zeros(nbin, dtype=np.float32)
+                p_inj, (p_best, _) = res[i][1], best[j]
+                lab = -1
+                if p_inj > 0 and p_best == p_best and PR.alias(p_best, p_inj) < 1e-3:
+                    lab = int(np.searchsorted(edges, p_best, side="right")) - 1
+                    y[max(0, lab - pc.label_dilate):lab + pc.label_dilate + 1] = 1.0
+                starts = [int(rng.integers(0, nbin - crop + 1)) for _ in range(negatives)]
+                if lab >= 0:  # a crop holding the label bin and its dilation
+                    lo = min(nbin - crop, max(0, lab - crop + 1 + pc.label_dilate))
+                    hi = max(lo, min(nbin - crop, lab - pc.label_dilate))
+                    starts.append(int(rng.integers(lo, hi + 1)))
+                for s0 in starts:
+                    xs.append(x[j, s0:s0 + crop].copy())
+                    ys.append(y[s0:s0 + crop].copy())
+        for dv in devs + bdevs:
+            dv.close()
+        log(f"training stars {span[-1] + 1 - indices[0]} of {len(indices)}, crops {len(xs)}, "
+            f"positive crops {int(sum(v.max() > 0 for v in ys))}")
+    return np.stack(xs), np.stack(ys)
 
 
 def main():
     import argparse
     import json
+    import multiprocessing as mp
 
     from . import Gpu
 
     ap = argparse.ArgumentParser(description="long-baseline learned-pruned batch")
+    ap.add_argument("--train", type=int, default=0,
+                    help="fine-tune the single-sector network on this many stars first")
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--validate", type=int, default=48)
     ap.add_argument("--heldout", type=int, default=96)
+    ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--report", default=None)
     ap.add_argument("--workers", type=int, default=16)
     args = ap.parse_args()
@@ -255,29 +348,56 @@ def main():
     def log(msg):
         print(f"[longbase] {msg}", flush=True)
 
-    vres = make_states(range(args.validate), 5_100_000, args.workers)
-    hres = make_states(range(args.heldout), 5_000_000, args.workers)
-    log(f"prepared {len(vres)} + {len(hres)} stars")
+    pool = cf.ProcessPoolExecutor(args.workers, mp_context=mp.get_context("spawn"))
     gpu = Gpu(0)
-    net, pc, _ = PR.load(gpu)
-    val = LongBatch(gpu, vres, net, pc)
+    t0 = time.perf_counter()
+    train_info = {}
+    if args.train:
+        net, pc, _ = PR.load(gpu)  # start from the single-sector network
+        x, y = training_set(gpu, pc, list(range(args.train)), 5_300_000, pool, log)
+        log(f"training crops {len(x)} ({time.perf_counter() - t0:.1f} s)")
+        t1 = time.perf_counter()
+        losses = PR.train(net, x, y, epochs=args.epochs, lr=args.lr, seed=1, log=log)
+        train_info = {"train_stars": args.train, "train_crops": int(len(x)),
+                      "positive_crops": int((y.max(axis=1) > 0).sum()),
+                      "epochs": args.epochs, "lr": args.lr,
+                      "train_seconds": time.perf_counter() - t1,
+                      "final_loss": float(np.mean(losses[-50:]))}
+        del x, y
+        PR.save(net, pc, args.out)
+        log(f"fine-tuned {train_info}")
+    else:
+        path = args.out if os.path.exists(args.out) else PR.DEFAULT_WEIGHTS
+        net, pc, _ = PR.load(gpu, path)
+        if path == WEIGHTS and os.path.exists(REPORT):  # the committed network's training
+            train_info = json.load(open(REPORT)).get("training", {})
+    vres = make_states(range(args.validate), 5_100_000, pool=pool)
+    hres = make_states(range(args.heldout), 5_000_000, pool=pool)
+    pool.shutdown()
+    log(f"prepared {len(vres)} + {len(hres)} stars")
+    val = LongBatch(gpu, vres, net, pc, log=log)
     log(f"validation full scan {val.summary()}")
     vs = val.sweep(POINTS)
     val.close()
     for r in vs:
-        log(f"validation {r}")
-    point = PR.fastest(vs, 0.99) or max(vs, key=lambda r: r["recall"])
-    op = (point["tau"], tuple(point["fallback"]))
-    held = LongBatch(gpu, hres, net, pc)
+        log(f"validation {r['tau']} {r['topk']} {r['fallback']}: recall {r['recall']:.4f} "
+            f"speedup {r['speedup']:.2f}")
+    point = PR.fastest(vs, 0.99) or max(vs, key=lambda r: (r["recall"], r["speedup"]))
+    op = (point["tau"], point["topk"], tuple(point["fallback"]))
+    held = LongBatch(gpu, hres, net, pc, log=log)
     summary = held.summary()
     log(f"held-out full scan {summary}")
     hs = held.sweep([op] + [p for p in POINTS if p != op])
     held.close()
-    report = {"operating_point": {"tau": op[0], "fallback": list(op[1])},
+    report = {"training": train_info,
+              "operating_point": {"tau": op[0], "topk": op[1], "fallback": list(op[2])},
               "validation_stars": len(vres), "validation_at_operating_point": point,
               "heldout": summary, "heldout_at_operating_point": hs[0],
               "heldout_sweep": hs}
     log(json.dumps(hs[0]))
+    if args.train:
+        PR.save(net, pc, args.out, tau=op[0], topk=op[1],
+                fallback=np.array(op[2], dtype=float))
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1)

@@ -55,17 +55,18 @@ def _duration_days(p, p_max):
             / (p_max ** (1 / 3) - 0.5 ** (1 / 3))) / 24.0
 
 
-def long_baseline_star(i, seed0=5_000_000):
+def long_baseline_star(i, seed0=5_000_000, transit=0.30, eb=0.05):
     """Star i of the long-baseline batch: 1-26 sectors in two seasons of
     ceil(k/2) and floor(k/2) contiguous 27.4-d sectors separated by a
     365-d gap (up to ~510k points over ~1080 d); sigma log-uniform
     200-3000 ppm of which an AR(1) (phi = 0.9) share uniform in [0, 0.5] of
-    the variance; 30 % carry a box (P log-uniform 0.5-100 d, depth
-    log-uniform 200-10000 ppm, duration 1-8 h growing as P^(1/3)), 5 % an
-    eclipsing-binary-like pair (primary depth log-uniform 2000-30000 ppm,
-    secondary at phase 0.5 with a depth ratio uniform in [0.1, 0.9], the
-    same period and duration laws).  Returns ((time, flux, flux_err), kw)
-    with kw["kind"] in {"none", "transit", "eb"}."""
+    the variance; a share `transit` (30 %) carries a box (P log-uniform
+    0.5-100 d, depth log-uniform 200-10000 ppm, duration 1-8 h growing as
+    P^(1/3)), a share `eb` (5 %) an eclipsing-binary-like pair (primary
+    depth log-uniform 2000-30000 ppm, secondary at phase 0.5 with a depth
+    ratio uniform in [0.1, 0.9], the same period and duration laws).
+    Returns ((time, flux, flux_err), kw) with kw["kind"] in
+    {"none", "transit", "eb"}."""
     rng = np.random.default_rng(seed0 + i)
     sectors = int(rng.integers(1, 27))
     sigma = float(np.exp(rng.uniform(np.log(200e-6), np.log(3000e-6))))
@@ -75,14 +76,14 @@ def long_baseline_star(i, seed0=5_000_000):
               baseline_gap_days=365.0 if sectors > 1 else 0.0)
     u = rng.uniform()
     p = float(np.exp(rng.uniform(np.log(0.5), np.log(100.0))))
-    if u < 0.30:
+    if u < transit:
         depth = float(np.exp(rng.uniform(np.log(200e-6), np.log(1e-2))))
         kw.update(period=p, depth=depth, duration=_duration_days(p, 100.0))
-    elif u < 0.35:
+    elif u < transit + eb:
         depth = float(np.exp(rng.uniform(np.log(2e-3), np.log(3e-2))))
         kw.update(period=p, depth=depth, duration=_duration_days(p, 100.0),
                   secondary_depth=depth * float(rng.uniform(0.1, 0.9)))
-    kind = "none" if u >= 0.35 else ("transit" if u < 0.30 else "eb")
+    kind = "none" if u >= transit + eb else ("transit" if u < transit else "eb")
     return synth_star(**kw), dict(kw, kind=kind)
 
 

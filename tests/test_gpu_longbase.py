@@ -118,3 +118,22 @@ def test_long_star_full_scan_recovers_transit(gpu, long_star):
     print(f"[longbase] best P={best[0]:.5f} SNR={best[3]:.2f} periods={len(grid)}",
           flush=True)
     assert PR.alias(best[0], 47.3) < 1e-3
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(L.WEIGHTS),
+                    reason="no fine-tuned long-baseline proposal")
+def test_long_proposal_keeps_recoveries(gpu):
+    """The committed long-baseline network at its operating point, on fresh
+    stars with injections: the pruned search keeps the full scan's
+    recoveries while scanning a small share of the periods."""
+    net, pc, extra = PR.load(gpu, L.WEIGHTS)
+    res = L.make_states(range(10), 5_950_000, workers=1, mix=(0.8, 0.2))
+    lb = L.LongBatch(gpu, res, net, pc)
+    op = (float(extra["tau"]), int(extra["topk"]),
+          tuple(float(v) for v in np.atleast_1d(extra["fallback"])))
+    r = lb.sweep([op])[0]
+    lb.close()
+    print("[longbase] trained proposal", r, flush=True)
+    assert r["full_recovered"] >= 4
+    assert r["recall"] >= 0.75
+    assert r["kept_period_fraction"] < 0.1
