@@ -227,9 +227,16 @@ class LongBatch:
             "speedup": self.t_full / (t_prop + t_pruned),
         }
 
-    def sweep(self, points):
+    def sweep(self, points, log=None):
         probs, edges, t_prop = self.proposal()
-        return [self.pruned(probs, edges, t_prop, tau, k, fb) for tau, k, fb in points]
+        out = []
+        for tau, k, fb in points:
+            out.append(self.pruned(probs, edges, t_prop, tau, k, fb))
+            if log:
+                r = out[-1]
+                log(f"tau {tau} top-k {k} fallback {list(fb)}: recall {r['recall']:.4f} "
+                    f"speedup {r['speedup']:.2f}")
+        return out
 
     def summary(self):
         inj = np.array([p > 0 for p in self.injected])
@@ -377,17 +384,17 @@ def main():
     log(f"prepared {len(vres)} + {len(hres)} stars")
     val = LongBatch(gpu, vres, net, pc, log=log)
     log(f"validation full scan {val.summary()}")
-    vs = val.sweep(POINTS)
+    vs = val.sweep(POINTS, log=lambda m: log("validation " + m))
     val.close()
-    for r in vs:
-        log(f"validation {r['tau']} {r['topk']} {r['fallback']}: recall {r['recall']:.4f} "
-            f"speedup {r['speedup']:.2f}")
     point = PR.fastest(vs, 0.99) or max(vs, key=lambda r: (r["recall"], r["speedup"]))
     op = (point["tau"], point["topk"], tuple(point["fallback"]))
     held = LongBatch(gpu, hres, net, pc, log=log)
     summary = held.summary()
     log(f"held-out full scan {summary}")
-    hs = held.sweep([op] + [p for p in POINTS if p != op])
+    # the held-out sweep: the operating point and the points without a
+    # fallback band (those with one rescan their fallback stars in full)
+    hs = held.sweep([op] + [p for p in POINTS if p != op and p[2] == (0.0, 0.0)],
+                    log=lambda m: log("held-out " + m))
     held.close()
     report = {"training": train_info,
               "operating_point": {"tau": op[0], "topk": op[1], "fallback": list(op[2])},
