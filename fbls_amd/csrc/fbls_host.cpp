@@ -329,6 +329,7 @@ struct Work {
   uint32_t* winner;   // np x 4 or null
   uint32_t retries = 0;
   double cost = 0.0;  // work estimate used to apportion launch time
+  bool score_only = false;  // the caller reads only score (records, depth stay on the device)
 };
 
 struct Shared {
@@ -519,10 +520,18 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   HIP_TRY(fbls::scan_launch(A, groups, block, lds_max, split, c->stream));
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
-  bool any_winner = false;
-  for (size_t i = 0; i < nw; i++) any_winner = any_winner || (w[i].winner && w[i].np);
-  const size_t copy_bytes = any_winner ? out_bytes : win_base;
-  HIP_TRY(hipMemcpyAsync(c->pin_out.ptr, dout, copy_bytes, hipMemcpyDeviceToHost, c->stream));
+  bool any_winner = false, all_score = true;
+  for (size_t i = 0; i < nw; i++) {
+    any_winner = any_winner || (w[i].winner && w[i].np);
+    all_score = all_score && (w[i].score_only || !w[i].np);
+  }
+  if (all_score) {  // a prescan launch: only the scores come back (8 of 88 B per period)
+    HIP_TRY(hipMemcpyAsync((unsigned char*)c->pin_out.ptr + score_base, dout + score_base,
+                           total * 8, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    const size_t copy_bytes = any_winner ? out_bytes : win_base;
+    HIP_TRY(hipMemcpyAsync(c->pin_out.ptr, dout, copy_bytes, hipMemcpyDeviceToHost, c->stream));
+  }
   HIP_TRY(hipMemcpyAsync(hin + cnt_off, din + cnt_off, nseg * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   float ms = 0.0f;
@@ -540,9 +549,11 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   out_at = 0;
   for (size_t s = 0; s < segs.size(); s++) {
     Work& x = w[seg_work[s]];
-    std::memcpy(x.records, (const double*)(hout + rec_base) + out_at * 9, x.np * 72);
     std::memcpy(x.score, (const double*)(hout + score_base) + out_at, x.np * 8);
-    std::memcpy(x.depth, (const double*)(hout + depth_base) + out_at, x.np * 8);
+    if (!x.score_only) {  // (a launch with any such work copies everything back)
+      std::memcpy(x.records, (const double*)(hout + rec_base) + out_at * 9, x.np * 72);
+      std::memcpy(x.depth, (const double*)(hout + depth_base) + out_at, x.np * 8);
+    }
     if (x.winner)
       std::memcpy(x.winner, (const uint32_t*)(hout + win_base) + out_at * 4, x.np * 16);
     bad = bad || cnt[2 * s] != 0;
@@ -688,9 +699,13 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
     if (bank_batch(nb0, b->period_batch, b->max_fold_mib) == 0 ||
         bank_batch(r.nb, b->period_batch, b->max_fold_mib) == 0)
       return fail(FBLS_E_FOLD_BUDGET, "one folded period exceeds max_fold_mib");
-    r.rec.resize(r.np * 9);
+    // a prescan reads only the scores; records and depths are the rows of a
+    // band without prescan
+    if (!r.nb_coarse) {
+      r.rec.resize(r.np * 9);
+      r.depth.resize(r.np);
+    }
     r.score.resize(r.np);
-    r.depth.resize(r.np);
     Work x;
     x.fam = r.fam;
     x.periods = r.periods;
@@ -703,6 +718,7 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
     x.score = r.score.data();
     x.depth = r.depth.data();
     x.winner = nullptr;
+    x.score_only = r.nb_coarse != 0;
     x.cost = work_cost(x);
     first.push_back(x);
     first_band.push_back(i);
@@ -722,7 +738,7 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
     if (!r.nb_coarse) continue;
     r.survivors.clear();
     for (size_t i = 0; i < r.np; i++)  // scan.rs:390-395
-      if (std::isfinite(r.score[i]) && r.score[i] >= cut) r.survivors.push_back(r.rec[9 * i]);
+      if (std::isfinite(r.score[i]) && r.score[i] >= cut) r.survivors.push_back(r.periods[i]);
     r.stats.prescan_survivors = r.survivors.size();
     if (r.survivors.empty()) continue;
     const size_t m = r.survivors.size();

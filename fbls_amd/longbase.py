@@ -258,8 +258,11 @@ class LongBatch:
             d.close()
 
 
-POINTS = [(tau, k, fb) for tau in (0.02, 0.05, 0.1, 0.2) for k in (0, 8)
-          for fb in ((0.0, 0.0), (0.005, 0.1))]
+# operating points: (tau, top-k, fallback band); the validation sweep tries
+# those without a fallback band (a star with one is rescanned in full, which
+# costs 2-4x the pruned search), the held-out report adds one with it
+POINTS = [(tau, k, (0.0, 0.0)) for tau in (0.02, 0.05, 0.1, 0.2) for k in (0, 8)]
+FALLBACK_POINT = (0.05, 8, (0.005, 0.1))
 WEIGHTS = os.path.join(PR.HERE, "data", "proposal_long.npz")
 REPORT = os.path.join(PR.HERE, "data", "longbase_report.json")
 ALIASES = (1 / 3, 0.5, 1.0, 2.0, 3.0)
@@ -345,11 +348,13 @@ def main():
                     help="fine-tune the single-sector network on this many stars first")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--validate", type=int, default=48)
-    ap.add_argument("--heldout", type=int, default=96)
+    ap.add_argument("--validate", type=int, default=320)
+    ap.add_argument("--heldout", type=int, default=480)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--report", default=None)
     ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--skip-fallback", action="store_true",
+                    help="leave the fallback-band point out of the held-out sweep")
     args = ap.parse_args()
 
     def log(msg):
@@ -376,7 +381,7 @@ def main():
     else:
         path = args.out if os.path.exists(args.out) else PR.DEFAULT_WEIGHTS
         net, pc, _ = PR.load(gpu, path)
-        if path == WEIGHTS and os.path.exists(REPORT):  # the committed network's training
+        if os.path.exists(REPORT):  # the training of the committed network
             train_info = json.load(open(REPORT)).get("training", {})
     vres = make_states(range(args.validate), 5_100_000, pool=pool)
     hres = make_states(range(args.heldout), 5_000_000, pool=pool)
@@ -391,9 +396,8 @@ def main():
     held = LongBatch(gpu, hres, net, pc, log=log)
     summary = held.summary()
     log(f"held-out full scan {summary}")
-    # the held-out sweep: the operating point and the points without a
-    # fallback band (those with one rescan their fallback stars in full)
-    hs = held.sweep([op] + [p for p in POINTS if p != op and p[2] == (0.0, 0.0)],
+    extra = [] if args.skip_fallback else [FALLBACK_POINT]
+    hs = held.sweep([op] + [p for p in POINTS + extra if p != op],
                     log=lambda m: log("held-out " + m))
     held.close()
     report = {"training": train_info,
@@ -402,9 +406,7 @@ def main():
               "heldout": summary, "heldout_at_operating_point": hs[0],
               "heldout_sweep": hs}
     log(json.dumps(hs[0]))
-    if args.train:
-        PR.save(net, pc, args.out, tau=op[0], topk=op[1],
-                fallback=np.array(op[2], dtype=float))
+    PR.save(net, pc, args.out, tau=op[0], topk=op[1], fallback=np.array(op[2], dtype=float))
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1)
