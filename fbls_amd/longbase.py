@@ -391,7 +391,11 @@ def main():
     log(f"validation full scan {val.summary()}")
     vs = val.sweep(POINTS, log=lambda m: log("validation " + m))
     val.close()
-    point = PR.fastest(vs, 0.99) or max(vs, key=lambda r: (r["recall"], r["speedup"]))
+    point = PR.fastest(vs, 0.99)
+    if point is None:  # none reaches 99 %: the most cautious of the best-recall points
+        top = max(r["recall"] for r in vs)
+        point = max((r for r in vs if r["recall"] == top),
+                    key=lambda r: r["kept_period_fraction"])
     op = (point["tau"], point["topk"], tuple(point["fallback"]))
     held = LongBatch(gpu, hres, net, pc, log=log)
     summary = held.summary()
