@@ -109,6 +109,7 @@ extern "C" int fbls_ctx_set_launch(fbls_ctx* c, uint32_t lds_bytes,
   c->lds_override = lds_bytes;
   c->block_override = block;
   c->run_fold = fold_mode == 2 ? 0u : 1u;
+  c->count_from_weights = fold_mode == 0 ? 1u : 0u;
   return FBLS_OK;
 }
 
@@ -167,6 +168,9 @@ int prepare_family(fbls_ctx* c, const double* t, const double* v,
   f->ew = fbls::quantize(w, n, f->wq.data());
   f->sorted = true;
   for (size_t i = 1; i < n; i++) f->sorted = f->sorted && t[i] >= t[i - 1];
+  f->w_uniform = f->wq[0] > 0 ? f->wq[0] : 0;
+  for (size_t i = 1; i < n && f->w_uniform; i++)
+    if (f->wq[i] != f->w_uniform) f->w_uniform = 0;
   return FBLS_OK;
 }
 
@@ -494,6 +498,7 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
     S.G = G;
     S.wide_cells = (f->t_max * smax >= 16777215.0) ? 1u : 0u;
     S.sorted = f->sorted ? 1u : 0u;
+    S.uniform_w = c->count_from_weights ? f->w_uniform : 0;
     S.group0 = groups;
     S.index = (uint32_t)segs.size();
     groups += (uint32_t)((x.np + G - 1) / G);

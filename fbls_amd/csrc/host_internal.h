@@ -104,6 +104,7 @@ struct fbls_ctx {
   uint32_t lds_override = 0, block_override = 0;
   size_t max_lds = fbls::host::kMaxLds;
   uint32_t run_fold = 1;
+  uint32_t count_from_weights = 1;  // equal weights: counts = W / w (no count atomics)
   fbls_launch_totals totals = {0, 0, 0, 0.0};
   int use_graphs = 1;                  // evidence calls replay a hipGraph
   hipGraphExec_t ev_exec = nullptr;    // the evidence call's executable graph
@@ -121,5 +122,6 @@ struct fbls_family {
   double C = 0.0, origin = 0.0, t_max = 0.0, t_last = 0.0;
   int ev = 0, ew = 0;
   bool sorted = false;   // times ascend
+  int32_t w_uniform = 0; // > 0: every fixed-point weight equals it
   std::vector<int32_t> vq, wq;
 };

@@ -30,7 +30,8 @@ struct Segment {
   uint32_t range_bins;    // range-split kernel: bins per range (x128)
   uint32_t halo_bins;     // range-split kernel: halo >= the longest template
   uint32_t sorted;        // 1: times ascend (incremental phase bins)
-  uint32_t pad_;
+  int32_t uniform_w;      // > 0: every fixed-point weight equals it; the fold
+                          // skips the count atomics, each count being W / uniform_w
 };
 
 struct LaunchArgs {

@@ -306,17 +306,25 @@ __device__ __forceinline__ uint32_t phase_bin(uint32_t u, uint32_t nb,
   return min(r, r - nb);
 }
 
-// UNI skips the weight field (for callers that rebuild it from the counts).
-template <bool UNI>
+// NOCNT skips the count field (equal weights: the counts are rebuilt from
+// the weight sums after the fold).
+template <bool NOCNT>
 __device__ __forceinline__ void lds_add(double* vw, uint32_t* n, uint32_t b,
                                         double v, double w, uint32_t c) {
   __hip_atomic_fetch_add(vw + 2 * (b + 1), v, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
-  if (!UNI)
-    __hip_atomic_fetch_add(vw + 2 * (b + 1) + 1, w, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
-  __hip_atomic_fetch_add(n + b + 1, c, __ATOMIC_RELAXED,
+  __hip_atomic_fetch_add(vw + 2 * (b + 1) + 1, w, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (!NOCNT)
+    __hip_atomic_fetch_add(n + b + 1, c, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Equal fixed-point weights w0: a bin's count is its weight sum / w0, exact
+// (the sum is count * w0, an integer below 2^53, and IEEE division of an
+// exact multiple returns the integer).
+__device__ __forceinline__ uint32_t count_of(double wsum, double w0) {
+  return (uint32_t)(wsum / w0);
 }
 
 // Fold one lane's 8 consecutive observations into one period's histogram.
@@ -324,7 +332,7 @@ __device__ __forceinline__ void lds_add(double* vw, uint32_t* n, uint32_t b,
 // registers and flushed once (exact: integer-valued doubles).  INCR (times
 // ascend and the lane's 8 cells span fewer than nb): each bin is the previous
 // one advanced by the difference of floor(t * scale), wrapped once.
-template <bool WIDE, bool RUN, bool FULL, bool UNI, bool INCR>
+template <bool WIDE, bool RUN, bool FULL, bool NOCNT, bool INCR>
 __device__ __forceinline__ void fold_lane(double* vw, uint32_t* n,
                                           const uint32_t* u, const double* vv,
                                           const double* ww, uint32_t nb,
@@ -332,7 +340,7 @@ __device__ __forceinline__ void fold_lane(double* vw, uint32_t* n,
                                           uint32_t nobs) {
   if (RUN && FULL) {
     uint32_t cb = phase_bin<WIDE>(u[0], nb, inv_lo);
-    double av = vv[0], aw = UNI ? 0.0 : ww[0];
+    double av = vv[0], aw = ww[0];
     uint32_t an = 1;
 #pragma unroll
     for (uint32_t r = 1; r < kObsPerLane; r++) {
@@ -347,22 +355,20 @@ __device__ __forceinline__ void fold_lane(double* vw, uint32_t* n,
         b = phase_bin<WIDE>(u[r], nb, inv_lo);
         flush = b != cb;
       }
-      if (flush) lds_add<UNI>(vw, n, cb, av, aw, an);
+      if (flush) lds_add<NOCNT>(vw, n, cb, av, aw, an);
       const double sv = av + vv[r];
       av = flush ? vv[r] : sv;
-      if (!UNI) {
-        const double sw = aw + ww[r];
-        aw = flush ? ww[r] : sw;
-      }
-      an = flush ? 1u : an + 1u;
+      const double sw = aw + ww[r];
+      aw = flush ? ww[r] : sw;
+      if (!NOCNT) an = flush ? 1u : an + 1u;
       cb = b;
     }
-    lds_add<UNI>(vw, n, cb, av, aw, an);
+    lds_add<NOCNT>(vw, n, cb, av, aw, an);
   } else {
 #pragma unroll
     for (uint32_t r = 0; r < kObsPerLane; r++) {
       const uint32_t b = phase_bin<WIDE>(u[r], nb, inv_lo);
-      if (FULL || base + r < nobs) lds_add<UNI>(vw, n, b, vv[r], ww[r], 1u);
+      if (FULL || base + r < nobs) lds_add<NOCNT>(vw, n, b, vv[r], ww[r], 1u);
     }
   }
 }
@@ -388,7 +394,7 @@ __device__ __forceinline__ float inv_nb_low(uint32_t nb) {
   return (float)((1.0 / (double)nb) * (1.0 - 0x1p-22));
 }
 
-template <bool WIDE, bool RUN, bool UNI>
+template <bool WIDE, bool RUN, bool NOCNT>
 __device__ void fold_all(const Segment& S, unsigned char* hist,
                          uint32_t p0, uint32_t gcount, uint32_t wave,
                          uint32_t lane, uint32_t nwaves) {
@@ -417,18 +423,18 @@ __device__ void fold_all(const Segment& S, unsigned char* hist,
 #pragma unroll
       for (uint32_t r = 0; r < kObsPerLane; r++) u[r] = phase_cell_u(tt[r], scale);
       if (!full) {
-        fold_lane<WIDE, RUN, false, UNI, false>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
+        fold_lane<WIDE, RUN, false, NOCNT, false>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
       } else if (RUN && S.sorted && __all(u[kObsPerLane - 1] - u[0] < nb)) {
-        fold_lane<WIDE, RUN, true, UNI, true>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
+        fold_lane<WIDE, RUN, true, NOCNT, true>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
       } else {
-        fold_lane<WIDE, RUN, true, UNI, false>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
+        fold_lane<WIDE, RUN, true, NOCNT, false>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
       }
     }
   }
 }
 
 // Fold into the window [x0, x0 + wlen) of the bins (mod nb) of one period.
-template <bool WIDE>
+template <bool WIDE, bool NOCNT>
 __device__ void fold_window(const Segment& S, double* vw, uint32_t* n,
                             double scale, uint32_t x0, uint32_t wlen,
                             uint32_t wave, uint32_t lane, uint32_t nwaves) {
@@ -443,7 +449,7 @@ __device__ void fold_window(const Segment& S, double* vw, uint32_t* n,
     for (uint32_t r = 0; r < kObsPerLane; r++) {
       const uint32_t b = phase_bin<WIDE>(phase_cell_u(tt[r], scale), nb, inv_lo);
       const uint32_t rel = b >= x0 ? b - x0 : b + nb - x0;
-      if (rel < wlen && base + r < S.n) lds_add<false>(vw, n, rel, vv[r], ww[r], 1u);
+      if (rel < wlen && base + r < S.n) lds_add<NOCNT>(vw, n, rel, vv[r], ww[r], 1u);
     }
   }
 }
@@ -781,14 +787,27 @@ void fused_scan(LaunchArgs A) {
   __syncthreads();
 
   // ---- phase 1: fold every observation into the G histograms
+  const bool nocnt = A.run_fold && S.uniform_w > 0;
   if (S.wide_cells) {
     if (!A.run_fold) fold_all<true, false, false>(S, hist, p0, gcount, wave, lane, kWaves);
+    else if (nocnt) fold_all<true, true, true>(S, hist, p0, gcount, wave, lane, kWaves);
     else fold_all<true, true, false>(S, hist, p0, gcount, wave, lane, kWaves);
   } else {
     if (!A.run_fold) fold_all<false, false, false>(S, hist, p0, gcount, wave, lane, kWaves);
+    else if (nocnt) fold_all<false, true, true>(S, hist, p0, gcount, wave, lane, kWaves);
     else fold_all<false, true, false>(S, hist, p0, gcount, wave, lane, kWaves);
   }
   __syncthreads();
+  if (nocnt) {  // counts from the weight sums
+    const double w0 = (double)S.uniform_w;
+    for (uint32_t i = tid; i < gcount * nb; i += BLOCK) {
+      const uint32_t g = i / nb, b = i - g * nb;
+      const double* vw = (const double*)(hist + (size_t)g * S.pstride);
+      uint32_t* n = (uint32_t*)(vw + 2 * nb1);
+      n[b + 1] = count_of(vw[2 * (b + 1) + 1], w0);
+    }
+    __syncthreads();
+  }
 
   // ---- phase 2: exclusive prefix sums in place (exact)
   for (uint32_t item = wave; item < 2 * gcount; item += kWaves) {
@@ -950,11 +969,20 @@ __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
       for (uint32_t i = tid; i < slots; i += BLOCK) wn[i] = 0;
     }
     __syncthreads();
-    if (S.wide_cells)
-      fold_window<true>(S, (double*)wvw, wn, scale, x0, wlen, wave, lane, kWaves);
-    else
-      fold_window<false>(S, (double*)wvw, wn, scale, x0, wlen, wave, lane, kWaves);
+    const bool nocnt = A.run_fold && S.uniform_w > 0;
+    if (S.wide_cells) {
+      if (nocnt) fold_window<true, true>(S, (double*)wvw, wn, scale, x0, wlen, wave, lane, kWaves);
+      else fold_window<true, false>(S, (double*)wvw, wn, scale, x0, wlen, wave, lane, kWaves);
+    } else {
+      if (nocnt) fold_window<false, true>(S, (double*)wvw, wn, scale, x0, wlen, wave, lane, kWaves);
+      else fold_window<false, false>(S, (double*)wvw, wn, scale, x0, wlen, wave, lane, kWaves);
+    }
     __syncthreads();
+    if (nocnt) {
+      const double w0 = (double)S.uniform_w;
+      for (uint32_t i = tid; i < wlen; i += BLOCK) wn[i + 1] = count_of(wvw[i + 1].y, w0);
+      __syncthreads();
+    }
     if (wave == 0) wave_prefix_vw(wvw, wlen, lane, S.sv, S.sw);
     if (wave == 1 % kWaves) wave_prefix_n(wn, wlen, lane);
     __syncthreads();

@@ -89,9 +89,11 @@ void fbls_ctx_destroy(fbls_ctx* ctx);
 /* `Gpu::memory` (scan.rs:80-90) */
 int fbls_mem_info(fbls_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 /* Launch shape override for tuning: LDS bytes per workgroup (0 = auto),
- * workgroup size (256 or 512, 0 = auto) and fold mode (0 = auto, 1 =
- * run-length pre-aggregation in registers, 2 = one LDS atomic per
- * observation and period).  Results never depend on them. */
+ * workgroup size (256 or 512, 0 = auto) and fold mode (0 = auto:
+ * run-length pre-aggregation in registers, and a family whose fixed-point
+ * weights are all equal folds no counts, each bin's count being its weight
+ * sum divided by that weight; 1 = run-length, counts always folded; 2 = one
+ * LDS atomic per observation and period).  Results never depend on them. */
 int fbls_ctx_set_launch(fbls_ctx* ctx, uint32_t lds_bytes, uint32_t block,
                         uint32_t fold_mode);
 

@@ -140,6 +140,28 @@ def test_range_split_kernel_bit_exact(gpu, nb):
     _compare(gpu, dev, s, vq, wq, d, periods, nb, cfg, f"split nb{nb}")
 
 
+def test_counts_from_weights_in_split_kernel(gpu):
+    """Equal fixed-point weights: by default (fold mode 0) the fold skips the
+    count atomics and derives each bin's count from its weight sum; folding
+    the counts (mode 1) gives the same bits in the range-split kernel too,
+    and both equal the oracle."""
+    cfg = F.SearchConfig(period_min=0.5, period_max=100.0)
+    s, dev, vq, wq, d = _setup(gpu, cfg, seed=9, sectors=2, period=31.0,
+                               depth=3e-3, duration=6.0 / 24)
+    assert len(np.unique(wq)) == 1
+    periods = np.linspace(30.0, 54.0, 60)
+    bank = F.make_bank(cfg, d, s.rho)
+    _compare(gpu, dev, s, vq, wq, d, periods, 12000, cfg, "split, counts from weights")
+    gpu.set_launch(0, 0, 1)
+    try:
+        got = F.scan_bank(gpu, dev, periods, bank, 12000)
+    finally:
+        gpu.set_launch(0, 0, 0)
+    want = F.scan_bank(gpu, dev, periods, bank, 12000)
+    for a, b in zip(want[:4], got[:4]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_signal_search_long_baseline(gpu):
     """A two-year baseline with a one-year gap at periods up to 100 d: the
     top bands use the range-split kernel (nb up to 19200)."""
