@@ -310,7 +310,8 @@ def main():
                     help="LDS bytes per workgroup (0 = library default)")
     ap.add_argument("--block", type=int, default=0, help="256 or 512")
     ap.add_argument("--fold-mode", type=int, default=0,
-                    help="1 run-length, 2 one atomic per pair")
+                    help="0 default (equal weights fold no counts), 1 run-length "
+                         "with the counts folded, 2 one atomic per pair")
     ap.add_argument("--proposal-stars", type=int, default=512,
                     help="stars of the learned-pruned comparison (0 = skip)")
     ap.add_argument("--batch-stars", type=int, default=512,

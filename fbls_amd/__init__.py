@@ -560,6 +560,9 @@ class Gpu:
         return fr.value, tot.value
 
     def set_launch(self, lds_bytes=0, block=0, fold_mode=0):
+        """Launch-shape override (fbls_ctx_set_launch): LDS bytes per
+        workgroup, workgroup size, fold mode (0 default, 1 counts always
+        folded, 2 one atomic per observation).  Results never change."""
         _check(lib().fbls_ctx_set_launch(self._h, lds_bytes, block, fold_mode))
 
     def set_graphs(self, enable=None):
