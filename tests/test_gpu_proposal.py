@@ -97,3 +97,27 @@ def test_trained_proposal_recall_and_gain(gpu):
     learned0, raw0 = sw["learned"][1], sw["raw_snr_topk"][1]
     print("[proposal] learned", learned0["recall"], "raw", raw0["recall"], flush=True)
     assert learned0["recall"] >= raw0["recall"]
+
+
+def test_coarse_features_unequal_weights_bit_exact(gpu):
+    """The periodogram of stars whose weights differ point to point (the
+    general three-field fold) equals the oracle, like that of stars with
+    equal weights."""
+    import dataclasses
+
+    pc = PR.ProposalConfig()
+    cfg = PR.search_config(pc)
+    res = PR.make_states(range(2), 4_900_200, cfg, workers=1)
+    rng = np.random.default_rng(5)
+    states = [r[0] for r in res]
+    states.append(dataclasses.replace(states[0], w=states[0].w * rng.uniform(0.6, 1.5, len(states[0].w))))
+    devs = F.DeviceReady.upload_packed(gpu, states)
+    periods = PR.bin_centers(pc)[::53]
+    gd = PR.group_durations(cfg, pc.groups)
+    got = F.coarse_features(gpu, devs, periods, gd, pc.nb_min, pc.nb_max)
+    for k, (s, dev) in enumerate(zip(states, devs)):
+        vq, wq, _, _ = dev.fields()
+        assert (len(np.unique(wq)) == 1) == (k < 2)
+        want = O.coarse_features(s.t, vq, wq, s.baseline_norm, periods, gd, pc.nb_min,
+                                 pc.nb_max)
+        np.testing.assert_array_equal(got[k], want)
