@@ -82,8 +82,12 @@ def measured_traffic():
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
     if not m:
         return None
-    return {"bytes_per_launch": float(m.group(1)),
-            "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
+    out = {"bytes_per_launch": float(m.group(1)),
+           "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
+    for key in ("valu_busy_coarse", "lds_busy_coarse"):
+        k = re.search(key + r":\s*([0-9.]+)", text)
+        out[key] = float(k.group(1)) if k else None
+    return out
 
 
 def run_gpu(cfg, d, state, steps, warmup, launch=(0, 0, 0)):
@@ -392,6 +396,10 @@ def main():
             "kernel_ms_per_step": tot["seconds"] / args.steps * 1e3,
             "bytes_per_period": 16.0 * n + 88.0,
             "traffic_source": None if traffic is None else traffic["source"],
+            # what binds the kernel: the coarse launch's VALU and LDS busy
+            # fractions (rocprofv3 VALUBusy / LdsUtil, same profile)
+            "valu_busy": None if traffic is None else traffic["valu_busy_coarse"],
+            "lds_busy": None if traffic is None else traffic["lds_busy_coarse"],
         },
     }
     if pstars:
