@@ -520,7 +520,11 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   A.gate = sh.gate;
   A.counters = (unsigned int*)(din + cnt_off);
   // 256 threads: measured fastest on MI355X for every LDS budget 16-48 KiB
-  const uint32_t block = c->block_override ? c->block_override : 256;
+  // when the grid fills the chip; a grid of fewer than two workgroups per CU
+  // (a fine pass over a few hundred prescan survivors) of the LDS-resident
+  // kernel gets 512 threads, so each of its periods folds with twice the waves
+  const uint32_t block =
+      c->block_override ? c->block_override : (!split && groups < 512 ? 512 : 256);
   HIP_TRY(hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   HIP_TRY(fbls::scan_launch(A, groups, block, lds_max, split, c->stream));
