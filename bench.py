@@ -389,7 +389,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None if traffic is None else traffic["bytes_per_launch"],
-            "kernel": "fused_scan<256>",
+            "kernel": "fused_scan<256> (coarse launch), fused_scan<512> (fine launch)",
             "launches_per_step": launches,
             "launch_ms": launch_s * 1e3,
             "algorithmic_bytes_per_launch": bytes_launch,
