@@ -10,7 +10,7 @@ struct FeatureStar {
   const double* t;    // n origin-relative times
   const int32_t* vw;  // n x {v, w} fixed point
   uint32_t n;
-  uint32_t pad_;
+  uint32_t sorted;    // 1: times ascend
   double sv, sw;      // fixed point -> value
   double C;           // baseline norm
 };
@@ -25,6 +25,8 @@ struct FeatureArgs {
   float* inputs;            // stars x np x ng network inputs, or null
   uint32_t nb_max;          // largest nb (sets the LDS slots per period)
   uint32_t G;               // periods per workgroup
+  uint32_t narrow;          // 1: every t * nb / P < 2^24 (32-bit cells, float modulo)
+  uint32_t pad_;
 };
 
 size_t feature_lds_bytes(uint32_t nb_max, uint32_t G);

@@ -34,6 +34,15 @@ __device__ __forceinline__ uint32_t slot_of(double t, double scale, uint32_t nb)
   return (uint32_t)((uint64_t)cell % nb) + 1;
 }
 
+// floor(t * scale) mod nb for a 32-bit cell u < 2^24: the quotient from a
+// float reciprocal biased low by ~2^-22 is floor(u / nb) or one less, so one
+// conditional subtraction (as an unsigned min) finishes.
+__device__ __forceinline__ uint32_t bin_of(uint32_t u, uint32_t nb, float inv_lo) {
+  const uint32_t q = (uint32_t)((float)u * inv_lo);
+  const uint32_t r = u - __umul24(q, nb);
+  return min(r, r - nb);
+}
+
 // The sums are integers below 2^53 held in doubles, so every addition is
 // exact and the LDS double atomics are order-independent.
 __device__ __forceinline__ void flush(double* hv, double* hw, uint32_t* hn, uint32_t b,
@@ -120,7 +129,20 @@ __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
   __syncthreads();
   // fold: each lane loads 8 consecutive observations once per step and, for
   // every period of the group, sums runs of equal bins in registers (times
-  // are sorted, so neighbours share bins) before one LDS update per run
+  // are sorted, so neighbours share bins) before one LDS update per run.
+  // The group's phase-bin counts and scales sit in registers (G <= 4): a
+  // memory read inside the period loop would wait on the counter the LDS
+  // atomics share and drain them.
+  uint32_t nbg[4];
+  double scg[4];
+  float invg[4];
+#pragma unroll
+  for (uint32_t g = 0; g < 4; g++) {
+    const uint32_t p = p0 + (g < gcount ? g : 0);
+    nbg[g] = A.nb[p];
+    scg[g] = (double)nbg[g] / A.periods[p];
+    invg[g] = (float)((1.0 / (double)nbg[g]) * (1.0 - 0x1p-22));
+  }
   for (uint32_t base = 0; base < S.n; base += kBlock * 8) {
     const uint32_t i0 = base + tid * 8;
     if (i0 >= S.n) continue;
@@ -133,48 +155,52 @@ __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
       vv[r] = (double)S.vw[2 * (size_t)i];
       ww[r] = (double)S.vw[2 * (size_t)i + 1];
     }
-    for (uint32_t g = 0; g < gcount; g++) {
-      const uint32_t nb = A.nb[p0 + g];
-      const double scale = (double)nb / A.periods[p0 + g];
+#pragma unroll
+    for (uint32_t g = 0; g < 4; g++) {
+      if (g >= gcount) break;
+      const uint32_t nb = nbg[g];
+      const double scale = scg[g];
       double* gv = hv + (size_t)g * stride;
       double* gw = hw + (size_t)g * stride;
       uint32_t* gn = hn + (size_t)g * stride;
-      // the first bin by a modulo, the next ones from the differences of
-      // floor(t * scale) when the cells ascend and span less than nb
-      double cell[8];
+      if (A.narrow && cnt == 8) {
+        // 32-bit cells; when the times ascend and the lane's cells span
+        // fewer than nb, each bin is the previous one advanced by the
+        // difference of the cells, wrapped once
+        uint32_t u[8];
 #pragma unroll
-      for (uint32_t r = 0; r < 8; r++) cell[r] = floor(tt[r] * scale);
-      bool incr = cell[0] < 4294967296.0 && cell[cnt - 1] - cell[0] < (double)nb;
+        for (uint32_t r = 0; r < 8; r++) u[r] = (uint32_t)(tt[r] * scale);
+        const bool incr = S.sorted && __all(u[7] - u[0] < nb);
+        uint32_t cb = bin_of(u[0], nb, invg[g]);
+        double av = vv[0], aw = ww[0];
+        uint32_t ac = 1;
 #pragma unroll
-      for (uint32_t r = 1; r < 8; r++) incr = incr && (r >= cnt || cell[r] >= cell[r - 1]);
-      const uint32_t b0 = slot_of(tt[0], scale, nb);
-      uint32_t cur = b0;
-      double av = vv[0], aw = ww[0];
-      uint32_t ac = 1;
-#pragma unroll
-      for (uint32_t r = 1; r < 8; r++) {
-        if (r < cnt) {
+        for (uint32_t r = 1; r < 8; r++) {
           uint32_t b;
+          bool change;
           if (incr) {
-            b = b0 + (uint32_t)(cell[r] - cell[0]);
-            b = b > nb ? b - nb : b;
+            const uint32_t dlt = u[r] - u[r - 1];
+            change = dlt != 0;
+            b = cb + dlt;
+            b = min(b, b - nb);
           } else {
-            b = slot_of(tt[r], scale, nb);
+            b = bin_of(u[r], nb, invg[g]);
+            change = b != cb;
           }
-          if (b == cur) {
-            av += vv[r];
-            aw += ww[r];
-            ac++;
-          } else {
-            flush(gv, gw, gn, cur, av, aw, ac);
-            cur = b;
-            av = vv[r];
-            aw = ww[r];
-            ac = 1;
-          }
+          if (change) flush(gv, gw, gn, cb + 1, av, aw, ac);
+          const double sv = av + vv[r], sw = aw + ww[r];
+          av = change ? vv[r] : sv;
+          aw = change ? ww[r] : sw;
+          ac = change ? 1u : ac + 1u;
+          cb = b;
         }
+        flush(gv, gw, gn, cb + 1, av, aw, ac);
+      } else {
+        // the star's last partial chunk, or cells beyond 2^24: one update
+        // per observation, bins by floor and modulo
+        for (uint32_t r = 0; r < cnt; r++)
+          flush(gv, gw, gn, slot_of(tt[r], scale, nb), vv[r], ww[r], 1u);
       }
-      flush(gv, gw, gn, cur, av, aw, ac);
     }
   }
   __syncthreads();

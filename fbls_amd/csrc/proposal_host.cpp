@@ -42,14 +42,17 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
     nb[i] = (uint32_t)std::min<double>(nb_max, std::max<double>(nb_min, want));
   }
   const double pmin = *std::min_element(periods, periods + np);
+  double cell_max = 0.0;  // largest phase cell t * nb / P of the launch
   std::vector<fbls::FeatureStar> stars(nfam);
   for (size_t k = 0; k < nfam; k++) {
     const fbls_family* f = fams[k];
     if (!f) return fail(FBLS_E_ARG, "null family %zu", k);
     if (f->t_max * ((double)nb_max / pmin) > 9.0e15) return fail(FBLS_E_ARG, "phase cells exceed 2^53");
+    cell_max = std::max(cell_max, f->t_max * ((double)nb_max / pmin));
     stars[k].t = f->t_dev;
     stars[k].vw = f->vw_dev;
     stars[k].n = f->n;
+    stars[k].sorted = f->sorted ? 1u : 0u;
     stars[k].sv = std::ldexp(1.0, -f->ev);
     stars[k].sw = std::ldexp(1.0, -f->ew);
     stars[k].C = f->C;
@@ -83,6 +86,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   A.nb_max = nb_top;
   A.G = 4;
   while (A.G > 1 && fbls::feature_lds_bytes(nb_top, A.G) > 24 * 1024) A.G--;
+  A.narrow = cell_max < 16777215.0 ? 1u : 0u;  // (nb <= nb_max, P >= pmin)
   HIP_TRY(fbls::feature_launch(A, (uint32_t)nfam, s));
   if (device_inputs) return FBLS_OK;  // stream-ordered before the network
   HIP_TRY(hipMemcpyAsync(out ? (void*)out : (void*)inputs, d + o_out, out_bytes,
