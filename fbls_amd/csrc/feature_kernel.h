@@ -23,9 +23,13 @@ struct FeatureArgs {
   const double* durations;  // ng
   double* out;              // stars x np x ng best SNRs, or null
   float* inputs;            // stars x np x ng network inputs, or null
-  uint32_t nb_max;          // largest nb (sets the LDS slots per period)
-  uint32_t G;               // periods per workgroup
+  uint32_t nb_max;          // largest nb
+  uint32_t G;               // most periods in one group (<= 4)
   uint32_t narrow;          // 1: every t * nb / P < 2^24 (32-bit cells, float modulo)
+  uint32_t ngroups;         // groups of consecutive periods, one workgroup each per star
+  const uint32_t* gstart;   // ngroups + 1: group k holds periods [gstart[k], gstart[k+1])
+  const uint32_t* gstride;  // ngroups: LDS slots per period of group k (its largest nb + 1)
+  uint32_t lds_bytes;       // LDS of the largest group
   uint32_t pad_;
 };
 

@@ -2,8 +2,9 @@
 // of a batch and every period of a shared log-spaced list, the largest box
 // SNR over phase for each of a few representative durations.
 //
-// One workgroup (256 threads) per star and group of G periods (each lane's
-// observations are loaded once for the whole group): the star's fixed-point
+// One workgroup (256 threads) per star and group of up to 4 consecutive
+// periods whose histograms fit 24 KiB of LDS (each lane's observations are
+// loaded once for the whole group): the star's fixed-point
 // fields are folded into an LDS histogram of nb bins (nb = clamp(ceil(2 P /
 // d_0), nb_min, nb_max), d_0 the shortest duration) with exact integer sums
 // held in doubles (runs of consecutive observations in one bin are summed in
@@ -112,14 +113,13 @@ __device__ __forceinline__ double wave_max(double x) {
 template <uint32_t NG>
 __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  const uint32_t groups = (A.np + A.G - 1) / A.G;
-  const uint32_t star = blockIdx.x / groups, p0 = (blockIdx.x % groups) * A.G;
-  const uint32_t gcount = min(A.G, A.np - p0);
+  const uint32_t gi = blockIdx.x % A.ngroups, star = blockIdx.x / A.ngroups;
+  const uint32_t p0 = A.gstart[gi], gcount = A.gstart[gi + 1] - p0;
   const FeatureStar S = A.stars[star];
-  const uint32_t stride = A.nb_max + 1;  // slots per period
-  double* hv = (double*)lds;             // [G][stride]
-  double* hw = hv + (size_t)A.G * stride;
-  uint32_t* hn = (uint32_t*)(hw + (size_t)A.G * stride);
+  const uint32_t stride = A.gstride[gi];  // slots per period of this group
+  double* hv = (double*)lds;              // [gcount][stride]
+  double* hw = hv + (size_t)gcount * stride;
+  uint32_t* hn = (uint32_t*)(hw + (size_t)gcount * stride);
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (uint32_t b = tid; b < gcount * stride; b += kBlock) {
     hv[b] = 0;
@@ -322,12 +322,12 @@ size_t feature_lds_bytes(uint32_t nb_max, uint32_t G) {
 }
 
 hipError_t feature_launch(const FeatureArgs& a, uint32_t nstars, hipStream_t s) {
-  const size_t lds = feature_lds_bytes(a.nb_max, a.G);
-  if (a.ng > 16 || a.G > 4) return hipErrorInvalidValue;
+  const size_t lds = a.lds_bytes;
+  if (a.ng > 16 || a.G > 4 || a.ngroups == 0) return hipErrorInvalidValue;
   const void* fn = a.ng <= 8 ? (const void*)coarse_features<8> : (const void*)coarse_features<16>;
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  const uint32_t groups = (a.np + a.G - 1) / a.G;
+  const uint32_t groups = a.ngroups;
   if (a.ng <= 8)
     hipLaunchKernelGGL(coarse_features<8>, dim3(nstars * groups), dim3(kBlock), lds, s, a);
   else

@@ -61,7 +61,31 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   const size_t o_per = (b_stars + 255) & ~(size_t)255;
   const size_t o_nb = o_per + ((np * 8 + 255) & ~(size_t)255);
   const size_t o_dur = o_nb + ((np * 4 + 255) & ~(size_t)255);
-  const size_t o_out = o_dur + ((ng * 8 + 255) & ~(size_t)255);
+  // groups of up to 4 consecutive periods whose histograms (20 B per slot,
+  // each sized by the group's largest nb) fit 24 KiB of LDS: a star's
+  // observations are loaded once per group (more LDS per workgroup costs
+  // occupancy, which the fold needs more than reuse)
+  std::vector<uint32_t> gstart{0}, gstride;
+  size_t lds_bytes = 0;
+  uint32_t gmax = 1;
+  for (size_t p = 0; p < np;) {
+    uint32_t G = 1, mx = nb[p];
+    while (G < 4 && p + G < np) {
+      const uint32_t m2 = std::max(mx, nb[p + G]);
+      if (fbls::feature_lds_bytes(m2, G + 1) > 24 * 1024) break;
+      mx = m2;
+      G++;
+    }
+    gstride.push_back(mx + 1);
+    lds_bytes = std::max(lds_bytes, fbls::feature_lds_bytes(mx, G));
+    gmax = std::max(gmax, G);
+    p += G;
+    gstart.push_back((uint32_t)p);
+  }
+  const size_t ngrp = gstride.size();
+  const size_t o_gs = o_dur + ((ng * 8 + 255) & ~(size_t)255);
+  const size_t o_gw = o_gs + (((ngrp + 1) * 4 + 255) & ~(size_t)255);
+  const size_t o_out = o_gw + ((ngrp * 4 + 255) & ~(size_t)255);
   const size_t out_bytes = device_inputs ? 0 : nfam * np * ng * (out ? 8 : 4);
   HIP_TRY(hipSetDevice(c->device));
   if (int st = c->dev_in.ensure(o_out + out_bytes)) return st;
@@ -71,6 +95,8 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   HIP_TRY(hipMemcpyAsync(d + o_per, periods, np * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d + o_nb, nb.data(), np * 4, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(d + o_dur, durations, ng * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d + o_gs, gstart.data(), (ngrp + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d + o_gw, gstride.data(), ngrp * 4, hipMemcpyHostToDevice, s));
   fbls::FeatureArgs A;
   A.stars = (const fbls::FeatureStar*)d;
   A.periods = (const double*)(d + o_per);
@@ -80,12 +106,12 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   A.durations = (const double*)(d + o_dur);
   A.out = out ? (double*)(d + o_out) : nullptr;
   A.inputs = out ? nullptr : (device_inputs ? device_inputs : (float*)(d + o_out));
-  // periods per workgroup: as many as 24 KiB of LDS holds, at most 4 (more
-  // LDS per workgroup costs occupancy, which the fold needs more than reuse)
-  const uint32_t nb_top = *std::max_element(nb.begin(), nb.end());
-  A.nb_max = nb_top;
-  A.G = 4;
-  while (A.G > 1 && fbls::feature_lds_bytes(nb_top, A.G) > 24 * 1024) A.G--;
+  A.nb_max = *std::max_element(nb.begin(), nb.end());
+  A.G = gmax;
+  A.ngroups = (uint32_t)ngrp;
+  A.gstart = (const uint32_t*)(d + o_gs);
+  A.gstride = (const uint32_t*)(d + o_gw);
+  A.lds_bytes = (uint32_t)lds_bytes;
   A.narrow = cell_max < 16777215.0 ? 1u : 0u;  // (nb <= nb_max, P >= pmin)
   HIP_TRY(fbls::feature_launch(A, (uint32_t)nfam, s));
   if (device_inputs) return FBLS_OK;  // stream-ordered before the network
