@@ -355,6 +355,9 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--skip-fallback", action="store_true",
                     help="leave the fallback-band point out of the held-out sweep")
+    ap.add_argument("--heldout-only", action="store_true",
+                    help="keep the saved operating point and validation figures of the "
+                         "committed report; measure the held-out stars again")
     args = ap.parse_args()
 
     def log(msg):
@@ -383,20 +386,30 @@ def main():
         net, pc, _ = PR.load(gpu, path)
         if os.path.exists(REPORT):  # the training of the committed network
             train_info = json.load(open(REPORT)).get("training", {})
-    vres = make_states(range(args.validate), 5_100_000, pool=pool)
+    prior = json.load(open(REPORT)) if os.path.exists(REPORT) else {}
+    if args.heldout_only:  # the committed operating point, chosen on validation
+        _, _, extra = PR.load(gpu, args.out)
+        op = (float(extra["tau"]), int(extra["topk"]),
+              tuple(float(v) for v in np.atleast_1d(extra["fallback"])))
+        args.validate = 0
+    vres = make_states(range(args.validate), 5_100_000, pool=pool) if args.validate else []
     hres = make_states(range(args.heldout), 5_000_000, pool=pool)
     pool.shutdown()
     log(f"prepared {len(vres)} + {len(hres)} stars")
-    val = LongBatch(gpu, vres, net, pc, log=log)
-    log(f"validation full scan {val.summary()}")
-    vs = val.sweep(POINTS, log=lambda m: log("validation " + m))
-    val.close()
-    point = PR.fastest(vs, 0.99)
-    if point is None:  # none reaches 99 %: the most cautious of the best-recall points
-        top = max(r["recall"] for r in vs)
-        point = max((r for r in vs if r["recall"] == top),
-                    key=lambda r: r["kept_period_fraction"])
-    op = (point["tau"], point["topk"], tuple(point["fallback"]))
+    if args.heldout_only:
+        point = prior.get("validation_at_operating_point")
+        vs = prior.get("validation_sweep")
+    else:
+        val = LongBatch(gpu, vres, net, pc, log=log)
+        log(f"validation full scan {val.summary()}")
+        vs = val.sweep(POINTS, log=lambda m: log("validation " + m))
+        val.close()
+        point = PR.fastest(vs, 0.99)
+        if point is None:  # none reaches 99 %: the most cautious of the best-recall points
+            top = max(r["recall"] for r in vs)
+            point = max((r for r in vs if r["recall"] == top),
+                        key=lambda r: r["kept_period_fraction"])
+        op = (point["tau"], point["topk"], tuple(point["fallback"]))
     held = LongBatch(gpu, hres, net, pc, log=log)
     summary = held.summary()
     log(f"held-out full scan {summary}")
@@ -405,10 +418,18 @@ def main():
                     log=lambda m: log("held-out " + m))
     held.close()
     report = {"training": train_info,
+              "selection_rule": ("the fastest point whose validation recall reaches 99 %; "
+                                 "when none does, the most cautious (largest kept fraction) "
+                                 "of the points with the best validation recall"),
               "operating_point": {"tau": op[0], "topk": op[1], "fallback": list(op[2])},
-              "validation_stars": len(vres), "validation_at_operating_point": point,
+              "validation_stars": prior.get("validation_stars") if args.heldout_only
+              else len(vres),
+              "validation_at_operating_point": point,
+              "validation_sweep": vs,
               "heldout": summary, "heldout_at_operating_point": hs[0],
               "heldout_sweep": hs}
+    if args.heldout_only and "validation_full_scan" in prior:
+        report["validation_full_scan"] = prior["validation_full_scan"]
     log(json.dumps(hs[0]))
     PR.save(net, pc, args.out, tau=op[0], topk=op[1], fallback=np.array(op[2], dtype=float))
     if args.report:
