@@ -135,3 +135,15 @@ def test_config_validation():
         F.SearchConfig(ingress_fractions=[0.0, 0.5]).validate()
     assert F.SearchConfig().shape_gate == 0.6 * 7.1
     assert F.SearchConfig(ingress_fractions=[0.1]).shape_gate == 0.0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: asking for more exits with a message
+    before any GPU work."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr or "one process per GPU" in r.stderr
