@@ -59,6 +59,7 @@ EXPORTS = (
     "fbls_search_transits", "fbls_single_event_search", "fbls_read_inputs",
     "fbls_format_f64", "fbls_json_reformat", "fbls_cli", "fbls_coarse_inputs",
     "fbls_cnn_set_fused", "fbls_signal_search_bins", "fbls_propose",
+    "fbls_signal_search_bins_ragged",
 )
 
 
@@ -336,6 +337,9 @@ def lib():
         L.fbls_signal_search_bins.argtypes = [
             _vp, _P(_vp), _sz, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
             _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8), _sz, _P(_d)]
+        L.fbls_signal_search_bins_ragged.argtypes = [
+            _vp, _P(_vp), _sz, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
+            _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8), _P(_sz), _P(_d)]
         L.fbls_coarse_inputs.argtypes = [_vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz,
                                            _u32, _u32, _P(ctypes.c_float)]
         L.fbls_family_state.argtypes = [
@@ -861,6 +865,45 @@ def signal_search_bins(gpu, devs, durations, rhos, cfg: SearchConfig, bin_keep, 
         offs.ctypes.data_as(_P(_sz)), stats, bk.ctypes.data_as(_P(ctypes.c_uint8)),
         bk.shape[1], _dp(edges)))
     return ([buf[int(offs[i]):int(offs[i + 1])] for i in range(len(devs))],
+            [s.as_dict() for s in stats])
+
+
+def signal_search_bins_ragged(gpu, devs, durations, rhos, cfg: SearchConfig, bin_keeps,
+                              edges, rows_cap=None):
+    """fbls_signal_search_bins with bins of its own for every star:
+    bin_keeps[s] (booleans) and edges[s] (len(bin_keeps[s]) + 1 ascending
+    values) restrict star s.  One call, the same merged launches.  Returns
+    ([rows k_s x 5 per star], [stats per star])."""
+    durations = _f64(durations)
+    rho = np.ascontiguousarray(np.asarray(rhos, dtype=np.float64).reshape(
+        len(devs), len(durations)))
+    offs = np.zeros(len(devs) + 1, dtype=np.uintp)
+    offs[1:] = np.cumsum([len(k) for k in bin_keeps])
+    bk = np.ascontiguousarray(np.concatenate([np.asarray(k, dtype=np.uint8) for k in bin_keeps]))
+    ed = _f64(np.concatenate([np.asarray(e, dtype=np.float64) for e in edges]))
+    for k, e in zip(bin_keeps, edges):
+        assert len(e) == len(k) + 1
+    bank = make_bank(cfg, durations, rho[0])
+    scfg = SearchCfgC(cfg.period_min, cfg.period_max, cfg.period_oversample)
+    hs = (_vp * len(devs))(*[d._h.value for d in devs])
+    roffs = np.zeros(len(devs) + 1, dtype=np.uintp)
+    stats = (ScanStats * len(devs))()
+    if rows_cap is None:
+        dmin = float(durations.min())
+        rows_cap = 0
+        for d in devs:
+            hi = min(cfg.period_max, d.t_last)
+            lo = max(cfg.period_min, 4.0 * dmin * (1.0 + 1e-9))
+            if hi > lo:
+                rows_cap += lib().fbls_period_grid_count(lo, hi, d.t_last, dmin,
+                                                         cfg.period_oversample)
+    buf, rows = _row_buffer(rows_cap)
+    _check(lib().fbls_signal_search_bins_ragged(
+        gpu._h, hs, len(devs), _dp(durations), _dp(rho), len(durations),
+        ctypes.byref(bank), ctypes.byref(scfg), rows, rows_cap,
+        roffs.ctypes.data_as(_P(_sz)), stats, bk.ctypes.data_as(_P(ctypes.c_uint8)),
+        offs.ctypes.data_as(_P(_sz)), _dp(ed)))
+    return ([buf[int(roffs[i]):int(roffs[i + 1])] for i in range(len(devs))],
             [s.as_dict() for s in stats])
 
 

@@ -1006,6 +1006,9 @@ struct Restriction {
   const uint8_t* bin_keep = nullptr;
   size_t nbins = 0;
   const double* edges = nullptr;
+  // ragged bins: star s has bins [bin_offsets[s], bin_offsets[s + 1]) of
+  // bin_keep and its own edges at edges + bin_offsets[s] + s
+  const size_t* bin_offsets = nullptr;
   bool any() const { return keep || bin_keep; }
 };
 
@@ -1034,7 +1037,12 @@ int plan_stars(const fbls_family* const* fams, size_t s0, size_t s1, const doubl
         return;
       }
       if (rs.keep) apply_keep((*plans)[k], rs.keep + keep_at[k]);
-      if (rs.bin_keep) apply_bins((*plans)[k], rs.bin_keep + s * rs.nbins, rs.nbins, rs.edges);
+      if (rs.bin_keep && rs.bin_offsets) {
+        const size_t b0 = rs.bin_offsets[s];
+        apply_bins((*plans)[k], rs.bin_keep + b0, rs.bin_offsets[s + 1] - b0, rs.edges + b0 + s);
+      } else if (rs.bin_keep) {
+        apply_bins((*plans)[k], rs.bin_keep + s * rs.nbins, rs.nbins, rs.edges);
+      }
     }
   };
   const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
@@ -1279,6 +1287,28 @@ extern "C" int fbls_signal_search_bins(fbls_ctx* c, const fbls_family* const* fa
   rs.bin_keep = bin_keep;
   rs.nbins = nbins;
   rs.edges = edges;
+  return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
+                      star_stats, rs);
+}
+
+extern "C" int fbls_signal_search_bins_ragged(
+    fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const double* durations,
+    const double* rho, size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
+    fbls_row* rows, size_t rows_cap, size_t* row_offsets, fbls_scan_stats* star_stats,
+    const uint8_t* bin_keep, const size_t* bin_offsets, const double* edges) {
+  if (!bin_keep || !bin_offsets || !edges) return fail(FBLS_E_ARG, "null argument");
+  if (bin_offsets[0] != 0) return fail(FBLS_E_ARG, "bin offsets must start at 0");
+  for (size_t s = 0; s < nfam; s++) {
+    const size_t b0 = bin_offsets[s], b1 = bin_offsets[s + 1];
+    if (b1 <= b0) return fail(FBLS_E_ARG, "star %zu has no bins", s);
+    const double* e = edges + b0 + s;
+    for (size_t k = 0; k < b1 - b0; k++)
+      if (!(e[k + 1] > e[k])) return fail(FBLS_E_ARG, "bin edges of star %zu must ascend", s);
+  }
+  Restriction rs;
+  rs.bin_keep = bin_keep;
+  rs.edges = edges;
+  rs.bin_offsets = bin_offsets;
   return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
                       star_stats, rs);
 }

@@ -19,8 +19,9 @@ baseline is the same as over one sector, and it is computed on the star
 binned to 10 minutes (sums of v and w per bin, the bin's mean time), which
 leaves the box statistic of every window of >= 30 minutes unchanged up to
 the phase smear of one bin.  Stars with the same sector count share their
-baseline, hence their bins: they go through the proposal and the pruned
-search as one group.
+baseline, hence their bins: they go through the proposal as one group;
+the pruned search takes every star with its own bins in one call
+(fbls_signal_search_bins_ragged).
 
 Recall is the share of the full scan's recovered injections (transits and
 eclipsing-binary pairs, alias rule of tools/benchmark.py:26-27) that the
@@ -36,7 +37,8 @@ import time
 import numpy as np
 
 from . import (DeviceReady, FamilyState, SearchConfig, coarse_inputs, family_state,
-               propose, signal_search_batch, signal_search_bins, synth)
+               propose, signal_search_batch, signal_search_bins, signal_search_bins_ragged,
+               synth)
 from . import proposal as PR
 
 PERIOD_MIN, PERIOD_MAX = 0.5, 100.0
@@ -189,23 +191,20 @@ class LongBatch:
         band), timed from bin selection to the rows."""
         lo, hi = fallback
         t0 = time.perf_counter()
-        best = [None] * len(self.states)
-        templates = periods = nfall = 0
-        for _, idx in self.groups.items():
-            keep = np.stack([keep_from(probs[k], self.pc.keep_dilate, tau, topk)
-                             for k in idx])
-            for j, k in enumerate(idx):
-                conf = float(probs[k].max())
-                if lo <= conf < hi:
-                    keep[j] = True
-                    nfall += 1
-            rows, stats = signal_search_bins(self.gpu, [self.devs[k] for k in idx], self.d,
-                                             [self.rhos[k] for k in idx], self.cfg, keep,
-                                             edges[idx[0]])
-            for k, b in zip(idx, PR.best_rows(rows, self.cfg.threshold)):
-                best[k] = b
-            templates += sum(st["templates_scanned"] for st in stats)
-            periods += sum(st["periods"] for st in stats)
+        keeps, nfall = [], 0
+        for k in range(len(self.states)):
+            keep = keep_from(probs[k], self.pc.keep_dilate, tau, topk)
+            if lo <= float(probs[k].max()) < hi:
+                keep[:] = True
+                nfall += 1
+            keeps.append(keep)
+        # every star in one call (its own bins and edges): the same merged
+        # launches for the whole batch
+        rows, stats = signal_search_bins_ragged(self.gpu, self.devs, self.d, self.rhos,
+                                                self.cfg, keeps, edges)
+        best = PR.best_rows(rows, self.cfg.threshold)
+        templates = sum(st["templates_scanned"] for st in stats)
+        periods = sum(st["periods"] for st in stats)
         t_pruned = time.perf_counter() - t0
         rp = PR.recovered(best, self.injected)
         n = len(self.states)

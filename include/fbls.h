@@ -222,6 +222,20 @@ int fbls_signal_search_bins(fbls_ctx* ctx, const fbls_family* const* fams, size_
                             fbls_scan_stats* star_stats, const uint8_t* bin_keep,
                             size_t nbins, const double* edges);
 
+/* fbls_signal_search_bins with bins of its own for every star (stars of
+ * different baselines have coarse periodograms of different resolution):
+ * star s has the bins [bin_offsets[s], bin_offsets[s + 1]) of bin_keep
+ * (bin_offsets: nfam + 1 values from 0) and their bin_offsets[s + 1] -
+ * bin_offsets[s] + 1 ascending edges at edges + bin_offsets[s] + s.  One
+ * call scans every star in the same merged launches.  No counterpart in the
+ * reference: the learned proposal's pruned search of long-baseline stars. */
+int fbls_signal_search_bins_ragged(fbls_ctx* ctx, const fbls_family* const* fams, size_t nfam,
+                                   const double* durations, const double* rho, size_t nd,
+                                   const fbls_bank* bank, const fbls_search_cfg* cfg,
+                                   fbls_row* rows, size_t rows_cap, size_t* row_offsets,
+                                   fbls_scan_stats* star_stats, const uint8_t* bin_keep,
+                                   const size_t* bin_offsets, const double* edges);
+
 /* -- evidence (model comparison) ---------------------------------------- */
 
 /* SearchConfig fields read by `compare_search` (config.rs:33-42,53-85).

@@ -137,3 +137,33 @@ def test_long_proposal_keeps_recoveries(gpu):
     assert r["full_recovered"] >= 4
     assert r["recall"] >= 0.75
     assert r["kept_period_fraction"] < 0.1
+
+
+def test_ragged_bins_equal_grouped_calls(gpu):
+    """Every star with its own bins in one call (fbls_signal_search_bins_ragged)
+    gives each star the rows and statistics of the per-baseline calls."""
+    res = L.make_states(range(5), 5_200_010, workers=1)
+    states = [r[0] for r in res]
+    cfg = L.search_config()
+    d = cfg.durations_days()
+    devs = F.DeviceReady.upload_packed(gpu, states)
+    rhos = [s.rho for s in states]
+    pc = PR.ProposalConfig()
+    rng = np.random.default_rng(3)
+    keeps, edges = [], []
+    for s in states:
+        c, e = L.feature_bins(float(s.t[-1]), pc)
+        keeps.append(rng.uniform(size=len(c)) < 0.03)
+        edges.append(e)
+    print(f"[longbase] ragged bins sectors={[r[3] for r in res]}", flush=True)
+    rrows, rstats = F.signal_search_bins_ragged(gpu, devs, d, rhos, cfg, keeps, edges)
+    for _, idx in L.groups_of(states).items():
+        brow, bstats = F.signal_search_bins(gpu, [devs[k] for k in idx], d,
+                                            [rhos[k] for k in idx], cfg,
+                                            np.stack([keeps[k] for k in idx]), edges[idx[0]])
+        for j, k in enumerate(idx):
+            np.testing.assert_array_equal(brow[j], rrows[k])
+            for key in ("periods", "templates_scanned", "prescan_survivors"):
+                assert bstats[j][key] == rstats[k][key], (key, k)
+    for dv in devs:
+        dv.close()
