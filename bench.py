@@ -331,7 +331,7 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: stars are independent and the "
               "multi-GPU layout (one process per GPU over a shared star list) "
-              "is not run in this build; use --gpus 1", file=sys.stderr)
+              "is not run here; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     pstars = proposal_stars(args.proposal_stars) if args.proposal_stars > 0 else None

@@ -263,7 +263,7 @@ bool read_inputs(const std::vector<std::string>& paths, Observations* out, std::
     const bool fits = (path.size() >= 5 && path.compare(path.size() - 5, 5, ".fits") == 0) ||
                       (path.size() >= 8 && path.compare(path.size() - 8, 8, ".fits.gz") == 0);
     if (fits) {
-      *error = path + ": FITS input is not supported by this build (convert it to NPZ)";
+      *error = path + ": FITS input is not supported (convert it to NPZ)";
       return false;
     }
     std::map<std::string, Array> src;
