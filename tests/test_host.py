@@ -147,3 +147,41 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr or "one process per GPU" in r.stderr
+
+
+def test_ragged_bins_argument_checks():
+    """fbls_signal_search_bins_ragged refuses offsets that do not start at
+    zero, a star without bins and edges that do not ascend, before any
+    device work (a null context is then reported as a null argument)."""
+    import ctypes
+
+    L = F.lib()
+    P = ctypes.POINTER
+    two = np.array([0.5, 1.0])
+    bank = F.make_bank(F.SearchConfig(), two, np.ones(2))
+    cfg = F.SearchCfgC(0.5, 13.7, 3.0)
+    roff = np.zeros(3, dtype=np.uintp)
+
+    def call(offs, keep, edges):
+        offs = np.asarray(offs, dtype=np.uintp)
+        keep = np.asarray(keep, dtype=np.uint8)
+        edges = np.asarray(edges, dtype=np.float64)
+        return L.fbls_signal_search_bins_ragged(
+            None, None, 2, two.ctypes.data_as(P(ctypes.c_double)),
+            np.ones(4).ctypes.data_as(P(ctypes.c_double)), 2, ctypes.byref(bank),
+            ctypes.byref(cfg), None, 0, roff.ctypes.data_as(P(ctypes.c_size_t)), None,
+            keep.ctypes.data_as(P(ctypes.c_uint8)), offs.ctypes.data_as(P(ctypes.c_size_t)),
+            edges.ctypes.data_as(P(ctypes.c_double)))
+
+    def message():
+        return L.fbls_last_error().decode()
+
+    assert call([1, 2, 3], [1, 1, 1], [0, 1, 2, 3, 4]) != 0
+    assert "start at 0" in message()
+    assert call([0, 2, 2], [1, 1], [0, 1, 2, 3]) != 0
+    assert "has no bins" in message()
+    assert call([0, 2, 3], [1, 1, 1], [0, 2, 1, 0, 1]) != 0
+    assert "must ascend" in message()
+    # valid bins: the null context is the first thing refused
+    assert call([0, 2, 3], [1, 1, 1], [0, 1, 2, 0, 1]) != 0
+    assert "null argument" in message()
