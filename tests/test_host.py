@@ -185,3 +185,23 @@ def test_ragged_bins_argument_checks():
     # valid bins: the null context is the first thing refused
     assert call([0, 2, 3], [1, 1, 1], [0, 1, 2, 0, 1]) != 0
     assert "null argument" in message()
+
+
+def test_missing_library_raises():
+    """The product path has no CPU fallback: without the built library every
+    entry into it raises, naming the build command."""
+    import subprocess
+    import sys
+
+    code = ("import fbls_amd as F\n"
+            "F.LIB_PATH = '/nonexistent/libfbls_hip.so'\n"
+            "try:\n"
+            "    F.lib()\n"
+            "except ImportError as e:\n"
+            "    print('raised:', e)\n"
+            "else:\n"
+            "    raise SystemExit('no error')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "raised:" in r.stdout and "build" in r.stdout
