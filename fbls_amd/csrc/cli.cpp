@@ -10,6 +10,7 @@
 #include <cstring>
 #include <fstream>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -547,8 +548,12 @@ extern "C" int fbls_read_inputs(const char* const* paths, size_t npaths, double*
   for (size_t i = 0; i < npaths; i++) ps.emplace_back(paths[i]);
   fbls::io::Observations obs;
   std::string error;
-  if (!fbls::io::read_inputs(ps, &obs, &error))
-    return fbls::host::fail(FBLS_E_DATA, "%s", error.c_str());
+  try {
+    if (!fbls::io::read_inputs(ps, &obs, &error))
+      return fbls::host::fail(FBLS_E_DATA, "%s", error.c_str());
+  } catch (const std::bad_alloc&) {
+    return fbls::host::fail(FBLS_E_DATA, "input too large for host memory");
+  }
   *n = obs.time.size();
   *has_err = obs.has_err;
   const size_t m = std::min(cap, obs.time.size());

@@ -1,6 +1,7 @@
 // JSON printing and parsing for the search report (see json.h).
 #include "json.h"
 
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -124,8 +125,11 @@ void emit(const Value& v, size_t indent, std::string& out) {
 }
 
 struct Parser {
+  // nesting deeper than serde_json's default recursion limit is refused
+  static constexpr int kMaxDepth = 128;
   const std::string& t;
   size_t p = 0;
+  int depth = 0;
   std::string err;
   explicit Parser(const std::string& text) : t(text) {}
   void ws() {
@@ -194,10 +198,13 @@ struct Parser {
         case 'u': {
           uint32_t u;
           if (!hex4(&u)) return false;
-          if (u >= 0xd800 && u < 0xdc00 && t.compare(p, 2, "\\u") == 0) {
+          if (u >= 0xdc00 && u < 0xe000) return fail("lone trailing surrogate in hex escape");
+          if (u >= 0xd800 && u < 0xdc00) {
+            if (t.compare(p, 2, "\\u") != 0) return fail("lone leading surrogate in hex escape");
             p += 2;
             uint32_t lo;
             if (!hex4(&lo)) return false;
+            if (lo < 0xdc00 || lo >= 0xe000) return fail("invalid surrogate pair in hex escape");
             u = 0x10000 + ((u - 0xd800) << 10) + (lo - 0xdc00);
           }
           utf8(u, *s);
@@ -230,12 +237,15 @@ struct Parser {
       *v = Value::string("");
       return str(&v->s);
     }
+    if ((c == '[' || c == '{') && depth >= kMaxDepth) return fail("recursion limit exceeded");
     if (c == '[') {
       p++;
+      depth++;
       *v = Value::array();
       ws();
       if (p < t.size() && t[p] == ']') {
         p++;
+        depth--;
         return true;
       }
       for (;;) {
@@ -249,6 +259,7 @@ struct Parser {
         }
         if (p < t.size() && t[p] == ']') {
           p++;
+          depth--;
           return true;
         }
         return fail("expected , or ]");
@@ -256,10 +267,12 @@ struct Parser {
     }
     if (c == '{') {
       p++;
+      depth++;
       *v = Value::object();
       ws();
       if (p < t.size() && t[p] == '}') {
         p++;
+        depth--;
         return true;
       }
       for (;;) {
@@ -279,23 +292,53 @@ struct Parser {
         }
         if (p < t.size() && t[p] == '}') {
           p++;
+          depth--;
           return true;
         }
         return fail("expected , or }");
       }
     }
     if (c == '-' || (c >= '0' && c <= '9')) {
+      // RFC 8259 number: -?(0|[1-9][0-9]*)(.[0-9]+)?([eE][+-]?[0-9]+)?
+      auto digit = [&](size_t i) { return i < t.size() && t[i] >= '0' && t[i] <= '9'; };
       size_t q = p + (c == '-');
       bool real = false;
-      while (q < t.size() && std::strchr("0123456789.eE+-", t[q])) {
-        real = real || t[q] == '.' || t[q] == 'e' || t[q] == 'E';
+      if (!digit(q)) return fail("invalid number");
+      if (t[q] == '0') {
         q++;
+      } else {
+        while (digit(q)) q++;
+      }
+      if (q < t.size() && t[q] == '.') {
+        real = true;
+        if (!digit(++q)) return fail("invalid number");
+        while (digit(q)) q++;
+      }
+      if (q < t.size() && (t[q] == 'e' || t[q] == 'E')) {
+        real = true;
+        q++;
+        if (q < t.size() && (t[q] == '+' || t[q] == '-')) q++;
+        if (!digit(q)) return fail("invalid number");
+        while (digit(q)) q++;
       }
       const std::string tok = t.substr(p, q - p);
       p = q;
-      if (real) *v = Value::number(std::strtod(tok.c_str(), nullptr));
-      else if (c == '-') *v = Value::integer(std::strtoll(tok.c_str(), nullptr, 10));
-      else *v = Value::unsigned_(std::strtoull(tok.c_str(), nullptr, 10));
+      // integers beyond 64 bits, and -0, are read as doubles, as serde_json does
+      errno = 0;
+      if (!real && c == '-') {
+        const long long x = std::strtoll(tok.c_str(), nullptr, 10);
+        if (errno != ERANGE && x != 0) {
+          *v = Value::integer(x);
+          return true;
+        }
+      } else if (!real) {
+        const unsigned long long x = std::strtoull(tok.c_str(), nullptr, 10);
+        if (errno != ERANGE) {
+          *v = Value::unsigned_(x);
+          return true;
+        }
+      }
+      *v = Value::number(std::strtod(tok.c_str(), nullptr));
       return true;
     }
     return fail("unexpected character");

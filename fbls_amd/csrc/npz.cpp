@@ -3,8 +3,10 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -54,7 +56,7 @@ T load(const unsigned char* p, bool little) {
 template <class T>
 bool read_all(const unsigned char* body, size_t len, size_t count, bool little,
               std::vector<double>* out, std::string* error) {
-  if (len < count * sizeof(T)) {
+  if (count > len / sizeof(T)) {
     *error = "truncated NPY body";
     return false;
   }
@@ -65,6 +67,15 @@ bool read_all(const unsigned char* body, size_t len, size_t count, bool little,
 
 bool inflate_raw(const unsigned char* src, size_t n, size_t want, std::vector<unsigned char>* out,
                  std::string* error) {
+  if (n > UINT32_MAX || want > UINT32_MAX) {
+    *error = "deflated members of 4 GiB or more are not supported";
+    return false;
+  }
+  // deflate expands at most 1032:1; a larger claim is corrupt, not memory to reserve
+  if (want / 1032 > n + 1) {
+    *error = "corrupt deflate stream";
+    return false;
+  }
   out->resize(want);
   z_stream z;
   std::memset(&z, 0, sizeof(z));
@@ -156,7 +167,13 @@ bool parse_npy(const unsigned char* bytes, size_t len, Array* out, std::string* 
     start = comma + 1;
   }
   size_t count = 1;
-  for (size_t d : out->shape) count *= d;
+  for (size_t d : out->shape) {
+    if (d != 0 && count > SIZE_MAX / 8 / d) {
+      *error = "NPY shape too large";
+      return false;
+    }
+    count *= d;
+  }
   const unsigned char* body = bytes + offset + header_len;
   const size_t blen = len - offset - header_len;
   std::string d = descr;
@@ -202,36 +219,40 @@ bool read_npz(const std::string& path, std::map<std::string, Array>* out, std::s
   if ((entries == 0xffff || cd_off == 0xffffffffu) && eocd >= 20 &&
       rd32(&z[eocd - 20]) == 0x07064b50u) {
     const uint64_t z64 = rd64(&z[eocd - 20 + 8]);
-    if (z64 + 56 > n || rd32(&z[z64]) != 0x06064b50u) return bad("invalid Zip64 record");
+    if (z64 > n || n - z64 < 56 || rd32(&z[z64]) != 0x06064b50u) return bad("invalid Zip64 record");
     entries = rd64(&z[z64 + 32]);
     cd_off = rd64(&z[z64 + 48]);
   }
   size_t p = cd_off;
   out->clear();
   for (uint64_t e = 0; e < entries; e++) {
-    if (p + 46 > n || rd32(&z[p]) != 0x02014b50u) return bad("invalid central directory");
+    if (p > n || n - p < 46 || rd32(&z[p]) != 0x02014b50u) return bad("invalid central directory");
     const uint16_t flags = rd16(&z[p + 8]), method = rd16(&z[p + 10]);
     uint64_t csize = rd32(&z[p + 20]), usize = rd32(&z[p + 24]);
     const uint16_t nlen = rd16(&z[p + 28]), xlen = rd16(&z[p + 30]), clen = rd16(&z[p + 32]);
     uint64_t local = rd32(&z[p + 42]);
-    if (p + 46 + nlen + xlen + clen > n) return bad("invalid central directory");
+    if (n - p - 46 < (size_t)nlen + xlen + clen) return bad("invalid central directory");
     std::string name((const char*)&z[p + 46], nlen);
-    // zip64 extended information (header id 1)
-    for (size_t x = p + 46 + nlen; x + 4 <= p + 46 + nlen + xlen;) {
+    // zip64 extended information (header id 1); a record never reads past
+    // the extra field
+    const size_t xend = p + 46 + nlen + xlen;
+    for (size_t x = p + 46 + nlen; x + 4 <= xend;) {
       const uint16_t id = rd16(&z[x]), sz = rd16(&z[x + 2]);
+      const size_t rend = std::min(xend, x + 4 + sz);
       if (id == 1) {
         size_t q = x + 4;
-        if (usize == 0xffffffffu && q + 8 <= x + 4 + sz) usize = rd64(&z[q]), q += 8;
-        if (csize == 0xffffffffu && q + 8 <= x + 4 + sz) csize = rd64(&z[q]), q += 8;
-        if (local == 0xffffffffu && q + 8 <= x + 4 + sz) local = rd64(&z[q]), q += 8;
+        if (usize == 0xffffffffu && q + 8 <= rend) usize = rd64(&z[q]), q += 8;
+        if (csize == 0xffffffffu && q + 8 <= rend) csize = rd64(&z[q]), q += 8;
+        if (local == 0xffffffffu && q + 8 <= rend) local = rd64(&z[q]), q += 8;
       }
       x += 4 + sz;
     }
     p += 46 + nlen + xlen + clen;
     if (flags & 1) return bad("encrypted archives are not supported");
-    if (local + 30 > n || rd32(&z[local]) != 0x04034b50u) return bad("invalid local file header");
+    if (local > n || n - local < 30 || rd32(&z[local]) != 0x04034b50u)
+      return bad("invalid local file header");
     const size_t data = local + 30 + rd16(&z[local + 26]) + rd16(&z[local + 28]);
-    if (data + csize > n) return bad("truncated member");
+    if (data > n || csize > n - data) return bad("truncated member");
     std::vector<unsigned char> bytes;
     if (method == 0) {
       bytes.assign(z.begin() + data, z.begin() + data + csize);

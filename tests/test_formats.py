@@ -70,6 +70,30 @@ def test_report_layout_reprints_byte_for_byte():
         '  "b": [],\n  "c": {}\n}')
 
 
+
+def test_report_reader_rejects_malformed_json():
+    """The report reader follows RFC 8259 and serde_json's limits: nesting
+    deeper than 128 and malformed numbers or surrogate escapes are errors;
+    integers beyond 64 bits, and -0, read as doubles."""
+    deep = "[" * 128 + "]" * 128
+    assert F.json_reformat(deep).count("[") == 128
+    with pytest.raises(RuntimeError, match="recursion limit exceeded"):
+        F.json_reformat("[" * 129 + "]" * 129)
+    with pytest.raises(RuntimeError, match="recursion limit exceeded"):
+        F.json_reformat('{"a": ' * 200_000 + "1" + "}" * 200_000)
+    for bad in ("01", "1.", "-", "1e", "1e+", ".5", "1-2", "[1.5.5]", "--1"):
+        with pytest.raises(RuntimeError):
+            F.json_reformat(bad)
+    assert F.json_reformat("[-0, 0.5e-3, 1E+2, 18446744073709551615]") == (
+        "[\n  -0.0,\n  0.0005,\n  100.0,\n  18446744073709551615\n]")
+    assert F.json_reformat("[18446744073709551616, -9223372036854775809]") == (
+        "[\n  1.8446744073709552e19,\n  -9.223372036854776e18\n]")
+    assert json.loads(F.json_reformat('"\\ud83d\\ude00"')) == "\U0001F600"
+    for bad in ('"\\ud83d"', '"\\ud83d\\u0041"', '"\\ude00"'):
+        with pytest.raises(RuntimeError, match="surrogate"):
+            F.json_reformat(bad)
+
+
 def _npz(path, compressed=False, **arrays):
     (np.savez_compressed if compressed else np.savez)(path, **arrays)
     return str(path)
@@ -152,3 +176,77 @@ def test_cli_rejects_bad_arguments(tmp_path, capfd):
     assert "--list FILE is required" in capfd.readouterr().err
     assert F.cli(["frobnicate"]) == 1
     assert "usage: fbls search | batch" in capfd.readouterr().err
+
+
+def _zip64_tail(data, cd_off):
+    """Rewrites a plain zip's end record as zip64 records whose central
+    directory offset is cd_off (the entry count and size are kept)."""
+    import struct
+
+    e = data.rindex(b"PK\x05\x06")
+    (entries,) = struct.unpack_from("<H", data, e + 10)
+    (cd_size,) = struct.unpack_from("<I", data, e + 12)
+    body = data[:e]
+    z64_at = len(body)
+    rec = struct.pack("<IQHHIIQQQQ", 0x06064b50, 44, 45, 45, 0, 0,
+                      entries, entries, cd_size, cd_off)
+    loc = struct.pack("<IIQI", 0x07064b50, 0, z64_at, 1)
+    end = struct.pack("<IHHHHIIH", 0x06054b50, 0, 0, 0xffff, 0xffff,
+                      0xffffffff, 0xffffffff, 0)
+    return body + rec + loc + end
+
+
+def test_npz_corrupt_sizes_fail_cleanly(tmp_path):
+    """Sizes and offsets read from a damaged archive are checked without
+    overflow: a shape whose byte count wraps, a deflated member claiming
+    more than deflate can expand to, and a zip64 directory offset near
+    2^64 each give an error rather than a huge allocation or a read past
+    the file."""
+    import io
+    import zipfile
+
+    t = np.arange(10.0)
+
+    def npy(header, body):
+        h = header.encode()
+        h += b" " * (63 - (10 + len(h)) % 64) + b"\n"
+        return b"\x93NUMPY\x01\x00" + len(h).to_bytes(2, "little") + h + body
+
+    # shape (2^61 + 1,): its byte count wraps to 8 in 64 bits
+    bad = npy("{'descr': '<f8', 'fortran_order': False, 'shape': (2305843009213693953,), }",
+              b"\0" * 8)
+    p = str(tmp_path / "shape.npz")
+    with zipfile.ZipFile(p, "w") as z:
+        buf = io.BytesIO()
+        np.lib.format.write_array(buf, t)
+        z.writestr("flux.npy", buf.getvalue())
+        z.writestr("time_bjd.npy", bad)
+    _fails([p], "NPY shape too large")
+
+    # a deflated member whose central record claims ~4 GiB uncompressed
+    p = str(tmp_path / "ratio.npz")
+    _npz(tmp_path / "ratio0.npz", compressed=True, time_bjd=t, flux=t)
+    with zipfile.ZipFile(str(tmp_path / "ratio0.npz")) as z:
+        assert all(i.compress_type == zipfile.ZIP_DEFLATED for i in z.infolist())
+    data = bytearray(open(str(tmp_path / "ratio0.npz"), "rb").read())
+    c = data.index(b"PK\x01\x02")
+    data[c + 24:c + 28] = (0xfffffff0).to_bytes(4, "little")
+    open(p, "wb").write(bytes(data))
+    _fails([p], "corrupt deflate stream")
+
+    # zip64 end records: the true offset reads back; one near 2^64 is refused
+    src = str(tmp_path / "plain.npz")
+    with zipfile.ZipFile(src, "w") as z:
+        for name, arr in (("time_bjd", t), ("flux", t)):
+            buf = io.BytesIO()
+            np.lib.format.write_array(buf, arr)
+            z.writestr(name + ".npy", buf.getvalue())
+    plain = open(src, "rb").read()
+    good = _zip64_tail(plain, plain.index(b"PK\x01\x02"))
+    p = str(tmp_path / "z64_ok.npz")
+    open(p, "wb").write(good)
+    np.testing.assert_array_equal(F.read_inputs([p])["time"], t)
+    wrapped = _zip64_tail(plain, (1 << 64) - 10)
+    p = str(tmp_path / "z64_wrap.npz")
+    open(p, "wb").write(wrapped)
+    _fails([p], "invalid central directory")
