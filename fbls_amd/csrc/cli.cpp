@@ -511,6 +511,9 @@ extern "C" int fbls_cli(int argc, const char* const* argv) {
   } catch (const fbls::Failure& e) {
     std::fprintf(stderr, "fbls: %s\n", e.message.c_str());
     return 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "fbls: %s\n", e.what());
+    return 1;
   }
   return 0;
 }
@@ -526,40 +529,40 @@ extern "C" size_t fbls_format_f64(double x, char* buf, size_t cap) {
 }
 
 extern "C" int fbls_json_reformat(const char* text, char* out, size_t cap, size_t* len) {
-  if (!text || !len) return fbls::host::fail(FBLS_E_ARG, "null argument");
-  fbls::json::Value v;
-  std::string err;
-  if (!fbls::json::parse(text, &v, &err)) return fbls::host::fail(FBLS_E_DATA, "%s", err.c_str());
-  const std::string s = fbls::json::pretty(v);
-  *len = s.size();
-  if (out && cap) {
-    const size_t m = std::min(cap - 1, s.size());
-    std::memcpy(out, s.data(), m);
-    out[m] = 0;
-  }
-  return 0;
+  return fbls::host::guard([&]() -> int {
+    if (!text || !len) return fbls::host::fail(FBLS_E_ARG, "null argument");
+    fbls::json::Value v;
+    std::string err;
+    if (!fbls::json::parse(text, &v, &err)) return fbls::host::fail(FBLS_E_DATA, "%s", err.c_str());
+    const std::string s = fbls::json::pretty(v);
+    *len = s.size();
+    if (out && cap) {
+      const size_t m = std::min(cap - 1, s.size());
+      std::memcpy(out, s.data(), m);
+      out[m] = 0;
+    }
+    return 0;
+  });
 }
 
 extern "C" int fbls_read_inputs(const char* const* paths, size_t npaths, double* time,
                                 double* flux, double* err, int64_t* group, size_t cap,
                                 size_t* n, int* has_err) {
-  if ((npaths && !paths) || !n || !has_err) return fbls::host::fail(FBLS_E_ARG, "null argument");
-  std::vector<std::string> ps;
-  for (size_t i = 0; i < npaths; i++) ps.emplace_back(paths[i]);
-  fbls::io::Observations obs;
-  std::string error;
-  try {
+  return fbls::host::guard([&]() -> int {
+    if ((npaths && !paths) || !n || !has_err) return fbls::host::fail(FBLS_E_ARG, "null argument");
+    std::vector<std::string> ps;
+    for (size_t i = 0; i < npaths; i++) ps.emplace_back(paths[i]);
+    fbls::io::Observations obs;
+    std::string error;
     if (!fbls::io::read_inputs(ps, &obs, &error))
       return fbls::host::fail(FBLS_E_DATA, "%s", error.c_str());
-  } catch (const std::bad_alloc&) {
-    return fbls::host::fail(FBLS_E_DATA, "input too large for host memory");
-  }
-  *n = obs.time.size();
-  *has_err = obs.has_err;
-  const size_t m = std::min(cap, obs.time.size());
-  if (time) std::memcpy(time, obs.time.data(), m * 8);
-  if (flux) std::memcpy(flux, obs.flux.data(), m * 8);
-  if (err && obs.has_err) std::memcpy(err, obs.err.data(), m * 8);
-  if (group) std::memcpy(group, obs.group.data(), m * 8);
-  return 0;
+    *n = obs.time.size();
+    *has_err = obs.has_err;
+    const size_t m = std::min(cap, obs.time.size());
+    if (time) std::memcpy(time, obs.time.data(), m * 8);
+    if (flux) std::memcpy(flux, obs.flux.data(), m * 8);
+    if (err && obs.has_err) std::memcpy(err, obs.err.data(), m * 8);
+    if (group) std::memcpy(group, obs.group.data(), m * 8);
+    return 0;
+  });
 }

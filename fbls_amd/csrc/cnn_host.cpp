@@ -248,52 +248,54 @@ int check_batch(const fbls_cnn* n, const float* x, uint32_t B, uint32_t L) {
 
 extern "C" int fbls_cnn_create(fbls_ctx* c, const fbls_cnn_cfg* cfg, uint64_t seed,
                                fbls_cnn** out) {
-  if (!c || !cfg || !out) return fail(FBLS_E_ARG, "null argument");
-  *out = nullptr;
-  if (cfg->in_channels < 1 || cfg->in_channels > 256 || cfg->width < 1 || cfg->width > 256 ||
-      cfg->layers < 1 || cfg->layers > 32 || cfg->kernel < 1 || cfg->kernel % 2 == 0 ||
-      cfg->kernel > 31)
-    return fail(FBLS_E_ARG, "network shape out of range");
-  HIP_TRY(hipSetDevice(c->device));
-  fbls_cnn* n = new fbls_cnn();
-  n->c = c;
-  n->cfg = *cfg;
-  size_t off = 0;
-  for (uint32_t l = 0; l < cfg->layers; l++) {
-    const size_t cin = l == 0 ? cfg->in_channels : cfg->width;
-    n->k_in.push_back(cfg->kernel * cin);
-    n->w_off.push_back(off);
-    off += (size_t)cfg->width * cfg->kernel * cin;
-    n->b_off.push_back(off);
+  return fbls::host::guard([&]() -> int {
+    if (!c || !cfg || !out) return fail(FBLS_E_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->in_channels < 1 || cfg->in_channels > 256 || cfg->width < 1 || cfg->width > 256 ||
+        cfg->layers < 1 || cfg->layers > 32 || cfg->kernel < 1 || cfg->kernel % 2 == 0 ||
+        cfg->kernel > 31)
+      return fail(FBLS_E_ARG, "network shape out of range");
+    HIP_TRY(hipSetDevice(c->device));
+    fbls_cnn* n = new fbls_cnn();
+    n->c = c;
+    n->cfg = *cfg;
+    size_t off = 0;
+    for (uint32_t l = 0; l < cfg->layers; l++) {
+      const size_t cin = l == 0 ? cfg->in_channels : cfg->width;
+      n->k_in.push_back(cfg->kernel * cin);
+      n->w_off.push_back(off);
+      off += (size_t)cfg->width * cfg->kernel * cin;
+      n->b_off.push_back(off);
+      off += cfg->width;
+    }
+    n->head_w = off;
     off += cfg->width;
-  }
-  n->head_w = off;
-  off += cfg->width;
-  n->head_b = off;
-  off += 1;
-  n->count = off;
-  std::vector<float> p(off, 0.f);
-  uint64_t s = seed ? seed : 0x9e3779b97f4a7c15ull;
-  for (uint32_t l = 0; l < cfg->layers; l++) {
-    const double bound = std::sqrt(6.0 / (double)n->k_in[l]);
-    for (size_t i = 0; i < (size_t)cfg->width * n->k_in[l]; i++)
-      p[n->w_off[l] + i] = (float)((2.0 * uniform(&s) - 1.0) * bound);
-  }
-  const double hb = std::sqrt(3.0 / (double)cfg->width);
-  for (uint32_t i = 0; i < cfg->width; i++) p[n->head_w + i] = (float)((2.0 * uniform(&s) - 1.0) * hb);
-  p[n->head_b] = -3.0f;  // sparse positives: start near sigmoid(-3)
-  int st;
-  if ((st = n->P.ensure(off * 4)) || (st = n->PB.ensure(off * 2)) || (st = n->G.ensure(off * 4)) ||
-      (st = n->Mo.ensure(off * 4)) || (st = n->Vo.ensure(off * 4))) {
-    delete n;
-    return st;
-  }
-  if (fbls_cnn_set_params(n, p.data()) != FBLS_OK) {
-    delete n;
-    return FBLS_E_HIP;
-  }
-  *out = n;
-  return FBLS_OK;
+    n->head_b = off;
+    off += 1;
+    n->count = off;
+    std::vector<float> p(off, 0.f);
+    uint64_t s = seed ? seed : 0x9e3779b97f4a7c15ull;
+    for (uint32_t l = 0; l < cfg->layers; l++) {
+      const double bound = std::sqrt(6.0 / (double)n->k_in[l]);
+      for (size_t i = 0; i < (size_t)cfg->width * n->k_in[l]; i++)
+        p[n->w_off[l] + i] = (float)((2.0 * uniform(&s) - 1.0) * bound);
+    }
+    const double hb = std::sqrt(3.0 / (double)cfg->width);
+    for (uint32_t i = 0; i < cfg->width; i++) p[n->head_w + i] = (float)((2.0 * uniform(&s) - 1.0) * hb);
+    p[n->head_b] = -3.0f;  // sparse positives: start near sigmoid(-3)
+    int st;
+    if ((st = n->P.ensure(off * 4)) || (st = n->PB.ensure(off * 2)) || (st = n->G.ensure(off * 4)) ||
+        (st = n->Mo.ensure(off * 4)) || (st = n->Vo.ensure(off * 4))) {
+      delete n;
+      return st;
+    }
+    if (fbls_cnn_set_params(n, p.data()) != FBLS_OK) {
+      delete n;
+      return FBLS_E_HIP;
+    }
+    *out = n;
+    return FBLS_OK;
+  });
 }
 
 extern "C" void fbls_cnn_destroy(fbls_cnn* n) {
@@ -305,80 +307,92 @@ extern "C" void fbls_cnn_destroy(fbls_cnn* n) {
 extern "C" size_t fbls_cnn_param_count(const fbls_cnn* n) { return n ? n->count : 0; }
 
 extern "C" int fbls_cnn_get_params(fbls_cnn* n, float* out) {
-  if (!n || !out) return fail(FBLS_E_ARG, "null argument");
-  HIP_TRY(hipMemcpy(out, n->P.ptr, n->count * 4, hipMemcpyDeviceToHost));
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!n || !out) return fail(FBLS_E_ARG, "null argument");
+    HIP_TRY(hipMemcpy(out, n->P.ptr, n->count * 4, hipMemcpyDeviceToHost));
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_cnn_set_params(fbls_cnn* n, const float* in) {
-  if (!n || !in) return fail(FBLS_E_ARG, "null argument");
-  hipStream_t s = n->c->stream;
-  HIP_TRY(hipMemcpyAsync(n->P.ptr, in, n->count * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(cn::to_bf16((const float*)n->P.ptr, (cn::bf16*)n->PB.ptr, (long long)n->count, s));
-  std::vector<float> zero(n->count, 0.f);
-  HIP_TRY(hipMemcpyAsync(n->Mo.ptr, zero.data(), n->count * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(n->Vo.ptr, zero.data(), n->count * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  n->step = 0;
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!n || !in) return fail(FBLS_E_ARG, "null argument");
+    hipStream_t s = n->c->stream;
+    HIP_TRY(hipMemcpyAsync(n->P.ptr, in, n->count * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(cn::to_bf16((const float*)n->P.ptr, (cn::bf16*)n->PB.ptr, (long long)n->count, s));
+    std::vector<float> zero(n->count, 0.f);
+    HIP_TRY(hipMemcpyAsync(n->Mo.ptr, zero.data(), n->count * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(n->Vo.ptr, zero.data(), n->count * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    n->step = 0;
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_cnn_forward(fbls_cnn* n, const float* x, uint32_t B, uint32_t L,
                                 float* logits) {
-  if (int st = check_batch(n, x, B, L)) return st;
-  if (!logits) return fail(FBLS_E_ARG, "null argument");
-  HIP_TRY(hipSetDevice(n->c->device));
-  if (int st = (n->fused && fusable(n)) ? infer(n, x, B, L) : forward(n, x, B, L)) return st;
-  HIP_TRY(hipMemcpyAsync(logits, n->z.ptr, (size_t)B * L * 4, hipMemcpyDeviceToHost, n->c->stream));
-  HIP_TRY(hipStreamSynchronize(n->c->stream));
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (int st = check_batch(n, x, B, L)) return st;
+    if (!logits) return fail(FBLS_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(n->c->device));
+    if (int st = (n->fused && fusable(n)) ? infer(n, x, B, L) : forward(n, x, B, L)) return st;
+    HIP_TRY(hipMemcpyAsync(logits, n->z.ptr, (size_t)B * L * 4, hipMemcpyDeviceToHost, n->c->stream));
+    HIP_TRY(hipStreamSynchronize(n->c->stream));
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_cnn_loss_grad(fbls_cnn* n, const float* x, const float* y, uint32_t B,
                                   uint32_t L, float* loss, float* grads) {
-  if (int st = check_batch(n, x, B, L)) return st;
-  if (!y) return fail(FBLS_E_ARG, "null argument");
-  HIP_TRY(hipSetDevice(n->c->device));
-  if (int st = forward(n, x, B, L)) return st;
-  if (int st = backward(n, y, B, L, loss)) return st;
-  if (grads) HIP_TRY(hipMemcpy(grads, n->G.ptr, n->count * 4, hipMemcpyDeviceToHost));
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (int st = check_batch(n, x, B, L)) return st;
+    if (!y) return fail(FBLS_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(n->c->device));
+    if (int st = forward(n, x, B, L)) return st;
+    if (int st = backward(n, y, B, L, loss)) return st;
+    if (grads) HIP_TRY(hipMemcpy(grads, n->G.ptr, n->count * 4, hipMemcpyDeviceToHost));
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_cnn_train_step(fbls_cnn* n, const float* x, const float* y, uint32_t B,
                                    uint32_t L, float lr, float weight_decay, float* loss) {
-  if (int st = check_batch(n, x, B, L)) return st;
-  if (!y) return fail(FBLS_E_ARG, "null argument");
-  HIP_TRY(hipSetDevice(n->c->device));
-  if (int st = forward(n, x, B, L)) return st;
-  if (int st = backward(n, y, B, L, loss)) return st;
-  n->step++;
-  hipStream_t s = n->c->stream;
-  float* P = (float*)n->P.ptr;
-  cn::bf16* PB = (cn::bf16*)n->PB.ptr;
-  const float* G = (const float*)n->G.ptr;
-  float* M = (float*)n->Mo.ptr;
-  float* V = (float*)n->Vo.ptr;
-  // decoupled weight decay on the weights, none on the biases
-  for (uint32_t l = 0; l < n->cfg.layers; l++) {
-    const size_t wo = n->w_off[l], wn = n->b_off[l] - wo, bo = n->b_off[l];
-    HIP_TRY(cn::adamw(P + wo, PB + wo, G + wo, M + wo, V + wo, wn, lr, 0.9f, 0.999f, 1e-8f,
-                      weight_decay, n->step, s));
-    HIP_TRY(cn::adamw(P + bo, PB + bo, G + bo, M + bo, V + bo, n->cfg.width, lr, 0.9f, 0.999f,
-                      1e-8f, 0.0f, n->step, s));
-  }
-  HIP_TRY(cn::adamw(P + n->head_w, PB + n->head_w, G + n->head_w, M + n->head_w, V + n->head_w,
-                    n->cfg.width, lr, 0.9f, 0.999f, 1e-8f, weight_decay, n->step, s));
-  HIP_TRY(cn::adamw(P + n->head_b, PB + n->head_b, G + n->head_b, M + n->head_b, V + n->head_b,
-                    1, lr, 0.9f, 0.999f, 1e-8f, 0.0f, n->step, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (int st = check_batch(n, x, B, L)) return st;
+    if (!y) return fail(FBLS_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(n->c->device));
+    if (int st = forward(n, x, B, L)) return st;
+    if (int st = backward(n, y, B, L, loss)) return st;
+    n->step++;
+    hipStream_t s = n->c->stream;
+    float* P = (float*)n->P.ptr;
+    cn::bf16* PB = (cn::bf16*)n->PB.ptr;
+    const float* G = (const float*)n->G.ptr;
+    float* M = (float*)n->Mo.ptr;
+    float* V = (float*)n->Vo.ptr;
+    // decoupled weight decay on the weights, none on the biases
+    for (uint32_t l = 0; l < n->cfg.layers; l++) {
+      const size_t wo = n->w_off[l], wn = n->b_off[l] - wo, bo = n->b_off[l];
+      HIP_TRY(cn::adamw(P + wo, PB + wo, G + wo, M + wo, V + wo, wn, lr, 0.9f, 0.999f, 1e-8f,
+                        weight_decay, n->step, s));
+      HIP_TRY(cn::adamw(P + bo, PB + bo, G + bo, M + bo, V + bo, n->cfg.width, lr, 0.9f, 0.999f,
+                        1e-8f, 0.0f, n->step, s));
+    }
+    HIP_TRY(cn::adamw(P + n->head_w, PB + n->head_w, G + n->head_w, M + n->head_w, V + n->head_w,
+                      n->cfg.width, lr, 0.9f, 0.999f, 1e-8f, weight_decay, n->step, s));
+    HIP_TRY(cn::adamw(P + n->head_b, PB + n->head_b, G + n->head_b, M + n->head_b, V + n->head_b,
+                      1, lr, 0.9f, 0.999f, 1e-8f, 0.0f, n->step, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_cnn_set_fused(fbls_cnn* n, int enable) {
-  if (!n) return fail(FBLS_E_ARG, "null argument");
-  n->fused = enable ? 1 : 0;
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!n) return fail(FBLS_E_ARG, "null argument");
+    n->fused = enable ? 1 : 0;
+    return FBLS_OK;
+  });
 }
 
 namespace fbls {

@@ -46,33 +46,35 @@ extern "C" const char* fbls_last_error(void) { return g_last_error.c_str(); }
 // ---------------------------------------------------------------- context
 
 extern "C" int fbls_ctx_create(int device, fbls_ctx** out) {
-  if (!out) return fail(FBLS_E_ARG, "null output pointer");
-  *out = nullptr;
-  int count = 0;
-  HIP_TRY(hipGetDeviceCount(&count));
-  if (device < 0 || device >= count)
-    return fail(FBLS_E_ARG, "device %d out of range (%d devices)", device, count);
-  HIP_TRY(hipSetDevice(device));
-  fbls_ctx* c = new fbls_ctx();
-  c->device = device;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess &&
-      prop.maxSharedMemoryPerMultiProcessor > 0)
-    c->max_lds = std::min<size_t>(kMaxLds, prop.maxSharedMemoryPerMultiProcessor);
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess ||
-      hipEventCreate(&c->ev1) != hipSuccess) {
-    delete c;
-    return fail(FBLS_E_HIP, "stream/event creation failed");
-  }
-  hipError_t e = fbls::scan_configure(c->max_lds);
-  if (e != hipSuccess) {
-    fbls_ctx_destroy(c);
-    return fail(FBLS_E_HIP, "kernel LDS configuration failed: %s",
-                hipGetErrorString(e));
-  }
-  *out = c;
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!out) return fail(FBLS_E_ARG, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+      return fail(FBLS_E_ARG, "device %d out of range (%d devices)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    fbls_ctx* c = new fbls_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess &&
+        prop.maxSharedMemoryPerMultiProcessor > 0)
+      c->max_lds = std::min<size_t>(kMaxLds, prop.maxSharedMemoryPerMultiProcessor);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess ||
+        hipEventCreate(&c->ev1) != hipSuccess) {
+      delete c;
+      return fail(FBLS_E_HIP, "stream/event creation failed");
+    }
+    hipError_t e = fbls::scan_configure(c->max_lds);
+    if (e != hipSuccess) {
+      fbls_ctx_destroy(c);
+      return fail(FBLS_E_HIP, "kernel LDS configuration failed: %s",
+                  hipGetErrorString(e));
+    }
+    *out = c;
+    return FBLS_OK;
+  });
 }
 
 extern "C" void fbls_ctx_destroy(fbls_ctx* c) {
@@ -92,46 +94,56 @@ extern "C" void fbls_ctx_destroy(fbls_ctx* c) {
 }
 
 extern "C" int fbls_mem_info(fbls_ctx* c, size_t* free_bytes, size_t* total) {
-  if (!c || !free_bytes || !total) return fail(FBLS_E_ARG, "null argument");
-  HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(hipMemGetInfo(free_bytes, total));
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !free_bytes || !total) return fail(FBLS_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemGetInfo(free_bytes, total));
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_ctx_set_launch(fbls_ctx* c, uint32_t lds_bytes,
                                    uint32_t block, uint32_t fold_mode) {
-  if (!c) return fail(FBLS_E_ARG, "null context");
-  if (fold_mode > 2) return fail(FBLS_E_ARG, "fold_mode must be 0, 1 or 2");
-  if (block != 0 && block != 256 && block != 512)
-    return fail(FBLS_E_ARG, "block must be 0, 256 or 512");
-  if (lds_bytes > c->max_lds)
-    return fail(FBLS_E_ARG, "lds_bytes %u exceeds %zu", lds_bytes, c->max_lds);
-  c->lds_override = lds_bytes;
-  c->block_override = block;
-  c->run_fold = fold_mode == 2 ? 0u : 1u;
-  c->count_from_weights = fold_mode == 0 ? 1u : 0u;
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c) return fail(FBLS_E_ARG, "null context");
+    if (fold_mode > 2) return fail(FBLS_E_ARG, "fold_mode must be 0, 1 or 2");
+    if (block != 0 && block != 256 && block != 512)
+      return fail(FBLS_E_ARG, "block must be 0, 256 or 512");
+    if (lds_bytes > c->max_lds)
+      return fail(FBLS_E_ARG, "lds_bytes %u exceeds %zu", lds_bytes, c->max_lds);
+    c->lds_override = lds_bytes;
+    c->block_override = block;
+    c->run_fold = fold_mode == 2 ? 0u : 1u;
+    c->count_from_weights = fold_mode == 0 ? 1u : 0u;
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_ctx_set_graphs(fbls_ctx* c, int enable, uint64_t* graph_launches) {
-  if (!c) return fail(FBLS_E_ARG, "null context");
-  if (enable >= 0) c->use_graphs = enable ? 1 : 0;
-  if (graph_launches) *graph_launches = c->graph_launches;
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c) return fail(FBLS_E_ARG, "null context");
+    if (enable >= 0) c->use_graphs = enable ? 1 : 0;
+    if (graph_launches) *graph_launches = c->graph_launches;
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_ctx_launch_totals(fbls_ctx* c, fbls_launch_totals* out,
                                       int reset) {
-  if (!c || !out) return fail(FBLS_E_ARG, "null argument");
-  *out = c->totals;
-  if (reset) c->totals = fbls_launch_totals{0, 0, 0, 0.0};
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !out) return fail(FBLS_E_ARG, "null argument");
+    *out = c->totals;
+    if (reset) c->totals = fbls_launch_totals{0, 0, 0, 0.0};
+    return FBLS_OK;
+  });
 }
 
 // ----------------------------------------------------------------- family
 
 extern "C" int fbls_quantize(const double* x, size_t n, int32_t* q) {
-  return fbls::quantize(x, n, q);
+  return fbls::host::guard([&]() -> int {
+    return fbls::quantize(x, n, q);
+  });
 }
 
 namespace {
@@ -223,18 +235,20 @@ extern "C" int fbls_family_upload(fbls_ctx* c, const double* t,
                                   const double* v, const double* w, size_t n,
                                   double baseline_norm, double origin,
                                   fbls_family** out) {
-  if (!c || !out) return fail(FBLS_E_ARG, "null argument");
-  *out = nullptr;
-  fbls_family* f = new fbls_family();
-  int st = prepare_family(c, t, v, w, n, baseline_norm, origin, f);
-  std::vector<fbls_family*> one{f};
-  if (!st) st = upload_arena(c, one, {t});
-  if (st) {
-    delete f;
-    return st;
-  }
-  *out = f;
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !out) return fail(FBLS_E_ARG, "null argument");
+    *out = nullptr;
+    fbls_family* f = new fbls_family();
+    int st = prepare_family(c, t, v, w, n, baseline_norm, origin, f);
+    std::vector<fbls_family*> one{f};
+    if (!st) st = upload_arena(c, one, {t});
+    if (st) {
+      delete f;
+      return st;
+    }
+    *out = f;
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_family_upload_packed(fbls_ctx* c, size_t nfam,
@@ -243,36 +257,38 @@ extern "C" int fbls_family_upload_packed(fbls_ctx* c, size_t nfam,
                                          const double* baseline_norm,
                                          const double* origin,
                                          fbls_family** out) {
-  if (!c || !offsets || !t || !v || !w || !baseline_norm || !origin || !out)
-    return fail(FBLS_E_ARG, "null argument");
-  std::vector<fbls_family*> fams;
-  std::vector<const double*> times;
-  int st = FBLS_OK;
-  for (size_t k = 0; k < nfam && !st; k++) {
-    out[k] = nullptr;
-    if (offsets[k + 1] < offsets[k]) {
-      st = fail(FBLS_E_ARG, "offsets must be nondecreasing");
-      break;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !offsets || !t || !v || !w || !baseline_norm || !origin || !out)
+      return fail(FBLS_E_ARG, "null argument");
+    std::vector<fbls_family*> fams;
+    std::vector<const double*> times;
+    int st = FBLS_OK;
+    for (size_t k = 0; k < nfam && !st; k++) {
+      out[k] = nullptr;
+      if (offsets[k + 1] < offsets[k]) {
+        st = fail(FBLS_E_ARG, "offsets must be nondecreasing");
+        break;
+      }
+      const size_t a = offsets[k], n = offsets[k + 1] - offsets[k];
+      fbls_family* f = new fbls_family();
+      fams.push_back(f);
+      times.push_back(t + a);
+      st = prepare_family(c, t + a, v + a, w + a, n, baseline_norm[k], origin[k], f);
+      if (st) {
+        char msg[600];
+        std::snprintf(msg, sizeof(msg), "family %zu: %s", k, fbls_last_error());
+        st = fail(st, "%s", msg);
+      }
     }
-    const size_t a = offsets[k], n = offsets[k + 1] - offsets[k];
-    fbls_family* f = new fbls_family();
-    fams.push_back(f);
-    times.push_back(t + a);
-    st = prepare_family(c, t + a, v + a, w + a, n, baseline_norm[k], origin[k], f);
+    if (!st && !fams.empty()) st = upload_arena(c, fams, times);
     if (st) {
-      char msg[600];
-      std::snprintf(msg, sizeof(msg), "family %zu: %s", k, fbls_last_error());
-      st = fail(st, "%s", msg);
+      for (fbls_family* f : fams) delete f;
+      for (size_t k = 0; k < nfam; k++) out[k] = nullptr;
+      return st;
     }
-  }
-  if (!st && !fams.empty()) st = upload_arena(c, fams, times);
-  if (st) {
-    for (fbls_family* f : fams) delete f;
-    for (size_t k = 0; k < nfam; k++) out[k] = nullptr;
-    return st;
-  }
-  for (size_t k = 0; k < nfam; k++) out[k] = fams[k];
-  return FBLS_OK;
+    for (size_t k = 0; k < nfam; k++) out[k] = fams[k];
+    return FBLS_OK;
+  });
 }
 
 extern "C" void fbls_family_free(fbls_family* f) {
@@ -282,14 +298,16 @@ extern "C" void fbls_family_free(fbls_family* f) {
 
 extern "C" int fbls_family_fields(const fbls_family* f, double* v_out,
                                   double* w_out, int* ev, int* ew) {
-  if (!f) return fail(FBLS_E_ARG, "null family");
-  for (uint32_t i = 0; i < f->n; i++) {
-    if (v_out) v_out[i] = std::ldexp((double)f->vq[i], -f->ev);
-    if (w_out) w_out[i] = std::ldexp((double)f->wq[i], -f->ew);
-  }
-  if (ev) *ev = f->ev;
-  if (ew) *ew = f->ew;
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!f) return fail(FBLS_E_ARG, "null family");
+    for (uint32_t i = 0; i < f->n; i++) {
+      if (v_out) v_out[i] = std::ldexp((double)f->vq[i], -f->ev);
+      if (w_out) w_out[i] = std::ldexp((double)f->wq[i], -f->ew);
+    }
+    if (ev) *ev = f->ev;
+    if (ew) *ew = f->ew;
+    return FBLS_OK;
+  });
 }
 
 // ------------------------------------------------------------- scan_bank
@@ -633,29 +651,31 @@ extern "C" int fbls_scan_bank(fbls_ctx* c, const fbls_family* f,
                               const fbls_bank* b, uint32_t nb, double* records,
                               double* score, double* depth, uint32_t* winner,
                               fbls_scan_stats* stats) {
-  if (!c || !f || (!periods && np) || !records || !score || !depth)
-    return fail(FBLS_E_ARG, "null argument");
-  int st = check_bank(b);
-  if (st) return st;
-  const uint64_t budget = bank_batch(nb, b->period_batch, b->max_fold_mib);
-  if (budget == 0) return fail(FBLS_E_FOLD_BUDGET, "one folded period exceeds max_fold_mib");
-  Work x;
-  x.fam = f;
-  x.periods = periods;
-  x.np = np;
-  x.nb = nb;
-  x.nd = (uint32_t)b->nd;
-  x.dur = b->durations;
-  x.rho = b->rho;
-  x.records = records;
-  x.score = score;
-  x.depth = depth;
-  x.winner = winner;
-  double secs = 0.0;
-  st = run_work(c, &x, 1, shared_of(b), &secs);
-  if (st) return st;
-  if (stats) account(stats, x, b, budget, secs);
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !f || (!periods && np) || !records || !score || !depth)
+      return fail(FBLS_E_ARG, "null argument");
+    int st = check_bank(b);
+    if (st) return st;
+    const uint64_t budget = bank_batch(nb, b->period_batch, b->max_fold_mib);
+    if (budget == 0) return fail(FBLS_E_FOLD_BUDGET, "one folded period exceeds max_fold_mib");
+    Work x;
+    x.fam = f;
+    x.periods = periods;
+    x.np = np;
+    x.nb = nb;
+    x.nd = (uint32_t)b->nd;
+    x.dur = b->durations;
+    x.rho = b->rho;
+    x.records = records;
+    x.score = score;
+    x.depth = depth;
+    x.winner = winner;
+    double secs = 0.0;
+    st = run_work(c, &x, 1, shared_of(b), &secs);
+    if (st) return st;
+    if (stats) account(stats, x, b, budget, secs);
+    return FBLS_OK;
+  });
 }
 
 namespace {
@@ -828,34 +848,36 @@ extern "C" int fbls_scan_periods(fbls_ctx* c, const fbls_family* f,
                                  const double* periods, size_t np,
                                  const fbls_bank* b, fbls_row* rows,
                                  size_t* nrows, fbls_scan_stats* stats) {
-  if (!c || !f || !nrows || (np && (!periods || !rows)))
-    return fail(FBLS_E_ARG, "null argument");
-  *nrows = 0;
-  if (np == 0 || b == nullptr || b->nd == 0) {
-    if (stats) {
-      std::memset(stats, 0, sizeof(*stats));
-      stats->periods = np;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !f || !nrows || (np && (!periods || !rows)))
+      return fail(FBLS_E_ARG, "null argument");
+    *nrows = 0;
+    if (np == 0 || b == nullptr || b->nd == 0) {
+      if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->periods = np;
+      }
+      return FBLS_OK;
     }
+    int st = check_bank(b);
+    if (st) return st;
+    std::vector<BandRun> bands(1);
+    BandRun& r = bands[0];
+    r.fam = f;
+    r.periods = periods;
+    r.np = np;
+    r.nb = band_nb(periods, np, b);
+    r.nb_coarse = (b->prescan_factor > 1 && r.nb / b->prescan_factor >= 8)
+                      ? r.nb / b->prescan_factor
+                      : 0u;
+    r.nd = (uint32_t)b->nd;
+    r.dur = b->durations;
+    r.rho = b->rho;
+    st = scan_bands(c, bands, b, rows, nrows, nullptr);
+    if (st) return st;
+    if (stats) *stats = r.stats;
     return FBLS_OK;
-  }
-  int st = check_bank(b);
-  if (st) return st;
-  std::vector<BandRun> bands(1);
-  BandRun& r = bands[0];
-  r.fam = f;
-  r.periods = periods;
-  r.np = np;
-  r.nb = band_nb(periods, np, b);
-  r.nb_coarse = (b->prescan_factor > 1 && r.nb / b->prescan_factor >= 8)
-                    ? r.nb / b->prescan_factor
-                    : 0u;
-  r.nd = (uint32_t)b->nd;
-  r.dur = b->durations;
-  r.rho = b->rho;
-  st = scan_bands(c, bands, b, rows, nrows, nullptr);
-  if (st) return st;
-  if (stats) *stats = r.stats;
-  return FBLS_OK;
+  });
 }
 
 // ---------------------------------------------------------- signal_search
@@ -869,9 +891,11 @@ extern "C" size_t fbls_period_grid_count(double minimum, double maximum,
 extern "C" int fbls_period_grid(double minimum, double maximum,
                                 double baseline, double dmin,
                                 double oversample, double* out, size_t count) {
-  if (!out || count == 0) return fail(FBLS_E_ARG, "empty grid");
-  fbls::period_grid(minimum, maximum, out, count);
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!out || count == 0) return fail(FBLS_E_ARG, "empty grid");
+    fbls::period_grid(minimum, maximum, out, count);
+    return FBLS_OK;
+  });
 }
 
 extern "C" size_t fbls_durations_days(const double* hours, size_t n,
@@ -884,16 +908,18 @@ extern "C" int fbls_plan_bands(const double* grid, size_t count,
                                uint32_t phase_oversample,
                                uint32_t prescan_factor, fbls_band* bands,
                                size_t bands_cap, size_t* nbands) {
-  if (!grid || !sorted_durations || !nbands) return fail(FBLS_E_ARG, "null argument");
-  std::vector<fbls_band> plan = fbls::plan_bands(grid, count, sorted_durations, nd,
-                                                 phase_oversample, prescan_factor);
-  *nbands = plan.size();
-  if (bands) {
-    if (plan.size() > bands_cap)
-      return fail(FBLS_E_ARG, "band capacity %zu < %zu", bands_cap, plan.size());
-    std::copy(plan.begin(), plan.end(), bands);
-  }
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!grid || !sorted_durations || !nbands) return fail(FBLS_E_ARG, "null argument");
+    std::vector<fbls_band> plan = fbls::plan_bands(grid, count, sorted_durations, nd,
+                                                   phase_oversample, prescan_factor);
+    *nbands = plan.size();
+    if (bands) {
+      if (plan.size() > bands_cap)
+        return fail(FBLS_E_ARG, "band capacity %zu < %zu", bands_cap, plan.size());
+      std::copy(plan.begin(), plan.end(), bands);
+    }
+    return FBLS_OK;
+  });
 }
 
 extern "C" uint64_t fbls_bank_batches(size_t np, uint32_t nb,
@@ -1100,41 +1126,43 @@ extern "C" int fbls_signal_search(fbls_ctx* c, const fbls_family* f,
                                   fbls_scan_stats* stats, size_t stats_cap,
                                   size_t* nbands, double* grid_out,
                                   size_t grid_cap, size_t* grid_count) {
-  if (!c || !f || !bank || !cfg || !nrows || !nbands || !grid_count)
-    return fail(FBLS_E_ARG, "null argument");
-  *nrows = 0;
-  *nbands = 0;
-  *grid_count = 0;
-  if (nd == 0) return FBLS_OK;
-  if (!durations || !rho) return fail(FBLS_E_ARG, "null argument");
-  SearchPlan p;
-  int st = plan_search(f, durations, rho, nd, bank, cfg, &p);
-  if (st) return st;
-  const size_t count = p.grid.size();
-  if (count == 0) return FBLS_OK;
-  *grid_count = count;
-  if (grid_out) {
-    if (grid_cap < count) return fail(FBLS_E_ARG, "grid capacity %zu < %zu", grid_cap, count);
-    std::memcpy(grid_out, p.grid.data(), count * sizeof(double));
-  }
-  if (rows_cap < count) return fail(FBLS_E_ARG, "rows capacity %zu < grid %zu", rows_cap, count);
-  if (stats && p.bands.size() > stats_cap)
-    return fail(FBLS_E_ARG, "band statistics capacity %zu < %zu bands", stats_cap,
-                p.bands.size());
-  fbls_bank b = budgeted(bank);
-  b.durations = p.sd.data();
-  b.rho = p.sr.data();
-  b.nd = nd;
-  st = check_bank(&b);
-  if (st) return st;
-  std::vector<BandRun> runs;
-  append_runs(p, runs);
-  st = scan_bands(c, runs, &b, rows, nrows, nullptr);
-  if (st) return st;
-  for (size_t i = 0; i < runs.size(); i++)
-    if (stats) stats[i] = runs[i].stats;
-  *nbands = runs.size();
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !f || !bank || !cfg || !nrows || !nbands || !grid_count)
+      return fail(FBLS_E_ARG, "null argument");
+    *nrows = 0;
+    *nbands = 0;
+    *grid_count = 0;
+    if (nd == 0) return FBLS_OK;
+    if (!durations || !rho) return fail(FBLS_E_ARG, "null argument");
+    SearchPlan p;
+    int st = plan_search(f, durations, rho, nd, bank, cfg, &p);
+    if (st) return st;
+    const size_t count = p.grid.size();
+    if (count == 0) return FBLS_OK;
+    *grid_count = count;
+    if (grid_out) {
+      if (grid_cap < count) return fail(FBLS_E_ARG, "grid capacity %zu < %zu", grid_cap, count);
+      std::memcpy(grid_out, p.grid.data(), count * sizeof(double));
+    }
+    if (rows_cap < count) return fail(FBLS_E_ARG, "rows capacity %zu < grid %zu", rows_cap, count);
+    if (stats && p.bands.size() > stats_cap)
+      return fail(FBLS_E_ARG, "band statistics capacity %zu < %zu bands", stats_cap,
+                  p.bands.size());
+    fbls_bank b = budgeted(bank);
+    b.durations = p.sd.data();
+    b.rho = p.sr.data();
+    b.nd = nd;
+    st = check_bank(&b);
+    if (st) return st;
+    std::vector<BandRun> runs;
+    append_runs(p, runs);
+    st = scan_bands(c, runs, &b, rows, nrows, nullptr);
+    if (st) return st;
+    for (size_t i = 0; i < runs.size(); i++)
+      if (stats) stats[i] = runs[i].stats;
+    *nbands = runs.size();
+    return FBLS_OK;
+  });
 }
 
 // Many stars (one family each) through the same two-launch band loop: the
@@ -1268,10 +1296,12 @@ extern "C" int fbls_signal_search_batch(fbls_ctx* c, const fbls_family* const* f
                                         size_t rows_cap, size_t* row_offsets,
                                         fbls_scan_stats* star_stats,
                                         const uint8_t* keep) {
-  Restriction rs;
-  rs.keep = keep;
-  return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
-                      star_stats, rs);
+  return fbls::host::guard([&]() -> int {
+    Restriction rs;
+    rs.keep = keep;
+    return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
+                        star_stats, rs);
+  });
 }
 
 extern "C" int fbls_signal_search_bins(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
@@ -1280,15 +1310,17 @@ extern "C" int fbls_signal_search_bins(fbls_ctx* c, const fbls_family* const* fa
                                        fbls_row* rows, size_t rows_cap, size_t* row_offsets,
                                        fbls_scan_stats* star_stats, const uint8_t* bin_keep,
                                        size_t nbins, const double* edges) {
-  if (!bin_keep || !edges || nbins == 0) return fail(FBLS_E_ARG, "null argument");
-  for (size_t k = 0; k < nbins; k++)
-    if (!(edges[k + 1] > edges[k])) return fail(FBLS_E_ARG, "bin edges must ascend");
-  Restriction rs;
-  rs.bin_keep = bin_keep;
-  rs.nbins = nbins;
-  rs.edges = edges;
-  return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
-                      star_stats, rs);
+  return fbls::host::guard([&]() -> int {
+    if (!bin_keep || !edges || nbins == 0) return fail(FBLS_E_ARG, "null argument");
+    for (size_t k = 0; k < nbins; k++)
+      if (!(edges[k + 1] > edges[k])) return fail(FBLS_E_ARG, "bin edges must ascend");
+    Restriction rs;
+    rs.bin_keep = bin_keep;
+    rs.nbins = nbins;
+    rs.edges = edges;
+    return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
+                        star_stats, rs);
+  });
 }
 
 extern "C" int fbls_signal_search_bins_ragged(
@@ -1296,21 +1328,23 @@ extern "C" int fbls_signal_search_bins_ragged(
     const double* rho, size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
     fbls_row* rows, size_t rows_cap, size_t* row_offsets, fbls_scan_stats* star_stats,
     const uint8_t* bin_keep, const size_t* bin_offsets, const double* edges) {
-  if (!bin_keep || !bin_offsets || !edges) return fail(FBLS_E_ARG, "null argument");
-  if (bin_offsets[0] != 0) return fail(FBLS_E_ARG, "bin offsets must start at 0");
-  for (size_t s = 0; s < nfam; s++) {
-    const size_t b0 = bin_offsets[s], b1 = bin_offsets[s + 1];
-    if (b1 <= b0) return fail(FBLS_E_ARG, "star %zu has no bins", s);
-    const double* e = edges + b0 + s;
-    for (size_t k = 0; k < b1 - b0; k++)
-      if (!(e[k + 1] > e[k])) return fail(FBLS_E_ARG, "bin edges of star %zu must ascend", s);
-  }
-  Restriction rs;
-  rs.bin_keep = bin_keep;
-  rs.edges = edges;
-  rs.bin_offsets = bin_offsets;
-  return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
-                      star_stats, rs);
+  return fbls::host::guard([&]() -> int {
+    if (!bin_keep || !bin_offsets || !edges) return fail(FBLS_E_ARG, "null argument");
+    if (bin_offsets[0] != 0) return fail(FBLS_E_ARG, "bin offsets must start at 0");
+    for (size_t s = 0; s < nfam; s++) {
+      const size_t b0 = bin_offsets[s], b1 = bin_offsets[s + 1];
+      if (b1 <= b0) return fail(FBLS_E_ARG, "star %zu has no bins", s);
+      const double* e = edges + b0 + s;
+      for (size_t k = 0; k < b1 - b0; k++)
+        if (!(e[k + 1] > e[k])) return fail(FBLS_E_ARG, "bin edges of star %zu must ascend", s);
+    }
+    Restriction rs;
+    rs.bin_keep = bin_keep;
+    rs.edges = edges;
+    rs.bin_offsets = bin_offsets;
+    return batch_search(c, fams, nfam, durations, rho, nd, bank, cfg, rows, rows_cap, row_offsets,
+                        star_stats, rs);
+  });
 }
 
 // ------------------------------------------------------------ host helpers
@@ -1323,23 +1357,27 @@ extern "C" int fbls_family_state(const double* time, const double* flux,
                                  double* baseline_norm, double* origin,
                                  double* rho, double* inflation,
                                  size_t* segments, double* residual_energy) {
-  std::string msg;
-  double energy = 0.0;
-  int st = fbls::family_state(time, flux, sigma, n, durations, nd, correlated,
-                              t, v, w, out_n, baseline_norm, origin, rho,
-                              inflation, segments, &energy, &msg);
-  if (residual_energy) *residual_energy = energy;
-  if (st) return fail(st, "%s", msg.c_str());
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    std::string msg;
+    double energy = 0.0;
+    int st = fbls::family_state(time, flux, sigma, n, durations, nd, correlated,
+                                t, v, w, out_n, baseline_norm, origin, rho,
+                                inflation, segments, &energy, &msg);
+    if (residual_energy) *residual_energy = energy;
+    if (st) return fail(st, "%s", msg.c_str());
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_synth_star(const fbls_synth_cfg* cfg, double* time,
                                double* flux, double* flux_err, size_t cap,
                                size_t* n) {
-  std::string msg;
-  int st = fbls::synth_star(cfg, time, flux, flux_err, cap, n, &msg);
-  if (st) return fail(st, "%s", msg.c_str());
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    std::string msg;
+    int st = fbls::synth_star(cfg, time, flux, flux_err, cap, n, &msg);
+    if (st) return fail(st, "%s", msg.c_str());
+    return FBLS_OK;
+  });
 }
 
 // ---------------------------------------------------------------- autotune
@@ -1353,57 +1391,59 @@ extern "C" int fbls_autotune(fbls_ctx* c, const fbls_family* f, const double* du
                              const fbls_search_cfg* cfg, uint32_t repeats,
                              fbls_tune_entry* table, size_t table_cap, size_t* ntable,
                              fbls_tune_entry* best) {
-  if (!c || !f || !bank || !cfg || !ntable || !best) return fail(FBLS_E_ARG, "null argument");
-  *ntable = 0;
-  const uint32_t lds_opts[] = {16384, 24576, 32768, 49152, 65536, 98304};
-  const uint32_t block_opts[] = {256, 512};
-  const uint32_t saved_lds = c->lds_override, saved_block = c->block_override;
-  const double d0 = *std::min_element(durations, durations + nd);
-  const double hi = std::min(cfg->period_max, f->t_last);
-  const double lo = std::max(cfg->period_min, 4.0 * d0 * (1.0 + 1e-9));
-  const size_t cap = hi > lo ? std::max<size_t>(1, fbls::period_grid_count(
-                                                       lo, hi, f->t_last, d0,
-                                                       cfg->period_oversample))
-                             : 1;
-  std::vector<fbls_row> rows(cap), ref;
-  std::vector<fbls_scan_stats> st(64);
-  size_t nrows = 0, nbands = 0, gc = 0;
-  best->seconds = INFINITY;
-  int status = FBLS_OK;
-  for (uint32_t block : block_opts) {
-    for (uint32_t lds : lds_opts) {
-      if (lds > c->max_lds) continue;
-      c->lds_override = lds;
-      c->block_override = block;
-      double secs = INFINITY;
-      for (uint32_t r = 0; r <= std::max<uint32_t>(repeats, 1); r++) {
-        fbls_launch_totals before = c->totals;
-        status = fbls_signal_search(c, f, durations, rho, nd, bank, cfg, rows.data(), cap,
-                                    &nrows, st.data(), st.size(), &nbands, nullptr, 0, &gc);
+  return fbls::host::guard([&]() -> int {
+    if (!c || !f || !bank || !cfg || !ntable || !best) return fail(FBLS_E_ARG, "null argument");
+    *ntable = 0;
+    const uint32_t lds_opts[] = {16384, 24576, 32768, 49152, 65536, 98304};
+    const uint32_t block_opts[] = {256, 512};
+    const uint32_t saved_lds = c->lds_override, saved_block = c->block_override;
+    const double d0 = *std::min_element(durations, durations + nd);
+    const double hi = std::min(cfg->period_max, f->t_last);
+    const double lo = std::max(cfg->period_min, 4.0 * d0 * (1.0 + 1e-9));
+    const size_t cap = hi > lo ? std::max<size_t>(1, fbls::period_grid_count(
+                                                         lo, hi, f->t_last, d0,
+                                                         cfg->period_oversample))
+                               : 1;
+    std::vector<fbls_row> rows(cap), ref;
+    std::vector<fbls_scan_stats> st(64);
+    size_t nrows = 0, nbands = 0, gc = 0;
+    best->seconds = INFINITY;
+    int status = FBLS_OK;
+    for (uint32_t block : block_opts) {
+      for (uint32_t lds : lds_opts) {
+        if (lds > c->max_lds) continue;
+        c->lds_override = lds;
+        c->block_override = block;
+        double secs = INFINITY;
+        for (uint32_t r = 0; r <= std::max<uint32_t>(repeats, 1); r++) {
+          fbls_launch_totals before = c->totals;
+          status = fbls_signal_search(c, f, durations, rho, nd, bank, cfg, rows.data(), cap,
+                                      &nrows, st.data(), st.size(), &nbands, nullptr, 0, &gc);
+          if (status) break;
+          if (r > 0) secs = std::min(secs, c->totals.seconds - before.seconds);  // r = 0 warms up
+        }
         if (status) break;
-        if (r > 0) secs = std::min(secs, c->totals.seconds - before.seconds);  // r = 0 warms up
+        if (ref.empty()) {
+          ref.assign(rows.begin(), rows.begin() + nrows);
+        } else if (nrows != ref.size() ||
+                   std::memcmp(ref.data(), rows.data(), nrows * sizeof(fbls_row)) != 0) {
+          status = fail(FBLS_E_HIP, "launch shape lds %u block %u changed the rows", lds, block);
+          break;
+        }
+        fbls_tune_entry e{lds, block, secs};
+        if (table && *ntable < table_cap) table[*ntable] = e;
+        (*ntable)++;
+        if (secs < best->seconds) *best = e;
       }
       if (status) break;
-      if (ref.empty()) {
-        ref.assign(rows.begin(), rows.begin() + nrows);
-      } else if (nrows != ref.size() ||
-                 std::memcmp(ref.data(), rows.data(), nrows * sizeof(fbls_row)) != 0) {
-        status = fail(FBLS_E_HIP, "launch shape lds %u block %u changed the rows", lds, block);
-        break;
-      }
-      fbls_tune_entry e{lds, block, secs};
-      if (table && *ntable < table_cap) table[*ntable] = e;
-      (*ntable)++;
-      if (secs < best->seconds) *best = e;
     }
-    if (status) break;
-  }
-  if (status) {
-    c->lds_override = saved_lds;
-    c->block_override = saved_block;
-    return status;
-  }
-  c->lds_override = best->lds_bytes;
-  c->block_override = best->block;
-  return FBLS_OK;
+    if (status) {
+      c->lds_override = saved_lds;
+      c->block_override = saved_block;
+      return status;
+    }
+    c->lds_override = best->lds_bytes;
+    c->block_override = best->block;
+    return FBLS_OK;
+  });
 }

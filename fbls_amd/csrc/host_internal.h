@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <exception>
 #include <memory>
+#include <new>
 #include <vector>
 
 #include "../../include/fbls.h"
@@ -15,6 +17,20 @@ namespace host {
 
 // Records the thread's last error message and returns code.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Runs an entry point's body: a C++ exception (host memory exhausted, a
+// container's size limit) becomes an error status instead of unwinding into
+// the C caller's frames.
+template <class Body>
+int guard(Body&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(FBLS_E_OOM, "host memory allocation failed");
+  } catch (const std::exception& e) {
+    return fail(FBLS_E_ARG, "%s", e.what());
+  }
+}
 double now_s();
 // The network's device input buffer for a B x L batch (channels-last
 // float32, in_channels per bin), and the fused inference from it into host

@@ -743,33 +743,35 @@ extern "C" int fbls_search_prepared(fbls_ctx* c, const double* time, const doubl
                                     size_t peaks_cap, size_t* npeaks,
                                     fbls_pass_stats* passes, size_t passes_cap,
                                     size_t* npasses, fbls_search_report* report) {
-  const double start = now_s();
-  if (!c || (n && (!time || !flux || !err)) || !npeaks || !npasses)
-    return fail(FBLS_E_ARG, "null argument");
-  *npeaks = 0;
-  *npasses = 0;
-  if (int st = check_pipeline(cfg)) return st;
-  // pipeline.rs:47-61
-  std::vector<size_t> order;
-  for (size_t i = 0; i < n; i++)
-    if (std::isfinite(time[i]) && std::isfinite(flux[i]) && std::isfinite(err[i]) && err[i] > 0.0)
-      order.push_back(i);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t b) { return time[a] < time[b]; });
-  Curve curve;
-  for (size_t i : order) {
-    curve.t.push_back(time[i]);
-    curve.f.push_back(flux[i]);
-    curve.s.push_back(err[i]);
-  }
-  HIP_TRY(hipSetDevice(c->device));
-  std::vector<size_t> all(cfg->nd);
-  for (size_t i = 0; i < cfg->nd; i++) all[i] = i;
-  std::vector<FamInput> input(1);
-  input[0].curve = std::move(curve);
-  input[0].indices = std::move(all);
-  return search_families(c, std::move(input), cfg, start, peaks, peaks_cap, npeaks, passes,
-                         passes_cap, npasses, report);
+  return fbls::host::guard([&]() -> int {
+    const double start = now_s();
+    if (!c || (n && (!time || !flux || !err)) || !npeaks || !npasses)
+      return fail(FBLS_E_ARG, "null argument");
+    *npeaks = 0;
+    *npasses = 0;
+    if (int st = check_pipeline(cfg)) return st;
+    // pipeline.rs:47-61
+    std::vector<size_t> order;
+    for (size_t i = 0; i < n; i++)
+      if (std::isfinite(time[i]) && std::isfinite(flux[i]) && std::isfinite(err[i]) && err[i] > 0.0)
+        order.push_back(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return time[a] < time[b]; });
+    Curve curve;
+    for (size_t i : order) {
+      curve.t.push_back(time[i]);
+      curve.f.push_back(flux[i]);
+      curve.s.push_back(err[i]);
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<size_t> all(cfg->nd);
+    for (size_t i = 0; i < cfg->nd; i++) all[i] = i;
+    std::vector<FamInput> input(1);
+    input[0].curve = std::move(curve);
+    input[0].indices = std::move(all);
+    return search_families(c, std::move(input), cfg, start, peaks, peaks_cap, npeaks, passes,
+                           passes_cap, npasses, report);
+  });
 }
 
 extern "C" int fbls_search_transits(fbls_ctx* c, const double* time, const double* flux,
@@ -778,33 +780,35 @@ extern "C" int fbls_search_transits(fbls_ctx* c, const double* time, const doubl
                                     fbls_peak* peaks, size_t peaks_cap, size_t* npeaks,
                                     fbls_pass_stats* passes, size_t passes_cap,
                                     size_t* npasses, fbls_search_report* report) {
-  const double start = now_s();
-  if (!c || !prep || (n && (!time || !flux)) || !npeaks || !npasses)
-    return fail(FBLS_E_ARG, "null argument");
-  *npeaks = 0;
-  *npasses = 0;
-  if (int st = check_pipeline(cfg)) return st;
-  HIP_TRY(hipSetDevice(c->device));
-  std::vector<fbls::Family> fams;
-  std::string msg;
-  auto flatten = [c](const std::vector<double>& t, const std::vector<double>& rel, double w,
-                     double gap, double cst, int it, std::vector<double>* out) {
-    return biweight_gpu(c, t, rel, w, gap, cst, it, out);
-  };
-  int st = fbls::prepare_families(time, flux, err, group, n, cfg->durations, cfg->nd, prep,
-                                  flatten, &fams, &msg);
-  if (st) return msg.empty() ? st : fail(st, "%s", msg.c_str());
-  std::vector<FamInput> input;
-  for (fbls::Family& f : fams) {
-    FamInput in;
-    in.curve.t = std::move(f.time);
-    in.curve.f = std::move(f.flux);
-    in.curve.s = std::move(f.sigma);
-    in.indices = std::move(f.indices);
-    in.window = f.window;
-    in.clipped = f.clipped;
-    input.push_back(std::move(in));
-  }
-  return search_families(c, std::move(input), cfg, start, peaks, peaks_cap, npeaks, passes,
-                         passes_cap, npasses, report);
+  return fbls::host::guard([&]() -> int {
+    const double start = now_s();
+    if (!c || !prep || (n && (!time || !flux)) || !npeaks || !npasses)
+      return fail(FBLS_E_ARG, "null argument");
+    *npeaks = 0;
+    *npasses = 0;
+    if (int st = check_pipeline(cfg)) return st;
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<fbls::Family> fams;
+    std::string msg;
+    auto flatten = [c](const std::vector<double>& t, const std::vector<double>& rel, double w,
+                       double gap, double cst, int it, std::vector<double>* out) {
+      return biweight_gpu(c, t, rel, w, gap, cst, it, out);
+    };
+    int st = fbls::prepare_families(time, flux, err, group, n, cfg->durations, cfg->nd, prep,
+                                    flatten, &fams, &msg);
+    if (st) return msg.empty() ? st : fail(st, "%s", msg.c_str());
+    std::vector<FamInput> input;
+    for (fbls::Family& f : fams) {
+      FamInput in;
+      in.curve.t = std::move(f.time);
+      in.curve.f = std::move(f.flux);
+      in.curve.s = std::move(f.sigma);
+      in.indices = std::move(f.indices);
+      in.window = f.window;
+      in.clipped = f.clipped;
+      input.push_back(std::move(in));
+    }
+    return search_families(c, std::move(input), cfg, start, peaks, peaks_cap, npeaks, passes,
+                           passes_cap, npasses, report);
+  });
 }

@@ -126,39 +126,45 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
 extern "C" int fbls_coarse_features(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
                                     const double* periods, size_t np, const double* durations,
                                     size_t ng, uint32_t nb_min, uint32_t nb_max, double* out) {
-  if (!out) return fail(FBLS_E_ARG, "null argument");
-  return coarse(c, fams, nfam, periods, np, durations, ng, nb_min, nb_max, out, nullptr);
+  return fbls::host::guard([&]() -> int {
+    if (!out) return fail(FBLS_E_ARG, "null argument");
+    return coarse(c, fams, nfam, periods, np, durations, ng, nb_min, nb_max, out, nullptr);
+  });
 }
 
 extern "C" int fbls_coarse_inputs(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
                                   const double* periods, size_t np, const double* durations,
                                   size_t ng, uint32_t nb_min, uint32_t nb_max, float* out) {
-  if (!out) return fail(FBLS_E_ARG, "null argument");
-  return coarse(c, fams, nfam, periods, np, durations, ng, nb_min, nb_max, nullptr, out);
+  return fbls::host::guard([&]() -> int {
+    if (!out) return fail(FBLS_E_ARG, "null argument");
+    return coarse(c, fams, nfam, periods, np, durations, ng, nb_min, nb_max, nullptr, out);
+  });
 }
 
 extern "C" int fbls_propose(fbls_ctx* c, fbls_cnn* net, const fbls_family* const* fams,
                             size_t nfam, const double* periods, size_t np,
                             const double* durations, size_t ng, uint32_t nb_min,
                             uint32_t nb_max, float* logits) {
-  if (!c || !net || !logits) return fail(FBLS_E_ARG, "null argument");
-  if (nfam == 0 || np == 0) return FBLS_OK;
-  if (fbls::host::cnn_context(net) != c)
-    return fail(FBLS_E_ARG, "the network belongs to another context");
-  if (fbls::host::cnn_in_channels(net) != ng)
-    return fail(FBLS_E_ARG, "the network reads %u durations, not %zu",
-                fbls::host::cnn_in_channels(net), ng);
-  // in batches of stars whose activations stay within a bounded footprint
-  const size_t per = std::max<size_t>(1, ((size_t)1 << 22) / np);
-  for (size_t s0 = 0; s0 < nfam; s0 += per) {
-    const size_t m = std::min(per, nfam - s0);
-    float* staged = nullptr;
-    if (int st = fbls::host::cnn_stage_input(net, (uint32_t)m, (uint32_t)np, &staged)) return st;
-    if (int st = coarse(c, fams + s0, m, periods, np, durations, ng, nb_min, nb_max, nullptr,
-                        nullptr, staged))
-      return st;
-    if (int st = fbls::host::cnn_run_staged(net, (uint32_t)m, (uint32_t)np, logits + s0 * np))
-      return st;
-  }
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !net || !logits) return fail(FBLS_E_ARG, "null argument");
+    if (nfam == 0 || np == 0) return FBLS_OK;
+    if (fbls::host::cnn_context(net) != c)
+      return fail(FBLS_E_ARG, "the network belongs to another context");
+    if (fbls::host::cnn_in_channels(net) != ng)
+      return fail(FBLS_E_ARG, "the network reads %u durations, not %zu",
+                  fbls::host::cnn_in_channels(net), ng);
+    // in batches of stars whose activations stay within a bounded footprint
+    const size_t per = std::max<size_t>(1, ((size_t)1 << 22) / np);
+    for (size_t s0 = 0; s0 < nfam; s0 += per) {
+      const size_t m = std::min(per, nfam - s0);
+      float* staged = nullptr;
+      if (int st = fbls::host::cnn_stage_input(net, (uint32_t)m, (uint32_t)np, &staged)) return st;
+      if (int st = coarse(c, fams + s0, m, periods, np, durations, ng, nb_min, nb_max, nullptr,
+                          nullptr, staged))
+        return st;
+      if (int st = fbls::host::cnn_run_staged(net, (uint32_t)m, (uint32_t)np, logits + s0 * np))
+        return st;
+    }
+    return FBLS_OK;
+  });
 }

@@ -93,15 +93,17 @@ int biweight_gpu(fbls_ctx* c, const std::vector<double>& t, const std::vector<do
 extern "C" int fbls_biweight_flatten(fbls_ctx* c, const double* time, const double* rel,
                                      size_t n, double window, double gap, double cst,
                                      uint32_t iterations, double* out) {
-  if (!c || (n && (!time || !rel || !out))) return fail(FBLS_E_ARG, "null argument");
-  if (!(window >= 0.0) || !std::isfinite(window))
-    return fail(FBLS_E_ARG, "window must be finite and non-negative");
-  for (size_t i = 1; i < n; i++)
-    if (!(time[i] >= time[i - 1])) return fail(FBLS_E_ARG, "times must be sorted and finite");
-  std::vector<double> t(time, time + n), r(rel, rel + n), o;
-  if (int st = biweight_gpu(c, t, r, window, gap, cst, (int)iterations, &o)) return st;
-  if (n) std::memcpy(out, o.data(), n * 8);
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c || (n && (!time || !rel || !out))) return fail(FBLS_E_ARG, "null argument");
+    if (!(window >= 0.0) || !std::isfinite(window))
+      return fail(FBLS_E_ARG, "window must be finite and non-negative");
+    for (size_t i = 1; i < n; i++)
+      if (!(time[i] >= time[i - 1])) return fail(FBLS_E_ARG, "times must be sorted and finite");
+    std::vector<double> t(time, time + n), r(rel, rel + n), o;
+    if (int st = biweight_gpu(c, t, r, window, gap, cst, (int)iterations, &o)) return st;
+    if (n) std::memcpy(out, o.data(), n * 8);
+    return FBLS_OK;
+  });
 }
 
 extern "C" int fbls_prepare_families(fbls_ctx* c, const double* time, const double* flux,
@@ -111,31 +113,33 @@ extern "C" int fbls_prepare_families(fbls_ctx* c, const double* time, const doub
                                      double* fam_time, double* fam_flux, double* fam_sigma,
                                      double* fam_window, size_t* fam_clipped,
                                      uint8_t* fam_members) {
-  if (!c || !cfg || !nfam || (n && (!time || !flux)) || (nd && !durations) || !fam_n ||
-      !fam_time || !fam_flux || !fam_sigma || !fam_window || !fam_clipped || !fam_members)
-    return fail(FBLS_E_ARG, "null argument");
-  *nfam = 0;
-  std::vector<fbls::Family> fams;
-  std::string msg;
-  auto flatten = [c](const std::vector<double>& t, const std::vector<double>& rel, double w,
-                     double gap, double cst, int it, std::vector<double>* out) {
-    return biweight_gpu(c, t, rel, w, gap, cst, it, out);
-  };
-  int st = fbls::prepare_families(time, flux, err, group, n, durations, nd, cfg, flatten,
-                                  &fams, &msg);
-  if (st) return msg.empty() ? st : fail(st, "%s", msg.c_str());
-  for (size_t k = 0; k < fams.size(); k++) {
-    const fbls::Family& f = fams[k];
-    const size_t m = f.time.size();
-    fam_n[k] = m;
-    std::memcpy(fam_time + k * n, f.time.data(), m * 8);
-    std::memcpy(fam_flux + k * n, f.flux.data(), m * 8);
-    std::memcpy(fam_sigma + k * n, f.sigma.data(), m * 8);
-    fam_window[k] = f.window;
-    fam_clipped[k] = f.clipped;
-    for (size_t i = 0; i < nd; i++) fam_members[k * nd + i] = 0;
-    for (size_t i : f.indices) fam_members[k * nd + i] = 1;
-  }
-  *nfam = fams.size();
-  return FBLS_OK;
+  return fbls::host::guard([&]() -> int {
+    if (!c || !cfg || !nfam || (n && (!time || !flux)) || (nd && !durations) || !fam_n ||
+        !fam_time || !fam_flux || !fam_sigma || !fam_window || !fam_clipped || !fam_members)
+      return fail(FBLS_E_ARG, "null argument");
+    *nfam = 0;
+    std::vector<fbls::Family> fams;
+    std::string msg;
+    auto flatten = [c](const std::vector<double>& t, const std::vector<double>& rel, double w,
+                       double gap, double cst, int it, std::vector<double>* out) {
+      return biweight_gpu(c, t, rel, w, gap, cst, it, out);
+    };
+    int st = fbls::prepare_families(time, flux, err, group, n, durations, nd, cfg, flatten,
+                                    &fams, &msg);
+    if (st) return msg.empty() ? st : fail(st, "%s", msg.c_str());
+    for (size_t k = 0; k < fams.size(); k++) {
+      const fbls::Family& f = fams[k];
+      const size_t m = f.time.size();
+      fam_n[k] = m;
+      std::memcpy(fam_time + k * n, f.time.data(), m * 8);
+      std::memcpy(fam_flux + k * n, f.flux.data(), m * 8);
+      std::memcpy(fam_sigma + k * n, f.sigma.data(), m * 8);
+      fam_window[k] = f.window;
+      fam_clipped[k] = f.clipped;
+      for (size_t i = 0; i < nd; i++) fam_members[k * nd + i] = 0;
+      for (size_t i : f.indices) fam_members[k * nd + i] = 1;
+    }
+    *nfam = fams.size();
+    return FBLS_OK;
+  });
 }

@@ -15,7 +15,8 @@
  *  - One fbls_ctx per host thread (scan.rs:47-64 keeps one `Gpu` per thread);
  *    distinct contexts may be used concurrently, one context must not be.
  *  - Status codes: FBLS_OK, or one of FBLS_E_*; the message is available from
- *    fbls_last_error() on the calling thread.  Infinite scores are data
+ *    fbls_last_error() on the calling thread.  No C++ exception leaves an
+ *    entry point (host allocation failure returns FBLS_E_OOM).  Infinite scores are data
  *    (score -inf when no template is usable, kernels.rs:823-827).
  *  - Functions marked [host] run on the CPU only and need no GPU.
  */
@@ -35,7 +36,7 @@ enum {
   FBLS_E_NONPOSITIVE_PRECISION = 2,  /* scan.rs:335-337 */
   FBLS_E_HIP = 3,                    /* device/runtime failure */
   FBLS_E_ARG = 4,                    /* invalid argument / configuration */
-  FBLS_E_OOM = 5,                    /* device allocation failed */
+  FBLS_E_OOM = 5,                    /* device or host allocation failed */
   FBLS_E_DATA = 6                    /* too few observations, bad flux, ... */
 };
 
