@@ -205,3 +205,50 @@ def test_missing_library_raises():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "raised:" in r.stdout and "build" in r.stdout
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/fbls.h compiles as strict C99 and C11 (the language cgo and
+    bindgen read), and a C program linked against the library calls the
+    host entry points: the grid's first and last periods are exactly the
+    bounds, and a double prints as its shortest text."""
+    import shutil
+    import subprocess
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "probe.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stdlib.h>
+#include "fbls.h"
+int main(void) {
+  size_t n = fbls_period_grid_count(0.5, 20.0, 50.0, 0.02, 3.0);
+  double* g = (double*)malloc(n * sizeof(double));
+  char text[32];
+  if (n < 2 || !g) return 2;
+  if (fbls_period_grid(0.5, 20.0, 50.0, 0.02, 3.0, g, n) != FBLS_OK) return 3;
+  fbls_format_f64(0.1, text, sizeof text);
+  printf("%zu %.17g %.17g %s\n", n, g[0], g[n - 1], text);
+  free(g);
+  return 0;
+}
+''')
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.dirname(F.LIB_PATH)
+    for std in ("c99", "c11"):
+        r = subprocess.run([gcc, f"-std={std}", "-Wall", "-Wextra", "-Werror", "-pedantic",
+                            "-fsyntax-only", "-I", inc, str(src)],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    exe = tmp_path / "probe"
+    r = subprocess.run([gcc, "-std=c99", "-I", inc, str(src), "-o", str(exe),
+                        "-L", libdir, "-lfbls_hip", f"-Wl,-rpath,{libdir}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    n, first, last, text = r.stdout.split()
+    assert int(n) == len(F.period_grid(0.5, 20.0, 50.0, 0.02, 3.0))
+    assert float(first) == 0.5 and float(last) == 20.0 and text == "0.1"
