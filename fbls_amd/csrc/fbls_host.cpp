@@ -15,6 +15,7 @@
 
 #include "../../include/fbls.h"
 #include "prep.h"
+#include "scan_host.h"
 #include "scan_kernel.h"
 
 #include "host_internal.h"
@@ -312,9 +313,7 @@ extern "C" int fbls_family_fields(const fbls_family* f, double* v_out,
 
 // ------------------------------------------------------------- scan_bank
 
-namespace {
-
-int check_bank(const fbls_bank* b) {
+int fbls::host::check_bank(const fbls_bank* b) {
   if (!b) return fail(FBLS_E_ARG, "null bank");
   if (b->nd == 0 || b->nd > 32)
     return fail(FBLS_E_ARG, "1..32 durations per band are supported, got %zu", b->nd);
@@ -330,40 +329,42 @@ int check_bank(const fbls_bank* b) {
   return FBLS_OK;
 }
 
-// scan.rs:177-183: periods per batch under the fold budget (0 = too small).
-uint64_t bank_batch(uint32_t nb, uint32_t period_batch, double max_fold_mib) {
+uint64_t fbls::host::bank_batch(uint32_t nb, uint32_t period_batch, double max_fold_mib) {
   double per_period =
       (double)(3 * (4 * (uint64_t)nb + 2) * 8 + 3 * (2 * (uint64_t)nb + 1) * 8);
   return (uint64_t)std::floor((max_fold_mib * 1048576.0) / per_period);
 }
 
-// One unit of scan work: the periods of one reference scan_bank call.
-struct Work {
-  const fbls_family* fam;
-  const double* periods;
-  size_t np;
-  uint32_t nb, nd;
-  const double* dur;  // nd (ascending)
-  const double* rho;  // nd
-  double* records;    // np x 9 (host)
-  double* score;      // np
-  double* depth;      // np
-  uint32_t* winner;   // np x 4 or null
-  uint32_t retries = 0;
-  double cost = 0.0;  // work estimate used to apportion launch time
-  bool score_only = false;  // the caller reads only score (records, depth stay on the device)
-};
+fbls::host::Shared fbls::host::shared_of(const fbls_bank* b) {
+  Shared sh;
+  sh.ns = (uint32_t)b->ns;
+  sh.layers = b->layers;
+  sh.ingress = b->ingress;
+  sh.gate = b->shape_gate;
+  return sh;
+}
 
-struct Shared {
-  uint32_t ns, layers;
-  const double* ingress;
-  double gate;
-};
+bool fbls::host::needs_split(const fbls_ctx* c, const Work& x, const Shared& sh) {
+  return fbls::scan_lds_bytes(x.nb, 1, x.nd, sh.ns, sh.layers, nullptr) > c->max_lds;
+}
+
+int fbls::host::check_periods(const Work& x, double* smax) {
+  double s = 0.0;
+  for (size_t k = 0; k < x.np; k++) {
+    const double P = x.periods[k];
+    if (!(P > 0.0) || !std::isfinite(P))
+      return fail(FBLS_E_ARG, "periods must be positive and finite");
+    s = std::max(s, (double)x.nb / P);  // scan.rs:204
+  }
+  if (x.fam->t_max * s >= 4294967295.0) return fail(FBLS_E_ARG, "t * nb / P exceeds 2^32");
+  *smax = s;
+  return FBLS_OK;
+}
+
+namespace {
 
 size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// Runs every Work in one kernel launch.  Writes each Work's host outputs and
-// retry count; returns the launch's device time through *seconds.
 // Longest template of a segment, in bins: max over its periods and
 // durations of min(max(1, floor(D nb / P + 0.5)), nb / 2) (kernels.rs:679-680).
 uint32_t longest_template(const Work& x) {
@@ -375,12 +376,127 @@ uint32_t longest_template(const Work& x) {
   return std::min(raw, x.nb / 2);
 }
 
-bool needs_split(const fbls_ctx* c, const Work& x, const Shared& sh) {
-  return fbls::scan_lds_bytes(x.nb, 1, x.nd, sh.ns, sh.layers, nullptr) > c->max_lds;
+}  // namespace
+
+int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, const Shared& sh,
+                               bool split, SegShape* shapes, size_t* lds_max,
+                               uint32_t* groups) {
+  const uint32_t ns = sh.ns, layers = sh.layers;
+  size_t total = 0;
+  for (size_t i = 0; i < nw; i++) total += w[i].np;
+  // the launch's LDS budget: the configured one, raised to the largest
+  // single-period need of its segments
+  size_t lds = c->lds_override ? c->lds_override : 32 * 1024;
+  for (size_t i = 0; i < nw; i++) {
+    shapes[i] = SegShape();
+    if (w[i].np == 0) continue;
+    if (w[i].nb < 2 || w[i].nb >= 65536)
+      return fail(FBLS_E_ARG, "phase bins must be in [2, 65535], got %u", w[i].nb);
+    if (split) {
+      if (sh.ingress[0] != 0.0)
+        return fail(FBLS_E_ARG, "phase bins beyond the LDS need ingress[0] = 0");
+      const uint32_t halo = longest_template(w[i]);
+      // widest multiple of 128 bins whose window fits the LDS
+      uint32_t B = 128;
+      while (fbls::split_lds_bytes(w[i].nb, B + 128, halo, w[i].nd, ns, layers) <=
+                 c->max_lds &&
+             B < w[i].nb)
+        B += 128;
+      if (fbls::split_lds_bytes(w[i].nb, B, halo, w[i].nd, ns, layers) > c->max_lds)
+        return fail(FBLS_E_ARG, "templates of %u bins do not fit the LDS", halo);
+      shapes[i].halo = halo;
+      shapes[i].range = B;
+      shapes[i].lds = fbls::split_lds_bytes(w[i].nb, B, halo, w[i].nd, ns, layers);
+    } else {
+      shapes[i].lds = fbls::scan_lds_bytes(w[i].nb, 1, w[i].nd, ns, layers, nullptr);
+      if (shapes[i].lds > c->max_lds)
+        return fail(FBLS_E_ARG, "%u phase bins need %zu bytes of LDS per period (limit %zu)",
+                    w[i].nb, shapes[i].lds, c->max_lds);
+    }
+    lds = std::max(lds, shapes[i].lds);
+  }
+  // at most total / 1024 periods per workgroup, so the grid fills 256 CUs
+  const uint32_t g_cap = (uint32_t)std::max<size_t>(1, total / 1024);
+  *lds_max = 0;
+  *groups = 0;
+  for (size_t i = 0; i < nw; i++) {
+    const Work& x = w[i];
+    if (x.np == 0) continue;
+    SegShape& g = shapes[i];
+    if (split) {
+      g.G = 1;
+      *lds_max = std::max(*lds_max, g.lds);
+    } else {
+      uint32_t G = (uint32_t)std::max<size_t>(1, lds / g.lds);
+      G = std::min<uint32_t>(std::min<uint32_t>(G, 16), g_cap);
+      while (G > 1 && fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr) > lds) G--;
+      g.G = G;
+      *lds_max = std::max(*lds_max, fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, &g.pstride));
+    }
+    *groups += (uint32_t)((x.np + g.G - 1) / g.G);
+  }
+  return FBLS_OK;
 }
 
+uint32_t fbls::host::launch_block(const fbls_ctx* c, bool split, uint32_t groups) {
+  // 256 threads: measured fastest on MI355X for every LDS budget 16-48 KiB
+  // when the grid fills the chip; a grid of fewer than two workgroups per CU
+  // (a fine pass over a few hundred prescan survivors) of the LDS-resident
+  // kernel gets 512 threads, so each of its periods folds with twice the waves
+  return c->block_override ? c->block_override : (!split && groups < 512 ? 512 : 256);
+}
+
+void fbls::host::segment_fields(const fbls_ctx* c, const Work& x, const SegShape& g,
+                                bool split, double smax, fbls::Segment* S) {
+  const fbls_family* f = x.fam;
+  std::memset(S, 0, sizeof(*S));
+  S->t = f->t_dev;
+  S->vw = f->vw_dev;
+  S->sv = std::ldexp(1.0, -f->ev);
+  S->sw = std::ldexp(1.0, -f->ew);
+  S->C = f->C;
+  S->invC = 1.0 / f->C;
+  S->n = f->n;
+  S->n_pad = f->n_pad;
+  S->np = (uint32_t)x.np;
+  S->nb = x.nb;
+  S->nd = x.nd;
+  S->G = g.G;
+  S->pstride = g.pstride;
+  if (split) {
+    S->range_bins = g.range;
+    S->halo_bins = g.halo;
+  }
+  S->wide_cells = (f->t_max * smax >= 16777215.0) ? 1u : 0u;
+  S->sorted = f->sorted ? 1u : 0u;
+  S->uniform_w = c->count_from_weights ? f->w_uniform : 0;
+}
+
+void fbls::host::bank_factors(const Shared& sh, double* factors) {
+  for (uint32_t s = 0; s < sh.ns; s++)
+    for (uint32_t l = 0; l < sh.layers; l++)  // scan.rs:157-171
+      factors[s * sh.layers + l] =
+          1.0 - 2.0 * sh.ingress[s] *
+                    (sh.layers > 1 ? (double)l / (double)(sh.layers - 1) : 0.0);
+}
+
+void fbls::host::account(fbls_scan_stats* st, const Work& x, const fbls_bank* b,
+                         uint64_t budget, double seconds) {
+  st->phase_bins = x.nb;
+  uint64_t batch = std::min<uint64_t>(std::min<uint64_t>(b->period_batch, budget), x.np);
+  if (batch) st->batches += (x.np + batch - 1) / batch;
+  st->templates_scanned += (uint64_t)x.np * x.nd * b->ns * x.nb;  // scan.rs:352
+  st->exact_tile_retries += x.retries;
+  st->seconds_device += seconds;
+  st->seconds_scan += seconds;
+}
+
+namespace {
+
 // One kernel launch over the Works: the LDS-resident kernel, or (split) the
-// range-split kernel for periods whose histogram exceeds the LDS.
+// range-split kernel for periods whose histogram exceeds the LDS.  Writes
+// each Work's host outputs and retry count; returns the launch's device time
+// through *seconds.
 int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
                double* seconds) {
   *seconds = 0.0;
@@ -390,42 +506,13 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   if (total > 0x7fffffffu) return fail(FBLS_E_ARG, "too many periods in one launch");
   HIP_TRY(hipSetDevice(c->device));
   const uint32_t ns = sh.ns, layers = sh.layers;
-  // LDS size of the launch and periods per workgroup of each segment
-  size_t lds = c->lds_override ? c->lds_override : 32 * 1024;
-  std::vector<size_t> one(nw);
-  std::vector<uint32_t> range(nw, 0), halo(nw, 0);
-  for (size_t i = 0; i < nw; i++) {
-    if (w[i].np == 0) continue;
-    if (w[i].nb < 2 || w[i].nb >= 65536)
-      return fail(FBLS_E_ARG, "phase bins must be in [2, 65535], got %u", w[i].nb);
-    if (split) {
-      if (sh.ingress[0] != 0.0)
-        return fail(FBLS_E_ARG, "phase bins beyond the LDS need ingress[0] = 0");
-      halo[i] = longest_template(w[i]);
-      // widest multiple of 128 bins whose window fits the LDS
-      uint32_t B = 128;
-      while (fbls::split_lds_bytes(w[i].nb, B + 128, halo[i], w[i].nd, ns, layers) <=
-                 c->max_lds &&
-             B < w[i].nb)
-        B += 128;
-      if (fbls::split_lds_bytes(w[i].nb, B, halo[i], w[i].nd, ns, layers) > c->max_lds)
-        return fail(FBLS_E_ARG, "templates of %u bins do not fit the LDS", halo[i]);
-      range[i] = B;
-      one[i] = fbls::split_lds_bytes(w[i].nb, B, halo[i], w[i].nd, ns, layers);
-    } else {
-      one[i] = fbls::scan_lds_bytes(w[i].nb, 1, w[i].nd, ns, layers, nullptr);
-      if (one[i] > c->max_lds)
-        return fail(FBLS_E_ARG, "%u phase bins need %zu bytes of LDS per period (limit %zu)",
-                    w[i].nb, one[i], c->max_lds);
-    }
-    lds = std::max(lds, one[i]);
-  }
-  const uint32_t g_cap = (uint32_t)std::max<size_t>(1, total / 1024);
+  std::vector<SegShape> shapes(nw);
+  size_t lds_max = 0;
+  uint32_t groups = 0;
+  if (int st = segment_shapes(c, w, nw, sh, split, shapes.data(), &lds_max, &groups)) return st;
   std::vector<fbls::Segment> segs;
   std::vector<size_t> seg_work;
   segs.reserve(nw);
-  uint32_t groups = 0;
-  size_t lds_max = 0;
   // staging layout (doubles): per segment periods | scales | dur | rho, then
   // ingress | factors, then the segment table and zeroed counters
   size_t in_doubles = ns + ns * layers;
@@ -449,29 +536,21 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   const size_t ing_at = cur;
   for (uint32_t s = 0; s < ns; s++) hd[cur++] = sh.ingress[s];
   const size_t fac_at = cur;
-  for (uint32_t s = 0; s < ns; s++)
-    for (uint32_t l = 0; l < layers; l++)  // scan.rs:157-171
-      hd[cur++] = 1.0 - 2.0 * sh.ingress[s] *
-                            (layers > 1 ? (double)l / (double)(layers - 1) : 0.0);
+  bank_factors(sh, hd + cur);
+  cur += ns * layers;
   size_t out_at = 0;  // in periods
+  uint32_t group0 = 0;
   const size_t rec_base = 0, score_base = total * 9 * 8,
                depth_base = total * 10 * 8, win_base = total * 11 * 8;
   for (size_t i = 0; i < nw; i++) {
     const Work& x = w[i];
     if (x.np == 0) continue;
-    const fbls_family* f = x.fam;
-    fbls::Segment S;
-    std::memset(&S, 0, sizeof(S));
-    const size_t per_at = cur;
     double smax = 0.0;
+    if (int e = check_periods(x, &smax)) return e;
+    const size_t per_at = cur;
     for (size_t k = 0; k < x.np; k++) {
-      const double P = x.periods[k];
-      if (!(P > 0.0) || !std::isfinite(P))
-        return fail(FBLS_E_ARG, "periods must be positive and finite");
-      hd[cur + k] = P;
-      const double sc = (double)x.nb / P;  // scan.rs:204
-      hd[cur + x.np + k] = sc;
-      smax = std::max(smax, sc);
+      hd[cur + k] = x.periods[k];
+      hd[cur + x.np + k] = (double)x.nb / x.periods[k];  // scan.rs:204
     }
     cur += 2 * x.np;
     const size_t dur_at = cur;
@@ -480,10 +559,8 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
       hd[cur + x.nd + k] = x.rho[k];
     }
     cur += 2 * x.nd;
-    if (f->t_max * smax >= 4294967295.0)
-      return fail(FBLS_E_ARG, "t * nb / P exceeds 2^32");
-    S.t = f->t_dev;
-    S.vw = f->vw_dev;
+    fbls::Segment S;
+    segment_fields(c, x, shapes[i], split, smax, &S);
     S.periods = (const double*)(din + per_at * 8);
     S.scales = S.periods + x.np;
     S.dur = (const double*)(din + dur_at * 8);
@@ -492,34 +569,9 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
     S.score = (double*)(dout + score_base) + out_at;
     S.depth = (double*)(dout + depth_base) + out_at;
     S.winner = x.winner ? (uint32_t*)(dout + win_base) + out_at * 4 : nullptr;
-    S.sv = std::ldexp(1.0, -f->ev);
-    S.sw = std::ldexp(1.0, -f->ew);
-    S.C = f->C;
-    S.invC = 1.0 / f->C;
-    S.n = f->n;
-    S.n_pad = f->n_pad;
-    S.np = (uint32_t)x.np;
-    S.nb = x.nb;
-    S.nd = x.nd;
-    uint32_t G = 1;
-    if (split) {
-      S.range_bins = range[i];
-      S.halo_bins = halo[i];
-      lds_max = std::max(lds_max, one[i]);
-    } else {
-      G = (uint32_t)std::max<size_t>(1, lds / one[i]);
-      G = std::min<uint32_t>(std::min<uint32_t>(G, 16), g_cap);
-      while (G > 1 && fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr) > lds) G--;
-      fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, &S.pstride);
-      lds_max = std::max(lds_max, fbls::scan_lds_bytes(x.nb, G, x.nd, ns, layers, nullptr));
-    }
-    S.G = G;
-    S.wide_cells = (f->t_max * smax >= 16777215.0) ? 1u : 0u;
-    S.sorted = f->sorted ? 1u : 0u;
-    S.uniform_w = c->count_from_weights ? f->w_uniform : 0;
-    S.group0 = groups;
+    S.group0 = group0;
     S.index = (uint32_t)segs.size();
-    groups += (uint32_t)((x.np + G - 1) / G);
+    group0 += (uint32_t)((x.np + S.G - 1) / S.G);
     segs.push_back(S);
     seg_work.push_back(i);
     out_at += x.np;
@@ -537,12 +589,7 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   A.factors = (const double*)(din + fac_at * 8);
   A.gate = sh.gate;
   A.counters = (unsigned int*)(din + cnt_off);
-  // 256 threads: measured fastest on MI355X for every LDS budget 16-48 KiB
-  // when the grid fills the chip; a grid of fewer than two workgroups per CU
-  // (a fine pass over a few hundred prescan survivors) of the LDS-resident
-  // kernel gets 512 threads, so each of its periods folds with twice the waves
-  const uint32_t block =
-      c->block_override ? c->block_override : (!split && groups < 512 ? 512 : 256);
+  const uint32_t block = launch_block(c, split, groups);
   HIP_TRY(hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   HIP_TRY(fbls::scan_launch(A, groups, block, lds_max, split, c->stream));
@@ -622,26 +669,6 @@ int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
 // Work estimate of a segment: fold pairs plus box templates.
 double work_cost(const Work& x) {
   return (double)x.np * ((double)x.fam->n + 3.0 * x.nb * x.nd);
-}
-
-void account(fbls_scan_stats* st, const Work& x, const fbls_bank* b,
-             uint64_t budget, double seconds) {
-  st->phase_bins = x.nb;
-  uint64_t batch = std::min<uint64_t>(std::min<uint64_t>(b->period_batch, budget), x.np);
-  if (batch) st->batches += (x.np + batch - 1) / batch;
-  st->templates_scanned += (uint64_t)x.np * x.nd * b->ns * x.nb;  // scan.rs:352
-  st->exact_tile_retries += x.retries;
-  st->seconds_device += seconds;
-  st->seconds_scan += seconds;
-}
-
-Shared shared_of(const fbls_bank* b) {
-  Shared sh;
-  sh.ns = (uint32_t)b->ns;
-  sh.layers = b->layers;
-  sh.ingress = b->ingress;
-  sh.gate = b->shape_gate;
-  return sh;
 }
 
 }  // namespace
