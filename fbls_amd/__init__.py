@@ -59,7 +59,7 @@ EXPORTS = (
     "fbls_search_transits", "fbls_single_event_search", "fbls_read_inputs",
     "fbls_format_f64", "fbls_json_reformat", "fbls_cli", "fbls_coarse_inputs",
     "fbls_cnn_set_fused", "fbls_signal_search_bins", "fbls_propose",
-    "fbls_signal_search_bins_ragged",
+    "fbls_signal_search_bins_ragged", "fbls_scan_bank_noalloc",
 )
 
 
@@ -284,6 +284,8 @@ def lib():
                                    _P(EvidenceCfg), _P(EvidenceRow), _P(_sz),
                                    _P(EvidenceDebug)]
         L.fbls_evidence_parameters.argtypes = [_P(EvidenceCfg), _P(_d)]
+        L.fbls_scan_bank_noalloc.argtypes = [_vp, _vp, _P(_d), _sz, _P(Bank), _u32, _P(_d),
+                                             _P(_d), _P(_d), _P(_u32), _P(ScanStats)]
         _fp = _P(ctypes.c_float)
         L.fbls_cnn_create.argtypes = [_vp, _P(CnnCfg), _u64, _P(_vp)]
         L.fbls_cnn_destroy.argtypes = [_vp]
@@ -747,6 +749,24 @@ def scan_bank(gpu, dev, periods, bank, nb, winner=True):
     win = np.empty((n, 4), dtype=np.uint32) if winner else None
     st = ScanStats()
     _check(lib().fbls_scan_bank(
+        gpu._h, dev._h, _dp(periods), n, ctypes.byref(bank), nb, _dp(rec),
+        _dp(score), _dp(depth),
+        win.ctypes.data_as(_P(_u32)) if win is not None else None,
+        ctypes.byref(st)))
+    return rec, score, depth, win, st.as_dict()
+
+
+def scan_bank_noalloc(gpu, dev, periods, bank, nb, winner=True):
+    """scan_bank through the entry point that allocates nothing (its staging
+    is a fixed per-device array of the library: at most 65536 periods per
+    call).  Same returns as scan_bank."""
+    periods = _f64(periods)
+    n = len(periods)
+    rec = np.empty((n, 9))
+    score, depth = np.empty(n), np.empty(n)
+    win = np.empty((n, 4), dtype=np.uint32) if winner else None
+    st = ScanStats()
+    _check(lib().fbls_scan_bank_noalloc(
         gpu._h, dev._h, _dp(periods), n, ctypes.byref(bank), nb, _dp(rec),
         _dp(score), _dp(depth),
         win.ctypes.data_as(_P(_u32)) if win is not None else None,

@@ -73,6 +73,7 @@ extern "C" int fbls_ctx_create(int device, fbls_ctx** out) {
       return fail(FBLS_E_HIP, "kernel LDS configuration failed: %s",
                   hipGetErrorString(e));
     }
+    fbls::host::noalloc_setup(device);
     *out = c;
     return FBLS_OK;
   });

@@ -69,5 +69,10 @@ void bank_factors(const Shared& sh, double* factors);
 void account(fbls_scan_stats* st, const Work& x, const fbls_bank* b, uint64_t budget,
              double seconds);
 
+// Resolves the device address of the allocation-free entry point's
+// scratch array on the current device (once per device); called by
+// fbls_ctx_create with `device` current.
+void noalloc_setup(int device);
+
 }  // namespace host
 }  // namespace fbls

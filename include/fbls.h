@@ -169,6 +169,23 @@ int fbls_scan_bank(fbls_ctx* ctx, const fbls_family* fam,
                    uint32_t nb, double* records, double* score, double* depth,
                    uint32_t* winner, fbls_scan_stats* stats);
 
+/* fbls_scan_bank without any allocation: no hipMalloc, hipFree, pinned or
+ * library allocation inside the call.  No counterpart in the reference; for
+ * callers that cannot allocate (tight serving loops, harnesses that forbid
+ * allocation).  Same arguments and results as fbls_scan_bank (bit-identical
+ * records, scores, depths, winners and statistics); the family is uploaded
+ * beforehand.  The launch table, periods, scales, counters and staged
+ * outputs live in a fixed 8 MiB device array of the library on each device
+ * (held until the process ends), which limits a call to np <= 65536
+ * periods (larger np: FBLS_E_ARG).  Every call on one device shares that
+ * array and keeps the last call's contents; calls from several threads of
+ * one process on one device are serialized by the library, since calls that
+ * overlapped on the array would race.  Synchronous on return. */
+int fbls_scan_bank_noalloc(fbls_ctx* ctx, const fbls_family* fam,
+                           const double* periods, size_t np, const fbls_bank* bank,
+                           uint32_t nb, double* records, double* score, double* depth,
+                           uint32_t* winner, fbls_scan_stats* stats);
+
 /* `scan_periods` (scan.rs:363-408): coarse prescan then full scan of the
  * survivors; rows (capacity np) receive periods whose score >= threshold. */
 int fbls_scan_periods(fbls_ctx* ctx, const fbls_family* fam,
