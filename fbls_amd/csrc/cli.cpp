@@ -444,8 +444,18 @@ void batch_cmd(const std::vector<std::string>& args) {
   const auto start = std::chrono::steady_clock::now();
   std::vector<std::thread> threads;
   std::vector<std::string> thread_errors;
-  for (int device : devices)
-    for (size_t k = 0; k < workers; k++)
+  std::string spawn_error;
+  // batch.rs:76-101: a star's error goes to <name>.error and the batch goes on
+  auto job_error = [&](const std::string& name, const std::string& msg) {
+    std::lock_guard<std::mutex> g(lock);
+    std::fprintf(stderr, "%s: %s\n", name.c_str(), msg.c_str());
+    std::ofstream(output_dir + "/" + name + ".error") << msg;
+  };
+  // a worker that cannot be started leaves the list to the ones that were
+  // (every thread is joined before an error is raised)
+  try {
+    for (int device : devices)
+      for (size_t k = 0; k < workers; k++)
       threads.emplace_back([&, device]() {
         try {
           Ctx ctx(device);
@@ -477,17 +487,30 @@ void batch_cmd(const std::vector<std::string>& args) {
               std::fprintf(stderr, "[%zu/%zu] %s hip:%d %.1fs\n", done, total, job.name.c_str(),
                            device, secs);
             } catch (const Failure& e) {
-              std::fprintf(stderr, "%s: %s\n", job.name.c_str(), e.message.c_str());
-              std::ofstream(output_dir + "/" + job.name + ".error") << e.message;
+              job_error(job.name, e.message);
+            } catch (const std::exception& e) {
+              // e.g. std::bad_alloc from an archive that declares a huge array
+              job_error(job.name, e.what());
             }
           }
         } catch (const Failure& e) {
           std::lock_guard<std::mutex> g(lock);
           thread_errors.push_back(e.message);
+        } catch (const std::exception& e) {
+          std::lock_guard<std::mutex> g(lock);
+          thread_errors.push_back(e.what());
         }
       });
+  } catch (const std::exception& e) {
+    spawn_error = e.what();
+  }
   for (std::thread& t : threads) t.join();
   if (!thread_errors.empty()) raise(thread_errors[0]);
+  if (!spawn_error.empty()) {
+    if (threads.empty()) raise("cannot start a batch worker: " + spawn_error);
+    std::fprintf(stderr, "fbls: started %zu of %zu workers: %s\n", threads.size(),
+                 workers * devices.size(), spawn_error.c_str());
+  }
   const double secs =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
   std::fprintf(stderr, "%zu stars in %.0f s\n", total, secs);
@@ -519,13 +542,18 @@ extern "C" int fbls_cli(int argc, const char* const* argv) {
 }
 
 extern "C" size_t fbls_format_f64(double x, char* buf, size_t cap) {
-  const std::string s = fbls::json::format_double(x);
-  if (buf && cap) {
-    const size_t m = std::min(cap - 1, s.size());
-    std::memcpy(buf, s.data(), m);
-    buf[m] = 0;
+  try {
+    const std::string s = fbls::json::format_double(x);
+    if (buf && cap) {
+      const size_t m = std::min(cap - 1, s.size());
+      std::memcpy(buf, s.data(), m);
+      buf[m] = 0;
+    }
+    return s.size();
+  } catch (const std::exception&) {  // no exception crosses the C boundary
+    if (buf && cap) buf[0] = 0;
+    return 0;
   }
-  return s.size();
 }
 
 extern "C" int fbls_json_reformat(const char* text, char* out, size_t cap, size_t* len) {

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -240,15 +241,12 @@ extern "C" int fbls_family_upload(fbls_ctx* c, const double* t,
   return fbls::host::guard([&]() -> int {
     if (!c || !out) return fail(FBLS_E_ARG, "null argument");
     *out = nullptr;
-    fbls_family* f = new fbls_family();
-    int st = prepare_family(c, t, v, w, n, baseline_norm, origin, f);
-    std::vector<fbls_family*> one{f};
-    if (!st) st = upload_arena(c, one, {t});
-    if (st) {
-      delete f;
-      return st;
-    }
-    *out = f;
+    std::unique_ptr<fbls_family> f(new fbls_family());
+    int st = prepare_family(c, t, v, w, n, baseline_norm, origin, f.get());
+    if (st) return st;
+    std::vector<fbls_family*> one{f.get()};
+    if ((st = upload_arena(c, one, {t}))) return st;
+    *out = f.release();
     return FBLS_OK;
   });
 }
@@ -262,33 +260,30 @@ extern "C" int fbls_family_upload_packed(fbls_ctx* c, size_t nfam,
   return fbls::host::guard([&]() -> int {
     if (!c || !offsets || !t || !v || !w || !baseline_norm || !origin || !out)
       return fail(FBLS_E_ARG, "null argument");
+    for (size_t k = 0; k < nfam; k++) out[k] = nullptr;
+    // the families are owned here until every one is uploaded
+    std::vector<std::unique_ptr<fbls_family>> owned;
     std::vector<fbls_family*> fams;
     std::vector<const double*> times;
-    int st = FBLS_OK;
-    for (size_t k = 0; k < nfam && !st; k++) {
-      out[k] = nullptr;
-      if (offsets[k + 1] < offsets[k]) {
-        st = fail(FBLS_E_ARG, "offsets must be nondecreasing");
-        break;
-      }
+    owned.reserve(nfam);
+    fams.reserve(nfam);
+    times.reserve(nfam);
+    for (size_t k = 0; k < nfam; k++) {
+      if (offsets[k + 1] < offsets[k]) return fail(FBLS_E_ARG, "offsets must be nondecreasing");
       const size_t a = offsets[k], n = offsets[k + 1] - offsets[k];
-      fbls_family* f = new fbls_family();
-      fams.push_back(f);
+      owned.emplace_back(new fbls_family());
+      fams.push_back(owned.back().get());
       times.push_back(t + a);
-      st = prepare_family(c, t + a, v + a, w + a, n, baseline_norm[k], origin[k], f);
-      if (st) {
+      if (int st = prepare_family(c, t + a, v + a, w + a, n, baseline_norm[k], origin[k],
+                                  fams.back())) {
         char msg[600];
         std::snprintf(msg, sizeof(msg), "family %zu: %s", k, fbls_last_error());
-        st = fail(st, "%s", msg);
+        return fail(st, "%s", msg);
       }
     }
-    if (!st && !fams.empty()) st = upload_arena(c, fams, times);
-    if (st) {
-      for (fbls_family* f : fams) delete f;
-      for (size_t k = 0; k < nfam; k++) out[k] = nullptr;
-      return st;
-    }
-    for (size_t k = 0; k < nfam; k++) out[k] = fams[k];
+    if (!fams.empty())
+      if (int st = upload_arena(c, fams, times)) return st;
+    for (size_t k = 0; k < nfam; k++) out[k] = owned[k].release();
     return FBLS_OK;
   });
 }
@@ -1009,6 +1004,29 @@ void apply_bins(SearchPlan& p, const uint8_t* bin_keep, size_t nbins, const doub
   }
 }
 
+// SearchConfig's period range and grid density (config.rs:122-133): an
+// oversample that is not finite and positive, or a range outside
+// 0 < period_min < period_max, would make the grid count meaningless.
+int check_search_cfg(const fbls_search_cfg* cfg) {
+  if (!(std::isfinite(cfg->period_oversample) && cfg->period_oversample > 0.0))
+    return fail(FBLS_E_ARG, "period_oversample must be finite and positive");
+  if (!(std::isfinite(cfg->period_min) && std::isfinite(cfg->period_max) &&
+        0.0 < cfg->period_min && cfg->period_min < cfg->period_max))
+    return fail(FBLS_E_ARG, "require 0 < period_min < period_max");
+  return FBLS_OK;
+}
+
+// Grid count with its size checked: at most 2^31 periods per family.
+int grid_count_checked(double minimum, double maximum, double baseline, double dmin,
+                       double oversample, size_t* count) {
+  const double log_step = std::log1p(dmin / (oversample * baseline));  // grid.rs:12-13
+  const double periods = std::ceil(std::log(maximum / minimum) / log_step) + 1.0;
+  if (!(periods >= 1.0 && periods <= 2147483648.0))
+    return fail(FBLS_E_ARG, "period grid of %g periods (at most 2^31 per family)", periods);
+  *count = fbls::period_grid_count(minimum, maximum, baseline, dmin, oversample);
+  return FBLS_OK;
+}
+
 int plan_search(const fbls_family* f, const double* durations, const double* rho,
                 size_t nd, const fbls_bank* bank, const fbls_search_cfg* cfg,
                 SearchPlan* p, bool lazy = false) {
@@ -1036,8 +1054,10 @@ int plan_search(const fbls_family* f, const double* durations, const double* rho
   const double maximum = std::min(cfg->period_max, baseline);
   const double minimum = std::max(cfg->period_min, 4.0 * p->sd[0] * (1.0 + 1e-9));
   if (maximum <= minimum) return FBLS_OK;
-  const size_t count = fbls::period_grid_count(minimum, maximum, baseline, p->sd[0],
-                                               cfg->period_oversample);
+  size_t count = 0;
+  if (int st = grid_count_checked(minimum, maximum, baseline, p->sd[0], cfg->period_oversample,
+                                  &count))
+    return st;
   p->count = count;
   p->formula = fbls::GridFormula(minimum, maximum, count);
   if (lazy) {
@@ -1076,26 +1096,45 @@ int plan_stars(const fbls_family* const* fams, size_t s0, size_t s1, const doubl
   plans->assign(m, SearchPlan());
   std::vector<int> status(m, FBLS_OK);
   std::vector<std::string> message(m);
-  auto work = [&](size_t a, size_t e) {
+  // a worker records its stars' errors, C++ exceptions included (a worker
+  // thread has no guard of its own: an escaping exception would terminate)
+  auto plan_one = [&](size_t k) {
+    const size_t s = s0 + k;
+    if (!fams[s]) {
+      status[k] = FBLS_E_ARG;
+      message[k] = "null family " + std::to_string(s);
+      return false;
+    }
+    status[k] = plan_search(fams[s], durations, rho + s * nd, nd, bank, cfg, &(*plans)[k],
+                            rs.any());
+    if (status[k]) {
+      message[k] = fbls_last_error();
+      return false;
+    }
+    if (rs.keep) apply_keep((*plans)[k], rs.keep + keep_at[k]);
+    if (rs.bin_keep && rs.bin_offsets) {
+      const size_t b0 = rs.bin_offsets[s];
+      apply_bins((*plans)[k], rs.bin_keep + b0, rs.bin_offsets[s + 1] - b0, rs.edges + b0 + s);
+    } else if (rs.bin_keep) {
+      apply_bins((*plans)[k], rs.bin_keep + s * rs.nbins, rs.nbins, rs.edges);
+    }
+    return true;
+  };
+  auto work = [&](size_t a, size_t e) noexcept {
     for (size_t k = a; k < e; k++) {
-      const size_t s = s0 + k;
-      if (!fams[s]) {
+      try {
+        if (!plan_one(k)) return;
+      } catch (const std::bad_alloc&) {
+        status[k] = FBLS_E_OOM;
+        message[k] = "host memory allocation failed";
+        return;
+      } catch (const std::exception& ex) {
         status[k] = FBLS_E_ARG;
-        message[k] = "null family " + std::to_string(s);
+        try {
+          message[k] = ex.what();
+        } catch (...) {
+        }
         return;
-      }
-      status[k] = plan_search(fams[s], durations, rho + s * nd, nd, bank, cfg, &(*plans)[k],
-                              rs.any());
-      if (status[k]) {
-        message[k] = fbls_last_error();
-        return;
-      }
-      if (rs.keep) apply_keep((*plans)[k], rs.keep + keep_at[k]);
-      if (rs.bin_keep && rs.bin_offsets) {
-        const size_t b0 = rs.bin_offsets[s];
-        apply_bins((*plans)[k], rs.bin_keep + b0, rs.bin_offsets[s + 1] - b0, rs.edges + b0 + s);
-      } else if (rs.bin_keep) {
-        apply_bins((*plans)[k], rs.bin_keep + s * rs.nbins, rs.nbins, rs.edges);
       }
     }
   };
@@ -1104,9 +1143,18 @@ int plan_stars(const fbls_family* const* fams, size_t s0, size_t s1, const doubl
   if (nt <= 1) {
     work(0, m);
   } else {
+    // the threads that did start are joined before any error returns; the
+    // ranges of threads that could not start are planned on this thread
     std::vector<std::thread> pool;
-    for (size_t t = 0; t < nt; t++) pool.emplace_back(work, m * t / nt, m * (t + 1) / nt);
+    pool.reserve(nt);
+    size_t started = 0;
+    try {
+      for (; started < nt; started++)
+        pool.emplace_back(work, m * started / nt, m * (started + 1) / nt);
+    } catch (const std::system_error&) {
+    }
     for (std::thread& th : pool) th.join();
+    if (started < nt) work(m * started / nt, m);
   }
   for (size_t k = 0; k < m; k++)
     if (status[k]) return fail(status[k], "%s", message[k].c_str());
@@ -1160,6 +1208,7 @@ extern "C" int fbls_signal_search(fbls_ctx* c, const fbls_family* f,
     *nrows = 0;
     *nbands = 0;
     *grid_count = 0;
+    if (int st = check_search_cfg(cfg)) return st;
     if (nd == 0) return FBLS_OK;
     if (!durations || !rho) return fail(FBLS_E_ARG, "null argument");
     SearchPlan p;
@@ -1202,11 +1251,14 @@ int batch_search(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
                  const double* durations, const double* rho, size_t nd, const fbls_bank* bank,
                  const fbls_search_cfg* cfg, fbls_row* rows, size_t rows_cap,
                  size_t* row_offsets, fbls_scan_stats* star_stats, const Restriction& rs) {
-  if (!c || (nfam && !fams) || !bank || !cfg || !row_offsets || (nd && (!durations || !rho)))
-    return fail(FBLS_E_ARG, "null argument");
+  if (!bank || !cfg || !row_offsets) return fail(FBLS_E_ARG, "null argument");
   constexpr size_t kBatchPeriods = size_t(1) << 21;
   for (size_t k = 0; k <= nfam; k++) row_offsets[k] = 0;
   if (star_stats) std::memset(star_stats, 0, nfam * sizeof(fbls_scan_stats));
+  // the configuration is checked before anything else touches the device
+  if (int st = check_search_cfg(cfg)) return st;
+  if (!c || (nfam && !fams) || (nd && (!durations || !rho)))
+    return fail(FBLS_E_ARG, "null argument");
   if (nd == 0 || nfam == 0) return FBLS_OK;
   fbls_bank b = budgeted(bank);
   size_t written = 0, needed = 0;
@@ -1222,8 +1274,9 @@ int batch_search(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
       const double maximum = std::min(cfg->period_max, baseline);
       const double minimum = std::max(cfg->period_min, 4.0 * dmin * (1.0 + 1e-9));
       if (maximum > minimum)
-        grid_size[s] = fbls::period_grid_count(minimum, maximum, baseline, dmin,
-                                               cfg->period_oversample);
+        if (int st = grid_count_checked(minimum, maximum, baseline, dmin,
+                                        cfg->period_oversample, &grid_size[s]))
+          return st;
     }
   }
   // a restricted search plans every star first (its grids are never
@@ -1422,6 +1475,7 @@ extern "C" int fbls_autotune(fbls_ctx* c, const fbls_family* f, const double* du
   return fbls::host::guard([&]() -> int {
     if (!c || !f || !bank || !cfg || !ntable || !best) return fail(FBLS_E_ARG, "null argument");
     *ntable = 0;
+    if (int st = check_search_cfg(cfg)) return st;
     const uint32_t lds_opts[] = {16384, 24576, 32768, 49152, 65536, 98304};
     const uint32_t block_opts[] = {256, 512};
     const uint32_t saved_lds = c->lds_override, saved_block = c->block_override;

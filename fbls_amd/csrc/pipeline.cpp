@@ -467,6 +467,12 @@ int check_pipeline(const fbls_pipeline_cfg* cfg) {
   if (!cfg || !cfg->durations || cfg->nd == 0) return fail(FBLS_E_ARG, "no durations");
   if (cfg->passes < 1) return fail(FBLS_E_ARG, "passes must be >= 1");
   if (!cfg->bank.ingress || cfg->bank.ns == 0) return fail(FBLS_E_ARG, "no shapes");
+  const fbls_search_cfg& s = cfg->search;
+  if (!(std::isfinite(s.period_oversample) && s.period_oversample > 0.0))
+    return fail(FBLS_E_ARG, "period_oversample must be finite and positive");
+  if (!(std::isfinite(s.period_min) && std::isfinite(s.period_max) && 0.0 < s.period_min &&
+        s.period_min < s.period_max))
+    return fail(FBLS_E_ARG, "require 0 < period_min < period_max");
   return FBLS_OK;
 }
 

@@ -28,7 +28,12 @@ size_t period_grid_count(double minimum, double maximum, double baseline,
                          double minimum_duration, double oversample) {
   // grid.rs:12-13
   const double log_step = std::log1p(minimum_duration / (oversample * baseline));
-  return (size_t)std::ceil(std::log(maximum / minimum) / log_step) + 1;
+  const double steps = std::ceil(std::log(maximum / minimum) / log_step);
+  // not a number (invalid arguments): no grid; beyond 2^62: saturated (the
+  // scan entry points refuse grids above 2^31 periods)
+  if (!(steps >= 0.0)) return 0;
+  if (steps >= 4611686018427387904.0) return (size_t)1 << 62;
+  return (size_t)steps + 1;
 }
 
 void period_grid(double minimum, double maximum, double* out, size_t count) {

@@ -96,16 +96,30 @@ def test_cli_batch(gpu, tmp_path):
         p = str(tmp_path / f"s{k}.npz")
         np.savez(p, time_bjd=t, flux=raw, flux_err=err)
         files.append(p)
+    # an archive whose time array declares 2^61 + 1 doubles (its byte count
+    # wraps in 64 bits): that star fails alone, the batch goes on
+    import io
+    import zipfile
+    h = b"{'descr': '<f8', 'fortran_order': False, 'shape': (2305843009213693953,), }"
+    h += b" " * (63 - (10 + len(h)) % 64) + b"\n"
+    huge = str(tmp_path / "huge.npz")
+    with zipfile.ZipFile(huge, "w") as z:
+        buf = io.BytesIO()
+        np.lib.format.write_array(buf, np.arange(10.0))
+        z.writestr("flux.npy", buf.getvalue())
+        z.writestr("time_bjd.npy", b"\x93NUMPY\x01\x00" + len(h).to_bytes(2, "little") + h
+                   + b"\0" * 8)
     lst = tmp_path / "stars.txt"
     lst.write_text(f"# name pmax files\nstar0 7 {files[0]}\nstar1 - {files[1]}\n"
-                   f"broken 5 {tmp_path / 'missing.npz'}\nshort 5\n")
+                   f"broken 5 {tmp_path / 'missing.npz'}\nhuge 5 {huge}\nshort 5\n")
     outdir = str(tmp_path / "out")
     r = _run(["batch", "--list", str(lst), "--output-dir", outdir, "--workers", "2"], 600)
-    assert "3 stars in" in r.stderr
+    assert "4 stars in" in r.stderr
     errors = {f: open(os.path.join(outdir, f)).read() for f in os.listdir(outdir)
               if f.endswith(".error")}
-    assert sorted(os.listdir(outdir)) == ["broken.error", "star0.json", "star1.json"], (
-        r.stderr, errors)
+    assert sorted(os.listdir(outdir)) == ["broken.error", "huge.error", "star0.json",
+                                          "star1.json"], (r.stderr, errors)
+    assert "too large" in errors["huge.error"]
     for name, pmax, path in (("star0", "7", files[0]), ("star1", "20", files[1])):
         got = json.load(open(os.path.join(outdir, name + ".json")))
         assert got["config"]["period_max"] == float(pmax)

@@ -187,6 +187,39 @@ def test_ragged_bins_argument_checks():
     assert "null argument" in message()
 
 
+def test_search_config_checked_before_device_work():
+    """A period range or grid density that would make the grid count
+    meaningless (NaN, negative, inverted range) is refused with FBLS_E_ARG
+    before any device work, for a batch of 64 stars as for one; the grid
+    count itself returns 0 for such arguments instead of casting NaN."""
+    import ctypes
+
+    L = F.lib()
+    P = ctypes.POINTER
+    d = np.array([0.02, 0.04])
+    bank = F.make_bank(F.SearchConfig(), d, np.ones(2))
+    roff = np.zeros(65, dtype=np.uintp)
+
+    def batch(pmin, pmax, over):
+        cfg = F.SearchCfgC(pmin, pmax, over)
+        return L.fbls_signal_search_batch(
+            None, None, 64, d.ctypes.data_as(P(ctypes.c_double)),
+            np.ones(128).ctypes.data_as(P(ctypes.c_double)), 2, ctypes.byref(bank),
+            ctypes.byref(cfg), None, 0, roff.ctypes.data_as(P(ctypes.c_size_t)), None, None)
+
+    for over in (float("nan"), -3.0, 0.0, float("inf")):
+        assert batch(0.5, 13.7, over) == 4
+        assert "period_oversample" in L.fbls_last_error().decode()
+    for pmin, pmax in ((13.7, 0.5), (-1.0, 5.0), (float("nan"), 5.0), (0.5, float("inf"))):
+        assert batch(pmin, pmax, 3.0) == 4
+        assert "period_min < period_max" in L.fbls_last_error().decode()
+    assert batch(0.5, 13.7, 3.0) == 4  # a valid configuration: the null context
+    assert "null argument" in L.fbls_last_error().decode()
+    assert L.fbls_period_grid_count(0.5, 13.7, 27.4, 0.02, float("nan")) == 0
+    assert L.fbls_period_grid_count(13.7, 0.5, 27.4, 0.02, 3.0) == 0
+    assert L.fbls_period_grid_count(0.5, 13.7, 27.4, 0.02, 1e300) == 1 << 62
+
+
 def test_missing_library_raises():
     """The product path has no CPU fallback: without the built library every
     entry into it raises, naming the build command."""
