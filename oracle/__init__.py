@@ -186,6 +186,11 @@ def lib():
         L.orc_select_events.argtypes = [_P(Row), _sz, _P(_d), _sz, _d, _sz,
                                         _P(_P(ctypes.c_uint8)), _sz, _P(SingleEvent)]
         L.orc_select_events.restype = _sz
+        L.orc_select_peaks.restype = _sz
+        L.orc_select_peaks.argtypes = [
+            _sz, _P(_d), _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz),
+            _P(Row), _P(_sz), _sz, _P(_d), _sz, _d, _sz, _sz, _P(ctypes.c_uint8), _sz,
+            _P(ctypes.c_uint8), _P(OrcPeak), _sz]
         L.orc_coarse_features.argtypes = [_P(_d), _P(_d), _P(_d), _sz, _d, _P(_d), _sz,
                                           _P(_d), _sz, _u32, _u32, _P(_d)]
         L.orc_coarse_features.restype = None
@@ -717,6 +722,41 @@ def single_event_search(t, v, w, C, origin, durations, rho, cfg, capacity=131072
                                   ctypes.byref(st))
     keys = ("period", "epoch", "duration", "score", "depth")
     return [{k: getattr(rows[i], k) for k in keys} for i in range(nr.value)], st.as_dict()
+
+
+def select_peaks(fams, rows, family, time_all, threshold, limit, min_events,
+                 masks=(), excluded=None):
+    """peaks.rs:308-487 on a given table.  fams: dicts with t (relative), v,
+    w (fixed-point values), C, origin, dur, rho; rows: dicts with period,
+    epoch, duration, score, depth; family: the family of each row.
+    Returns (peaks, excluded after the selection)."""
+    ft = np.concatenate([_f64(f["t"]) for f in fams])
+    fv = np.concatenate([_f64(f["v"]) for f in fams])
+    fw = np.concatenate([_f64(f["w"]) for f in fams])
+    foff = np.cumsum([0] + [len(f["t"]) for f in fams]).astype(np.uintp)
+    fdur = np.concatenate([_f64(f["dur"]) for f in fams])
+    frho = np.concatenate([_f64(f["rho"]) for f in fams])
+    fdoff = np.cumsum([0] + [len(f["dur"]) for f in fams]).astype(np.uintp)
+    fC = _f64([f["C"] for f in fams])
+    forg = _f64([f["origin"] for f in fams])
+    tab = (Row * max(len(rows), 1))()
+    for i, r in enumerate(rows):
+        tab[i] = Row(r["period"], r["epoch"], r["duration"], r["score"], r["depth"])
+    fam_idx = np.ascontiguousarray(family, dtype=np.uintp)
+    time_all = _f64(time_all)
+    n_all = len(time_all)
+    ms = np.ascontiguousarray(np.array(masks, dtype=np.uint8).reshape(-1, n_all))
+    ex = (np.zeros(n_all, np.uint8) if excluded is None
+          else np.ascontiguousarray(excluded, dtype=np.uint8).copy())
+    out = (OrcPeak * max(limit, 1))()
+    sz = _P(ctypes.c_size_t)
+    u8 = _P(ctypes.c_uint8)
+    n = lib().orc_select_peaks(
+        len(fams), _dp(ft), _dp(fv), _dp(fw), foff.ctypes.data_as(sz), _dp(fC), _dp(forg),
+        _dp(fdur), _dp(frho), fdoff.ctypes.data_as(sz), tab, fam_idx.ctypes.data_as(sz),
+        len(rows), _dp(time_all), n_all, threshold, limit, min_events, ms.ctypes.data_as(u8),
+        len(ms), ex.ctypes.data_as(u8), out, limit)
+    return [out[i].as_dict() for i in range(n)], ex.astype(bool)
 
 
 def select_events(rows, time, threshold, limit, masks=()):

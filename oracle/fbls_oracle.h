@@ -274,6 +274,15 @@ void orc_single_event_search(const double* t, const double* v, const double* w, 
                              size_t layers, double threshold, uint32_t phase_oversample,
                              size_t capacity, orc_row* rows, size_t* nrows,
                              orc_event_stats* stats);
+/* peaks.rs:308-487 on a caller's table (the selection step alone): see
+ * pipeline.c for the family layout */
+size_t orc_select_peaks(size_t nfam, const double* ft, const double* fv, const double* fw,
+                        const size_t* foff, const double* fC, const double* forigin,
+                        const double* fdur, const double* frho, const size_t* fdoff,
+                        const orc_row* table, const size_t* family, size_t nrows,
+                        const double* time_all, size_t n_all, double threshold, size_t limit,
+                        size_t min_events, const uint8_t* masks_in, size_t n_masks_in,
+                        uint8_t* excluded, orc_peak* out, size_t cap);
 /* peaks.rs:490-550: masks n_masks x n_time; out capacity limit */
 size_t orc_select_events(const orc_row* table, size_t nrows, const double* time,
                          size_t n_time, double threshold, size_t limit,

@@ -551,6 +551,54 @@ static void select_peaks(sel_ctx* S, size_t limit, size_t min_events,
   free(heap.a);
 }
 
+/* select_peaks (peaks.rs:308-487) on a caller's table: the selection step
+ * alone, for tests.  Family k: relative times, fixed-point fields and rho
+ * at ft/fv/fw[foff[k] .. foff[k+1]), baseline norm fC[k], origin
+ * forigin[k], durations/rho at fdur/frho[fdoff[k] .. fdoff[k+1]).
+ * masks_in: n_masks_in x n_all accepted masks of earlier passes; excluded:
+ * n_all flags, updated in place.  Returns the number of peaks written. */
+size_t orc_select_peaks(size_t nfam, const double* ft, const double* fv, const double* fw,
+                        const size_t* foff, const double* fC, const double* forigin,
+                        const double* fdur, const double* frho, const size_t* fdoff,
+                        const orc_row* table, const size_t* family, size_t nrows,
+                        const double* time_all, size_t n_all, double threshold, size_t limit,
+                        size_t min_events, const uint8_t* masks_in, size_t n_masks_in,
+                        uint8_t* excluded, orc_peak* out, size_t cap) {
+  fam_t* fams = (fam_t*)calloc(nfam + 1, sizeof(fam_t));
+  size_t** pos = (size_t**)calloc(nfam + 1, sizeof(size_t*));
+  for (size_t k = 0; k < nfam; k++) {
+    fam_t* f = &fams[k];
+    f->n = foff[k + 1] - foff[k];
+    f->t = (double*)(ft + foff[k]);
+    f->v = (double*)(fv + foff[k]);
+    f->w = (double*)(fw + foff[k]);
+    f->C = fC[k];
+    f->origin = forigin[k];
+    f->nd = fdoff[k + 1] - fdoff[k];
+    f->dur = (double*)(fdur + fdoff[k]);
+    f->rho = (double*)(frho + fdoff[k]);
+    pos[k] = (size_t*)malloc((f->n + 1) * sizeof(size_t));
+    for (size_t i = 0; i < f->n; i++) {
+      size_t p = lower_bound(time_all, n_all, f->t[i] + f->origin);
+      pos[k][i] = p < n_all - 1 ? p : n_all - 1;
+    }
+  }
+  uint8_t** masks = (uint8_t**)calloc(n_masks_in + limit + 1, sizeof(uint8_t*));
+  for (size_t k = 0; k < n_masks_in; k++) {
+    masks[k] = (uint8_t*)malloc(n_all + 1);
+    memcpy(masks[k], masks_in + k * n_all, n_all);
+  }
+  size_t n_masks = n_masks_in, nout = 0;
+  sel_ctx S = {table, nrows, family, fams, nfam, pos, time_all, n_all, threshold, excluded};
+  select_peaks(&S, limit, min_events, masks, &n_masks, 1, out, &nout, cap);
+  for (size_t k = 0; k < n_masks; k++) free(masks[k]);
+  free(masks);
+  for (size_t k = 0; k < nfam; k++) free(pos[k]);
+  free(pos);
+  free(fams);
+  return nout;
+}
+
 /* ------------------------------------------------------------ pipeline */
 
 static int cmp_dbl(const void* a, const void* b) {
