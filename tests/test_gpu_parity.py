@@ -108,6 +108,39 @@ def test_signal_search_bench_star(gpu):
     assert abs(best[0] - 11.3) < 0.01 and best[3] > 7.1
 
 
+def test_signal_search_single_sector_star(gpu):
+    """The single-sector workload (one 27.4-d sector at 2-min cadence, box
+    P = 3.7 d, periods 0.5-13.7 d, the 13 default durations, prescan 2): the
+    full signal search -- 13,065 grid periods in five octave bands (nb 192 to
+    2631) -- against the oracle's signal_search: every row and every band
+    statistic bit-exact."""
+    from fbls_amd import synth
+
+    t, f, e = synth.single_sector(seed=1)
+    cfg = F.SearchConfig(period_min=0.5, period_max=13.7)
+    d = cfg.durations_days()
+    s = F.family_state(t, f - 1.0, e, d, cfg.correlated_noise)
+    dev = F.DeviceReady.upload(gpu, s)
+    vq, wq, _, _ = dev.fields()
+    print(f"[parity] signal_search 1-sector n={len(s.t)}", flush=True)
+    rows, grid, bands = F.signal_search(gpu, dev, d, s.rho, cfg)
+    orows, obands, og, ost = O.signal_search(
+        s.t, vq, wq, s.baseline_norm, s.origin, d, s.rho,
+        cfg.ingress_fractions, cfg, cfg.period_min, cfg.period_max,
+        cfg.period_oversample)
+    assert ost == 0
+    assert len(grid) == og == 13065
+    assert [b["phase_bins"] for b in bands] == [192, 384, 768, 1536, 2631]
+    assert len(bands) == len(obands)
+    for b, ob in zip(bands, obands):
+        for k in ("periods", "prescan_survivors", "phase_bins", "batches",
+                  "templates_scanned"):
+            assert b[k] == ob[k], (k, b, ob)
+    np.testing.assert_array_equal(rows, orows)
+    best = rows[np.argmax(rows[:, 3])]
+    assert abs(best[0] - 3.7) < 0.01 and best[3] > 7.1
+
+
 @pytest.mark.parametrize("launch", [(0, 256, 2), (32768, 512, 1), (160 * 1024, 256, 1)])
 def test_launch_shape_does_not_change_results(gpu, launch):
     """LDS budget, workgroup size and fold mode change the schedule only."""
