@@ -3,7 +3,7 @@
 // bank selection, two-period bin count, duration log masses, shape priors,
 // staircase coefficients, orbit bank, pruning margin, and the final merge of
 // tile parts into evidence rows -- around the kernels of evidence_kernel.hip.
-// One pinned upload of the tables, six kernels on the context's stream, one
+// One pinned upload of the tables, seven kernels on the context's stream, one
 // copy back.
 #include <hip/hip_runtime.h>
 
@@ -206,7 +206,8 @@ extern "C" int fbls_compare(fbls_ctx* c, const fbls_family* f, double period,
     // one staging buffer: tables in, floors + parts (+ screen) out
     const size_t n_tab = 4 * nd + ns * layers * 3 + 2 * ns + fbls::kEvParams + nd * 12;
     const size_t tab_bytes = n_tab * 8;
-    const size_t scratch = fbls::evidence_scratch_bytes((uint32_t)nb2, (uint32_t)tiles);
+    const size_t scratch =
+        fbls::evidence_scratch_bytes((uint32_t)nb2, (uint32_t)tiles, (uint32_t)(nd * ns));
     const size_t tab_off = (scratch + 255) & ~(size_t)255;
     if (int st = c->dev_ev.ensure(tab_off + tab_bytes)) return st;
     const size_t out_doubles = 4 + tiles * 2 * fbls::kEvPart + tiles * 4;
@@ -263,7 +264,7 @@ extern "C" int fbls_compare(fbls_ctx* c, const fbls_family* f, double period,
 
     HIP_TRY(hipSetDevice(c->device));
     double* hout = (double*)c->pin_out.ptr;
-    // The call's whole device sequence: tables in, six kernels, results out.
+    // The call's whole device sequence: tables in, seven kernels, results out.
     auto enqueue = [&]() -> hipError_t {
       hipError_t e;
       if ((e = hipMemcpyAsync((void*)dtab, tab, tab_bytes, hipMemcpyHostToDevice, c->stream)))

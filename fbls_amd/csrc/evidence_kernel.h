@@ -37,10 +37,20 @@ struct EvArgs {
   double* screen;         // tiles x 4
   double* parts;          // tiles x 2 x 15
   double* floors;         // {floor_p, floor_b, pilot key, passed}
+  // per (template, phase), template-major with a row of tiles x 64 phases;
+  // a template is t = duration index x ns + shape index
+  double* t_lp;           // planet log evidence of the template
+  double* t_lb;           // binary log evidence
+  double* t_snr;          // SNR of a valid template, -inf otherwise
+  double* t_depth;        // depth
+  uint8_t* t_code;        // winning scenario (1-15 planet, 16-22 odd/even, 0 none),
+                          // bit 7: the binary diagnostics are not certain
+  uint32_t* t_kept;       // tiles x templates: scenario groups evaluated (bits 0-2
+                          // secondary phases, bit 3 odd/even)
 };
 
 // Bytes of device scratch the kernels need beyond the inputs.
-size_t evidence_scratch_bytes(uint32_t nb2, uint32_t tiles);
+size_t evidence_scratch_bytes(uint32_t nb2, uint32_t tiles, uint32_t templates);
 // Carves the scratch regions of `a` out of `base` (evidence_scratch_bytes).
 void evidence_bind_scratch(EvArgs* a, void* base);
 // Enqueues zero, fold, prefix, screen, pilot and joint kernels on `s`.
