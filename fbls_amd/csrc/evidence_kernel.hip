@@ -472,8 +472,9 @@ __global__ __launch_bounds__(64) void ev_pilot(EvArgs A) {
     const double high = sums[sel] > -kInf ? sums[sel] : -kInf;
     double log_sum = high;
     if (high > -kInf) {
+      const double term = fm::exp(sums[sel] - high);  // the same for all 64 lanes
       double acc = 0.0;
-      for (uint32_t l = 0; l < kEvTile; l++) acc += fm::exp(sums[sel] - high);
+      for (uint32_t l = 0; l < kEvTile; l++) acc += term;
       log_sum = high + fm::log(acc);
     }
     A.floors[sel] = log_sum + A.margin;
@@ -631,13 +632,14 @@ __global__ __launch_bounds__(64) void ev_reduce(EvArgs A) {
     uint32_t chosen = first_lane(v == maximum && kk == best_key && kk < kKeyMax);
     if (chosen > kEvTile - 1) chosen = kEvTile - 1;
     const double high = lanes_max(s);
-    s_sum[lane] = s;
+    // each lane's term in parallel, summed by one lane in lane order
+    s_sum[lane] = high > -kInf ? fm::exp(s - high) : 0.0;
     __syncthreads();
     if (lane == 0) {
       double log_sum = high;
       if (high > -kInf) {
         double acc = 0.0;
-        for (uint32_t l = 0; l < kEvTile; l++) acc += fm::exp(s_sum[l] - high);
+        for (uint32_t l = 0; l < kEvTile; l++) acc += s_sum[l];
         log_sum = high + fm::log(acc);
       }
       s_log = log_sum;
