@@ -439,7 +439,11 @@ uint32_t fbls::host::launch_block(const fbls_ctx* c, bool split, uint32_t groups
   // when the grid fills the chip; a grid of fewer than two workgroups per CU
   // (a fine pass over a few hundred prescan survivors) of the LDS-resident
   // kernel gets 512 threads, so each of its periods folds with twice the waves
-  return c->block_override ? c->block_override : (!split && groups < 512 ? 512 : 256);
+  // The range-split kernel holds one period's window in up to the whole
+  // LDS, one workgroup per CU: 512 threads (8 waves, two per SIMD, with no
+  // register spills) hide its loads.
+  if (split) return 512;
+  return c->block_override ? c->block_override : (groups < 512 ? 512 : 256);
 }
 
 void fbls::host::segment_fields(const fbls_ctx* c, const Work& x, const SegShape& g,

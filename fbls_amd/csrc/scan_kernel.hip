@@ -288,6 +288,50 @@ __device__ void wave_prefix_n(uint32_t* arr, uint32_t len_slots, uint32_t lane) 
   for (uint32_t s = lo; s < hi; s++) arr[s] += ex;
 }
 
+// In-place inclusive prefix of slots [1, len] of the {v, w} pairs (then the
+// power-of-two scales) and of the counts, by the whole workgroup: each
+// thread sums a contiguous run of slots, the run totals are scanned across
+// the waves, and each run adds its offset.  Exact: every partial sum is an
+// integer-valued double below 2^53, so any order gives the same doubles.
+// wtot: 3 x BLOCK/64 doubles of LDS.
+template <uint32_t BLOCK>
+__device__ void block_prefix(double2* vw, uint32_t* n, uint32_t len, double sv,
+                             double sw, double* wtot, uint32_t tid) {
+  constexpr uint32_t kW = BLOCK / 64;
+  const uint32_t lane = tid & 63, wave = tid >> 6;
+  const uint32_t per = (len + BLOCK - 1) / BLOCK;
+  const uint32_t lo = min(1 + tid * per, len + 1);
+  const uint32_t hi = min(lo + per, len + 1);
+  double av = 0.0, aw = 0.0;
+  uint32_t an = 0;
+  for (uint32_t s = lo; s < hi; s++) {
+    const double2 x = vw[s];
+    av += x.x;
+    aw += x.y;
+    vw[s] = make_double2(av, aw);
+    an += n[s];
+    n[s] = an;
+  }
+  double ev = wave_excl_scan(av, lane), ew = wave_excl_scan(aw, lane);
+  uint32_t en = wave_excl_scan(an, lane);
+  if (lane == 63) {
+    wtot[wave] = ev + av;
+    wtot[kW + wave] = ew + aw;
+    wtot[2 * kW + wave] = (double)(en + an);
+  }
+  __syncthreads();
+  for (uint32_t w = 0; w < wave; w++) {
+    ev += wtot[w];
+    ew += wtot[kW + w];
+    en += (uint32_t)wtot[2 * kW + w];
+  }
+  for (uint32_t s = lo; s < hi; s++) {
+    const double2 x = vw[s];
+    vw[s] = make_double2((x.x + ev) * sv, (x.y + ew) * sw);
+    n[s] += en;
+  }
+}
+
 // ------------------------------------------------------------------ fold
 
 // floor(fl(t * scale)) mod nb (kernels.rs:314-327).  Narrow: t*scale < 2^24,
@@ -433,7 +477,13 @@ __device__ void fold_all(const Segment& S, unsigned char* hist,
   }
 }
 
-// Fold into the window [x0, x0 + wlen) of the bins (mod nb) of one period.
+// Fold into the window [x0, x0 + wlen) of the bins (mod nb) of one period:
+// slot rel + 1 accumulates bin (x0 + rel) mod nb for rel < wlen.  With
+// ascending times, the 512 observations of a wave's block fall in the cells
+// floor(t_first * scale) .. floor(t_last * scale), so a block spanning fewer
+// than nb cells whose bins all miss the window is skipped from its two end
+// times alone (scalar loads).  Inside the window, runs of equal cells are
+// summed in registers and bins advance by cell differences, as in fold_lane.
 template <bool WIDE, bool NOCNT>
 __device__ void fold_window(const Segment& S, double* vw, uint32_t* n,
                             double scale, uint32_t x0, uint32_t wlen,
@@ -441,15 +491,54 @@ __device__ void fold_window(const Segment& S, double* vw, uint32_t* n,
   const uint32_t nb = S.nb;
   const float inv_lo = inv_nb_low(nb);
   const uint32_t nblk = S.n_pad / kObsPerWave;
+  gdouble* tg = (gdouble*)S.t;
+  auto rel_of = [&](uint32_t u) {
+    const uint32_t b = phase_bin<WIDE>(u, nb, inv_lo);
+    return b >= x0 ? b - x0 : b + nb - x0;
+  };
   for (uint32_t blk = wave; blk < nblk; blk += nwaves) {
-    const uint32_t base = blk * kObsPerWave + lane * kObsPerLane;
+    const uint32_t first = blk * kObsPerWave;
+    const bool full = first + kObsPerWave <= S.n;
+    if (S.sorted && full) {
+      const uint32_t uf = phase_cell_u(tg[first], scale);
+      const uint32_t ul = phase_cell_u(tg[first + kObsPerWave - 1], scale);
+      // the block's bins relative to x0: rf, rf + 1, ..., rf + (ul - uf) mod nb
+      if (ul - uf < nb) {
+        const uint32_t rf = rel_of(uf);
+        if (rf >= wlen && rf + (ul - uf) < nb) continue;
+      }
+    }
+    const uint32_t base = first + lane * kObsPerLane;
     double tt[kObsPerLane], vv[kObsPerLane], ww[kObsPerLane];
     load_obs(S, base, tt, vv, ww);
+    uint32_t u[kObsPerLane];
 #pragma unroll
-    for (uint32_t r = 0; r < kObsPerLane; r++) {
-      const uint32_t b = phase_bin<WIDE>(phase_cell_u(tt[r], scale), nb, inv_lo);
-      const uint32_t rel = b >= x0 ? b - x0 : b + nb - x0;
-      if (rel < wlen && base + r < S.n) lds_add<NOCNT>(vw, n, rel, vv[r], ww[r], 1u);
+    for (uint32_t r = 0; r < kObsPerLane; r++) u[r] = phase_cell_u(tt[r], scale);
+    if (full && S.sorted && __all(u[kObsPerLane - 1] - u[0] < nb)) {
+      uint32_t cr = rel_of(u[0]);
+      double av = vv[0], aw = ww[0];
+      uint32_t an = 1;
+#pragma unroll
+      for (uint32_t r = 1; r < kObsPerLane; r++) {
+        const uint32_t d = u[r] - u[r - 1];
+        const bool flush = d != 0;
+        uint32_t rr = cr + d;
+        rr = min(rr, rr - nb);
+        if (flush && cr < wlen) lds_add<NOCNT>(vw, n, cr, av, aw, an);
+        const double sv = av + vv[r];
+        av = flush ? vv[r] : sv;
+        const double sw = aw + ww[r];
+        aw = flush ? ww[r] : sw;
+        if (!NOCNT) an = flush ? 1u : an + 1u;
+        cr = rr;
+      }
+      if (cr < wlen) lds_add<NOCNT>(vw, n, cr, av, aw, an);
+    } else {
+#pragma unroll
+      for (uint32_t r = 0; r < kObsPerLane; r++) {
+        const uint32_t rel = rel_of(u[r]);
+        if (rel < wlen && base + r < S.n) lds_add<NOCNT>(vw, n, rel, vv[r], ww[r], 1u);
+      }
     }
   }
 }
@@ -936,6 +1025,8 @@ __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
   p += tiles * 4;
   p = (unsigned char*)(((uintptr_t)p + 7) & ~(uintptr_t)7);
   double* carry = (double*)p;  // {bv, bw, bin0 v, bin0 w}, bn as double
+  p += 5 * 8;
+  double* wtot = (double*)p;   // [3][BLOCK / 64] wave totals of the prefix
 
   if (wave == 0) duration_tables(S, A, f, ktab, stab, lane, 64);
   if (tid == 0) {
@@ -983,8 +1074,7 @@ __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
       for (uint32_t i = tid; i < wlen; i += BLOCK) wn[i + 1] = count_of(wvw[i + 1].y, w0);
       __syncthreads();
     }
-    if (wave == 0) wave_prefix_vw(wvw, wlen, lane, S.sv, S.sw);
-    if (wave == 1 % kWaves) wave_prefix_n(wn, wlen, lane);
+    block_prefix<BLOCK>(wvw, wn, wlen, S.sv, S.sw, wtot, tid);
     __syncthreads();
     WindowCum cm;
     cm.vw = wvw;
@@ -1074,13 +1164,15 @@ size_t split_lds_bytes(uint32_t nb, uint32_t range_bins, uint32_t halo_bins,
   bytes += (size_t)tiles * (32 + 8 + 4);
   bytes = (bytes + 7) & ~(size_t)7;
   bytes += 5 * 8;
+  bytes += 3 * 16 * 8;  // wave totals of the prefix (up to 1024 threads)
   return bytes;
 }
 
 hipError_t scan_configure(size_t max_lds) {
   const void* fns[] = {(const void*)fused_scan<256>, (const void*)fused_scan<512>,
                        (const void*)fused_scan_split<256>,
-                       (const void*)fused_scan_split<512>};
+                       (const void*)fused_scan_split<512>,
+                       (const void*)fused_scan_split<1024>};
   for (const void* fn : fns) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)max_lds);
@@ -1093,7 +1185,9 @@ hipError_t scan_launch(const LaunchArgs& args, uint32_t groups, uint32_t block,
                        size_t lds, bool split, hipStream_t stream) {
   if (groups == 0) return hipSuccess;
   if (split) {
-    if (block == 512)
+    if (block == 1024)
+      hipLaunchKernelGGL(fused_scan_split<1024>, dim3(groups), dim3(1024), lds, stream, args);
+    else if (block == 512)
       hipLaunchKernelGGL(fused_scan_split<512>, dim3(groups), dim3(512), lds, stream, args);
     else
       hipLaunchKernelGGL(fused_scan_split<256>, dim3(groups), dim3(256), lds, stream, args);
