@@ -398,6 +398,13 @@ int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, cons
                  c->max_lds &&
              B < w[i].nb)
         B += 128;
+      // a window of B + halo bins holds each bin at most once (the fold
+      // places an observation in one slot): B <= nb - halo
+      const uint32_t cap = (w[i].nb > halo ? (w[i].nb - halo) / 128 * 128 : 0);
+      B = std::min(B, std::max<uint32_t>(cap, 128));
+      if (B + halo > w[i].nb)
+        return fail(FBLS_E_ARG, "templates of %u bins leave no range of %u bins", halo,
+                    w[i].nb);
       if (fbls::split_lds_bytes(w[i].nb, B, halo, w[i].nd, ns, layers) > c->max_lds)
         return fail(FBLS_E_ARG, "templates of %u bins do not fit the LDS", halo);
       shapes[i].halo = halo;
