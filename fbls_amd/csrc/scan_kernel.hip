@@ -382,23 +382,37 @@ __device__ __forceinline__ void fold_lane(double* vw, uint32_t* n,
                                           const double* ww, uint32_t nb,
                                           float inv_lo, uint32_t base,
                                           uint32_t nobs) {
-  if (RUN && FULL) {
+  if (RUN && FULL && INCR) {
+    // bin r = (b0 + u[r] - u[0]) mod nb, at most one wrap: u[r] - off with
+    // off = u[0] - b0, then a conditional subtraction
+    const uint32_t b0 = phase_bin<WIDE>(u[0], nb, inv_lo);
+    const uint32_t off = u[0] - b0;
+    uint32_t cb = b0;
+    double av = vv[0], aw = ww[0];
+    uint32_t an = 1;
+#pragma unroll
+    for (uint32_t r = 1; r < kObsPerLane; r++) {
+      uint32_t b = u[r] - off;
+      b = min(b, b - nb);
+      const bool flush = u[r] != u[r - 1];
+      if (flush) lds_add<NOCNT>(vw, n, cb, av, aw, an);
+      // a run restarts at a flush: av * 0 + v or av * 1 + v, exact for the
+      // integer-valued sums
+      const double keep = flush ? 0.0 : 1.0;
+      av = __builtin_fma(av, keep, vv[r]);
+      aw = __builtin_fma(aw, keep, ww[r]);
+      if (!NOCNT) an = flush ? 1u : an + 1u;
+      cb = b;
+    }
+    lds_add<NOCNT>(vw, n, cb, av, aw, an);
+  } else if (RUN && FULL) {
     uint32_t cb = phase_bin<WIDE>(u[0], nb, inv_lo);
     double av = vv[0], aw = ww[0];
     uint32_t an = 1;
 #pragma unroll
     for (uint32_t r = 1; r < kObsPerLane; r++) {
-      uint32_t b;
-      bool flush;
-      if (INCR) {
-        const uint32_t d = u[r] - u[r - 1];
-        flush = d != 0;
-        b = cb + d;
-        b = min(b, b - nb);
-      } else {
-        b = phase_bin<WIDE>(u[r], nb, inv_lo);
-        flush = b != cb;
-      }
+      const uint32_t b = phase_bin<WIDE>(u[r], nb, inv_lo);
+      const bool flush = b != cb;
       if (flush) lds_add<NOCNT>(vw, n, cb, av, aw, an);
       const double sv = av + vv[r];
       av = flush ? vv[r] : sv;
