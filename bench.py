@@ -109,89 +109,156 @@ def run_gpu(cfg, d, state, steps, warmup, launch=(0, 0, 0)):
     return gpu, dev, rows, grid, per_step, elapsed, totals
 
 
-def proposal_stars(n_stars):
-    """Fresh single-sector injection stars (seeds never used in training),
-    prepared on the host before the GPU is touched (worker processes)."""
-    from fbls_amd import proposal as PR
-
-    pc = PR.ProposalConfig()
-    return PR.make_states(range(n_stars), 4_300_000, PR.search_config(pc), workers=16)
+def log(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def _batch_state(args):
-    i, durations, correlated = args
-    from fbls_amd import family_state, synth
-
-    (t, f, e), kw = synth.batch_star(i)
-    return family_state(t, f - 1.0, e, durations, correlated), kw.get("period", 0.0)
-
-
-def batch_stars(n_stars):
-    """The 512-star batch of uneven length (1-13 sectors), prepared on the
-    host before the GPU is touched (worker processes)."""
+def _threads(fn, items, workers=16):
+    """fn over items on host threads (the library's calls release the
+    interpreter lock)."""
     import concurrent.futures as cf
-    import multiprocessing as mp
 
+    with cf.ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(fn, items))
+
+
+def single_sector_curves(n_stars):
+    """Fresh single-sector injection stars (seeds never used in training):
+    [((time, flux, flux_err), injected period or 0)]."""
+    from fbls_amd import proposal as PR
+    from fbls_amd import synth_star
+
+    def make(i):
+        kw = PR.star_params(i, 4_300_000)
+        return synth_star(**kw), kw.get("period", 0.0)
+
+    return _threads(make, range(n_stars))
+
+
+def batch_curves(n_stars):
+    """The 512-star batch of uneven length (1-13 sectors)."""
+    from fbls_amd import synth
+
+    def make(i):
+        curve, kw = synth.batch_star(i)
+        return curve, kw.get("period", 0.0)
+
+    return _threads(make, range(n_stars))
+
+
+def long_curves(n_stars):
+    """Fresh long-baseline stars (1-26 sectors around a one-year gap,
+    periods 0.5-100 d; seeds never used in training or selection):
+    [(curve, injected period or 0, kind, sectors)]."""
+    from fbls_amd import synth
+
+    def make(i):
+        curve, kw = synth.long_baseline_star(i, 5_400_000)
+        return curve, kw.get("period", 0.0), kw["kind"], kw["sectors"]
+
+    return _threads(make, range(n_stars))
+
+
+def prepare(curves, cfg):
+    from fbls_amd import family_state
+
+    d = cfg.durations_days()
+    return _threads(lambda c: family_state(c[0], c[1] - 1.0, c[2], d, cfg.correlated_noise),
+                    curves)
+
+
+def end_to_end(gpu, curves, injected, cfg, net=None, pc=None, op=None, mode="pruned"):
+    """The full scan and the learned-pruned search of the same stars, each
+    timed end to end from light curves in host memory (preparation on 16 host
+    threads pipelined with the GPU, upload, proposal, search, selection),
+    with the recall of the pruned search against the full scan and the host
+    preparation rate on 1 and 16 threads."""
+    from fbls_amd import endtoend as E
+    from fbls_amd import proposal as PR
+
+    E.run(gpu, curves[:4], cfg, "full", chunk=4)  # staging growth outside timing
+    full = E.run(gpu, curves, cfg, "full")
+    out = {"stars": len(curves), "host_workers": full["host_workers"],
+           "chunk_stars": full["chunk"],
+           "full": {k: full[k] for k in ("seconds", "stars_per_hour", "stage_seconds")}}
+    rf = PR.recovered(full["best"], injected)
+    out["full_scan_recovered"] = int(rf.sum())
+    if net is not None:
+        E.run(gpu, curves[:4], cfg, mode, net=net, pc=pc, op=op, chunk=4)
+        pr = E.run(gpu, curves, cfg, mode, net=net, pc=pc, op=op)
+        rp = PR.recovered(pr["best"], injected)
+        out["pruned"] = {k: pr[k] for k in ("seconds", "stars_per_hour", "stage_seconds")}
+        out["recall_vs_full_scan"] = float((rf & rp).sum() / max(rf.sum(), 1))
+        out["speedup"] = pr["stars_per_hour"] / full["stars_per_hour"]
+    sample = curves[:min(64, len(curves))]
+    out["host_preparation_stars_per_hour"] = {
+        "1_thread": E.preparation_rate(sample, cfg, 1),
+        "16_threads": E.preparation_rate(curves, cfg, 16)}
+    return out
+
+
+def batched(gpu, data):
+    """The full scan of every star of the batch, end to end: preparation on
+    16 host threads pipelined with the packed upload and the merged-launch
+    search (fbls_signal_search_batch) of chunks of 128 stars, and selection.
+    The device-side rate counts the search calls only."""
     import fbls_amd as F
-
-    cfg = F.SearchConfig(period_min=0.5, period_max=20.0)
-    jobs = [(i, cfg.durations_days(), cfg.correlated_noise) for i in range(n_stars)]
-    with cf.ProcessPoolExecutor(16, mp_context=mp.get_context("spawn")) as ex:
-        return list(ex.map(_batch_state, jobs, chunksize=4))
-
-
-def batched(gpu, res):
-    """The full scan of every star of the batch in merged launches
-    (fbls_signal_search_batch over one packed upload)."""
-    import fbls_amd as F
+    from fbls_amd import endtoend as E
     from fbls_amd import proposal as PR
 
     cfg = F.SearchConfig(period_min=0.5, period_max=20.0)
-    states = [r[0] for r in res]
-    devs = F.DeviceReady.upload_packed(gpu, states)
-    d = cfg.durations_days()
-    rhos = [s.rho for s in states]
-    F.signal_search_batch(gpu, devs[:8], d, rhos[:8], cfg)  # staging growth outside timing
+    curves = [c for c, _ in data]
+    inj = [p for _, p in data]
+    E.run(gpu, curves[:4], cfg, "full", chunk=4)  # staging growth outside timing
     gpu.launch_totals(reset=True)
-    t0 = time.perf_counter()
-    rows, stats = F.signal_search_batch(gpu, devs, d, rhos, cfg)
-    dt = time.perf_counter() - t0
+    r = E.run(gpu, curves, cfg, "full")
     tot = gpu.launch_totals(reset=True)
-    templates = sum(st["templates_scanned"] for st in stats)
-    best = PR.best_rows(rows, cfg.threshold)
-    inj = [r[1] for r in res]
-    found = sum(1 for (p, _), c in zip(best, inj) if c > 0 and p == p and PR.alias(p, c) < 1e-3)
-    for dv in devs:
-        dv.close()
-    sizes = [len(s.t) for s in states]
+    found = PR.recovered(r["best"], inj)
+    sizes = [len(c[0]) for c in curves]
+    search_s = r["stage_seconds"]["search"]
     return {
-        "workload": f"{len(res)} stars, 1-13 sectors ({min(sizes)}-{max(sizes)} points), "
+        "workload": f"{len(curves)} stars, 1-13 sectors ({min(sizes)}-{max(sizes)} points), "
                     "sigma 200-3000 ppm, half with a box P 0.5-20 d; periods 0.5-20 d",
-        "stars_per_hour": len(res) * 3600.0 / dt,
-        "trial_evals_per_s": templates / dt,
-        "seconds": dt,
+        "stars_per_hour": r["stars_per_hour"],
+        "seconds": r["seconds"],
+        "stage_seconds": r["stage_seconds"],
+        "host_workers": r["host_workers"],
+        "device_stars_per_hour": len(curves) * 3600.0 / search_s,
+        "trial_evals_per_s": r["trial_evals"] / search_s,
         "kernel_seconds": tot["seconds"],
         "launches": tot["launches"],
-        "periods": int(sum(st["periods"] for st in stats)),
-        "injections_recovered": found,
+        "periods": int(tot["periods"]),
+        "fold_pairs": int(tot["fold_pairs"]),
+        "fold_pairs_per_kernel_s": tot["fold_pairs"] / tot["seconds"],
+        "injections_recovered": int(found.sum()),
         "injections": sum(1 for c in inj if c > 0),
+        "host_preparation_stars_per_hour": {
+            "1_thread": E.preparation_rate(curves[:16], cfg, 1),
+            "16_threads": E.preparation_rate(curves, cfg, 16)},
     }
 
 
-def learned_pruned(gpu, res):
+def learned_pruned(gpu, data):
     """The learned-pruned search against the full scan at the committed
-    network's operating point."""
+    network's operating point: device-side (stars prepared and uploaded
+    beforehand) and end to end."""
     from fbls_amd import proposal as PR
 
-    n_stars = len(res)
+    n_stars = len(data)
     net, pc, extra = PR.load(gpu)
-    ev = PR.EvalSet(gpu, [r[0] for r in res], [r[1] for r in res], pc)
+    cfg = PR.search_config(pc)
+    curves = [c for c, _ in data]
+    inj = [p for _, p in data]
+    states = prepare(curves, cfg)
+    ev = PR.EvalSet(gpu, states, inj, pc)
     topk, tau = int(extra["topk"]), float(extra["tau"])
     fb = tuple(float(v) for v in np.atleast_1d(extra["fallback"]))
     sw = ev.sweep(net, points=[(tau, topk, fb)])
     ev.close()
     r = sw["learned"][0]
     raw = sw["raw_snr_topk"][0]
+    e2e = end_to_end(gpu, curves, inj, cfg, net, pc, (tau, topk, fb), "pruned")
     return {
         "workload": f"{n_stars} single-sector injection stars (sigma 200-3000 ppm, "
                     "half with a box), periods 0.5-13.7 d",
@@ -207,18 +274,11 @@ def learned_pruned(gpu, res):
         "fallback_stars": r["fallback_stars"],
         "raw_snr_ranking_recall_same_kept_bins": raw["recall"],
         "raw_snr_ranking_kept_period_fraction": raw["kept_period_fraction"],
+        "end_to_end": e2e,
     }
 
 
-def long_stars(n_stars):
-    """Fresh long-baseline stars (1-26 sectors around a one-year gap,
-    periods 0.5-100 d; seeds never used in training or selection)."""
-    from fbls_amd import longbase as LB
-
-    return LB.make_states(range(n_stars), 5_400_000, workers=16)
-
-
-def long_baseline(gpu, res):
+def long_baseline(gpu, data):
     """The learned-pruned search of long-baseline stars (the network
     fine-tuned on them, at its committed operating point) against their
     full scan."""
@@ -228,6 +288,10 @@ def long_baseline(gpu, res):
     net, pc, extra = PR.load(gpu, LB.WEIGHTS)
     op = (float(extra["tau"]), int(extra["topk"]),
           tuple(float(v) for v in np.atleast_1d(extra["fallback"])))
+    cfg = LB.search_config()
+    curves = [d[0] for d in data]
+    states = prepare(curves, cfg)
+    res = [(s, d[1], d[2], d[3]) for s, d in zip(states, data)]
     lb = LB.LongBatch(gpu, res, net, pc)
     summary = lb.summary()
     r = lb.sweep([op])[0]
@@ -249,6 +313,7 @@ def long_baseline(gpu, res):
         "trial_evals_pruned": r["templates_pruned"],
         "kept_period_fraction": r["kept_period_fraction"],
         "fallback_stars": r["fallback_stars"],
+        "end_to_end": end_to_end(gpu, curves, [d[1] for d in data], cfg, net, pc, op, "long"),
     }
 
 
@@ -334,9 +399,10 @@ def main():
               "is not run here; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    pstars = proposal_stars(args.proposal_stars) if args.proposal_stars > 0 else None
-    bstars = batch_stars(args.batch_stars) if args.batch_stars > 0 else None
-    lstars = long_stars(args.long_stars) if args.long_stars > 0 else None
+    pstars = single_sector_curves(args.proposal_stars) if args.proposal_stars > 0 else None
+    bstars = batch_curves(args.batch_stars) if args.batch_stars > 0 else None
+    lstars = long_curves(args.long_stars) if args.long_stars > 0 else None
+    log("stars generated")
     cfg, d, state = workload()
     n = len(state.t)
     gpu, dev, rows, grid, per_step, elapsed, tot = run_gpu(
@@ -402,12 +468,16 @@ def main():
             "lds_busy": None if traffic is None else traffic["lds_busy_coarse"],
         },
     }
+    log(f"four-sector star: {ms:.2f} ms per step")
     if pstars:
         out["learned_pruned"] = learned_pruned(gpu, pstars)
+        log("single-sector learned-pruned vs full scan done")
     if bstars:
         out["batched"] = batched(gpu, bstars)
+        log("batch of uneven stars done")
     if lstars:
         out["long_baseline"] = long_baseline(gpu, lstars)
+        log("long-baseline learned-pruned vs full scan done")
     if not args.no_cpu:
         cb, orows = cpu_baseline(cfg, d, state)
         out["cpu_baseline"] = cb

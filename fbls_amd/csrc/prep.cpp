@@ -149,15 +149,19 @@ std::vector<fbls_band> plan_bands(const GridFormula& grid, const double* sd, siz
 
 namespace {
 
-// stats.rs:4-11
+// stats.rs:4-11: the lower median of the finite values, the element a sort
+// would place at (m - 1) / 2, found by selection in linear time (a +0.0 and
+// a -0.0 may trade places, which no caller can see: each median is a divisor
+// checked positive, or subtracted before an absolute value)
 double median_lower(const std::vector<double>& values) {
   std::vector<double> v;
   v.reserve(values.size());
   for (double x : values)
     if (std::isfinite(x)) v.push_back(x);
   if (v.empty()) return NAN;
-  std::sort(v.begin(), v.end());
-  return v[(v.size() - 1) / 2];
+  const size_t k = (v.size() - 1) / 2;
+  std::nth_element(v.begin(), v.begin() + (ptrdiff_t)k, v.end());
+  return v[k];
 }
 
 // stats.rs:14-18
@@ -187,8 +191,12 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
     if (std::isfinite(time[i]) && std::isfinite(flux[i]) && std::isfinite(e) && e > 0.0)
       keep.push_back(i);
   }
-  std::stable_sort(keep.begin(), keep.end(),
-                   [&](size_t a, size_t b) { return time[a] < time[b]; });
+  bool ascending = true;
+  for (size_t k = 1; k < keep.size() && ascending; k++)
+    ascending = !(time[keep[k]] < time[keep[k - 1]]);
+  if (!ascending)  // a stable sort of ascending times is the identity
+    std::stable_sort(keep.begin(), keep.end(),
+                     [&](size_t a, size_t b) { return time[a] < time[b]; });
   if (keep.size() < 6) {
     *msg = "insufficient usable observations";
     return FBLS_E_DATA;
@@ -202,23 +210,40 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
     e[k] = err ? err[keep[k]] : 1.0;
     sec[k] = group ? group[keep[k]] : 0;
   }
-  std::vector<int64_t> labels = sec;
+  // distinct group labels in ascending order, and each observation's group
+  std::vector<int64_t> labels;
+  for (size_t k = 0; k < m; k++)
+    if (std::find(labels.begin(), labels.end(), sec[k]) == labels.end()) {
+      labels.push_back(sec[k]);
+      if (labels.size() > 64) {  // many groups: sort them all
+        labels = sec;
+        std::sort(labels.begin(), labels.end());
+        labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
+        break;
+      }
+    }
   std::sort(labels.begin(), labels.end());
-  labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
-  for (int64_t label : labels) {
+  std::vector<uint32_t> gi(m);
+  std::vector<size_t> gcount(labels.size(), 0);
+  for (size_t k = 0; k < m; k++) {
+    gi[k] = (uint32_t)(std::lower_bound(labels.begin(), labels.end(), sec[k]) - labels.begin());
+    gcount[gi[k]]++;
+  }
+  std::vector<double> med(labels.size());
+  for (size_t g = 0; g < labels.size(); g++) {
     std::vector<double> vals;
+    vals.reserve(gcount[g]);
     for (size_t k = 0; k < m; k++)
-      if (sec[k] == label) vals.push_back(f[k]);
-    const double med = median_lower(vals);
-    if (!(med > 0.0)) {
+      if (gi[k] == g) vals.push_back(f[k]);
+    med[g] = median_lower(vals);
+    if (!(med[g] > 0.0)) {
       *msg = "mission flux must have a positive median";
       return FBLS_E_DATA;
     }
-    for (size_t k = 0; k < m; k++)
-      if (sec[k] == label) {
-        f[k] = f[k] / med - 1.0;
-        e[k] /= med;
-      }
+  }
+  for (size_t k = 0; k < m; k++) {
+    f[k] = f[k] / med[gi[k]] - 1.0;
+    e[k] /= med[gi[k]];
   }
   const double origin = t[0];
   for (double& x : t) x -= origin;
@@ -338,8 +363,18 @@ void correlated_noise_factors(const std::vector<double>& time,
       if (cnt > 0 && (double)cnt >= min_fill * d / cadence)
         z.push_back((acc / total) * std::sqrt(total));
     };
+    // cell = floor(fl(x / d)) for x = t - t0: from x * (1 / d) when that
+    // product is farther than 1e-12 relative from an integer (the two differ
+    // by a few ulps, so their floors agree), from the division otherwise
+    const double inv = 1.0 / d;
     for (size_t i = 0; i < n; i++) {
-      const int64_t cell = (int64_t)std::floor((time[i] - time[0]) / d);
+      const double x = time[i] - time[0];
+      const double q = x * inv;
+      const double fq = std::floor(q);
+      const double margin = 1e-12 * (q > 1.0 ? q : 1.0);
+      const int64_t cell = (q - fq > margin && fq + 1.0 - q > margin)
+                               ? (int64_t)fq
+                               : (int64_t)std::floor(x / d);
       if (cell != current) {
         flush();
         current = cell;
