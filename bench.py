@@ -12,14 +12,15 @@ metric value: trial evals/s, "templates_scanned" of the reference
 (periods x durations x shapes x phase bins, coarse + full pass) per second of
 the whole step (host band loop included; inputs already resident in HBM).
 
-roofline: the fused scan kernel against HBM bandwidth, with the algorithmic
-bytes of one trial period = 16 n + 88 B (FP64 time, 32-bit v and w read once
-per period, one 11-double record).  The kernel keeps the star resident in
-L2/LDS, so this streaming roofline can be exceeded; the measured HBM traffic
-is reported separately from rocprofv3 counters (profiles/).
+roofline: the fused scan kernel against VALU issue, the unit that binds it.
+The star (16 B per observation) stays in L2 and each workgroup folds it into
+LDS, so HBM carries little more than the period records; VALU issue and LDS
+are both about 70 % busy on the coarse launch.  The VALU instruction count
+and the HBM bytes per launch come from the committed rocprofv3 counter
+passes (profiles/), the launch duration from HIP events in this run.
 
 cpu_baseline: the FP64 C oracle (same algorithm, OpenMP over periods) on the
-host cores, over the same star.
+host cores, over the same star, and on one core.
 """
 import argparse
 import json
@@ -65,27 +66,31 @@ def calibrate_copy(mib=256, reps=4):
     print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps}))
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_four_sector_star.md")
+COUNTER_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_four_sector_star.md")
+
+# VALU issue ceiling of the chip: each CU issues one wave64 VALU instruction
+# per cycle (four 16-lane SIMDs, four cycles per wave64 instruction), 256 CUs
+# at the 2.4 GHz peak engine clock
+VALU_PEAK_G = 256 * 2.4
 
 
-def measured_traffic():
-    """HBM bytes per launch of the scan kernel on this workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (already scaled by the
-    copy-kernel calibration).  The counters need their own profiler runs, so
-    the figure is read from the summary those runs produced."""
+def measured_counters():
+    """Per-launch counters of the scan kernel on this workload, from the
+    committed rocprofv3 passes (HBM bytes from FETCH_SIZE / WRITE_SIZE scaled
+    by the copy-kernel calibration; VALU wave-instructions from SQ_INSTS_VALU;
+    the coarse launch's VALU and LDS busy fractions).  The counters need
+    their own profiler runs, so the figures are read from the summary those
+    runs produced."""
     import re
 
     try:
-        text = open(TRAFFIC_PROFILE).read()
+        text = open(COUNTER_PROFILE).read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
-    if not m:
-        return None
-    out = {"bytes_per_launch": float(m.group(1)),
-           "source": os.path.relpath(TRAFFIC_PROFILE, ROOT)}
-    for key in ("valu_busy_coarse", "lds_busy_coarse"):
-        k = re.search(key + r":\s*([0-9.]+)", text)
+    out = {"source": os.path.relpath(COUNTER_PROFILE, ROOT)}
+    for key in ("traffic_bytes_per_launch", "valu_insts_per_launch", "valu_busy_coarse",
+                "lds_busy_coarse", "valu_lane_insts_per_fold_pair", "fold_pairs_per_step"):
+        k = re.search(key + r":\s*([0-9.eE+]+)", text)
         out[key] = float(k.group(1)) if k else None
     return out
 
@@ -309,6 +314,9 @@ def long_baseline(gpu, data):
         "stars_per_hour_pruned": r["stars_per_hour_pruned"],
         "speedup": r["speedup"],
         "full_scan_trial_evals_per_s": summary["full_scan_trial_evals_per_s"],
+        "full_scan_kernel_seconds": summary["full_scan_kernel_seconds"],
+        "full_scan_launches": summary["full_scan_launches"],
+        "full_scan_fold_pairs_per_kernel_s": summary["full_scan_fold_pairs_per_kernel_s"],
         "trial_evals_full": r["templates_full"],
         "trial_evals_pruned": r["templates_pruned"],
         "kept_period_fraction": r["kept_period_fraction"],
@@ -317,21 +325,43 @@ def long_baseline(gpu, data):
     }
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(cfg, d, state):
+    """The FP64 C oracle's signal search of the same star on the host: on
+    every OpenMP thread (the reported value) and on one thread."""
     import oracle as O
 
+    def search(threads):
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        rows, bands, g, st = O.signal_search(
+            state.t, state.v, state.w, state.baseline_norm, state.origin, d,
+            state.rho, cfg.ingress_fractions, cfg, cfg.period_min, cfg.period_max,
+            cfg.period_oversample)
+        dt = time.perf_counter() - t0
+        return rows, sum(b["templates_scanned"] for b in bands), dt
+
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    rows, bands, g, st = O.signal_search(
-        state.t, state.v, state.w, state.baseline_norm, state.origin, d,
-        state.rho, cfg.ingress_fractions, cfg, cfg.period_min, cfg.period_max,
-        cfg.period_oversample)
-    dt = time.perf_counter() - t0
-    templates = sum(b["templates_scanned"] for b in bands)
+    rows, templates, dt = search(threads)
+    _, templates1, dt1 = search(1)
+    O.set_threads(0)
     return {"value": templates / dt, "unit": "trial evals/s", "cores": threads,
             "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"full star, one signal search ({templates:.3e} "
-                      f"templates, {dt:.2f} s, FP64 C oracle, OpenMP)"}, rows
+                      f"templates, {dt:.2f} s, FP64 C oracle, OpenMP)",
+            "one_core": {"value": templates1 / dt1, "seconds": dt1,
+                         "sample": "the same search on one thread"}}, rows
 
 
 def oracle_errors(gpu, dev, state, d, cfg, rows, orows):
@@ -369,6 +399,48 @@ def oracle_errors(gpu, dev, state, d, cfg, rows, orows):
             "evidence_best_row_max_rel": ev, "evidence_period": best}
 
 
+def roofline(ctr, launch_s, launches, tot, steps, stream_bytes, fold_pairs):
+    """The scan kernel against the unit that binds it.  The star (16 B per
+    observation) stays in L2 and every workgroup re-reads it from there, so
+    HBM carries only the records: the kernel is bound on chip, by VALU issue
+    (with LDS close behind).  `achieved` is the VALU wave-instructions of one
+    launch (SQ_INSTS_VALU, mean of the coarse and fine launch, from the
+    committed profile) over the launch's mean duration measured here with HIP
+    events on the library's stream; `peak` is one wave64 VALU instruction per
+    CU per cycle at 2.4 GHz.  The HBM fraction is the measured FETCH + WRITE
+    bytes per launch over the same duration."""
+    valu = None if ctr is None else ctr["valu_insts_per_launch"]
+    traffic = None if ctr is None else ctr["traffic_bytes_per_launch"]
+    achieved = None if valu is None else valu / launch_s / 1e9
+    hbm = None if traffic is None else traffic / launch_s / 1e9
+    return {
+        "bound": "valu",
+        "achieved": achieved,
+        "peak": VALU_PEAK_G,
+        "unit": "G VALU wave-instructions/s",
+        "frac": None if achieved is None else achieved / VALU_PEAK_G,
+        "traffic": traffic,
+        "kernel": "fused_scan<256> (coarse launch), fused_scan<512> (fine launch)",
+        "launches_per_step": launches,
+        "launch_ms": launch_s * 1e3,
+        "kernel_ms_per_step": tot["seconds"] / steps * 1e3,
+        "valu_insts_per_launch": valu,
+        "valu_lane_insts_per_fold_pair": None if ctr is None
+        else ctr["valu_lane_insts_per_fold_pair"],
+        "fold_pairs_per_step": fold_pairs,
+        "fold_pairs_per_kernel_s": fold_pairs / (tot["seconds"] / steps),
+        "valu_busy_coarse": None if ctr is None else ctr["valu_busy_coarse"],
+        "lds_busy_coarse": None if ctr is None else ctr["lds_busy_coarse"],
+        "hbm": {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None if hbm is None else hbm / HBM_PEAK_GBS,
+                "traffic_bytes_per_launch": traffic},
+        # what a kernel that streamed the star from HBM for every period
+        # would move (16 B per fold pair + 88 B per period record)
+        "streaming_equivalent_bytes_per_launch": stream_bytes,
+        "counter_source": None if ctr is None else ctr["source"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,7 +449,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lds", type=int, default=0,
                     help="LDS bytes per workgroup (0 = library default)")
-    ap.add_argument("--block", type=int, default=0, help="256 or 512")
+    ap.add_argument("--block", type=int, default=0, help="256, 512 or 1024")
     ap.add_argument("--fold-mode", type=int, default=0,
                     help="0 default (equal weights fold no counts), 1 run-length "
                          "with the counts folded, 2 one atomic per pair")
@@ -415,9 +487,8 @@ def main():
     # own stream over the timed steps
     launches = tot["launches"] / args.steps
     launch_s = tot["seconds"] / tot["launches"]
-    bytes_launch = (16.0 * tot["fold_pairs"] + 88.0 * tot["periods"]) / tot["launches"]
-    achieved = bytes_launch / launch_s / 1e9
-    traffic = measured_traffic()
+    stream_bytes = (16.0 * tot["fold_pairs"] + 88.0 * tot["periods"]) / tot["launches"]
+    ctr = measured_counters()
     ms = elapsed / args.steps * 1e3
     value = templates / (elapsed / args.steps)
     best = rows[np.argmax(rows[:, 3])] if len(rows) else None
@@ -448,25 +519,8 @@ def main():
             "best_period": None if best is None else float(best[0]),
             "best_snr": None if best is None else float(best[3]),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None if traffic is None else traffic["bytes_per_launch"],
-            "kernel": "fused_scan<256> (coarse launch), fused_scan<512> (fine launch)",
-            "launches_per_step": launches,
-            "launch_ms": launch_s * 1e3,
-            "algorithmic_bytes_per_launch": bytes_launch,
-            "kernel_ms_per_step": tot["seconds"] / args.steps * 1e3,
-            "bytes_per_period": 16.0 * n + 88.0,
-            "traffic_source": None if traffic is None else traffic["source"],
-            # what binds the kernel: the coarse launch's VALU and LDS busy
-            # fractions (rocprofv3 VALUBusy / LdsUtil, same profile)
-            "valu_busy": None if traffic is None else traffic["valu_busy_coarse"],
-            "lds_busy": None if traffic is None else traffic["lds_busy_coarse"],
-        },
+        "roofline": roofline(ctr, launch_s, launches, tot, args.steps, stream_bytes,
+                             launch_periods * n),
     }
     log(f"four-sector star: {ms:.2f} ms per step")
     if pstars:

@@ -110,8 +110,8 @@ extern "C" int fbls_ctx_set_launch(fbls_ctx* c, uint32_t lds_bytes,
   return fbls::host::guard([&]() -> int {
     if (!c) return fail(FBLS_E_ARG, "null context");
     if (fold_mode > 2) return fail(FBLS_E_ARG, "fold_mode must be 0, 1 or 2");
-    if (block != 0 && block != 256 && block != 512)
-      return fail(FBLS_E_ARG, "block must be 0, 256 or 512");
+    if (block != 0 && block != 256 && block != 512 && block != 1024)
+      return fail(FBLS_E_ARG, "block must be 0, 256, 512 or 1024");
     if (lds_bytes > c->max_lds)
       return fail(FBLS_E_ARG, "lds_bytes %u exceeds %zu", lds_bytes, c->max_lds);
     c->lds_override = lds_bytes;
@@ -381,8 +381,10 @@ int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, cons
   size_t total = 0;
   for (size_t i = 0; i < nw; i++) total += w[i].np;
   // the launch's LDS budget: the configured one, raised to the largest
-  // single-period need of its segments
-  size_t lds = c->lds_override ? c->lds_override : 32 * 1024;
+  // single-period need of its segments; by default half the CU's LDS (two
+  // 512-thread workgroups per CU), or all of it when one period needs more
+  // than half (one 1024-thread workgroup per CU)
+  size_t lds = c->lds_override ? c->lds_override : c->max_lds / 2;
   for (size_t i = 0; i < nw; i++) {
     shapes[i] = SegShape();
     if (w[i].np == 0) continue;
@@ -418,6 +420,7 @@ int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, cons
     }
     lds = std::max(lds, shapes[i].lds);
   }
+  if (!c->lds_override && !split && 2 * lds > c->max_lds) lds = c->max_lds;
   // at most total / 1024 periods per workgroup, so the grid fills 256 CUs
   const uint32_t g_cap = (uint32_t)std::max<size_t>(1, total / 1024);
   *lds_max = 0;
@@ -441,16 +444,20 @@ int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, cons
   return FBLS_OK;
 }
 
-uint32_t fbls::host::launch_block(const fbls_ctx* c, bool split, uint32_t groups) {
-  // 256 threads: measured fastest on MI355X for every LDS budget 16-48 KiB
-  // when the grid fills the chip; a grid of fewer than two workgroups per CU
-  // (a fine pass over a few hundred prescan survivors) of the LDS-resident
-  // kernel gets 512 threads, so each of its periods folds with twice the waves
+uint32_t fbls::host::launch_block(const fbls_ctx* c, bool split, size_t lds) {
   // The range-split kernel holds one period's window in up to the whole
   // LDS, one workgroup per CU: 512 threads (8 waves, two per SIMD, with no
   // register spills) hide its loads.
   if (split) return 512;
-  return c->block_override ? c->block_override : (groups < 512 ? 512 : 256);
+  if (c->block_override) return c->block_override;
+  // 16 waves per CU (four per SIMD, the occupancy of the kernel's 128
+  // VGPRs): two 512-thread workgroups when two fit the LDS, else one of
+  // 1024.  Measured on MI355X over stars of 76k-494k observations: against
+  // 256-thread workgroups of 32 KiB (four per CU), 2-3 % faster at 384 phase
+  // bins, up to 1.3x at 1536 (two periods per workgroup instead of one) and
+  // 1.3-1.9x at 3072-6144, where one period fills more than 40 KiB and
+  // 256-thread workgroups leave 4-8 waves per CU.
+  return 2 * lds > c->max_lds ? 1024 : 512;
 }
 
 void fbls::host::segment_fields(const fbls_ctx* c, const Work& x, const SegShape& g,
@@ -596,7 +603,7 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   A.factors = (const double*)(din + fac_at * 8);
   A.gate = sh.gate;
   A.counters = (unsigned int*)(din + cnt_off);
-  const uint32_t block = launch_block(c, split, groups);
+  const uint32_t block = launch_block(c, split, lds_max);
   HIP_TRY(hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   HIP_TRY(fbls::scan_launch(A, groups, block, lds_max, split, c->stream));
@@ -1487,8 +1494,8 @@ extern "C" int fbls_autotune(fbls_ctx* c, const fbls_family* f, const double* du
     if (!c || !f || !bank || !cfg || !ntable || !best) return fail(FBLS_E_ARG, "null argument");
     *ntable = 0;
     if (int st = check_search_cfg(cfg)) return st;
-    const uint32_t lds_opts[] = {16384, 24576, 32768, 49152, 65536, 98304};
-    const uint32_t block_opts[] = {256, 512};
+    const uint32_t lds_opts[] = {16384, 32768, 49152, 65536, 81920, 98304, 163840};
+    const uint32_t block_opts[] = {256, 512, 1024};
     const uint32_t saved_lds = c->lds_override, saved_block = c->block_override;
     const double d0 = *std::min_element(durations, durations + nd);
     const double hi = std::min(cfg->period_max, f->t_last);

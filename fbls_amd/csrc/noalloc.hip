@@ -180,7 +180,7 @@ extern "C" int fbls_scan_bank_noalloc(fbls_ctx* c, const fbls_family* f,
                            c->stream, d_periods, d_scales, (uint32_t)np, (double)nb);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
-        HIP_TRY(fbls::scan_launch(A, groups, launch_block(c, split, groups), lds, split,
+        HIP_TRY(fbls::scan_launch(A, groups, launch_block(c, split, lds), lds, split,
                                   c->stream));
         HIP_TRY(hipEventRecord(c->ev1, c->stream));
         HIP_TRY(hipMemcpyAsync(records, d_records, np * 72, hipMemcpyDeviceToHost, c->stream));

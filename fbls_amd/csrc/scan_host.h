@@ -57,8 +57,8 @@ int check_periods(const Work& x, double* smax);
 // w[i] (nw entries), the launch's LDS bytes and workgroup count.
 int segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, const Shared& sh, bool split,
                    SegShape* shapes, size_t* lds_max, uint32_t* groups);
-// Workgroup size of a launch of `groups` workgroups.
-uint32_t launch_block(const fbls_ctx* c, bool split, uint32_t groups);
+// Workgroup size of a launch whose workgroups hold `lds` bytes of LDS.
+uint32_t launch_block(const fbls_ctx* c, bool split, size_t lds);
 // Every Segment field that does not point into the launch's staging
 // (periods, scales, dur, rho, outputs, group0 and index are the caller's).
 void segment_fields(const fbls_ctx* c, const Work& x, const SegShape& g, bool split,

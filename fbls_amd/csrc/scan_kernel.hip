@@ -1184,6 +1184,7 @@ size_t split_lds_bytes(uint32_t nb, uint32_t range_bins, uint32_t halo_bins,
 
 hipError_t scan_configure(size_t max_lds) {
   const void* fns[] = {(const void*)fused_scan<256>, (const void*)fused_scan<512>,
+                       (const void*)fused_scan<1024>,
                        (const void*)fused_scan_split<256>,
                        (const void*)fused_scan_split<512>,
                        (const void*)fused_scan_split<1024>};
@@ -1206,7 +1207,9 @@ hipError_t scan_launch(const LaunchArgs& args, uint32_t groups, uint32_t block,
     else
       hipLaunchKernelGGL(fused_scan_split<256>, dim3(groups), dim3(256), lds, stream, args);
   } else {
-    if (block == 512)
+    if (block == 1024)
+      hipLaunchKernelGGL(fused_scan<1024>, dim3(groups), dim3(1024), lds, stream, args);
+    else if (block == 512)
       hipLaunchKernelGGL(fused_scan<512>, dim3(groups), dim3(512), lds, stream, args);
     else
       hipLaunchKernelGGL(fused_scan<256>, dim3(groups), dim3(256), lds, stream, args);

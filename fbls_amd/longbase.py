@@ -146,6 +146,7 @@ class LongBatch:
         self.groups = groups_of(self.states)
         # full scan (warm the staging buffers on the first chunk first)
         signal_search_batch(gpu, self.devs[:2], self.d, self.rhos[:2], self.cfg)
+        gpu.launch_totals(reset=True)
         t0 = time.perf_counter()
         best, templates, periods = [], 0, 0
         for c0 in range(0, len(self.devs), scan_chunk):
@@ -158,6 +159,7 @@ class LongBatch:
                 log(f"full scan {min(len(self.devs), c0 + scan_chunk)} of {len(self.devs)} "
                     f"stars ({time.perf_counter() - t0:.1f} s)")
         self.t_full = time.perf_counter() - t0
+        self.full_totals = gpu.launch_totals(reset=True)
         self.templates_full, self.periods_full = templates, periods
         self.best_full = best
         self.rf = PR.recovered(best, self.injected)
@@ -250,6 +252,10 @@ class LongBatch:
             "full_scan_stars_per_hour": len(self.states) * 3600.0 / self.t_full,
             "full_scan_trial_evals_per_s": self.templates_full / self.t_full,
             "full_scan_periods": self.periods_full,
+            "full_scan_kernel_seconds": self.full_totals["seconds"],
+            "full_scan_launches": self.full_totals["launches"],
+            "full_scan_fold_pairs_per_kernel_s": (self.full_totals["fold_pairs"]
+                                                  / self.full_totals["seconds"]),
         }
 
     def close(self):

@@ -90,7 +90,7 @@ void fbls_ctx_destroy(fbls_ctx* ctx);
 /* `Gpu::memory` (scan.rs:80-90) */
 int fbls_mem_info(fbls_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 /* Launch shape override for tuning: LDS bytes per workgroup (0 = auto),
- * workgroup size (256 or 512, 0 = auto) and fold mode (0 = auto:
+ * workgroup size (256, 512 or 1024, 0 = auto) and fold mode (0 = auto:
  * run-length pre-aggregation in registers, and a family whose fixed-point
  * weights are all equal folds no counts, each bin's count being its weight
  * sum divided by that weight; 1 = run-length, counts always folded; 2 = one
@@ -110,8 +110,8 @@ typedef struct {
 int fbls_ctx_launch_totals(fbls_ctx* ctx, fbls_launch_totals* out, int reset);
 
 /* Launch-shape autotuner: times fbls_signal_search of `fam` under each
- * candidate (LDS budget per workgroup 16-96 KiB, which sets the periods per
- * workgroup, x workgroup size 256/512; best of `repeats` after a warm-up),
+ * candidate (LDS budget per workgroup 16-160 KiB, which sets the periods per
+ * workgroup, x workgroup size 256/512/1024; best of `repeats` after a warm-up),
  * fails with FBLS_E_HIP if any shape changes the rows, and keeps the
  * fastest on the context (as fbls_ctx_set_launch).  table (capacity
  * table_cap, may be NULL) receives every measured shape. */

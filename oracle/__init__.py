@@ -138,6 +138,11 @@ class PrepCfg(ctypes.Structure):
                                   "window_max", "clip_sigma", "edge_trim")]
 
 
+def set_threads(n):
+    """Worker threads of the oracle's parallel loops (0: the OpenMP default)."""
+    lib().orc_set_threads(int(n))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -145,6 +150,8 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_durations_days.restype = _sz
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_period_grid_count.restype = _sz
         L.orc_period_grid_count.argtypes = [_d, _d, _d, _d, _d]
         L.orc_period_grid.restype = None

@@ -8,6 +8,7 @@
 #include "fbls_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -40,6 +41,11 @@ size_t orc_period_grid_count(double minimum, double maximum, double baseline,
 }
 
 /* grid.rs:14-27 */
+/* Worker threads of the oracle's parallel loops (0: the OpenMP default). */
+void orc_set_threads(int n) {
+    omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+}
+
 void orc_period_grid(double minimum, double maximum, double baseline,
                      double minimum_duration, double oversample, double* out,
                      size_t count) {

@@ -43,6 +43,7 @@ size_t orc_durations_days(const double* hours, size_t n, double* out_days);
 /* grid.rs:5-27 */
 size_t orc_period_grid_count(double minimum, double maximum, double baseline,
                              double minimum_duration, double oversample);
+void orc_set_threads(int n);
 void orc_period_grid(double minimum, double maximum, double baseline,
                      double minimum_duration, double oversample, double* out,
                      size_t count);

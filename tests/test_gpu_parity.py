@@ -141,7 +141,8 @@ def test_signal_search_single_sector_star(gpu):
     assert abs(best[0] - 3.7) < 0.01 and best[3] > 7.1
 
 
-@pytest.mark.parametrize("launch", [(0, 256, 2), (32768, 512, 1), (160 * 1024, 256, 1)])
+@pytest.mark.parametrize("launch", [(0, 256, 2), (32768, 512, 1), (160 * 1024, 256, 1),
+                                    (0, 1024, 0), (128 * 1024, 1024, 0)])
 def test_launch_shape_does_not_change_results(gpu, launch):
     """LDS budget, workgroup size and fold mode change the schedule only."""
     cfg = F.SearchConfig(period_min=0.5, period_max=13.7)
