@@ -438,7 +438,10 @@ def main():
             pending = submit(spans[k + 1]) if k + 1 < len(spans) else None
             yield idx, [r[0] for r in res], [r[1] for r in res]
 
-    seed_train, seed_val, seed_test = 4_000_000, 4_100_000, 4_200_000
+    # disjoint seed ranges: training 4,000,000 + i (i < 200,000), held-out
+    # 4,200,000 + i (i < 20,000), validation 4,250,000 + i
+    seed_train, seed_val, seed_test = 4_000_000, 4_250_000, 4_200_000
+    assert args.train <= seed_test - seed_train and args.heldout <= seed_val - seed_test
     t0 = time.perf_counter()
     prior_path = os.path.join(HERE, "data", "proposal_report.json")
     if args.reselect:
