@@ -291,8 +291,7 @@ def long_baseline(gpu, data):
     from fbls_amd import proposal as PR
 
     net, pc, extra = PR.load(gpu, LB.WEIGHTS)
-    op = (float(extra["tau"]), int(extra["topk"]),
-          tuple(float(v) for v in np.atleast_1d(extra["fallback"])))
+    op = LB.operating_point(extra)
     cfg = LB.search_config()
     curves = [d[0] for d in data]
     states = prepare(curves, cfg)
@@ -306,7 +305,8 @@ def long_baseline(gpu, data):
                     "sectors in two seasons around a 365-d gap (up to "
                     f"{summary['points_max']} points), 30% transits + 5% eclipsing "
                     "binaries; periods 0.5-100 d",
-        "operating_point": {"tau": op[0], "top_k_bins": op[1], "fallback_band": list(op[2])},
+        "operating_point": {"tau": op[0], "top_k_bins": op[1], "fallback_band": list(op[2]),
+                            "raw_top_k_bins": op[3]},
         "recall_vs_full_scan": r["recall"],
         "full_scan_recovered": r["full_recovered"],
         "injections": summary["transits"] + summary["eclipsing_binaries"],

@@ -264,7 +264,7 @@ def lib():
         L.fbls_period_grid.argtypes = [_d, _d, _d, _d, _d, _P(_d), _sz]
         L.fbls_durations_days.restype = _sz
         L.fbls_durations_days.argtypes = [_P(_d), _sz, _P(_d)]
-        L.fbls_quantize.argtypes = [_P(_d), _sz, _P(_i32)]
+        L.fbls_quantize.argtypes = [_P(_d), _sz, _P(_i32), _P(ctypes.c_int)]
         L.fbls_ctx_create.argtypes = [ctypes.c_int, _P(_vp)]
         L.fbls_ctx_destroy.argtypes = [_vp]
         L.fbls_ctx_destroy.restype = None
@@ -335,7 +335,7 @@ def lib():
         L.fbls_coarse_features.argtypes = [_vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz,
                                            _u32, _u32, _P(_d)]
         L.fbls_propose.argtypes = [_vp, _vp, _P(_vp), _sz, _P(_d), _sz, _P(_d), _sz, _u32,
-                                   _u32, _P(ctypes.c_float)]
+                                   _u32, _P(ctypes.c_float), _P(ctypes.c_float)]
         L.fbls_signal_search_bins.argtypes = [
             _vp, _P(_vp), _sz, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
             _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8), _sz, _P(_d)]
@@ -492,10 +492,12 @@ def bank_batches(np_, nb, period_batch=4096, max_fold_mib=1024.0):
 
 
 def quantize(x):
+    """(words, e, block) of the device's fixed-point fields (fbls_quantize)."""
     x = _f64(x)
     q = np.empty(len(x), dtype=np.int32)
-    e = lib().fbls_quantize(_dp(x), len(x), q.ctypes.data_as(_P(_i32)))
-    return q, e
+    block = ctypes.c_int(0)
+    e = lib().fbls_quantize(_dp(x), len(x), q.ctypes.data_as(_P(_i32)), ctypes.byref(block))
+    return q, e, bool(block.value)
 
 
 @dataclasses.dataclass
@@ -1152,16 +1154,19 @@ def coarse_inputs(gpu, devs, periods, durations, nb_min=16, nb_max=1024):
     return out
 
 
-def propose(gpu, net, devs, periods, durations, nb_min=16, nb_max=1024):
+def propose(gpu, net, devs, periods, durations, nb_min=16, nb_max=1024, peaks=False):
     """fbls_propose: the network's logits [star, period bin] from the coarse
-    periodogram computed and consumed on the device."""
+    periodogram computed and consumed on the device; with peaks, also the
+    periodogram's largest SNR over the durations per bin (logits, peaks)."""
     periods, durations = _f64(periods), _f64(durations)
     hs = (_vp * len(devs))(*[d._h.value for d in devs])
     out = np.empty((len(devs), len(periods)), dtype=np.float32)
+    pk = np.empty((len(devs), len(periods)), dtype=np.float32) if peaks else None
     _check(lib().fbls_propose(gpu._h, net._h, hs, len(devs), _dp(periods), len(periods),
                               _dp(durations), len(durations), nb_min, nb_max,
-                              out.ctypes.data_as(_P(ctypes.c_float))))
-    return out
+                              out.ctypes.data_as(_P(ctypes.c_float)),
+                              pk.ctypes.data_as(_P(ctypes.c_float)) if peaks else None))
+    return (out, pk) if peaks else out
 
 
 def coarse_features(gpu, devs, periods, durations, nb_min=16, nb_max=1024):

@@ -70,6 +70,7 @@ extern "C" int fbls_single_event_search(fbls_ctx* c, const fbls_family* fam,
     fbls::EventPrefixArgs P;
     P.t = fam->t_dev;
     P.vw = fam->vw_dev;
+    P.fx = fam->fx;
     P.n = fam->n;
     P.nb = nb;
     P.scale = scale;

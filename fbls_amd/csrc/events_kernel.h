@@ -12,6 +12,7 @@ struct EventPrefixArgs {
   const double* t;    // n ascending origin-relative times
   const int32_t* vw;  // n x {v, w} fixed point
   uint32_t n, nb;
+  uint32_t fx;        // fixed-point modes: bit 0 v, bit 1 w in block mode
   double scale;       // nb / trial period
   double sv, sw;      // 2^-ev, 2^-ew
   double* cv;         // nb + 1: exclusive prefix of v over bins

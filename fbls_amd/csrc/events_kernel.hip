@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include "events_kernel.h"
+#include "fixed_point.h"
 
 namespace fbls {
 
@@ -56,8 +57,8 @@ __global__ __launch_bounds__(kEventPrefixBlock) void event_prefix(EventPrefixArg
     lw[x] = aw;
     A.cn[x] = i;
     while (i < A.n && cell(A.t[i], A.scale) == x) {
-      av += A.vw[2 * (size_t)i];
-      aw += A.vw[2 * (size_t)i + 1];
+      av += fixed_ivalue(A.vw[2 * (size_t)i], A.fx & 1u);
+      aw += fixed_ivalue(A.vw[2 * (size_t)i + 1], A.fx & 2u);
       i++;
     }
   }

@@ -239,6 +239,7 @@ extern "C" int fbls_compare(fbls_ctx* c, const fbls_family* f, double period,
     fbls::EvArgs A{};
     A.t = f->t_dev;
     A.vw = f->vw_dev;
+    A.fx = f->fx;
     A.n = n;
     A.nb = (uint32_t)nb;
     A.nb2 = (uint32_t)nb2;

@@ -16,6 +16,7 @@ struct EvArgs {
   const double* t;        // n origin-relative times
   const int32_t* vw;      // n x {v, w} fixed point
   uint32_t n;
+  uint32_t fx;            // fixed-point modes: bit 0 v, bit 1 w in block mode
   uint32_t nb, nb2, nd, ns, layers, tiles;
   uint32_t pad_;
   double sv, sw;          // 2^-ev, 2^-ew

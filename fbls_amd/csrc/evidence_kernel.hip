@@ -42,6 +42,7 @@
 #include <stdint.h>
 
 #include "evidence_kernel.h"
+#include "fixed_point.h"
 #include "fmath.h"
 
 namespace fbls {
@@ -137,8 +138,8 @@ __global__ __launch_bounds__(256) void ev_fold(EvArgs A) {
       sv = sw = 0;
       cnt = 0;
     }
-    sv += A.vw[2 * i];
-    sw += A.vw[2 * i + 1];
+    sv += fixed_ivalue(A.vw[2 * i], A.fx & 1u);
+    sw += fixed_ivalue(A.vw[2 * i + 1], A.fx & 2u);
     cnt++;
   }
   if (cnt) fold_flush(A, cur, sv, sw, cnt);

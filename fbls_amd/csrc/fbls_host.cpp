@@ -45,6 +45,26 @@ double fbls::host::now_s() {
 
 extern "C" const char* fbls_last_error(void) { return g_last_error.c_str(); }
 
+void fbls::host::parallel_ranges(size_t m, size_t per,
+                                 const std::function<void(size_t, size_t)>& work) {
+  const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), (m + per - 1) / std::max<size_t>(per, 1));
+  if (nt <= 1) {
+    work(0, m);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(nt);
+  size_t started = 0;
+  try {
+    for (; started < nt; started++)
+      pool.emplace_back(work, m * started / nt, m * (started + 1) / nt);
+  } catch (const std::system_error&) {
+  }
+  for (std::thread& th : pool) th.join();
+  if (started < nt) work(m * started / nt, m);
+}
+
 // ---------------------------------------------------------------- context
 
 extern "C" int fbls_ctx_create(int device, fbls_ctx** out) {
@@ -63,6 +83,7 @@ extern "C" int fbls_ctx_create(int device, fbls_ctx** out) {
         prop.maxSharedMemoryPerMultiProcessor > 0)
       c->max_lds = std::min<size_t>(kMaxLds, prop.maxSharedMemoryPerMultiProcessor);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess ||
         hipEventCreate(&c->ev1) != hipSuccess) {
       delete c;
@@ -84,15 +105,18 @@ extern "C" void fbls_ctx_destroy(fbls_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->up_stream) (void)hipStreamSynchronize(c->up_stream);
   if (c->ev_exec) (void)hipGraphExecDestroy(c->ev_exec);
   c->dev_in.release();
   c->dev_out.release();
   c->dev_ev.release();
   c->pin_in.release();
   c->pin_out.release();
+  c->pin_up.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->up_stream) (void)hipStreamDestroy(c->up_stream);
   delete c;
 }
 
@@ -143,9 +167,11 @@ extern "C" int fbls_ctx_launch_totals(fbls_ctx* c, fbls_launch_totals* out,
 
 // ----------------------------------------------------------------- family
 
-extern "C" int fbls_quantize(const double* x, size_t n, int32_t* q) {
+extern "C" int fbls_quantize(const double* x, size_t n, int32_t* q, int* block) {
   return fbls::host::guard([&]() -> int {
-    return fbls::quantize(x, n, q);
+    const fbls::Fixed f = fbls::quantize(x, n, q);
+    if (block) *block = f.block ? 1 : 0;
+    return f.e;
   });
 }
 
@@ -179,11 +205,15 @@ int prepare_family(fbls_ctx* c, const double* t, const double* v,
   f->t_last = t[n - 1];
   f->vq.resize(n);
   f->wq.resize(n);
-  f->ev = fbls::quantize(v, n, f->vq.data());
-  f->ew = fbls::quantize(w, n, f->wq.data());
+  const fbls::Fixed fv = fbls::quantize(v, n, f->vq.data());
+  const fbls::Fixed fw = fbls::quantize(w, n, f->wq.data());
+  f->ev = fv.e;
+  f->ew = fw.e;
+  f->fx = (fv.block ? 1u : 0u) | (fw.block ? 2u : 0u);
   f->sorted = true;
   for (size_t i = 1; i < n; i++) f->sorted = f->sorted && t[i] >= t[i - 1];
-  f->w_uniform = f->wq[0] > 0 ? f->wq[0] : 0;
+  // equal weights are never in block mode (max = mean)
+  f->w_uniform = f->wq[0] > 0 && !fw.block ? f->wq[0] : 0;
   for (size_t i = 1; i < n && f->w_uniform; i++)
     if (f->wq[i] != f->w_uniform) f->w_uniform = 0;
   return FBLS_OK;
@@ -204,30 +234,43 @@ void write_family_image(const fbls_family* f, const double* t, unsigned char* ds
   std::fill(vw + 2 * (size_t)f->n, vw + 2 * (size_t)f->n_pad, 0);
 }
 
-// Uploads the families (already prepared) into one arena.
+// Uploads the families (already prepared) into one arena: the device image
+// is written into pinned staging by up to 16 threads and copied on the
+// context's upload stream.
 int upload_arena(fbls_ctx* c, std::vector<fbls_family*>& fams,
                  const std::vector<const double*>& times) {
-  size_t total = 0;
-  for (fbls_family* f : fams) total += family_bytes(f);
+  std::vector<size_t> at(fams.size() + 1, 0);
+  for (size_t k = 0; k < fams.size(); k++) at[k + 1] = at[k] + family_bytes(fams[k]);
+  const size_t total = at.back();
   auto arena = std::make_shared<DevArena>();
   arena->device = c->device;
   HIP_TRY(hipSetDevice(c->device));
   if (int st = arena->buf.ensure(total)) return st;
-  std::vector<unsigned char> image(total);
-  size_t at = 0;
-  for (size_t k = 0; k < fams.size(); k++) {
-    write_family_image(fams[k], times[k], image.data() + at);
-    at += family_bytes(fams[k]);
+  {
+    // in pieces of at most 256 MiB of staging (a larger family alone)
+    constexpr size_t kPiece = (size_t)256 << 20;
+    std::lock_guard<std::mutex> hold(c->up_mutex);
+    for (size_t k0 = 0; k0 < fams.size();) {
+      size_t k1 = k0 + 1;
+      while (k1 < fams.size() && at[k1 + 1] - at[k0] <= kPiece) k1++;
+      const size_t bytes = at[k1] - at[k0];
+      if (int st = c->pin_up.ensure(bytes)) return st;
+      unsigned char* img = (unsigned char*)c->pin_up.ptr;
+      fbls::host::parallel_ranges(k1 - k0, 4, [&](size_t a, size_t e) {
+        for (size_t k = k0 + a; k < k0 + e; k++)
+          write_family_image(fams[k], times[k], img + (at[k] - at[k0]));
+      });
+      HIP_TRY(hipMemcpyAsync((unsigned char*)arena->buf.ptr + at[k0], img, bytes,
+                             hipMemcpyHostToDevice, c->up_stream));
+      HIP_TRY(hipStreamSynchronize(c->up_stream));
+      k0 = k1;
+    }
   }
-  if (hipMemcpy(arena->buf.ptr, image.data(), total, hipMemcpyHostToDevice) != hipSuccess)
-    return fail(FBLS_E_HIP, "family upload failed");
-  at = 0;
-  for (fbls_family* f : fams) {
-    unsigned char* base = (unsigned char*)arena->buf.ptr + at;
-    f->arena = arena;
-    f->t_dev = (const double*)base;
-    f->vw_dev = (const int32_t*)(base + (size_t)f->n_pad * 8);
-    at += family_bytes(f);
+  for (size_t k = 0; k < fams.size(); k++) {
+    unsigned char* base = (unsigned char*)arena->buf.ptr + at[k];
+    fams[k]->arena = arena;
+    fams[k]->t_dev = (const double*)base;
+    fams[k]->vw_dev = (const int32_t*)(base + (size_t)fams[k]->n_pad * 8);
   }
   return FBLS_OK;
 }
@@ -270,17 +313,35 @@ extern "C" int fbls_family_upload_packed(fbls_ctx* c, size_t nfam,
     times.reserve(nfam);
     for (size_t k = 0; k < nfam; k++) {
       if (offsets[k + 1] < offsets[k]) return fail(FBLS_E_ARG, "offsets must be nondecreasing");
-      const size_t a = offsets[k], n = offsets[k + 1] - offsets[k];
       owned.emplace_back(new fbls_family());
       fams.push_back(owned.back().get());
-      times.push_back(t + a);
-      if (int st = prepare_family(c, t + a, v + a, w + a, n, baseline_norm[k], origin[k],
-                                  fams.back())) {
-        char msg[600];
-        std::snprintf(msg, sizeof(msg), "family %zu: %s", k, fbls_last_error());
-        return fail(st, "%s", msg);
-      }
+      times.push_back(t + offsets[k]);
     }
+    // the families' checks and fixed-point fields on up to 16 threads
+    std::vector<int> status(nfam, FBLS_OK);
+    std::vector<std::string> message(nfam);
+    fbls::host::parallel_ranges(nfam, 8, [&](size_t lo, size_t hi) {
+      for (size_t k = lo; k < hi; k++) {
+        const size_t a = offsets[k], n = offsets[k + 1] - offsets[k];
+        try {
+          status[k] = prepare_family(c, t + a, v + a, w + a, n, baseline_norm[k], origin[k],
+                                     fams[k]);
+          if (status[k]) message[k] = fbls_last_error();
+        } catch (const std::bad_alloc&) {
+          status[k] = FBLS_E_OOM;
+          message[k] = "host memory allocation failed";
+        } catch (const std::exception& ex) {
+          status[k] = FBLS_E_ARG;
+          try {
+            message[k] = ex.what();
+          } catch (...) {
+          }
+        }
+        if (status[k]) return;
+      }
+    });
+    for (size_t k = 0; k < nfam; k++)
+      if (status[k]) return fail(status[k], "family %zu: %s", k, message[k].c_str());
     if (!fams.empty())
       if (int st = upload_arena(c, fams, times)) return st;
     for (size_t k = 0; k < nfam; k++) out[k] = owned[k].release();
@@ -298,8 +359,8 @@ extern "C" int fbls_family_fields(const fbls_family* f, double* v_out,
   return fbls::host::guard([&]() -> int {
     if (!f) return fail(FBLS_E_ARG, "null family");
     for (uint32_t i = 0; i < f->n; i++) {
-      if (v_out) v_out[i] = std::ldexp((double)f->vq[i], -f->ev);
-      if (w_out) w_out[i] = std::ldexp((double)f->wq[i], -f->ew);
+      if (v_out) v_out[i] = std::ldexp(fbls::fixed_value(f->vq[i], f->fx & 1), -f->ev);
+      if (w_out) w_out[i] = std::ldexp(fbls::fixed_value(f->wq[i], f->fx & 2), -f->ew);
     }
     if (ev) *ev = f->ev;
     if (ew) *ew = f->ew;
@@ -394,10 +455,14 @@ int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, cons
       if (sh.ingress[0] != 0.0)
         return fail(FBLS_E_ARG, "phase bins beyond the LDS need ingress[0] = 0");
       const uint32_t halo = longest_template(w[i]);
-      // widest multiple of 128 bins whose window fits the LDS
+      // widest multiple of 128 bins whose window fits the budget: by default
+      // the whole LDS (one workgroup per CU).  Measured on MI355X at
+      // 9000-27000 bins: windows of half the LDS (two workgroups per CU)
+      // are 0-26 % slower with 512 threads, the extra ranges costing more
+      // than the overlap gains.
+      const size_t budget = c->lds_override ? c->lds_override : c->max_lds;
       uint32_t B = 128;
-      while (fbls::split_lds_bytes(w[i].nb, B + 128, halo, w[i].nd, ns, layers) <=
-                 c->max_lds &&
+      while (fbls::split_lds_bytes(w[i].nb, B + 128, halo, w[i].nd, ns, layers) <= budget &&
              B < w[i].nb)
         B += 128;
       // a window of B + halo bins holds each bin at most once (the fold
@@ -445,10 +510,9 @@ int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, cons
 }
 
 uint32_t fbls::host::launch_block(const fbls_ctx* c, bool split, size_t lds) {
-  // The range-split kernel holds one period's window in up to the whole
-  // LDS, one workgroup per CU: 512 threads (8 waves, two per SIMD, with no
-  // register spills) hide its loads.
-  if (split) return 512;
+  // The range-split kernel: 512 threads (its 155 VGPRs allow three waves
+  // per SIMD; 1024 threads would spill), or 256 when asked.
+  if (split) return c->block_override == 256 ? 256 : 512;
   if (c->block_override) return c->block_override;
   // 16 waves per CU (four per SIMD, the occupancy of the kernel's 128
   // VGPRs): two 512-thread workgroups when two fit the LDS, else one of
@@ -468,6 +532,7 @@ void fbls::host::segment_fields(const fbls_ctx* c, const Work& x, const SegShape
   S->vw = f->vw_dev;
   S->sv = std::ldexp(1.0, -f->ev);
   S->sw = std::ldexp(1.0, -f->ew);
+  S->fx = f->fx;
   S->C = f->C;
   S->invC = 1.0 / f->C;
   S->n = f->n;

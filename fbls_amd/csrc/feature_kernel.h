@@ -11,6 +11,7 @@ struct FeatureStar {
   const int32_t* vw;  // n x {v, w} fixed point
   uint32_t n;
   uint32_t sorted;    // 1: times ascend
+  uint32_t fx;        // fixed-point modes: bit 0 v, bit 1 w in block mode
   double sv, sw;      // fixed point -> value
   double C;           // baseline norm
 };
@@ -23,6 +24,7 @@ struct FeatureArgs {
   const double* durations;  // ng
   double* out;              // stars x np x ng best SNRs, or null
   float* inputs;            // stars x np x ng network inputs, or null
+  float* peaks;             // stars x np largest SNR over the durations, or null
   uint32_t nb_max;          // largest nb
   uint32_t G;               // most periods in one group (<= 4)
   uint32_t narrow;          // 1: every t * nb / P < 2^24 (32-bit cells, float modulo)

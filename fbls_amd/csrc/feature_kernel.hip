@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include "feature_kernel.h"
+#include "fixed_point.h"
 
 namespace fbls {
 
@@ -152,8 +153,8 @@ __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
     for (uint32_t r = 0; r < 8; r++) {
       const uint32_t i = r < cnt ? i0 + r : i0;
       tt[r] = S.t[i];
-      vv[r] = (double)S.vw[2 * (size_t)i];
-      ww[r] = (double)S.vw[2 * (size_t)i + 1];
+      vv[r] = fixed_dvalue(S.vw[2 * (size_t)i], S.fx & 1u);
+      ww[r] = fixed_dvalue(S.vw[2 * (size_t)i + 1], S.fx & 2u);
     }
 #pragma unroll
     for (uint32_t g = 0; g < 4; g++) {
@@ -285,7 +286,9 @@ __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
     }
   }
   __syncthreads();
-  if (tid < gcount * NG && tid % NG < A.ng) {
+  double m = -kInf;
+  const bool mine = tid < gcount * NG && tid % NG < A.ng;
+  if (mine) {
     const uint32_t g = tid / NG, d = tid % NG;
     double bk = red_key[0][tid];
     uint32_t b = red_j[0][tid];
@@ -297,7 +300,6 @@ __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
         b = oj;
       }
     }
-    double m = -kInf;
     if (b != 0xffffffffu) {
       const uint32_t nb = A.nb[p0 + g];
       const double scale = (double)nb / A.periods[p0 + g];
@@ -311,6 +313,16 @@ __global__ __launch_bounds__(kBlock) void coarse_features(FeatureArgs A) {
     if (A.inputs) {  // the network input: clip(SNR, -5, 40) / 8, 0 when none
       const double x = m > -kInf ? m : 0.0;
       A.inputs[at] = (float)(fmin(fmax(x, -5.0), 40.0) / 8.0);
+    }
+  }
+  if (A.peaks) {  // per period, the largest SNR over the durations (unclipped)
+    __syncthreads();  // every read of red_key above is done
+    if (tid < gcount * NG) red_key[0][tid] = mine ? m : -kInf;
+    __syncthreads();
+    if (tid < gcount) {
+      double mx = -kInf;
+      for (uint32_t d = 0; d < A.ng; d++) mx = fmax(mx, red_key[0][tid * NG + d]);
+      A.peaks[(size_t)star * A.np + p0 + tid] = mx > -kInf ? (float)mx : -INFINITY;
     }
   }
 }

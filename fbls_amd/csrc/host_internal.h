@@ -6,7 +6,9 @@
 #include <cstddef>
 #include <cstdint>
 #include <exception>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -32,6 +34,10 @@ int guard(Body&& body) noexcept {
   }
 }
 double now_s();
+// work(a, e) over [0, m) split into contiguous ranges on up to 16 host
+// threads (about `per` items per thread at least); ranges whose thread
+// cannot start run on the caller's thread.  work must not throw.
+void parallel_ranges(size_t m, size_t per, const std::function<void(size_t, size_t)>& work);
 // The network's device input buffer for a B x L batch (channels-last
 // float32, in_channels per bin), and the fused inference from it into host
 // logits (cnn_host.cpp).
@@ -127,6 +133,11 @@ struct fbls_ctx {
   uint64_t graph_launches = 0;
   fbls::host::DevBuf dev_in, dev_out, dev_ev;
   fbls::host::HostBuf pin_in, pin_out;
+  // family uploads: their own stream (they overlap the scans of the main
+  // stream) and pinned staging, one upload at a time
+  hipStream_t up_stream = nullptr;
+  fbls::host::HostBuf pin_up;
+  std::mutex up_mutex;
 };
 
 struct fbls_family {
@@ -137,6 +148,7 @@ struct fbls_family {
   uint32_t n = 0, n_pad = 0;
   double C = 0.0, origin = 0.0, t_max = 0.0, t_last = 0.0;
   int ev = 0, ew = 0;
+  uint32_t fx = 0;       // bit 0: v, bit 1: w in block mode (fixed_value)
   bool sorted = false;   // times ascend
   int32_t w_uniform = 0; // > 0: every fixed-point weight equals it
   std::vector<int32_t> vq, wq;

@@ -98,8 +98,8 @@ int build_fam(fbls_ctx* c, Curve curve, const std::vector<double>& durs,
   out->v.resize(m);
   out->w.resize(m);
   for (size_t i = 0; i < m; i++) {
-    out->v[i] = std::ldexp((double)dev->vq[i], -dev->ev);
-    out->w[i] = std::ldexp((double)dev->wq[i], -dev->ew);
+    out->v[i] = std::ldexp(fbls::fixed_value(dev->vq[i], dev->fx & 1), -dev->ev);
+    out->w[i] = std::ldexp(fbls::fixed_value(dev->wq[i], dev->fx & 2), -dev->ew);
   }
   out->rho = rho;
   out->C = C;
@@ -222,6 +222,7 @@ struct Selector {
       fbls::RescoreArgs ra;
       ra.t = fam->dev->t_dev;
       ra.vw = fam->dev->vw_dev;
+      ra.fx = fam->dev->fx;
       ra.excluded = (const uint8_t*)d_ex_fam.ptr;
       ra.n = (uint32_t)n;
       ra.origin = fam->origin;

@@ -11,17 +11,46 @@
 
 namespace fbls {
 
-int quantize(const double* x, size_t n, int32_t* q) {
-  double m = 0.0;
-  for (size_t i = 0; i < n; i++) m = std::max(m, std::fabs(x[i]));
-  int e = 0;
-  if (m > 0.0) {
-    int ex = 0;
-    std::frexp(m, &ex);
-    e = 30 - ex;
+Fixed quantize(const double* x, size_t n, int32_t* q) {
+  Fixed f;
+  double m = 0.0, sum = 0.0;
+  for (size_t i = 0; i < n; i++) {
+    m = std::max(m, std::fabs(x[i]));
+    sum += std::fabs(x[i]);
   }
-  for (size_t i = 0; i < n; i++) q[i] = (int32_t)std::nearbyint(std::ldexp(x[i], e));
-  return e;
+  if (!(m > 0.0)) {
+    for (size_t i = 0; i < n; i++) q[i] = 0;
+    return f;
+  }
+  int ex = 0;
+  f.block = m > 16.0 * (sum / (double)n);
+  if (!f.block) {
+    std::frexp(m, &ex);  // m = g * 2^ex, g in [0.5, 1)
+    f.e = 30 - ex;
+    for (size_t i = 0; i < n; i++) q[i] = (int32_t)std::nearbyint(std::ldexp(x[i], f.e));
+    return f;
+  }
+  // sum |x| * 2^e < 2^52; each value rounds to an integer r (|r| < 2^52),
+  // then to 26 significant bits when |r| >= 2^26 (mantissa in [2^25, 2^26))
+  std::frexp(sum, &ex);
+  f.e = 52 - ex;
+  for (size_t i = 0; i < n; i++) {
+    const double r = std::nearbyint(std::ldexp(x[i], f.e));
+    int64_t mant = (int64_t)r;
+    int shift = 0;
+    if (std::fabs(r) >= 67108864.0) {  // 2^26
+      int k = 0;
+      std::frexp(std::fabs(r), &k);  // |r| = g * 2^k, k >= 27
+      shift = k - 26;
+      mant = (int64_t)std::nearbyint(std::ldexp(r, -shift));
+      if (mant == 67108864 || mant == -67108864) {  // rounded up to 2^26
+        mant /= 2;
+        shift++;
+      }
+    }
+    q[i] = (int32_t)(mant * 32 + shift);
+  }
+  return f;
 }
 
 size_t period_grid_count(double minimum, double maximum, double baseline,

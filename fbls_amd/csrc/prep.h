@@ -14,8 +14,26 @@
 
 namespace fbls {
 
-// q = rint(x * 2^e) with e = 30 - exponent(max |x|), so |q| <= 2^30; returns e.
-int quantize(const double* x, size_t n, int32_t* q);
+// Fixed-point fields.  Every word stands for an integer in units of 2^-e,
+// and the integers of one array sum in absolute value to below 2^53, so any
+// sum of them is exact in FP64 whatever the order.
+//  plain: word = rint(x * 2^e), e = 30 - exponent(max |x|), |word| <= 2^30;
+//    absolute error <= 2^-30 max |x| per value.
+//  block: word = mantissa * 32 + shift, value = mantissa * 2^shift, a 27-bit
+//    signed mantissa over a 5-bit shift, e = 52 - exponent(sum |x|);
+//    relative error <= 2^-26 per value (absolute below 2^-52 sum |x| for
+//    the values under 2^26 units).
+// plain when max |x| <= 16 mean |x|, block otherwise, so that in both
+// modes |error| <= 2^-26 max(|x|, mean |x|) for every value.
+struct Fixed {
+  int e = 0;
+  bool block = false;
+};
+Fixed quantize(const double* x, size_t n, int32_t* q);
+// The integer a word stands for (units of 2^-e).
+inline double fixed_value(int32_t word, bool block) {
+  return block ? (double)(word >> 5) * (double)(1ull << (word & 31)) : (double)word;
+}
 
 // grid.rs:5-27
 size_t period_grid_count(double minimum, double maximum, double baseline,

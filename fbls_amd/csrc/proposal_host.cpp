@@ -17,10 +17,12 @@ namespace {
 
 // The periodogram into host `out` (doubles) or host `inputs` (network
 // inputs), or, with device_inputs, the network inputs left in that device
-// buffer (no copy back).
+// buffer (no copy back).  peaks (host, may be null): per (star, period) the
+// largest unclipped SNR over the durations, copied back on the stream (the
+// caller synchronizes it when the inputs stay on the device).
 int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const double* periods,
            size_t np, const double* durations, size_t ng, uint32_t nb_min, uint32_t nb_max,
-           double* out, float* inputs, float* device_inputs = nullptr) {
+           double* out, float* inputs, float* device_inputs = nullptr, float* peaks = nullptr) {
   if (!c || (nfam && !fams) || !periods || !durations || (!out && !inputs && !device_inputs))
     return fail(FBLS_E_ARG, "null argument");
   if (np == 0 || ng == 0 || nfam == 0) return FBLS_OK;
@@ -51,6 +53,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
     cell_max = std::max(cell_max, f->t_max * ((double)nb_max / pmin));
     stars[k].t = f->t_dev;
     stars[k].vw = f->vw_dev;
+    stars[k].fx = f->fx;
     stars[k].n = f->n;
     stars[k].sorted = f->sorted ? 1u : 0u;
     stars[k].sv = std::ldexp(1.0, -f->ev);
@@ -87,8 +90,10 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   const size_t o_gw = o_gs + (((ngrp + 1) * 4 + 255) & ~(size_t)255);
   const size_t o_out = o_gw + ((ngrp * 4 + 255) & ~(size_t)255);
   const size_t out_bytes = device_inputs ? 0 : nfam * np * ng * (out ? 8 : 4);
+  const size_t o_peaks = o_out + ((out_bytes + 255) & ~(size_t)255);
+  const size_t peak_bytes = peaks ? nfam * np * 4 : 0;
   HIP_TRY(hipSetDevice(c->device));
-  if (int st = c->dev_in.ensure(o_out + out_bytes)) return st;
+  if (int st = c->dev_in.ensure(o_peaks + peak_bytes)) return st;
   unsigned char* d = (unsigned char*)c->dev_in.ptr;
   hipStream_t s = c->stream;
   HIP_TRY(hipMemcpyAsync(d, stars.data(), b_stars, hipMemcpyHostToDevice, s));
@@ -106,6 +111,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   A.durations = (const double*)(d + o_dur);
   A.out = out ? (double*)(d + o_out) : nullptr;
   A.inputs = out ? nullptr : (device_inputs ? device_inputs : (float*)(d + o_out));
+  A.peaks = peaks ? (float*)(d + o_peaks) : nullptr;
   A.nb_max = *std::max_element(nb.begin(), nb.end());
   A.G = gmax;
   A.ngroups = (uint32_t)ngrp;
@@ -114,6 +120,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   A.lds_bytes = (uint32_t)lds_bytes;
   A.narrow = cell_max < 16777215.0 ? 1u : 0u;  // (nb <= nb_max, P >= pmin)
   HIP_TRY(fbls::feature_launch(A, (uint32_t)nfam, s));
+  if (peaks) HIP_TRY(hipMemcpyAsync(peaks, d + o_peaks, peak_bytes, hipMemcpyDeviceToHost, s));
   if (device_inputs) return FBLS_OK;  // stream-ordered before the network
   HIP_TRY(hipMemcpyAsync(out ? (void*)out : (void*)inputs, d + o_out, out_bytes,
                          hipMemcpyDeviceToHost, s));
@@ -144,7 +151,7 @@ extern "C" int fbls_coarse_inputs(fbls_ctx* c, const fbls_family* const* fams, s
 extern "C" int fbls_propose(fbls_ctx* c, fbls_cnn* net, const fbls_family* const* fams,
                             size_t nfam, const double* periods, size_t np,
                             const double* durations, size_t ng, uint32_t nb_min,
-                            uint32_t nb_max, float* logits) {
+                            uint32_t nb_max, float* logits, float* peaks) {
   return fbls::host::guard([&]() -> int {
     if (!c || !net || !logits) return fail(FBLS_E_ARG, "null argument");
     if (nfam == 0 || np == 0) return FBLS_OK;
@@ -160,10 +167,11 @@ extern "C" int fbls_propose(fbls_ctx* c, fbls_cnn* net, const fbls_family* const
       float* staged = nullptr;
       if (int st = fbls::host::cnn_stage_input(net, (uint32_t)m, (uint32_t)np, &staged)) return st;
       if (int st = coarse(c, fams + s0, m, periods, np, durations, ng, nb_min, nb_max, nullptr,
-                          nullptr, staged))
+                          nullptr, staged, peaks ? peaks + s0 * np : nullptr))
         return st;
       if (int st = fbls::host::cnn_run_staged(net, (uint32_t)m, (uint32_t)np, logits + s0 * np))
         return st;
+      HIP_TRY(hipStreamSynchronize(c->stream));  // the peaks' copy, on the same stream
     }
     return FBLS_OK;
   });

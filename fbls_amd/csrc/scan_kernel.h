@@ -32,6 +32,7 @@ struct Segment {
   uint32_t sorted;        // 1: times ascend (incremental phase bins)
   int32_t uniform_w;      // > 0: every fixed-point weight equals it; the fold
                           // skips the count atomics, each count being W / uniform_w
+  uint32_t fx;            // fixed-point modes: bit 0 v, bit 1 w in block mode
 };
 
 struct LaunchArgs {

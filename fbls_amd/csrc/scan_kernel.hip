@@ -34,6 +34,7 @@
 #include <stdint.h>
 
 #include "scan_kernel.h"
+#include "fixed_point.h"
 
 namespace fbls {
 
@@ -440,11 +441,21 @@ __device__ __forceinline__ void load_obs(const Segment& S, uint32_t base,
                                          double* tt, double* vv, double* ww) {
   gdouble* tp = (gdouble*)__builtin_assume_aligned(S.t + base, 64);
   gint* qp = (gint*)__builtin_assume_aligned(S.vw + 2 * (size_t)base, 64);
+  if (S.fx == 0) {  // plain words (a uniform branch: no cost to the common case)
 #pragma unroll
-  for (uint32_t r = 0; r < kObsPerLane; r++) {
-    tt[r] = tp[r];
-    vv[r] = (double)qp[2 * r];
-    ww[r] = (double)qp[2 * r + 1];
+    for (uint32_t r = 0; r < kObsPerLane; r++) {
+      tt[r] = tp[r];
+      vv[r] = (double)qp[2 * r];
+      ww[r] = (double)qp[2 * r + 1];
+    }
+  } else {
+    const bool bv = S.fx & 1u, bw = S.fx & 2u;
+#pragma unroll
+    for (uint32_t r = 0; r < kObsPerLane; r++) {
+      tt[r] = tp[r];
+      vv[r] = fixed_dvalue(qp[2 * r], bv);
+      ww[r] = fixed_dvalue(qp[2 * r + 1], bw);
+    }
   }
 }
 

@@ -10,6 +10,7 @@ struct RescoreArgs {
   const int32_t* vw;       // n x {v, w} fixed point
   const uint8_t* excluded; // n: 1 when the observation is excluded
   uint32_t n;
+  uint32_t fx;             // fixed-point modes: bit 0 v, bit 1 w in block mode
   double origin;
   const double* rows;      // nrows x {period, epoch (absolute), duration}
   long long* sums;         // nrows x {sum v, sum w} (fixed point)

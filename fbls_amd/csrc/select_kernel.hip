@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "select_kernel.h"
+#include "fixed_point.h"
 
 namespace fbls {
 
@@ -49,8 +50,8 @@ __global__ __launch_bounds__(kBlock) void rescore_rows(RescoreArgs A) {
     if (A.excluded[i]) continue;
     // (time + origin - epoch) as the reference forms it
     if (in_transit(A.t[i] + A.origin, epoch, inverse, period, half)) {
-      a += A.vw[2 * i];
-      w += A.vw[2 * i + 1];
+      a += fixed_ivalue(A.vw[2 * i], A.fx & 1u);
+      w += fixed_ivalue(A.vw[2 * i + 1], A.fx & 2u);
     }
   }
   const long long sa = block_sum(a, red);

@@ -3,9 +3,10 @@ src/batch.rs:74-106: every second from a star's light curve to its result
 counts): from light curves in host memory to each star's best row.
 
 Host preparation (`family_state`, curve.rs:541-569) runs on worker threads
--- the library's calls release the interpreter lock -- for chunk k + 1 while
-the GPU uploads, searches and selects chunk k.  The search of a chunk is one
-of:
+-- the library's calls release the interpreter lock -- for the chunks ahead,
+the upload of chunk k + 1 runs on its own thread (and the library's upload
+stream) while the GPU searches and selects chunk k.  The search of a chunk
+is one of:
 
   full     the packed upload and the merged-launch full scan of every star
            (fbls_signal_search_batch);
@@ -13,10 +14,12 @@ of:
            (fbls_propose) and the bin-restricted search (fbls_signal_search_bins)
            at an operating point (tau, top-k, fallback band);
   long     the long-baseline variant: the binned proposal per baseline group
-           and the ragged bin-restricted search (fbls_signal_search_bins_ragged).
+           (network bins and the periodogram's raw top bins) and the ragged
+           bin-restricted search (fbls_signal_search_bins_ragged).
 
 Returns the stars' best (period, SNR), the wall seconds of the whole run and
-the seconds of each stage (preparation waiting time included in `wait`).
+the seconds of each stage (the time the search waited for a prepared and
+uploaded chunk in `wait`).
 """
 import concurrent.futures as cf
 import time
@@ -41,29 +44,33 @@ class Stages:
         self.seconds[name] = self.seconds.get(name, 0.0) + dt
 
 
-def _search_full(gpu, states, cfg, st):
-    t0 = time.perf_counter()
+def _upload(gpu, states, mode):
+    """The chunk's device families (and, for the long mode, its binned
+    families for the periodogram)."""
     devs = DeviceReady.upload_packed(gpu, states)
+    if mode != "long":
+        return devs, None
+    from . import longbase as LB
+
+    return devs, DeviceReady.upload_packed(gpu, [LB.binned_state(s) for s in states])
+
+
+def _search_full(gpu, states, devs, cfg, st):
     t1 = time.perf_counter()
     rows, stats = signal_search_batch(gpu, devs, cfg.durations_days(),
                                       [s.rho for s in states], cfg)
     t2 = time.perf_counter()
     best = PR.best_rows(rows, cfg.threshold)
-    for d in devs:
-        d.close()
     t3 = time.perf_counter()
-    st.add("upload", t1 - t0)
     st.add("search", t2 - t1)
     st.add("select", t3 - t2)
     return best, sum(s["templates_scanned"] for s in stats)
 
 
-def _search_pruned(gpu, states, cfg, net, pc, op, st):
-    tau, topk, (lo, hi) = op
+def _search_pruned(gpu, states, devs, cfg, net, pc, op, st):
+    tau, topk, (lo, hi) = op[:3]
     gd = PR.group_durations(cfg, pc.groups)
     centers, edges = PR.bin_centers(pc), PR.bin_edges(pc)
-    t0 = time.perf_counter()
-    devs = DeviceReady.upload_packed(gpu, states)
     t1 = time.perf_counter()
     z = propose(gpu, net, devs, centers, gd, pc.nb_min, pc.nb_max)
     with np.errstate(over="ignore"):  # exp overflow is p = 0
@@ -76,25 +83,17 @@ def _search_pruned(gpu, states, cfg, net, pc, op, st):
                                      cfg, bins, edges)
     t3 = time.perf_counter()
     best = PR.best_rows(rows, cfg.threshold)
-    for d in devs:
-        d.close()
     t4 = time.perf_counter()
-    st.add("upload", t1 - t0)
     st.add("proposal", t2 - t1)
     st.add("search", t3 - t2)
     st.add("select", t4 - t3)
     return best, sum(s["templates_scanned"] for s in stats)
 
 
-def _search_long(gpu, states, cfg, net, pc, op, st):
+def _search_long(gpu, states, devs, bdevs, cfg, net, pc, op, st):
     from . import longbase as LB
 
-    tau, topk, (lo, hi) = op
     gd = PR.group_durations(cfg, pc.groups)
-    t0 = time.perf_counter()
-    devs = DeviceReady.upload_packed(gpu, states)
-    binned = [LB.binned_state(s) for s in states]
-    bdevs = DeviceReady.upload_packed(gpu, binned)
     t1 = time.perf_counter()
     keeps, edges = [None] * len(states), [None] * len(states)
     for t_last, idx in LB.groups_of(states).items():
@@ -102,25 +101,19 @@ def _search_long(gpu, states, cfg, net, pc, op, st):
         per = max(1, (1 << 24) // len(centers))  # network batch limit
         for c0 in range(0, len(idx), per):
             part = idx[c0:c0 + per]
-            z = propose(gpu, net, [bdevs[k] for k in part], centers, gd, pc.nb_min, pc.nb_max)
+            z, pk = propose(gpu, net, [bdevs[k] for k in part], centers, gd, pc.nb_min,
+                            pc.nb_max, peaks=True)
             with np.errstate(over="ignore"):
                 p = 1.0 / (1.0 + np.exp(-z))
             for j, k in enumerate(part):
-                keep = LB.keep_from(p[j], pc.keep_dilate, tau, topk)
-                if lo <= float(p[j].max()) < hi:
-                    keep[:] = True
-                keeps[k], edges[k] = keep, e
-    for b in bdevs:
-        b.close()
+                keeps[k] = LB.keep_mask(p[j], pk[j], pc.keep_dilate, op)
+                edges[k] = e
     t2 = time.perf_counter()
     rows, stats = signal_search_bins_ragged(gpu, devs, cfg.durations_days(),
                                             [s.rho for s in states], cfg, keeps, edges)
     t3 = time.perf_counter()
     best = PR.best_rows(rows, cfg.threshold)
-    for d in devs:
-        d.close()
     t4 = time.perf_counter()
-    st.add("upload", t1 - t0)
     st.add("proposal", t2 - t1)
     st.add("search", t3 - t2)
     st.add("select", t4 - t3)
@@ -129,33 +122,50 @@ def _search_long(gpu, states, cfg, net, pc, op, st):
 
 def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, workers=16):
     """One end-to-end batch run over `curves` ((time, flux, flux_err) per
-    star, flux normalised to 1).  Returns a dict with the stars' best rows,
-    the wall seconds, stars/hour, trial evals, the stage seconds and the
-    worker count."""
+    star, flux normalised to 1).  Three stages overlap: preparation of
+    chunks k + 1 and k + 2 on `workers` threads, the upload of chunk k + 1
+    (its own thread and HIP stream) and the search of chunk k.  Returns a
+    dict with the stars' best rows, the wall seconds, stars/hour, trial
+    evals, the stage seconds (upload seconds overlap the search; `wait` is
+    the time the search waited for its chunk) and the worker count."""
+    if mode not in ("full", "pruned", "long"):
+        raise ValueError(mode)
     durations = cfg.durations_days()
     st = Stages()
     best, templates = [], 0
-    with cf.ThreadPoolExecutor(workers) as pool:
-        def submit(a, b):
+    spans = [(a, min(len(curves), a + chunk)) for a in range(0, len(curves), chunk)]
+    with cf.ThreadPoolExecutor(workers) as pool, cf.ThreadPoolExecutor(1) as up:
+        def prepare(k):
+            a, b = spans[k]
             return [pool.submit(_prepare, c, durations, cfg.correlated_noise)
                     for c in curves[a:b]]
 
-        t_start = time.perf_counter()
-        pending = submit(0, chunk)
-        for c0 in range(0, len(curves), chunk):
+        def upload(futs):
+            states = [f.result() for f in futs]
             t0 = time.perf_counter()
-            states = [f.result() for f in pending]
+            devs, bdevs = _upload(gpu, states, mode)
+            return states, devs, bdevs, time.perf_counter() - t0
+
+        t_start = time.perf_counter()
+        preps = {k: prepare(k) for k in range(min(2, len(spans)))}
+        ups = {0: up.submit(upload, preps.pop(0))} if spans else {}
+        for k in range(len(spans)):
+            if k + 1 < len(spans):
+                ups[k + 1] = up.submit(upload, preps.pop(k + 1))
+            if k + 2 < len(spans):
+                preps[k + 2] = prepare(k + 2)
+            t0 = time.perf_counter()
+            states, devs, bdevs, t_up = ups.pop(k).result()
             st.add("wait", time.perf_counter() - t0)
-            if c0 + chunk < len(curves):
-                pending = submit(c0 + chunk, c0 + 2 * chunk)
+            st.add("upload", t_up)
             if mode == "full":
-                b, n = _search_full(gpu, states, cfg, st)
+                b, n = _search_full(gpu, states, devs, cfg, st)
             elif mode == "pruned":
-                b, n = _search_pruned(gpu, states, cfg, net, pc, op, st)
-            elif mode == "long":
-                b, n = _search_long(gpu, states, cfg, net, pc, op, st)
+                b, n = _search_pruned(gpu, states, devs, cfg, net, pc, op, st)
             else:
-                raise ValueError(mode)
+                b, n = _search_long(gpu, states, devs, bdevs, cfg, net, pc, op, st)
+            for d in devs + (bdevs or []):
+                d.close()
             best += b
             templates += n
         wall = time.perf_counter() - t_start

@@ -116,17 +116,53 @@ def groups_of(states):
     return out
 
 
-def keep_from(probs, dilate, tau, topk=0):
-    """Bins with p >= tau plus the topk most probable, dilated by `dilate`
-    bins on each side."""
-    sel = probs >= tau
-    if topk:
-        sel[np.argpartition(-probs, topk - 1)[:topk]] = True
+def _dilate(sel, dilate):
     out = sel.copy()
     for o in range(1, dilate + 1):
         out[o:] |= sel[:-o]
         out[:-o] |= sel[o:]
     return out
+
+
+def _top(x, k, sel):
+    k = min(k, len(x))
+    if k > 0:
+        sel[np.argpartition(-x, k - 1)[:k]] = True
+
+
+def keep_from(probs, dilate, tau, topk=0):
+    """Bins with p >= tau plus the topk most probable, dilated by `dilate`
+    bins on each side."""
+    sel = probs >= tau
+    _top(probs, topk, sel)
+    return _dilate(sel, dilate)
+
+
+def operating_point(extra):
+    """(tau, top-k, fallback band, raw top-k) of a saved proposal's extras
+    (raw top-k 0 when the file has none)."""
+    return (float(extra["tau"]), int(extra["topk"]),
+            tuple(float(v) for v in np.atleast_1d(extra["fallback"])),
+            int(extra["raw_topk"]) if "raw_topk" in extra else 0)
+
+
+def keep_mask(probs, peaks, dilate, op):
+    """The bins the pruned search scans at operating point op = (tau, top-k,
+    (lo, hi), raw top-k): the network's (p >= tau and its top-k most
+    probable) and the coarse periodogram's raw top-k highest-SNR bins, all
+    dilated by `dilate` bins; every bin when the network's largest
+    probability lies in the fallback band [lo, hi).  The raw bins catch
+    strong signals the network ranks low (a short-period eclipsing binary
+    whose subharmonic it prefers); they cost a few bins per star."""
+    tau, topk, (lo, hi) = op[:3]
+    raw = op[3] if len(op) > 3 else 0
+    if lo <= float(probs.max()) < hi:
+        return np.ones(len(probs), dtype=bool)
+    sel = probs >= tau
+    _top(probs, topk, sel)
+    if raw and peaks is not None:
+        _top(np.where(np.isfinite(peaks), peaks, -np.inf), raw, sel)
+    return _dilate(sel, dilate)
 
 
 class LongBatch:
@@ -165,41 +201,40 @@ class LongBatch:
         self.rf = PR.recovered(best, self.injected)
 
     def proposal(self):
-        """Per star: bin probabilities and edges; the seconds of binning,
-        upload and the device-resident periodogram + network."""
+        """Per star: bin probabilities, the coarse periodogram's peak SNRs
+        and the bin edges; the seconds of binning, upload and the
+        device-resident periodogram + network."""
         gd = PR.group_durations(self.cfg, self.pc.groups)
         t0 = time.perf_counter()
         binned = [binned_state(s) for s in self.states]
         bdevs = DeviceReady.upload_packed(self.gpu, binned)
-        probs, edges = [None] * len(self.states), [None] * len(self.states)
+        n = len(self.states)
+        probs, peaks, edges = [None] * n, [None] * n, [None] * n
         for t_last, idx in self.groups.items():
             centers, e = feature_bins(t_last, self.pc)
             per = max(1, (1 << 24) // len(centers))  # network batch limit
             for c0 in range(0, len(idx), per):
                 part = idx[c0:c0 + per]
-                z = propose(self.gpu, self.net, [bdevs[k] for k in part], centers, gd,
-                            self.pc.nb_min, self.pc.nb_max)
+                z, pk = propose(self.gpu, self.net, [bdevs[k] for k in part], centers, gd,
+                                self.pc.nb_min, self.pc.nb_max, peaks=True)
                 with np.errstate(over="ignore"):  # exp overflow is p = 0
                     p = 1.0 / (1.0 + np.exp(-z))
                 for j, k in enumerate(part):
-                    probs[k], edges[k] = p[j], e
+                    probs[k], peaks[k], edges[k] = p[j], pk[j], e
         dt = time.perf_counter() - t0
         for b in bdevs:
             b.close()
-        return probs, edges, dt
+        return probs, peaks, edges, dt
 
-    def pruned(self, probs, edges, t_prop, tau, topk, fallback):
-        """The bin-restricted search of every group at (tau, top-k, fallback
-        band), timed from bin selection to the rows."""
-        lo, hi = fallback
+    def pruned(self, probs, peaks, edges, t_prop, op):
+        """The bin-restricted search of every star at operating point op
+        (keep_mask), timed from bin selection to the rows."""
+        lo, hi = op[2]
         t0 = time.perf_counter()
         keeps, nfall = [], 0
         for k in range(len(self.states)):
-            keep = keep_from(probs[k], self.pc.keep_dilate, tau, topk)
-            if lo <= float(probs[k].max()) < hi:
-                keep[:] = True
-                nfall += 1
-            keeps.append(keep)
+            keeps.append(keep_mask(probs[k], peaks[k], self.pc.keep_dilate, op))
+            nfall += int(lo <= float(probs[k].max()) < hi)
         # every star in one call (its own bins and edges): the same merged
         # launches for the whole batch
         rows, stats = signal_search_bins_ragged(self.gpu, self.devs, self.d, self.rhos,
@@ -212,7 +247,8 @@ class LongBatch:
         n = len(self.states)
         eb = np.array([k == "eb" for k in self.kinds])
         return {
-            "stars": n, "tau": tau, "topk": topk, "fallback": list(fallback),
+            "stars": n, "tau": op[0], "topk": op[1], "fallback": list(op[2]),
+            "raw_topk": op[3] if len(op) > 3 else 0,
             "full_recovered": int(self.rf.sum()),
             "pruned_recovered": int((self.rf & rp).sum()),
             "recall": float((self.rf & rp).sum() / max(self.rf.sum(), 1)),
@@ -220,6 +256,7 @@ class LongBatch:
             "eb_pruned_recovered": int((self.rf & rp & eb).sum()),
             "fallback_stars": nfall,
             "kept_period_fraction": periods / max(self.periods_full, 1),
+            "periods_full": self.periods_full,
             "templates_full": self.templates_full, "templates_pruned": templates,
             "seconds_full_scan": self.t_full, "seconds_proposal": t_prop,
             "seconds_pruned_scan": t_pruned,
@@ -229,13 +266,13 @@ class LongBatch:
         }
 
     def sweep(self, points, log=None):
-        probs, edges, t_prop = self.proposal()
+        probs, peaks, edges, t_prop = self.proposal()
         out = []
-        for tau, k, fb in points:
-            out.append(self.pruned(probs, edges, t_prop, tau, k, fb))
+        for op in points:
+            out.append(self.pruned(probs, peaks, edges, t_prop, op))
             if log:
                 r = out[-1]
-                log(f"tau {tau} top-k {k} fallback {list(fb)}: recall {r['recall']:.4f} "
+                log(f"operating point {op}: recall {r['recall']:.4f} "
                     f"speedup {r['speedup']:.2f}")
         return out
 
@@ -263,14 +300,66 @@ class LongBatch:
             d.close()
 
 
-# operating points: (tau, top-k, fallback band); the validation sweep tries
-# those without a fallback band (a star with one is rescanned in full, which
-# costs 2-4x the pruned search), the held-out report adds one with it
-POINTS = [(tau, k, (0.0, 0.0)) for tau in (0.02, 0.05, 0.1, 0.2) for k in (0, 8)]
-FALLBACK_POINT = (0.05, 8, (0.005, 0.1))
+# operating points: (tau, top-k, fallback band, raw top-k); the validation
+# sweep tries those without a fallback band (a star with one is rescanned in
+# full, which costs 2-4x the pruned search), the held-out report adds one
+# with it
+POINTS = [(tau, k, (0.0, 0.0), raw) for tau in (0.02, 0.05, 0.1, 0.2) for k in (0, 8)
+          for raw in (0, 8)]
+FALLBACK_POINT = (0.05, 8, (0.005, 0.1), 8)
+# the coarse periodogram's raw top-k bins alone (the network keeps no bin):
+# what the network adds over the method without it
+RAW_ONLY = [(1.01, 0, (0.0, 0.0), 8), (1.01, 0, (0.0, 0.0), 32)]
 WEIGHTS = os.path.join(PR.HERE, "data", "proposal_long.npz")
 REPORT = os.path.join(PR.HERE, "data", "longbase_report.json")
 ALIASES = (1 / 3, 0.5, 1.0, 2.0, 3.0)
+
+
+def merge_sweeps(parts):
+    """Per operating point, the sums over disjoint sets of stars."""
+    out = []
+    for j in range(len(parts[0])):
+        rs = [p[j] for p in parts]
+        m = {k: rs[0][k] for k in ("tau", "topk", "fallback", "raw_topk")}
+        assert all({k: r[k] for k in m} == m for r in rs), "different operating points"
+        for k in ("stars", "full_recovered", "pruned_recovered", "eb_full_recovered",
+                  "eb_pruned_recovered", "fallback_stars", "templates_full",
+                  "templates_pruned"):
+            m[k] = int(sum(r[k] for r in rs))
+        for k in ("seconds_full_scan", "seconds_proposal", "seconds_pruned_scan"):
+            m[k] = float(sum(r[k] for r in rs))
+        m["recall"] = m["pruned_recovered"] / max(m["full_recovered"], 1)
+        m["periods_full"] = int(sum(r["periods_full"] for r in rs))
+        m["kept_period_fraction"] = float(sum(r["kept_period_fraction"] * r["periods_full"]
+                                              for r in rs) / m["periods_full"])
+        m["stars_per_hour_full"] = m["stars"] * 3600.0 / m["seconds_full_scan"]
+        m["stars_per_hour_pruned"] = m["stars"] * 3600.0 / (m["seconds_proposal"]
+                                                            + m["seconds_pruned_scan"])
+        m["speedup"] = m["stars_per_hour_pruned"] / m["stars_per_hour_full"]
+        out.append(m)
+    return out
+
+
+def merge_summaries(parts):
+    """The full-scan summary of disjoint sets of stars."""
+    m = {"stars": sum(p["stars"] for p in parts),
+         "sectors_min": min(p["sectors_min"] for p in parts),
+         "sectors_max": max(p["sectors_max"] for p in parts),
+         "points_max": max(p["points_max"] for p in parts)}
+    for k in ("transits", "eclipsing_binaries", "full_scan_periods", "full_scan_launches"):
+        m[k] = sum(p[k] for p in parts)
+    m["full_recovered_of_injected"] = [sum(p["full_recovered_of_injected"][i] for p in parts)
+                                       for i in (0, 1)]
+    for k in ("full_scan_seconds", "full_scan_kernel_seconds"):
+        m[k] = sum(p[k] for p in parts)
+    m["full_scan_stars_per_hour"] = m["stars"] * 3600.0 / m["full_scan_seconds"]
+    m["full_scan_trial_evals_per_s"] = sum(
+        p["full_scan_trial_evals_per_s"] * p["full_scan_seconds"] for p in parts
+    ) / m["full_scan_seconds"]
+    m["full_scan_fold_pairs_per_kernel_s"] = sum(
+        p["full_scan_fold_pairs_per_kernel_s"] * p["full_scan_kernel_seconds"] for p in parts
+    ) / m["full_scan_kernel_seconds"]
+    return m
 
 
 def training_set(gpu, pc, indices, seed0, pool, log, crop=2048, negatives=2,
@@ -355,18 +444,42 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--validate", type=int, default=320)
     ap.add_argument("--heldout", type=int, default=480)
+    ap.add_argument("--weights", default=WEIGHTS,
+                    help="the fine-tuned network to evaluate when not training")
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--report", default=None)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--skip-fallback", action="store_true",
                     help="leave the fallback-band point out of the held-out sweep")
     ap.add_argument("--heldout-only", action="store_true",
-                    help="keep the saved operating point and validation figures of the "
-                         "committed report; measure the held-out stars again")
+                    help="keep the operating point of --weights and the validation figures "
+                         "of --prior; measure the held-out stars again")
+    ap.add_argument("--prior", default=REPORT,
+                    help="the report whose training and validation figures are kept")
+    ap.add_argument("--heldout-start", type=int, default=0,
+                    help="first held-out star (seed 5,000,000 + index)")
+    ap.add_argument("--short-sweep", action="store_true",
+                    help="held-out sweep of the operating point, the raw-only points and "
+                         "the fallback point only")
+    ap.add_argument("--merge", nargs="+", default=None,
+                    help="combine held-out reports of consecutive star ranges into --report")
     args = ap.parse_args()
 
     def log(msg):
         print(f"[longbase] {msg}", flush=True)
+
+    if args.merge:
+        parts = [json.load(open(p)) for p in args.merge]
+        report = dict(parts[0])
+        report["heldout"] = merge_summaries([r["heldout"] for r in parts])
+        report["heldout_sweep"] = merge_sweeps([r["heldout_sweep"] for r in parts])
+        report["heldout_at_operating_point"] = report["heldout_sweep"][0]
+        report["heldout_stars"] = report["heldout"]["stars"]
+        report["heldout_parts"] = [r.get("heldout_range") for r in parts]
+        with open(args.report, "w") as f:
+            json.dump(report, f, indent=1)
+        log(json.dumps(report["heldout_at_operating_point"]))
+        return
 
     pool = cf.ProcessPoolExecutor(args.workers, mp_context=mp.get_context("spawn"))
     gpu = Gpu(0)
@@ -387,26 +500,29 @@ def main():
         PR.save(net, pc, args.out)
         log(f"fine-tuned {train_info}")
     else:
-        path = args.out if os.path.exists(args.out) else PR.DEFAULT_WEIGHTS
-        net, pc, _ = PR.load(gpu, path)
-        if os.path.exists(REPORT):  # the training of the committed network
-            train_info = json.load(open(REPORT)).get("training", {})
-    prior = json.load(open(REPORT)) if os.path.exists(REPORT) else {}
+        net, pc, _ = PR.load(gpu, args.weights)
+        if os.path.exists(args.prior):  # the training of the evaluated network
+            train_info = json.load(open(args.prior)).get("training", {})
+    prior = json.load(open(args.prior)) if os.path.exists(args.prior) else {}
     if args.heldout_only:  # the committed operating point, chosen on validation
-        _, _, extra = PR.load(gpu, args.out)
-        op = (float(extra["tau"]), int(extra["topk"]),
-              tuple(float(v) for v in np.atleast_1d(extra["fallback"])))
+        _, _, extra = PR.load(gpu, args.weights)
+        op = operating_point(extra)
         args.validate = 0
     vres = make_states(range(args.validate), 5_100_000, pool=pool) if args.validate else []
-    hres = make_states(range(args.heldout), 5_000_000, pool=pool)
+    span = range(args.heldout_start, args.heldout_start + args.heldout)
+    t_prep = time.perf_counter()
+    hres = make_states(span, 5_000_000, pool=pool) if args.heldout else []
     pool.shutdown()
-    log(f"prepared {len(vres)} + {len(hres)} stars")
+    log(f"prepared {len(vres)} + {len(hres)} stars ({time.perf_counter() - t_prep:.1f} s "
+        "for the held-out ones)")
+    val_summary = prior.get("validation_full_scan")
     if args.heldout_only:
         point = prior.get("validation_at_operating_point")
         vs = prior.get("validation_sweep")
     else:
         val = LongBatch(gpu, vres, net, pc, log=log)
-        log(f"validation full scan {val.summary()}")
+        val_summary = val.summary()
+        log(f"validation full scan {val_summary}")
         vs = val.sweep(POINTS, log=lambda m: log("validation " + m))
         val.close()
         point = PR.fastest(vs, 0.99)
@@ -414,29 +530,34 @@ def main():
             top = max(r["recall"] for r in vs)
             point = max((r for r in vs if r["recall"] == top),
                         key=lambda r: r["kept_period_fraction"])
-        op = (point["tau"], point["topk"], tuple(point["fallback"]))
-    held = LongBatch(gpu, hres, net, pc, log=log)
-    summary = held.summary()
-    log(f"held-out full scan {summary}")
-    extra = [] if args.skip_fallback else [FALLBACK_POINT]
-    hs = held.sweep([op] + [p for p in POINTS + extra if p != op],
-                    log=lambda m: log("held-out " + m))
-    held.close()
+        op = (point["tau"], point["topk"], tuple(point["fallback"]), point["raw_topk"])
+    summary, hs = None, [None]
+    if hres:  # --heldout 0: select on validation only
+        held = LongBatch(gpu, hres, net, pc, log=log)
+        summary = held.summary()
+        log(f"held-out full scan {summary}")
+        extra = ([] if args.skip_fallback else [FALLBACK_POINT]) + RAW_ONLY
+        rest = extra if args.short_sweep else POINTS + extra
+        hs = held.sweep([op] + [p for p in rest if p != op],
+                        log=lambda m: log("held-out " + m))
+        held.close()
     report = {"training": train_info,
               "selection_rule": ("the fastest point whose validation recall reaches 99 %; "
                                  "when none does, the most cautious (largest kept fraction) "
                                  "of the points with the best validation recall"),
-              "operating_point": {"tau": op[0], "topk": op[1], "fallback": list(op[2])},
+              "operating_point": {"tau": op[0], "topk": op[1], "fallback": list(op[2]),
+                                  "raw_topk": op[3]},
               "validation_stars": prior.get("validation_stars") if args.heldout_only
               else len(vres),
               "validation_at_operating_point": point,
+              "validation_full_scan": val_summary,
               "validation_sweep": vs,
               "heldout": summary, "heldout_at_operating_point": hs[0],
-              "heldout_sweep": hs}
-    if args.heldout_only and "validation_full_scan" in prior:
-        report["validation_full_scan"] = prior["validation_full_scan"]
-    log(json.dumps(hs[0]))
-    PR.save(net, pc, args.out, tau=op[0], topk=op[1], fallback=np.array(op[2], dtype=float))
+              "heldout_sweep": hs,
+              "heldout_range": [span.start, span.stop] if hres else None}
+    log(json.dumps(hs[0] if hres else point))
+    PR.save(net, pc, args.out, tau=op[0], topk=op[1], fallback=np.array(op[2], dtype=float),
+            raw_topk=op[3])
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1)

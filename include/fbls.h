@@ -472,10 +472,12 @@ int fbls_coarse_inputs(fbls_ctx* ctx, const fbls_family* const* fams, size_t nfa
  * computed into the network's device buffer and the fused inference runs
  * on them (no copy of the inputs to the host and back); logits[k][i] for
  * star k and period bin i.  net must belong to ctx and read ng durations;
- * stars go through in batches of about 2^22 bins. */
+ * stars go through in batches of about 2^22 bins.  peaks (may be NULL):
+ * peaks[k][i], the periodogram's largest SNR over the durations at bin i
+ * (unclipped; -inf where no phase qualifies), from the same kernel. */
 int fbls_propose(fbls_ctx* ctx, fbls_cnn* net, const fbls_family* const* fams, size_t nfam,
                  const double* periods, size_t np, const double* durations, size_t ng,
-                 uint32_t nb_min, uint32_t nb_max, float* logits);
+                 uint32_t nb_min, uint32_t nb_max, float* logits, float* peaks);
 
 /* -- raw-mode preparation ----------------------------------------------- */
 
@@ -566,9 +568,13 @@ int fbls_family_state(const double* time, const double* flux,
                       double* origin, double* rho, double* inflation,
                       size_t* segments, double* residual_energy);
 
-/* [host] fixed-point representation used on the device; returns e with
- * q = rint(x * 2^e), |q| <= 2^30. */
-int fbls_quantize(const double* x, size_t n, int32_t* q);
+/* [host] fixed-point representation used on the device; returns e, the
+ * words standing for integers in units of 2^-e.  *block (may be NULL) = 0:
+ * q = rint(x * 2^e), |q| <= 2^30 (max |x| <= 16 mean |x|); 1: block mode,
+ * q = mantissa * 32 + shift for the integer mantissa * 2^shift (27-bit
+ * signed mantissa, 5-bit shift).  Either way each value's error is at most
+ * 2^-26 max(|x|, mean |x|) and the integers' absolute sum is below 2^53. */
+int fbls_quantize(const double* x, size_t n, int32_t* q, int* block);
 
 /* [host] seeded synthetic light curve (xorshift64 + Box-Muller as in
  * survey/numeric.rs:345-364).  See fbls_amd/synth.py for the workloads. */

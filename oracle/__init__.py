@@ -170,6 +170,8 @@ def lib():
         L.orc_rng_normal.restype = _d
         L.orc_rng_normal.argtypes = [_P(_u64), _d, _d]
         L.orc_quantize.restype = ctypes.c_int
+        L.orc_fixed_value.restype = _d
+        L.orc_fixed_value.argtypes = [ctypes.c_int32, ctypes.c_int]
         for f in ("orc_exp", "orc_log", "orc_erfc"):
             getattr(L, f).restype = _d
             getattr(L, f).argtypes = [_d]
@@ -317,10 +319,28 @@ def family_state(time, flux, sigma, durations, correlated=True):
 
 
 def quantize(x):
+    """(words, e, block) of the fixed-point representation (orc_quantize)."""
     x = _f64(x)
     q = np.empty(len(x), dtype=np.int32)
-    e = lib().orc_quantize(_dp(x), _sz(len(x)), q.ctypes.data_as(_P(ctypes.c_int32)))
-    return q, e
+    block = ctypes.c_int(0)
+    e = lib().orc_quantize(_dp(x), _sz(len(x)), q.ctypes.data_as(_P(ctypes.c_int32)),
+                           ctypes.byref(block))
+    return q, e, bool(block.value)
+
+
+def fixed_values(q, block):
+    """The integers (units of 2^-e) the words stand for."""
+    q = np.asarray(q, dtype=np.int64)
+    if not block:
+        return q.astype(np.float64)
+    s = q & 31
+    return np.ldexp(((q - s) // 32).astype(np.float64), s.astype(np.int32))
+
+
+def quantized(x):
+    """The values the device sees for x: fixed_values(q) * 2^-e."""
+    q, e, block = quantize(x)
+    return np.ldexp(fixed_values(q, block), -e)
 
 
 def phase_cell(t, scale, nb):

@@ -315,20 +315,48 @@ void orc_correlated_noise_factors(const double* time, const double* residual,
 
 /* ------------------------------------------------------------ quantizer */
 
-int orc_quantize(const double* x, size_t n, int32_t* q) {
-  double m = 0.0;
+int orc_quantize(const double* x, size_t n, int32_t* q, int* block) {
+  double mx = 0.0, sum = 0.0;
   for (size_t i = 0; i < n; i++) {
     double a = fabs(x[i]);
-    if (a > m) m = a;
+    if (a > mx) mx = a;
+    sum += a;
   }
-  int e = 0;
-  if (m > 0.0) {
-    int ex;
-    frexp(m, &ex); /* m = f * 2^ex, f in [0.5, 1) */
-    e = 30 - ex;
+  *block = 0;
+  if (!(mx > 0.0)) {
+    for (size_t i = 0; i < n; i++) q[i] = 0;
+    return 0;
   }
-  for (size_t i = 0; i < n; i++) q[i] = (int32_t)nearbyint(ldexp(x[i], e));
+  int ex;
+  if (mx <= 16.0 * (sum / (double)n)) {
+    frexp(mx, &ex); /* mx = f * 2^ex, f in [0.5, 1) */
+    int e = 30 - ex;
+    for (size_t i = 0; i < n; i++) q[i] = (int32_t)nearbyint(ldexp(x[i], e));
+    return e;
+  }
+  /* block mode: an integer r per value (sum |r| < 2^52 + n), kept with 26
+   * significant bits: r ~ m * 2^s, 2^25 <= |m| < 2^26, word m * 32 + s */
+  *block = 1;
+  frexp(sum, &ex);
+  int e = 52 - ex;
+  for (size_t i = 0; i < n; i++) {
+    double r = nearbyint(ldexp(x[i], e));
+    double m = r;
+    int s = 0;
+    while (fabs(m) >= 67108864.0) { /* 2^26: halve the scale until it fits */
+      s++;
+      m = nearbyint(ldexp(r, -s));
+    }
+    q[i] = (int32_t)((int64_t)m * 32 + s);
+  }
   return e;
+}
+
+double orc_fixed_value(int32_t word, int block) {
+  if (!block) return (double)word;
+  int32_t s = word & 31;
+  int32_t m = (word - s) / 32; /* exact: word - s is a multiple of 32 */
+  return ldexp((double)m, s);
 }
 
 /* ------------------------------------------------------------- the scan */

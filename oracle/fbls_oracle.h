@@ -72,9 +72,15 @@ void orc_correlated_noise_factors(const double* time, const double* residual,
                                   const double* durations, size_t nd,
                                   double* rho);
 
-/* Fixed-point step of the build's input representation: q = rint(x * 2^e),
- * e = 30 - ceil-exponent(max |x|), so |q| <= 2^30.  Returns e. */
-int orc_quantize(const double* x, size_t n, int32_t* q);
+/* The build's fixed-point input representation; returns e, the words
+ * standing for integers in units of 2^-e.  When max |x| <= 16 mean |x|
+ * (*block = 0): q = rint(x * 2^e), e = 30 - ceil-exponent(max |x|), so
+ * |q| <= 2^30.  Otherwise (*block = 1): e = 52 - ceil-exponent(sum |x|),
+ * r = rint(x * 2^e), and the word m * 32 + s holds r to 26 significant bits
+ * as m * 2^s (|m| < 2^26, the smallest such s). */
+int orc_quantize(const double* x, size_t n, int32_t* q, int* block);
+/* The integer (units of 2^-e) a word of orc_quantize stands for. */
+double orc_fixed_value(int32_t word, int block);
 
 /* kernels.rs:314-327: floor(t * scale) mod nb */
 uint32_t orc_phase_cell(double t, double scale, uint32_t nb);

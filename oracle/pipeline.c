@@ -194,10 +194,11 @@ static int fam_build(const double* ct, const double* cf, const double* cs,
   free(dv);
   /* fixed point as the library uploads it */
   int32_t* q = (int32_t*)malloc((m + 1) * 4);
-  int e = orc_quantize(f->v, m, q);
-  for (size_t i = 0; i < m; i++) f->v[i] = ldexp((double)q[i], -e);
-  e = orc_quantize(f->w, m, q);
-  for (size_t i = 0; i < m; i++) f->w[i] = ldexp((double)q[i], -e);
+  int block = 0;
+  int e = orc_quantize(f->v, m, q, &block);
+  for (size_t i = 0; i < m; i++) f->v[i] = ldexp(orc_fixed_value(q[i], block), -e);
+  e = orc_quantize(f->w, m, q, &block);
+  for (size_t i = 0; i < m; i++) f->w[i] = ldexp(orc_fixed_value(q[i], block), -e);
   free(q);
   return 0;
 }

@@ -28,9 +28,7 @@ import oracle as O  # noqa: E402
 
 
 def _quantised(state):
-    vq, ev = O.quantize(state["v"])
-    wq, ew = O.quantize(state["w"])
-    return vq * 2.0 ** -ev, wq * 2.0 ** -ew
+    return O.quantized(state["v"]), O.quantized(state["w"])
 
 
 def _family(seed=3, n=400, span=4.0, dip=(2.1, 0.12, 6e-3), sigma=1e-3):

@@ -1,7 +1,9 @@
 """GPU parity: the HIP scan against the FP64 oracle on the same inputs.
 
-The device scans v and w in 32-bit fixed point (power-of-two scale), so the
-oracle is fed the same fixed-point values (DeviceReady.fields()).  Every
+The device scans v and w as 32-bit fixed-point words (plain integers under a
+power-of-two scale, or, for fields of wide dynamic range, a 27-bit mantissa
+over a 5-bit shift), so the oracle is fed the same fixed-point values
+(DeviceReady.fields()).  Every
 interval sum is then exact on both sides and the contract is bit-exact:
 winner indices (phase, width, shape, duration) and the records/scores.
 A second comparison against the unquantised FP64 fields bounds the effect of
@@ -77,6 +79,42 @@ def test_scan_bank_vs_unquantised(gpu):
     assert rel.max() < 1e-6
     same = (win == owin).all(axis=1)
     assert same.mean() > 0.99
+
+
+def _wide_range_star(cfg):
+    """The single-sector star with a 1e4-sigma flux spike and one point whose
+    weight is 1e6 times the others' (flux error 1e-3 of theirs): both fields
+    go to block mode."""
+    t, f, e = _star(seed=1)
+    f = f.copy()
+    e = e.copy()
+    f[5000] += 1e4 * e[5000]
+    e[9000] *= 1e-3
+    d = cfg.durations_days()
+    s = F.family_state(t, f, e, d, cfg.correlated_noise)
+    return s, d
+
+
+def test_block_fixed_point_bit_exact_and_bounded(gpu):
+    """Block-mode fields: the GPU equals the oracle on the same fixed-point
+    values, and both stay within 1e-6 relative of the unquantised scores."""
+    cfg = F.SearchConfig(period_min=0.5, period_max=13.7)
+    s, d = _wide_range_star(cfg)
+    assert F.quantize(s.v)[2] and F.quantize(s.w)[2]
+    dev = F.DeviceReady.upload(gpu, s)
+    vq, wq, _, _ = dev.fields()
+    periods = np.concatenate([np.linspace(0.5, 13.7, 200), [3.7]])
+    for nb in (96, 384):
+        _compare(gpu, dev, s, vq, wq, d, periods, nb, cfg, f"block nb{nb}")
+    bank = F.make_bank(cfg, d, s.rho)
+    rec, score, depth, win, st = F.scan_bank(gpu, dev, periods, bank, 384)
+    orec, oscore, odepth, owin, ost = O.scan_bank(
+        s.t, s.v, s.w, s.baseline_norm, periods, d, s.rho,
+        cfg.ingress_fractions, cfg.shape_layers, cfg.shape_gate, 384)
+    ok = np.isfinite(oscore)
+    rel = np.abs(score[ok] - oscore[ok]) / np.abs(oscore[ok])
+    print("block mode: max rel score error vs FP64 fields:", rel.max(), flush=True)
+    assert rel.max() < 1e-6
 
 
 def test_signal_search_bench_star(gpu):

@@ -84,7 +84,7 @@ def test_keep_mask_rows_are_full_rows(gpu):
 def test_trained_proposal_recall_and_gain(gpu):
     net, pc, extra = PR.load(gpu)
     cfg = PR.search_config(pc)
-    res = PR.make_states(range(192), 4_950_000, cfg, workers=1)
+    res = PR.make_states(range(1024), 4_950_000, cfg, workers=8)
     ev = PR.EvalSet(gpu, [r[0] for r in res], [r[1] for r in res], pc)
     topk, tau = int(extra["topk"]), float(extra["tau"])
     fb = tuple(float(v) for v in np.atleast_1d(extra["fallback"]))
@@ -92,11 +92,14 @@ def test_trained_proposal_recall_and_gain(gpu):
     ev.close()
     at = sw["learned"][0]
     print("[proposal]", at, flush=True)
-    assert at["full_recovered"] >= 20
-    assert at["recall"] >= 0.95
+    # 20,000 held-out stars: 99.9 % recall; ~430 recoveries here
+    assert at["full_recovered"] >= 300
+    assert at["recall"] >= 0.99
+    assert at["stars_per_hour_pruned"] >= 5 * at["stars_per_hour_full"]
     learned0, raw0 = sw["learned"][1], sw["raw_snr_topk"][1]
     print("[proposal] learned", learned0["recall"], "raw", raw0["recall"], flush=True)
-    assert learned0["recall"] >= raw0["recall"]
+    # the network against the raw periodogram's top bins at the same kept share
+    assert learned0["recall"] >= raw0["recall"] + 0.1
 
 
 def test_coarse_features_unequal_weights_bit_exact(gpu):

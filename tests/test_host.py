@@ -99,11 +99,13 @@ def test_family_state_kat_and_errors():
 def test_quantize_matches_oracle():
     rng = np.random.default_rng(9)
     for scale in (1e-3, 1.0, 1e6):
-        x = rng.normal(0, scale, 5000)
-        q1, e1 = F.quantize(x)
-        q2, e2 = O.quantize(x)
-        assert e1 == e2
-        np.testing.assert_array_equal(q1, q2)
+        for spike in (0.0, 1e3, 1e7):
+            x = rng.normal(0, scale, 5000)
+            x[17] += spike * scale
+            q1, e1, b1 = F.quantize(x)
+            q2, e2, b2 = O.quantize(x)
+            assert e1 == e2 and b1 == b2 == (spike > 0)
+            np.testing.assert_array_equal(q1, q2)
 
 
 def test_synth_uses_reference_rng():

@@ -89,3 +89,44 @@ def test_keep_rule_dilates():
     k = L.keep_from(p, 2, 0.25)
     assert list(np.flatnonzero(k)) == [3, 4, 5, 6, 7, 17, 18, 19]
     assert not L.keep_from(np.zeros(8), 2, 0.1).any()
+
+
+def test_keep_mask_adds_raw_peaks_and_falls_back():
+    p = np.zeros(30)
+    p[5] = 0.5
+    peaks = np.full(30, -np.inf, dtype=np.float32)
+    peaks[20], peaks[25] = 90.0, 40.0
+    # the network's bin alone, then with the periodogram's top bin
+    assert list(np.flatnonzero(L.keep_mask(p, peaks, 1, (0.25, 0, (0.0, 0.0), 0)))) == [4, 5, 6]
+    k = L.keep_mask(p, peaks, 1, (0.25, 0, (0.0, 0.0), 1))
+    assert list(np.flatnonzero(k)) == [4, 5, 6, 19, 20, 21]
+    k = L.keep_mask(p, peaks, 0, (0.25, 0, (0.0, 0.0), 2))
+    assert list(np.flatnonzero(k)) == [5, 20, 25]
+    # a three-element point is the same rule without raw bins
+    assert (L.keep_mask(p, peaks, 1, (0.25, 0, (0.0, 0.0)))
+            == L.keep_from(p, 1, 0.25)).all()
+    # the largest probability inside the fallback band: every bin
+    assert L.keep_mask(p, peaks, 1, (0.25, 0, (0.4, 0.6), 1)).all()
+    assert not L.keep_mask(p, peaks, 1, (0.25, 0, (0.6, 0.9), 0))[10:].any()
+
+
+def test_operating_point_reads_raw_topk():
+    base = {"tau": np.asarray(0.05), "topk": np.asarray(8), "fallback": np.zeros(2)}
+    assert L.operating_point(base) == (0.05, 8, (0.0, 0.0), 0)
+    assert L.operating_point(dict(base, raw_topk=np.asarray(8)))[3] == 8
+
+
+def test_merge_sweeps_sums_disjoint_sets():
+    def part(stars, full, pruned, sec, kept, periods):
+        return {"tau": 0.2, "topk": 0, "fallback": [0.0, 0.0], "raw_topk": 8,
+                "stars": stars, "full_recovered": full, "pruned_recovered": pruned,
+                "eb_full_recovered": 1, "eb_pruned_recovered": 1, "fallback_stars": 0,
+                "templates_full": 10, "templates_pruned": 1, "seconds_full_scan": sec,
+                "seconds_proposal": 1.0, "seconds_pruned_scan": 1.0,
+                "kept_period_fraction": kept, "periods_full": periods}
+    m = L.merge_sweeps([[part(10, 4, 4, 100.0, 0.01, 1000)],
+                        [part(30, 6, 5, 300.0, 0.03, 3000)]])[0]
+    assert (m["stars"], m["full_recovered"], m["pruned_recovered"]) == (40, 10, 9)
+    assert m["recall"] == 0.9
+    assert abs(m["kept_period_fraction"] - 0.025) < 1e-15
+    assert m["speedup"] == (40 * 3600 / 4.0) / (40 * 3600 / 400.0)

@@ -14,6 +14,7 @@ import os
 import numpy as np
 import pytest
 
+import fbls_amd as F
 import oracle as O
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -189,12 +190,94 @@ def test_correlated_noise_unsupported_inherits():
 def test_quantize_exact_and_bounded():
     rng = np.random.default_rng(4)
     x = rng.normal(0, 3e5, 10000)
-    q, e = O.quantize(x)
+    q, e, block = O.quantize(x)
+    assert not block  # Gaussian: max |x| ~ 5 mean |x|
     assert np.abs(q).max() <= 2 ** 30
     assert np.abs(q).max() > 2 ** 29  # the scale uses the range
     assert np.all(np.abs(q * 2.0 ** -e - x) <= 2.0 ** (-e - 1))
     # power-of-two scale: the dequantised values are exact binary fractions
     assert np.array_equal(np.ldexp(q.astype(np.float64), -e) * 2.0 ** e, q)
+
+
+def _fixed_sum_bound(x, idx):
+    """2^-26 (sum over the window of |x| + |window| mean |x|)."""
+    return 2.0 ** -26 * (np.abs(x[idx]).sum() + len(idx) * np.abs(x).mean())
+
+
+@pytest.mark.parametrize("case", ["clean", "spike_1e4", "spike_1e6", "weights_1e6"])
+def test_fixed_point_error_bound(case):
+    """Every value's error is at most 2^-26 max(|x|, mean |x|) and window sums
+    are exact: a plain Gaussian field, one 1e4- and one 1e6-sigma spike, and
+    weights spread over six decades (1/sigma^2 with one point 1e6 times the
+    rest) each stay within the bound, which is checked against the exact
+    rational value of the window sums (Python integers)."""
+    from fractions import Fraction
+
+    rng = np.random.default_rng(12)
+    n = 20000
+    if case.startswith("weights"):
+        x = rng.uniform(0.8, 1.2, n) * 1e6
+        x[777] *= 1e6
+    else:
+        x = rng.normal(0.0, 1.0, n) * 3e3
+        if case != "clean":
+            x[4321] = float(case.split("_")[1]) * 3e3
+    q, e, block = O.quantize(x)
+    assert block == (case != "clean")
+    ints = O.fixed_values(q, block)
+    assert np.all(ints == np.round(ints)) and np.abs(ints).sum() < 2.0 ** 53
+    got = np.ldexp(ints, -e)
+    per = 2.0 ** -26 * np.maximum(np.abs(x), np.abs(x).mean())
+    assert np.all(np.abs(got - x) <= per)
+    # windows: a box over the spike, a long run and scattered points
+    for idx in (np.arange(4000, 4600), np.arange(0, n, 3), rng.choice(n, 500, replace=False)):
+        exact = sum(Fraction(float(v)) for v in x[idx])
+        fixed = sum(Fraction(float(v)) for v in got[idx])
+        s = float(np.sum(got[idx]))  # FP64 sum of the fixed values: exact
+        assert Fraction(s) == fixed
+        assert abs(float(fixed - exact)) <= _fixed_sum_bound(x, idx)
+
+
+@pytest.mark.parametrize("spike", [0.0, 1e4, 1e5])
+def test_fixed_point_score_error(spike):
+    """Box scores on the fixed-point fields against the unquantised FP64
+    fields, on a single-sector star with a flux spike of `spike` sigma and one
+    weight 1e6 times the rest: within 1e-6 relative (the bound of the value
+    errors, 2^-26 max(|x|, mean |x|), through the box sums)."""
+    t, f, e = F.synth_star(seed=1, sectors=1, sigma=1e-3, period=3.7, depth=2e-3,
+                           duration=2.5 / 24)
+    f, e = f - 1.0, e.copy()
+    if spike:
+        f[5000] += spike * e[5000]
+        e[9000] *= 1e-3
+    cfg = F.SearchConfig(period_min=0.5, period_max=13.7)
+    d = cfg.durations_days()
+    s = F.family_state(t, f, e, d, cfg.correlated_noise)
+    vq, wq = O.quantized(s.v), O.quantized(s.w)
+    periods = np.concatenate([np.linspace(0.5, 13.7, 150), [3.7]])
+    args = (s.baseline_norm, periods, d, s.rho, cfg.ingress_fractions, cfg.shape_layers,
+            cfg.shape_gate, 384)
+    _, sq, _, wn_q, _ = O.scan_bank(s.t, vq, wq, *args)
+    _, sx, _, wn_x, _ = O.scan_bank(s.t, s.v, s.w, *args)
+    ok = np.isfinite(sx)
+    rel = np.abs(sq[ok] - sx[ok]) / np.abs(sx[ok])
+    assert rel.max() < 1e-6, rel.max()
+    assert (wn_q == wn_x).all(axis=1).mean() > 0.99
+
+
+def test_fixed_point_block_words():
+    """Block words: a 27-bit mantissa over a 5-bit shift, the smallest shift
+    that keeps the mantissa below 2^26 (checked by brute force)."""
+    x = np.array([1.0, -3.0, 2.5e7, -7.77e9, 123456789.0, 0.0] + [1.0] * 200)
+    q, e, block = O.quantize(x)
+    assert block
+    for word, xv in zip(q[:6], x[:6]):
+        s = int(word) & 31
+        m = (int(word) - s) // 32
+        r = round(xv * 2.0 ** e)
+        assert abs(m) < 2 ** 26
+        assert all(abs(round(r / 2 ** k)) >= 2 ** 26 for k in range(s))
+        assert m == round(r / 2 ** s)
 
 
 # ---------------------------------------------------------- phase cell
@@ -320,8 +403,9 @@ def _small_star(seed, n=1500, depth=4e-3, period=1.37, gap=True, sigma=1e-3):
     C = float(w.sum())
     mean = float((w * y).sum()) / C
     v = w * y - w * mean
-    vq, ev = O.quantize(v)
-    wq, ew = O.quantize(w)
+    vq, ev, bv = O.quantize(v)
+    wq, ew, bw = O.quantize(w)
+    assert not bv and not bw  # plain words: the value is q * 2^-e
     return t, vq, ev, wq, ew, C
 
 
@@ -355,8 +439,9 @@ def test_scan_bank_exact_ties_take_smallest_key():
     w = np.full(len(t), 1e6)
     C = float(w.sum())
     v = w * y - w * (float((w * y).sum()) / C)
-    vq, ev = O.quantize(v)
-    wq, ew = O.quantize(w)
+    vq, ev, bv = O.quantize(v)
+    wq, ew, bw = O.quantize(w)
+    assert not bv and not bw  # plain words: the value is q * 2^-e
     vv, ww = vq * 2.0 ** -ev, wq * 2.0 ** -ew
     durations = np.array([0.04, 0.05, 0.06])
     rho = np.ones(3)
