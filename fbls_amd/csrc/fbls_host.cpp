@@ -6,8 +6,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <system_error>
@@ -16,6 +19,7 @@
 
 #include "../../include/fbls.h"
 #include "prep.h"
+#include "band_kernel.h"
 #include "scan_host.h"
 #include "scan_kernel.h"
 
@@ -45,24 +49,104 @@ double fbls::host::now_s() {
 
 extern "C" const char* fbls_last_error(void) { return g_last_error.c_str(); }
 
+namespace {
+
+// Host worker threads shared by every call of the library: a call splits its
+// range into pieces, queues all but the first, runs the first itself and
+// then, until its own pieces are done, runs queued pieces (its own or
+// another call's) or waits.  Nothing blocks on a piece that no thread will
+// run, whatever the number of concurrent callers or nesting.
+class Pool {
+ public:
+  static Pool& get() {
+    static Pool* pool = new Pool();  // never destroyed: workers outlive exit handlers
+    return *pool;
+  }
+  size_t workers() const { return threads_.size(); }
+
+  void run(size_t m, size_t pieces, const std::function<void(size_t, size_t)>& work) {
+    Group g;
+    g.left = pieces;
+    {
+      std::lock_guard<std::mutex> hold(mu_);
+      for (size_t k = 1; k < pieces; k++)
+        queue_.push_back(Task{&g, &work, m * k / pieces, m * (k + 1) / pieces});
+    }
+    cv_.notify_all();
+    work(0, m / pieces);
+    finish(&g);
+    std::unique_lock<std::mutex> lock(mu_);
+    while (g.left > 0) {
+      if (!queue_.empty()) {
+        Task t = queue_.front();
+        queue_.pop_front();
+        lock.unlock();
+        (*t.work)(t.a, t.e);
+        finish(t.g);
+        lock.lock();
+      } else {
+        done_.wait(lock);
+      }
+    }
+  }
+
+ private:
+  struct Group {
+    size_t left = 0;
+  };
+  struct Task {
+    Group* g;
+    const std::function<void(size_t, size_t)>* work;
+    size_t a, e;
+  };
+
+  Pool() {
+    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+    const size_t n = std::min<size_t>(hw, 16) - 1;
+    for (size_t k = 0; k < n; k++) {
+      try {
+        threads_.emplace_back([this]() { loop(); });
+      } catch (const std::system_error&) {
+        break;  // fewer workers: the callers run more pieces themselves
+      }
+    }
+    for (std::thread& t : threads_) t.detach();
+  }
+  void finish(Group* g) {
+    std::lock_guard<std::mutex> hold(mu_);
+    if (--g->left == 0) done_.notify_all();
+  }
+  void loop() {
+    std::unique_lock<std::mutex> lock(mu_);
+    for (;;) {
+      cv_.wait(lock, [this]() { return !queue_.empty(); });
+      Task t = queue_.front();
+      queue_.pop_front();
+      lock.unlock();
+      (*t.work)(t.a, t.e);
+      finish(t.g);
+      lock.lock();
+    }
+  }
+
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  std::deque<Task> queue_;
+  std::vector<std::thread> threads_;
+};
+
+}  // namespace
+
 void fbls::host::parallel_ranges(size_t m, size_t per,
                                  const std::function<void(size_t, size_t)>& work) {
-  const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), (m + per - 1) / std::max<size_t>(per, 1));
+  if (m == 0) return;
+  Pool& pool = Pool::get();
+  const size_t nt = std::min<size_t>(pool.workers() + 1, (m + per - 1) / std::max<size_t>(per, 1));
   if (nt <= 1) {
     work(0, m);
     return;
   }
-  std::vector<std::thread> pool;
-  pool.reserve(nt);
-  size_t started = 0;
-  try {
-    for (; started < nt; started++)
-      pool.emplace_back(work, m * started / nt, m * (started + 1) / nt);
-  } catch (const std::system_error&) {
-  }
-  for (std::thread& th : pool) th.join();
-  if (started < nt) work(m * started / nt, m);
+  pool.run(m, nt, work);
 }
 
 // ---------------------------------------------------------------- context
@@ -82,9 +166,12 @@ extern "C" int fbls_ctx_create(int device, fbls_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess &&
         prop.maxSharedMemoryPerMultiProcessor > 0)
       c->max_lds = std::min<size_t>(kMaxLds, prop.maxSharedMemoryPerMultiProcessor);
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->cus = (uint32_t)prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess ||
+        hipEventCreate(&c->evm) != hipSuccess ||
         hipEventCreate(&c->ev1) != hipSuccess) {
       delete c;
       return fail(FBLS_E_HIP, "stream/event creation failed");
@@ -114,6 +201,7 @@ extern "C" void fbls_ctx_destroy(fbls_ctx* c) {
   c->pin_out.release();
   c->pin_up.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->evm) (void)hipEventDestroy(c->evm);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->up_stream) (void)hipStreamDestroy(c->up_stream);
@@ -437,7 +525,7 @@ uint32_t longest_template(const Work& x) {
 
 int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, const Shared& sh,
                                bool split, SegShape* shapes, size_t* lds_max,
-                               uint32_t* groups) {
+                               uint32_t* groups, bool cap_by_total) {
   const uint32_t ns = sh.ns, layers = sh.layers;
   size_t total = 0;
   for (size_t i = 0; i < nw; i++) total += w[i].np;
@@ -487,7 +575,9 @@ int fbls::host::segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, cons
   }
   if (!c->lds_override && !split && 2 * lds > c->max_lds) lds = c->max_lds;
   // at most total / 1024 periods per workgroup, so the grid fills 256 CUs
-  const uint32_t g_cap = (uint32_t)std::max<size_t>(1, total / 1024);
+  // (without the cap: the largest G the LDS allows, at most 16)
+  const uint32_t g_cap =
+      cap_by_total ? (uint32_t)std::max<size_t>(1, total / 1024) : 16u;
   *lds_max = 0;
   *groups = 0;
   for (size_t i = 0; i < nw; i++) {
@@ -560,15 +650,27 @@ void fbls::host::bank_factors(const Shared& sh, double* factors) {
 }
 
 void fbls::host::account(fbls_scan_stats* st, const Work& x, const fbls_bank* b,
-                         uint64_t budget, double seconds) {
+                         uint64_t budget, double seconds, const unsigned long long* cycles) {
   st->phase_bins = x.nb;
   uint64_t batch = std::min<uint64_t>(std::min<uint64_t>(b->period_batch, budget), x.np);
   if (batch) st->batches += (x.np + batch - 1) / batch;
   st->templates_scanned += (uint64_t)x.np * x.nd * b->ns * x.nb;  // scan.rs:352
   st->exact_tile_retries += x.retries;
   st->seconds_device += seconds;
-  st->seconds_scan += seconds;
+  // FBLS_PROFILE (scan.rs:152,222-229): the kernel's shader cycles per stage
+  // share out the work's device time; unprofiled, the stages stay 0 as in
+  // the reference
+  if (cycles) {
+    const double tot = (double)cycles[0] + (double)cycles[1] + (double)cycles[2];
+    if (tot > 0.0) {
+      st->seconds_fold += seconds * (double)cycles[0] / tot;
+      st->seconds_cumulative += seconds * (double)cycles[1] / tot;
+      st->seconds_scan += seconds * (double)cycles[2] / tot;
+    }
+  }
 }
+
+bool fbls::host::profiling() { return std::getenv("FBLS_PROFILE") != nullptr; }
 
 namespace {
 
@@ -601,7 +703,10 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   for (size_t i = 0; i < nw; i++) nseg += w[i].np ? 1 : 0;
   const size_t seg_off = align16(in_doubles * 8);
   const size_t cnt_off = align16(seg_off + nseg * sizeof(fbls::Segment));
-  const size_t in_bytes = cnt_off + align16(nseg * 8);
+  // counters: per segment two u32, then (profiled) three u64 stage cycles
+  const bool prof = profiling();
+  const size_t cnt_bytes = nseg * 8 + (prof ? nseg * 24 : 0);
+  const size_t in_bytes = cnt_off + align16(cnt_bytes);
   const size_t out_bytes = total * (9 + 2) * 8 + total * 16;
   int st;
   if ((st = c->dev_in.ensure(in_bytes)) || (st = c->dev_out.ensure(out_bytes)) ||
@@ -656,7 +761,7 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
     out_at += x.np;
   }
   std::memcpy(hin + seg_off, segs.data(), segs.size() * sizeof(fbls::Segment));
-  std::memset(hin + cnt_off, 0, align16(nseg * 8));
+  std::memset(hin + cnt_off, 0, align16(cnt_bytes));
   fbls::LaunchArgs A;
   std::memset(&A, 0, sizeof(A));
   A.segs = (const fbls::Segment*)(din + seg_off);
@@ -668,6 +773,7 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
   A.factors = (const double*)(din + fac_at * 8);
   A.gate = sh.gate;
   A.counters = (unsigned int*)(din + cnt_off);
+  if (prof) A.stages = (unsigned long long*)(din + cnt_off + nseg * 8);
   const uint32_t block = launch_block(c, split, lds_max);
   HIP_TRY(hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
@@ -685,7 +791,8 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
     const size_t copy_bytes = any_winner ? out_bytes : win_base;
     HIP_TRY(hipMemcpyAsync(c->pin_out.ptr, dout, copy_bytes, hipMemcpyDeviceToHost, c->stream));
   }
-  HIP_TRY(hipMemcpyAsync(hin + cnt_off, din + cnt_off, nseg * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(hin + cnt_off, din + cnt_off, cnt_bytes, hipMemcpyDeviceToHost,
+                         c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -711,6 +818,12 @@ int run_launch(fbls_ctx* c, Work* w, size_t nw, const Shared& sh, bool split,
       std::memcpy(x.winner, (const uint32_t*)(hout + win_base) + out_at * 4, x.np * 16);
     bad = bad || cnt[2 * s] != 0;
     x.retries = cnt[2 * s + 1];
+    x.profiled = prof;
+    if (prof) {
+      const unsigned long long* cyc =
+          (const unsigned long long*)(hin + cnt_off + nseg * 8) + 3 * s;
+      for (int k = 0; k < 3; k++) x.cycles[k] = cyc[k];
+    }
     out_at += x.np;
   }
   if (bad)
@@ -739,8 +852,8 @@ int run_work(fbls_ctx* c, Work* w, size_t nw, const Shared& sh,
   if (st) return st;
   st = run_launch(c, hi.data(), hi.size(), sh, true, &s2);
   if (st) return st;
-  for (size_t k = 0; k < lo.size(); k++) w[lo_at[k]].retries = lo[k].retries;
-  for (size_t k = 0; k < hi.size(); k++) w[hi_at[k]].retries = hi[k].retries;
+  for (size_t k = 0; k < lo.size(); k++) w[lo_at[k]] = lo[k];
+  for (size_t k = 0; k < hi.size(); k++) w[hi_at[k]] = hi[k];
   *seconds = s1 + s2;
   return FBLS_OK;
 }
@@ -779,31 +892,15 @@ extern "C" int fbls_scan_bank(fbls_ctx* c, const fbls_family* f,
     double secs = 0.0;
     st = run_work(c, &x, 1, shared_of(b), &secs);
     if (st) return st;
-    if (stats) account(stats, x, b, budget, secs);
+    if (stats) account(stats, x, b, budget, secs, x.profiled ? x.cycles : nullptr);
     return FBLS_OK;
   });
 }
 
 namespace {
 
-// Host state of one band (one scan_periods call) across the two launches.
-// A host array that is not zero-filled on allocation (every element the
-// scan reads back is written first).
-struct Raw {
-  std::unique_ptr<double[]> p;
-  size_t n = 0;
-  void resize(size_t m) {
-    if (m > n) {
-      p.reset(new double[m]);
-      n = m;
-    }
-  }
-  double* data() { return p.get(); }
-  const double* data() const { return p.get(); }
-  double& operator[](size_t i) { return p[i]; }
-  const double& operator[](size_t i) const { return p[i]; }
-};
-
+// One band (one reference scan_periods call): its periods (host), phase bins
+// of the full scan and of the prescan (0: none), bank durations and rho.
 struct BandRun {
   const fbls_family* fam;
   const double* periods;
@@ -811,130 +908,388 @@ struct BandRun {
   uint32_t nb, nb_coarse, nd;
   const double* dur;
   const double* rho;
-  Raw rec, score, depth;
-  std::vector<double> survivors;
   fbls_scan_stats stats;
 };
 
-// scan.rs:363-408 for any number of bands of one or more families: every
-// band's first scan (coarse, or full when there is no prescan) in one
-// launch, then every band's full scan of the survivors in a second launch.
+// The launch of one segment table: the Works (their periods on the host),
+// shapes, LDS bytes, workgroup size and workgroups (an upper bound for a
+// persistent table).
+struct Table {
+  std::vector<Work> w;
+  std::vector<size_t> band;  // the active band of each Work
+  std::vector<SegShape> shape;
+  size_t lds = 0;
+  uint32_t groups = 0, block = 0;
+  bool split = false;
+};
+
+size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// scan.rs:363-408 for any number of bands of one or more families, as one
+// stream of launches: every band's first scan (coarse, or full when there is
+// no prescan) in one launch per kernel, the survivors compacted and the full
+// scan's segment tables completed on the device, the full scan of every
+// band's survivors in one persistent launch per kernel, the rows extracted
+// on the device; the host waits once, at the end.
 int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
                fbls_row* rows, size_t* nrows, size_t* row_counts) {
-  const double t0 = now_s();
+  const double t_start = now_s();
   const Shared sh = shared_of(b);
-  std::vector<Work> first, second;
-  std::vector<size_t> first_band, second_band;
+  const uint32_t ns = sh.ns, layers = sh.layers;
+  *nrows = 0;
+  std::vector<size_t> act;
   for (size_t i = 0; i < bands.size(); i++) {
     BandRun& r = bands[i];
     std::memset(&r.stats, 0, sizeof(r.stats));
     r.stats.periods = r.np;
+    if (row_counts) row_counts[i] = 0;
     if (r.np == 0 || r.nd == 0) continue;
     const uint32_t nb0 = r.nb_coarse ? r.nb_coarse : r.nb;
     if (bank_batch(nb0, b->period_batch, b->max_fold_mib) == 0 ||
         bank_batch(r.nb, b->period_batch, b->max_fold_mib) == 0)
       return fail(FBLS_E_FOLD_BUDGET, "one folded period exceeds max_fold_mib");
-    // a prescan reads only the scores; records and depths are the rows of a
-    // band without prescan
-    if (!r.nb_coarse) {
-      r.rec.resize(r.np * 9);
-      r.depth.resize(r.np);
-    }
-    r.score.resize(r.np);
+    if (r.np > 0x7fffffffu) return fail(FBLS_E_ARG, "too many periods in one band");
+    act.push_back(i);
+  }
+  const size_t na = act.size();
+  if (na == 0) return FBLS_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  // tables: [0] first scan LDS-resident, [1] first scan range-split,
+  // [2] full scan LDS-resident, [3] full scan range-split
+  Table T[4];
+  T[1].split = T[3].split = true;
+  size_t np_total = 0, nprescan = 0;
+  for (size_t k = 0; k < na; k++) {
+    const BandRun& r = bands[act[k]];
     Work x;
     x.fam = r.fam;
-    x.periods = r.periods;
+    x.periods = r.periods;  // the host's copy: checks, scales, halo
     x.np = r.np;
-    x.nb = nb0;
+    x.nb = r.nb_coarse ? r.nb_coarse : r.nb;
     x.nd = r.nd;
     x.dur = r.dur;
     x.rho = r.rho;
-    x.records = r.rec.data();
-    x.score = r.score.data();
-    x.depth = r.depth.data();
+    x.records = x.score = x.depth = nullptr;
     x.winner = nullptr;
-    x.score_only = r.nb_coarse != 0;
     x.cost = work_cost(x);
-    first.push_back(x);
-    first_band.push_back(i);
-  }
-  double s1 = 0.0;
-  int st = run_work(c, first.data(), first.size(), sh, &s1);
-  if (st) return st;
-  double c1 = 0.0;
-  for (const Work& x : first) c1 += x.cost;
-  const double cut = b->prescan_fraction * b->threshold;
-  std::vector<Raw> rec2(bands.size()), sc2(bands.size()), dp2(bands.size());
-  for (size_t k = 0; k < first.size(); k++) {
-    BandRun& r = bands[first_band[k]];
-    const Work& x = first[k];
-    account(&r.stats, x, b, bank_batch(x.nb, b->period_batch, b->max_fold_mib),
-            c1 > 0 ? s1 * x.cost / c1 : 0.0);
-    if (!r.nb_coarse) continue;
-    r.survivors.clear();
-    for (size_t i = 0; i < r.np; i++)  // scan.rs:390-395
-      if (std::isfinite(r.score[i]) && r.score[i] >= cut) r.survivors.push_back(r.periods[i]);
-    r.stats.prescan_survivors = r.survivors.size();
-    if (r.survivors.empty()) continue;
-    const size_t m = r.survivors.size();
-    rec2[first_band[k]].resize(m * 9);
-    sc2[first_band[k]].resize(m);
-    dp2[first_band[k]].resize(m);
-    Work y;
-    y.fam = r.fam;
-    y.periods = r.survivors.data();
-    y.np = m;
-    y.nb = r.nb;
-    y.nd = r.nd;
-    y.dur = r.dur;
-    y.rho = r.rho;
-    y.records = rec2[first_band[k]].data();
-    y.score = sc2[first_band[k]].data();
-    y.depth = dp2[first_band[k]].data();
-    y.winner = nullptr;
-    y.cost = work_cost(y);
-    second.push_back(y);
-    second_band.push_back(first_band[k]);
-  }
-  double s2 = 0.0;
-  if (!second.empty()) {
-    st = run_work(c, second.data(), second.size(), sh, &s2);
-    if (st) return st;
-  }
-  double c2 = 0.0;
-  for (const Work& x : second) c2 += x.cost;
-  for (size_t k = 0; k < second.size(); k++) {
-    BandRun& r = bands[second_band[k]];
-    const Work& x = second[k];
-    account(&r.stats, x, b, bank_batch(x.nb, b->period_batch, b->max_fold_mib),
-            c2 > 0 ? s2 * x.cost / c2 : 0.0);
-  }
-  // rows in band order (scan.rs:397-405, 466-467)
-  *nrows = 0;
-  for (size_t i = 0; i < bands.size(); i++) {
-    BandRun& r = bands[i];
-    const bool full_in_first = r.nb_coarse == 0;
-    const size_t m = full_in_first ? (r.nd ? r.np : 0) : r.survivors.size();
-    const double* rec = full_in_first ? r.rec.data() : rec2[i].data();
-    const double* sc = full_in_first ? r.score.data() : sc2[i].data();
-    const double* dp = full_in_first ? r.depth.data() : dp2[i].data();
-    size_t got = 0;
-    for (size_t k = 0; k < m; k++) {
-      if (!(std::isfinite(sc[k]) && sc[k] >= b->threshold)) continue;
-      const double* q = rec + 9 * k;
-      fbls_row& row = rows[(*nrows)++];
-      row.period = q[0];
-      row.epoch = (q[2] + q[3] / 2.0) / q[1] + r.fam->origin;  // scan.rs:345
-      row.duration = q[3] / q[1];
-      row.score = sc[k];
-      row.depth = dp[k];
-      got++;
+    Table& t1 = T[needs_split(c, x, sh) ? 1 : 0];
+    t1.w.push_back(x);
+    t1.band.push_back(k);
+    np_total += r.np;
+    if (r.nb_coarse) {
+      // the full scan's bound: every period of the band survives
+      x.nb = r.nb;
+      Table& t2 = T[needs_split(c, x, sh) ? 3 : 2];
+      t2.w.push_back(x);
+      t2.band.push_back(k);
+      nprescan++;
     }
-    if (row_counts) row_counts[i] = got;
   }
-  // wall time of the two launches, apportioned like the device time
-  const double wall = now_s() - t0;
-  double sdev = s1 + s2;
+  if (np_total > 0x7fffffffu) return fail(FBLS_E_ARG, "too many periods in one launch");
+  for (int t = 0; t < 4; t++) {
+    Table& tb = T[t];
+    if (tb.w.empty()) continue;
+    tb.shape.resize(tb.w.size());
+    if (int st = segment_shapes(c, tb.w.data(), tb.w.size(), sh, tb.split, tb.shape.data(),
+                                &tb.lds, &tb.groups, /*cap_by_total=*/t < 2))
+      return st;
+    tb.block = launch_block(c, tb.split, tb.lds);
+    if (t >= 2) {  // persistent: at most the workgroups the chip holds at once
+      const uint32_t per_cu = (tb.split || tb.block == 1024) ? 1u : 2u;
+      tb.groups = std::min<uint32_t>(tb.groups, std::max<uint32_t>(1, c->cus) * per_cu);
+    }
+  }
+  // ---- staging (one host-to-device copy): bank tables, per-band periods,
+  // scales, durations and rho, the four segment tables, the band table and
+  // the zeroed counters
+  std::vector<size_t> per_at(na), dur_at(na);
+  size_t cur = 0;  // doubles
+  const size_t ing_at = cur;
+  cur += ns;
+  const size_t fac_at = cur;
+  cur += (size_t)ns * layers;
+  for (size_t k = 0; k < na; k++) {
+    const BandRun& r = bands[act[k]];
+    per_at[k] = cur;
+    cur += 2 * r.np;  // periods | first-scan scales
+    dur_at[k] = cur;
+    cur += 2 * (size_t)r.nd;
+  }
+  size_t nseg = 0;
+  for (const Table& tb : T) nseg += tb.w.size();
+  const size_t seg_off = align16(cur * 8);
+  const size_t band_off = align16(seg_off + nseg * sizeof(fbls::Segment));
+  const size_t cnt_off = align16(band_off + na * sizeof(fbls::BandDesc));
+  // counters (u32): counts[2 na + 1] | vgroups[2] | tickets[2] | per segment [2],
+  // then (profiled) per segment three u64 stage-cycle sums
+  const bool prof = profiling();
+  const size_t ncnt = 2 * na + 1 + 4 + 2 * nseg;
+  const size_t stage_at = ((ncnt * 4 + 7) & ~(size_t)7);
+  const size_t cnt_bytes = stage_at + (prof ? nseg * 24 : 0);
+  const size_t in_bytes = cnt_off + align16(cnt_bytes);
+  // device outputs, per band: first scan records | score | depth, then for a
+  // band with prescan survivor periods | scales | full-scan records | score |
+  // depth, then the band's rows; after the bands every row in band order and
+  // the row offsets
+  std::vector<size_t> out_at(na);
+  size_t out_bytes = 0;
+  for (size_t k = 0; k < na; k++) {
+    const BandRun& r = bands[act[k]];
+    out_at[k] = out_bytes;
+    out_bytes += round256(r.np * 88);
+    if (r.nb_coarse) out_bytes += round256(r.np * 104);
+    out_bytes += round256(r.np * sizeof(fbls::RowOut));
+  }
+  const size_t rows_at = out_bytes;
+  out_bytes += round256(np_total * sizeof(fbls::RowOut));
+  const size_t off_at = out_bytes;
+  out_bytes += round256(na * 4);
+  const size_t guess = std::min<size_t>(np_total, std::max<size_t>(c->rows_hint, 1024));
+  const size_t back_bytes = align16(cnt_bytes) + guess * sizeof(fbls::RowOut);
+  int st;
+  if ((st = c->dev_in.ensure(in_bytes)) || (st = c->dev_out.ensure(out_bytes)) ||
+      (st = c->pin_in.ensure(in_bytes)) || (st = c->pin_out.ensure(back_bytes)))
+    return st;
+  unsigned char* hin = (unsigned char*)c->pin_in.ptr;
+  unsigned char* din = (unsigned char*)c->dev_in.ptr;
+  unsigned char* dout = (unsigned char*)c->dev_out.ptr;
+  double* hd = (double*)hin;
+  for (uint32_t s = 0; s < ns; s++) hd[ing_at + s] = sh.ingress[s];
+  bank_factors(sh, hd + fac_at);
+  // periods and scales in pieces of at most 16384 periods on the host threads
+  struct Piece {
+    size_t k, a, e;
+  };
+  std::vector<Piece> pieces;
+  for (size_t k = 0; k < na; k++) {
+    const BandRun& r = bands[act[k]];
+    for (size_t a = 0; a < r.np; a += 16384) pieces.push_back({k, a, std::min(r.np, a + 16384)});
+    double* d = hd + dur_at[k];
+    for (uint32_t i = 0; i < r.nd; i++) {
+      d[i] = r.dur[i];
+      d[r.nd + i] = r.rho[i];
+    }
+  }
+  parallel_ranges(pieces.size(), 2, [&](size_t lo, size_t hi) {
+    for (size_t q = lo; q < hi; q++) {
+      const BandRun& r = bands[act[pieces[q].k]];
+      const uint32_t nb0 = r.nb_coarse ? r.nb_coarse : r.nb;
+      double* p = hd + per_at[pieces[q].k];
+      for (size_t i = pieces[q].a; i < pieces[q].e; i++) {
+        p[i] = r.periods[i];
+        p[r.np + i] = (double)nb0 / r.periods[i];  // scan.rs:204
+      }
+    }
+  });
+  auto dev_d = [&](size_t at_doubles) { return (double*)(din + at_doubles * 8); };
+  uint32_t* dcnt = (uint32_t*)(din + cnt_off);
+  uint32_t* d_counts = dcnt;
+  uint32_t* d_vgroups = dcnt + 2 * na + 1;
+  unsigned int* d_tickets = d_vgroups + 2;
+  unsigned int* d_segcnt = d_tickets + 2;
+  std::vector<fbls::BandDesc> bd(na);
+  for (size_t k = 0; k < na; k++) {
+    const BandRun& r = bands[act[k]];
+    unsigned char* o = dout + out_at[k];
+    fbls::BandDesc& B = bd[k];
+    std::memset(&B, 0, sizeof(B));
+    B.periods = dev_d(per_at[k]);
+    B.rec1 = (const double*)o;
+    B.score1 = (const double*)(o + r.np * 72);
+    B.depth1 = (const double*)(o + r.np * 80);
+    o += round256(r.np * 88);
+    if (r.nb_coarse) {
+      B.surv_p = (double*)o;
+      B.surv_s = (double*)(o + r.np * 8);
+      B.rec2 = (const double*)(o + r.np * 16);
+      B.score2 = (const double*)(o + r.np * 88);
+      B.depth2 = (const double*)(o + r.np * 96);
+      o += round256(r.np * 104);
+    }
+    B.rows = (fbls::RowOut*)o;
+    B.origin = r.fam->origin;
+    B.nb = (double)r.nb;
+    B.np = (uint32_t)r.np;
+    B.prescan = r.nb_coarse ? 1u : 0u;
+  }
+  // segment tables
+  fbls::Segment* hseg = (fbls::Segment*)(hin + seg_off);
+  const fbls::Segment* dseg = (const fbls::Segment*)(din + seg_off);
+  size_t seg_at[4];
+  size_t sbase = 0;
+  for (int t = 0; t < 4; t++) {
+    Table& tb = T[t];
+    seg_at[t] = sbase;
+    uint32_t group0 = 0;
+    for (size_t j = 0; j < tb.w.size(); j++) {
+      const Work& x = tb.w[j];
+      const size_t k = tb.band[j];
+      double smax = 0.0;
+      if (int e = check_periods(x, &smax)) return e;
+      fbls::Segment S;
+      segment_fields(c, x, tb.shape[j], tb.split, smax, &S);
+      const fbls::BandDesc& B = bd[k];
+      S.dur = dev_d(dur_at[k]);
+      S.rho = S.dur + x.nd;
+      S.winner = nullptr;
+      S.index = (uint32_t)j;
+      if (t < 2) {
+        S.periods = B.periods;
+        S.scales = B.periods + x.np;
+        S.records = const_cast<double*>(B.rec1);
+        S.score = const_cast<double*>(B.score1);
+        S.depth = const_cast<double*>(B.depth1);
+        S.group0 = group0;
+        group0 += (uint32_t)((x.np + S.G - 1) / S.G);
+      } else {
+        // np, G (capped) and group0 are completed on the device
+        S.periods = B.surv_p;
+        S.scales = B.surv_s;
+        S.records = const_cast<double*>(B.rec2);
+        S.score = const_cast<double*>(B.score2);
+        S.depth = const_cast<double*>(B.depth2);
+        S.np = 0;
+        S.group0 = 0;
+        bd[k].fine_table = t - 2;
+        bd[k].fine_seg = (uint32_t)j;
+      }
+      hseg[sbase + j] = S;
+    }
+    sbase += tb.w.size();
+  }
+  std::memcpy(hin + band_off, bd.data(), na * sizeof(fbls::BandDesc));
+  std::memset(hin + cnt_off, 0, align16(cnt_bytes));
+  unsigned long long* d_stages = (unsigned long long*)(din + cnt_off + stage_at);
+  auto args_of = [&](int t) {
+    fbls::LaunchArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.segs = dseg + seg_at[t];
+    A.nseg = (uint32_t)T[t].w.size();
+    A.ns = ns;
+    A.layers = layers;
+    A.run_fold = c->run_fold;
+    A.ingress = dev_d(ing_at);
+    A.factors = dev_d(fac_at);
+    A.gate = sh.gate;
+    A.counters = d_segcnt + 2 * seg_at[t];
+    if (prof) A.stages = d_stages + 3 * seg_at[t];
+    if (t >= 2) {
+      A.vgroups = d_vgroups + (t - 2);
+      A.ticket = d_tickets + (t - 2);
+    }
+    return A;
+  };
+  fbls::BandArgs BA;
+  std::memset(&BA, 0, sizeof(BA));
+  BA.bands = (fbls::BandDesc*)(din + band_off);
+  BA.nbands = (uint32_t)na;
+  BA.cut = b->prescan_fraction * b->threshold;
+  BA.threshold = b->threshold;
+  BA.counts = d_counts;
+  BA.row_off = (uint32_t*)(dout + off_at);
+  BA.out = (fbls::RowOut*)(dout + rows_at);
+  for (int t = 0; t < 2; t++) {
+    BA.fine[t].segs = (fbls::Segment*)(din + seg_off) + seg_at[2 + t];
+    BA.fine[t].nseg = (uint32_t)T[2 + t].w.size();
+    BA.fine[t].cap_by_total = t == 0 ? 1u : 0u;
+    BA.fine[t].vgroups = d_vgroups + t;
+    BA.fine[t].ticket = d_tickets + t;
+  }
+  // ---- the stream
+  HIP_TRY(hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  for (int t = 0; t < 2; t++)
+    if (!T[t].w.empty())
+      HIP_TRY(fbls::scan_launch(args_of(t), T[t].groups, T[t].block, T[t].lds, T[t].split,
+                                c->stream));
+  HIP_TRY(hipEventRecord(c->evm, c->stream));
+  if (nprescan) {
+    HIP_TRY(fbls::band_survivors(BA, c->stream));
+    HIP_TRY(fbls::band_plan(BA, c->stream));
+    for (int t = 2; t < 4; t++)
+      if (!T[t].w.empty())
+        HIP_TRY(fbls::scan_launch(args_of(t), T[t].groups, T[t].block, T[t].lds, T[t].split,
+                                  c->stream));
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(fbls::band_rows(BA, c->stream));
+  unsigned char* hback = (unsigned char*)c->pin_out.ptr;
+  const size_t rows_back = align16(cnt_bytes);
+  HIP_TRY(hipMemcpyAsync(hback, din + cnt_off, cnt_bytes, hipMemcpyDeviceToHost, c->stream));
+  if (guess)
+    HIP_TRY(hipMemcpyAsync(hback + rows_back, dout + rows_at, guess * sizeof(fbls::RowOut),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const uint32_t* hc = (const uint32_t*)hback;
+  const size_t total_rows = hc[2 * na];
+  if (total_rows > np_total)
+    return fail(FBLS_E_HIP, "row count %zu exceeds the periods scanned", total_rows);
+  const fbls_row* src = (const fbls_row*)(hback + rows_back);
+  std::vector<fbls_row> more;
+  if (total_rows > guess) {
+    more.resize(total_rows);
+    HIP_TRY(hipMemcpyAsync(more.data(), dout + rows_at, total_rows * sizeof(fbls::RowOut),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    src = more.data();
+    c->rows_hint = total_rows + total_rows / 4;
+  }
+  float ms1 = 0.0f, ms2 = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms1, c->ev0, c->evm));
+  HIP_TRY(hipEventElapsedTime(&ms2, c->evm, c->ev1));
+  const double s1 = ms1 * 1e-3, s2 = ms2 * 1e-3;
+  // ---- statistics (scan.rs:352 and the prescan survivors), device time
+  // apportioned by each pass's work estimate
+  bool bad = false;
+  double c1 = 0.0, c2 = 0.0;
+  std::vector<uint32_t> surv(na, 0);
+  for (size_t k = 0; k < na; k++) surv[k] = bands[act[k]].nb_coarse ? hc[2 * k] : 0u;
+  for (int t = 0; t < 4; t++)
+    for (size_t j = 0; j < T[t].w.size(); j++) {
+      Work& x = T[t].w[j];
+      if (t >= 2) {
+        x.np = surv[T[t].band[j]];
+        x.cost = work_cost(x);
+      }
+      (t < 2 ? c1 : c2) += x.cost;
+    }
+  uint64_t launches = 0;
+  for (int t = 0; t < 4; t++) {
+    if (T[t].w.empty() || (t >= 2 && !nprescan)) continue;
+    launches++;
+    for (size_t j = 0; j < T[t].w.size(); j++) {
+      Work& x = T[t].w[j];
+      const uint32_t* cnt = hc + 2 * na + 1 + 4 + 2 * (seg_at[t] + j);
+      bad = bad || cnt[0] != 0;
+      x.retries = cnt[1];
+      // a band without survivors has no full scan (its phase_bins stay the
+      // prescan's, scan.rs:397-398)
+      if (t >= 2 && x.np == 0) continue;
+      BandRun& r = bands[act[T[t].band[j]]];
+      const double secs = t < 2 ? (c1 > 0 ? s1 * x.cost / c1 : 0.0)
+                                : (c2 > 0 ? s2 * x.cost / c2 : 0.0);
+      const unsigned long long* cyc =
+          prof ? (const unsigned long long*)(hback + stage_at) + 3 * (seg_at[t] + j) : nullptr;
+      account(&r.stats, x, b, bank_batch(x.nb, b->period_batch, b->max_fold_mib), secs, cyc);
+      c->totals.periods += x.np;
+      c->totals.fold_pairs += (uint64_t)x.np * x.fam->n;
+    }
+  }
+  c->totals.launches += launches;
+  c->totals.seconds += s1 + s2;
+  if (bad)
+    return fail(FBLS_E_NONPOSITIVE_PRECISION,
+                "nonpositive template precision; inspect noise model");
+  for (size_t k = 0; k < na; k++) {
+    BandRun& r = bands[act[k]];
+    if (r.nb_coarse) r.stats.prescan_survivors = surv[k];
+    if (row_counts) row_counts[act[k]] = hc[2 * k + 1];
+  }
+  std::memcpy(rows, src, total_rows * sizeof(fbls_row));
+  *nrows = total_rows;
+  // wall time of the call, apportioned like the device time
+  const double wall = now_s() - t_start;
+  const double sdev = s1 + s2;
   for (BandRun& r : bands)
     r.stats.seconds = sdev > 0 ? wall * r.stats.seconds_device / sdev : 0.0;
   return FBLS_OK;
@@ -999,7 +1354,9 @@ extern "C" int fbls_period_grid(double minimum, double maximum,
                                 double oversample, double* out, size_t count) {
   return fbls::host::guard([&]() -> int {
     if (!out || count == 0) return fail(FBLS_E_ARG, "empty grid");
-    fbls::period_grid(minimum, maximum, out, count);
+    parallel_ranges(count, 8192, [&](size_t a, size_t e) {
+      fbls::period_grid_range(minimum, maximum, out, count, a, e);
+    });
     return FBLS_OK;
   });
 }
@@ -1149,7 +1506,10 @@ int plan_search(const fbls_family* f, const double* durations, const double* rho
     return FBLS_OK;
   }
   p->grid.resize(count);
-  fbls::period_grid(minimum, maximum, p->grid.data(), count);
+  double* g = p->grid.data();
+  parallel_ranges(count, 8192, [&](size_t a, size_t e) {
+    fbls::period_grid_range(minimum, maximum, g, count, a, e);
+  });
   p->bands = fbls::plan_bands(p->grid.data(), count, p->sd.data(), nd,
                               bank->phase_oversample, bank->prescan_factor);
   return FBLS_OK;

@@ -122,7 +122,9 @@ constexpr uint32_t kObsAlign = 512;     // observations per wave block
 struct fbls_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
+  uint32_t cus = 256;       // compute units (persistent launch grids)
+  size_t rows_hint = 4096;  // rows copied back with a search's counters
   uint32_t lds_override = 0, block_override = 0;
   size_t max_lds = fbls::host::kMaxLds;
   uint32_t run_fold = 1;

@@ -66,12 +66,17 @@ size_t period_grid_count(double minimum, double maximum, double baseline,
 }
 
 void period_grid(double minimum, double maximum, double* out, size_t count) {
+  period_grid_range(minimum, maximum, out, count, 0, count);
+}
+
+void period_grid_range(double minimum, double maximum, double* out, size_t count, size_t i0,
+                       size_t i1) {
   // grid.rs:14-26
   const double a = std::log(minimum), b = std::log(maximum);
-  for (size_t i = 0; i < count; i++)
+  for (size_t i = i0; i < i1; i++)
     out[i] = std::exp(count == 1 ? a : a + (b - a) * (double)i / (double)(count - 1));
-  out[0] = minimum;
-  out[count - 1] = maximum;
+  if (i0 == 0 && i1 > 0) out[0] = minimum;
+  if (i0 < count && i1 == count) out[count - 1] = maximum;
 }
 
 GridFormula::GridFormula(double mn, double mx, size_t n)

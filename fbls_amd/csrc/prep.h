@@ -39,6 +39,9 @@ inline double fixed_value(int32_t word, bool block) {
 size_t period_grid_count(double minimum, double maximum, double baseline,
                          double minimum_duration, double oversample);
 void period_grid(double minimum, double maximum, double* out, size_t count);
+// Elements [i0, i1) of the same grid (each element is computed alone).
+void period_grid_range(double minimum, double maximum, double* out, size_t count, size_t i0,
+                       size_t i1);
 // Element i of period_grid(minimum, maximum, ., count), evaluated alone.
 struct GridFormula {
   double minimum = 0.0, maximum = 0.0, a = 0.0, b = 0.0;

@@ -26,6 +26,8 @@ struct Work {
   uint32_t retries = 0;
   double cost = 0.0;  // work estimate used to apportion launch time
   bool score_only = false;  // the caller reads only score (records, depth stay on the device)
+  bool profiled = false;             // cycles hold a profiled launch's stage cycles
+  unsigned long long cycles[3] = {0, 0, 0};  // fold, prefix sums, scan and merge
 };
 
 // The bank's shape fields, shared by every segment of a launch.
@@ -55,8 +57,11 @@ bool needs_split(const fbls_ctx* c, const Work& x, const Shared& sh);
 int check_periods(const Work& x, double* smax);
 // The geometry run_launch gives the segments of one launch: shapes[i] for
 // w[i] (nw entries), the launch's LDS bytes and workgroup count.
+// cap_by_total: at most max(1, total periods / 1024) periods per workgroup
+// (off for a table whose periods are counted on the device).
 int segment_shapes(const fbls_ctx* c, const Work* w, size_t nw, const Shared& sh, bool split,
-                   SegShape* shapes, size_t* lds_max, uint32_t* groups);
+                   SegShape* shapes, size_t* lds_max, uint32_t* groups,
+                   bool cap_by_total = true);
 // Workgroup size of a launch whose workgroups hold `lds` bytes of LDS.
 uint32_t launch_block(const fbls_ctx* c, bool split, size_t lds);
 // Every Segment field that does not point into the launch's staging
@@ -65,9 +70,14 @@ void segment_fields(const fbls_ctx* c, const Work& x, const SegShape& g, bool sp
                     double smax, fbls::Segment* S);
 // Staircase width factors (scan.rs:157-171): factors[s * layers + l].
 void bank_factors(const Shared& sh, double* factors);
-// Band statistics of one Work's scan (scan.rs:352 for templates_scanned).
+// Band statistics of one Work's scan (scan.rs:352 for templates_scanned);
+// cycles (or null): the Work's fold / prefix / scan shader cycles of a
+// profiled launch, which share out `seconds` into the stage fields.
 void account(fbls_scan_stats* st, const Work& x, const fbls_bank* b, uint64_t budget,
-             double seconds);
+             double seconds, const unsigned long long* cycles = nullptr);
+// FBLS_PROFILE set in the environment (scan.rs:152): launches record stage
+// cycles.
+bool profiling();
 
 // Resolves the device address of the allocation-free entry point's
 // scratch array on the current device (once per device); called by

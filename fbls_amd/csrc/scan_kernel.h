@@ -44,6 +44,17 @@ struct LaunchArgs {
   const double* factors;  // ns x layers staircase width factors
   double gate;            // shape gate on the exact merit
   unsigned int* counters; // per segment: nonpositive-precision flag, exact tile rescans
+  // Persistent launch (both non-null): a fixed grid whose workgroups take
+  // the launch's groups in order from *ticket (zero when the launch starts)
+  // until *vgroups are taken; the segment table and *vgroups may have been
+  // written on the device by an earlier kernel of the stream.  Null: one
+  // group per workgroup of the grid.
+  const uint32_t* vgroups;
+  unsigned int* ticket;
+  // Stage attribution (the reference's FBLS_PROFILE): per segment three
+  // shader-clock sums over its workgroups -- fold, prefix sums, template
+  // scan and merge.  Null: not recorded.
+  unsigned long long* stages;
 };
 
 // LDS bytes of one workgroup holding G periods of nb bins.

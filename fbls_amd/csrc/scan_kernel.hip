@@ -27,9 +27,11 @@
 //                        (kernels.rs:154-198, 1134-1143)
 //   winner recompute     src/search/kernels.rs:800-843
 //
-// LDS per period: vw[nb+1] {v, w} double pairs (16 B) then n[nb+1] u32;
-// slot b+1 accumulates bin b, so after the in-place scan slot x holds the
-// exclusive prefix E[x] = sum of bins < x.
+// LDS per period: vw[nb+1+kOver] {v, w} double pairs (16 B) then
+// n[nb+1+kOver] u32; slot b+1 accumulates bin b (slots past nb are the
+// fold's overflow slots, added back into the first bins before the scan),
+// so after the in-place scan slot x holds the exclusive prefix E[x] = sum of
+// bins < x.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -43,6 +45,10 @@ namespace {
 constexpr uint32_t kTile = 128;       // gate granularity (kernels.rs:26)
 constexpr uint32_t kObsPerLane = 8;   // contiguous observations per lane
 constexpr uint32_t kObsPerWave = 64 * kObsPerLane;
+// Overflow slots after bin nb - 1 of a period's LDS histogram: the
+// incremental fold may address up to kOver bins past the end (bins 0 ..
+// kOver - 1 wrapped once); they are added back after the fold.
+constexpr uint32_t kOver = 16;
 constexpr double kInf = __builtin_inf();
 constexpr double kNear = 1e-10;       // near-tie window of the surrogate
 
@@ -372,62 +378,159 @@ __device__ __forceinline__ uint32_t count_of(double wsum, double w0) {
   return (uint32_t)(wsum / w0);
 }
 
+// One period's histogram in LDS, addressed in bytes from the LDS base:
+// slot s (bin s - 1; slots nb + 1 .. nb + kOver are the overflow slots of
+// bins 0 .. kOver - 1) holds {v, w} at vo + 16 s and its count at no + 4 s.
+struct HistAt {
+  uint32_t vo, no;
+};
+
+
+// One run of observations added to a slot: v (and, unless the weights are
+// equal, w) as FP64 atomics and the count as a u32 atomic.  UNI (equal
+// fixed-point weights w0): a slot's weight sum is count * w0, rebuilt after
+// the fold, so only v and the count are folded.
+template <bool UNI>
+__device__ __forceinline__ void add_run(unsigned char* lds, uint32_t va, uint32_t na, double v,
+                                        double w, uint32_t c) {
+  double* p = (double*)(lds + va);
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (!UNI) __hip_atomic_fetch_add(p + 1, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_add((uint32_t*)(lds + na), c, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The observations R0 .. R0 + LEN - 1 of a lane whose cells lie in
+// u[R0] .. u[R0] + S (S wave-uniform and small): one atomic set per cell
+// rather than one per observation.  Times ascend, so the observations below
+// cell u[R0] + k are a prefix of the span: the prefix sum at the last of
+// them is picked by selects, and the run of cell u[R0] + k - 1 is the
+// difference of two such prefixes.  A lane whose cells end before a cell
+// adds nothing there.  Slot addresses: cell * 16 + cv, cell * 4 + cn.
+template <uint32_t S, uint32_t R0, uint32_t LEN, bool UNI>
+__device__ __forceinline__ void fold_cells(unsigned char* lds, uint32_t cv, uint32_t cn,
+                                           const uint32_t* u, const double* pv,
+                                           const double* pw) {
+  constexpr uint32_t L = R0 + LEN - 1;
+  double lv = R0 ? pv[R0 - 1] : 0.0;
+  double lw = (!UNI && R0) ? pw[R0 - 1] : 0.0;
+  uint32_t lr = R0;
+#pragma unroll
+  for (uint32_t k = 1; k <= S; k++) {
+    const uint32_t ck = u[R0] + k;
+    double sv = pv[R0], sw = UNI ? 0.0 : pw[R0];
+    uint32_t sr = R0 + 1;
+#pragma unroll
+    for (uint32_t r = R0 + 1; r <= L; r++) {
+      const bool below = u[r] < ck;
+      sv = below ? pv[r] : sv;
+      if (!UNI) sw = below ? pw[r] : sw;
+      sr = below ? r + 1 : sr;
+      // keeps the chain as register selects (the compiler would otherwise
+      // turn it into an indexed load from a copy of pv in scratch memory)
+      asm("" : "+v"(sv));
+      if (!UNI) asm("" : "+v"(sw));
+    }
+    if (ck - 1 <= u[L])
+      add_run<UNI>(lds, (ck - 1) * 16u + cv, (ck - 1) * 4u + cn, sv - lv, sw - lw, sr - lr);
+    lv = sv;
+    lw = sw;
+    lr = sr;
+  }
+  const uint32_t cl = u[R0] + S;
+  if (cl <= u[L])
+    add_run<UNI>(lds, cl * 16u + cv, cl * 4u + cn, pv[L] - lv, UNI ? 0.0 : pw[L] - lw,
+                 L + 1 - lr);
+}
+
+// The same observations with one flush per observation position: a run ends
+// wherever the next cell differs.
+template <uint32_t R0, uint32_t LEN, bool UNI>
+__device__ __forceinline__ void fold_positions(unsigned char* lds, uint32_t cv, uint32_t cn,
+                                               const uint32_t* u, const double* pv,
+                                               const double* pw) {
+  constexpr uint32_t L = R0 + LEN - 1;
+  double lv = R0 ? pv[R0 - 1] : 0.0;
+  double lw = (!UNI && R0) ? pw[R0 - 1] : 0.0;
+  uint32_t lr = R0;
+#pragma unroll
+  for (uint32_t r = R0; r < L; r++) {
+    if (u[r] != u[r + 1]) {
+      add_run<UNI>(lds, u[r] * 16u + cv, u[r] * 4u + cn, pv[r] - lv, UNI ? 0.0 : pw[r] - lw,
+                   r + 1 - lr);
+      lv = pv[r];
+      if (!UNI) lw = pw[r];
+      lr = r + 1;
+    }
+  }
+  add_run<UNI>(lds, u[L] * 16u + cv, u[L] * 4u + cn, pv[L] - lv, UNI ? 0.0 : pw[L] - lw,
+               L + 1 - lr);
+}
+
+// A lane's 8 observations in the cheapest form its wave allows: one atomic
+// set per cell when every lane's cells span at most 3 cells, else one flush
+// per observation position.
+template <bool UNI>
+__device__ __forceinline__ void fold_runs(unsigned char* lds, uint32_t cv, uint32_t cn,
+                                          const uint32_t* u, const double* pv,
+                                          const double* pw) {
+  const uint32_t span = u[kObsPerLane - 1] - u[0];
+  if (__all(span <= 1))
+    fold_cells<1, 0, kObsPerLane, UNI>(lds, cv, cn, u, pv, pw);
+  else if (__all(span <= 2))
+    fold_cells<2, 0, kObsPerLane, UNI>(lds, cv, cn, u, pv, pw);
+  else
+    fold_positions<0, kObsPerLane, UNI>(lds, cv, cn, u, pv, pw);
+}
+
 // Fold one lane's 8 consecutive observations into one period's histogram.
-// RUN: consecutive observations falling in the same bin are summed in
-// registers and flushed once (exact: integer-valued doubles).  INCR (times
-// ascend and the lane's 8 cells span fewer than nb): each bin is the previous
-// one advanced by the difference of floor(t * scale), wrapped once.
-template <bool WIDE, bool RUN, bool FULL, bool NOCNT, bool INCR>
-__device__ __forceinline__ void fold_lane(double* vw, uint32_t* n,
-                                          const uint32_t* u, const double* vv,
-                                          const double* ww, uint32_t nb,
-                                          float inv_lo, uint32_t base,
-                                          uint32_t nobs) {
+// pv, pw: the lane's inclusive prefix sums of v and w (exact integers), so a
+// run of observations r0..r in one bin adds pv[r] - pv[r0 - 1].
+// INCR (times ascend, the lane's 8 cells span fewer than min(kOver, nb)):
+// cell u lands in slot u - off + 1 with off = u[0] - bin(u[0]), a slot below
+// nb + 1 + kOver: the bins past nb - 1 are the overflow slots, added back
+// into bins 0 .. kOver - 1 after the fold, so a run's address is one
+// shift-add of its cell.  RUN: runs of equal bins flushed once.  Otherwise
+// one atomic set per observation.  FULL: all 8 observations are real.
+template <bool WIDE, bool RUN, bool FULL, bool UNI, bool INCR>
+__device__ __forceinline__ void fold_lane(unsigned char* lds, HistAt h, const uint32_t* u,
+                                          const double* pv, const double* pw, uint32_t nb,
+                                          float inv_lo, uint32_t base, uint32_t nobs) {
   if (RUN && FULL && INCR) {
-    // bin r = (b0 + u[r] - u[0]) mod nb, at most one wrap: u[r] - off with
-    // off = u[0] - b0, then a conditional subtraction
-    const uint32_t b0 = phase_bin<WIDE>(u[0], nb, inv_lo);
-    const uint32_t off = u[0] - b0;
-    uint32_t cb = b0;
-    double av = vv[0], aw = ww[0];
-    uint32_t an = 1;
-#pragma unroll
-    for (uint32_t r = 1; r < kObsPerLane; r++) {
-      uint32_t b = u[r] - off;
-      b = min(b, b - nb);
-      const bool flush = u[r] != u[r - 1];
-      if (flush) lds_add<NOCNT>(vw, n, cb, av, aw, an);
-      // a run restarts at a flush: av * 0 + v or av * 1 + v, exact for the
-      // integer-valued sums
-      const double keep = flush ? 0.0 : 1.0;
-      av = __builtin_fma(av, keep, vv[r]);
-      aw = __builtin_fma(aw, keep, ww[r]);
-      if (!NOCNT) an = flush ? 1u : an + 1u;
-      cb = b;
-    }
-    lds_add<NOCNT>(vw, n, cb, av, aw, an);
+    const uint32_t off = u[0] - phase_bin<WIDE>(u[0], nb, inv_lo);
+    // slot addresses u * 16 + cv (and u * 4 + cn), modulo 2^32; the empty
+    // asm keeps each base in one register (otherwise the compiler expands
+    // it again at every add)
+    uint32_t cv = h.vo + 16u - off * 16u;
+    uint32_t cn = h.no + 4u - off * 4u;
+    asm("" : "+v"(cv), "+v"(cn));
+    fold_runs<UNI>(lds, cv, cn, u, pv, pw);
   } else if (RUN && FULL) {
-    uint32_t cb = phase_bin<WIDE>(u[0], nb, inv_lo);
-    double av = vv[0], aw = ww[0];
-    uint32_t an = 1;
+    uint32_t b[kObsPerLane];
 #pragma unroll
-    for (uint32_t r = 1; r < kObsPerLane; r++) {
-      const uint32_t b = phase_bin<WIDE>(u[r], nb, inv_lo);
-      const bool flush = b != cb;
-      if (flush) lds_add<NOCNT>(vw, n, cb, av, aw, an);
-      const double sv = av + vv[r];
-      av = flush ? vv[r] : sv;
-      const double sw = aw + ww[r];
-      aw = flush ? ww[r] : sw;
-      if (!NOCNT) an = flush ? 1u : an + 1u;
-      cb = b;
+    for (uint32_t r = 0; r < kObsPerLane; r++) b[r] = phase_bin<WIDE>(u[r], nb, inv_lo);
+    double lv = 0.0, lw = 0.0;
+    uint32_t lr = 0;
+#pragma unroll
+    for (uint32_t r = 0; r + 1 < kObsPerLane; r++) {
+      if (b[r] != b[r + 1]) {
+        add_run<UNI>(lds, h.vo + 16u * (b[r] + 1), h.no + 4u * (b[r] + 1), pv[r] - lv,
+                     UNI ? 0.0 : pw[r] - lw, r + 1 - lr);
+        lv = pv[r];
+        if (!UNI) lw = pw[r];
+        lr = r + 1;
+      }
     }
-    lds_add<NOCNT>(vw, n, cb, av, aw, an);
+    constexpr uint32_t L = kObsPerLane - 1;
+    add_run<UNI>(lds, h.vo + 16u * (b[L] + 1), h.no + 4u * (b[L] + 1), pv[L] - lv,
+                 UNI ? 0.0 : pw[L] - lw, kObsPerLane - lr);
   } else {
 #pragma unroll
     for (uint32_t r = 0; r < kObsPerLane; r++) {
-      const uint32_t b = phase_bin<WIDE>(u[r], nb, inv_lo);
-      if (FULL || base + r < nobs) lds_add<NOCNT>(vw, n, b, vv[r], ww[r], 1u);
+      const uint32_t bb = phase_bin<WIDE>(u[r], nb, inv_lo) + 1;
+      const double v = r ? pv[r] - pv[r - 1] : pv[0];
+      const double w = UNI ? 0.0 : (r ? pw[r] - pw[r - 1] : pw[0]);
+      if (FULL || base + r < nobs) add_run<UNI>(lds, h.vo + 16u * bb, h.no + 4u * bb, v, w, 1u);
     }
   }
 }
@@ -463,12 +566,13 @@ __device__ __forceinline__ float inv_nb_low(uint32_t nb) {
   return (float)((1.0 / (double)nb) * (1.0 - 0x1p-22));
 }
 
-template <bool WIDE, bool RUN, bool NOCNT>
-__device__ void fold_all(const Segment& S, unsigned char* hist,
+template <bool WIDE, bool RUN, bool UNI>
+__device__ void fold_all(const Segment& S, unsigned char* lds, uint32_t hist_off,
                          uint32_t p0, uint32_t gcount, uint32_t wave,
                          uint32_t lane, uint32_t nwaves) {
   const uint32_t nb = S.nb;
-  const uint32_t nb1 = nb + 1;
+  const uint32_t nsl = nb + 1 + kOver;
+  const uint32_t span = min(kOver, nb);
   const float inv_lo = inv_nb_low(nb);
   const uint32_t nblk = S.n_pad / kObsPerWave;
   // Lane l holds the scale of period p0 + l (gcount <= 16), loaded once; a
@@ -479,24 +583,30 @@ __device__ void fold_all(const Segment& S, unsigned char* hist,
   const int my_lo = (int)(uint32_t)my_bits, my_hi = (int)(uint32_t)(my_bits >> 32);
   for (uint32_t blk = wave; blk < nblk; blk += nwaves) {
     const uint32_t base = blk * kObsPerWave + lane * kObsPerLane;
-    double tt[kObsPerLane], vv[kObsPerLane], ww[kObsPerLane];
-    load_obs(S, base, tt, vv, ww);
+    double tt[kObsPerLane], pv[kObsPerLane], pw[kObsPerLane];
+    load_obs(S, base, tt, pv, pw);
+#pragma unroll
+    for (uint32_t r = 1; r < kObsPerLane; r++) {  // inclusive prefix, exact
+      pv[r] += pv[r - 1];
+      if (!UNI) pw[r] += pw[r - 1];
+    }
     const bool full = blk * kObsPerWave + kObsPerWave <= S.n;
     for (uint32_t g = 0; g < gcount; g++) {
       const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(my_lo, (int)g);
       const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(my_hi, (int)g);
       const double scale = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-      double* vw = (double*)(hist + (size_t)g * S.pstride);
-      uint32_t* n = (uint32_t*)(vw + 2 * nb1);
+      HistAt h;
+      h.vo = hist_off + g * S.pstride;
+      h.no = h.vo + 16u * nsl;
       uint32_t u[kObsPerLane];
 #pragma unroll
       for (uint32_t r = 0; r < kObsPerLane; r++) u[r] = phase_cell_u(tt[r], scale);
       if (!full) {
-        fold_lane<WIDE, RUN, false, NOCNT, false>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
-      } else if (RUN && S.sorted && __all(u[kObsPerLane - 1] - u[0] < nb)) {
-        fold_lane<WIDE, RUN, true, NOCNT, true>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
+        fold_lane<WIDE, RUN, false, UNI, false>(lds, h, u, pv, pw, nb, inv_lo, base, S.n);
+      } else if (RUN && S.sorted && __all(u[kObsPerLane - 1] - u[0] < span)) {
+        fold_lane<WIDE, RUN, true, UNI, true>(lds, h, u, pv, pw, nb, inv_lo, base, S.n);
       } else {
-        fold_lane<WIDE, RUN, true, NOCNT, false>(vw, n, u, vv, ww, nb, inv_lo, base, S.n);
+        fold_lane<WIDE, RUN, true, UNI, false>(lds, h, u, pv, pw, nb, inv_lo, base, S.n);
       }
     }
   }
@@ -859,22 +969,46 @@ __device__ void write_record(const Segment& S, uint32_t f, uint64_t key,
 
 // ------------------------------------------------- LDS-resident kernel
 
-template <uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-void fused_scan(LaunchArgs A) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+// Stage clock of a profiled launch: thread 0 adds the shader-clock cycles
+// since its previous mark to the segment's stage k (0 fold, 1 prefix sums,
+// 2 template scan and merge); every mark follows a barrier, so the cycles
+// are the workgroup's.
+// PROF false: no clock (the kernels without it keep their registers).
+template <bool PROF>
+struct StageClock {
+  unsigned long long* acc;
+  uint64_t last;
+  __device__ __forceinline__ explicit StageClock(unsigned long long* a) : acc(a), last(0) {
+    if (PROF && threadIdx.x == 0) last = clock64();
+  }
+  __device__ __forceinline__ void mark(uint32_t k) {
+    if (PROF && threadIdx.x == 0) {
+      const uint64_t now = clock64();
+      atomicAdd(acc + k, (unsigned long long)(now - last));
+      last = now;
+    }
+  }
+};
+
+// One group of the launch: G periods of one segment, from the fold to the
+// records.  LDS: the workgroup's dynamic LDS.
+template <uint32_t BLOCK, bool PROF>
+__device__ __forceinline__ void scan_group(const LaunchArgs& A, uint32_t group,
+                                           unsigned char* lds) {
   constexpr uint32_t kWaves = BLOCK / 64;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
   const uint32_t wave = tid >> 6;
-  const Segment S = A.segs[find_segment(A, blockIdx.x)];
+  const Segment S = A.segs[find_segment(A, group)];
   const uint32_t nb = S.nb;
   const uint32_t nb1 = nb + 1;
-  const uint32_t p0 = (blockIdx.x - S.group0) * S.G;
+  const uint32_t slots = nb1 + kOver;  // {v, w} and count slots per period
+  const uint32_t p0 = (group - S.group0) * S.G;
   const uint32_t gcount = min(S.G, S.np - p0);
   const uint32_t nd = S.nd, ns = A.ns, layers = A.layers;
   const uint32_t tiles = (nb + kTile - 1) / kTile;
   const uint32_t nsl = (ns - 1) * (layers - 1);
+  StageClock<PROF> clk(PROF ? A.stages + 3 * (size_t)S.index : nullptr);
 
   unsigned char* p = lds;
   unsigned char* hist = p;
@@ -901,38 +1035,54 @@ void fused_scan(LaunchArgs A) {
   __syncthreads();
 
   // ---- phase 1: fold every observation into the G histograms
-  const bool nocnt = A.run_fold && S.uniform_w > 0;
+  const bool uni = A.run_fold && S.uniform_w > 0;
+  const uint32_t hist_off = (uint32_t)(hist - lds);
   if (S.wide_cells) {
-    if (!A.run_fold) fold_all<true, false, false>(S, hist, p0, gcount, wave, lane, kWaves);
-    else if (nocnt) fold_all<true, true, true>(S, hist, p0, gcount, wave, lane, kWaves);
-    else fold_all<true, true, false>(S, hist, p0, gcount, wave, lane, kWaves);
+    if (!A.run_fold) fold_all<true, false, false>(S, lds, hist_off, p0, gcount, wave, lane, kWaves);
+    else if (uni) fold_all<true, true, true>(S, lds, hist_off, p0, gcount, wave, lane, kWaves);
+    else fold_all<true, true, false>(S, lds, hist_off, p0, gcount, wave, lane, kWaves);
   } else {
-    if (!A.run_fold) fold_all<false, false, false>(S, hist, p0, gcount, wave, lane, kWaves);
-    else if (nocnt) fold_all<false, true, true>(S, hist, p0, gcount, wave, lane, kWaves);
-    else fold_all<false, true, false>(S, hist, p0, gcount, wave, lane, kWaves);
+    if (!A.run_fold) fold_all<false, false, false>(S, lds, hist_off, p0, gcount, wave, lane, kWaves);
+    else if (uni) fold_all<false, true, true>(S, lds, hist_off, p0, gcount, wave, lane, kWaves);
+    else fold_all<false, true, false>(S, lds, hist_off, p0, gcount, wave, lane, kWaves);
   }
   __syncthreads();
-  if (nocnt) {  // counts from the weight sums
+  {  // the overflow slots back into bins 0 .. min(kOver, nb) - 1 (exact)
+    const uint32_t wrap = min(kOver, nb);
+    for (uint32_t i = tid; i < gcount * wrap; i += BLOCK) {
+      const uint32_t g = i / wrap, x = i - g * wrap;
+      double2* vw = (double2*)(hist + (size_t)g * S.pstride);
+      uint32_t* n = (uint32_t*)(vw + slots);
+      const double2 o = vw[nb1 + x];
+      const double2 e = vw[1 + x];
+      vw[1 + x] = make_double2(e.x + o.x, e.y + o.y);
+      n[1 + x] += n[nb1 + x];
+    }
+  }
+  __syncthreads();
+  if (uni) {  // equal weights: each bin's weight sum is its count times w0 (exact)
     const double w0 = (double)S.uniform_w;
     for (uint32_t i = tid; i < gcount * nb; i += BLOCK) {
       const uint32_t g = i / nb, b = i - g * nb;
-      const double* vw = (const double*)(hist + (size_t)g * S.pstride);
-      uint32_t* n = (uint32_t*)(vw + 2 * nb1);
-      n[b + 1] = count_of(vw[2 * (b + 1) + 1], w0);
+      double* vw = (double*)(hist + (size_t)g * S.pstride);
+      const uint32_t* n = (const uint32_t*)(vw + 2 * slots);
+      vw[2 * (b + 1) + 1] = (double)n[b + 1] * w0;
     }
     __syncthreads();
   }
+  clk.mark(0);
 
   // ---- phase 2: exclusive prefix sums in place (exact)
   for (uint32_t item = wave; item < 2 * gcount; item += kWaves) {
     uint32_t g = item >> 1;
     double2* vw = (double2*)(hist + (size_t)g * S.pstride);
     if (item & 1)
-      wave_prefix_n((uint32_t*)(vw + nb1), nb, lane);
+      wave_prefix_n((uint32_t*)(vw + slots), nb, lane);
     else
       wave_prefix_vw(vw, nb, lane, S.sv, S.sw);
   }
   __syncthreads();
+  clk.mark(1);
 
   // ---- phase 3: template scan, one 128-bin tile per wave at a time
   TileCtx k;
@@ -953,7 +1103,7 @@ void fused_scan(LaunchArgs A) {
     const uint32_t g = item / tiles, tile = item % tiles;
     WrapCum cm;
     cm.vw = (const double2*)(hist + (size_t)g * S.pstride);
-    cm.n = (const uint32_t*)(cm.vw + nb1);
+    cm.n = (const uint32_t*)(cm.vw + slots);
     cm.nb = nb;
     const double2 t = cm.vw[nb];
     cm.tv = t.x;
@@ -993,7 +1143,7 @@ void fused_scan(LaunchArgs A) {
     }
     WrapCum cm;
     cm.vw = (const double2*)(hist + (size_t)g * S.pstride);
-    cm.n = (const uint32_t*)(cm.vw + nb1);
+    cm.n = (const uint32_t*)(cm.vw + slots);
     cm.nb = nb;
     cm.tv = cm.vw[nb].x;
     cm.tw = cm.vw[nb].y;
@@ -1009,23 +1159,59 @@ void fused_scan(LaunchArgs A) {
                       A.ingress[sic], A.factors, sic, layers, &a0, &hp, &norm);
     write_record(S, p0 + g, key, bdi, valid, ns, A.ingress, a0, hp, norm);
   }
+  clk.mark(2);
+}
+
+// Runs body(group) for the launch's groups.  PERSIST: a fixed grid whose
+// workgroups take every group from the ticket counter (the ticket is passed
+// through the first word of the LDS; body may use all of the LDS, so each
+// turn ends at a barrier).  Otherwise one group per workgroup.  The two are
+// separate kernels: the loop raises register pressure, and the launches
+// whose groups the host knows keep the loop-free body.
+template <bool PERSIST, class Body>
+__device__ __forceinline__ void for_each_group(const LaunchArgs& A, unsigned char* lds,
+                                               Body body) {
+  if (!PERSIST) {
+    body(A, blockIdx.x);
+    return;
+  }
+  const uint32_t total = *A.vgroups;
+  volatile uint32_t* slot = (volatile uint32_t*)lds;
+  for (;;) {
+    if (threadIdx.x == 0) *slot = atomicAdd(A.ticket, 1u);
+    __syncthreads();
+    const uint32_t group = __builtin_amdgcn_readfirstlane(*slot);
+    __syncthreads();
+    if (group >= total) break;
+    body(A, group);
+    __syncthreads();
+  }
+}
+
+template <uint32_t BLOCK, bool PERSIST, bool PROF>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+void fused_scan(LaunchArgs A) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  for_each_group<PERSIST>(A, lds, [&](const LaunchArgs& a, uint32_t group) {
+    scan_group<BLOCK, PROF>(a, group, lds);
+  });
 }
 
 // ----------------------------------------------------- range-split kernel
 
 // One period per workgroup; the bins are processed in ranges of
 // S.range_bins with a halo of S.halo_bins (>= the longest template).
-template <uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+template <uint32_t BLOCK, bool PROF>
+__device__ __forceinline__ void split_group(const LaunchArgs& A, uint32_t group,
+                                            unsigned char* lds) {
   constexpr uint32_t kWaves = BLOCK / 64;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
   const uint32_t wave = tid >> 6;
-  const Segment S = A.segs[find_segment(A, blockIdx.x)];
+  const Segment S = A.segs[find_segment(A, group)];
   const uint32_t nb = S.nb;
   const uint32_t nb1 = nb + 1;
-  const uint32_t f = blockIdx.x - S.group0;  // G = 1
+  const uint32_t f = group - S.group0;  // G = 1
   const uint32_t nd = S.nd, ns = A.ns, layers = A.layers;
   const uint32_t tiles = (nb + kTile - 1) / kTile;
   const uint32_t nsl = (ns - 1) * (layers - 1);
@@ -1076,6 +1262,7 @@ __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
   k.lane = lane;
   unsigned int* rescans = A.counters + 2 * S.index + 1;
   const double scale = S.scales[f];
+  StageClock<PROF> clk(PROF ? A.stages + 3 * (size_t)S.index : nullptr);
   for (uint32_t x0 = 0; x0 < nb; x0 += B) {
     const uint32_t W = min(B, nb - x0);
     const uint32_t wlen = W + H;
@@ -1099,8 +1286,10 @@ __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
       for (uint32_t i = tid; i < wlen; i += BLOCK) wn[i + 1] = count_of(wvw[i + 1].y, w0);
       __syncthreads();
     }
+    clk.mark(0);
     block_prefix<BLOCK>(wvw, wn, wlen, S.sv, S.sw, wtot, tid);
     __syncthreads();
+    clk.mark(1);
     WindowCum cm;
     cm.vw = wvw;
     cm.n = wn;
@@ -1133,6 +1322,7 @@ __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
       carry[4] = (double)(cm.bn + wn[W]);
     }
     __syncthreads();
+    clk.mark(2);
   }
   if (tid == 0) {
     double best = -kInf, a0 = 0.0, hp = 0.0, norm = 0.0;
@@ -1160,7 +1350,16 @@ __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
       norm = carry[3];
     }
     write_record(S, f, key, bdi, valid, ns, A.ingress, a0, hp, norm);
+    clk.mark(2);
   }
+}
+
+template <uint32_t BLOCK, bool PERSIST, bool PROF>
+__global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  for_each_group<PERSIST>(A, lds, [&](const LaunchArgs& a, uint32_t group) {
+    split_group<BLOCK, PROF>(a, group, lds);
+  });
 }
 
 }  // namespace
@@ -1168,7 +1367,7 @@ __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
 size_t scan_lds_bytes(uint32_t nb, uint32_t G, uint32_t nd, uint32_t ns,
                       uint32_t layers, uint32_t* period_stride) {
   const uint32_t nb1 = nb + 1;
-  uint32_t ps = (uint32_t)((20ull * nb1 + 15) & ~15ull);
+  uint32_t ps = (uint32_t)((20ull * (nb1 + kOver) + 15) & ~15ull);
   if (period_stride) *period_stride = ps;
   const uint32_t tiles = (nb + kTile - 1) / kTile;
   size_t bytes = (size_t)G * ps;
@@ -1193,12 +1392,29 @@ size_t split_lds_bytes(uint32_t nb, uint32_t range_bins, uint32_t halo_bins,
   return bytes;
 }
 
-hipError_t scan_configure(size_t max_lds) {
-  const void* fns[] = {(const void*)fused_scan<256>, (const void*)fused_scan<512>,
-                       (const void*)fused_scan<1024>,
-                       (const void*)fused_scan_split<256>,
-                       (const void*)fused_scan_split<512>,
-                       (const void*)fused_scan_split<1024>};
+namespace {
+
+// Every instantiation of a kernel: workgroup size x persistent x profiled.
+template <uint32_t BLOCK, bool PERSIST, bool PROF>
+struct ScanK {
+  static const void* fn() { return (const void*)fused_scan<BLOCK, PERSIST, PROF>; }
+  static void launch(const LaunchArgs& a, uint32_t groups, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((fused_scan<BLOCK, PERSIST, PROF>), dim3(groups), dim3(BLOCK), lds, s, a);
+  }
+};
+template <uint32_t BLOCK, bool PERSIST, bool PROF>
+struct SplitK {
+  static const void* fn() { return (const void*)fused_scan_split<BLOCK, PERSIST, PROF>; }
+  static void launch(const LaunchArgs& a, uint32_t groups, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((fused_scan_split<BLOCK, PERSIST, PROF>), dim3(groups), dim3(BLOCK), lds,
+                       s, a);
+  }
+};
+
+template <template <uint32_t, bool, bool> class K, bool PERSIST, bool PROF>
+hipError_t configure_all(size_t max_lds) {
+  const void* fns[] = {K<256, PERSIST, PROF>::fn(), K<512, PERSIST, PROF>::fn(),
+                       K<1024, PERSIST, PROF>::fn()};
   for (const void* fn : fns) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)max_lds);
@@ -1207,24 +1423,50 @@ hipError_t scan_configure(size_t max_lds) {
   return hipSuccess;
 }
 
+template <template <uint32_t, bool, bool> class K, bool PERSIST, bool PROF>
+void launch_block_of(const LaunchArgs& a, uint32_t groups, uint32_t block, size_t lds,
+                     hipStream_t s) {
+  if (block == 1024)
+    K<1024, PERSIST, PROF>::launch(a, groups, lds, s);
+  else if (block == 512)
+    K<512, PERSIST, PROF>::launch(a, groups, lds, s);
+  else
+    K<256, PERSIST, PROF>::launch(a, groups, lds, s);
+}
+
+template <template <uint32_t, bool, bool> class K>
+void launch_kind(const LaunchArgs& a, uint32_t groups, uint32_t block, size_t lds,
+                 hipStream_t s) {
+  const bool persist = a.ticket != nullptr, prof = a.stages != nullptr;
+  if (persist && prof) launch_block_of<K, true, true>(a, groups, block, lds, s);
+  else if (persist) launch_block_of<K, true, false>(a, groups, block, lds, s);
+  else if (prof) launch_block_of<K, false, true>(a, groups, block, lds, s);
+  else launch_block_of<K, false, false>(a, groups, block, lds, s);
+}
+
+}  // namespace
+
+hipError_t scan_configure(size_t max_lds) {
+  hipError_t e;
+  if ((e = configure_all<ScanK, false, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<ScanK, false, true>(max_lds)) != hipSuccess ||
+      (e = configure_all<ScanK, true, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<ScanK, true, true>(max_lds)) != hipSuccess ||
+      (e = configure_all<SplitK, false, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<SplitK, false, true>(max_lds)) != hipSuccess ||
+      (e = configure_all<SplitK, true, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<SplitK, true, true>(max_lds)) != hipSuccess)
+    return e;
+  return hipSuccess;
+}
+
 hipError_t scan_launch(const LaunchArgs& args, uint32_t groups, uint32_t block,
                        size_t lds, bool split, hipStream_t stream) {
   if (groups == 0) return hipSuccess;
-  if (split) {
-    if (block == 1024)
-      hipLaunchKernelGGL(fused_scan_split<1024>, dim3(groups), dim3(1024), lds, stream, args);
-    else if (block == 512)
-      hipLaunchKernelGGL(fused_scan_split<512>, dim3(groups), dim3(512), lds, stream, args);
-    else
-      hipLaunchKernelGGL(fused_scan_split<256>, dim3(groups), dim3(256), lds, stream, args);
-  } else {
-    if (block == 1024)
-      hipLaunchKernelGGL(fused_scan<1024>, dim3(groups), dim3(1024), lds, stream, args);
-    else if (block == 512)
-      hipLaunchKernelGGL(fused_scan<512>, dim3(groups), dim3(512), lds, stream, args);
-    else
-      hipLaunchKernelGGL(fused_scan<256>, dim3(groups), dim3(256), lds, stream, args);
-  }
+  if (split)
+    launch_kind<SplitK>(args, groups, block, lds, stream);
+  else
+    launch_kind<ScanK>(args, groups, block, lds, stream);
   return hipGetLastError();
 }
 

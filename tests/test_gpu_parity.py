@@ -284,3 +284,39 @@ def test_nonuniform_weights_and_unsorted_times(gpu):
     b = F.scan_bank(gpu, shuffled, periods, bank, 768)
     for x, y in zip(a[:4], b[:4]):
         np.testing.assert_array_equal(x, y)
+
+
+def test_profile_stage_attribution(gpu, monkeypatch):
+    """FBLS_PROFILE (scan.rs:152,222-229): unset, the fold / cumulative /
+    scan seconds of every band stay 0, as in the reference's own report;
+    set, the kernels' per-stage shader cycles share out each band's device
+    time -- every stage positive, their sum the band's device seconds -- and
+    the rows are those of the unprofiled search, also through the range-split
+    kernel and scan_bank."""
+    cfg = F.SearchConfig(period_min=0.5, period_max=13.7)
+    s, dev, vq, wq, d = _setup(gpu, cfg, seed=5)
+    print("[parity] profiled signal_search", flush=True)
+    monkeypatch.delenv("FBLS_PROFILE", raising=False)
+    rows0, grid0, bands0 = F.signal_search(gpu, dev, d, s.rho, cfg)
+    for b in bands0:
+        assert b["seconds_fold"] == b["seconds_cumulative"] == b["seconds_scan"] == 0.0
+    monkeypatch.setenv("FBLS_PROFILE", "1")
+    rows1, grid1, bands1 = F.signal_search(gpu, dev, d, s.rho, cfg)
+    np.testing.assert_array_equal(rows0, rows1)
+    np.testing.assert_array_equal(grid0, grid1)
+    for b in bands1:
+        stages = b["seconds_fold"] + b["seconds_cumulative"] + b["seconds_scan"]
+        assert b["seconds_fold"] > 0 and b["seconds_cumulative"] > 0 and b["seconds_scan"] > 0
+        assert abs(stages - b["seconds_device"]) <= 1e-9 * max(1.0, b["seconds_device"])
+    # scan_bank through both kernels (12,000 bins need the range split)
+    bank = F.make_bank(cfg, d, s.rho)
+    for nb in (384, 12000):
+        periods = np.linspace(3.5, 3.9, 24)
+        monkeypatch.delenv("FBLS_PROFILE", raising=False)
+        r0 = F.scan_bank(gpu, dev, periods, bank, nb)
+        monkeypatch.setenv("FBLS_PROFILE", "1")
+        r1 = F.scan_bank(gpu, dev, periods, bank, nb)
+        for a, b in zip(r0[:4], r1[:4]):
+            np.testing.assert_array_equal(a, b)
+        st = r1[4]
+        assert st["seconds_fold"] > 0 and st["seconds_cumulative"] > 0 and st["seconds_scan"] > 0

@@ -32,6 +32,23 @@ def test_period_grid_matches_oracle():
         np.testing.assert_array_equal(F.period_grid(*args), O.period_grid(*args))
 
 
+def test_period_grid_concurrent_callers():
+    """The grid is computed in pieces on the library's shared host threads;
+    many callers at once (as the batch pipeline's threads are) each get the
+    oracle's grid, element for element."""
+    import concurrent.futures as cf
+
+    cases = [(0.5, 20.0, 109.6, 0.5 / 24, 3.0), (0.3, 13.7, 27.4, 1.0 / 24, 5.0),
+             (0.5, 100.0, 1077.0, 0.5 / 24, 3.0), (1.0, 2.0, 30.0, 0.1, 1.0)]
+    want = [O.period_grid(*c) for c in cases]
+
+    def one(k):
+        return np.array_equal(F.period_grid(*cases[k % len(cases)]), want[k % len(cases)])
+
+    with cf.ThreadPoolExecutor(12) as ex:
+        assert all(ex.map(one, range(96)))
+
+
 def test_durations_days():
     cfg = F.SearchConfig()
     np.testing.assert_array_equal(cfg.durations_days(),
