@@ -59,7 +59,7 @@ EXPORTS = (
     "fbls_search_transits", "fbls_single_event_search", "fbls_read_inputs",
     "fbls_format_f64", "fbls_json_reformat", "fbls_cli", "fbls_coarse_inputs",
     "fbls_cnn_set_fused", "fbls_signal_search_bins", "fbls_propose",
-    "fbls_signal_search_bins_ragged", "fbls_scan_bank_noalloc",
+    "fbls_signal_search_bins_ragged", "fbls_scan_bank_noalloc", "fbls_family_upload_list",
 )
 
 
@@ -329,6 +329,8 @@ def lib():
             _sz, _P(_sz), _P(ScanStats), _sz, _P(_sz), _P(_d), _sz, _P(_sz)]
         L.fbls_family_upload_packed.argtypes = [
             _vp, _sz, _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_d), _P(_vp)]
+        L.fbls_family_upload_list.argtypes = [
+            _vp, _sz, _P(_P(_d)), _P(_P(_d)), _P(_P(_d)), _P(_sz), _P(_d), _P(_d), _P(_vp)]
         L.fbls_signal_search_batch.argtypes = [
             _vp, _P(_vp), _sz, _P(_d), _P(_d), _sz, _P(Bank), _P(SearchCfgC),
             _P(Row), _sz, _P(_sz), _P(ScanStats), _P(ctypes.c_uint8)]
@@ -528,8 +530,9 @@ def family_state(time, flux, sigma, durations, correlated=True):
         ctypes.byref(c), ctypes.byref(org), _dp(rho), ctypes.byref(infl),
         ctypes.byref(seg), ctypes.byref(en)))
     k = m.value
-    return FamilyState(t[:k].copy(), v[:k].copy(), w[:k].copy(), c.value,
-                       org.value, durations, rho, infl.value, seg.value,
+    if k < n:  # points were dropped: keep only the prepared ones
+        t, v, w = t[:k].copy(), v[:k].copy(), w[:k].copy()
+    return FamilyState(t, v, w, c.value, org.value, durations, rho, infl.value, seg.value,
                        en.value)
 
 
@@ -623,19 +626,20 @@ class DeviceReady:
 
     @classmethod
     def upload_packed(cls, gpu, states):
-        """Many families in one device allocation (ragged CSR layout)."""
+        """Many families in one device allocation (ragged layout), read from
+        each state's own arrays (fbls_family_upload_list)."""
         k = len(states)
-        offs = np.zeros(k + 1, dtype=np.uintp)
-        offs[1:] = np.cumsum([len(s.t) for s in states])
-        t = np.concatenate([_f64(s.t) for s in states])
-        v = np.concatenate([_f64(s.v) for s in states])
-        w = np.concatenate([_f64(s.w) for s in states])
+        arrs = [(_f64(s.t), _f64(s.v), _f64(s.w)) for s in states]  # kept alive for the call
+        PD = _P(_d)
+        tp = (PD * k)(*[_dp(a[0]) for a in arrs])
+        vp = (PD * k)(*[_dp(a[1]) for a in arrs])
+        wp = (PD * k)(*[_dp(a[2]) for a in arrs])
+        n = np.array([len(a[0]) for a in arrs], dtype=np.uintp)
         c = _f64([s.baseline_norm for s in states])
         o = _f64([s.origin for s in states])
         hs = (_vp * k)()
-        _check(lib().fbls_family_upload_packed(
-            gpu._h, k, offs.ctypes.data_as(_P(_sz)), _dp(t), _dp(v), _dp(w),
-            _dp(c), _dp(o), hs))
+        _check(lib().fbls_family_upload_list(
+            gpu._h, k, tp, vp, wp, n.ctypes.data_as(_P(_sz)), _dp(c), _dp(o), hs))
         out = []
         for i, st in enumerate(states):
             d = cls.__new__(cls)

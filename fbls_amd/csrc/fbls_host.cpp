@@ -183,6 +183,7 @@ extern "C" int fbls_ctx_create(int device, fbls_ctx** out) {
                   hipGetErrorString(e));
     }
     fbls::host::noalloc_setup(device);
+    c->arenas = std::make_shared<ArenaPool>(device);
     *out = c;
     return FBLS_OK;
   });
@@ -331,9 +332,9 @@ int upload_arena(fbls_ctx* c, std::vector<fbls_family*>& fams,
   for (size_t k = 0; k < fams.size(); k++) at[k + 1] = at[k] + family_bytes(fams[k]);
   const size_t total = at.back();
   auto arena = std::make_shared<DevArena>();
-  arena->device = c->device;
+  arena->pool = c->arenas;
   HIP_TRY(hipSetDevice(c->device));
-  if (int st = arena->buf.ensure(total)) return st;
+  if (int st = c->arenas->take(total, &arena->buf)) return st;
   {
     // in pieces of at most 256 MiB of staging (a larger family alone)
     constexpr size_t kPiece = (size_t)256 << 20;
@@ -382,6 +383,56 @@ extern "C" int fbls_family_upload(fbls_ctx* c, const double* t,
   });
 }
 
+namespace {
+
+// Many families into one arena: family k holds n[k] observations at t[k],
+// v[k], w[k].  Checks and fixed point on up to 16 host threads, then one
+// upload (upload_arena); every handle or none is written to out.
+int upload_many(fbls_ctx* c, size_t nfam, const double* const* t, const double* const* v,
+                const double* const* w, const size_t* n, const double* baseline_norm,
+                const double* origin, fbls_family** out) {
+  for (size_t k = 0; k < nfam; k++) out[k] = nullptr;
+  // the families are owned here until every one is uploaded
+  std::vector<std::unique_ptr<fbls_family>> owned;
+  std::vector<fbls_family*> fams;
+  std::vector<const double*> times(t, t + nfam);
+  owned.reserve(nfam);
+  fams.reserve(nfam);
+  for (size_t k = 0; k < nfam; k++) {
+    owned.emplace_back(new fbls_family());
+    fams.push_back(owned.back().get());
+  }
+  std::vector<int> status(nfam, FBLS_OK);
+  std::vector<std::string> message(nfam);
+  fbls::host::parallel_ranges(nfam, 8, [&](size_t lo, size_t hi) {
+    for (size_t k = lo; k < hi; k++) {
+      try {
+        status[k] = prepare_family(c, t[k], v[k], w[k], n[k], baseline_norm[k], origin[k],
+                                   fams[k]);
+        if (status[k]) message[k] = fbls_last_error();
+      } catch (const std::bad_alloc&) {
+        status[k] = FBLS_E_OOM;
+        message[k] = "host memory allocation failed";
+      } catch (const std::exception& ex) {
+        status[k] = FBLS_E_ARG;
+        try {
+          message[k] = ex.what();
+        } catch (...) {
+        }
+      }
+      if (status[k]) return;
+    }
+  });
+  for (size_t k = 0; k < nfam; k++)
+    if (status[k]) return fail(status[k], "family %zu: %s", k, message[k].c_str());
+  if (!fams.empty())
+    if (int st = upload_arena(c, fams, times)) return st;
+  for (size_t k = 0; k < nfam; k++) out[k] = owned[k].release();
+  return FBLS_OK;
+}
+
+}  // namespace
+
 extern "C" int fbls_family_upload_packed(fbls_ctx* c, size_t nfam,
                                          const size_t* offsets, const double* t,
                                          const double* v, const double* w,
@@ -392,48 +443,31 @@ extern "C" int fbls_family_upload_packed(fbls_ctx* c, size_t nfam,
     if (!c || !offsets || !t || !v || !w || !baseline_norm || !origin || !out)
       return fail(FBLS_E_ARG, "null argument");
     for (size_t k = 0; k < nfam; k++) out[k] = nullptr;
-    // the families are owned here until every one is uploaded
-    std::vector<std::unique_ptr<fbls_family>> owned;
-    std::vector<fbls_family*> fams;
-    std::vector<const double*> times;
-    owned.reserve(nfam);
-    fams.reserve(nfam);
-    times.reserve(nfam);
+    std::vector<const double*> tp(nfam), vp(nfam), wp(nfam);
+    std::vector<size_t> n(nfam);
     for (size_t k = 0; k < nfam; k++) {
       if (offsets[k + 1] < offsets[k]) return fail(FBLS_E_ARG, "offsets must be nondecreasing");
-      owned.emplace_back(new fbls_family());
-      fams.push_back(owned.back().get());
-      times.push_back(t + offsets[k]);
+      tp[k] = t + offsets[k];
+      vp[k] = v + offsets[k];
+      wp[k] = w + offsets[k];
+      n[k] = offsets[k + 1] - offsets[k];
     }
-    // the families' checks and fixed-point fields on up to 16 threads
-    std::vector<int> status(nfam, FBLS_OK);
-    std::vector<std::string> message(nfam);
-    fbls::host::parallel_ranges(nfam, 8, [&](size_t lo, size_t hi) {
-      for (size_t k = lo; k < hi; k++) {
-        const size_t a = offsets[k], n = offsets[k + 1] - offsets[k];
-        try {
-          status[k] = prepare_family(c, t + a, v + a, w + a, n, baseline_norm[k], origin[k],
-                                     fams[k]);
-          if (status[k]) message[k] = fbls_last_error();
-        } catch (const std::bad_alloc&) {
-          status[k] = FBLS_E_OOM;
-          message[k] = "host memory allocation failed";
-        } catch (const std::exception& ex) {
-          status[k] = FBLS_E_ARG;
-          try {
-            message[k] = ex.what();
-          } catch (...) {
-          }
-        }
-        if (status[k]) return;
-      }
-    });
+    return upload_many(c, nfam, tp.data(), vp.data(), wp.data(), n.data(), baseline_norm,
+                       origin, out);
+  });
+}
+
+extern "C" int fbls_family_upload_list(fbls_ctx* c, size_t nfam, const double* const* t,
+                                       const double* const* v, const double* const* w,
+                                       const size_t* n, const double* baseline_norm,
+                                       const double* origin, fbls_family** out) {
+  return fbls::host::guard([&]() -> int {
+    if (!c || !out || (nfam && (!t || !v || !w || !n || !baseline_norm || !origin)))
+      return fail(FBLS_E_ARG, "null argument");
+    for (size_t k = 0; k < nfam; k++) out[k] = nullptr;
     for (size_t k = 0; k < nfam; k++)
-      if (status[k]) return fail(status[k], "family %zu: %s", k, message[k].c_str());
-    if (!fams.empty())
-      if (int st = upload_arena(c, fams, times)) return st;
-    for (size_t k = 0; k < nfam; k++) out[k] = owned[k].release();
-    return FBLS_OK;
+      if (!t[k] || !v[k] || !w[k]) return fail(FBLS_E_ARG, "null array of family %zu", k);
+    return upload_many(c, nfam, t, v, w, n, baseline_norm, origin, out);
   });
 }
 
@@ -1797,6 +1831,8 @@ int batch_search(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
         a.templates_scanned += r.templates_scanned;
         a.exact_tile_retries += r.exact_tile_retries;
         a.seconds += r.seconds;
+        a.seconds_fold += r.seconds_fold;
+        a.seconds_cumulative += r.seconds_cumulative;
         a.seconds_scan += r.seconds_scan;
         a.seconds_device += r.seconds_device;
       }

@@ -94,14 +94,65 @@ struct HostBuf {
   }
 };
 
+// Device buffers of released uploads, kept for the next ones (a hipFree
+// waits for the whole device, which would stall a pipelined batch).  Shared
+// by the context and every arena it made; the last holder frees them.
+class ArenaPool {
+ public:
+  explicit ArenaPool(int device) : device_(device) {}
+  ~ArenaPool() {
+    (void)hipSetDevice(device_);
+    for (DevBuf& b : free_) b.release();
+  }
+  // A buffer of at least `bytes` (the smallest kept one that fits, else a
+  // new allocation).
+  int take(size_t bytes, DevBuf* out) {
+    {
+      std::lock_guard<std::mutex> hold(mu_);
+      size_t best = free_.size();
+      for (size_t k = 0; k < free_.size(); k++)
+        if (free_[k].cap >= bytes && (best == free_.size() || free_[k].cap < free_[best].cap))
+          best = k;
+      if (best < free_.size()) {
+        *out = free_[best];
+        free_.erase(free_.begin() + (ptrdiff_t)best);
+        return FBLS_OK;
+      }
+    }
+    (void)hipSetDevice(device_);
+    return out->ensure(bytes);
+  }
+  // Keeps a released buffer; beyond kKeep buffers the smallest is freed.
+  void give(DevBuf b) {
+    if (!b.ptr) return;
+    std::lock_guard<std::mutex> hold(mu_);
+    free_.push_back(b);
+    if (free_.size() > kKeep) {
+      size_t small = 0;
+      for (size_t k = 1; k < free_.size(); k++)
+        if (free_[k].cap < free_[small].cap) small = k;
+      (void)hipSetDevice(device_);
+      free_[small].release();
+      free_.erase(free_.begin() + (ptrdiff_t)small);
+    }
+  }
+
+ private:
+  static constexpr size_t kKeep = 4;
+  int device_;
+  std::mutex mu_;
+  std::vector<DevBuf> free_;
+};
+
 // Device memory shared by the families of one upload (one family, or a
-// packed batch of stars); released with the last family that uses it.
+// packed batch of stars); returned to the context's pool with the last
+// family that uses it.
 struct DevArena {
-  int device = 0;
+  std::shared_ptr<ArenaPool> pool;
   DevBuf buf;
   ~DevArena() {
-    (void)hipSetDevice(device);
-    buf.release();
+    if (pool) pool->give(buf);
+    else buf.release();
   }
 };
 
@@ -140,6 +191,7 @@ struct fbls_ctx {
   hipStream_t up_stream = nullptr;
   fbls::host::HostBuf pin_up;
   std::mutex up_mutex;
+  std::shared_ptr<fbls::host::ArenaPool> arenas;  // released upload buffers
 };
 
 struct fbls_family {

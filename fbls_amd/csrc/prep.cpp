@@ -184,16 +184,46 @@ std::vector<fbls_band> plan_bands(const GridFormula& grid, const double* sd, siz
 namespace {
 
 // stats.rs:4-11: the lower median of the finite values, the element a sort
-// would place at (m - 1) / 2, found by selection in linear time (a +0.0 and
-// a -0.0 may trade places, which no caller can see: each median is a divisor
-// checked positive, or subtracted before an absolute value)
+// would place at (m - 1) / 2, found by selection in linear time: a pass of
+// counts over value buckets, then selection within the one bucket that holds
+// it (a +0.0 and a -0.0 may trade places, which no caller can see: each
+// median is a divisor checked positive, or subtracted before an absolute
+// value)
 double median_lower(const std::vector<double>& values) {
   std::vector<double> v;
   v.reserve(values.size());
+  double lo = INFINITY, hi = -INFINITY;
   for (double x : values)
-    if (std::isfinite(x)) v.push_back(x);
+    if (std::isfinite(x)) {
+      v.push_back(x);
+      lo = std::min(lo, x);
+      hi = std::max(hi, x);
+    }
   if (v.empty()) return NAN;
-  const size_t k = (v.size() - 1) / 2;
+  const size_t m = v.size(), k = (m - 1) / 2;
+  const double span = hi - lo;
+  const size_t nbk = std::min<size_t>(4096, m / 8);
+  const double scale = span > 0.0 ? (double)nbk / span : 0.0;
+  if (m >= 512 && std::isfinite(span) && scale > 0.0 && std::isfinite(scale)) {
+    // value buckets: bucket(x) = floor((x - lo) * scale) is nondecreasing in
+    // x, so the k-th smallest value lies in the bucket where the running
+    // count passes k, at rank k minus the count of the buckets below it
+    auto bucket = [&](double x) {
+      const size_t b = (size_t)((x - lo) * scale);
+      return b < nbk ? b : nbk - 1;
+    };
+    std::vector<uint32_t> count(nbk, 0);
+    for (double x : v) count[bucket(x)]++;
+    size_t below = 0, b = 0;
+    while (below + count[b] <= k) below += count[b++];
+    std::vector<double> in;
+    in.reserve(count[b]);
+    for (double x : v)
+      if (bucket(x) == b) in.push_back(x);
+    const size_t r = k - below;
+    std::nth_element(in.begin(), in.begin() + (ptrdiff_t)r, in.end());
+    return in[r];
+  }
   std::nth_element(v.begin(), v.begin() + (ptrdiff_t)k, v.end());
   return v[k];
 }
@@ -386,43 +416,72 @@ void correlated_noise_factors(const std::vector<double>& time,
   const double cadence = median_lower(steps);
   std::vector<double> measured(nd);
   std::vector<size_t> counts(nd);
-  std::vector<double> z;
-  for (size_t di = 0; di < nd; di++) {
-    const double d = durations[di];
-    z.clear();
-    double total = 0.0, acc = 0.0;
-    size_t cnt = 0;
-    int64_t current = INT64_MIN;
-    auto flush = [&]() {
-      if (cnt > 0 && (double)cnt >= min_fill * d / cadence)
-        z.push_back((acc / total) * std::sqrt(total));
-    };
-    // cell = floor(fl(x / d)) for x = t - t0: from x * (1 / d) when that
-    // product is farther than 1e-12 relative from an integer (the two differ
-    // by a few ulps, so their floors agree), from the division otherwise
-    const double inv = 1.0 / d;
-    for (size_t i = 0; i < n; i++) {
-      const double x = time[i] - time[0];
-      const double q = x * inv;
-      const double fq = std::floor(q);
-      const double margin = 1e-12 * (q > 1.0 ? q : 1.0);
-      const int64_t cell = (q - fq > margin && fq + 1.0 - q > margin)
-                               ? (int64_t)fq
-                               : (int64_t)std::floor(x / d);
-      if (cell != current) {
-        flush();
-        current = cell;
-        total = 0.0;
-        acc = 0.0;
-        cnt = 0;
-      }
-      total += weights[i];
-      acc += weights[i] * residual[i];
-      cnt++;
+  // Durations four at a time: one pass over the points advances the four
+  // durations' bins together (four independent chains of sums, and a branch
+  // only where some duration's cell changes); within a duration the points
+  // are added in the same order as a one-duration pass, so every bin's sums
+  // are the same doubles.
+  // cell = floor(fl(x / d)) for x = t - t0: from x * (1 / d) when that
+  // product is farther than 1e-12 relative from an integer (the two differ
+  // by a few ulps, so their floors agree), from the division otherwise
+  std::vector<double> x(n), wr(n);
+  for (size_t i = 0; i < n; i++) {
+    x[i] = time[i] - time[0];
+    wr[i] = weights[i] * residual[i];
+  }
+  std::vector<std::vector<double>> z(nd);
+  constexpr size_t kLanes = 4;
+  for (size_t d0 = 0; d0 < nd; d0 += kLanes) {
+    const size_t lanes = std::min(kLanes, nd - d0);
+    double inv[kLanes], dd[kLanes], need[kLanes], total[kLanes], acc[kLanes];
+    int64_t current[kLanes];
+    size_t cnt[kLanes];
+    for (size_t l = 0; l < kLanes; l++) {
+      // lanes past the last duration repeat it and are never read out
+      dd[l] = durations[d0 + std::min(l, lanes - 1)];
+      inv[l] = 1.0 / dd[l];
+      need[l] = min_fill * dd[l] / cadence;
+      total[l] = acc[l] = 0.0;
+      current[l] = INT64_MIN;
+      cnt[l] = 0;
     }
-    flush();
-    counts[di] = z.size();
-    measured[di] = z.size() >= 3 ? max1(mad_scale(z)) : 1.0;
+    auto flush = [&](size_t l) {
+      if (l < lanes && cnt[l] > 0 && (double)cnt[l] >= need[l])
+        z[d0 + l].push_back((acc[l] / total[l]) * std::sqrt(total[l]));
+    };
+    for (size_t i = 0; i < n; i++) {
+      const double xi = x[i];
+      int64_t cell[kLanes];
+      bool change = false;
+      for (size_t l = 0; l < kLanes; l++) {
+        const double q = xi * inv[l];
+        const double fq = std::floor(q);
+        const double margin = 1e-12 * (q > 1.0 ? q : 1.0);
+        cell[l] = (int64_t)fq;
+        if (!(q - fq > margin && fq + 1.0 - q > margin)) cell[l] = (int64_t)std::floor(xi / dd[l]);
+        change = change || cell[l] != current[l];
+      }
+      if (change) {
+        for (size_t l = 0; l < kLanes; l++) {
+          if (cell[l] == current[l]) continue;
+          flush(l);
+          current[l] = cell[l];
+          total[l] = 0.0;
+          acc[l] = 0.0;
+          cnt[l] = 0;
+        }
+      }
+      for (size_t l = 0; l < kLanes; l++) {
+        total[l] += weights[i];
+        acc[l] += wr[i];
+        cnt[l]++;
+      }
+    }
+    for (size_t l = 0; l < lanes; l++) flush(l);
+  }
+  for (size_t di = 0; di < nd; di++) {
+    counts[di] = z[di].size();
+    measured[di] = z[di].size() >= 3 ? max1(mad_scale(z[di])) : 1.0;
   }
   bool any = false;
   for (size_t c : counts) any = any || c >= minimum_bins;

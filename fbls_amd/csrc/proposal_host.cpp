@@ -45,6 +45,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   }
   const double pmin = *std::min_element(periods, periods + np);
   double cell_max = 0.0;  // largest phase cell t * nb / P of the launch
+  bool uniform = true;    // every star has equal fixed-point weights
   std::vector<fbls::FeatureStar> stars(nfam);
   for (size_t k = 0; k < nfam; k++) {
     const fbls_family* f = fams[k];
@@ -55,7 +56,11 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
     stars[k].vw = f->vw_dev;
     stars[k].fx = f->fx;
     stars[k].n = f->n;
+    stars[k].n_pad = f->n_pad;
     stars[k].sorted = f->sorted ? 1u : 0u;
+    stars[k].uniform_w = f->w_uniform;
+    stars[k].pad_ = 0;
+    uniform = uniform && f->w_uniform > 0;
     stars[k].sv = std::ldexp(1.0, -f->ev);
     stars[k].sw = std::ldexp(1.0, -f->ew);
     stars[k].C = f->C;
@@ -64,22 +69,23 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   const size_t o_per = (b_stars + 255) & ~(size_t)255;
   const size_t o_nb = o_per + ((np * 8 + 255) & ~(size_t)255);
   const size_t o_dur = o_nb + ((np * 4 + 255) & ~(size_t)255);
-  // groups of up to 4 consecutive periods whose histograms (20 B per slot,
-  // each sized by the group's largest nb) fit 24 KiB of LDS: a star's
-  // observations are loaded once per group (more LDS per workgroup costs
-  // occupancy, which the fold needs more than reuse)
+  // groups of up to 16 consecutive periods whose histograms (20 B per slot,
+  // each sized by the group's largest nb) fit half the CU's LDS (two
+  // 512-thread workgroups per CU): a star's observations are loaded once
+  // per group and folded into every period of it
+  const size_t budget = c->max_lds / 2 - 4096;  // the kernel's static LDS aside
   std::vector<uint32_t> gstart{0}, gstride;
   size_t lds_bytes = 0;
   uint32_t gmax = 1;
   for (size_t p = 0; p < np;) {
     uint32_t G = 1, mx = nb[p];
-    while (G < 4 && p + G < np) {
+    while (G < fbls::kFeatureMaxGroup && p + G < np) {
       const uint32_t m2 = std::max(mx, nb[p + G]);
-      if (fbls::feature_lds_bytes(m2, G + 1) > 24 * 1024) break;
+      if (fbls::feature_lds_bytes(m2, G + 1) > budget) break;
       mx = m2;
       G++;
     }
-    gstride.push_back(mx + 1);
+    gstride.push_back(fbls::feature_slots(mx));
     lds_bytes = std::max(lds_bytes, fbls::feature_lds_bytes(mx, G));
     gmax = std::max(gmax, G);
     p += G;
@@ -119,6 +125,7 @@ int coarse(fbls_ctx* c, const fbls_family* const* fams, size_t nfam, const doubl
   A.gstride = (const uint32_t*)(d + o_gw);
   A.lds_bytes = (uint32_t)lds_bytes;
   A.narrow = cell_max < 16777215.0 ? 1u : 0u;  // (nb <= nb_max, P >= pmin)
+  A.uniform = uniform ? 1u : 0u;
   HIP_TRY(fbls::feature_launch(A, (uint32_t)nfam, s));
   if (peaks) HIP_TRY(hipMemcpyAsync(peaks, d + o_peaks, peak_bytes, hipMemcpyDeviceToHost, s));
   if (device_inputs) return FBLS_OK;  // stream-ordered before the network

@@ -22,6 +22,7 @@ the seconds of each stage (the time the search waited for a prepared and
 uploaded chunk in `wait`).
 """
 import concurrent.futures as cf
+import threading
 import time
 
 import numpy as np
@@ -39,9 +40,11 @@ def _prepare(curve, durations, correlated):
 class Stages:
     def __init__(self):
         self.seconds = {}
+        self._lock = threading.Lock()
 
     def add(self, name, dt):
-        self.seconds[name] = self.seconds.get(name, 0.0) + dt
+        with self._lock:
+            self.seconds[name] = self.seconds.get(name, 0.0) + dt
 
 
 def _upload(gpu, states, mode):
@@ -120,58 +123,120 @@ def _search_long(gpu, states, devs, bdevs, cfg, net, pc, op, st):
     return best, sum(s["templates_scanned"] for s in stats)
 
 
-def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, workers=16):
+def _ramp(chunk):
+    """Chunk sizes: chunk / 4, chunk / 2, then chunk for ever."""
+    yield max(1, chunk // 4)
+    yield max(1, chunk // 2)
+    while True:
+        yield chunk
+
+
+def _net_on(net, gpu):
+    """A copy of the proposal network on another device context."""
+    from . import Cnn
+
+    c = net.cfg
+    copy = Cnn(gpu, in_channels=c.in_channels, width=c.width, layers=c.layers,
+               kernel=c.kernel, pos_weight=c.pos_weight)
+    copy.set_params(net.params())
+    return copy
+
+
+def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, workers=16,
+        device_workers=2):
     """One end-to-end batch run over `curves` ((time, flux, flux_err) per
-    star, flux normalised to 1).  Three stages overlap: preparation of
-    chunks k + 1 and k + 2 on `workers` threads, the upload of chunk k + 1
-    (its own thread and HIP stream) and the search of chunk k.  Returns a
-    dict with the stars' best rows, the wall seconds, stars/hour, trial
-    evals, the stage seconds (upload seconds overlap the search; `wait` is
-    the time the search waited for its chunk) and the worker count."""
+    star, flux normalised to 1).  The stages overlap: preparation on
+    `workers` host threads (every star queued at once, in order), the
+    uploads in chunk order on their own thread and HIP stream (at most
+    device_workers + 1 chunks on the device and not yet searched), and the
+    searches on `device_workers` threads, each with a device context of its
+    own (and, for the learned modes, its own copy of the network), taking
+    chunks in turn, so that one chunk's search overlaps the next one's
+    proposal and host planning on the GPU -- the reference's `batch
+    --workers` (src/batch.rs:47-68).  The first two chunks are a quarter and
+    a half of `chunk` stars.  Returns a dict with the stars' best rows, the
+    wall seconds, stars/hour, trial evals, the stage seconds summed over the
+    device threads (`wait`: time a device thread waited for its chunk) and
+    the worker counts."""
     if mode not in ("full", "pruned", "long"):
         raise ValueError(mode)
+    from . import Gpu
+
     durations = cfg.durations_days()
     st = Stages()
-    best, templates = [], 0
-    spans = [(a, min(len(curves), a + chunk)) for a in range(0, len(curves), chunk)]
-    with cf.ThreadPoolExecutor(workers) as pool, cf.ThreadPoolExecutor(1) as up:
-        def prepare(k):
-            a, b = spans[k]
-            return [pool.submit(_prepare, c, durations, cfg.correlated_noise)
-                    for c in curves[a:b]]
-
+    # the first chunks are small, so the device starts while the host still
+    # prepares the rest (chunk / 4, chunk / 2, then chunk)
+    spans, a = [], 0
+    for size in _ramp(chunk):
+        if a >= len(curves):
+            break
+        spans.append((a, min(len(curves), a + size)))
+        a += size
+    nw = max(1, min(device_workers, len(spans)))
+    ctxs = [gpu] + [Gpu(gpu.device) for _ in range(nw - 1)]
+    before = gpu.launch_totals()
+    nets = [net] + [_net_on(net, g) for g in ctxs[1:]] if net is not None else [None] * nw
+    ahead = threading.Semaphore(nw + 1)
+    stop = threading.Event()  # a device thread failed: pending uploads give up
+    results = [None] * len(spans)
+    with cf.ThreadPoolExecutor(workers) as pool, cf.ThreadPoolExecutor(1) as up, \
+            cf.ThreadPoolExecutor(nw) as dev:
         def upload(futs):
             states = [f.result() for f in futs]
+            while not ahead.acquire(timeout=0.1):
+                if stop.is_set():
+                    raise RuntimeError("the search of an earlier chunk failed")
             t0 = time.perf_counter()
             devs, bdevs = _upload(gpu, states, mode)
             return states, devs, bdevs, time.perf_counter() - t0
 
+        def device_work(j):
+            try:
+                search_chunks(j)
+            except BaseException:
+                stop.set()
+                raise
+
+        def search_chunks(j):
+            g, nt = ctxs[j], nets[j]
+            for k in range(j, len(spans), nw):
+                t0 = time.perf_counter()
+                states, devs, bdevs, t_up = ups[k].result()
+                st.add("wait", time.perf_counter() - t0)
+                st.add("upload", t_up)
+                try:
+                    if mode == "full":
+                        results[k] = _search_full(g, states, devs, cfg, st)
+                    elif mode == "pruned":
+                        results[k] = _search_pruned(g, states, devs, cfg, nt, pc, op, st)
+                    else:
+                        results[k] = _search_long(g, states, devs, bdevs, cfg, nt, pc, op, st)
+                finally:
+                    for d in devs + (bdevs or []):
+                        d.close()
+                    ahead.release()
+
         t_start = time.perf_counter()
-        preps = {k: prepare(k) for k in range(min(2, len(spans)))}
-        ups = {0: up.submit(upload, preps.pop(0))} if spans else {}
-        for k in range(len(spans)):
-            if k + 1 < len(spans):
-                ups[k + 1] = up.submit(upload, preps.pop(k + 1))
-            if k + 2 < len(spans):
-                preps[k + 2] = prepare(k + 2)
-            t0 = time.perf_counter()
-            states, devs, bdevs, t_up = ups.pop(k).result()
-            st.add("wait", time.perf_counter() - t0)
-            st.add("upload", t_up)
-            if mode == "full":
-                b, n = _search_full(gpu, states, devs, cfg, st)
-            elif mode == "pruned":
-                b, n = _search_pruned(gpu, states, devs, cfg, net, pc, op, st)
-            else:
-                b, n = _search_long(gpu, states, devs, bdevs, cfg, net, pc, op, st)
-            for d in devs + (bdevs or []):
-                d.close()
-            best += b
-            templates += n
+        preps = [[pool.submit(_prepare, c, durations, cfg.correlated_noise)
+                  for c in curves[a:b]] for a, b in spans]
+        ups = [up.submit(upload, preps[k]) for k in range(len(spans))]
+        for f in [dev.submit(device_work, j) for j in range(nw)]:
+            f.result()
         wall = time.perf_counter() - t_start
+    best, templates = [], 0
+    for b, n in results:
+        best += b
+        templates += n
+    # scan launches of this run on every context (the contexts' kernels
+    # overlap in time, so their seconds add up to more than the wall time)
+    totals = {k: v - before[k] for k, v in gpu.launch_totals().items()}
+    for g in ctxs[1:]:
+        for k, v in g.launch_totals().items():
+            totals[k] += v
     return {"best": best, "seconds": wall, "stars": len(curves),
             "stars_per_hour": len(curves) * 3600.0 / wall, "trial_evals": templates,
-            "stage_seconds": st.seconds, "host_workers": workers, "chunk": chunk}
+            "stage_seconds": st.seconds, "host_workers": workers, "device_workers": nw,
+            "chunk": chunk, "launch_totals": totals}
 
 
 def preparation_rate(curves, cfg, workers=1):

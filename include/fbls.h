@@ -155,6 +155,14 @@ int fbls_family_upload_packed(fbls_ctx* ctx, size_t nfam, const size_t* offsets,
                               const double* t, const double* v, const double* w,
                               const double* baseline_norm, const double* origin,
                               fbls_family** out);
+/* The same upload from one array per star: family k holds n[k] observations
+ * at t[k], v[k], w[k] (no packed copy needed); same handles and ownership as
+ * fbls_family_upload_packed.  The per-star `DeviceReady::upload` of `fbls
+ * batch` (batch.rs:76, scan.rs:110-125) for a whole batch at once. */
+int fbls_family_upload_list(fbls_ctx* ctx, size_t nfam, const double* const* t,
+                            const double* const* v, const double* const* w, const size_t* n,
+                            const double* baseline_norm, const double* origin,
+                            fbls_family** out);
 /* The fixed-point values the device scans, as doubles (exact), and the
  * power-of-two exponents: v_dev = round(v * 2^ev) * 2^-ev. */
 int fbls_family_fields(const fbls_family* fam, double* v_out, double* w_out,

@@ -78,3 +78,52 @@ def test_batch_chunks_and_repeat(gpu):
     for k in (0, 7, 11):
         r1, _, _ = F.signal_search(gpu, devs[k], d, states[k].rho, cfg)
         np.testing.assert_array_equal(rows[k], r1)
+
+
+def test_packed_and_list_uploads_agree(gpu, six_star_batch):
+    """fbls_family_upload_packed (one concatenated array per field) and
+    fbls_family_upload_list (one array per star, what DeviceReady.upload_packed
+    calls) give the same fixed-point fields and the same search rows; a
+    released upload's device buffer is reused by the next upload."""
+    import ctypes
+
+    cfg, d, states, devs, rows, stats = six_star_batch
+    print("[batch] packed C entry vs list upload", flush=True)
+    L = F.lib()
+    k = len(states)
+    offs = np.zeros(k + 1, dtype=np.uintp)
+    offs[1:] = np.cumsum([len(s.t) for s in states])
+    t = np.concatenate([s.t for s in states])
+    v = np.concatenate([s.v for s in states])
+    w = np.concatenate([s.w for s in states])
+    c = np.array([s.baseline_norm for s in states])
+    o = np.array([s.origin for s in states])
+    P = ctypes.POINTER(ctypes.c_double)
+    hs = (ctypes.c_void_p * k)()
+    assert L.fbls_family_upload_packed(
+        gpu._h, k, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)), t.ctypes.data_as(P),
+        v.ctypes.data_as(P), w.ctypes.data_as(P), c.ctypes.data_as(P), o.ctypes.data_as(P),
+        hs) == 0
+    packed = []
+    for i, st in enumerate(states):
+        dv = F.DeviceReady.__new__(F.DeviceReady)
+        dv.gpu, dv.n, dv.baseline_norm, dv.origin = gpu, len(st.t), st.baseline_norm, st.origin
+        dv.t_last = float(st.t[-1])
+        dv._h = ctypes.c_void_p(hs[i])
+        packed.append(dv)
+    for a, b in zip(devs, packed):
+        fa, fb = a.fields(), b.fields()
+        for x, y in zip(fa, fb):
+            np.testing.assert_array_equal(x, y)
+    rows2, _ = F.signal_search_batch(gpu, packed, d, [s.rho for s in states], cfg)
+    for a, b in zip(rows, rows2):
+        np.testing.assert_array_equal(a, b)
+    for dv in packed:
+        dv.close()
+    # the released buffer serves the next upload of the same stars
+    again = F.DeviceReady.upload_packed(gpu, states)
+    rows3, _ = F.signal_search_batch(gpu, again, d, [s.rho for s in states], cfg)
+    for a, b in zip(rows, rows3):
+        np.testing.assert_array_equal(a, b)
+    for dv in again:
+        dv.close()

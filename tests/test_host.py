@@ -304,3 +304,19 @@ int main(void) {
     n, first, last, text = r.stdout.split()
     assert int(n) == len(F.period_grid(0.5, 20.0, 50.0, 0.02, 3.0))
     assert float(first) == 0.5 and float(last) == 20.0 and text == "0.1"
+
+
+def test_median_selection_matches_sort(tmp_path):
+    """The preparation's lower median (stats.rs:4-11) by bucket selection
+    equals the element a full sort places at (m - 1) / 2 on 20,000 arrays
+    built to break it (tests/native/median_check.cpp)."""
+    import subprocess
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = str(tmp_path / "median_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
+                    os.path.join(here, "native", "median_check.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "mismatches 0" in out.stdout
+
