@@ -8,6 +8,8 @@ namespace fbls {
 
 // Most consecutive periods one workgroup folds from one load of a star.
 constexpr uint32_t kFeatureMaxGroup = 16;
+// Most durations of one launch.
+constexpr uint32_t kFeatureMaxDurations = 16;
 
 struct FeatureStar {
   const double* t;    // n_pad origin-relative times (pad 0)

@@ -2,16 +2,15 @@
 // of a batch and every period of a shared log-spaced list, the largest box
 // SNR over phase for each of a few representative durations.
 //
-// One workgroup (256 threads) per star and group of up to 4 consecutive
-// periods whose histograms fit 24 KiB of LDS (each lane's observations are
-// loaded once for the whole group): the star's fixed-point
-// fields are folded into an LDS histogram of nb bins (nb = clamp(ceil(2 P /
-// d_0), nb_min, nb_max), d_0 the shortest duration) with exact integer sums
-// held in doubles (runs of consecutive observations in one bin are summed in
-// registers first; the bins of a lane's 8 ascending times follow from the
-// first by the differences of floor(t * scale)), turned into an exclusive
-// prefix by the waves in parallel.  For every duration d_g (width k =
-// max(1, floor(d_g nb / P + 0.5)) bins, circular) the phases j that hold at
+// One workgroup (512 threads) per star and group of up to 16 consecutive
+// periods whose histograms fit the LDS budget (each lane's observations are
+// loaded once for the whole group): the star's fixed-point fields are folded
+// into an LDS histogram of nb bins per period (nb = clamp(ceil(2 P / d_0),
+// nb_min, nb_max), d_0 the shortest duration) with exact integer sums held
+// in doubles (fold.h: runs of consecutive observations in one bin summed
+// from per-lane prefix sums), turned into an exclusive prefix by the waves in
+// parallel.  Then one wave per (period, duration d_g) task (width k =
+// max(1, floor(d_g nb / P + 0.5)) bins, circular): the phases j that hold at
 // least three points (and at most n - 3) with a positive precision q = w -
 // w^2 / C qualify; the value is a / sqrt(q) at the qualifying phase with the
 // largest key a |a| / q (a monotone function of a / sqrt(q); the smallest
@@ -44,37 +43,11 @@ __device__ __forceinline__ uint32_t width_of(double D, double scale, uint32_t nb
   return k > nb ? nb : k;
 }
 
-// Sums of the circular window [j, j + k) from the exclusive prefix ({v, w}
-// pairs and counts); true when it holds 3 .. n - 3 points and a positive
-// precision q = w - w^2 / C.
-__device__ __forceinline__ bool window(const double2* vw, const uint32_t* gn, uint32_t nb,
-                                       uint32_t tn, uint32_t j, uint32_t k,
-                                       const FeatureStar& S, double* pa, double* pq) {
-  const uint32_t e = j + k;
-  double a, w;
-  uint32_t c;
-  if (e <= nb) {
-    a = (vw[e].x - vw[j].x) * S.sv;
-    w = (vw[e].y - vw[j].y) * S.sw;
-    c = gn[e] - gn[j];
-  } else {
-    a = (vw[e - nb].x + (vw[nb].x - vw[j].x)) * S.sv;
-    w = (vw[e - nb].y + (vw[nb].y - vw[j].y)) * S.sw;
-    c = gn[e - nb] + (gn[nb] - gn[j]);
-  }
-  const double q = w - w * w / S.C;
-  *pa = a;
-  *pq = q;
-  return c >= 3 && c + 3 <= tn && q > 0.0;
-}
-
-template <uint32_t NG, bool UNI>
+template <bool UNI>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void coarse_features(FeatureArgs A) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  __shared__ double red_key[kBlock / 64][NG];
-  __shared__ uint32_t red_j[kBlock / 64][NG];
-  __shared__ double red_m[NG];
+  __shared__ double red_m[kFeatureMaxGroup][kFeatureMaxDurations];
   const uint32_t gi = blockIdx.x % A.ngroups, star = blockIdx.x / A.ngroups;
   const uint32_t p0 = A.gstart[gi], gcount = A.gstart[gi + 1] - p0;
   const FeatureStar S = A.stars[star];
@@ -179,106 +152,81 @@ void coarse_features(FeatureArgs A) {
       fold::wave_prefix_vw(vw, A.nb[p0 + g], lane, 1.0, 1.0);
   }
   __syncthreads();
-  // best SNR over phase per (period, duration): the phase with the largest
-  // key a|a| / q (monotone in a / sqrt(q); the smallest phase on ties),
-  // found per thread, then by wave and workgroup reductions; its SNR
-  // a / sqrt(q) is computed once at the end
-  for (uint32_t g = 0; g < gcount; g++) {
-    const uint32_t nb = A.nb[p0 + g];
-    const double scale = (double)nb / A.periods[p0 + g];
+  // best SNR over phase per (period, duration): one wave per (period,
+  // duration) task.  Each lane keeps the largest key a|a| / q (monotone in
+  // a / sqrt(q)) over its phases j = lane, lane + 64, ... (ascending, so its
+  // phase is the smallest one of its key) with the sums a, q it came from;
+  // the wave's winner is the largest key, the smallest phase on equal keys,
+  // and its SNR a / sqrt(q).
+  const uint32_t ntask = gcount * A.ng;
+  for (uint32_t task = wave; task < ntask; task += kBlock / 64) {
+    const uint32_t g = task / A.ng, d = task - g * A.ng;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(my_lo, (int)g);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(my_hi, (int)g);
+    const double scale = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    const uint32_t nb = (uint32_t)__builtin_amdgcn_readlane((int)my_nb, (int)g);
+    const uint32_t k = width_of(A.durations[d], scale, nb);
     const double2* vw = (const double2*)(lds + (size_t)g * pstride);
     const uint32_t* gn = (const uint32_t*)(vw + slots);
     const uint32_t tn = gn[nb];
     const double tv = vw[nb].x, tw = vw[nb].y;
-    uint32_t kd[NG];
-    double best[NG];
-    uint32_t bj[NG];
-#pragma unroll
-    for (uint32_t d = 0; d < NG; d++) {
-      kd[d] = d < A.ng ? width_of(A.durations[d], scale, nb) : 1u;
-      best[d] = -kInf;
-      bj[d] = 0xffffffffu;
-    }
-    for (uint32_t j = tid; j < nb; j += kBlock) {
-      const double vj = vw[j].x, wj = vw[j].y;
-      const uint32_t nj = gn[j];
-#pragma unroll
-      for (uint32_t d = 0; d < NG; d++) {
-        if (d < A.ng) {
-          const uint32_t e = j + kd[d];
-          double a, w;
-          uint32_t c;
-          if (e <= nb) {
-            a = (vw[e].x - vj) * S.sv;
-            w = (vw[e].y - wj) * S.sw;
-            c = gn[e] - nj;
-          } else {  // exact integers: the order of the sums does not matter
-            a = (vw[e - nb].x + (tv - vj)) * S.sv;
-            w = (vw[e - nb].y + (tw - wj)) * S.sw;
-            c = gn[e - nb] + (tn - nj);
-          }
-          const double qq = w - w * w / S.C;
-          if (c >= 3 && c + 3 <= tn && qq > 0.0) {
-            const double key = a * fabs(a) / qq;
-            if (key > best[d]) {
-              best[d] = key;
-              bj[d] = j;
-            }
-          }
+    double best = -kInf, ba = 0.0, bq = 1.0;
+    uint32_t bj = 0xffffffffu;
+    for (uint32_t j = lane; j < nb; j += 64) {
+      const uint32_t e = j + k;
+      double a, w;
+      uint32_t c;
+      if (e <= nb) {
+        a = (vw[e].x - vw[j].x) * S.sv;
+        w = (vw[e].y - vw[j].y) * S.sw;
+        c = gn[e] - gn[j];
+      } else {  // exact integers: the order of the sums does not matter
+        a = (vw[e - nb].x + (tv - vw[j].x)) * S.sv;
+        w = (vw[e - nb].y + (tw - vw[j].y)) * S.sw;
+        c = gn[e - nb] + (tn - gn[j]);
+      }
+      const double qq = w - w * w / S.C;
+      if (c >= 3 && c + 3 <= tn && qq > 0.0) {
+        const double key = a * fabs(a) / qq;
+        if (key > best) {
+          best = key;
+          bj = j;
+          ba = a;
+          bq = qq;
         }
       }
     }
+    double mx = best;
 #pragma unroll
-    for (uint32_t d = 0; d < NG; d++) {
-      double bk = best[d];
-      uint32_t b = bj[d];
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double ok = __shfl_xor(bk, off, 64);
-        const uint32_t oj = __shfl_xor(b, off, 64);
-        if (ok > bk || (ok == bk && oj < b)) {
-          bk = ok;
-          b = oj;
-        }
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    double m = -kInf;
+    if (mx > -kInf) {
+      uint64_t at_max = __ballot(best == mx);
+      if (__popcll(at_max) > 1) {  // equal keys: the smallest phase
+        uint32_t jm = best == mx ? bj : 0xffffffffu;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) jm = min(jm, (uint32_t)__shfl_xor((int)jm, off, 64));
+        at_max = __ballot(best == mx && bj == jm);
       }
-      if (lane == 0) {
-        red_key[wave][d] = bk;
-        red_j[wave][d] = b;
-      }
+      const int wl = __builtin_ctzll(at_max);
+      m = __shfl(ba, wl, 64) / sqrt(__shfl(bq, wl, 64));
     }
+    if (lane == 0) {
+      const size_t at = ((size_t)star * A.np + p0 + g) * A.ng + d;
+      if (A.out) A.out[at] = m;
+      if (A.inputs) {  // the network input: clip(SNR, -5, 40) / 8, 0 when none
+        const double x = m > -kInf ? m : 0.0;
+        A.inputs[at] = (float)(fmin(fmax(x, -5.0), 40.0) / 8.0);
+      }
+      red_m[g][d] = m;
+    }
+  }
+  if (A.peaks) {  // the largest SNR over the durations (unclipped)
     __syncthreads();
-    if (tid < NG) {
-      const uint32_t d = tid;
-      double m = -kInf;
-      if (d < A.ng) {
-        double bk = red_key[0][d];
-        uint32_t b = red_j[0][d];
-        for (uint32_t w = 1; w < kBlock / 64; w++) {
-          const double ok = red_key[w][d];
-          const uint32_t oj = red_j[w][d];
-          if (ok > bk || (ok == bk && oj < b)) {
-            bk = ok;
-            b = oj;
-          }
-        }
-        if (b != 0xffffffffu) {
-          double a, qq;
-          window(vw, gn, nb, tn, b, width_of(A.durations[d], scale, nb), S, &a, &qq);
-          m = a / sqrt(qq);
-        }
-        const size_t at = ((size_t)star * A.np + p0 + g) * A.ng + d;
-        if (A.out) A.out[at] = m;
-        if (A.inputs) {  // the network input: clip(SNR, -5, 40) / 8, 0 when none
-          const double x = m > -kInf ? m : 0.0;
-          A.inputs[at] = (float)(fmin(fmax(x, -5.0), 40.0) / 8.0);
-        }
-      }
-      red_m[d] = m;
-    }
-    __syncthreads();
-    if (A.peaks && tid == 0) {  // the largest SNR over the durations (unclipped)
+    if (tid < gcount) {
       double mx = -kInf;
-      for (uint32_t d = 0; d < A.ng; d++) mx = fmax(mx, red_m[d]);
-      A.peaks[(size_t)star * A.np + p0 + g] = mx > -kInf ? (float)mx : -INFINITY;
+      for (uint32_t d = 0; d < A.ng; d++) mx = fmax(mx, red_m[tid][d]);
+      A.peaks[(size_t)star * A.np + p0 + tid] = mx > -kInf ? (float)mx : -INFINITY;
     }
   }
 }
@@ -293,37 +241,16 @@ uint32_t feature_slots(uint32_t nb_max) { return nb_max + 1 + fold::kOver; }
 
 hipError_t feature_launch(const FeatureArgs& a, uint32_t nstars, hipStream_t s) {
   const size_t lds = a.lds_bytes;
-  if (a.ng > 16 || a.G > kFeatureMaxGroup || a.ngroups == 0) return hipErrorInvalidValue;
-  const void* fn;
-  void (*launch)(const FeatureArgs&, uint32_t, size_t, hipStream_t);
-  if (a.ng <= 8) {
-    if (a.uniform) {
-      fn = (const void*)coarse_features<8, true>;
-      launch = [](const FeatureArgs& x, uint32_t g, size_t l, hipStream_t st) {
-        hipLaunchKernelGGL((coarse_features<8, true>), dim3(g), dim3(kBlock), l, st, x);
-      };
-    } else {
-      fn = (const void*)coarse_features<8, false>;
-      launch = [](const FeatureArgs& x, uint32_t g, size_t l, hipStream_t st) {
-        hipLaunchKernelGGL((coarse_features<8, false>), dim3(g), dim3(kBlock), l, st, x);
-      };
-    }
-  } else {
-    if (a.uniform) {
-      fn = (const void*)coarse_features<16, true>;
-      launch = [](const FeatureArgs& x, uint32_t g, size_t l, hipStream_t st) {
-        hipLaunchKernelGGL((coarse_features<16, true>), dim3(g), dim3(kBlock), l, st, x);
-      };
-    } else {
-      fn = (const void*)coarse_features<16, false>;
-      launch = [](const FeatureArgs& x, uint32_t g, size_t l, hipStream_t st) {
-        hipLaunchKernelGGL((coarse_features<16, false>), dim3(g), dim3(kBlock), l, st, x);
-      };
-    }
-  }
+  if (a.ng > kFeatureMaxDurations || a.G > kFeatureMaxGroup || a.ngroups == 0)
+    return hipErrorInvalidValue;
+  const void* fn = a.uniform ? (const void*)coarse_features<true>
+                            : (const void*)coarse_features<false>;
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  launch(a, nstars * a.ngroups, lds, s);
+  if (a.uniform)
+    hipLaunchKernelGGL((coarse_features<true>), dim3(nstars * a.ngroups), dim3(kBlock), lds, s, a);
+  else
+    hipLaunchKernelGGL((coarse_features<false>), dim3(nstars * a.ngroups), dim3(kBlock), lds, s, a);
   return hipGetLastError();
 }
 
