@@ -164,18 +164,40 @@ __device__ __forceinline__ void state_update(LaneState& st, double s,
   }
 }
 
+// Staircase layer count and its reciprocals.  With a power-of-two count n,
+// x / n and x / (n n) are exact scalings, equal to x * (1 / n) and
+// x * (1 / (n n)) for every x, so the hot loop multiplies instead of
+// dividing.
+struct Layers {
+  uint32_t n;
+  bool pow2;
+  double l, inv_l, inv_l2;
+};
+
+__device__ __forceinline__ Layers layers_of(uint32_t n) {
+  Layers y;
+  y.n = n;
+  y.pow2 = (n & (n - 1)) == 0;
+  y.l = (double)n;
+  y.inv_l = 1.0 / y.l;
+  y.inv_l2 = 1.0 / (y.l * y.l);
+  return y;
+}
+
 // Staircase sums (kernels.rs:84-113) with precomputed layer widths/offsets.
-template <class CumT>
+// POW2 (a power-of-two layer count): the divisions by the count are exact
+// scalings, done as multiplications by its reciprocals.
+template <bool POW2, class CumT>
 __device__ __forceinline__ void shape_sums(const CumT& c, uint32_t jj,
                                            double a0, double w0,
-                                           uint32_t layers,
+                                           const Layers& ly,
                                            const uint32_t* tab, double* pa,
                                            double* php, double* pnorm) {
-  double l = (double)layers;
-  double a = a0 / l;
-  double hp = w0 / l;
-  double norm = w0 / (l * l);
-  for (uint32_t layer = 1; layer < layers; layer++) {
+  const double l = ly.l;
+  double a = POW2 ? a0 * ly.inv_l : a0 / l;
+  double hp = POW2 ? w0 * ly.inv_l : w0 / l;
+  double norm = POW2 ? w0 * ly.inv_l2 : w0 / (l * l);
+  for (uint32_t layer = 1; layer < ly.n; layer++) {
     uint32_t e = tab[layer - 1];
     uint32_t width = e & 0xffffu;
     uint32_t start = jj + (e >> 16);
@@ -183,9 +205,10 @@ __device__ __forceinline__ void shape_sums(const CumT& c, uint32_t jj,
     double2 lo = c.vw_at(start);
     double dw = hi.y - lo.y;
     double dv = hi.x - lo.x;
-    a += dv / l;
-    hp += dw / l;
-    norm += dw * ((2.0 * (double)layer + 1.0) / (l * l));
+    a += POW2 ? dv * ly.inv_l : dv / l;
+    hp += POW2 ? dw * ly.inv_l : dw / l;
+    norm += dw * (POW2 ? (2.0 * (double)layer + 1.0) * ly.inv_l2
+                       : (2.0 * (double)layer + 1.0) / (l * l));
   }
   *pa = a;
   *php = hp;
@@ -435,6 +458,7 @@ struct TileCtx {
   const double* ingress;  // [ns]
   double c, inv_c, gate, g2;
   uint32_t nb, nd, ns, layers, nobs, lane;
+  Layers ly;
 };
 
 struct TileOut {
@@ -447,8 +471,8 @@ struct TileOut {
 // for both of the lane's bins, the tile gate, the staircase shapes, the
 // surrogate lane states, the exact value of the winner and an exact rescan
 // when the top is near-tied.  Wave-uniform result.
-template <class CumT>
-__device__ TileOut scan_tile(const CumT& cm, const TileCtx& k, uint32_t tile,
+template <bool POW2, class CumT>
+__device__ __forceinline__ TileOut scan_tile(const CumT& cm, const TileCtx& k, uint32_t tile,
                              bool* bad, unsigned int* rescans) {
   const uint32_t nb = k.nb, nb1 = nb + 1, nd = k.nd, ns = k.ns;
   const uint32_t layers = k.layers, lane = k.lane;
@@ -510,7 +534,7 @@ __device__ TileOut scan_tile(const CumT& cm, const TileCtx& k, uint32_t tile,
 #pragma unroll
       for (int x = 0; x < 2; x++) {
         double a = bv[x], hp = bw[x], norm = bw[x];
-        if (staircase) shape_sums(cm, jjx[x], bv[x], bw[x], layers, tab, &a, &hp, &norm);
+        if (staircase) shape_sums<POW2>(cm, jjx[x], bv[x], bw[x], k.ly, tab, &a, &hp, &norm);
         if (busable[x] && a > 0.0) {
           const uint64_t key = ((uint64_t)si * nb1 + kk) * nb1 + jjx[x];
           const double s = surrogate(a, hp, norm, rho, rho2, k.c, k.inv_c, bad);
@@ -594,7 +618,7 @@ __device__ TileOut scan_tile(const CumT& cm, const TileCtx& k, uint32_t tile,
         const uint32_t* tab = k.stab + (di * (ns - 1) + (si - 1)) * (layers - 1);
         double sa = a, shp = w, snorm = w;
         if (k.ingress[si] != 0.0)
-          shape_sums(cm, jjx[x], a, w, layers, tab, &sa, &shp, &snorm);
+          shape_sums<POW2>(cm, jjx[x], a, w, k.ly, tab, &sa, &shp, &snorm);
         if (usable && sa > 0.0) {
           const uint64_t key = ((uint64_t)si * nb1 + kk) * nb1 + jjx[x];
           const double mm = exact_merit(sa, shp, snorm, rho, k.c, bad);
@@ -633,7 +657,7 @@ __device__ TileOut scan_tile(const CumT& cm, const TileCtx& k, uint32_t tile,
   const double2 lo = cm.vw_at(wj);
   double a = hi.x - lo.x, w = hi.y - lo.y, hp = w, norm = w;
   if (wsi > 0 && k.ingress[wsi] != 0.0)
-    shape_sums(cm, wj, hi.x - lo.x, w, layers,
+    shape_sums<POW2>(cm, wj, hi.x - lo.x, w, k.ly,
                k.stab + (out.di * (ns - 1) + (wsi - 1)) * (layers - 1), &a, &hp,
                &norm);
   out.a = a;
@@ -822,6 +846,7 @@ __device__ __forceinline__ void scan_group(const LaunchArgs& A, uint32_t group,
   k.nd = nd;
   k.ns = ns;
   k.layers = layers;
+  k.ly = layers_of(layers);
   k.nobs = S.n;
   k.lane = lane;
   unsigned int* rescans = A.counters + 2 * S.index + 1;
@@ -838,7 +863,8 @@ __device__ __forceinline__ void scan_group(const LaunchArgs& A, uint32_t group,
     k.kt = ktab + g * nd;
     k.stab = stab + (size_t)g * nd * nsl;
     bool bad = false;
-    const TileOut o = scan_tile(cm, k, tile, &bad, rescans);
+    const TileOut o = k.ly.pow2 ? scan_tile<true>(cm, k, tile, &bad, rescans)
+                                : scan_tile<false>(cm, k, tile, &bad, rescans);
     if (__any(bad) && lane == 0) atomicOr(A.counters + 2 * S.index, 1u);
     if (lane == 0) {
       tval[g * tiles + tile] = o.value;
@@ -984,6 +1010,7 @@ __device__ __forceinline__ void split_group(const LaunchArgs& A, uint32_t group,
   k.nd = nd;
   k.ns = ns;
   k.layers = layers;
+  k.ly = layers_of(layers);
   k.nobs = S.n;
   k.lane = lane;
   unsigned int* rescans = A.counters + 2 * S.index + 1;
@@ -1030,7 +1057,8 @@ __device__ __forceinline__ void split_group(const LaunchArgs& A, uint32_t group,
     const uint32_t t0 = x0 / kTile, t1 = (x0 + W + kTile - 1) / kTile;
     for (uint32_t tile = t0 + wave; tile < t1; tile += kWaves) {
       bool bad = false;
-      const TileOut o = scan_tile(cm, k, tile, &bad, rescans);
+      const TileOut o = k.ly.pow2 ? scan_tile<true>(cm, k, tile, &bad, rescans)
+                                : scan_tile<false>(cm, k, tile, &bad, rescans);
       if (__any(bad) && lane == 0) atomicOr(A.counters + 2 * S.index, 1u);
       if (lane == 0) {
         tval[4 * tile] = o.value;
