@@ -964,9 +964,14 @@ size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
 // no prescan) in one launch per kernel, the survivors compacted and the full
 // scan's segment tables completed on the device, the full scan of every
 // band's survivors in one persistent launch per kernel, the rows extracted
-// on the device; the host waits once, at the end.
+// on the device; the host waits once, at the end.  box_first (searches
+// restricted to proposed period bins, where nearly every period passes the
+// prescan): each prescan period scans its boxes first and skips its
+// staircase templates when its best box already reaches the cut (the
+// survivors are the same; where few periods survive the extra box pass
+// would cost more than it saves).
 int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
-               fbls_row* rows, size_t* nrows, size_t* row_counts) {
+               fbls_row* rows, size_t* nrows, size_t* row_counts, bool box_first) {
   const double t_start = now_s();
   const Shared sh = shared_of(b);
   const uint32_t ns = sh.ns, layers = sh.layers;
@@ -1147,6 +1152,7 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
     B.prescan = r.nb_coarse ? 1u : 0u;
   }
   // segment tables
+  const double cut = b->prescan_fraction * b->threshold;
   fbls::Segment* hseg = (fbls::Segment*)(hin + seg_off);
   const fbls::Segment* dseg = (const fbls::Segment*)(din + seg_off);
   size_t seg_at[4];
@@ -1173,6 +1179,8 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
         S.records = const_cast<double*>(B.rec1);
         S.score = const_cast<double*>(B.score1);
         S.depth = const_cast<double*>(B.depth1);
+        // a prescan only decides survival (score >= cut)
+        if (box_first && B.prescan && cut > 0.0) S.survive = cut;
         S.group0 = group0;
         group0 += (uint32_t)((x.np + S.G - 1) / S.G);
       } else {
@@ -1207,6 +1215,7 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
     A.gate = sh.gate;
     A.counters = d_segcnt + 2 * seg_at[t];
     if (prof) A.stages = d_stages + 3 * seg_at[t];
+    A.box_first = (box_first && t < 2) ? 1u : 0u;
     if (t >= 2) {
       A.vgroups = d_vgroups + (t - 2);
       A.ticket = d_tickets + (t - 2);
@@ -1217,7 +1226,7 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
   std::memset(&BA, 0, sizeof(BA));
   BA.bands = (fbls::BandDesc*)(din + band_off);
   BA.nbands = (uint32_t)na;
-  BA.cut = b->prescan_fraction * b->threshold;
+  BA.cut = cut;
   BA.threshold = b->threshold;
   BA.counts = d_counts;
   BA.row_off = (uint32_t*)(dout + off_at);
@@ -1368,7 +1377,7 @@ extern "C" int fbls_scan_periods(fbls_ctx* c, const fbls_family* f,
     r.nd = (uint32_t)b->nd;
     r.dur = b->durations;
     r.rho = b->rho;
-    st = scan_bands(c, bands, b, rows, nrows, nullptr);
+    st = scan_bands(c, bands, b, rows, nrows, nullptr, false);
     if (st) return st;
     if (stats) *stats = r.stats;
     return FBLS_OK;
@@ -1710,7 +1719,7 @@ extern "C" int fbls_signal_search(fbls_ctx* c, const fbls_family* f,
     if (st) return st;
     std::vector<BandRun> runs;
     append_runs(p, runs);
-    st = scan_bands(c, runs, &b, rows, nrows, nullptr);
+    st = scan_bands(c, runs, &b, rows, nrows, nullptr, false);
     if (st) return st;
     for (size_t i = 0; i < runs.size(); i++)
       if (stats) stats[i] = runs[i].stats;
@@ -1810,7 +1819,7 @@ int batch_search(fbls_ctx* c, const fbls_family* const* fams, size_t nfam,
     std::vector<size_t> counts(runs.size(), 0);
     size_t nchunk = 0;
     if (!runs.empty()) {
-      int st = scan_bands(c, runs, &b, chunk_rows.get(), &nchunk, counts.data());
+      int st = scan_bands(c, runs, &b, chunk_rows.get(), &nchunk, counts.data(), rs.any());
       if (st) return st;
     }
     size_t at = 0;

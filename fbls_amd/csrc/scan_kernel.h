@@ -21,6 +21,14 @@ struct Segment {
   uint32_t* winner;       // np x 4 or null
   double sv, sw;          // 2^-ev, 2^-ew: fixed point -> value
   double C, invC;         // baseline norm and its reciprocal
+  double survive;         // > 0: a prescan segment's survival cut.  Its periods
+                          // only need to be known above or below it: in a
+                          // box_first launch the boxes are scanned first, and
+                          // a period whose best box reaches the cut keeps that
+                          // score (>= the cut) without its staircase templates
+                          // (the LDS-resident kernel; the range-split kernel
+                          // scans everything).  0: every template, the exact
+                          // maximum.
   uint32_t n, n_pad;      // observations, padded to a multiple of 512
   uint32_t np, nb, G, nd; // periods, phase bins, periods per group, durations
   uint32_t pstride;       // LDS bytes per period histogram
@@ -55,6 +63,9 @@ struct LaunchArgs {
   // shader-clock sums over its workgroups -- fold, prefix sums, template
   // scan and merge.  Null: not recorded.
   unsigned long long* stages;
+  // 1: segments with a survival cut scan their boxes first (non-persistent
+  // LDS-resident launches; ignored elsewhere).
+  uint32_t box_first;
 };
 
 // LDS bytes of one workgroup holding G periods of nb bins.

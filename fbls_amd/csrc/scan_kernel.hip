@@ -764,7 +764,7 @@ struct StageClock {
 
 // One group of the launch: G periods of one segment, from the fold to the
 // records.  LDS: the workgroup's dynamic LDS.
-template <uint32_t BLOCK, bool PROF>
+template <uint32_t BLOCK, bool PROF, bool BOX>
 __device__ __forceinline__ void scan_group(const LaunchArgs& A, uint32_t group,
                                            unsigned char* lds) {
   constexpr uint32_t kWaves = BLOCK / 64;
@@ -795,6 +795,8 @@ __device__ __forceinline__ void scan_group(const LaunchArgs& A, uint32_t group,
   uint64_t* tkey = (uint64_t*)p;
   p += S.G * tiles * 8;
   uint32_t* tdi = (uint32_t*)p;
+  p += S.G * tiles * 4;
+  uint32_t* sflag = (uint32_t*)p;  // [G] prescan survival flags
 
   // ---- phase 0: clear histograms, duration tables
   {
@@ -834,14 +836,17 @@ __device__ __forceinline__ void scan_group(const LaunchArgs& A, uint32_t group,
   __syncthreads();
   clk.mark(1);
 
-  // ---- phase 3: template scan, one 128-bin tile per wave at a time
+  // ---- phase 3: template scan, one 128-bin tile per wave at a time.  In a
+  // box-first launch (BOX) a prescan segment (survival cut > 0) first scans
+  // the boxes alone (the
+  // shape gate at +inf); a period whose best box reaches the cut survives
+  // whatever its staircases give and keeps that score, the others are
+  // scanned in full in a second pass.
   TileCtx k;
   k.rho = S.rho;
   k.ingress = A.ingress;
   k.c = S.C;
   k.inv_c = S.invC;
-  k.gate = A.gate;
-  k.g2 = A.gate * A.gate;
   k.nb = nb;
   k.nd = nd;
   k.ns = ns;
@@ -850,29 +855,43 @@ __device__ __forceinline__ void scan_group(const LaunchArgs& A, uint32_t group,
   k.nobs = S.n;
   k.lane = lane;
   unsigned int* rescans = A.counters + 2 * S.index + 1;
-  for (uint32_t item = wave; item < gcount * tiles; item += kWaves) {
-    const uint32_t g = item / tiles, tile = item % tiles;
-    WrapCum cm;
-    cm.vw = (const double2*)(hist + (size_t)g * S.pstride);
-    cm.n = (const uint32_t*)(cm.vw + slots);
-    cm.nb = nb;
-    const double2 t = cm.vw[nb];
-    cm.tv = t.x;
-    cm.tw = t.y;
-    cm.tn = cm.n[nb];
-    k.kt = ktab + g * nd;
-    k.stab = stab + (size_t)g * nd * nsl;
-    bool bad = false;
-    const TileOut o = k.ly.pow2 ? scan_tile<true>(cm, k, tile, &bad, rescans)
-                                : scan_tile<false>(cm, k, tile, &bad, rescans);
-    if (__any(bad) && lane == 0) atomicOr(A.counters + 2 * S.index, 1u);
-    if (lane == 0) {
-      tval[g * tiles + tile] = o.value;
-      tkey[g * tiles + tile] = o.key;
-      tdi[g * tiles + tile] = o.di;
+  const bool surv = BOX && S.survive > 0.0;
+  for (uint32_t pass = surv ? 0u : 1u; pass < 2; pass++) {
+    k.gate = pass == 0 ? kInf : A.gate;
+    k.g2 = k.gate * k.gate;
+    for (uint32_t item = wave; item < gcount * tiles; item += kWaves) {
+      const uint32_t g = item / tiles, tile = item % tiles;
+      if (surv && pass == 1 && sflag[g]) continue;  // survives by its boxes
+      WrapCum cm;
+      cm.vw = (const double2*)(hist + (size_t)g * S.pstride);
+      cm.n = (const uint32_t*)(cm.vw + slots);
+      cm.nb = nb;
+      const double2 t = cm.vw[nb];
+      cm.tv = t.x;
+      cm.tw = t.y;
+      cm.tn = cm.n[nb];
+      k.kt = ktab + g * nd;
+      k.stab = stab + (size_t)g * nd * nsl;
+      bool bad = false;
+      const TileOut o = k.ly.pow2 ? scan_tile<true>(cm, k, tile, &bad, rescans)
+                                  : scan_tile<false>(cm, k, tile, &bad, rescans);
+      if (__any(bad) && lane == 0) atomicOr(A.counters + 2 * S.index, 1u);
+      if (lane == 0) {
+        tval[g * tiles + tile] = o.value;
+        tkey[g * tiles + tile] = o.key;
+        tdi[g * tiles + tile] = o.di;
+      }
+    }
+    __syncthreads();
+    if (pass == 0) {
+      for (uint32_t g = tid; g < gcount; g += BLOCK) {
+        double m = -kInf;
+        for (uint32_t t = 0; t < tiles; t++) m = fmax(m, tval[g * tiles + t]);
+        sflag[g] = m >= S.survive ? 1u : 0u;
+      }
+      __syncthreads();
     }
   }
-  __syncthreads();
 
   // ---- phase 4: merge tiles (kernels.rs:1113-1155), exact record
   for (uint32_t g = tid; g < gcount; g += BLOCK) {
@@ -940,12 +959,14 @@ __device__ __forceinline__ void for_each_group(const LaunchArgs& A, unsigned cha
   }
 }
 
-template <uint32_t BLOCK, bool PERSIST, bool PROF>
+// MODE: 0 one group per workgroup, 1 persistent, 2 one group per workgroup
+// with the box-first prescan.
+template <uint32_t BLOCK, uint32_t MODE, bool PROF>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 void fused_scan(LaunchArgs A) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  for_each_group<PERSIST>(A, lds, [&](const LaunchArgs& a, uint32_t group) {
-    scan_group<BLOCK, PROF>(a, group, lds);
+  for_each_group<MODE == 1>(A, lds, [&](const LaunchArgs& a, uint32_t group) {
+    scan_group<BLOCK, PROF, MODE == 2>(a, group, lds);
   });
 }
 
@@ -1058,7 +1079,7 @@ __device__ __forceinline__ void split_group(const LaunchArgs& A, uint32_t group,
     for (uint32_t tile = t0 + wave; tile < t1; tile += kWaves) {
       bool bad = false;
       const TileOut o = k.ly.pow2 ? scan_tile<true>(cm, k, tile, &bad, rescans)
-                                : scan_tile<false>(cm, k, tile, &bad, rescans);
+                                  : scan_tile<false>(cm, k, tile, &bad, rescans);
       if (__any(bad) && lane == 0) atomicOr(A.counters + 2 * S.index, 1u);
       if (lane == 0) {
         tval[4 * tile] = o.value;
@@ -1108,10 +1129,11 @@ __device__ __forceinline__ void split_group(const LaunchArgs& A, uint32_t group,
   }
 }
 
-template <uint32_t BLOCK, bool PERSIST, bool PROF>
+// MODE: 0 one group per workgroup, 1 persistent.
+template <uint32_t BLOCK, uint32_t MODE, bool PROF>
 __global__ __launch_bounds__(BLOCK) void fused_scan_split(LaunchArgs A) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  for_each_group<PERSIST>(A, lds, [&](const LaunchArgs& a, uint32_t group) {
+  for_each_group<MODE == 1>(A, lds, [&](const LaunchArgs& a, uint32_t group) {
     split_group<BLOCK, PROF>(a, group, lds);
   });
 }
@@ -1129,6 +1151,7 @@ size_t scan_lds_bytes(uint32_t nb, uint32_t G, uint32_t nd, uint32_t ns,
   bytes += (size_t)G * nd * (ns - 1) * (layers - 1) * 4;
   bytes = (bytes + 7) & ~(size_t)7;
   bytes += (size_t)G * tiles * (8 + 8 + 4);
+  bytes += (size_t)G * 4;  // prescan survival flags
   return bytes;
 }
 
@@ -1149,26 +1172,26 @@ size_t split_lds_bytes(uint32_t nb, uint32_t range_bins, uint32_t halo_bins,
 namespace {
 
 // Every instantiation of a kernel: workgroup size x persistent x profiled.
-template <uint32_t BLOCK, bool PERSIST, bool PROF>
+template <uint32_t BLOCK, uint32_t MODE, bool PROF>
 struct ScanK {
-  static const void* fn() { return (const void*)fused_scan<BLOCK, PERSIST, PROF>; }
+  static const void* fn() { return (const void*)fused_scan<BLOCK, MODE, PROF>; }
   static void launch(const LaunchArgs& a, uint32_t groups, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((fused_scan<BLOCK, PERSIST, PROF>), dim3(groups), dim3(BLOCK), lds, s, a);
+    hipLaunchKernelGGL((fused_scan<BLOCK, MODE, PROF>), dim3(groups), dim3(BLOCK), lds, s, a);
   }
 };
-template <uint32_t BLOCK, bool PERSIST, bool PROF>
+template <uint32_t BLOCK, uint32_t MODE, bool PROF>
 struct SplitK {
-  static const void* fn() { return (const void*)fused_scan_split<BLOCK, PERSIST, PROF>; }
+  static const void* fn() { return (const void*)fused_scan_split<BLOCK, MODE, PROF>; }
   static void launch(const LaunchArgs& a, uint32_t groups, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((fused_scan_split<BLOCK, PERSIST, PROF>), dim3(groups), dim3(BLOCK), lds,
+    hipLaunchKernelGGL((fused_scan_split<BLOCK, MODE, PROF>), dim3(groups), dim3(BLOCK), lds,
                        s, a);
   }
 };
 
-template <template <uint32_t, bool, bool> class K, bool PERSIST, bool PROF>
+template <template <uint32_t, uint32_t, bool> class K, uint32_t MODE, bool PROF>
 hipError_t configure_all(size_t max_lds) {
-  const void* fns[] = {K<256, PERSIST, PROF>::fn(), K<512, PERSIST, PROF>::fn(),
-                       K<1024, PERSIST, PROF>::fn()};
+  const void* fns[] = {K<256, MODE, PROF>::fn(), K<512, MODE, PROF>::fn(),
+                       K<1024, MODE, PROF>::fn()};
   for (const void* fn : fns) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)max_lds);
@@ -1177,39 +1200,44 @@ hipError_t configure_all(size_t max_lds) {
   return hipSuccess;
 }
 
-template <template <uint32_t, bool, bool> class K, bool PERSIST, bool PROF>
+template <template <uint32_t, uint32_t, bool> class K, uint32_t MODE, bool PROF>
 void launch_block_of(const LaunchArgs& a, uint32_t groups, uint32_t block, size_t lds,
                      hipStream_t s) {
   if (block == 1024)
-    K<1024, PERSIST, PROF>::launch(a, groups, lds, s);
+    K<1024, MODE, PROF>::launch(a, groups, lds, s);
   else if (block == 512)
-    K<512, PERSIST, PROF>::launch(a, groups, lds, s);
+    K<512, MODE, PROF>::launch(a, groups, lds, s);
   else
-    K<256, PERSIST, PROF>::launch(a, groups, lds, s);
+    K<256, MODE, PROF>::launch(a, groups, lds, s);
 }
 
-template <template <uint32_t, bool, bool> class K>
+template <template <uint32_t, uint32_t, bool> class K, uint32_t BOXMODE>
 void launch_kind(const LaunchArgs& a, uint32_t groups, uint32_t block, size_t lds,
                  hipStream_t s) {
   const bool persist = a.ticket != nullptr, prof = a.stages != nullptr;
-  if (persist && prof) launch_block_of<K, true, true>(a, groups, block, lds, s);
-  else if (persist) launch_block_of<K, true, false>(a, groups, block, lds, s);
-  else if (prof) launch_block_of<K, false, true>(a, groups, block, lds, s);
-  else launch_block_of<K, false, false>(a, groups, block, lds, s);
+  const bool box = !persist && a.box_first;
+  if (persist && prof) launch_block_of<K, 1, true>(a, groups, block, lds, s);
+  else if (persist) launch_block_of<K, 1, false>(a, groups, block, lds, s);
+  else if (box && prof) launch_block_of<K, BOXMODE, true>(a, groups, block, lds, s);
+  else if (box) launch_block_of<K, BOXMODE, false>(a, groups, block, lds, s);
+  else if (prof) launch_block_of<K, 0, true>(a, groups, block, lds, s);
+  else launch_block_of<K, 0, false>(a, groups, block, lds, s);
 }
 
 }  // namespace
 
 hipError_t scan_configure(size_t max_lds) {
   hipError_t e;
-  if ((e = configure_all<ScanK, false, false>(max_lds)) != hipSuccess ||
-      (e = configure_all<ScanK, false, true>(max_lds)) != hipSuccess ||
-      (e = configure_all<ScanK, true, false>(max_lds)) != hipSuccess ||
-      (e = configure_all<ScanK, true, true>(max_lds)) != hipSuccess ||
-      (e = configure_all<SplitK, false, false>(max_lds)) != hipSuccess ||
-      (e = configure_all<SplitK, false, true>(max_lds)) != hipSuccess ||
-      (e = configure_all<SplitK, true, false>(max_lds)) != hipSuccess ||
-      (e = configure_all<SplitK, true, true>(max_lds)) != hipSuccess)
+  if ((e = configure_all<ScanK, 0, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<ScanK, 0, true>(max_lds)) != hipSuccess ||
+      (e = configure_all<ScanK, 1, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<ScanK, 1, true>(max_lds)) != hipSuccess ||
+      (e = configure_all<ScanK, 2, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<ScanK, 2, true>(max_lds)) != hipSuccess ||
+      (e = configure_all<SplitK, 0, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<SplitK, 0, true>(max_lds)) != hipSuccess ||
+      (e = configure_all<SplitK, 1, false>(max_lds)) != hipSuccess ||
+      (e = configure_all<SplitK, 1, true>(max_lds)) != hipSuccess)
     return e;
   return hipSuccess;
 }
@@ -1217,10 +1245,10 @@ hipError_t scan_configure(size_t max_lds) {
 hipError_t scan_launch(const LaunchArgs& args, uint32_t groups, uint32_t block,
                        size_t lds, bool split, hipStream_t stream) {
   if (groups == 0) return hipSuccess;
-  if (split)
-    launch_kind<SplitK>(args, groups, block, lds, stream);
+  if (split)  // the range-split kernel has no box-first mode
+    launch_kind<SplitK, 0>(args, groups, block, lds, stream);
   else
-    launch_kind<ScanK>(args, groups, block, lds, stream);
+    launch_kind<ScanK, 2>(args, groups, block, lds, stream);
   return hipGetLastError();
 }
 
