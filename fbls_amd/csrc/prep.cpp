@@ -183,6 +183,41 @@ std::vector<fbls_band> plan_bands(const GridFormula& grid, const double* sd, siz
 
 namespace {
 
+// Per-thread working buffers of the preparation.  Each call of family_state
+// reuses the capacity the thread's previous calls left (a fresh allocation
+// of a few hundred KiB is mapped and faulted in page by page, and returned
+// to the system when freed: several times the arithmetic's cost, and
+// serialised between threads); trim() frees them after an unusually large
+// star.
+struct Scratch {
+  std::vector<double> flux1, t, f, e, deltas, standardized, resid, sig;
+  std::vector<double> weights, steps, x, wr, st, vals;
+  std::vector<double> med_v, med_in, mad_dev;
+  std::vector<size_t> keep;
+  std::vector<int64_t> sec;
+  std::vector<uint32_t> gi, med_count;
+  std::vector<std::vector<double>> z;
+
+  size_t bytes() const {
+    size_t b = 0;
+    for (const auto* v : {&flux1, &t, &f, &e, &deltas, &standardized, &resid, &sig, &weights,
+                          &steps, &x, &wr, &st, &vals, &med_v, &med_in, &mad_dev})
+      b += v->capacity() * sizeof(double);
+    b += keep.capacity() * sizeof(size_t) + sec.capacity() * sizeof(int64_t);
+    b += (gi.capacity() + med_count.capacity()) * sizeof(uint32_t);
+    for (const auto& v : z) b += v.capacity() * sizeof(double);
+    return b;
+  }
+  void trim(size_t limit) {
+    if (bytes() > limit) *this = Scratch();
+  }
+};
+
+Scratch& scratch() {
+  thread_local Scratch s;
+  return s;
+}
+
 // stats.rs:4-11: the lower median of the finite values, the element a sort
 // would place at (m - 1) / 2, found by selection in linear time: a pass of
 // counts over value buckets, then selection within the one bucket that holds
@@ -190,7 +225,9 @@ namespace {
 // median is a divisor checked positive, or subtracted before an absolute
 // value)
 double median_lower(const std::vector<double>& values) {
-  std::vector<double> v;
+  Scratch& sc = scratch();
+  std::vector<double>& v = sc.med_v;
+  v.clear();
   v.reserve(values.size());
   double lo = INFINITY, hi = -INFINITY;
   for (double x : values)
@@ -212,11 +249,13 @@ double median_lower(const std::vector<double>& values) {
       const size_t b = (size_t)((x - lo) * scale);
       return b < nbk ? b : nbk - 1;
     };
-    std::vector<uint32_t> count(nbk, 0);
+    std::vector<uint32_t>& count = sc.med_count;
+    count.assign(nbk, 0);
     for (double x : v) count[bucket(x)]++;
     size_t below = 0, b = 0;
     while (below + count[b] <= k) below += count[b++];
-    std::vector<double> in;
+    std::vector<double>& in = sc.med_in;
+    in.clear();
     in.reserve(count[b]);
     for (double x : v)
       if (bucket(x) == b) in.push_back(x);
@@ -231,7 +270,8 @@ double median_lower(const std::vector<double>& values) {
 // stats.rs:14-18
 double mad_scale(const std::vector<double>& values) {
   const double m = median_lower(values);
-  std::vector<double> dev(values.size());
+  std::vector<double>& dev = scratch().mad_dev;
+  dev.resize(values.size());
   for (size_t i = 0; i < values.size(); i++) dev[i] = std::fabs(values[i] - m);
   return 1.4826 * median_lower(dev);
 }
@@ -249,7 +289,9 @@ struct Noise {
 int fit_segment_noise(const double* time, const double* flux, const double* err,
                       const int64_t* group, size_t n, double bin_minutes, double gap,
                       Noise* out, std::string* msg) {
-  std::vector<size_t> keep;
+  Scratch& sc = scratch();
+  std::vector<size_t>& keep = sc.keep;
+  keep.clear();
   for (size_t i = 0; i < n; i++) {
     const double e = err ? err[i] : 1.0;
     if (std::isfinite(time[i]) && std::isfinite(flux[i]) && std::isfinite(e) && e > 0.0)
@@ -266,8 +308,12 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
     return FBLS_E_DATA;
   }
   size_t m = keep.size();
-  std::vector<double> t(m), f(m), e(m);
-  std::vector<int64_t> sec(m);
+  std::vector<double>&t = sc.t, &f = sc.f, &e = sc.e;
+  std::vector<int64_t>& sec = sc.sec;
+  t.resize(m);
+  f.resize(m);
+  e.resize(m);
+  sec.resize(m);
   for (size_t k = 0; k < m; k++) {
     t[k] = time[keep[k]];
     f[k] = flux[keep[k]];
@@ -287,7 +333,8 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
       }
     }
   std::sort(labels.begin(), labels.end());
-  std::vector<uint32_t> gi(m);
+  std::vector<uint32_t>& gi = sc.gi;
+  gi.resize(m);
   std::vector<size_t> gcount(labels.size(), 0);
   for (size_t k = 0; k < m; k++) {
     gi[k] = (uint32_t)(std::lower_bound(labels.begin(), labels.end(), sec[k]) - labels.begin());
@@ -295,7 +342,8 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
   }
   std::vector<double> med(labels.size());
   for (size_t g = 0; g < labels.size(); g++) {
-    std::vector<double> vals;
+    std::vector<double>& vals = sc.vals;
+    vals.clear();
     vals.reserve(gcount[g]);
     for (size_t k = 0; k < m; k++)
       if (gi[k] == g) vals.push_back(f[k]);
@@ -354,7 +402,9 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
     }
     m = u;
   }
-  std::vector<double> deltas, standardized;
+  std::vector<double>&deltas = sc.deltas, &standardized = sc.standardized;
+  deltas.clear();
+  standardized.clear();
   for (size_t i = 1; i < m; i++) {
     if (t[i] - t[i - 1] < gap && sec[i] == sec[i - 1]) {
       const double d = f[i] - f[i - 1];
@@ -386,7 +436,7 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
     if (t[i] - t[i - 1] > gap || sec[i] != sec[i - 1]) seg++;
   out->deficit.resize(m);
   for (size_t k = 0; k < m; k++) out->deficit[k] = -f[k];
-  out->t = std::move(t);
+  out->t.assign(t.begin(), t.begin() + m);
   out->origin = origin;
   out->inflation = inflation;
   out->segments = seg;
@@ -406,9 +456,12 @@ void correlated_noise_factors(const std::vector<double>& time,
   const double min_fill = 0.5;
   const size_t minimum_bins = 20;
   const size_t n = time.size();
-  std::vector<double> weights(n);
+  Scratch& sc = scratch();
+  std::vector<double>& weights = sc.weights;
+  weights.resize(n);
   for (size_t i = 0; i < n; i++) weights[i] = 1.0 / (sigma[i] * sigma[i]);
-  std::vector<double> steps;
+  std::vector<double>& steps = sc.steps;
+  steps.clear();
   for (size_t i = 1; i < n; i++) {
     const double s = time[i] - time[i - 1];
     if (s > 0.0) steps.push_back(s);
@@ -416,69 +469,62 @@ void correlated_noise_factors(const std::vector<double>& time,
   const double cadence = median_lower(steps);
   std::vector<double> measured(nd);
   std::vector<size_t> counts(nd);
-  // Durations four at a time: one pass over the points advances the four
-  // durations' bins together (four independent chains of sums, and a branch
-  // only where some duration's cell changes); within a duration the points
-  // are added in the same order as a one-duration pass, so every bin's sums
-  // are the same doubles.
-  // cell = floor(fl(x / d)) for x = t - t0: from x * (1 / d) when that
-  // product is farther than 1e-12 relative from an integer (the two differ
-  // by a few ulps, so their floors agree), from the division otherwise
-  std::vector<double> x(n), wr(n);
+  // One pass over the points advances every duration's bins.  A point's
+  // bin is floor(fl(x / d)) for x = t - t0, nondecreasing in x because the
+  // division is; each duration keeps the smallest x of its next bin (the
+  // boundary), so a point costs one comparison per duration, and the bin of
+  // a point at or past it comes from the division.  Within a duration the
+  // points are added in order from zero at each bin's first point, as in a
+  // pass of that duration alone, so every bin's sums are the same doubles.
+  std::vector<double>&x = sc.x, &wr = sc.wr;
+  x.resize(n);
+  wr.resize(n);
   for (size_t i = 0; i < n; i++) {
     x[i] = time[i] - time[0];
     wr[i] = weights[i] * residual[i];
   }
-  std::vector<std::vector<double>> z(nd);
-  constexpr size_t kLanes = 4;
-  for (size_t d0 = 0; d0 < nd; d0 += kLanes) {
-    const size_t lanes = std::min(kLanes, nd - d0);
-    double inv[kLanes], dd[kLanes], need[kLanes], total[kLanes], acc[kLanes];
-    int64_t current[kLanes];
-    size_t cnt[kLanes];
-    for (size_t l = 0; l < kLanes; l++) {
-      // lanes past the last duration repeat it and are never read out
-      dd[l] = durations[d0 + std::min(l, lanes - 1)];
-      inv[l] = 1.0 / dd[l];
-      need[l] = min_fill * dd[l] / cadence;
-      total[l] = acc[l] = 0.0;
-      current[l] = INT64_MIN;
-      cnt[l] = 0;
+  std::vector<std::vector<double>>& z = sc.z;
+  if (z.size() < nd) z.resize(nd);
+  for (size_t d = 0; d < nd; d++) z[d].clear();
+  std::vector<double> need(nd), total(nd, 0.0), acc(nd, 0.0), edge(nd, -INFINITY);
+  std::vector<size_t> cnt(nd, 0);
+  for (size_t l = 0; l < nd; l++) need[l] = min_fill * durations[l] / cadence;
+  auto flush = [&](size_t l) {
+    if (cnt[l] > 0 && (double)cnt[l] >= need[l])
+      z[l].push_back((acc[l] / total[l]) * std::sqrt(total[l]));
+  };
+  // the smallest double b with fl(b / d) >= c + 1: the first x of the bins
+  // after bin c (b = (c + 1) d within a few ulps, then stepped exactly)
+  auto next_edge = [](double c, double d) {
+    const double target = c + 1.0;
+    double e = target * d;
+    while (!(e / d >= target)) e = std::nextafter(e, INFINITY);
+    for (;;) {
+      const double below = std::nextafter(e, -INFINITY);
+      if (!(below / d >= target)) break;
+      e = below;
     }
-    auto flush = [&](size_t l) {
-      if (l < lanes && cnt[l] > 0 && (double)cnt[l] >= need[l])
-        z[d0 + l].push_back((acc[l] / total[l]) * std::sqrt(total[l]));
-    };
-    for (size_t i = 0; i < n; i++) {
-      const double xi = x[i];
-      int64_t cell[kLanes];
-      bool change = false;
-      for (size_t l = 0; l < kLanes; l++) {
-        const double q = xi * inv[l];
-        const double fq = std::floor(q);
-        const double margin = 1e-12 * (q > 1.0 ? q : 1.0);
-        cell[l] = (int64_t)fq;
-        if (!(q - fq > margin && fq + 1.0 - q > margin)) cell[l] = (int64_t)std::floor(xi / dd[l]);
-        change = change || cell[l] != current[l];
-      }
-      if (change) {
-        for (size_t l = 0; l < kLanes; l++) {
-          if (cell[l] == current[l]) continue;
-          flush(l);
-          current[l] = cell[l];
-          total[l] = 0.0;
-          acc[l] = 0.0;
-          cnt[l] = 0;
-        }
-      }
-      for (size_t l = 0; l < kLanes; l++) {
-        total[l] += weights[i];
-        acc[l] += wr[i];
-        cnt[l]++;
+    return e;
+  };
+  for (size_t i = 0; i < n; i++) {
+    const double xi = x[i];
+    for (size_t l = 0; l < nd; l++) {
+      if (xi >= edge[l]) {  // a new bin of duration l
+        flush(l);
+        edge[l] = next_edge(std::floor(xi / durations[l]), durations[l]);
+        total[l] = 0.0;
+        acc[l] = 0.0;
+        cnt[l] = 0;
       }
     }
-    for (size_t l = 0; l < lanes; l++) flush(l);
+    const double wi = weights[i], ri = wr[i];
+    for (size_t l = 0; l < nd; l++) {
+      total[l] += wi;
+      acc[l] += ri;
+      cnt[l]++;
+    }
   }
+  for (size_t l = 0; l < nd; l++) flush(l);
   for (size_t di = 0; di < nd; di++) {
     counts[di] = z[di].size();
     measured[di] = z[di].size() >= 3 ? max1(mad_scale(z[di])) : 1.0;
@@ -503,7 +549,8 @@ void correlated_noise_factors(const std::vector<double>& time,
       inherited[di] = measured[anchor] * std::sqrt(ratio >= 1.0 ? ratio : 1.0);
     }
   } else {
-    std::vector<double> st(n);
+    std::vector<double>& st = sc.st;
+    st.resize(n);
     for (size_t i = 0; i < n; i++) st[i] = residual[i] / sigma[i];
     const double scatter = max1(mad_scale(st));
     for (size_t di = 0; di < nd; di++) {
@@ -602,9 +649,11 @@ int family_state(const double* time, const double* flux, const double* sigma,
                  double* inflation, size_t* segments, double* residual_energy,
                  std::string* msg) {
   // curve.rs:547-548: flux1 = 1 + flux, fit without binning, gap 0.75 d
-  std::vector<double> flux1(n);
+  Scratch& sc = scratch();
+  std::vector<double>& flux1 = sc.flux1;
+  flux1.resize(n);
   for (size_t i = 0; i < n; i++) flux1[i] = 1.0 + flux[i];
-  Noise model;
+  thread_local Noise model;
   int st = fit_segment_noise(time, flux1.data(), sigma, n, 0.75, &model, msg);
   if (st) return st;
   const size_t m = model.t.size();
@@ -640,7 +689,9 @@ int family_state(const double* time, const double* flux, const double* sigma,
   *inflation = model.inflation;
   *segments = model.segments;
   if (correlated) {
-    std::vector<double> resid(m), sig(m);
+    std::vector<double>&resid = sc.resid, &sig = sc.sig;
+    resid.resize(m);
+    sig.resize(m);
     for (size_t i = 0; i < m; i++) {
       resid[i] = -model.deficit[i];
       sig[i] = std::sqrt(model.variance[i]);
@@ -649,6 +700,9 @@ int family_state(const double* time, const double* flux, const double* sigma,
   } else {
     for (size_t i = 0; i < nd; i++) rho[i] = 1.0;
   }
+  // an unusually large star does not keep its buffers (64 MiB per thread)
+  sc.trim((size_t)64 << 20);
+  if (model.t.capacity() > ((size_t)8 << 20)) model = Noise();
   return 0;
 }
 

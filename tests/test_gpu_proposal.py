@@ -3,7 +3,8 @@
 * fbls_coarse_features against the oracle (bit-exact: exact fixed-point
   sums, the same FP64 expressions).
 * A keep-masked batch search returns exactly the full search's rows at the
-  kept periods.
+  kept periods, also when nearly every period passes the prescan (the
+  restricted search's box-first prescan).
 * With the committed trained network: on fresh held-out stars (seeds never
   used in training), the pruned search keeps >= 95 % of the full scan's
   recovered injections at the saved operating point, and the network
@@ -81,6 +82,31 @@ def test_keep_mask_rows_are_full_rows(gpu):
 
 
 @pytest.mark.skipif(not os.path.exists(PR.DEFAULT_WEIGHTS), reason="no trained proposal")
+def test_box_first_prescan_keeps_full_rows(gpu):
+    """At a low threshold nearly every period passes the prescan.  A search
+    restricted to every period (boxes first in the prescan; the staircase
+    templates only for periods whose best box stays below the cut) returns
+    the full search's rows and prescan survivor counts."""
+    pc = PR.ProposalConfig()
+    cfg = F.SearchConfig(period_min=pc.period_min, period_max=pc.period_max, threshold=3.0)
+    res = PR.make_states(range(3), 4_900_200, cfg, workers=1)
+    states = [r[0] for r in res]
+    devs = F.DeviceReady.upload_packed(gpu, states)
+    d = cfg.durations_days()
+    rho = [s.rho for s in states]
+    rows, st = F.signal_search_batch(gpu, devs, d, rho, cfg)
+    keeps = [np.ones(len(g), bool) for g in PR.grids_of(devs, cfg)]
+    krow, kst = F.signal_search_batch(gpu, devs, d, rho, cfg, keep=keeps)
+    for k in range(len(states)):
+        print(f"[box-first] star {k}: rows {len(rows[k])}, periods {st[k]['periods']}, "
+              f"survivors {st[k]['prescan_survivors']}", flush=True)
+        np.testing.assert_array_equal(krow[k], rows[k])
+        assert kst[k]["prescan_survivors"] == st[k]["prescan_survivors"]
+        assert st[k]["prescan_survivors"] >= 0.5 * st[k]["periods"]
+    for x in devs:
+        x.close()
+
+
 def test_trained_proposal_recall_and_gain(gpu):
     net, pc, extra = PR.load(gpu)
     cfg = PR.search_config(pc)
