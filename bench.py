@@ -166,19 +166,22 @@ def long_curves(n_stars):
 
 
 def prepare(curves, cfg):
-    from fbls_amd import family_state
+    from fbls_amd import family_states
 
     d = cfg.durations_days()
-    return _threads(lambda c: family_state(c[0], c[1] - 1.0, c[2], d, cfg.correlated_noise),
-                    curves)
+    out = []
+    for a in range(0, len(curves), 128):
+        out += family_states(curves[a:a + 128], d, cfg.correlated_noise, flux_offset=-1.0)
+    return out
 
 
 def end_to_end(gpu, curves, injected, cfg, net=None, pc=None, op=None, mode="pruned"):
     """The full scan and the learned-pruned search of the same stars, each
-    timed end to end from light curves in host memory (preparation on 16 host
-    threads pipelined with the GPU, upload, proposal, search, selection),
-    with the recall of the pruned search against the full scan and the host
-    preparation rate on 1 and 16 threads."""
+    timed end to end from light curves in host memory (preparation on the
+    library's preparation threads pipelined with the GPU, upload, proposal,
+    search, selection), with the recall of the pruned search against the
+    full scan and the host preparation rate one star at a time and on the
+    library's threads."""
     from fbls_amd import endtoend as E
     from fbls_amd import proposal as PR
 
@@ -200,13 +203,13 @@ def end_to_end(gpu, curves, injected, cfg, net=None, pc=None, op=None, mode="pru
     sample = curves[:min(64, len(curves))]
     out["host_preparation_stars_per_hour"] = {
         "1_thread": E.preparation_rate(sample, cfg, 1),
-        "16_threads": E.preparation_rate(curves, cfg, 16)}
+        f"{E.PREP_THREADS}_threads": E.preparation_rate(curves, cfg, E.PREP_THREADS)}
     return out
 
 
 def batched(gpu, data):
     """The full scan of every star of the batch, end to end: preparation on
-    16 host threads pipelined with the packed upload and the merged-launch
+    the library's preparation threads pipelined with the packed upload and the merged-launch
     search (fbls_signal_search_batch) of chunks of 128 stars on two device
     contexts, and selection.  Rates are per wall second; the kernel seconds
     add up both contexts' launches (which overlap in time)."""
@@ -240,7 +243,7 @@ def batched(gpu, data):
         "injections": sum(1 for c in inj if c > 0),
         "host_preparation_stars_per_hour": {
             "1_thread": E.preparation_rate(curves[:16], cfg, 1),
-            "16_threads": E.preparation_rate(curves, cfg, 16)},
+            f"{E.PREP_THREADS}_threads": E.preparation_rate(curves, cfg, E.PREP_THREADS)},
     }
 
 

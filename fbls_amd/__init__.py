@@ -60,6 +60,7 @@ EXPORTS = (
     "fbls_format_f64", "fbls_json_reformat", "fbls_cli", "fbls_coarse_inputs",
     "fbls_cnn_set_fused", "fbls_signal_search_bins", "fbls_propose",
     "fbls_signal_search_bins_ragged", "fbls_scan_bank_noalloc", "fbls_family_upload_list",
+    "fbls_family_state_many", "fbls_family_prepare_upload_many",
 )
 
 
@@ -350,6 +351,14 @@ def lib():
             _P(_d), _P(_d), _P(_d), _sz, _P(_d), _sz, ctypes.c_int, _P(_d),
             _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz),
             _P(_d)]
+        L.fbls_family_state_many.argtypes = [
+            _sz, _P(_P(_d)), _P(_P(_d)), _P(_P(_d)), _P(_sz), _d, _P(_d), _sz, ctypes.c_int,
+            _P(_P(_d)), _P(_P(_d)), _P(_P(_d)), _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d),
+            _P(_sz), _P(_d), _P(ctypes.c_int)]
+        L.fbls_family_prepare_upload_many.argtypes = [
+            _vp, _sz, _P(_P(_d)), _P(_P(_d)), _P(_P(_d)), _P(_sz), _d, _P(_d), _sz,
+            ctypes.c_int, _P(_sz), _P(_d), _P(_d), _P(_d), _P(_d), _P(_sz), _P(_d), _P(_d),
+            _P(ctypes.c_int), _P(_vp)]
         L.fbls_synth_star.argtypes = [_P(SynthCfg), _P(_d), _P(_d), _P(_d),
                                       _sz, _P(_sz)]
         L.fbls_plan_bands.argtypes = [_P(_d), _sz, _P(_d), _sz, _u32, _u32,
@@ -536,6 +545,45 @@ def family_state(time, flux, sigma, durations, correlated=True):
                        en.value)
 
 
+def family_states(curves, durations, correlated=True, flux_offset=0.0):
+    """family_state of many curves ((time, flux, sigma) each) in one call,
+    prepared on the library's own threads (fbls_family_state_many); with
+    flux_offset=-1.0 the fluxes are normalised to 1 and the result equals
+    family_state(time, flux - 1.0, sigma, ...).  The states' arrays are
+    views into three shared buffers."""
+    k = len(curves)
+    if k == 0:
+        return []
+    durations = _f64(durations)
+    nd = len(durations)
+    ins = [(_f64(a), _f64(b), _f64(c)) for a, b, c in curves]  # kept alive for the call
+    n = np.array([len(a[0]) for a in ins], dtype=np.uintp)
+    at = np.concatenate(([0], np.cumsum(n)))
+    T, V, W = np.empty(int(at[-1])), np.empty(int(at[-1])), np.empty(int(at[-1]))
+    PD = _P(_d)
+
+    def ptrs(arrs):
+        return (PD * k)(*[_dp(x) for x in arrs])
+
+    def parts(buf):
+        return (PD * k)(*[ctypes.cast(buf.ctypes.data + 8 * int(at[i]), PD) for i in range(k)])
+
+    m, c, org, infl, en = np.empty(k, np.uintp), np.empty(k), np.empty(k), np.empty(k), np.empty(k)
+    seg = np.empty(k, np.uintp)
+    rho = np.empty((k, nd))
+    _check(lib().fbls_family_state_many(
+        k, ptrs([a[0] for a in ins]), ptrs([a[1] for a in ins]), ptrs([a[2] for a in ins]),
+        n.ctypes.data_as(_P(_sz)), float(flux_offset), _dp(durations), nd,
+        int(bool(correlated)), parts(T), parts(V), parts(W), m.ctypes.data_as(_P(_sz)),
+        _dp(c), _dp(org), _dp(rho), _dp(infl), seg.ctypes.data_as(_P(_sz)), _dp(en), None))
+    out = []
+    for i in range(k):
+        a, e = int(at[i]), int(at[i]) + int(m[i])
+        out.append(FamilyState(T[a:e], V[a:e], W[a:e], float(c[i]), float(org[i]), durations,
+                               rho[i], float(infl[i]), int(seg[i]), float(en[i])))
+    return out
+
+
 def synth_star(**kw):
     """Seeded synthetic light curve: returns (time, flux, flux_err)."""
     cfg = SynthCfg(seed=1, sectors=1, sector_days=27.4, gap_days=1.0,
@@ -651,6 +699,49 @@ class DeviceReady:
             d._h = _vp(hs[i])
             out.append(d)
         return out
+
+    @classmethod
+    def prepare_upload(cls, gpu, curves, durations, correlated=True, flux_offset=0.0):
+        """family_states of the curves ((time, flux, sigma) each) and their
+        upload in one call (fbls_family_prepare_upload_many): the families
+        on the device, and a dict of the preparation's per-star outputs
+        (rho as a stars x durations array; n, baseline_norm, origin,
+        inflation, segments, residual_energy)."""
+        k = len(curves)
+        durations = _f64(durations)
+        nd = len(durations)
+        if k == 0:
+            return [], {"rho": np.empty((0, nd))}
+        ins = [(_f64(a), _f64(b), _f64(c)) for a, b, c in curves]  # kept alive for the call
+        PD = _P(_d)
+
+        def ptrs(arrs):
+            return (PD * k)(*[_dp(x) for x in arrs])
+
+        n = np.array([len(a[0]) for a in ins], dtype=np.uintp)
+        m, seg = np.empty(k, np.uintp), np.empty(k, np.uintp)
+        c, org, infl, en, tl = np.empty(k), np.empty(k), np.empty(k), np.empty(k), np.empty(k)
+        rho = np.empty((k, nd))
+        hs = (_vp * k)()
+        _check(lib().fbls_family_prepare_upload_many(
+            gpu._h, k, ptrs([a[0] for a in ins]), ptrs([a[1] for a in ins]),
+            ptrs([a[2] for a in ins]), n.ctypes.data_as(_P(_sz)), float(flux_offset),
+            _dp(durations), nd, int(bool(correlated)), m.ctypes.data_as(_P(_sz)), _dp(c),
+            _dp(org), _dp(rho), _dp(infl), seg.ctypes.data_as(_P(_sz)), _dp(en), _dp(tl), None,
+            hs))
+        out = []
+        for i in range(k):
+            d = cls.__new__(cls)
+            d.gpu = gpu
+            d.n = int(m[i])
+            d.baseline_norm = float(c[i])
+            d.origin = float(org[i])
+            d._h = _vp(hs[i])
+            d.t_last = float(tl[i])
+            out.append(d)
+        info = {"rho": rho, "n": m.astype(np.int64), "baseline_norm": c, "origin": org,
+                "inflation": infl, "segments": seg.astype(np.int64), "residual_energy": en}
+        return out, info
 
     def fields(self):
         """The fixed-point values the device scans (exact doubles)."""

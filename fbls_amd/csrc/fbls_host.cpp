@@ -17,6 +17,10 @@
 #include <thread>
 #include <vector>
 
+#include <sys/resource.h>
+#include <sys/syscall.h>
+#include <unistd.h>
+
 #include "../../include/fbls.h"
 #include "prep.h"
 #include "band_kernel.h"
@@ -59,7 +63,14 @@ namespace {
 class Pool {
  public:
   static Pool& get() {
-    static Pool* pool = new Pool();  // never destroyed: workers outlive exit handlers
+    static Pool* pool = new Pool(false);  // never destroyed: workers outlive exit handlers
+    return *pool;
+  }
+  // The preparation threads (fbls_family_state_many): a pool of their own at
+  // a lower CPU priority (nice + 5), so that a batch being prepared does not
+  // delay the threads that plan searches and feed the device.
+  static Pool& background() {
+    static Pool* pool = new Pool(true);
     return *pool;
   }
   size_t workers() const { return threads_.size(); }
@@ -100,12 +111,16 @@ class Pool {
     size_t a, e;
   };
 
-  Pool() {
+  explicit Pool(bool background) {
     const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
-    const size_t n = std::min<size_t>(hw, 16) - 1;
+    const size_t n = std::min<size_t>(hw, 16) - (background ? 0 : 1);
     for (size_t k = 0; k < n; k++) {
       try {
-        threads_.emplace_back([this]() { loop(); });
+        threads_.emplace_back([this, background]() {
+          if (background)  // this thread only; refused or not, it still works
+            (void)setpriority(PRIO_PROCESS, (id_t)syscall(SYS_gettid), 5);
+          loop();
+        });
       } catch (const std::system_error&) {
         break;  // fewer workers: the callers run more pieces themselves
       }
@@ -432,6 +447,100 @@ int upload_many(fbls_ctx* c, size_t nfam, const double* const* t, const double* 
 }
 
 }  // namespace
+
+extern "C" int fbls_family_prepare_upload_many(
+    fbls_ctx* c, size_t count, const double* const* time, const double* const* flux,
+    const double* const* sigma, const size_t* n, double flux_offset, const double* durations,
+    size_t nd, int correlated, size_t* out_n, double* baseline_norm, double* origin,
+    double* rho, double* inflation, size_t* segments, double* residual_energy, double* t_last,
+    int* status, fbls_family** out) {
+  return fbls::host::guard([&]() -> int {
+    if (!c || !out) return fail(FBLS_E_ARG, "null argument");
+    for (size_t k = 0; k < count; k++) out[k] = nullptr;
+    if (count == 0) return FBLS_OK;
+    if (!time || !flux || !sigma || !n || !out_n || !baseline_norm || !origin ||
+        !inflation || !segments || (nd && (!durations || !rho)))
+      return fail(FBLS_E_ARG, "null argument");
+    // staging slots sized by the input counts (a star keeps at most n[k])
+    std::vector<size_t> at(count + 1, 0);
+    for (size_t k = 0; k < count; k++)
+      at[k + 1] = at[k] + (n[k] + kObsAlign - 1) / kObsAlign * kObsAlign * 16;
+    std::vector<std::unique_ptr<fbls_family>> owned(count);
+    for (auto& f : owned) f.reset(new fbls_family());
+    std::vector<int> codes(count, FBLS_OK);
+    std::vector<std::string> msgs(count);
+    auto arena = std::make_shared<DevArena>();
+    arena->pool = c->arenas;
+    HIP_TRY(hipSetDevice(c->device));
+    if (int st = c->arenas->take(at[count], &arena->buf)) return st;
+    {
+      // in pieces of at most 256 MiB of staging (a larger star alone)
+      constexpr size_t kPiece = (size_t)256 << 20;
+      std::lock_guard<std::mutex> hold(c->up_mutex);
+      for (size_t k0 = 0; k0 < count;) {
+        size_t k1 = k0 + 1;
+        while (k1 < count && at[k1 + 1] - at[k0] <= kPiece) k1++;
+        if (int st = c->pin_up.ensure(at[k1] - at[k0])) return st;
+        unsigned char* img = (unsigned char*)c->pin_up.ptr;
+        Pool::background().run(k1 - k0, k1 - k0, [&](size_t a, size_t e) {
+          struct Out {
+            std::vector<double> t, v, w;
+          };
+          thread_local Out o;
+          for (size_t k = k0 + a; k < k0 + e; k++) {
+            try {
+              o.t.resize(n[k]);
+              o.v.resize(n[k]);
+              o.w.resize(n[k]);
+              double energy = 0.0;
+              codes[k] = fbls::family_state(time[k], flux[k], sigma[k], n[k], durations, nd,
+                                            correlated, o.t.data(), o.v.data(), o.w.data(),
+                                            out_n + k, baseline_norm + k, origin + k,
+                                            rho + k * nd, inflation + k, segments + k,
+                                            &energy, &msgs[k], flux_offset);
+              if (residual_energy) residual_energy[k] = energy;
+              if (codes[k]) continue;
+              codes[k] = prepare_family(c, o.t.data(), o.v.data(), o.w.data(), out_n[k],
+                                        baseline_norm[k], origin[k], owned[k].get());
+              if (codes[k]) {
+                msgs[k] = fbls_last_error();
+                continue;
+              }
+              write_family_image(owned[k].get(), o.t.data(), img + (at[k] - at[k0]));
+              if (o.t.capacity() > ((size_t)4 << 20)) o = Out();  // an unusually large star
+            } catch (const std::bad_alloc&) {
+              codes[k] = FBLS_E_OOM;
+              msgs[k] = "host memory allocation failed";
+            } catch (const std::exception& x) {
+              codes[k] = FBLS_E_ARG;
+              msgs[k] = x.what();
+            }
+          }
+        });
+        HIP_TRY(hipMemcpyAsync((unsigned char*)arena->buf.ptr + at[k0], img, at[k1] - at[k0],
+                               hipMemcpyHostToDevice, c->up_stream));
+        HIP_TRY(hipStreamSynchronize(c->up_stream));
+        k0 = k1;
+      }
+    }
+    size_t first = count;
+    for (size_t k = 0; k < count; k++) {
+      if (status) status[k] = codes[k];
+      if (codes[k] && first == count) first = k;
+    }
+    if (first < count) return fail(codes[first], "star %zu: %s", first, msgs[first].c_str());
+    for (size_t k = 0; k < count; k++) {
+      fbls_family* f = owned[k].get();
+      if (t_last) t_last[k] = f->t_last;
+      unsigned char* base = (unsigned char*)arena->buf.ptr + at[k];
+      f->arena = arena;
+      f->t_dev = (const double*)base;
+      f->vw_dev = (const int32_t*)(base + (size_t)f->n_pad * 8);
+    }
+    for (size_t k = 0; k < count; k++) out[k] = owned[k].release();
+    return FBLS_OK;
+  });
+}
 
 extern "C" int fbls_family_upload_packed(fbls_ctx* c, size_t nfam,
                                          const size_t* offsets, const double* t,
@@ -1934,6 +2043,49 @@ extern "C" int fbls_family_state(const double* time, const double* flux,
                                 inflation, segments, &energy, &msg);
     if (residual_energy) *residual_energy = energy;
     if (st) return fail(st, "%s", msg.c_str());
+    return FBLS_OK;
+  });
+}
+
+extern "C" int fbls_family_state_many(
+    size_t count, const double* const* time, const double* const* flux,
+    const double* const* sigma, const size_t* n, double flux_offset,
+    const double* durations, size_t nd, int correlated, double* const* t, double* const* v,
+    double* const* w, size_t* out_n, double* baseline_norm, double* origin, double* rho,
+    double* inflation, size_t* segments, double* residual_energy, int* status) {
+  return fbls::host::guard([&]() -> int {
+    if (count == 0) return FBLS_OK;
+    if (!time || !flux || !sigma || !n || !t || !v || !w || !out_n || !baseline_norm ||
+        !origin || !inflation || !segments || (nd && (!durations || !rho)))
+      return fail(FBLS_E_ARG, "null argument");
+    std::vector<int> codes(count, FBLS_OK);
+    std::vector<std::string> msgs(count);
+    // one star per piece: the threads take the stars in order as they free
+    Pool::background().run(count, count, [&](size_t a, size_t e) {
+      for (size_t k = a; k < e; k++) {
+        try {
+          double energy = 0.0;
+          codes[k] = fbls::family_state(time[k], flux[k], sigma[k], n[k], durations, nd,
+                                        correlated, t[k], v[k], w[k], out_n + k,
+                                        baseline_norm + k, origin + k, rho + k * nd,
+                                        inflation + k, segments + k, &energy, &msgs[k],
+                                        flux_offset);
+          if (residual_energy) residual_energy[k] = energy;
+        } catch (const std::bad_alloc&) {
+          codes[k] = FBLS_E_OOM;
+          msgs[k] = "host memory allocation failed";
+        } catch (const std::exception& x) {
+          codes[k] = FBLS_E_ARG;
+          msgs[k] = x.what();
+        }
+      }
+    });
+    size_t first = count;
+    for (size_t k = 0; k < count; k++) {
+      if (status) status[k] = codes[k];
+      if (codes[k] && first == count) first = k;
+    }
+    if (first < count) return fail(codes[first], "star %zu: %s", first, msgs[first].c_str());
     return FBLS_OK;
   });
 }

@@ -647,12 +647,14 @@ int family_state(const double* time, const double* flux, const double* sigma,
                  double* t, double* v, double* w, size_t* out_n,
                  double* baseline_norm, double* origin, double* rho,
                  double* inflation, size_t* segments, double* residual_energy,
-                 std::string* msg) {
+                 std::string* msg, double flux_offset) {
   // curve.rs:547-548: flux1 = 1 + flux, fit without binning, gap 0.75 d
+  // (flux + flux_offset: the flux relative to zero of a curve normalised to
+  // 1 - flux_offset, rounded as the caller's own subtraction would be)
   Scratch& sc = scratch();
   std::vector<double>& flux1 = sc.flux1;
   flux1.resize(n);
-  for (size_t i = 0; i < n; i++) flux1[i] = 1.0 + flux[i];
+  for (size_t i = 0; i < n; i++) flux1[i] = 1.0 + (flux[i] + flux_offset);
   thread_local Noise model;
   int st = fit_segment_noise(time, flux1.data(), sigma, n, 0.75, &model, msg);
   if (st) return st;

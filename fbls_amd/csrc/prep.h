@@ -69,7 +69,7 @@ int family_state(const double* time, const double* flux, const double* sigma,
                  double* t, double* v, double* w, size_t* out_n,
                  double* baseline_norm, double* origin, double* rho,
                  double* inflation, size_t* segments, double* residual_energy,
-                 std::string* msg);
+                 std::string* msg, double flux_offset = 0.0);
 
 // One detrended family of the raw path (curve.rs:23-41 PreparedCurve).
 struct Family {

@@ -2,11 +2,12 @@
 src/batch.rs:74-106: every second from a star's light curve to its result
 counts): from light curves in host memory to each star's best row.
 
-Host preparation (`family_state`, curve.rs:541-569) runs on worker threads
--- the library's calls release the interpreter lock -- for the chunks ahead,
-the upload of chunk k + 1 runs on its own thread (and the library's upload
-stream) while the GPU searches and selects chunk k.  The search of a chunk
-is one of:
+Host preparation (`family_state`, curve.rs:541-569) and upload of the
+chunks ahead run on the library's own preparation threads, one
+fbls_family_prepare_upload_many call per chunk (each star quantised and
+written to pinned staging as it is prepared, then one copy on the library's
+upload stream), in chunk order from one Python thread, while the GPU
+searches and selects the chunks before.  The search of a chunk is one of:
 
   full     the packed upload and the merged-launch full scan of every star
            (fbls_signal_search_batch);
@@ -22,19 +23,27 @@ the seconds of each stage (the time the search waited for a prepared and
 uploaded chunk in `wait`).
 """
 import concurrent.futures as cf
+import os
 import threading
 import time
 
 import numpy as np
 
-from . import (DeviceReady, family_state, propose, signal_search_batch, signal_search_bins,
-               signal_search_bins_ragged)
+from . import (DeviceReady, family_state, family_states, propose, signal_search_batch,
+               signal_search_bins, signal_search_bins_ragged)
 from . import proposal as PR
+
+# threads of the library's preparation pool (fbls_family_state_many)
+PREP_THREADS = min(os.cpu_count() or 1, 16)
 
 
 def _prepare(curve, durations, correlated):
     t, f, e = curve
     return family_state(t, f - 1.0, e, durations, correlated)
+
+
+def _prepare_many(curves, durations, correlated):
+    return family_states(curves, durations, correlated, flux_offset=-1.0)
 
 
 class Stages:
@@ -58,10 +67,9 @@ def _upload(gpu, states, mode):
     return devs, DeviceReady.upload_packed(gpu, [LB.binned_state(s) for s in states])
 
 
-def _search_full(gpu, states, devs, cfg, st):
+def _search_full(gpu, rho, devs, cfg, st):
     t1 = time.perf_counter()
-    rows, stats = signal_search_batch(gpu, devs, cfg.durations_days(),
-                                      [s.rho for s in states], cfg)
+    rows, stats = signal_search_batch(gpu, devs, cfg.durations_days(), rho, cfg)
     t2 = time.perf_counter()
     best = PR.best_rows(rows, cfg.threshold)
     t3 = time.perf_counter()
@@ -70,7 +78,7 @@ def _search_full(gpu, states, devs, cfg, st):
     return best, sum(s["templates_scanned"] for s in stats)
 
 
-def _search_pruned(gpu, states, devs, cfg, net, pc, op, st):
+def _search_pruned(gpu, rho, devs, cfg, net, pc, op, st):
     tau, topk, (lo, hi) = op[:3]
     gd = PR.group_durations(cfg, pc.groups)
     centers, edges = PR.bin_centers(pc), PR.bin_edges(pc)
@@ -82,8 +90,7 @@ def _search_pruned(gpu, states, devs, cfg, net, pc, op, st):
     conf = probs.max(axis=1)
     bins[(conf >= lo) & (conf < hi)] = True
     t2 = time.perf_counter()
-    rows, stats = signal_search_bins(gpu, devs, cfg.durations_days(), [s.rho for s in states],
-                                     cfg, bins, edges)
+    rows, stats = signal_search_bins(gpu, devs, cfg.durations_days(), rho, cfg, bins, edges)
     t3 = time.perf_counter()
     best = PR.best_rows(rows, cfg.threshold)
     t4 = time.perf_counter()
@@ -142,13 +149,12 @@ def _net_on(net, gpu):
     return copy
 
 
-def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, workers=16,
-        device_workers=2):
+def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, device_workers=2):
     """One end-to-end batch run over `curves` ((time, flux, flux_err) per
-    star, flux normalised to 1).  The stages overlap: preparation on
-    `workers` host threads (every star queued at once, in order), the
-    uploads in chunk order on their own thread and HIP stream (at most
-    device_workers + 1 chunks on the device and not yet searched), and the
+    star, flux normalised to 1).  The stages overlap: preparation and
+    upload on the library's preparation threads and upload stream (chunk
+    after chunk, in order; at most device_workers + 1 chunks on the device
+    and not yet searched), and the
     searches on `device_workers` threads, each with a device context of its
     own (and, for the learned modes, its own copy of the network), taking
     chunks in turn, so that one chunk's search overlaps the next one's
@@ -156,8 +162,9 @@ def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, wo
     --workers` (src/batch.rs:47-68).  The first two chunks are a quarter and
     a half of `chunk` stars.  Returns a dict with the stars' best rows, the
     wall seconds, stars/hour, trial evals, the stage seconds summed over the
-    device threads (`wait`: time a device thread waited for its chunk) and
-    the worker counts."""
+    device threads (`wait`: time a device thread waited for its chunk;
+    `prepare`, `upload`: the feeding thread's busy time) and the thread
+    counts."""
     if mode not in ("full", "pruned", "long"):
         raise ValueError(mode)
     from . import Gpu
@@ -179,16 +186,31 @@ def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, wo
     ahead = threading.Semaphore(nw + 1)
     stop = threading.Event()  # a device thread failed: pending uploads give up
     results = [None] * len(spans)
-    with cf.ThreadPoolExecutor(workers) as pool, cf.ThreadPoolExecutor(1) as up, \
-            cf.ThreadPoolExecutor(nw) as dev:
-        def upload(futs):
-            states = [f.result() for f in futs]
+    with cf.ThreadPoolExecutor(1) as feed, cf.ThreadPoolExecutor(nw) as dev:
+        def prepare_upload(a, b):
+            # at most nw + 1 chunks on the device and not yet searched
             while not ahead.acquire(timeout=0.1):
                 if stop.is_set():
                     raise RuntimeError("the search of an earlier chunk failed")
-            t0 = time.perf_counter()
-            devs, bdevs = _upload(gpu, states, mode)
-            return states, devs, bdevs, time.perf_counter() - t0
+            try:
+                t0 = time.perf_counter()
+                if mode == "long":  # the binned families come from the prepared arrays
+                    states = _prepare_many(curves[a:b], durations, cfg.correlated_noise)
+                    t1 = time.perf_counter()
+                    devs, bdevs = _upload(gpu, states, mode)
+                    rho = [s.rho for s in states]
+                else:  # each star's fixed point and image written as it is prepared
+                    devs, info = DeviceReady.prepare_upload(gpu, curves[a:b], durations,
+                                                            cfg.correlated_noise,
+                                                            flux_offset=-1.0)
+                    states, bdevs, rho = None, None, list(info["rho"])
+                    t1 = time.perf_counter()
+                st.add("prepare", t1 - t0)
+                st.add("upload", time.perf_counter() - t1)
+            except BaseException:
+                ahead.release()
+                raise
+            return states, rho, devs, bdevs
 
         def device_work(j):
             try:
@@ -201,14 +223,13 @@ def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, wo
             g, nt = ctxs[j], nets[j]
             for k in range(j, len(spans), nw):
                 t0 = time.perf_counter()
-                states, devs, bdevs, t_up = ups[k].result()
+                states, rho, devs, bdevs = ups[k].result()
                 st.add("wait", time.perf_counter() - t0)
-                st.add("upload", t_up)
                 try:
                     if mode == "full":
-                        results[k] = _search_full(g, states, devs, cfg, st)
+                        results[k] = _search_full(g, rho, devs, cfg, st)
                     elif mode == "pruned":
-                        results[k] = _search_pruned(g, states, devs, cfg, nt, pc, op, st)
+                        results[k] = _search_pruned(g, rho, devs, cfg, nt, pc, op, st)
                     else:
                         results[k] = _search_long(g, states, devs, bdevs, cfg, nt, pc, op, st)
                 finally:
@@ -217,9 +238,7 @@ def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, wo
                     ahead.release()
 
         t_start = time.perf_counter()
-        preps = [[pool.submit(_prepare, c, durations, cfg.correlated_noise)
-                  for c in curves[a:b]] for a, b in spans]
-        ups = [up.submit(upload, preps[k]) for k in range(len(spans))]
+        ups = [feed.submit(prepare_upload, a, b) for a, b in spans]
         for f in [dev.submit(device_work, j) for j in range(nw)]:
             f.result()
         wall = time.perf_counter() - t_start
@@ -235,18 +254,20 @@ def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, wo
             totals[k] += v
     return {"best": best, "seconds": wall, "stars": len(curves),
             "stars_per_hour": len(curves) * 3600.0 / wall, "trial_evals": templates,
-            "stage_seconds": st.seconds, "host_workers": workers, "device_workers": nw,
+            "stage_seconds": st.seconds, "host_workers": PREP_THREADS, "device_workers": nw,
             "chunk": chunk, "launch_totals": totals}
 
 
-def preparation_rate(curves, cfg, workers=1):
-    """Stars/hour of the host preparation alone on `workers` threads."""
+def preparation_rate(curves, cfg, threads=1):
+    """Stars/hour of the host preparation alone: one star at a time on the
+    calling thread (threads=1), or in chunks of 128 stars on the library's
+    preparation threads."""
     durations = cfg.durations_days()
     t0 = time.perf_counter()
-    if workers <= 1:
+    if threads <= 1:
         for c in curves:
             _prepare(c, durations, cfg.correlated_noise)
     else:
-        with cf.ThreadPoolExecutor(workers) as pool:
-            list(pool.map(lambda c: _prepare(c, durations, cfg.correlated_noise), curves))
+        for a in range(0, len(curves), 128):
+            _prepare_many(curves[a:a + 128], durations, cfg.correlated_noise)
     return len(curves) * 3600.0 / (time.perf_counter() - t0)

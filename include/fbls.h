@@ -576,6 +576,45 @@ int fbls_family_state(const double* time, const double* flux,
                       double* origin, double* rho, double* inflation,
                       size_t* segments, double* residual_energy);
 
+/* [host] fbls_family_state for `count` stars at once, on the library's
+ * preparation threads (a pool of their own at a lower CPU priority), as the
+ * reference's `batch` prepares its stars on worker threads before the
+ * search (src/batch.rs:47-68, curve.rs:541-569 per star).  Star k reads
+ * time[k], flux[k], sigma[k] of n[k] points and writes t[k], v[k], w[k]
+ * (sized n[k]), rho + k nd and element k of every other output, exactly as
+ * fbls_family_state of that star would; flux_offset is added to every flux
+ * value first (-1 for a curve normalised to 1, rounded as the caller's own
+ * flux - 1.0), 0 for flux already relative to zero.  residual_energy and
+ * status (each star's code) may be NULL.  Returns FBLS_OK, or the code of
+ * the first star that failed, with a message that names it; the other
+ * stars are prepared all the same. */
+int fbls_family_state_many(size_t count, const double* const* time,
+                           const double* const* flux, const double* const* sigma,
+                           const size_t* n, double flux_offset, const double* durations,
+                           size_t nd, int correlated, double* const* t, double* const* v,
+                           double* const* w, size_t* out_n, double* baseline_norm,
+                           double* origin, double* rho, double* inflation,
+                           size_t* segments, double* residual_energy, int* status);
+
+/* [device] fbls_family_state_many and the upload of the prepared families
+ * in one call: each preparation thread quantises its star and writes the
+ * star's device image into pinned staging as soon as the star is prepared,
+ * then the images are copied into one device allocation on the context's
+ * upload stream (the work of fbls_family_state_many followed by
+ * fbls_family_upload_list, without handing the prepared arrays back).
+ * Outputs as fbls_family_state_many (no t, v, w; t_last, which may be
+ * NULL, receives each star's last origin-relative time) plus one handle
+ * per star in out; every handle or none is written.  Returns FBLS_OK or the code of
+ * the first star that failed, with a message that names it. */
+int fbls_family_prepare_upload_many(fbls_ctx* ctx, size_t count, const double* const* time,
+                                    const double* const* flux, const double* const* sigma,
+                                    const size_t* n, double flux_offset,
+                                    const double* durations, size_t nd, int correlated,
+                                    size_t* out_n, double* baseline_norm, double* origin,
+                                    double* rho, double* inflation, size_t* segments,
+                                    double* residual_energy, double* t_last, int* status,
+                                    fbls_family** out);
+
 /* [host] fixed-point representation used on the device; returns e, the
  * words standing for integers in units of 2^-e.  *block (may be NULL) = 0:
  * q = rint(x * 2^e), |q| <= 2^30 (max |x| <= 16 mean |x|); 1: block mode,

@@ -127,3 +127,30 @@ def test_packed_and_list_uploads_agree(gpu, six_star_batch):
         np.testing.assert_array_equal(a, b)
     for dv in again:
         dv.close()
+
+
+def test_prepare_upload_equals_prepare_then_upload(gpu, six_star_batch):
+    """fbls_family_prepare_upload_many (each star quantised and staged by its
+    preparation thread) uploads the same fixed-point fields, times and
+    correlated-noise factors as family_state followed by the list upload,
+    and the batch search over its families returns the same rows."""
+    cfg, d, states, devs, rows, stats = six_star_batch
+    curves = [synth.batch_star(i)[0] for i in range(6)]
+    print("[batch] prepare + upload in one call", flush=True)
+    got, info = F.DeviceReady.prepare_upload(gpu, curves, d, cfg.correlated_noise,
+                                             flux_offset=-1.0)
+    for k, (g, s, dv) in enumerate(zip(got, states, devs)):
+        assert g.n == len(s.t) and g.t_last == dv.t_last and g.origin == s.origin
+        assert g.baseline_norm == s.baseline_norm
+        np.testing.assert_array_equal(info["rho"][k], s.rho)
+        for a, b in zip(g.fields(), dv.fields()):
+            np.testing.assert_array_equal(a, b)
+    rows2, _ = F.signal_search_batch(gpu, got, d, list(info["rho"]), cfg)
+    for a, b in zip(rows2, rows):
+        np.testing.assert_array_equal(a, b)
+    for g in got:
+        g.close()
+    bad = (np.arange(4.0), np.ones(4), np.ones(4))
+    with pytest.raises(F.FblsError) as ei:
+        F.DeviceReady.prepare_upload(gpu, [curves[0], bad], d, True, flux_offset=-1.0)
+    assert "star 1" in str(ei.value)

@@ -113,6 +113,45 @@ def test_family_state_kat_and_errors():
         F.family_state(t, -np.ones(n) * 3.0, np.ones(n), [0.05])  # median <= 0
 
 
+def test_family_states_many_equals_one_at_a_time():
+    """fbls_family_state_many (the library's preparation threads) gives each
+    star exactly fbls_family_state's result: normalised fluxes with the -1
+    offset, unsorted times with duplicates and non-finite points, the
+    uncorrelated path, and a failing star that names itself while the others
+    are still prepared."""
+    from fbls_amd import synth
+
+    d = F.SearchConfig().durations_days()
+    rng = np.random.default_rng(3)
+    curves = [synth.single_sector(), synth.four_sector_red()]
+    for k in range(6):
+        n = 300 + 700 * k
+        t = np.sort(rng.uniform(0.0, 20.0 + k, n))
+        if k % 2:
+            t = rng.permutation(np.concatenate([t, t[: n // 10]]))  # unsorted, duplicates
+        f = 1.0 + 1e-3 * rng.standard_normal(len(t))
+        e = np.full(len(t), 1e-3 * (1 + k))
+        if k == 3:
+            f[::17] = np.nan
+        curves.append((t, f, e))
+    for corr in (True, False):
+        many = F.family_states(curves, d, corr, flux_offset=-1.0)
+        for (t, f, e), b in zip(curves, many):
+            a = F.family_state(t, f - 1.0, e, d, corr)
+            for k in ("t", "v", "w", "rho"):
+                np.testing.assert_array_equal(getattr(b, k), getattr(a, k))
+            for k in ("baseline_norm", "origin", "inflation", "segments", "residual_energy"):
+                assert getattr(b, k) == getattr(a, k), k
+    o = O.family_state(*curves[0][:1], curves[0][1] - 1.0, curves[0][2], d)
+    np.testing.assert_array_equal(many[0].v, F.family_state(curves[0][0], curves[0][1] - 1.0,
+                                                            curves[0][2], d, False).v)
+    assert np.array_equal(F.family_states(curves[:1], d, True, -1.0)[0].rho, o["rho"])
+    bad = (np.arange(4.0), np.ones(4), np.ones(4))
+    with pytest.raises(F.FblsError) as ei:
+        F.family_states([curves[2], bad, curves[3]], d, True, -1.0)
+    assert ei.value.code == 6 and "star 1" in str(ei.value)
+
+
 def test_quantize_matches_oracle():
     rng = np.random.default_rng(9)
     for scale in (1e-3, 1.0, 1e6):
