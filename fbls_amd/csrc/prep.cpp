@@ -27,15 +27,24 @@ Fixed quantize(const double* x, size_t n, int32_t* q) {
   if (!f.block) {
     std::frexp(m, &ex);  // m = g * 2^ex, g in [0.5, 1)
     f.e = 30 - ex;
-    for (size_t i = 0; i < n; i++) q[i] = (int32_t)std::nearbyint(std::ldexp(x[i], f.e));
+    if (f.e >= -1022 && f.e <= 1023) {
+      // x * 2^e with 2^e a normal double is ldexp(x, e) exactly (one
+      // correctly rounded product, no libm call per value)
+      const double p2 = std::ldexp(1.0, f.e);
+      for (size_t i = 0; i < n; i++) q[i] = (int32_t)std::nearbyint(x[i] * p2);
+    } else {
+      for (size_t i = 0; i < n; i++) q[i] = (int32_t)std::nearbyint(std::ldexp(x[i], f.e));
+    }
     return f;
   }
   // sum |x| * 2^e < 2^52; each value rounds to an integer r (|r| < 2^52),
   // then to 26 significant bits when |r| >= 2^26 (mantissa in [2^25, 2^26))
   std::frexp(sum, &ex);
   f.e = 52 - ex;
+  const bool normal = f.e >= -1022 && f.e <= 1023;
+  const double p2 = normal ? std::ldexp(1.0, f.e) : 0.0;
   for (size_t i = 0; i < n; i++) {
-    const double r = std::nearbyint(std::ldexp(x[i], f.e));
+    const double r = std::nearbyint(normal ? x[i] * p2 : std::ldexp(x[i], f.e));
     int64_t mant = (int64_t)r;
     int shift = 0;
     if (std::fabs(r) >= 67108864.0) {  // 2^26
@@ -226,18 +235,16 @@ Scratch& scratch() {
 // value)
 double median_lower(const std::vector<double>& values) {
   Scratch& sc = scratch();
-  std::vector<double>& v = sc.med_v;
-  v.clear();
-  v.reserve(values.size());
+  size_t m = 0;
   double lo = INFINITY, hi = -INFINITY;
   for (double x : values)
     if (std::isfinite(x)) {
-      v.push_back(x);
+      m++;
       lo = std::min(lo, x);
       hi = std::max(hi, x);
     }
-  if (v.empty()) return NAN;
-  const size_t m = v.size(), k = (m - 1) / 2;
+  if (m == 0) return NAN;
+  const size_t k = (m - 1) / 2;
   const double span = hi - lo;
   const size_t nbk = std::min<size_t>(4096, m / 8);
   const double scale = span > 0.0 ? (double)nbk / span : 0.0;
@@ -251,18 +258,24 @@ double median_lower(const std::vector<double>& values) {
     };
     std::vector<uint32_t>& count = sc.med_count;
     count.assign(nbk, 0);
-    for (double x : v) count[bucket(x)]++;
+    for (double x : values)
+      if (std::isfinite(x)) count[bucket(x)]++;
     size_t below = 0, b = 0;
     while (below + count[b] <= k) below += count[b++];
     std::vector<double>& in = sc.med_in;
     in.clear();
     in.reserve(count[b]);
-    for (double x : v)
-      if (bucket(x) == b) in.push_back(x);
+    for (double x : values)
+      if (std::isfinite(x) && bucket(x) == b) in.push_back(x);
     const size_t r = k - below;
     std::nth_element(in.begin(), in.begin() + (ptrdiff_t)r, in.end());
     return in[r];
   }
+  std::vector<double>& v = sc.med_v;
+  v.clear();
+  v.reserve(m);
+  for (double x : values)
+    if (std::isfinite(x)) v.push_back(x);
   std::nth_element(v.begin(), v.begin() + (ptrdiff_t)k, v.end());
   return v[k];
 }
@@ -320,42 +333,54 @@ int fit_segment_noise(const double* time, const double* flux, const double* err,
     e[k] = err ? err[keep[k]] : 1.0;
     sec[k] = group ? group[keep[k]] : 0;
   }
-  // distinct group labels in ascending order, and each observation's group
-  std::vector<int64_t> labels;
-  for (size_t k = 0; k < m; k++)
-    if (std::find(labels.begin(), labels.end(), sec[k]) == labels.end()) {
-      labels.push_back(sec[k]);
-      if (labels.size() > 64) {  // many groups: sort them all
-        labels = sec;
-        std::sort(labels.begin(), labels.end());
-        labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
-        break;
-      }
-    }
-  std::sort(labels.begin(), labels.end());
-  std::vector<uint32_t>& gi = sc.gi;
-  gi.resize(m);
-  std::vector<size_t> gcount(labels.size(), 0);
-  for (size_t k = 0; k < m; k++) {
-    gi[k] = (uint32_t)(std::lower_bound(labels.begin(), labels.end(), sec[k]) - labels.begin());
-    gcount[gi[k]]++;
-  }
-  std::vector<double> med(labels.size());
-  for (size_t g = 0; g < labels.size(); g++) {
-    std::vector<double>& vals = sc.vals;
-    vals.clear();
-    vals.reserve(gcount[g]);
-    for (size_t k = 0; k < m; k++)
-      if (gi[k] == g) vals.push_back(f[k]);
-    med[g] = median_lower(vals);
-    if (!(med[g] > 0.0)) {
+  if (!group) {  // one group: its median alone
+    const double med = median_lower(f);
+    if (!(med > 0.0)) {
       *msg = "mission flux must have a positive median";
       return FBLS_E_DATA;
     }
-  }
-  for (size_t k = 0; k < m; k++) {
-    f[k] = f[k] / med[gi[k]] - 1.0;
-    e[k] /= med[gi[k]];
+    for (size_t k = 0; k < m; k++) {
+      f[k] = f[k] / med - 1.0;
+      e[k] /= med;
+    }
+  } else {
+    // distinct group labels in ascending order, and each observation's group
+    std::vector<int64_t> labels;
+    for (size_t k = 0; k < m; k++)
+      if (std::find(labels.begin(), labels.end(), sec[k]) == labels.end()) {
+        labels.push_back(sec[k]);
+        if (labels.size() > 64) {  // many groups: sort them all
+          labels = sec;
+          std::sort(labels.begin(), labels.end());
+          labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
+          break;
+        }
+      }
+    std::sort(labels.begin(), labels.end());
+    std::vector<uint32_t>& gi = sc.gi;
+    gi.resize(m);
+    std::vector<size_t> gcount(labels.size(), 0);
+    for (size_t k = 0; k < m; k++) {
+      gi[k] = (uint32_t)(std::lower_bound(labels.begin(), labels.end(), sec[k]) - labels.begin());
+      gcount[gi[k]]++;
+    }
+    std::vector<double> med(labels.size());
+    for (size_t g = 0; g < labels.size(); g++) {
+      std::vector<double>& vals = sc.vals;
+      vals.clear();
+      vals.reserve(gcount[g]);
+      for (size_t k = 0; k < m; k++)
+        if (gi[k] == g) vals.push_back(f[k]);
+      med[g] = median_lower(vals);
+      if (!(med[g] > 0.0)) {
+        *msg = "mission flux must have a positive median";
+        return FBLS_E_DATA;
+      }
+    }
+    for (size_t k = 0; k < m; k++) {
+      f[k] = f[k] / med[gi[k]] - 1.0;
+      e[k] /= med[gi[k]];
+    }
   }
   const double origin = t[0];
   for (double& x : t) x -= origin;
@@ -469,11 +494,10 @@ void correlated_noise_factors(const std::vector<double>& time,
   const double cadence = median_lower(steps);
   std::vector<double> measured(nd);
   std::vector<size_t> counts(nd);
-  // One pass over the points advances every duration's bins.  A point's
-  // bin is floor(fl(x / d)) for x = t - t0, nondecreasing in x because the
-  // division is; each duration keeps the smallest x of its next bin (the
-  // boundary), so a point costs one comparison per duration, and the bin of
-  // a point at or past it comes from the division.  Within a duration the
+  // A point's bin is floor(fl(x / d)) for x = t - t0, nondecreasing in x
+  // because the division is; each duration keeps the smallest x of its next
+  // bin (the edge), so a point is compared with the edges, and the bin of a
+  // point at or past one comes from the division.  Within a duration the
   // points are added in order from zero at each bin's first point, as in a
   // pass of that duration alone, so every bin's sums are the same doubles.
   std::vector<double>&x = sc.x, &wr = sc.wr;
@@ -486,12 +510,10 @@ void correlated_noise_factors(const std::vector<double>& time,
   std::vector<std::vector<double>>& z = sc.z;
   if (z.size() < nd) z.resize(nd);
   for (size_t d = 0; d < nd; d++) z[d].clear();
-  std::vector<double> need(nd), total(nd, 0.0), acc(nd, 0.0), edge(nd, -INFINITY);
-  std::vector<size_t> cnt(nd, 0);
+  std::vector<double> need(nd);
   for (size_t l = 0; l < nd; l++) need[l] = min_fill * durations[l] / cadence;
-  auto flush = [&](size_t l) {
-    if (cnt[l] > 0 && (double)cnt[l] >= need[l])
-      z[l].push_back((acc[l] / total[l]) * std::sqrt(total[l]));
+  auto flush = [&](size_t l, size_t count, double total, double acc) {
+    if (count > 0 && (double)count >= need[l]) z[l].push_back((acc / total) * std::sqrt(total));
   };
   // the smallest double b with fl(b / d) >= c + 1: the first x of the bins
   // after bin c (b = (c + 1) d within a few ulps, then stepped exactly)
@@ -506,25 +528,42 @@ void correlated_noise_factors(const std::vector<double>& time,
     }
     return e;
   };
-  for (size_t i = 0; i < n; i++) {
-    const double xi = x[i];
-    for (size_t l = 0; l < nd; l++) {
-      if (xi >= edge[l]) {  // a new bin of duration l
-        flush(l);
-        edge[l] = next_edge(std::floor(xi / durations[l]), durations[l]);
-        total[l] = 0.0;
-        acc[l] = 0.0;
-        cnt[l] = 0;
+  // eight durations per pass, their sums in registers; a point below every
+  // lane's edge costs one comparison and the sixteen additions
+  constexpr size_t kLanes = 8;
+  for (size_t d0 = 0; d0 < nd; d0 += kLanes) {
+    const size_t lanes = std::min(kLanes, nd - d0);
+    double total[kLanes], acc[kLanes], edge[kLanes];
+    size_t first[kLanes];
+    for (size_t l = 0; l < kLanes; l++) {
+      total[l] = acc[l] = 0.0;
+      edge[l] = l < lanes ? -INFINITY : INFINITY;
+      first[l] = 0;
+    }
+    double low = -INFINITY;  // the smallest edge
+    for (size_t i = 0; i < n; i++) {
+      const double xi = x[i];
+      if (xi >= low) {  // a new bin of some duration
+        low = INFINITY;
+        for (size_t l = 0; l < lanes; l++) {
+          if (xi >= edge[l]) {
+            const double d = durations[d0 + l];
+            flush(d0 + l, i - first[l], total[l], acc[l]);
+            edge[l] = next_edge(std::floor(xi / d), d);
+            total[l] = acc[l] = 0.0;
+            first[l] = i;
+          }
+          low = std::min(low, edge[l]);
+        }
+      }
+      const double wi = weights[i], ri = wr[i];
+      for (size_t l = 0; l < kLanes; l++) {  // lanes past the last duration are never read
+        total[l] += wi;
+        acc[l] += ri;
       }
     }
-    const double wi = weights[i], ri = wr[i];
-    for (size_t l = 0; l < nd; l++) {
-      total[l] += wi;
-      acc[l] += ri;
-      cnt[l]++;
-    }
+    for (size_t l = 0; l < lanes; l++) flush(d0 + l, n - first[l], total[l], acc[l]);
   }
-  for (size_t l = 0; l < nd; l++) flush(l);
   for (size_t di = 0; di < nd; di++) {
     counts[di] = z[di].size();
     measured[di] = z[di].size() >= 3 ? max1(mad_scale(z[di])) : 1.0;

@@ -154,10 +154,13 @@ def test_family_states_many_equals_one_at_a_time():
 
 def test_quantize_matches_oracle():
     rng = np.random.default_rng(9)
-    for scale in (1e-3, 1.0, 1e6):
+    # extreme scales: the exponent leaves the normal range (1e-300) and
+    # scaled values underflow (1e300 with a few tiny entries)
+    for scale in (1e-300, 1e-3, 1.0, 1e6, 1e300):
         for spike in (0.0, 1e3, 1e7):
             x = rng.normal(0, scale, 5000)
             x[17] += spike * scale
+            x[23] = 1e-310
             q1, e1, b1 = F.quantize(x)
             q2, e2, b2 = O.quantize(x)
             assert e1 == e2 and b1 == b2 == (spike > 0)
