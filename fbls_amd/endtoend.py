@@ -138,15 +138,33 @@ def _ramp(chunk):
         yield chunk
 
 
-def _net_on(net, gpu):
-    """A copy of the proposal network on another device context."""
+def _peers(gpu, k):
+    """k further device contexts on gpu's device, kept on gpu and reused by
+    later runs (their staging and device arenas stay warm, as in a batch
+    service that keeps its workers)."""
+    from . import Gpu
+
+    peers = gpu.__dict__.setdefault("_e2e_peers", [])
+    while len(peers) < k:
+        peers.append(Gpu(gpu.device))
+    return peers[:k]
+
+
+def _net_copy(net, gpu):
+    """A copy of the proposal network on another device context, made once
+    per (network, context) and refreshed from the network's current
+    parameters on each run."""
     from . import Cnn
 
-    c = net.cfg
-    copy = Cnn(gpu, in_channels=c.in_channels, width=c.width, layers=c.layers,
-               kernel=c.kernel, pos_weight=c.pos_weight)
-    copy.set_params(net.params())
-    return copy
+    copies = gpu.__dict__.setdefault("_e2e_nets", {})
+    copy = copies.get(id(net))
+    if copy is None or copy[0] is not net:
+        c = net.cfg
+        copy = (net, Cnn(gpu, in_channels=c.in_channels, width=c.width, layers=c.layers,
+                         kernel=c.kernel, pos_weight=c.pos_weight))
+        copies[id(net)] = copy
+    copy[1].set_params(net.params())
+    return copy[1]
 
 
 def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, device_workers=2):
@@ -167,8 +185,6 @@ def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, de
     counts."""
     if mode not in ("full", "pruned", "long"):
         raise ValueError(mode)
-    from . import Gpu
-
     durations = cfg.durations_days()
     st = Stages()
     # the first chunks are small, so the device starts while the host still
@@ -180,9 +196,9 @@ def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, de
         spans.append((a, min(len(curves), a + size)))
         a += size
     nw = max(1, min(device_workers, len(spans)))
-    ctxs = [gpu] + [Gpu(gpu.device) for _ in range(nw - 1)]
-    before = gpu.launch_totals()
-    nets = [net] + [_net_on(net, g) for g in ctxs[1:]] if net is not None else [None] * nw
+    ctxs = [gpu] + _peers(gpu, nw - 1)
+    before = [g.launch_totals() for g in ctxs]
+    nets = [net] + [_net_copy(net, g) for g in ctxs[1:]] if net is not None else [None] * nw
     ahead = threading.Semaphore(nw + 1)
     stop = threading.Event()  # a device thread failed: pending uploads give up
     results = [None] * len(spans)
@@ -248,10 +264,10 @@ def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, de
         templates += n
     # scan launches of this run on every context (the contexts' kernels
     # overlap in time, so their seconds add up to more than the wall time)
-    totals = {k: v - before[k] for k, v in gpu.launch_totals().items()}
-    for g in ctxs[1:]:
+    totals = {k: 0 for k in before[0]}
+    for g, b0 in zip(ctxs, before):
         for k, v in g.launch_totals().items():
-            totals[k] += v
+            totals[k] += v - b0[k]
     return {"best": best, "seconds": wall, "stars": len(curves),
             "stars_per_hour": len(curves) * 3600.0 / wall, "trial_evals": templates,
             "stage_seconds": st.seconds, "host_workers": PREP_THREADS, "device_workers": nw,
