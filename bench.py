@@ -233,6 +233,17 @@ def batched(gpu, data):
     tot = r["launch_totals"]
     found = PR.recovered(r["best"], inj)
     sizes = [len(c[0]) for c in curves]
+    # the kernels alone, one context, no other work on the device: the scan's
+    # own fold rate on these stars (the end-to-end run's two contexts overlap
+    # in time, so the sum of their kernel seconds overstates the device time)
+    sub = curves[:min(64, len(curves))]
+    devs, info = F.DeviceReady.prepare_upload(gpu, sub, cfg.durations_days(),
+                                              cfg.correlated_noise, flux_offset=-1.0)
+    gpu.launch_totals(reset=True)
+    F.signal_search_batch(gpu, devs, cfg.durations_days(), list(info["rho"]), cfg)
+    one = gpu.launch_totals(reset=True)
+    for x in devs:
+        x.close()
     return {
         "workload": f"{len(curves)} stars, 1-13 sectors ({min(sizes)}-{max(sizes)} points), "
                     "sigma 200-3000 ppm, half with a box P 0.5-20 d; periods 0.5-20 d",
@@ -246,7 +257,10 @@ def batched(gpu, data):
         "launches": tot["launches"],
         "periods": int(tot["periods"]),
         "fold_pairs": int(tot["fold_pairs"]),
-        "fold_pairs_per_kernel_s": tot["fold_pairs"] / tot["seconds"],
+        "fold_pairs_per_kernel_s_all_contexts": tot["fold_pairs"] / tot["seconds"],
+        "one_context": {"stars": len(sub), "kernel_seconds": one["seconds"],
+                        "launches": one["launches"], "fold_pairs": int(one["fold_pairs"]),
+                        "fold_pairs_per_kernel_s": one["fold_pairs"] / one["seconds"]},
         "injections_recovered": int(found.sum()),
         "injections": sum(1 for c in inj if c > 0),
         "host_preparation_stars_per_hour": {
@@ -472,9 +486,24 @@ def main():
                     help="long-baseline stars of the learned-pruned comparison (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale counters")
+    ap.add_argument("--only", choices=("learned_pruned", "batched", "long_baseline"),
+                    help="run one section alone (its star count from the options above) and "
+                         "print its JSON: the workload of a profile of that section's launches")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.only:
+        import fbls_amd as F
+
+        gpu = F.Gpu(0)
+        if args.only == "learned_pruned":
+            sec = learned_pruned(gpu, single_sector_curves(args.proposal_stars))
+        elif args.only == "batched":
+            sec = batched(gpu, batch_curves(args.batch_stars))
+        else:
+            sec = long_baseline(gpu, long_curves(args.long_stars))
+        print(json.dumps({args.only: sec}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: stars are independent and the "
