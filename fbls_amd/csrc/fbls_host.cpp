@@ -1142,7 +1142,11 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
                                 &tb.lds, &tb.groups, /*cap_by_total=*/t < 2))
       return st;
     tb.block = launch_block(c, tb.split, tb.lds);
-    if (t >= 2) {  // persistent: at most the workgroups the chip holds at once
+    // the full pass: persistent (at most the workgroups the chip holds at
+    // once), or, where nearly every period survives (box_first), one group
+    // per workgroup of a grid sized for every period surviving at the host's
+    // G (the loop-free kernel spills fewer registers; its idle tail is short)
+    if (t >= 2 && !box_first) {
       const uint32_t per_cu = (tb.split || tb.block == 1024) ? 1u : 2u;
       tb.groups = std::min<uint32_t>(tb.groups, std::max<uint32_t>(1, c->cus) * per_cu);
     }
@@ -1327,7 +1331,7 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
     A.box_first = (box_first && t < 2) ? 1u : 0u;
     if (t >= 2) {
       A.vgroups = d_vgroups + (t - 2);
-      A.ticket = d_tickets + (t - 2);
+      A.ticket = box_first ? nullptr : d_tickets + (t - 2);
     }
     return A;
   };
@@ -1343,7 +1347,8 @@ int scan_bands(fbls_ctx* c, std::vector<BandRun>& bands, const fbls_bank* b,
   for (int t = 0; t < 2; t++) {
     BA.fine[t].segs = (fbls::Segment*)(din + seg_off) + seg_at[2 + t];
     BA.fine[t].nseg = (uint32_t)T[2 + t].w.size();
-    BA.fine[t].cap_by_total = t == 0 ? 1u : 0u;
+    // the device keeps the host's G for a bounded grid
+    BA.fine[t].cap_by_total = (t == 0 && !box_first) ? 1u : 0u;
     BA.fine[t].vgroups = d_vgroups + t;
     BA.fine[t].ticket = d_tickets + t;
   }

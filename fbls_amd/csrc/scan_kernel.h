@@ -55,8 +55,10 @@ struct LaunchArgs {
   // Persistent launch (both non-null): a fixed grid whose workgroups take
   // the launch's groups in order from *ticket (zero when the launch starts)
   // until *vgroups are taken; the segment table and *vgroups may have been
-  // written on the device by an earlier kernel of the stream.  Null: one
-  // group per workgroup of the grid.
+  // written on the device by an earlier kernel of the stream.  vgroups
+  // alone: one group per workgroup of a grid at least *vgroups long, the
+  // workgroups past *vgroups exit at once.  Both null: one group per
+  // workgroup of the grid.
   const uint32_t* vgroups;
   unsigned int* ticket;
   // Stage attribution (the reference's FBLS_PROFILE): per segment three

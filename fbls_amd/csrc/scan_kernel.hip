@@ -943,6 +943,9 @@ template <bool PERSIST, class Body>
 __device__ __forceinline__ void for_each_group(const LaunchArgs& A, unsigned char* lds,
                                                Body body) {
   if (!PERSIST) {
+    // a grid sized for every period surviving: the groups past the
+    // device-written total have nothing to do
+    if (A.vgroups && blockIdx.x >= *A.vgroups) return;
     body(A, blockIdx.x);
     return;
   }
