@@ -218,7 +218,7 @@ def end_to_end(gpu, curves, injected, cfg, net=None, pc=None, op=None, mode="pru
 def batched(gpu, data):
     """The full scan of every star of the batch, end to end: preparation on
     the library's preparation threads pipelined with the packed upload and the merged-launch
-    search (fbls_signal_search_batch) of chunks of 128 stars on two device
+    search (fbls_signal_search_batch) of chunks of 96 stars on two device
     contexts, and selection.  Rates are per wall second; the kernel seconds
     add up both contexts' launches (which overlap in time)."""
     import fbls_amd as F

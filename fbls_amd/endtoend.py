@@ -167,7 +167,7 @@ def _net_copy(net, gpu):
     return copy[1]
 
 
-def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=128, device_workers=2):
+def run(gpu, curves, cfg, mode="full", net=None, pc=None, op=None, chunk=96, device_workers=2):
     """One end-to-end batch run over `curves` ((time, flux, flux_err) per
     star, flux normalised to 1).  The stages overlap: preparation and
     upload on the library's preparation threads and upload stream (chunk
