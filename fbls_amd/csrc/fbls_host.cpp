@@ -67,8 +67,11 @@ class Pool {
     return *pool;
   }
   // The preparation threads (fbls_family_state_many): a pool of their own at
-  // a lower CPU priority (nice + 5), so that a batch being prepared does not
-  // delay the threads that plan searches and feed the device.
+  // a lower CPU priority (nice + 5) and four threads short of the cores, so
+  // that a batch being prepared does not delay the threads that plan
+  // searches and feed the device (measured on the single-sector pruned batch
+  // on 16 cores: 12 preparation threads 53-55 ms per 512 stars, 16 threads
+  // 55-57, 10 threads 57-58, 8 threads 60-61).
   static Pool& background() {
     static Pool* pool = new Pool(true);
     return *pool;
@@ -113,7 +116,8 @@ class Pool {
 
   explicit Pool(bool background) {
     const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
-    const size_t n = std::min<size_t>(hw, 16) - (background ? 0 : 1);
+    const size_t cap = std::min<size_t>(hw, 16);
+    const size_t n = background ? std::max<size_t>(1, cap > 4 ? cap - 4 : 1) : cap - 1;
     for (size_t k = 0; k < n; k++) {
       try {
         threads_.emplace_back([this, background]() {

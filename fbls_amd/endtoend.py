@@ -33,8 +33,10 @@ from . import (DeviceReady, family_state, family_states, propose, signal_search_
                signal_search_bins, signal_search_bins_ragged)
 from . import proposal as PR
 
-# threads of the library's preparation pool (fbls_family_state_many)
-PREP_THREADS = min(os.cpu_count() or 1, 16)
+# threads that run the library's preparation (fbls_family_state_many): its
+# pool, four short of the cores (at most 16), and the calling thread
+_CORES = min(os.cpu_count() or 1, 16)
+PREP_THREADS = max(1, _CORES - 4 if _CORES > 4 else 1) + 1
 
 
 def _prepare(curve, durations, correlated):

@@ -577,7 +577,8 @@ int fbls_family_state(const double* time, const double* flux,
                       size_t* segments, double* residual_energy);
 
 /* [host] fbls_family_state for `count` stars at once, on the library's
- * preparation threads (a pool of their own at a lower CPU priority), as the
+ * preparation threads (a pool of their own at a lower CPU priority, four
+ * threads short of the cores, and the calling thread), as the
  * reference's `batch` prepares its stars on worker threads before the
  * search (src/batch.rs:47-68, curve.rs:541-569 per star).  Star k reads
  * time[k], flux[k], sigma[k] of n[k] points and writes t[k], v[k], w[k]
