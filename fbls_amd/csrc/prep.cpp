@@ -205,6 +205,7 @@ struct Scratch {
   std::vector<size_t> keep;
   std::vector<int64_t> sec;
   std::vector<uint32_t> gi, med_count;
+  std::vector<uint16_t> med_id;
   std::vector<std::vector<double>> z;
 
   size_t bytes() const {
@@ -214,6 +215,7 @@ struct Scratch {
       b += v->capacity() * sizeof(double);
     b += keep.capacity() * sizeof(size_t) + sec.capacity() * sizeof(int64_t);
     b += (gi.capacity() + med_count.capacity()) * sizeof(uint32_t);
+    b += med_id.capacity() * sizeof(uint16_t);
     for (const auto& v : z) b += v.capacity() * sizeof(double);
     return b;
   }
@@ -252,18 +254,38 @@ double median_lower(const std::vector<double>& values) {
     // value buckets: bucket(x) = floor((x - lo) * scale) is nondecreasing in
     // x, so the k-th smallest value lies in the bucket where the running
     // count passes k, at rank k minus the count of the buckets below it
+    // (x - lo) * scale lies in [0, nbk] (up to rounding): a signed
+    // conversion, one instruction, gives the same floor as an unsigned one
+    const int64_t top = (int64_t)nbk - 1;
     auto bucket = [&](double x) {
-      const size_t b = (size_t)((x - lo) * scale);
-      return b < nbk ? b : nbk - 1;
+      const int64_t b = (int64_t)((x - lo) * scale);
+      return (size_t)(b < top ? b : top);
     };
     std::vector<uint32_t>& count = sc.med_count;
     count.assign(nbk, 0);
+    std::vector<double>& in = sc.med_in;
+    in.clear();
+    if (m == values.size()) {  // every value finite: bucket ids kept (nbk < 2^16)
+      std::vector<uint16_t>& id = sc.med_id;
+      id.resize(m);
+      for (size_t i = 0; i < m; i++) {
+        const size_t bi = bucket(values[i]);
+        id[i] = (uint16_t)bi;
+        count[bi]++;
+      }
+      size_t below = 0, b = 0;
+      while (below + count[b] <= k) below += count[b++];
+      in.reserve(count[b]);
+      for (size_t i = 0; i < m; i++)
+        if (id[i] == b) in.push_back(values[i]);
+      const size_t r = k - below;
+      std::nth_element(in.begin(), in.begin() + (ptrdiff_t)r, in.end());
+      return in[r];
+    }
     for (double x : values)
       if (std::isfinite(x)) count[bucket(x)]++;
     size_t below = 0, b = 0;
     while (below + count[b] <= k) below += count[b++];
-    std::vector<double>& in = sc.med_in;
-    in.clear();
     in.reserve(count[b]);
     for (double x : values)
       if (std::isfinite(x) && bucket(x) == b) in.push_back(x);
