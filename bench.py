@@ -350,6 +350,51 @@ def long_baseline(gpu, data):
     }
 
 
+def long_star_stages(gpu, curves):
+    """Stage attribution of each star's full search, one star at a time
+    (FBLS_PROFILE: each band's kernel seconds split by the workgroups'
+    shader cycles in the fold, the prefix sums and the template scan): per
+    star its points, periods, prescan survivors, device seconds and fold
+    pairs per device second; every band of the largest and the smallest."""
+    import fbls_amd as F
+    from fbls_amd import longbase as LB
+
+    os.environ["FBLS_PROFILE"] = "1"
+    cfg = LB.search_config()
+    d = cfg.durations_days()
+    keys = ("seconds_fold", "seconds_cumulative", "seconds_scan", "seconds_device")
+    by_size = sorted((c[0] for c in curves), key=lambda c: len(c[0]))
+    stars, detail = [], []
+    for k, curve in enumerate(by_size):
+        state = prepare([curve], cfg)[0]
+        dev = F.DeviceReady.upload(gpu, state)
+        for _ in range(2):  # the second search is reported
+            _, _, bands = F.signal_search(gpu, dev, d, state.rho, cfg)
+        dev.close()
+        n = len(state.t)
+        periods = sum(b["periods"] for b in bands)
+        survivors = sum(b["prescan_survivors"] for b in bands)
+        tot = {k2: sum(b[k2] for b in bands) for k2 in keys}
+        stars.append({"points": n, "periods": periods, "survivors": survivors,
+                      "fold_pairs": (periods + survivors) * n, **tot})
+        if k in (0, len(by_size) - 1):
+            detail.append([{k2: b[k2] for k2 in ("periods", "phase_bins", "prescan_survivors")
+                            + keys} for b in bands])
+
+    def rate(group):
+        sec = sum(x["seconds_device"] for x in group)
+        return sum(x["fold_pairs"] for x in group) / sec if sec else 0.0
+
+    few = [x for x in stars if x["survivors"] < 0.1 * x["periods"]]
+    return {"workload": f"full search of each of {len(stars)} long-baseline stars, one at a "
+                        "time, periods 0.5-100 d",
+            "stars": stars,
+            "fold_pairs_per_device_s": rate(stars),
+            "stars_with_few_survivors": len(few),
+            "fold_pairs_per_device_s_few_survivors": rate(few),
+            "bands_smallest": detail[0], "bands_largest": detail[-1]}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -486,7 +531,8 @@ def main():
                     help="long-baseline stars of the learned-pruned comparison (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale counters")
-    ap.add_argument("--only", choices=("learned_pruned", "batched", "long_baseline"),
+    ap.add_argument("--only", choices=("learned_pruned", "batched", "long_baseline",
+                                       "long_star_stages"),
                     help="run one section alone (its star count from the options above) and "
                          "print its JSON: the workload of a profile of that section's launches")
     args = ap.parse_args()
@@ -501,8 +547,10 @@ def main():
             sec = learned_pruned(gpu, single_sector_curves(args.proposal_stars))
         elif args.only == "batched":
             sec = batched(gpu, batch_curves(args.batch_stars))
-        else:
+        elif args.only == "long_baseline":
             sec = long_baseline(gpu, long_curves(args.long_stars))
+        else:
+            sec = long_star_stages(gpu, long_curves(args.long_stars))
         print(json.dumps({args.only: sec}), flush=True)
         return
     if args.gpus != 1:
