@@ -604,6 +604,8 @@ def main():
             "templates_per_step": templates,
             "fold_pairs_per_step": launch_periods * n,
             "stars_per_hour": 3600.0 / (elapsed / args.steps),
+            "stars_per_hour_basis": "the star resident in HBM; the end_to_end sections "
+                                    "include preparation and upload",
             "best_period": None if best is None else float(best[0]),
             "best_snr": None if best is None else float(best[3]),
         },
